@@ -1,0 +1,192 @@
+/*
+ * thp.h -- C ABI of libthp, the MI355X-native AMOEBA induced-dipole solver.
+ *
+ * Every entry point takes plain pointers and sizes.  Conventions:
+ *   - return value: THP_OK (0) or a negative THP_ERR_* code; never exit().
+ *     thp_last_error() returns a message for the last failure on a context.
+ *   - atom indices are 0-based at this boundary.
+ *   - host arrays are caller-owned and copied; device state is library-owned.
+ *     The *_dev variants take device pointers (stream-ordered, zero copy).
+ *   - array layouts follow the reference's Fortran arrays in C row-major
+ *     order: rpole[n][13] (q, dx,dy,dz, Qxx,Qxy,Qxz,Qyx,Qyy,Qyz,Qzx,Qzy,Qzz,
+ *     global frame, quadrupole in the internal /3 convention), vectors over
+ *     both right-hand sides as [n][2][3] (rhs 0 = d, rhs 1 = p), cphi[n][10].
+ *   - boxes are orthorhombic (three edge lengths).
+ *   - a thp_ctx is used from one host thread at a time, except
+ *     thp_serve_submit / thp_serve_poll, which may be called from any thread.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   thp_init/thp_finalize ...... C_init_env, C_get_cell
+ *                                (GPU/source/cu_tmatxb_pme.cu:1035-1078,
+ *                                 GPU/source/MOD_interfaces.f:684-690)
+ *   thp_nblist ................. nblist / mlistcell
+ *                                (v1.2/source/nblist.f:17-247;
+ *                                 GPU/source/nblist.f:24-186)
+ *   thp_efld0_direct ........... efld0_direct / cu_efld0_direct
+ *                                (v1.2/source/newinduce_pme.f:948-1389;
+ *                                 GPU/source/MOD_interfaces.f:610-636)
+ *   thp_efld0_recip ............ efld0_recip (v1.2/source/newinduce_pme.f:1671-1917)
+ *   thp_tmatxb ................. tmatxb_pme / cu_tmatxb_pme
+ *                                (v1.2/source/newinduce_pme.f:1392-1668;
+ *                                 GPU/source/MOD_interfaces.f:584-606)
+ *   thp_tmatxb_recip ........... tmatxbrecip (v1.2/source/newinduce_pme.f:1919-2128)
+ *   thp_grid_uind .............. grid_uind_site (v1.2/source/pmestuff.f:280-400)
+ *   thp_fphi_uind .............. fphi_uind_site (v1.2/source/pmestuff.f:746-1037)
+ *   thp_induce ................. newinduce_pme2 + inducepcg_pme2
+ *                                (v1.2/source/newinduce_pme2.f:16-636;
+ *                                 GPU/source/newinduce_pme2gpu.f:17-1137)
+ */
+#ifndef THP_H
+#define THP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int thp_status;
+#define THP_OK 0
+#define THP_ERR_ARG (-1)      /* invalid argument */
+#define THP_ERR_HIP (-2)      /* HIP runtime failure */
+#define THP_ERR_FFT (-3)      /* rocFFT failure */
+#define THP_ERR_NOMEM (-4)    /* device allocation failed / over budget */
+#define THP_ERR_LIST (-5)     /* cutoff+buffer exceeds half the box, or list capacity */
+#define THP_ERR_NOCONV (-6)   /* PCG did not reach poleps within politer */
+#define THP_ERR_NAN (-7)      /* non-finite residual in the PCG loop */
+#define THP_ERR_STATE (-8)    /* call out of order (e.g. serve not started) */
+#define THP_ERR_BUSY (-9)     /* serve queue full */
+
+#define THP_PREC_FP64 0       /* FP64 pair, grid, FFT and vectors ("plain") */
+#define THP_PREC_MIXED 1      /* FP32 pair/grid/FFT, FP64 accumulation and CG */
+
+typedef struct thp_ctx thp_ctx;
+typedef struct thp_sys thp_sys;
+
+typedef struct {
+  int n;                      /* number of polarizable sites */
+  double box[3];              /* orthorhombic edge lengths (Angstrom) */
+  const double* x;            /* [n] coordinates (Angstrom) */
+  const double* y;
+  const double* z;
+  const double* rpole;        /* [n][13] global-frame multipoles */
+  const double* polarity;     /* [n] polarizability (Angstrom^3) */
+  const double* thole;        /* [n] Thole damping parameter */
+  int nscaled;                /* number of scaled pairs */
+  const int* scaled_ik;       /* [nscaled][2] atom pairs (any order) */
+  const double* scaled_dpu;   /* [nscaled][3] d-, p-, u-scale of each pair */
+  double aewald;              /* Ewald coefficient (1/Angstrom) */
+  double cut;                 /* real-space cutoff (Angstrom) */
+  double buffer;              /* neighbor-list buffer (Angstrom) */
+  int nfft[3];                /* PME grid */
+  int order;                  /* B-spline order (5) */
+  int precision;              /* THP_PREC_FP64 or THP_PREC_MIXED */
+} thp_system_desc;
+
+typedef struct {
+  double poleps;              /* PCG tolerance, reference criterion sqrt(r.z/(3n)) */
+  int politer;                /* iteration cap */
+  int guess;                  /* 0: mu = alpha*E (polgsf=0); 1: warm start from uind/uinp inputs */
+} thp_solve_opts;
+
+typedef struct {
+  int niter;                  /* PCG iterations performed */
+  int status;                 /* THP_OK, THP_ERR_NOCONV or THP_ERR_NAN */
+  double resnrm;              /* final residual norm (reference units) */
+  double ep;                  /* polarization energy, kcal/mol */
+  int n_indefinite;           /* iterations where some rhs had p.(A p) <= 0 */
+  double t_field_ms;          /* device time in efld0 (direct + recip + self) */
+  double t_solve_ms;          /* device time in the PCG loop */
+} thp_solve_stats;
+
+/* ---------------------------------------------------------------- context */
+thp_status thp_init(thp_ctx** ctx, int device);
+thp_status thp_finalize(thp_ctx* ctx);
+const char* thp_last_error(thp_ctx* ctx);
+const char* thp_version(void);
+
+/* ---------------------------------------------------------------- systems */
+thp_status thp_system_create(thp_ctx* ctx, const thp_system_desc* desc, thp_sys** sys);
+thp_status thp_system_update_coords(thp_sys* sys, const double* x, const double* y, const double* z);
+thp_status thp_system_destroy(thp_sys* sys);
+/* device memory held by the system (bytes) */
+thp_status thp_system_footprint(const thp_sys* sys, int64_t* bytes);
+
+/* ---------------------------------------------------- path entry points */
+/* (re)build the Verlet list; force_rebuild=0 rebuilds only when atoms moved
+ * more than buffer/2 since the last build */
+thp_status thp_nblist(thp_sys* sys, int force_rebuild);
+/* pair set of the list as (i,k) rows with i<k, sorted; pass ik=NULL to
+ * query the count.  cap = rows available in ik. */
+thp_status thp_nblist_get(thp_sys* sys, int64_t* npairs, int* ik, int64_t cap);
+
+thp_status thp_efld0_direct(thp_sys* sys, double* ef /* [n][2][3] */);
+thp_status thp_efld0_recip(thp_sys* sys, double* cphi /* [n][10] */);
+thp_status thp_tmatxb(thp_sys* sys, const double* mu, double* efi, int dodiag);
+thp_status thp_tmatxb_recip(thp_sys* sys, const double* mu, double* dipfield);
+/* spread fractional induced dipoles fuind/fuinp [n][3] onto the device grid;
+ * qgrid (optional) receives the grid as [n3][n2][n1][2] doubles */
+thp_status thp_grid_uind(thp_sys* sys, const double* fuind, const double* fuinp, double* qgrid);
+/* gather first derivatives of the potential of a grid [n3][n2][n1][2]
+ * (NULL: the device grid) into fdip_phi1/fdip_phi2 [n][3] */
+thp_status thp_fphi_uind(thp_sys* sys, const double* qgrid, double* fdip_phi1, double* fdip_phi2);
+/* multiply the device grid by the scalar-sum table between the forward and
+ * backward 3-D FFTs (efld0_recip / tmatxbrecip, convolution step) */
+thp_status thp_pme_convolve(thp_sys* sys);
+
+thp_status thp_induce(thp_sys* sys, const thp_solve_opts* opts, double* uind, double* uinp,
+                      double* ep, thp_solve_stats* stats);
+
+/* device-pointer variants: mu, efi, ef, uind... are device addresses */
+thp_status thp_tmatxb_dev(thp_sys* sys, const double* mu, double* efi, int dodiag);
+thp_status thp_induce_dev(thp_sys* sys, const thp_solve_opts* opts, double* mu_dev /* [n][2][3] */,
+                          double* ep, thp_solve_stats* stats);
+
+/* --------------------------------------------------------------- serving */
+typedef struct {
+  int max_systems;            /* resident systems (slots) */
+  int64_t hbm_budget;         /* device bytes the loop may hold; 0 = 80% of free */
+  int iters_per_graph;        /* CG iterations per captured graph launch */
+  int autotune;               /* 1: pick pair-tile/grid-block shapes by measurement */
+  double poleps;
+  int politer;
+} thp_serve_opts;
+
+typedef struct {
+  uint64_t ticket;
+  int system_id;
+  int status;                 /* THP_OK / THP_ERR_NOCONV / THP_ERR_NAN ... */
+  int niter;
+  double ep;
+  double latency_ms;          /* submit -> result */
+  double resnrm;
+} thp_result;
+
+thp_status thp_serve_start(thp_ctx* ctx, const thp_serve_opts* opts);
+/* register a system with the loop (its data stays resident while admitted) */
+thp_status thp_serve_add_system(thp_ctx* ctx, const thp_system_desc* desc, int* system_id);
+/* request a solve of system_id at new coordinates (NULL: unchanged) */
+thp_status thp_serve_submit(thp_ctx* ctx, int system_id, const double* x, const double* y,
+                            const double* z, uint64_t* ticket);
+/* 1 if ready (result filled), 0 if not yet; wait_ms<0 blocks */
+thp_status thp_serve_poll(thp_ctx* ctx, uint64_t ticket, thp_result* result, double wait_ms);
+/* copy the last induced dipoles of a system [n][2][3] (original order) */
+thp_status thp_serve_get_dipoles(thp_ctx* ctx, int system_id, double* mu);
+thp_status thp_serve_stop(thp_ctx* ctx);
+
+/* ------------------------------------------------------ synthetic inputs */
+/* Seeded water box / water+solute system (host-only, no device use).
+ * n atoms in a cubic box of edge L; n_solute atoms form a compact blob of
+ * radius blob_radius at the box centre.  Outputs must hold n (x,y,z,
+ * polarity, thole), 13n (rpole); scaled pairs: at most 3n rows. */
+thp_status thp_make_system(int n, double box, int n_solute, double blob_radius, uint64_t seed,
+                           double* x, double* y, double* z, double* rpole, double* polarity,
+                           double* thole, int* nscaled, int* scaled_ik, double* scaled_dpu);
+/* smallest 2,3,5-smooth even grid size >= minsize from the reference table
+ * (v1.2/source/kewald.f:67-73) */
+int thp_pick_grid(double minsize);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* THP_H */

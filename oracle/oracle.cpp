@@ -1,0 +1,1146 @@
+// FP64 CPU oracle for the AMOEBA induced-dipole solve.
+//
+// TEST INFRASTRUCTURE ONLY: the checker for the HIP product path.  Only
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+// Every routine cites the reference routine it restates (paths relative to
+// the reference tree).  Compiled with -ffp-contract=off so that the
+// neighbor-list distance test follows the reference's operation order.
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <functional>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr double kSqrtPi = 1.772453850905516027298167483341145183;
+constexpr double kCoulomb = 332.063714;  // v1.2/source/MOD_units.f:81
+constexpr double kTinyPol = 1.0e-5;      // v1.2/source/MOD_polar.f:36
+constexpr int kMaxOrder = 12;
+
+struct Scaled {
+  int k;
+  double d, p, u;
+};
+
+}  // namespace
+
+struct orc_sys {
+  int n = 0;
+  double bx = 0, by = 0, bz = 0;
+  std::vector<double> x, y, z, rpole, polarity, thole, pdamp;
+  std::vector<std::vector<Scaled>> scl;  // per-atom scaled partners
+  double aewald = 0.4, cut = 7.0, buffer = 2.0;
+  int nfft1 = 0, nfft2 = 0, nfft3 = 0, order = 5;
+  std::vector<std::vector<int>> elst;  // elst(:,i): partners k>i
+  bool have_list = false;
+  std::vector<double> bsmod1, bsmod2, bsmod3;
+};
+
+namespace {
+
+// ---------------------------------------------------------------- image
+// v1.2/source/image.f:113-140 (orthogonal lattice branch)
+inline void image1(double& xr, double cell) {
+  const double cell2 = 0.5 * cell;
+  if (std::fabs(xr) > cell2) {
+    const double cel = std::copysign(cell, xr);
+    xr = xr - cel * std::floor(std::fabs(xr) / cell);
+    if (std::fabs(xr) > cell2) xr = xr - std::copysign(cell, xr);
+  }
+}
+inline void image(const orc_sys* s, double& xr, double& yr, double& zr) {
+  image1(xr, s->bx);
+  image1(yr, s->by);
+  image1(zr, s->bz);
+}
+
+// ------------------------------------------------------------- damping
+// dampewald, v1.2/source/damping.f:17-56; dmpe[0..6] = orders 1..7
+void dampewald(double aewald, double r, double r2, double scale, double* dmpe, int rorder) {
+  for (int i = 0; i < rorder; ++i) dmpe[i] = scale;
+  const double ra = aewald * r;
+  double bn[6];
+  bn[0] = std::erfc(ra) / r;
+  dmpe[0] = scale * bn[0];
+  const double expterm = std::exp(-ra * ra);
+  const double aesq2 = 2.0 * aewald * aewald;
+  double afac = 0.0;
+  if (aewald > 0.0) afac = 1.0 / (kSqrtPi * aewald);
+  const int niter = (rorder - 1) / 2;
+  for (int i = 1; i <= niter; ++i) {
+    const double bfac = double(2 * i - 1);
+    afac = aesq2 * afac;
+    bn[i] = (bfac * bn[i - 1] + afac * expterm) / r2;
+    dmpe[2 * i] = scale * bn[i];
+  }
+}
+
+// dampthole / damptholed (original Thole branch), v1.2/source/damping.f:74-215.
+// dmpik[0..8] = orders 1..9; pgamma = thlval(it,kt) = min(a_i,a_k) (kpolar.f:280-293)
+void dampthole(double pdi, double pdk, double pgamma, double r, int rorder, double* dmpik) {
+  for (int j = 0; j < rorder; ++j) dmpik[j] = 1.0;
+  double damp = pdi * pdk;
+  if (damp != 0.0 && pgamma != 0.0) {
+    const double rd = r / damp;
+    damp = pgamma * (rd * rd * rd);
+    if (damp < 50.0) {
+      const double expdamp = std::exp(-damp);
+      dmpik[2] = 1.0 - expdamp;
+      dmpik[4] = 1.0 - expdamp * (1.0 + damp);
+      if (rorder >= 7) {
+        const double damp2 = damp * damp;
+        dmpik[6] = 1.0 - expdamp * (1.0 + damp + 0.6 * damp2);
+      }
+    }
+  }
+}
+
+inline double thlval(const orc_sys* s, int i, int k) {
+  // kpolar.f:280-293: min, unless zero then max
+  double thl = std::min(s->thole[i], s->thole[k]);
+  if (thl == 0.0) thl = std::max(s->thole[i], s->thole[k]);
+  return thl;
+}
+
+// scale factor lookup for pair (i,k); default 1
+inline void pair_scales(const orc_sys* s, int i, int k, double& d, double& p, double& u) {
+  d = p = u = 1.0;
+  for (const Scaled& e : s->scl[i])
+    if (e.k == k) {
+      d = e.d;
+      p = e.p;
+      u = e.u;
+      return;
+    }
+}
+
+int nthreads_cap = 0;
+int n_threads() {
+#ifdef _OPENMP
+  return nthreads_cap > 0 ? nthreads_cap : omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+// ------------------------------------------------------------- nblist
+// mlistcell, v1.2/source/nblist.f:122-247: pairs k>i with
+// r^2 <= mbuf2 = (mpolecut+lbuffer)^2 (cutoffs.f:284), distances from
+// midpointimage (domdecstuff.f:1773) = image(xi-xk), r2 = x*x+y*y+z*z.
+void build_list(orc_sys* s) {
+  const int n = s->n;
+  const double mb = s->cut + s->buffer;
+  const double mbuf2 = mb * mb;
+  s->elst.assign(n, {});
+  // cell grid with cells >= mb so that 27 neighbor cells cover the cutoff;
+  // fall back to all pairs when fewer than 3 cells fit along an edge.
+  const int nc1 = int(std::floor(s->bx / mb)), nc2 = int(std::floor(s->by / mb)),
+            nc3 = int(std::floor(s->bz / mb));
+  auto test = [&](int i, int k) {
+    double xr = s->x[i] - s->x[k];
+    double yr = s->y[i] - s->y[k];
+    double zr = s->z[i] - s->z[k];
+    image(s, xr, yr, zr);
+    const double r2 = xr * xr + yr * yr + zr * zr;
+    return r2 <= mbuf2;
+  };
+  if (nc1 < 3 || nc2 < 3 || nc3 < 3) {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(n_threads())
+    for (int i = 0; i < n; ++i)
+      for (int k = i + 1; k < n; ++k)
+        if (test(i, k)) s->elst[i].push_back(k);
+  } else {
+    const int ncell = nc1 * nc2 * nc3;
+    std::vector<std::vector<int>> cells(ncell);
+    auto cellof = [&](double v, double L, int nc) {
+      double f = v / L;
+      f -= std::floor(f);
+      int c = int(f * nc);
+      return c >= nc ? nc - 1 : c;
+    };
+    std::vector<int> home(n);
+    for (int i = 0; i < n; ++i) {
+      const int c = (cellof(s->z[i], s->bz, nc3) * nc2 + cellof(s->y[i], s->by, nc2)) * nc1 +
+                    cellof(s->x[i], s->bx, nc1);
+      home[i] = c;
+      cells[c].push_back(i);
+    }
+#pragma omp parallel for schedule(dynamic, 16) num_threads(n_threads())
+    for (int i = 0; i < n; ++i) {
+      const int c = home[i];
+      const int c1 = c % nc1, c2 = (c / nc1) % nc2, c3 = c / (nc1 * nc2);
+      std::vector<int>& out = s->elst[i];
+      for (int d3 = -1; d3 <= 1; ++d3)
+        for (int d2 = -1; d2 <= 1; ++d2)
+          for (int d1 = -1; d1 <= 1; ++d1) {
+            const int k1 = (c1 + d1 + nc1) % nc1, k2 = (c2 + d2 + nc2) % nc2,
+                      k3 = (c3 + d3 + nc3) % nc3;
+            for (int k : cells[(k3 * nc2 + k2) * nc1 + k1])
+              if (k > i && test(i, k)) out.push_back(k);
+          }
+      std::sort(out.begin(), out.end());
+    }
+  }
+  s->have_list = true;
+}
+
+void ensure_list(orc_sys* s) {
+  if (!s->have_list) build_list(s);
+}
+
+// ----------------------------------------------------------- B-splines
+// bsplgen, v1.2/source/pmestuff.f:67-156 (level 4); theta[i][j] with
+// i = 0..order-1 spline node, j = 0..3 derivative level
+void bsplgen(double w, int bsorder, double* theta) {
+  double temp[kMaxOrder + 1][kMaxOrder + 1];
+  temp[2][2] = w;
+  temp[2][1] = 1.0 - w;
+  temp[3][3] = 0.5 * w * temp[2][2];
+  temp[3][2] = 0.5 * ((1.0 + w) * temp[2][1] + (2.0 - w) * temp[2][2]);
+  temp[3][1] = 0.5 * (1.0 - w) * temp[2][1];
+  for (int i = 4; i <= bsorder; ++i) {
+    const int k = i - 1;
+    const double denom = 1.0 / double(k);
+    temp[i][i] = denom * w * temp[k][k];
+    for (int j = 1; j <= i - 2; ++j)
+      temp[i][i - j] = denom * ((w + double(j)) * temp[k][i - j - 1] + (double(i - j) - w) * temp[k][i - j]);
+    temp[i][1] = denom * (1.0 - w) * temp[k][1];
+  }
+  // first derivative
+  int k = bsorder - 1;
+  temp[k][bsorder] = temp[k][bsorder - 1];
+  for (int i = bsorder - 1; i >= 2; --i) temp[k][i] = temp[k][i - 1] - temp[k][i];
+  temp[k][1] = -temp[k][1];
+  // second derivative
+  k = bsorder - 2;
+  temp[k][bsorder - 1] = temp[k][bsorder - 2];
+  for (int i = bsorder - 2; i >= 2; --i) temp[k][i] = temp[k][i - 1] - temp[k][i];
+  temp[k][1] = -temp[k][1];
+  temp[k][bsorder] = temp[k][bsorder - 1];
+  for (int i = bsorder - 1; i >= 2; --i) temp[k][i] = temp[k][i - 1] - temp[k][i];
+  temp[k][1] = -temp[k][1];
+  // third derivative
+  k = bsorder - 3;
+  temp[k][bsorder - 2] = temp[k][bsorder - 3];
+  for (int i = bsorder - 3; i >= 2; --i) temp[k][i] = temp[k][i - 1] - temp[k][i];
+  temp[k][1] = -temp[k][1];
+  temp[k][bsorder - 1] = temp[k][bsorder - 2];
+  for (int i = bsorder - 2; i >= 2; --i) temp[k][i] = temp[k][i - 1] - temp[k][i];
+  temp[k][1] = -temp[k][1];
+  temp[k][bsorder] = temp[k][bsorder - 1];
+  for (int i = bsorder - 1; i >= 2; --i) temp[k][i] = temp[k][i - 1] - temp[k][i];
+  temp[k][1] = -temp[k][1];
+  for (int i = 1; i <= bsorder; ++i)
+    for (int j = 1; j <= 4; ++j) theta[(i - 1) * 4 + (j - 1)] = temp[bsorder - j + 1][i];
+}
+
+// bspline, v1.2/source/kewald.f:720-745
+void bspline_coef(double x, int n, double* c) {
+  c[0] = 1.0 - x;
+  c[1] = x;
+  for (int k = 3; k <= n; ++k) {
+    const double denom = 1.0 / double(k - 1);
+    c[k - 1] = x * c[k - 2] * denom;
+    for (int i = 1; i <= k - 2; ++i)
+      c[k - i - 1] = ((x + double(i)) * c[k - i - 2] + (double(k - i) - x) * c[k - i - 1]) * denom;
+    c[0] = (1.0 - x) * c[0] * denom;
+  }
+}
+
+// dftmod, v1.2/source/kewald.f:756-812; 'zeta' selects the zeta correction
+void dftmod(double* bsmod, const double* bsarray, int nfft, int order, bool zeta_fix) {
+  const double factor = 2.0 * kPi / double(nfft);
+  for (int i = 1; i <= nfft; ++i) {
+    double sum1 = 0.0, sum2 = 0.0;
+    for (int j = 1; j <= nfft; ++j) {
+      const double arg = factor * double((long)(i - 1) * (j - 1));
+      sum1 += bsarray[j - 1] * std::cos(arg);
+      sum2 += bsarray[j - 1] * std::sin(arg);
+    }
+    bsmod[i - 1] = sum1 * sum1 + sum2 * sum2;
+  }
+  const double eps = 1.0e-7;
+  if (bsmod[0] < eps) bsmod[0] = 0.5 * bsmod[1];
+  for (int i = 2; i <= nfft - 1; ++i)
+    if (bsmod[i - 1] < eps) bsmod[i - 1] = 0.5 * (bsmod[i - 2] + bsmod[i]);
+  if (bsmod[nfft - 1] < eps) bsmod[nfft - 1] = 0.5 * bsmod[nfft - 2];
+  if (!zeta_fix) return;
+  const int jcut = 50;
+  const int order2 = 2 * order;
+  for (int i = 1; i <= nfft; ++i) {
+    int k = i - 1;
+    if (i > nfft / 2) k = k - nfft;
+    double zeta;
+    if (k == 0) {
+      zeta = 1.0;
+    } else {
+      double sum1 = 1.0, sum2 = 1.0;
+      const double fac = kPi * double(k) / double(nfft);
+      for (int j = 1; j <= jcut; ++j) {
+        const double arg = fac / (fac + kPi * double(j));
+        sum1 += std::pow(arg, order);
+        sum2 += std::pow(arg, order2);
+      }
+      for (int j = 1; j <= jcut; ++j) {
+        const double arg = fac / (fac - kPi * double(j));
+        sum1 += std::pow(arg, order);
+        sum2 += std::pow(arg, order2);
+      }
+      zeta = sum2 / sum1;
+    }
+    bsmod[i - 1] = bsmod[i - 1] * zeta * zeta;
+  }
+}
+
+// moduli, v1.2/source/kewald.f:672-705
+void moduli_1d(int nfft, int order, double* bsmod, bool zeta_fix) {
+  double array[kMaxOrder];
+  bspline_coef(0.0, order, array);
+  std::vector<double> bsarray(std::max(nfft, order + 2), 0.0);
+  for (int i = 2; i <= order + 1; ++i) bsarray[i - 1] = array[i - 2];
+  dftmod(bsmod, bsarray.data(), nfft, order, zeta_fix);
+}
+
+void ensure_moduli(orc_sys* s) {
+  if (!s->bsmod1.empty()) return;
+  s->bsmod1.resize(s->nfft1);
+  s->bsmod2.resize(s->nfft2);
+  s->bsmod3.resize(s->nfft3);
+  moduli_1d(s->nfft1, s->order, s->bsmod1.data(), true);
+  moduli_1d(s->nfft2, s->order, s->bsmod2.data(), true);
+  moduli_1d(s->nfft3, s->order, s->bsmod3.data(), true);
+}
+
+// ----------------------------------------------------------------- FFT
+// Unnormalised complex FFT along one line, mixed radix (any factors), used
+// for the 3-D C2C transform of fft2d_frontmpi/backmpi (v1.2/source/fft_mpi.f90:65-160).
+using cplx = std::complex<double>;
+
+void fft_line(cplx* line, int n, int sign) {
+  std::vector<cplx> in(line, line + n), out(n);
+  std::function<void(const cplx*, int, int, cplx*)> rec;
+  rec = [&](const cplx* a, int len, int stride, cplx* o) {
+    if (len == 1) {
+      o[0] = a[0];
+      return;
+    }
+    int p = 2;
+    while (p * p <= len && len % p) ++p;
+    if (len % p) p = len;
+    const int m = len / p;
+    std::vector<cplx> sub((size_t)len);
+    for (int r = 0; r < p; ++r) rec(a + (size_t)r * stride, m, stride * p, sub.data() + (size_t)r * m);
+    for (int k = 0; k < len; ++k) {
+      cplx acc = 0.0;
+      const int km = k % m;
+      for (int r = 0; r < p; ++r) {
+        const double ang = sign * 2.0 * kPi * double((long)r * k % len) / double(len);
+        acc += sub[(size_t)r * m + km] * cplx(std::cos(ang), std::sin(ang));
+      }
+      o[k] = acc;
+    }
+  };
+  rec(in.data(), n, 1, out.data());
+  std::copy(out.begin(), out.end(), line);
+}
+
+void fft3d(int n1, int n2, int n3, cplx* g, int sign) {
+  // along x (fastest)
+#pragma omp parallel for num_threads(n_threads())
+  for (int kj = 0; kj < n2 * n3; ++kj) fft_line(g + (size_t)kj * n1, n1, sign);
+  // along y
+#pragma omp parallel for num_threads(n_threads())
+  for (int ki = 0; ki < n1 * n3; ++ki) {
+    const int i = ki % n1, k = ki / n1;
+    std::vector<cplx> line(n2);
+    for (int j = 0; j < n2; ++j) line[j] = g[((size_t)k * n2 + j) * n1 + i];
+    fft_line(line.data(), n2, sign);
+    for (int j = 0; j < n2; ++j) g[((size_t)k * n2 + j) * n1 + i] = line[j];
+  }
+  // along z
+#pragma omp parallel for num_threads(n_threads())
+  for (int ji = 0; ji < n1 * n2; ++ji) {
+    std::vector<cplx> line(n3);
+    for (int k = 0; k < n3; ++k) line[k] = g[(size_t)k * n1 * n2 + ji];
+    fft_line(line.data(), n3, sign);
+    for (int k = 0; k < n3; ++k) g[(size_t)k * n1 * n2 + ji] = line[k];
+  }
+}
+
+// ------------------------------------------------------------- PME grid
+struct Site {
+  int igrid[3];
+  double th[3][kMaxOrder][4];  // thetai1/2/3(4,order)
+};
+
+// Fortran anint rounds half away from zero; nearbyint rounds half to even.
+inline double anint(double w) { return std::round(w); }
+
+// bspline_fill_site, v1.2/source/pmestuff.f:13-50
+void bspline_fill_site(const orc_sys* s, int i, Site& st) {
+  const double eps = 1.0e-8;
+  const double rc[3] = {1.0 / s->bx, 1.0 / s->by, 1.0 / s->bz};
+  const double xyz[3] = {s->x[i], s->y[i], s->z[i]};
+  const int nf[3] = {s->nfft1, s->nfft2, s->nfft3};
+  for (int d = 0; d < 3; ++d) {
+    double w = xyz[d] * rc[d];
+    const double fr = double(nf[d]) * (w - anint(w) + 0.5);
+    const int ifr = int(fr - eps);  // int() truncates toward zero like Fortran
+    w = fr - double(ifr);
+    st.igrid[d] = ifr - s->order;
+    double th[kMaxOrder * 4];
+    bsplgen(w, s->order, th);
+    for (int m = 0; m < s->order; ++m)
+      for (int l = 0; l < 4; ++l) st.th[d][m][l] = th[m * 4 + l];
+  }
+}
+
+// grid index (0-based) of spline node m (0-based) along one axis.
+// grid_*_site: nearpt = igrid+grdoff, points nearpt-nlpts .. nearpt+nrpts
+// (1-based), wrapped by +nfft when < 1 (pmestuff.f:190-215, adjust :404-440);
+// the same points are visited by fphi_*_site via k0+1+(nfft-isign(nfft,k0))/2.
+inline int node_index(int igrid, int m, int order, int nfft) {
+  const int grdoff = (order + 1) / 2 + 1;
+  const int nlpts = (order - 1) / 2;
+  int g1 = igrid + grdoff - nlpts + m;  // 1-based
+  if (g1 < 1) g1 += nfft;
+  return g1 - 1;
+}
+
+// cart_to_frac / cmp_to_fmp_site, v1.2/source/pmestuff.f:1088-1187
+void cart_to_frac(const orc_sys* s, double ctf[10][10]) {
+  double a[3][3] = {};
+  const double rc[3] = {1.0 / s->bx, 1.0 / s->by, 1.0 / s->bz};
+  const int nf[3] = {s->nfft1, s->nfft2, s->nfft3};
+  // a(k,i) = nfft_k * recip(i,k); orthorhombic: recip diagonal
+  for (int k = 0; k < 3; ++k) a[k][k] = nf[k] * rc[k];
+  static const int qi1[6] = {0, 1, 2, 0, 0, 1};
+  static const int qi2[6] = {0, 1, 2, 1, 2, 2};
+  for (int i = 0; i < 10; ++i)
+    for (int j = 0; j < 10; ++j) ctf[i][j] = 0.0;
+  ctf[0][0] = 1.0;
+  for (int i = 1; i < 4; ++i)
+    for (int j = 1; j < 4; ++j) ctf[i][j] = a[i - 1][j - 1];
+  for (int i1 = 0; i1 < 3; ++i1) {
+    const int k = qi1[i1];
+    for (int i2 = 0; i2 < 6; ++i2) {
+      const int i = qi1[i2], j = qi2[i2];
+      ctf[i1 + 4][i2 + 4] = a[k][i] * a[k][j];
+    }
+  }
+  for (int i1 = 3; i1 < 6; ++i1) {
+    const int k = qi1[i1], m = qi2[i1];
+    for (int i2 = 0; i2 < 6; ++i2) {
+      const int i = qi1[i2], j = qi2[i2];
+      ctf[i1 + 4][i2 + 4] = a[k][i] * a[m][j] + a[k][j] * a[m][i];
+    }
+  }
+}
+
+// frac_to_cart, v1.2/source/pmestuff.f:1230-1298
+void frac_to_cart(const orc_sys* s, double ftc[10][10]) {
+  double a[3][3] = {};
+  const double rc[3] = {1.0 / s->bx, 1.0 / s->by, 1.0 / s->bz};
+  const int nf[3] = {s->nfft1, s->nfft2, s->nfft3};
+  // a(i,k) = nfft_k * recip(i,k)
+  for (int k = 0; k < 3; ++k) a[k][k] = nf[k] * rc[k];
+  static const int qi1[6] = {0, 1, 2, 0, 0, 1};
+  static const int qi2[6] = {0, 1, 2, 1, 2, 2};
+  for (int i = 0; i < 10; ++i)
+    for (int j = 0; j < 10; ++j) ftc[i][j] = 0.0;
+  ftc[0][0] = 1.0;
+  for (int i = 1; i < 4; ++i)
+    for (int j = 1; j < 4; ++j) ftc[i][j] = a[i - 1][j - 1];
+  for (int i1 = 0; i1 < 3; ++i1) {
+    const int k = qi1[i1];
+    for (int i2 = 0; i2 < 3; ++i2) {
+      const int i = qi1[i2];
+      ftc[i1 + 4][i2 + 4] = a[k][i] * a[k][i];
+    }
+    for (int i2 = 3; i2 < 6; ++i2) {
+      const int i = qi1[i2], j = qi2[i2];
+      ftc[i1 + 4][i2 + 4] = 2.0 * a[k][i] * a[k][j];
+    }
+  }
+  for (int i1 = 3; i1 < 6; ++i1) {
+    const int k = qi1[i1], m = qi2[i1];
+    for (int i2 = 0; i2 < 3; ++i2) {
+      const int i = qi1[i2];
+      ftc[i1 + 4][i2 + 4] = a[k][i] * a[m][i];
+    }
+    for (int i2 = 3; i2 < 6; ++i2) {
+      const int i = qi1[i2], j = qi2[i2];
+      ftc[i1 + 4][i2 + 4] = a[k][i] * a[m][j] + a[m][i] * a[k][j];
+    }
+  }
+}
+
+// qfac scalar-sum table, v1.2/source/newinduce_pme.f:1790-1844 (periodic)
+void make_qfac(orc_sys* s, std::vector<double>& qfac) {
+  ensure_moduli(s);
+  const int n1 = s->nfft1, n2 = s->nfft2, n3 = s->nfft3;
+  qfac.assign((size_t)n1 * n2 * n3, 0.0);
+  const double pterm = (kPi / s->aewald) * (kPi / s->aewald);
+  const double volterm = kPi * s->bx * s->by * s->bz;
+  const int nf1 = (n1 + 1) / 2, nf2 = (n2 + 1) / 2, nf3 = (n3 + 1) / 2;
+  const double rc[3] = {1.0 / s->bx, 1.0 / s->by, 1.0 / s->bz};
+  for (int k3 = 1; k3 <= n3; ++k3)
+    for (int k2 = 1; k2 <= n2; ++k2)
+      for (int k1 = 1; k1 <= n1; ++k1) {
+        int m1 = k1 - 1, m2 = k2 - 1, m3 = k3 - 1;
+        if (k1 > nf1) m1 -= n1;
+        if (k2 > nf2) m2 -= n2;
+        if (k3 > nf3) m3 -= n3;
+        if (m1 == 0 && m2 == 0 && m3 == 0) continue;
+        const double h1 = rc[0] * m1, h2 = rc[1] * m2, h3 = rc[2] * m3;
+        const double hsq = h1 * h1 + h2 * h2 + h3 * h3;
+        const double term = -pterm * hsq;
+        double expterm = 0.0;
+        if (term > -50.0) {
+          const double denom = volterm * hsq * s->bsmod1[k1 - 1] * s->bsmod2[k2 - 1] * s->bsmod3[k3 - 1];
+          expterm = std::exp(term) / denom;
+        }
+        qfac[((size_t)(k3 - 1) * n2 + (k2 - 1)) * n1 + (k1 - 1)] = expterm;
+      }
+}
+
+void convolve(orc_sys* s, std::vector<cplx>& grid) {
+  std::vector<double> qfac;
+  make_qfac(s, qfac);
+  fft3d(s->nfft1, s->nfft2, s->nfft3, grid.data(), -1);
+  for (size_t i = 0; i < grid.size(); ++i) grid[i] *= qfac[i];
+  fft3d(s->nfft1, s->nfft2, s->nfft3, grid.data(), +1);
+}
+
+// efld0_recip, v1.2/source/newinduce_pme.f:1671-1917: returns cphi(10,n)
+void efld0_recip(orc_sys* s, double* cphi, double* fphi_out /* [n][20] or null */) {
+  const int n = s->n, order = s->order;
+  const int n1 = s->nfft1, n2 = s->nfft2, n3 = s->nfft3;
+  std::vector<cplx> grid((size_t)n1 * n2 * n3, 0.0);
+  std::vector<Site> sites(n);
+  double ctf[10][10];
+  cart_to_frac(s, ctf);
+  for (int i = 0; i < n; ++i) bspline_fill_site(s, i, sites[i]);
+  for (int i = 0; i < n; ++i) {
+    const double* rp = &s->rpole[(size_t)i * 13];
+    double cmp[10] = {rp[0], rp[1], rp[2], rp[3], rp[4], rp[8], rp[12], 2.0 * rp[5], 2.0 * rp[6], 2.0 * rp[9]};
+    double fmp[10];
+    fmp[0] = ctf[0][0] * cmp[0];
+    for (int j = 1; j < 4; ++j) {
+      fmp[j] = 0.0;
+      for (int k = 1; k < 4; ++k) fmp[j] += ctf[j][k] * cmp[k];
+    }
+    for (int j = 4; j < 10; ++j) {
+      fmp[j] = 0.0;
+      for (int k = 4; k < 10; ++k) fmp[j] += ctf[j][k] * cmp[k];
+    }
+    // grid_mpole_site, pmestuff.f:163-274
+    const Site& st = sites[i];
+    for (int mk = 0; mk < order; ++mk) {
+      const int k = node_index(st.igrid[2], mk, order, n3);
+      const double v0 = st.th[2][mk][0], v1 = st.th[2][mk][1], v2 = st.th[2][mk][2];
+      for (int mj = 0; mj < order; ++mj) {
+        const int j = node_index(st.igrid[1], mj, order, n2);
+        const double u0 = st.th[1][mj][0], u1 = st.th[1][mj][1], u2 = st.th[1][mj][2];
+        const double term0 = fmp[0] * u0 * v0 + fmp[2] * u1 * v0 + fmp[3] * u0 * v1 + fmp[5] * u2 * v0 +
+                             fmp[6] * u0 * v2 + fmp[9] * u1 * v1;
+        const double term1 = fmp[1] * u0 * v0 + fmp[7] * u1 * v0 + fmp[8] * u0 * v1;
+        const double term2 = fmp[4] * u0 * v0;
+        for (int mi = 0; mi < order; ++mi) {
+          const int ii = node_index(st.igrid[0], mi, order, n1);
+          const double t0 = st.th[0][mi][0], t1 = st.th[0][mi][1], t2 = st.th[0][mi][2];
+          grid[((size_t)k * n2 + j) * n1 + ii] += term0 * t0 + term1 * t1 + term2 * t2;
+        }
+      }
+    }
+  }
+  convolve(s, grid);
+  double ftc[10][10];
+  frac_to_cart(s, ftc);
+#pragma omp parallel for num_threads(n_threads())
+  for (int i = 0; i < n; ++i) {
+    // fphi_mpole_site, pmestuff.f:453-631
+    const Site& st = sites[i];
+    double tuv[20] = {};
+    for (int it3 = 0; it3 < order; ++it3) {
+      const int k = node_index(st.igrid[2], it3, order, n3);
+      const double v0 = st.th[2][it3][0], v1 = st.th[2][it3][1], v2 = st.th[2][it3][2], v3 = st.th[2][it3][3];
+      double tu00 = 0, tu10 = 0, tu01 = 0, tu20 = 0, tu11 = 0, tu02 = 0, tu30 = 0, tu21 = 0, tu12 = 0, tu03 = 0;
+      for (int it2 = 0; it2 < order; ++it2) {
+        const int j = node_index(st.igrid[1], it2, order, n2);
+        const double u0 = st.th[1][it2][0], u1 = st.th[1][it2][1], u2 = st.th[1][it2][2], u3 = st.th[1][it2][3];
+        double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+        for (int it1 = 0; it1 < order; ++it1) {
+          const int ii = node_index(st.igrid[0], it1, order, n1);
+          const double tq = grid[((size_t)k * n2 + j) * n1 + ii].real();
+          t0 += tq * st.th[0][it1][0];
+          t1 += tq * st.th[0][it1][1];
+          t2 += tq * st.th[0][it1][2];
+          t3 += tq * st.th[0][it1][3];
+        }
+        tu00 += t0 * u0;
+        tu10 += t1 * u0;
+        tu01 += t0 * u1;
+        tu20 += t2 * u0;
+        tu11 += t1 * u1;
+        tu02 += t0 * u2;
+        tu30 += t3 * u0;
+        tu21 += t2 * u1;
+        tu12 += t1 * u2;
+        tu03 += t0 * u3;
+      }
+      tuv[0] += tu00 * v0;
+      tuv[1] += tu10 * v0;
+      tuv[2] += tu01 * v0;
+      tuv[3] += tu00 * v1;
+      tuv[4] += tu20 * v0;
+      tuv[5] += tu02 * v0;
+      tuv[6] += tu00 * v2;
+      tuv[7] += tu11 * v0;
+      tuv[8] += tu10 * v1;
+      tuv[9] += tu01 * v1;
+      tuv[10] += tu30 * v0;
+      tuv[11] += tu03 * v0;
+      tuv[12] += tu00 * v3;
+      tuv[13] += tu21 * v0;
+      tuv[14] += tu20 * v1;
+      tuv[15] += tu12 * v0;
+      tuv[16] += tu02 * v1;
+      tuv[17] += tu10 * v2;
+      tuv[18] += tu01 * v2;
+      tuv[19] += tu11 * v1;
+    }
+    if (fphi_out)
+      for (int m = 0; m < 20; ++m) fphi_out[(size_t)i * 20 + m] = tuv[m];
+    // fphi_to_cphi_site, pmestuff.f:1190-1215
+    double* c = &cphi[(size_t)i * 10];
+    c[0] = ftc[0][0] * tuv[0];
+    for (int j = 1; j < 4; ++j) {
+      c[j] = 0.0;
+      for (int k = 1; k < 4; ++k) c[j] += ftc[j][k] * tuv[k];
+    }
+    for (int j = 4; j < 10; ++j) {
+      c[j] = 0.0;
+      for (int k = 4; k < 10; ++k) c[j] += ftc[j][k] * tuv[k];
+    }
+  }
+}
+
+// tmatxbrecip, v1.2/source/newinduce_pme.f:1919-2128
+void tmatxb_recip(orc_sys* s, const double* mu, double* dipfield) {
+  const int n = s->n, order = s->order;
+  const int n1 = s->nfft1, n2 = s->nfft2, n3 = s->nfft3;
+  std::vector<cplx> grid((size_t)n1 * n2 * n3, 0.0);
+  std::vector<Site> sites(n);
+  for (int i = 0; i < n; ++i) bspline_fill_site(s, i, sites[i]);
+  double a[3][3] = {};
+  a[0][0] = n1 / s->bx;
+  a[1][1] = n2 / s->by;
+  a[2][2] = n3 / s->bz;
+  for (int i = 0; i < n; ++i) {
+    double fuind[3], fuinp[3];
+    for (int k = 0; k < 3; ++k) {
+      fuind[k] = a[k][0] * mu[i * 6 + 0] + a[k][1] * mu[i * 6 + 1] + a[k][2] * mu[i * 6 + 2];
+      fuinp[k] = a[k][0] * mu[i * 6 + 3] + a[k][1] * mu[i * 6 + 4] + a[k][2] * mu[i * 6 + 5];
+    }
+    // grid_uind_site, pmestuff.f:280-400
+    const Site& st = sites[i];
+    for (int mk = 0; mk < order; ++mk) {
+      const int k = node_index(st.igrid[2], mk, order, n3);
+      const double v0 = st.th[2][mk][0], v1 = st.th[2][mk][1];
+      for (int mj = 0; mj < order; ++mj) {
+        const int j = node_index(st.igrid[1], mj, order, n2);
+        const double u0 = st.th[1][mj][0], u1 = st.th[1][mj][1];
+        const double term01 = fuind[1] * u1 * v0 + fuind[2] * u0 * v1;
+        const double term11 = fuind[0] * u0 * v0;
+        const double term02 = fuinp[1] * u1 * v0 + fuinp[2] * u0 * v1;
+        const double term12 = fuinp[0] * u0 * v0;
+        for (int mi = 0; mi < order; ++mi) {
+          const int ii = node_index(st.igrid[0], mi, order, n1);
+          const double t0 = st.th[0][mi][0], t1 = st.th[0][mi][1];
+          grid[((size_t)k * n2 + j) * n1 + ii] += cplx(term01 * t0 + term11 * t1, term02 * t0 + term12 * t1);
+        }
+      }
+    }
+  }
+  convolve(s, grid);
+#pragma omp parallel for num_threads(n_threads())
+  for (int i = 0; i < n; ++i) {
+    // fphi_uind_site (first derivatives only are consumed), pmestuff.f:746-1037
+    const Site& st = sites[i];
+    double p1[3] = {}, p2[3] = {};
+    for (int it3 = 0; it3 < order; ++it3) {
+      const int k = node_index(st.igrid[2], it3, order, n3);
+      const double v0 = st.th[2][it3][0], v1 = st.th[2][it3][1];
+      double tu00_1 = 0, tu10_1 = 0, tu01_1 = 0, tu00_2 = 0, tu10_2 = 0, tu01_2 = 0;
+      for (int it2 = 0; it2 < order; ++it2) {
+        const int j = node_index(st.igrid[1], it2, order, n2);
+        const double u0 = st.th[1][it2][0], u1 = st.th[1][it2][1];
+        double t0_1 = 0, t1_1 = 0, t0_2 = 0, t1_2 = 0;
+        for (int it1 = 0; it1 < order; ++it1) {
+          const int ii = node_index(st.igrid[0], it1, order, n1);
+          const cplx q = grid[((size_t)k * n2 + j) * n1 + ii];
+          t0_1 += q.real() * st.th[0][it1][0];
+          t1_1 += q.real() * st.th[0][it1][1];
+          t0_2 += q.imag() * st.th[0][it1][0];
+          t1_2 += q.imag() * st.th[0][it1][1];
+        }
+        tu00_1 += t0_1 * u0;
+        tu10_1 += t1_1 * u0;
+        tu01_1 += t0_1 * u1;
+        tu00_2 += t0_2 * u0;
+        tu10_2 += t1_2 * u0;
+        tu01_2 += t0_2 * u1;
+      }
+      p1[0] += tu10_1 * v0;
+      p1[1] += tu01_1 * v0;
+      p1[2] += tu00_1 * v1;
+      p2[0] += tu10_2 * v0;
+      p2[1] += tu01_2 * v0;
+      p2[2] += tu00_2 * v1;
+    }
+    for (int k = 0; k < 3; ++k) {
+      dipfield[i * 6 + k] = a[k][0] * p1[0] + a[k][1] * p1[1] + a[k][2] * p1[2];
+      dipfield[i * 6 + 3 + k] = a[k][0] * p2[0] + a[k][1] * p2[1] + a[k][2] * p2[2];
+    }
+  }
+}
+
+// efld0_direct, v1.2/source/newinduce_pme.f:948-1389 (Thole branch)
+void efld0_direct(orc_sys* s, double* ef) {
+  ensure_list(s);
+  const int n = s->n;
+  const double cutoff2 = s->cut * s->cut;
+  const int nt = n_threads();
+  std::vector<std::vector<double>> priv(nt, std::vector<double>((size_t)n * 6, 0.0));
+#pragma omp parallel num_threads(nt)
+  {
+#ifdef _OPENMP
+    const int tid = omp_get_thread_num();
+#else
+    const int tid = 0;
+#endif
+    double* e = priv[tid].data();
+#pragma omp for schedule(dynamic, 16)
+    for (int i = 0; i < n; ++i) {
+      const double* rpi = &s->rpole[(size_t)i * 13];
+      const double ci = rpi[0], dix = rpi[1], diy = rpi[2], diz = rpi[3];
+      const double qixx = rpi[4], qixy = rpi[5], qixz = rpi[6], qiyy = rpi[8], qiyz = rpi[9], qizz = rpi[12];
+      for (int k : s->elst[i]) {
+        double xr = s->x[k] - s->x[i];
+        double yr = s->y[k] - s->y[i];
+        double zr = s->z[k] - s->z[i];
+        image(s, xr, yr, zr);
+        const double r2 = xr * xr + yr * yr + zr * zr;
+        if (r2 > cutoff2) continue;
+        const double r = std::sqrt(r2);
+        const double rr1 = 1.0 / r;
+        const double rr2 = rr1 * rr1;
+        const double rr3 = rr2 * rr1;
+        const double rr5 = 3.0 * rr2 * rr3;
+        const double rr7 = 5.0 * rr2 * rr5;
+        const double* rpk = &s->rpole[(size_t)k * 13];
+        const double ck = rpk[0], dkx = rpk[1], dky = rpk[2], dkz = rpk[3];
+        const double qkxx = rpk[4], qkxy = rpk[5], qkxz = rpk[6], qkyy = rpk[8], qkyz = rpk[9], qkzz = rpk[12];
+        const double dir = dix * xr + diy * yr + diz * zr;
+        const double qix = qixx * xr + qixy * yr + qixz * zr;
+        const double qiy = qixy * xr + qiyy * yr + qiyz * zr;
+        const double qiz = qixz * xr + qiyz * yr + qizz * zr;
+        const double qir = qix * xr + qiy * yr + qiz * zr;
+        const double dkr = dkx * xr + dky * yr + dkz * zr;
+        const double qkx = qkxx * xr + qkxy * yr + qkxz * zr;
+        const double qky = qkxy * xr + qkyy * yr + qkyz * zr;
+        const double qkz = qkxz * xr + qkyz * yr + qkzz * zr;
+        const double qkr = qkx * xr + qky * yr + qkz * zr;
+        double dmpe[7], dmpik[9];
+        dampewald(s->aewald, r, r2, 1.0, dmpe, 7);
+        dampthole(s->pdamp[i], s->pdamp[k], thlval(s, i, k), r, 7, dmpik);
+        double dsc, psc, usc;
+        pair_scales(s, i, k, dsc, psc, usc);
+        double fid[3], fkd[3], fip[3], fkp[3];
+        for (int pass = 0; pass < 2; ++pass) {
+          const double scalek = pass == 0 ? dsc : psc;
+          const double dmp3 = dmpe[2] - (1.0 - scalek * dmpik[2]) * rr3;
+          const double dmp5 = dmpe[4] - (1.0 - scalek * dmpik[4]) * rr5;
+          const double dmp7 = dmpe[6] - (1.0 - scalek * dmpik[6]) * rr7;
+          double* fi = pass == 0 ? fid : fip;
+          double* fk = pass == 0 ? fkd : fkp;
+          const double ai = dmp3 * ck - dmp5 * dkr + dmp7 * qkr;
+          fi[0] = -xr * ai - dmp3 * dkx + 2.0 * dmp5 * qkx;
+          fi[1] = -yr * ai - dmp3 * dky + 2.0 * dmp5 * qky;
+          fi[2] = -zr * ai - dmp3 * dkz + 2.0 * dmp5 * qkz;
+          const double ak = dmp3 * ci + dmp5 * dir + dmp7 * qir;
+          fk[0] = xr * ak - dmp3 * dix - 2.0 * dmp5 * qix;
+          fk[1] = yr * ak - dmp3 * diy - 2.0 * dmp5 * qiy;
+          fk[2] = zr * ak - dmp3 * diz - 2.0 * dmp5 * qiz;
+        }
+        for (int j = 0; j < 3; ++j) {
+          e[(size_t)i * 6 + j] += fid[j];
+          e[(size_t)k * 6 + j] += fkd[j];
+          e[(size_t)i * 6 + 3 + j] += fip[j];
+          e[(size_t)k * 6 + 3 + j] += fkp[j];
+        }
+      }
+    }
+  }
+  for (size_t m = 0; m < (size_t)n * 6; ++m) {
+    double acc = 0.0;
+    for (int t = 0; t < nt; ++t) acc += priv[t][m];
+    ef[m] = acc;
+  }
+}
+
+// tmatxb_pme, v1.2/source/newinduce_pme.f:1392-1668 (Thole branch)
+void tmatxb(orc_sys* s, const double* mu, double* efi, int dodiag) {
+  ensure_list(s);
+  const int n = s->n;
+  const double off2 = s->cut * s->cut;
+  const int nt = n_threads();
+  std::vector<std::vector<double>> priv(nt, std::vector<double>((size_t)n * 6, 0.0));
+#pragma omp parallel num_threads(nt)
+  {
+#ifdef _OPENMP
+    const int tid = omp_get_thread_num();
+#else
+    const int tid = 0;
+#endif
+    double* e = priv[tid].data();
+#pragma omp for schedule(dynamic, 16)
+    for (int i = 0; i < n; ++i) {
+      const double* mi = &mu[(size_t)i * 6];
+      for (int k : s->elst[i]) {
+        double xr = s->x[k] - s->x[i];
+        double yr = s->y[k] - s->y[i];
+        double zr = s->z[k] - s->z[i];
+        image(s, xr, yr, zr);
+        const double r2 = xr * xr + yr * yr + zr * zr;
+        if (r2 > off2) continue;
+        const double r = std::sqrt(r2);
+        const double rr1 = 1.0 / r;
+        const double rr2 = rr1 * rr1;
+        const double rr3 = rr2 * rr1;
+        const double rr5 = 3.0 * rr2 * rr3;
+        const double* mk = &mu[(size_t)k * 6];
+        double dmpe[7], dmpik[9];
+        dampewald(s->aewald, r, r2, 1.0, dmpe, 7);
+        dampthole(s->pdamp[i], s->pdamp[k], thlval(s, i, k), r, 5, dmpik);
+        double dsc, psc, usc;
+        pair_scales(s, i, k, dsc, psc, usc);
+        const double dmp3 = dmpe[2] - (1.0 - usc * dmpik[2]) * rr3;
+        const double dmp5 = dmpe[4] - (1.0 - usc * dmpik[4]) * rr5;
+        const double d1 = -dmp3 + dmp5 * xr * xr, d2 = dmp5 * xr * yr, d3 = dmp5 * xr * zr;
+        const double d4 = -dmp3 + dmp5 * yr * yr, d5 = dmp5 * yr * zr, d6 = -dmp3 + dmp5 * zr * zr;
+        for (int rhs = 0; rhs < 2; ++rhs) {
+          const double ukx = mk[rhs * 3], uky = mk[rhs * 3 + 1], ukz = mk[rhs * 3 + 2];
+          const double uix = mi[rhs * 3], uiy = mi[rhs * 3 + 1], uiz = mi[rhs * 3 + 2];
+          e[(size_t)i * 6 + rhs * 3 + 0] -= d1 * ukx + d2 * uky + d3 * ukz;
+          e[(size_t)i * 6 + rhs * 3 + 1] -= d2 * ukx + d4 * uky + d5 * ukz;
+          e[(size_t)i * 6 + rhs * 3 + 2] -= d3 * ukx + d5 * uky + d6 * ukz;
+          e[(size_t)k * 6 + rhs * 3 + 0] -= d1 * uix + d2 * uiy + d3 * uiz;
+          e[(size_t)k * 6 + rhs * 3 + 1] -= d2 * uix + d4 * uiy + d5 * uiz;
+          e[(size_t)k * 6 + rhs * 3 + 2] -= d3 * uix + d5 * uiy + d6 * uiz;
+        }
+      }
+    }
+  }
+  for (size_t m = 0; m < (size_t)n * 6; ++m) {
+    double acc = 0.0;
+    for (int t = 0; t < nt; ++t) acc += priv[t][m];
+    efi[m] = acc;
+  }
+  if (dodiag) {
+    for (int i = 0; i < n; ++i) {
+      const double pol = s->polarity[i] == 0.0 ? 1.0 / kTinyPol : 1.0 / s->polarity[i];
+      for (int m = 0; m < 6; ++m) efi[(size_t)i * 6 + m] += mu[(size_t)i * 6 + m] * pol;
+    }
+  }
+}
+
+double self_term(const orc_sys* s) {
+  return (4.0 / 3.0) * s->aewald * s->aewald * s->aewald / kSqrtPi;
+}
+
+// total permanent field: newinduce_pme2.f:111-147
+void efld0_total(orc_sys* s, double* ef) {
+  const int n = s->n;
+  efld0_direct(s, ef);
+  std::vector<double> cphi((size_t)n * 10);
+  efld0_recip(s, cphi.data(), nullptr);
+  const double term = self_term(s);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const double d = -cphi[(size_t)i * 10 + 1 + j] + term * s->rpole[(size_t)i * 13 + 1 + j];
+      ef[(size_t)i * 6 + j] += d;
+      ef[(size_t)i * 6 + 3 + j] += d;
+    }
+}
+
+// h = tmatxb(mu) (with diag) + dipfield(mu) - term*mu: newinduce_pme2.f:406-424
+void matvec_full(orc_sys* s, const double* mu, double* h) {
+  const int n = s->n;
+  std::vector<double> dip((size_t)n * 6);
+  tmatxb_recip(s, mu, dip.data());
+  tmatxb(s, mu, h, 1);
+  const double term = self_term(s);
+  for (size_t m = 0; m < (size_t)n * 6; ++m) h[m] = h[m] + dip[m] - term * mu[m];
+}
+
+}  // namespace
+
+// ===================================================================== API
+extern "C" {
+
+orc_sys* orc_create(int n, const double* box, const double* x, const double* y, const double* z,
+                    const double* rpole, const double* polarity, const double* thole, int nscaled,
+                    const int* ik, const double* dpu, double aewald, double cut, double buffer, int nfft1,
+                    int nfft2, int nfft3, int order) {
+  if (n <= 0 || order < 3 || order > kMaxOrder) return nullptr;
+  orc_sys* s = new orc_sys;
+  s->n = n;
+  s->bx = box[0];
+  s->by = box[1];
+  s->bz = box[2];
+  s->x.assign(x, x + n);
+  s->y.assign(y, y + n);
+  s->z.assign(z, z + n);
+  s->rpole.assign(rpole, rpole + (size_t)13 * n);
+  s->polarity.assign(polarity, polarity + n);
+  s->thole.assign(thole, thole + n);
+  s->pdamp.resize(n);
+  // kpolar.f:235: pdamp = polarity**(1/6)
+  for (int i = 0; i < n; ++i) s->pdamp[i] = std::pow(s->polarity[i], 1.0 / 6.0);
+  s->scl.assign(n, {});
+  for (int m = 0; m < nscaled; ++m) {
+    const int a = ik[2 * m], b = ik[2 * m + 1];
+    s->scl[a].push_back({b, dpu[3 * m], dpu[3 * m + 1], dpu[3 * m + 2]});
+    s->scl[b].push_back({a, dpu[3 * m], dpu[3 * m + 1], dpu[3 * m + 2]});
+  }
+  s->aewald = aewald;
+  s->cut = cut;
+  s->buffer = buffer;
+  s->nfft1 = nfft1;
+  s->nfft2 = nfft2;
+  s->nfft3 = nfft3;
+  s->order = order;
+  return s;
+}
+
+void orc_destroy(orc_sys* s) { delete s; }
+
+void orc_set_coords(orc_sys* s, const double* x, const double* y, const double* z) {
+  s->x.assign(x, x + s->n);
+  s->y.assign(y, y + s->n);
+  s->z.assign(z, z + s->n);
+}
+
+void orc_set_threads(int nthreads) { nthreads_cap = nthreads; }
+
+long orc_nblist(orc_sys* s) {
+  build_list(s);
+  long c = 0;
+  for (auto& l : s->elst) c += (long)l.size();
+  return c;
+}
+
+void orc_nblist_pairs(orc_sys* s, int* ik) {
+  ensure_list(s);
+  long m = 0;
+  for (int i = 0; i < s->n; ++i)
+    for (int k : s->elst[i]) {
+      ik[2 * m] = i;
+      ik[2 * m + 1] = k;
+      ++m;
+    }
+}
+
+void orc_efld0_direct(orc_sys* s, double* ef) { efld0_direct(s, ef); }
+void orc_efld0_recip(orc_sys* s, double* cphi) { efld0_recip(s, cphi, nullptr); }
+void orc_efld0_total(orc_sys* s, double* ef) { efld0_total(s, ef); }
+void orc_tmatxb(orc_sys* s, const double* mu, double* efi, int dodiag) { tmatxb(s, mu, efi, dodiag); }
+void orc_tmatxb_recip(orc_sys* s, const double* mu, double* dipfield) { tmatxb_recip(s, mu, dipfield); }
+void orc_matvec_full(orc_sys* s, const double* mu, double* h) { matvec_full(s, mu, h); }
+
+// newinduce_pme2 + inducepcg_pme2, v1.2/source/newinduce_pme2.f:16-636
+int orc_induce(orc_sys* s, double poleps, int politer, int guess, const double* mu0, double* mu,
+               double* ef_out, double* ep, int* niter, double* resnrm_out, double* enehist) {
+  const int n = s->n;
+  const size_t nv = (size_t)n * 6;
+  std::vector<double> ef(nv);
+  efld0_total(s, ef.data());
+  if (ef_out) std::copy(ef.begin(), ef.end(), ef_out);
+  // guess: polgsf = 0 -> mu = polarity*ef (:180-188); polgsf = 1 -> previous dipoles
+  for (int i = 0; i < n; ++i)
+    for (int m = 0; m < 6; ++m)
+      mu[(size_t)i * 6 + m] = guess ? mu0[(size_t)i * 6 + m] : s->polarity[i] * ef[(size_t)i * 6 + m];
+  std::vector<double> res(nv), h(nv), pp(nv), zr(nv), diag(n);
+  for (int i = 0; i < n; ++i) diag[i] = s->polarity[i] == 0.0 ? kTinyPol : s->polarity[i];
+  matvec_full(s, mu, h.data());
+  double ggold[2] = {0, 0}, ggnew[2], gg[2], ene[2], gnorm[2];
+  for (int i = 0; i < n; ++i)
+    for (int m = 0; m < 6; ++m) {
+      const size_t q = (size_t)i * 6 + m;
+      res[q] = ef[q] - h[q];
+      zr[q] = diag[i] * res[q];
+      pp[q] = zr[q];
+      ggold[m / 3] += res[q] * zr[q];
+    }
+  int status = 1, it;
+  double resnrm = 0.0;
+  for (it = 1; it <= politer; ++it) {
+    matvec_full(s, pp.data(), h.data());
+    gg[0] = gg[1] = 0.0;
+    for (size_t q = 0; q < nv; ++q) gg[(q % 6) / 3] += pp[q] * h[q];
+    if (gg[0] == 0.0 || gg[1] == 0.0) {
+      status = 2;
+      break;
+    }
+    const double alpha[2] = {ggold[0] / gg[0], ggold[1] / gg[1]};
+    for (size_t q = 0; q < nv; ++q) {
+      const int k = (q % 6) / 3;
+      mu[q] = mu[q] + alpha[k] * pp[q];
+      res[q] = res[q] - alpha[k] * h[q];
+    }
+    ggnew[0] = ggnew[1] = ene[0] = ene[1] = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int m = 0; m < 6; ++m) {
+        const size_t q = (size_t)i * 6 + m;
+        zr[q] = diag[i] * res[q];
+        ggnew[m / 3] += res[q] * zr[q];
+        ene[m / 3] += -0.5 * (mu[q] * (res[q] + ef[q]));
+      }
+    resnrm = 0.0;
+    for (int k = 0; k < 2; ++k) {
+      gnorm[k] = std::sqrt(ggnew[k] / double(3 * n));
+      resnrm = std::max(resnrm, gnorm[k]);
+    }
+    if (enehist) {
+      enehist[(it - 1) * 2] = ene[0] * kCoulomb;
+      enehist[(it - 1) * 2 + 1] = ene[1] * kCoulomb;
+    }
+    for (size_t q = 0; q < nv; ++q) {
+      const int k = (q % 6) / 3;
+      pp[q] = zr[q] + ggnew[k] / ggold[k] * pp[q];
+    }
+    ggold[0] = ggnew[0];
+    ggold[1] = ggnew[1];
+    if (resnrm < poleps) {
+      status = 0;
+      break;
+    }
+  }
+  if (niter) *niter = std::min(it, politer);
+  if (resnrm_out) *resnrm_out = resnrm;
+  // E_pol = -1/2 f sum mu_d . E_p (epolar0c decomposition identity)
+  double e = 0.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < 3; ++j) e += mu[(size_t)i * 6 + j] * ef[(size_t)i * 6 + 3 + j];
+  if (ep) *ep = -0.5 * kCoulomb * e;
+  return status;
+}
+
+// epolar0c real + recip + self, v1.2/source/epolar.f:47-205, 216-480, 503-712
+void orc_epolar_decomp(orc_sys* s, const double* uind, double* ereal, double* erecip, double* eself) {
+  ensure_list(s);
+  const int n = s->n;
+  const double f = 0.5 * kCoulomb;  // epreal0c: f = 0.5*electric/dielec
+  const double off2 = s->cut * s->cut;
+  double er = 0.0;
+#pragma omp parallel for reduction(+ : er) schedule(dynamic, 16) num_threads(n_threads())
+  for (int i = 0; i < n; ++i) {
+    const double* rpi = &s->rpole[(size_t)i * 13];
+    const double ci = rpi[0], dix = rpi[1], diy = rpi[2], diz = rpi[3];
+    const double qixx = rpi[4], qixy = rpi[5], qixz = rpi[6], qiyy = rpi[8], qiyz = rpi[9], qizz = rpi[12];
+    const double uix = uind[i * 3], uiy = uind[i * 3 + 1], uiz = uind[i * 3 + 2];
+    for (int k : s->elst[i]) {
+      double xr = s->x[k] - s->x[i];
+      double yr = s->y[k] - s->y[i];
+      double zr = s->z[k] - s->z[i];
+      image(s, xr, yr, zr);
+      const double r2 = xr * xr + yr * yr + zr * zr;
+      if (r2 > off2) continue;
+      const double r = std::sqrt(r2);
+      const double* rpk = &s->rpole[(size_t)k * 13];
+      const double ck = rpk[0], dkx = rpk[1], dky = rpk[2], dkz = rpk[3];
+      const double qkxx = rpk[4], qkxy = rpk[5], qkxz = rpk[6], qkyy = rpk[8], qkyz = rpk[9], qkzz = rpk[12];
+      const double ukx = uind[k * 3], uky = uind[k * 3 + 1], ukz = uind[k * 3 + 2];
+      const double dir = dix * xr + diy * yr + diz * zr;
+      const double qix = qixx * xr + qixy * yr + qixz * zr;
+      const double qiy = qixy * xr + qiyy * yr + qiyz * zr;
+      const double qiz = qixz * xr + qiyz * yr + qizz * zr;
+      const double qir = qix * xr + qiy * yr + qiz * zr;
+      const double dkr = dkx * xr + dky * yr + dkz * zr;
+      const double qkx = qkxx * xr + qkxy * yr + qkxz * zr;
+      const double qky = qkxy * xr + qkyy * yr + qkyz * zr;
+      const double qkz = qkxz * xr + qkyz * yr + qkzz * zr;
+      const double qkr = qkx * xr + qky * yr + qkz * zr;
+      const double diu = dix * ukx + diy * uky + diz * ukz;
+      const double qiu = qix * ukx + qiy * uky + qiz * ukz;
+      const double uir = uix * xr + uiy * yr + uiz * zr;
+      const double dku = dkx * uix + dky * uiy + dkz * uiz;
+      const double qku = qkx * uix + qky * uiy + qkz * uiz;
+      const double ukr = ukx * xr + uky * yr + ukz * zr;
+      double dmpe[7], dmpik[9];
+      dampewald(s->aewald, r, r2, f, dmpe, 7);
+      dampthole(s->pdamp[i], s->pdamp[k], thlval(s, i, k), r, 7, dmpik);
+      double dsc, psc, usc;
+      pair_scales(s, i, k, dsc, psc, usc);
+      const double rr3 = f / (r * r2);
+      const double rr5 = 3.0 * rr3 / r2;
+      const double rr7 = 5.0 * rr5 / r2;
+      double sr3 = psc * dmpik[2] * rr3, sr5 = psc * dmpik[4] * rr5, sr7 = psc * dmpik[6] * rr7;
+      sr3 = dmpe[2] - rr3 + sr3;
+      sr5 = dmpe[4] - rr5 + sr5;
+      sr7 = dmpe[6] - rr7 + sr7;
+      const double term1 = ck * uir - ci * ukr + diu + dku;
+      const double term2 = 2.0 * (qiu - qku) - uir * dkr - dir * ukr;
+      const double term3 = uir * qkr - ukr * qir;
+      er += term1 * sr3 + term2 * sr5 + term3 * sr7;
+    }
+  }
+  // eprecip: e = 1/2 electric sum fuind . fphirec(2:4)
+  std::vector<double> cphi((size_t)n * 10), fphi((size_t)n * 20);
+  efld0_recip(s, cphi.data(), fphi.data());
+  double a[3] = {s->nfft1 / s->bx, s->nfft2 / s->by, s->nfft3 / s->bz};
+  double erc = 0.0;
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) erc += a[k] * uind[i * 3 + k] * fphi[(size_t)i * 20 + k + 1];
+  erc *= 0.5 * kCoulomb;
+  // self: fterm*term*uii/3 with term = 2 a^2, fterm = -f a / sqrt(pi), f = electric
+  const double term = 2.0 * s->aewald * s->aewald;
+  const double fterm = -kCoulomb * s->aewald / kSqrtPi;
+  double es = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double* rp = &s->rpole[(size_t)i * 13];
+    const double uii = rp[1] * uind[i * 3] + rp[2] * uind[i * 3 + 1] + rp[3] * uind[i * 3 + 2];
+    es += fterm * term * uii / 3.0;
+  }
+  *ereal = er;
+  *erecip = erc;
+  *eself = es;
+}
+
+void orc_bsplgen(double w, int order, double* theta) { bsplgen(w, order, theta); }
+void orc_bsmod(int nfft, int order, double* bsmod) { moduli_1d(nfft, order, bsmod, true); }
+void orc_bsmod_raw(int nfft, int order, double* bsmod) { moduli_1d(nfft, order, bsmod, false); }
+
+void orc_fft3d(int n1, int n2, int n3, double* data, int sign) {
+  fft3d(n1, n2, n3, reinterpret_cast<cplx*>(data), sign);
+}
+
+void orc_dampewald(double aewald, double r, double* dmpe) { dampewald(aewald, r, r * r, 1.0, dmpe, 7); }
+
+void orc_damptholed(double pdi, double pdk, double pgamma, double r, int rorder, double* dmpik) {
+  dampthole(pdi, pdk, pgamma, r, rorder, dmpik);
+}
+
+double orc_erfc(double x) { return std::erfc(x); }
+
+}  // extern "C"

@@ -1,0 +1,187 @@
+"""ctypes wrapper of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker of the HIP product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} not built (make -C oracle)")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "orc_create": (vp, [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, _ip, _dp,
+                            C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "orc_destroy": (None, [vp]),
+        "orc_set_coords": (None, [vp, _dp, _dp, _dp]),
+        "orc_set_threads": (None, [C.c_int]),
+        "orc_nblist": (C.c_long, [vp]),
+        "orc_nblist_pairs": (None, [vp, _ip]),
+        "orc_efld0_direct": (None, [vp, _dp]),
+        "orc_efld0_recip": (None, [vp, _dp]),
+        "orc_efld0_total": (None, [vp, _dp]),
+        "orc_tmatxb": (None, [vp, _dp, _dp, C.c_int]),
+        "orc_tmatxb_recip": (None, [vp, _dp, _dp]),
+        "orc_matvec_full": (None, [vp, _dp, _dp]),
+        "orc_induce": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _ip, _dp, _dp]),
+        "orc_epolar_decomp": (None, [vp, _dp, _dp, _dp, _dp]),
+        "orc_bsplgen": (None, [C.c_double, C.c_int, _dp]),
+        "orc_bsmod": (None, [C.c_int, C.c_int, _dp]),
+        "orc_bsmod_raw": (None, [C.c_int, C.c_int, _dp]),
+        "orc_fft3d": (None, [C.c_int, C.c_int, C.c_int, _dp, C.c_int]),
+        "orc_dampewald": (None, [C.c_double, C.c_double, _dp]),
+        "orc_damptholed": (None, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp]),
+        "orc_erfc": (C.c_double, [C.c_double]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def set_threads(n: int):
+    load().orc_set_threads(int(n))
+
+
+class Oracle:
+    """FP64 restatement of the reference CPU path for one system."""
+
+    def __init__(self, system, aewald=None, cut=None, buffer=None, nfft=None):
+        lib = load()
+        s = system
+        self.n = s.n
+        self.sys = s
+        self._arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                      (s.box, s.x, s.y, s.z, s.rpole, s.polarity, s.thole)]
+        self._ik = np.ascontiguousarray(s.scaled_ik, dtype=np.int32).reshape(-1)
+        self._dpu = np.ascontiguousarray(s.scaled_dpu, dtype=np.float64).reshape(-1)
+        nf = nfft if nfft is not None else s.nfft
+        self.h = lib.orc_create(s.n, *(_d(a) for a in self._arrs), len(self._ik) // 2,
+                                self._ik.ctypes.data_as(_ip), _d(self._dpu),
+                                aewald if aewald is not None else s.aewald,
+                                cut if cut is not None else s.cut,
+                                buffer if buffer is not None else s.buffer,
+                                int(nf[0]), int(nf[1]), int(nf[2]), s.order)
+        if not self.h:
+            raise RuntimeError("orc_create failed")
+        self.lib = lib
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def set_coords(self, x, y, z):
+        self._xyz = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z)]
+        self.lib.orc_set_coords(self.h, *(_d(v) for v in self._xyz))
+
+    def nblist_pairs(self) -> np.ndarray:
+        cnt = self.lib.orc_nblist(self.h)
+        ik = np.zeros((cnt, 2), dtype=np.int32)
+        self.lib.orc_nblist_pairs(self.h, ik.ctypes.data_as(_ip))
+        return ik
+
+    def efld0_direct(self):
+        ef = np.zeros((self.n, 2, 3))
+        self.lib.orc_efld0_direct(self.h, _d(ef))
+        return ef
+
+    def efld0_recip(self):
+        c = np.zeros((self.n, 10))
+        self.lib.orc_efld0_recip(self.h, _d(c))
+        return c
+
+    def efld0_total(self):
+        ef = np.zeros((self.n, 2, 3))
+        self.lib.orc_efld0_total(self.h, _d(ef))
+        return ef
+
+    def tmatxb_pme(self, mu, dodiag=True):
+        mu = np.ascontiguousarray(mu, dtype=np.float64).reshape(self.n, 2, 3)
+        out = np.zeros_like(mu)
+        self.lib.orc_tmatxb(self.h, _d(mu), _d(out), 1 if dodiag else 0)
+        return out
+
+    def tmatxbrecip(self, mu):
+        mu = np.ascontiguousarray(mu, dtype=np.float64).reshape(self.n, 2, 3)
+        out = np.zeros_like(mu)
+        self.lib.orc_tmatxb_recip(self.h, _d(mu), _d(out))
+        return out
+
+    def matvec(self, mu):
+        mu = np.ascontiguousarray(mu, dtype=np.float64).reshape(self.n, 2, 3)
+        out = np.zeros_like(mu)
+        self.lib.orc_matvec_full(self.h, _d(mu), _d(out))
+        return out
+
+    def induce(self, poleps=1e-8, politer=500, mu0=None):
+        mu = np.zeros((self.n, 2, 3))
+        ef = np.zeros((self.n, 2, 3))
+        m0 = np.ascontiguousarray(mu0 if mu0 is not None else mu, dtype=np.float64)
+        ep = C.c_double(0)
+        nit = C.c_int(0)
+        rn = C.c_double(0)
+        hist = np.zeros((politer, 2))
+        st = self.lib.orc_induce(self.h, poleps, politer, 1 if mu0 is not None else 0, _d(m0), _d(mu),
+                                 _d(ef), C.byref(ep), C.byref(nit), C.byref(rn), _d(hist))
+        return dict(mu=mu, ef=ef, ep=ep.value, niter=nit.value, resnrm=rn.value, status=st,
+                    ene_hist=hist[:nit.value] if nit.value > 0 else hist[:0])
+
+    def epolar_decomp(self, uind):
+        uind = np.ascontiguousarray(uind, dtype=np.float64).reshape(self.n, 3)
+        a, b, c = C.c_double(0), C.c_double(0), C.c_double(0)
+        self.lib.orc_epolar_decomp(self.h, _d(uind), C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+
+def bsplgen(w, order=5):
+    th = np.zeros((order, 4))
+    load().orc_bsplgen(w, order, _d(th))
+    return th
+
+
+def bsmod(nfft, order=5, zeta=True):
+    b = np.zeros(nfft)
+    (load().orc_bsmod if zeta else load().orc_bsmod_raw)(nfft, order, _d(b))
+    return b
+
+
+def fft3d(grid, sign):
+    g = np.ascontiguousarray(grid, dtype=np.complex128).copy()
+    n3, n2, n1 = g.shape
+    load().orc_fft3d(n1, n2, n3, g.ctypes.data_as(_dp), sign)
+    return g
+
+
+def dampewald(aewald, r):
+    d = np.zeros(7)
+    load().orc_dampewald(aewald, r, _d(d))
+    return d
+
+
+def dampthole(pdi, pdk, pgamma, r, order=7):
+    d = np.zeros(9)
+    load().orc_damptholed(pdi, pdk, pgamma, r, order, _d(d))
+    return d

@@ -1,0 +1,941 @@
+// libthp host side: context, systems, the path entry points of the C ABI
+// and the PCG driver.  Kernels live in thp_kernels.hip.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <tuple>
+
+#include "thp_internal.h"
+#include "thp_kernels.h"
+
+using namespace thp;
+
+namespace {
+
+thp_status fail(thp_ctx* ctx, thp_status code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                 \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail((ctx), THP_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                   \
+  } while (0)
+
+#define FFTCHK(ctx, call)                                                                \
+  do {                                                                                   \
+    rocfft_status s_ = (call);                                                           \
+    if (s_ != rocfft_status_success)                                                     \
+      return fail((ctx), THP_ERR_FFT, "%s failed: rocfft status %d", #call, (int)s_);   \
+  } while (0)
+
+#define RET(call)                      \
+  do {                                 \
+    thp_status r_ = (call);            \
+    if (r_ != THP_OK) return r_;       \
+  } while (0)
+
+std::once_flag g_fft_once;
+
+template <typename T>
+thp_status dalloc(thp_sys* s, T** p, size_t count) {
+  void* q = nullptr;
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) {
+    return fail(s->ctx, THP_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+  }
+  s->allocs.push_back(q);
+  s->bytes += (int64_t)bytes;
+  *p = static_cast<T*>(q);
+  return THP_OK;
+}
+
+void dfree(thp_sys* s, void* p) {
+  if (!p) return;
+  auto it = std::find(s->allocs.begin(), s->allocs.end(), p);
+  if (it != s->allocs.end()) s->allocs.erase(it);
+  (void)hipFree(p);
+}
+
+// ------------------------------------------------ PME tables (host, FP64)
+// bspline (kewald.f:720-745)
+void bspline_coef(double x, int n, double* c) {
+  c[0] = 1.0 - x;
+  c[1] = x;
+  for (int k = 3; k <= n; ++k) {
+    const double denom = 1.0 / double(k - 1);
+    c[k - 1] = x * c[k - 2] * denom;
+    for (int i = 1; i <= k - 2; ++i)
+      c[k - i - 1] = ((x + double(i)) * c[k - i - 2] + (double(k - i) - x) * c[k - i - 1]) * denom;
+    c[0] = (1.0 - x) * c[0] * denom;
+  }
+}
+
+// moduli + dftmod with the Euler-spline fix and zeta factor (kewald.f:672-812)
+std::vector<double> bsmod_table(int nfft, int order) {
+  double arr[16];
+  bspline_coef(0.0, order, arr);
+  std::vector<double> bsarray(std::max(nfft, order + 2), 0.0), bsmod(nfft);
+  for (int i = 2; i <= order + 1; ++i) bsarray[i - 1] = arr[i - 2];
+  const double pi = M_PI;
+  const double factor = 2.0 * pi / nfft;
+  for (int i = 0; i < nfft; ++i) {
+    double s1 = 0, s2 = 0;
+    for (int j = 0; j < nfft; ++j) {
+      const double arg = factor * double((long)i * j);
+      s1 += bsarray[j] * std::cos(arg);
+      s2 += bsarray[j] * std::sin(arg);
+    }
+    bsmod[i] = s1 * s1 + s2 * s2;
+  }
+  const double eps = 1.0e-7;
+  if (bsmod[0] < eps) bsmod[0] = 0.5 * bsmod[1];
+  for (int i = 1; i < nfft - 1; ++i)
+    if (bsmod[i] < eps) bsmod[i] = 0.5 * (bsmod[i - 1] + bsmod[i + 1]);
+  if (bsmod[nfft - 1] < eps) bsmod[nfft - 1] = 0.5 * bsmod[nfft - 2];
+  for (int i = 0; i < nfft; ++i) {
+    int k = i;
+    if (i + 1 > nfft / 2) k -= nfft;
+    double zeta = 1.0;
+    if (k != 0) {
+      double s1 = 1.0, s2 = 1.0;
+      const double f = pi * double(k) / double(nfft);
+      for (int j = 1; j <= 50; ++j) {
+        const double a = f / (f + pi * j), b = f / (f - pi * j);
+        s1 += std::pow(a, order) + std::pow(b, order);
+        s2 += std::pow(a, 2 * order) + std::pow(b, 2 * order);
+      }
+      zeta = s2 / s1;
+    }
+    bsmod[i] *= zeta * zeta;
+  }
+  return bsmod;
+}
+
+// scalar-sum table qfac (newinduce_pme.f:1790-1844), periodic boxes
+std::vector<double> qfac_table(const double box[3], const int nf[3], double aewald, int order) {
+  const auto b1 = bsmod_table(nf[0], order), b2 = bsmod_table(nf[1], order), b3 = bsmod_table(nf[2], order);
+  std::vector<double> q((size_t)nf[0] * nf[1] * nf[2], 0.0);
+  const double pterm = (M_PI / aewald) * (M_PI / aewald);
+  const double volterm = M_PI * box[0] * box[1] * box[2];
+  const int h1n = (nf[0] + 1) / 2, h2n = (nf[1] + 1) / 2, h3n = (nf[2] + 1) / 2;
+  for (int k3 = 0; k3 < nf[2]; ++k3) {
+    const int m3 = k3 + 1 > h3n ? k3 - nf[2] : k3;
+    const double r3 = m3 / box[2];
+    for (int k2 = 0; k2 < nf[1]; ++k2) {
+      const int m2 = k2 + 1 > h2n ? k2 - nf[1] : k2;
+      const double r2 = m2 / box[1];
+      for (int k1 = 0; k1 < nf[0]; ++k1) {
+        const int m1 = k1 + 1 > h1n ? k1 - nf[0] : k1;
+        if (m1 == 0 && m2 == 0 && m3 == 0) continue;
+        const double r1 = m1 / box[0];
+        const double hsq = r1 * r1 + r2 * r2 + r3 * r3;
+        const double term = -pterm * hsq;
+        double e = 0.0;
+        if (term > -50.0) e = std::exp(term) / (volterm * hsq * b1[k1] * b2[k2] * b3[k3]);
+        q[((size_t)k3 * nf[1] + k2) * nf[0] + k1] = e;
+      }
+    }
+  }
+  return q;
+}
+
+// cart_to_frac / frac_to_cart (pmestuff.f:1122-1298) for an orthorhombic cell
+void conversion_mats(const double box[3], const int nf[3], double ctf[10][10], double ftc[10][10]) {
+  double a[3][3] = {};
+  for (int k = 0; k < 3; ++k) a[k][k] = nf[k] / box[k];
+  static const int qi1[6] = {0, 1, 2, 0, 0, 1};
+  static const int qi2[6] = {0, 1, 2, 1, 2, 2};
+  for (int i = 0; i < 10; ++i)
+    for (int j = 0; j < 10; ++j) ctf[i][j] = ftc[i][j] = 0.0;
+  ctf[0][0] = ftc[0][0] = 1.0;
+  for (int i = 1; i < 4; ++i)
+    for (int j = 1; j < 4; ++j) ctf[i][j] = ftc[i][j] = a[i - 1][j - 1];
+  for (int i1 = 0; i1 < 3; ++i1) {
+    const int k = qi1[i1];
+    for (int i2 = 0; i2 < 6; ++i2) {
+      const int i = qi1[i2], j = qi2[i2];
+      ctf[i1 + 4][i2 + 4] = a[k][i] * a[k][j];
+      ftc[i1 + 4][i2 + 4] = i2 < 3 ? a[k][i] * a[k][i] : 2.0 * a[k][i] * a[k][j];
+    }
+  }
+  for (int i1 = 3; i1 < 6; ++i1) {
+    const int k = qi1[i1], m = qi2[i1];
+    for (int i2 = 0; i2 < 6; ++i2) {
+      const int i = qi1[i2], j = qi2[i2];
+      ctf[i1 + 4][i2 + 4] = a[k][i] * a[m][j] + a[k][j] * a[m][i];
+      ftc[i1 + 4][i2 + 4] = i2 < 3 ? a[k][i] * a[m][i] : a[k][i] * a[m][j] + a[m][i] * a[k][j];
+    }
+  }
+}
+
+// Spatial order: Morton key of ~2.5 A cells of the wrapped coordinates,
+// ties broken by the original index (deterministic).
+std::vector<int> spatial_order(int n, const double* x, const double* y, const double* z, const double box[3]) {
+  auto spread_bits = [](uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+  };
+  int nc[3];
+  for (int d = 0; d < 3; ++d) nc[d] = std::max(1, std::min(1 << 20, int(box[d] / 2.5)));
+  std::vector<std::pair<uint64_t, int>> key(n);
+  for (int i = 0; i < n; ++i) {
+    const double p[3] = {x[i], y[i], z[i]};
+    uint64_t c[3];
+    for (int d = 0; d < 3; ++d) {
+      double f = p[d] / box[d];
+      f -= std::floor(f);
+      c[d] = (uint64_t)std::min(nc[d] - 1, int(f * nc[d]));
+    }
+    key[i] = {spread_bits(c[0]) | spread_bits(c[1]) << 1 | spread_bits(c[2]) << 2, i};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int> perm(n);
+  for (int i = 0; i < n; ++i) perm[i] = key[i].second;
+  return perm;
+}
+
+thp_status upload_view(thp_sys* s) {
+  HIPCHK(s->ctx, hipMemcpyAsync(s->d_view, &s->view, sizeof(SysView), hipMemcpyHostToDevice, s->ctx->stream));
+  return THP_OK;
+}
+
+thp_status sync(thp_sys* s) {
+  HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+  return THP_OK;
+}
+
+thp_status check_err_flag(thp_sys* s) {
+  int err = 0;
+  HIPCHK(s->ctx, hipMemcpyAsync(&err, s->view.err, sizeof(int), hipMemcpyDeviceToHost, s->ctx->stream));
+  RET(sync(s));
+  if (err) return fail(s->ctx, err, "device reported error %d (neighbor-list capacity)", err);
+  return THP_OK;
+}
+
+// ---------------------------------------------------------- kernel launch
+inline dim3 g1(int n) { return dim3((unsigned)n); }
+
+thp_status launch_check(thp_ctx* ctx, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ctx, THP_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e));
+  return THP_OK;
+}
+
+thp_status pair_tmatxb(thp_sys* s, int vin, int vout, int dodiag, int check_conv) {
+  hipStream_t st = s->ctx->stream;
+  const int S = s->segs;
+  if (s->prec == THP_PREC_FP64) {
+    if (S == 1) hipLaunchKernelGGL((k_tmatxb<double, 1>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work, vin, vout, dodiag, check_conv);
+    else if (S == 2) hipLaunchKernelGGL((k_tmatxb<double, 2>), g1(s->nblk), dim3(128), 0, st, s->d_view, s->d_work, vin, vout, dodiag, check_conv);
+    else hipLaunchKernelGGL((k_tmatxb<double, 4>), g1(s->nblk), dim3(256), 0, st, s->d_view, s->d_work, vin, vout, dodiag, check_conv);
+  } else {
+    if (S == 1) hipLaunchKernelGGL((k_tmatxb<float, 1>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work, vin, vout, dodiag, check_conv);
+    else if (S == 2) hipLaunchKernelGGL((k_tmatxb<float, 2>), g1(s->nblk), dim3(128), 0, st, s->d_view, s->d_work, vin, vout, dodiag, check_conv);
+    else hipLaunchKernelGGL((k_tmatxb<float, 4>), g1(s->nblk), dim3(256), 0, st, s->d_view, s->d_work, vin, vout, dodiag, check_conv);
+  }
+  return launch_check(s->ctx, "k_tmatxb");
+}
+
+thp_status pair_efld0(thp_sys* s) {
+  hipStream_t st = s->ctx->stream;
+  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_efld0<double, 4>), g1(s->nblk), dim3(256), 0, st, s->d_view, s->d_work);
+  else hipLaunchKernelGGL((k_efld0<float, 4>), g1(s->nblk), dim3(256), 0, st, s->d_view, s->d_work);
+  return launch_check(s->ctx, "k_efld0");
+}
+
+size_t grid_points(const thp_sys* s) { return (size_t)s->nfft[0] * s->nfft[1] * s->nfft[2]; }
+
+thp_status zero_grid(thp_sys* s) {
+  const size_t cnt = 2 * grid_points(s);
+  const int nb = (int)std::min<size_t>(4096, (cnt + 255) / 256);
+  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_zero_grid<double>), g1(nb), dim3(256), 0, s->ctx->stream, s->d_view, 0, cnt);
+  else hipLaunchKernelGGL((k_zero_grid<float>), g1(nb), dim3(256), 0, s->ctx->stream, s->d_view, 0, cnt);
+  return launch_check(s->ctx, "k_zero_grid");
+}
+
+thp_status spread_uind(thp_sys* s, int vin, int frac, int check_conv) {
+  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_spread_uind<double>), g1(s->nblk), dim3(64), 0, s->ctx->stream, s->d_view, s->d_work, vin, frac, check_conv);
+  else hipLaunchKernelGGL((k_spread_uind<float>), g1(s->nblk), dim3(64), 0, s->ctx->stream, s->d_view, s->d_work, vin, frac, check_conv);
+  return launch_check(s->ctx, "k_spread_uind");
+}
+
+thp_status convolve(thp_sys* s, int check_conv) {
+  void* buf[1] = {s->view.grid};
+  FFTCHK(s->ctx, rocfft_execute(s->fwd, buf, nullptr, s->info));
+  const size_t np = grid_points(s);
+  const int nb = (int)std::min<size_t>(4096, (np + 255) / 256);
+  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_conv<double>), g1(nb), dim3(256), 0, s->ctx->stream, s->d_view, 0, np, check_conv);
+  else hipLaunchKernelGGL((k_conv<float>), g1(nb), dim3(256), 0, s->ctx->stream, s->d_view, 0, np, check_conv);
+  RET(launch_check(s->ctx, "k_conv"));
+  FFTCHK(s->ctx, rocfft_execute(s->bwd, buf, nullptr, s->info));
+  return THP_OK;
+}
+
+template <int MODE>
+thp_status launch_gather_uind(thp_sys* s, int vin, int vout, int frac_out) {
+  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_uind<double, MODE>), g1(s->nblk), dim3(64), 0, s->ctx->stream, s->d_view, s->d_work, vin, vout, frac_out);
+  else hipLaunchKernelGGL((k_gather_uind<float, MODE>), g1(s->nblk), dim3(64), 0, s->ctx->stream, s->d_view, s->d_work, vin, vout, frac_out);
+  return launch_check(s->ctx, "k_gather_uind");
+}
+
+// permanent field: ef = E_direct - grad phi_recip + self (newinduce_pme2.f:111-147)
+thp_status field(thp_sys* s, double* cphi_dev, int add_to_ef, int do_direct) {
+  hipStream_t st = s->ctx->stream;
+  if (do_direct) RET(pair_efld0(s));
+  RET(zero_grid(s));
+  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_spread_mpole<double>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
+  else hipLaunchKernelGGL((k_spread_mpole<float>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
+  RET(launch_check(s->ctx, "k_spread_mpole"));
+  RET(convolve(s, 0));
+  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_mpole<double>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work, cphi_dev, add_to_ef);
+  else hipLaunchKernelGGL((k_gather_mpole<float>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work, cphi_dev, add_to_ef);
+  return launch_check(s->ctx, "k_gather_mpole");
+}
+
+thp_status scatter_in(thp_sys* s, double* dst_dev, const double* host, int ncomp) {
+  double* tmp = nullptr;
+  RET(dalloc(s, &tmp, (size_t)s->n * ncomp));
+  HIPCHK(s->ctx, hipMemcpyAsync(tmp, host, sizeof(double) * s->n * ncomp, hipMemcpyHostToDevice, s->ctx->stream));
+  hipLaunchKernelGGL(k_scatter_in, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream, dst_dev, tmp, s->iperm_dev, s->n, s->npad, ncomp);
+  thp_status r = launch_check(s->ctx, "k_scatter_in");
+  RET(sync(s));
+  dfree(s, tmp);
+  return r;
+}
+
+thp_status gather_out(thp_sys* s, double* host, const double* src_dev, int ncomp) {
+  double* tmp = nullptr;
+  RET(dalloc(s, &tmp, (size_t)s->n * ncomp));
+  hipLaunchKernelGGL(k_gather_out, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream, tmp, src_dev, s->iperm_dev, s->n, s->npad, ncomp);
+  RET(launch_check(s->ctx, "k_gather_out"));
+  HIPCHK(s->ctx, hipMemcpyAsync(host, tmp, sizeof(double) * s->n * ncomp, hipMemcpyDeviceToHost, s->ctx->stream));
+  RET(sync(s));
+  dfree(s, tmp);
+  return THP_OK;
+}
+
+thp_status ensure_lists(thp_sys* s) {
+  if (!s->list_valid) RET(thp_nblist(s, 1));
+  if (!s->tight_valid) {
+    hipLaunchKernelGGL(k_tight, g1(s->nblk), dim3(64), 0, s->ctx->stream, s->d_view, s->d_work);
+    RET(launch_check(s->ctx, "k_tight"));
+    s->tight_valid = true;
+  }
+  return THP_OK;
+}
+
+}  // namespace
+
+// ====================================================================== API
+extern "C" {
+
+const char* thp_version(void) { return "thp 0.1 (gfx950)"; }
+
+thp_status thp_init(thp_ctx** out, int device) {
+  if (!out) return THP_ERR_ARG;
+  thp_ctx* ctx = new thp_ctx;
+  ctx->device = device;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
+    *out = ctx;
+    return fail(ctx, THP_ERR_HIP, "no HIP device %d (found %d)", device, ndev);
+  }
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    *out = ctx;
+    return fail(ctx, THP_ERR_HIP, "cannot initialise device %d", device);
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
+  std::call_once(g_fft_once, [] { rocfft_setup(); });
+  *out = ctx;
+  return THP_OK;
+}
+
+thp_status thp_finalize(thp_ctx* ctx) {
+  if (!ctx) return THP_ERR_ARG;
+  if (ctx->serve) thp_serve_stop(ctx);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return THP_OK;
+}
+
+const char* thp_last_error(thp_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+thp_status thp_system_create(thp_ctx* ctx, const thp_system_desc* d, thp_sys** out) {
+  if (!ctx || !d || !out) return THP_ERR_ARG;
+  if (d->n <= 0 || d->n >= kMaxAtoms) return fail(ctx, THP_ERR_ARG, "n = %d out of range", d->n);
+  if (d->order != kOrder) return fail(ctx, THP_ERR_ARG, "only B-spline order %d is implemented", kOrder);
+  for (int k = 0; k < 3; ++k) {
+    if (!(d->box[k] > 0)) return fail(ctx, THP_ERR_ARG, "box edge %d must be positive", k);
+    if (d->nfft[k] < 2 * kOrder) return fail(ctx, THP_ERR_ARG, "nfft[%d] = %d too small", k, d->nfft[k]);
+  }
+  if (!(d->cut > 0) || d->buffer < 0 || !(d->aewald > 0)) return fail(ctx, THP_ERR_ARG, "bad cut/buffer/aewald");
+  // nblist.f:76-83: (cut + buffer) must not exceed half the box
+  const double mb = d->cut + d->buffer;
+  const double minhalf = 0.5 * std::min({d->box[0], d->box[1], d->box[2]});
+  if (mb * mb > minhalf * minhalf)
+    return fail(ctx, THP_ERR_LIST, "cutoff + buffer %.3f exceeds half the box %.3f", mb, minhalf);
+  if (d->precision != THP_PREC_FP64 && d->precision != THP_PREC_MIXED) return fail(ctx, THP_ERR_ARG, "bad precision");
+  if (!d->x || !d->y || !d->z || !d->rpole || !d->polarity || !d->thole) return fail(ctx, THP_ERR_ARG, "null array");
+  if (d->nscaled < 0 || (d->nscaled > 0 && (!d->scaled_ik || !d->scaled_dpu))) return fail(ctx, THP_ERR_ARG, "bad scaled pairs");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+
+  thp_sys* s = new thp_sys;
+  s->ctx = ctx;
+  s->n = d->n;
+  s->npad = (d->n + kBlock - 1) / kBlock * kBlock;
+  s->nblk = s->npad / kBlock;
+  s->prec = d->precision;
+  for (int k = 0; k < 3; ++k) {
+    s->box[k] = d->box[k];
+    s->nfft[k] = d->nfft[k];
+  }
+  s->aewald = d->aewald;
+  s->cut = d->cut;
+  s->buffer = d->buffer;
+  s->order = d->order;
+  const int n = s->n, np = s->npad;
+
+  s->perm = spatial_order(n, d->x, d->y, d->z, d->box);
+  s->iperm.assign(n, 0);
+  for (int q = 0; q < n; ++q) s->iperm[s->perm[q]] = q;
+
+  auto cleanup = [&](thp_status r) {
+    thp_system_destroy(s);
+    return r;
+  };
+  // ---- per-atom arrays in sorted order
+  std::vector<double> hx(np), hy(np), hz(np), rp((size_t)10 * np, 0.0), pol(np, 0.0), pd(np, 0.0), th(np, 0.0);
+  for (int q = 0; q < np; ++q) {
+    const int a = q < n ? s->perm[q] : -1;
+    if (a < 0) {
+      hx[q] = hy[q] = hz[q] = 0.0;
+      continue;
+    }
+    hx[q] = d->x[a];
+    hy[q] = d->y[a];
+    hz[q] = d->z[a];
+    const double* r = d->rpole + (size_t)a * 13;
+    const double v[10] = {r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[8], r[9], r[12]};
+    for (int c = 0; c < 10; ++c) rp[(size_t)c * np + q] = v[c];
+    pol[q] = d->polarity[a];
+    pd[q] = std::pow(d->polarity[a], 1.0 / 6.0);  // kpolar.f:235
+    th[q] = d->thole[a];
+  }
+  s->hx.assign(hx.begin(), hx.begin() + n);
+  s->hy.assign(hy.begin(), hy.begin() + n);
+  s->hz.assign(hz.begin(), hz.begin() + n);
+
+  // ---- scaled pairs -> per-atom CSR with codes into a table of (d,p,u)
+  std::map<std::tuple<double, double, double>, int> codes;
+  std::vector<double> tab = {1.0, 1.0, 1.0};
+  codes[{1.0, 1.0, 1.0}] = 0;
+  std::vector<std::vector<std::pair<int, int>>> ex(n);
+  for (int m = 0; m < d->nscaled; ++m) {
+    const int a = d->scaled_ik[2 * m], b = d->scaled_ik[2 * m + 1];
+    if (a < 0 || b < 0 || a >= n || b >= n || a == b) return cleanup(fail(ctx, THP_ERR_ARG, "bad scaled pair %d", m));
+    const auto key = std::make_tuple(d->scaled_dpu[3 * m], d->scaled_dpu[3 * m + 1], d->scaled_dpu[3 * m + 2]);
+    auto it = codes.find(key);
+    int code;
+    if (it == codes.end()) {
+      code = (int)codes.size();
+      if (code >= 127) return cleanup(fail(ctx, THP_ERR_ARG, "too many distinct scale triples"));
+      codes[key] = code;
+      tab.push_back(std::get<0>(key));
+      tab.push_back(std::get<1>(key));
+      tab.push_back(std::get<2>(key));
+    } else {
+      code = it->second;
+    }
+    const int sa = s->iperm[a], sb = s->iperm[b];
+    ex[sa].push_back({sb, code});
+    ex[sb].push_back({sa, code});
+  }
+  std::vector<int> eoff(n + 1, 0), ek, ec;
+  for (int q = 0; q < n; ++q) {
+    std::sort(ex[q].begin(), ex[q].end());
+    eoff[q + 1] = eoff[q] + (int)ex[q].size();
+    for (auto& e : ex[q]) {
+      ek.push_back(e.first);
+      ec.push_back(e.second);
+    }
+  }
+
+  SysView& v = s->view;
+  memset(&v, 0, sizeof v);
+  v.n = n;
+  v.npad = np;
+  v.nblk = s->nblk;
+  v.prec = s->prec;
+  for (int k = 0; k < 3; ++k) {
+    v.box[k] = s->box[k];
+    v.nfft[k] = s->nfft[k];
+    v.afrac[k] = s->nfft[k] / s->box[k];
+  }
+  v.aewald = s->aewald;
+  v.cut2 = s->cut * s->cut;
+  v.mbuf2 = mb * mb;
+  v.term_self = (4.0 / 3.0) * s->aewald * s->aewald * s->aewald / std::sqrt(M_PI);
+  v.ncode = (int)codes.size();
+  conversion_mats(s->box, s->nfft, v.ctf, v.ftc);
+
+  hipStream_t st = ctx->stream;
+  double *dx, *dy, *dz, *drp, *dpol, *dpd, *dth, *dtab;
+  int *deoff, *dek, *dec;
+#define ALLOC(p, c) do { thp_status r_ = dalloc(s, &(p), (c)); if (r_ != THP_OK) return cleanup(r_); } while (0)
+#define H2D(dst, src, bytes) do { if (hipMemcpyAsync((dst), (src), (bytes), hipMemcpyHostToDevice, st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "H2D copy failed")); } while (0)
+  ALLOC(dx, np); ALLOC(dy, np); ALLOC(dz, np);
+  ALLOC(drp, (size_t)10 * np); ALLOC(dpol, np); ALLOC(dpd, np); ALLOC(dth, np);
+  ALLOC(dtab, tab.size()); ALLOC(deoff, n + 1); ALLOC(dek, ek.size()); ALLOC(dec, ec.size());
+  H2D(dx, hx.data(), sizeof(double) * np);
+  H2D(dy, hy.data(), sizeof(double) * np);
+  H2D(dz, hz.data(), sizeof(double) * np);
+  H2D(drp, rp.data(), sizeof(double) * rp.size());
+  H2D(dpol, pol.data(), sizeof(double) * np);
+  H2D(dpd, pd.data(), sizeof(double) * np);
+  H2D(dth, th.data(), sizeof(double) * np);
+  H2D(dtab, tab.data(), sizeof(double) * tab.size());
+  H2D(deoff, eoff.data(), sizeof(int) * (n + 1));
+  if (!ek.empty()) {
+    H2D(dek, ek.data(), sizeof(int) * ek.size());
+    H2D(dec, ec.data(), sizeof(int) * ec.size());
+  }
+  v.x = dx; v.y = dy; v.z = dz; v.rpole = drp; v.pol = dpol; v.pdamp = dpd; v.thole = dth;
+  v.scale_tab = dtab; v.excl_off = deoff; v.excl_k = dek; v.excl_code = dec;
+  ALLOC(s->iperm_dev, n);
+  H2D(s->iperm_dev, s->iperm.data(), sizeof(int) * n);
+  ALLOC(v.nbcount, np); ALLOC(v.tcount, np); ALLOC(v.bbox, (size_t)s->nblk * 6);
+  ALLOC(s->bcount, s->nblk);
+  ALLOC(v.ef, (size_t)6 * np); ALLOC(v.mu, (size_t)6 * np); ALLOC(v.res, (size_t)6 * np);
+  ALLOC(v.h, (size_t)6 * np); ALLOC(v.hreal, (size_t)6 * np); ALLOC(v.pp, (size_t)6 * np);
+  ALLOC(v.zr, (size_t)6 * np); ALLOC(v.partial, (size_t)8 * s->nblk);
+  ALLOC(s->cphi, (size_t)10 * np);
+  ALLOC(v.st, 1); ALLOC(v.err, 1);
+  if (hipMemsetAsync(v.err, 0, sizeof(int), st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "memset"));
+  if (hipMemsetAsync(v.st, 0, sizeof(SolveState), st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "memset"));
+  for (double* vec : {v.ef, v.mu, v.res, v.h, v.hreal, v.pp, v.zr})
+    if (hipMemsetAsync(vec, 0, sizeof(double) * 6 * np, st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "memset"));
+
+  // ---- PME grid, scalar-sum table and rocFFT plans
+  const size_t npts = grid_points(s);
+  const auto q = qfac_table(s->box, s->nfft, s->aewald, s->order);
+  if (s->prec == THP_PREC_FP64) {
+    double *g, *qf;
+    ALLOC(g, 2 * npts);
+    ALLOC(qf, npts);
+    H2D(qf, q.data(), sizeof(double) * npts);
+    v.grid = g;
+    v.qfac = qf;
+  } else {
+    float *g, *qf;
+    ALLOC(g, 2 * npts);
+    ALLOC(qf, npts);
+    std::vector<float> qs(q.begin(), q.end());
+    H2D(qf, qs.data(), sizeof(float) * npts);
+    if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "sync"));
+    v.grid = g;
+    v.qfac = qf;
+  }
+  {
+    const size_t len[3] = {(size_t)s->nfft[0], (size_t)s->nfft[1], (size_t)s->nfft[2]};
+    const rocfft_precision pr = s->prec == THP_PREC_FP64 ? rocfft_precision_double : rocfft_precision_single;
+    if (rocfft_plan_create(&s->fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward, pr, 3, len, 1, nullptr) != rocfft_status_success ||
+        rocfft_plan_create(&s->bwd, rocfft_placement_inplace, rocfft_transform_type_complex_inverse, pr, 3, len, 1, nullptr) != rocfft_status_success)
+      return cleanup(fail(ctx, THP_ERR_FFT, "rocfft_plan_create failed"));
+    size_t w1 = 0, w2 = 0;
+    rocfft_plan_get_work_buffer_size(s->fwd, &w1);
+    rocfft_plan_get_work_buffer_size(s->bwd, &w2);
+    if (rocfft_execution_info_create(&s->info) != rocfft_status_success) return cleanup(fail(ctx, THP_ERR_FFT, "execution info"));
+    rocfft_execution_info_set_stream(s->info, st);
+    const size_t wb = std::max(w1, w2);
+    if (wb) {
+      char* wp;
+      ALLOC(wp, wb);
+      s->fft_work = wp;
+      rocfft_execution_info_set_work_buffer(s->info, wp, wb);
+    }
+  }
+
+  // ---- view + work table
+  ALLOC(s->d_view, 1);
+  std::vector<Work> wk(s->nblk);
+  for (int b = 0; b < s->nblk; ++b) wk[b] = {0, b};
+  ALLOC(s->d_work, s->nblk);
+  H2D(s->d_work, wk.data(), sizeof(Work) * s->nblk);
+  for (int e = 0; e < 4; ++e)
+    if (hipEventCreate(&s->ev[e]) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "event"));
+  if (upload_view(s) != THP_OK) return cleanup(THP_ERR_HIP);
+  if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "sync"));
+#undef ALLOC
+#undef H2D
+  *out = s;
+  return THP_OK;
+}
+
+thp_status thp_system_destroy(thp_sys* s) {
+  if (!s) return THP_ERR_ARG;
+  if (s->ctx && s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
+  if (s->fwd) rocfft_plan_destroy(s->fwd);
+  if (s->bwd) rocfft_plan_destroy(s->bwd);
+  if (s->info) rocfft_execution_info_destroy(s->info);
+  for (void* p : s->allocs) (void)hipFree(p);
+  for (auto& e : s->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete s;
+  return THP_OK;
+}
+
+thp_status thp_system_footprint(const thp_sys* s, int64_t* bytes) {
+  if (!s || !bytes) return THP_ERR_ARG;
+  *bytes = s->bytes;
+  return THP_OK;
+}
+
+thp_status thp_system_update_coords(thp_sys* s, const double* x, const double* y, const double* z) {
+  if (!s || !x || !y || !z) return THP_ERR_ARG;
+  const int n = s->n, np = s->npad;
+  std::vector<double> hx(np, 0.0), hy(np, 0.0), hz(np, 0.0);
+  double maxd2 = 0.0;
+  for (int q = 0; q < n; ++q) {
+    const int a = s->perm[q];
+    hx[q] = x[a];
+    hy[q] = y[a];
+    hz[q] = z[a];
+    if (s->list_valid) {
+      const double dx = hx[q] - s->bx0[q], dy = hy[q] - s->by0[q], dz = hz[q] - s->bz0[q];
+      maxd2 = std::max(maxd2, dx * dx + dy * dy + dz * dz);
+    }
+  }
+  hipStream_t st = s->ctx->stream;
+  HIPCHK(s->ctx, hipMemcpyAsync((void*)s->view.x, hx.data(), sizeof(double) * np, hipMemcpyHostToDevice, st));
+  HIPCHK(s->ctx, hipMemcpyAsync((void*)s->view.y, hy.data(), sizeof(double) * np, hipMemcpyHostToDevice, st));
+  HIPCHK(s->ctx, hipMemcpyAsync((void*)s->view.z, hz.data(), sizeof(double) * np, hipMemcpyHostToDevice, st));
+  RET(sync(s));
+  s->hx.assign(hx.begin(), hx.begin() + n);
+  s->hy.assign(hy.begin(), hy.begin() + n);
+  s->hz.assign(hz.begin(), hz.begin() + n);
+  // the Verlet list stays valid while no atom moved more than buffer/2
+  if (s->list_valid && maxd2 > 0.25 * s->buffer * s->buffer) s->list_valid = false;
+  s->tight_valid = false;
+  return THP_OK;
+}
+
+thp_status thp_nblist(thp_sys* s, int force_rebuild) {
+  if (!s) return THP_ERR_ARG;
+  if (s->list_valid && !force_rebuild) return THP_OK;
+  hipStream_t st = s->ctx->stream;
+  hipLaunchKernelGGL(k_bbox, g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
+  RET(launch_check(s->ctx, "k_bbox"));
+  hipLaunchKernelGGL((k_blockpairs<false>), g1(s->nblk), dim3(256), 0, st, s->d_view, s->d_work, s->bcount, (int*)nullptr, 0);
+  RET(launch_check(s->ctx, "k_blockpairs"));
+  std::vector<int> bc(s->nblk);
+  HIPCHK(s->ctx, hipMemcpyAsync(bc.data(), s->bcount, sizeof(int) * s->nblk, hipMemcpyDeviceToHost, st));
+  RET(sync(s));
+  const int maxbl = *std::max_element(bc.begin(), bc.end());
+  if (maxbl > s->maxbl) {
+    dfree(s, s->blist);
+    s->blist = nullptr;
+    RET(dalloc(s, &s->blist, (size_t)s->nblk * maxbl));
+    s->maxbl = maxbl;
+  }
+  hipLaunchKernelGGL((k_blockpairs<true>), g1(s->nblk), dim3(256), 0, st, s->d_view, s->d_work, s->bcount, s->blist, s->maxbl);
+  RET(launch_check(s->ctx, "k_blockpairs"));
+  hipLaunchKernelGGL((k_nblist_atoms<false>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work, (const int*)s->bcount, (const int*)s->blist, s->maxbl);
+  RET(launch_check(s->ctx, "k_nblist_atoms"));
+  std::vector<int> cnt(s->npad);
+  HIPCHK(s->ctx, hipMemcpyAsync(cnt.data(), s->view.nbcount, sizeof(int) * s->npad, hipMemcpyDeviceToHost, st));
+  RET(sync(s));
+  int maxnb = 0;
+  for (int q = 0; q < s->n; ++q) maxnb = std::max(maxnb, cnt[q]);
+  maxnb = std::max(32, (maxnb + 31) / 32 * 32);
+  if (maxnb > s->maxnb) {
+    dfree(s, s->view.nb);
+    dfree(s, s->view.tnb);
+    s->view.nb = s->view.tnb = nullptr;
+    RET(dalloc(s, &s->view.nb, (size_t)maxnb * s->npad));
+    RET(dalloc(s, &s->view.tnb, (size_t)maxnb * s->npad));
+    s->maxnb = maxnb;
+    s->view.maxnb = maxnb;
+    RET(upload_view(s));
+  }
+  hipLaunchKernelGGL((k_nblist_atoms<true>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work, (const int*)s->bcount, (const int*)s->blist, s->maxbl);
+  RET(launch_check(s->ctx, "k_nblist_atoms"));
+  RET(check_err_flag(s));
+  s->bx0 = s->hx;
+  s->by0 = s->hy;
+  s->bz0 = s->hz;
+  s->list_valid = true;
+  s->tight_valid = false;
+  return THP_OK;
+}
+
+thp_status thp_nblist_get(thp_sys* s, int64_t* npairs, int* ik, int64_t cap) {
+  if (!s || !npairs) return THP_ERR_ARG;
+  RET(thp_nblist(s, 0));
+  std::vector<int> cnt(s->npad), nb((size_t)s->maxnb * s->npad);
+  HIPCHK(s->ctx, hipMemcpy(cnt.data(), s->view.nbcount, sizeof(int) * s->npad, hipMemcpyDeviceToHost));
+  HIPCHK(s->ctx, hipMemcpy(nb.data(), s->view.nb, sizeof(int) * nb.size(), hipMemcpyDeviceToHost));
+  std::vector<std::pair<int, int>> pairs;
+  for (int q = 0; q < s->n; ++q) {
+    const int a = s->perm[q];
+    for (int t = 0; t < cnt[q]; ++t) {
+      const int b = s->perm[nb[(size_t)t * s->npad + q]];
+      if (a < b) pairs.push_back({a, b});
+    }
+  }
+  std::sort(pairs.begin(), pairs.end());
+  *npairs = (int64_t)pairs.size();
+  if (!ik) return THP_OK;
+  if (cap < (int64_t)pairs.size()) return fail(s->ctx, THP_ERR_ARG, "pair buffer too small (%lld < %zu)", (long long)cap, pairs.size());
+  for (size_t m = 0; m < pairs.size(); ++m) {
+    ik[2 * m] = pairs[m].first;
+    ik[2 * m + 1] = pairs[m].second;
+  }
+  return THP_OK;
+}
+
+thp_status thp_efld0_direct(thp_sys* s, double* ef) {
+  if (!s || !ef) return THP_ERR_ARG;
+  RET(ensure_lists(s));
+  RET(pair_efld0(s));
+  return gather_out(s, ef, s->view.ef, 6);
+}
+
+thp_status thp_efld0_recip(thp_sys* s, double* cphi) {
+  if (!s || !cphi) return THP_ERR_ARG;
+  RET(field(s, s->cphi, 0, 0));
+  return gather_out(s, cphi, s->cphi, 10);
+}
+
+thp_status thp_tmatxb(thp_sys* s, const double* mu, double* efi, int dodiag) {
+  if (!s || !mu || !efi) return THP_ERR_ARG;
+  RET(ensure_lists(s));
+  RET(scatter_in(s, s->view.pp, mu, 6));
+  RET(pair_tmatxb(s, 1 /*pp*/, 3 /*hreal*/, dodiag, 0));
+  return gather_out(s, efi, s->view.hreal, 6);
+}
+
+thp_status thp_tmatxb_dev(thp_sys* s, const double* mu, double* efi, int dodiag) {
+  if (!s || !mu || !efi) return THP_ERR_ARG;
+  RET(ensure_lists(s));
+  hipLaunchKernelGGL(k_scatter_in, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream, s->view.pp, mu, s->iperm_dev, s->n, s->npad, 6);
+  RET(launch_check(s->ctx, "k_scatter_in"));
+  RET(pair_tmatxb(s, 1, 3, dodiag, 0));
+  hipLaunchKernelGGL(k_gather_out, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream, efi, s->view.hreal, s->iperm_dev, s->n, s->npad, 6);
+  RET(launch_check(s->ctx, "k_gather_out"));
+  return sync(s);
+}
+
+thp_status thp_tmatxb_recip(thp_sys* s, const double* mu, double* dipfield) {
+  if (!s || !mu || !dipfield) return THP_ERR_ARG;
+  RET(scatter_in(s, s->view.pp, mu, 6));
+  RET(zero_grid(s));
+  RET(spread_uind(s, 1, 0, 0));
+  RET(convolve(s, 0));
+  RET(launch_gather_uind<0>(s, 1, 3, 0));
+  return gather_out(s, dipfield, s->view.hreal, 6);
+}
+
+thp_status thp_grid_uind(thp_sys* s, const double* fuind, const double* fuinp, double* qgrid) {
+  if (!s || !fuind || !fuinp) return THP_ERR_ARG;
+  std::vector<double> f((size_t)s->n * 6);
+  for (int a = 0; a < s->n; ++a)
+    for (int c = 0; c < 3; ++c) {
+      f[(size_t)a * 6 + c] = fuind[(size_t)a * 3 + c];
+      f[(size_t)a * 6 + 3 + c] = fuinp[(size_t)a * 3 + c];
+    }
+  RET(scatter_in(s, s->view.pp, f.data(), 6));
+  RET(zero_grid(s));
+  RET(spread_uind(s, 1, 1, 0));
+  if (qgrid) {
+    const size_t cnt = 2 * grid_points(s);
+    if (s->prec == THP_PREC_FP64) {
+      HIPCHK(s->ctx, hipMemcpyAsync(qgrid, s->view.grid, sizeof(double) * cnt, hipMemcpyDeviceToHost, s->ctx->stream));
+      RET(sync(s));
+    } else {
+      std::vector<float> g(cnt);
+      HIPCHK(s->ctx, hipMemcpyAsync(g.data(), s->view.grid, sizeof(float) * cnt, hipMemcpyDeviceToHost, s->ctx->stream));
+      RET(sync(s));
+      for (size_t q = 0; q < cnt; ++q) qgrid[q] = g[q];
+    }
+  }
+  return sync(s);
+}
+
+thp_status thp_pme_convolve(thp_sys* s) {
+  if (!s) return THP_ERR_ARG;
+  RET(convolve(s, 0));
+  return sync(s);
+}
+
+thp_status thp_fphi_uind(thp_sys* s, const double* qgrid, double* phi1, double* phi2) {
+  if (!s || !phi1 || !phi2) return THP_ERR_ARG;
+  if (qgrid) {
+    const size_t cnt = 2 * grid_points(s);
+    if (s->prec == THP_PREC_FP64) {
+      HIPCHK(s->ctx, hipMemcpyAsync(s->view.grid, qgrid, sizeof(double) * cnt, hipMemcpyHostToDevice, s->ctx->stream));
+      RET(sync(s));
+    } else {
+      std::vector<float> g(qgrid, qgrid + cnt);
+      HIPCHK(s->ctx, hipMemcpyAsync(s->view.grid, g.data(), sizeof(float) * cnt, hipMemcpyHostToDevice, s->ctx->stream));
+      RET(sync(s));
+    }
+  }
+  RET(launch_gather_uind<0>(s, 1, 3, 1));
+  std::vector<double> out((size_t)s->n * 6);
+  RET(gather_out(s, out.data(), s->view.hreal, 6));
+  for (int a = 0; a < s->n; ++a)
+    for (int c = 0; c < 3; ++c) {
+      phi1[(size_t)a * 3 + c] = out[(size_t)a * 6 + c];
+      phi2[(size_t)a * 3 + c] = out[(size_t)a * 6 + 3 + c];
+    }
+  return THP_OK;
+}
+
+}  // extern "C"
+
+// ============================================================ PCG driver
+namespace thp {
+
+// Enqueue one PCG iteration (newinduce_pme2.f:442-569) on the context stream.
+thp_status enqueue_iteration(thp_sys* s) {
+  RET(pair_tmatxb(s, 1 /*pp*/, 3 /*hreal*/, 1, 1));
+  RET(spread_uind(s, 1, 0, 1));
+  RET(convolve(s, 1));
+  RET(launch_gather_uind<2>(s, 1, 3, 0));
+  hipStream_t st = s->ctx->stream;
+  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_cg_update1<double>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
+  else hipLaunchKernelGGL((k_cg_update1<float>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
+  RET(launch_check(s->ctx, "k_cg_update1"));
+  hipLaunchKernelGGL(k_cg_update2, g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
+  return launch_check(s->ctx, "k_cg_update2");
+}
+
+// Field, guess and first residual (newinduce_pme2.f:111-196, 406-440).
+thp_status enqueue_setup(thp_sys* s, const thp_solve_opts* o, bool warm) {
+  hipStream_t st = s->ctx->stream;
+  SolveState init;
+  memset(&init, 0, sizeof init);
+  init.poleps = o->poleps;
+  init.politer = o->politer;
+  HIPCHK(s->ctx, hipMemcpyAsync(s->view.st, &init, sizeof init, hipMemcpyHostToDevice, st));
+  RET(ensure_lists(s));
+  RET(field(s, nullptr, 1, 1));
+  if (!warm) {
+    hipLaunchKernelGGL(k_guess, g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
+    RET(launch_check(s->ctx, "k_guess"));
+  }
+  RET(pair_tmatxb(s, 0 /*mu*/, 3 /*hreal*/, 1, 0));
+  RET(zero_grid(s));
+  RET(spread_uind(s, 0, 0, 0));
+  RET(convolve(s, 0));
+  RET(launch_gather_uind<1>(s, 0, 3, 0));
+  return zero_grid(s);
+}
+
+thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_stats* stats) {
+  hipStream_t st = s->ctx->stream;
+  HIPCHK(s->ctx, hipEventRecord(s->ev[0], st));
+  RET(enqueue_setup(s, o, warm));
+  HIPCHK(s->ctx, hipEventRecord(s->ev[1], st));
+  int conv = 0;
+  for (int it = 0; it < o->politer; ++it) {
+    RET(enqueue_iteration(s));
+    // the reference reads the residual norm back on the host every
+    // iteration (newinduce_pme2gpu.f:1040-1041)
+    HIPCHK(s->ctx, hipMemcpyAsync(&conv, &s->view.st->conv, sizeof(int), hipMemcpyDeviceToHost, st));
+    RET(sync(s));
+    if (conv) break;
+  }
+  hipLaunchKernelGGL(k_energy, g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
+  RET(launch_check(s->ctx, "k_energy"));
+  HIPCHK(s->ctx, hipEventRecord(s->ev[2], st));
+  SolveState fin;
+  HIPCHK(s->ctx, hipMemcpyAsync(&fin, s->view.st, sizeof fin, hipMemcpyDeviceToHost, st));
+  RET(sync(s));
+  if (stats) {
+    float t1 = 0, t2 = 0;
+    hipEventElapsedTime(&t1, s->ev[0], s->ev[1]);
+    hipEventElapsedTime(&t2, s->ev[1], s->ev[2]);
+    stats->niter = fin.niter;
+    stats->resnrm = fin.resnrm;
+    stats->ep = fin.ep;
+    stats->n_indefinite = fin.n_indef;
+    stats->t_field_ms = t1;
+    stats->t_solve_ms = t2;
+    stats->status = fin.conv == kConverged ? THP_OK : fin.conv == kNonFinite ? THP_ERR_NAN : fin.conv == kZeroCurv ? THP_OK : THP_ERR_NOCONV;
+  }
+  return THP_OK;
+}
+
+}  // namespace thp
+
+extern "C" {
+
+thp_status thp_induce(thp_sys* s, const thp_solve_opts* o, double* uind, double* uinp, double* ep,
+                      thp_solve_stats* stats) {
+  if (!s || !o || !uind || !uinp) return THP_ERR_ARG;
+  if (!(o->poleps > 0) || o->politer <= 0) return fail(s->ctx, THP_ERR_ARG, "bad poleps/politer");
+  const bool warm = o->guess == 1;
+  if (warm) {
+    std::vector<double> m((size_t)s->n * 6);
+    for (int a = 0; a < s->n; ++a)
+      for (int c = 0; c < 3; ++c) {
+        m[(size_t)a * 6 + c] = uind[(size_t)a * 3 + c];
+        m[(size_t)a * 6 + 3 + c] = uinp[(size_t)a * 3 + c];
+      }
+    RET(scatter_in(s, s->view.mu, m.data(), 6));
+  }
+  thp_solve_stats local;
+  thp_solve_stats* sp = stats ? stats : &local;
+  RET(run_solve(s, o, warm, sp));
+  std::vector<double> m((size_t)s->n * 6);
+  RET(gather_out(s, m.data(), s->view.mu, 6));
+  for (int a = 0; a < s->n; ++a)
+    for (int c = 0; c < 3; ++c) {
+      uind[(size_t)a * 3 + c] = m[(size_t)a * 6 + c];
+      uinp[(size_t)a * 3 + c] = m[(size_t)a * 6 + 3 + c];
+    }
+  if (ep) *ep = sp->ep;
+  return sp->status == THP_ERR_NAN ? fail(s->ctx, THP_ERR_NAN, "non-finite residual") : THP_OK;
+}
+
+thp_status thp_induce_dev(thp_sys* s, const thp_solve_opts* o, double* mu_dev, double* ep, thp_solve_stats* stats) {
+  if (!s || !o || !mu_dev) return THP_ERR_ARG;
+  const bool warm = o->guess == 1;
+  if (warm) {
+    hipLaunchKernelGGL(k_scatter_in, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream, s->view.mu, mu_dev, s->iperm_dev, s->n, s->npad, 6);
+    RET(launch_check(s->ctx, "k_scatter_in"));
+  }
+  thp_solve_stats local;
+  thp_solve_stats* sp = stats ? stats : &local;
+  RET(run_solve(s, o, warm, sp));
+  hipLaunchKernelGGL(k_gather_out, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream, mu_dev, s->view.mu, s->iperm_dev, s->n, s->npad, 6);
+  RET(launch_check(s->ctx, "k_gather_out"));
+  RET(sync(s));
+  if (ep) *ep = sp->ep;
+  return THP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,138 @@
+// Internal definitions shared by the libthp host and device code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/thp.h"
+
+namespace thp {
+
+constexpr int kBlock = 64;  // atoms per block = one wavefront
+constexpr int kOrder = 5;   // PME B-spline order handled by the kernels
+constexpr double kCoulomb = 332.063714;  // v1.2/source/MOD_units.f:81
+constexpr double kTinyPol = 1.0e-5;      // v1.2/source/MOD_polar.f:36
+constexpr int kCodeShift = 24;           // tight-list entry = k | code << 24
+constexpr int kMaxAtoms = 1 << kCodeShift;
+
+// PCG termination codes held in SolveState::conv
+enum : int { kRunning = 0, kConverged = 1, kMaxIter = 2, kNonFinite = 3, kZeroCurv = 4 };
+
+// Device-resident state of one PCG solve (all scalars FP64 on the device).
+struct SolveState {
+  double ggold[2];
+  double gg[2];
+  double ggnew[2];
+  double ene[2];
+  double resnrm;
+  double poleps;
+  double ep;
+  int niter;
+  int politer;
+  int conv;
+  int n_indef;
+  unsigned int counter[4];  // last-block detection, one per reducing kernel
+};
+
+// Everything a kernel needs about one system; lives in device memory so a
+// batched launch can reach many systems through one array.
+struct SysView {
+  int n, npad, nblk;
+  int prec;
+  double box[3];
+  double aewald, cut2, mbuf2, term_self;
+  double afrac[3];  // nfft_k / box_k (orthorhombic fractional scaling)
+  int nfft[3];
+  int ncode;
+  double ctf[10][10];  // Cartesian -> fractional multipoles (cart_to_frac)
+  double ftc[10][10];  // fractional -> Cartesian potential (frac_to_cart)
+  // sorted-order per-atom arrays
+  const double* x;
+  const double* y;
+  const double* z;
+  const double* rpole;  // [10][npad]: q, dx,dy,dz, qxx,qxy,qxz,qyy,qyz,qzz
+  const double* pol;
+  const double* pdamp;
+  const double* thole;
+  const double* scale_tab;  // [ncode][3] d,p,u
+  const int* excl_off;      // [n+1]
+  const int* excl_k;
+  const int* excl_code;
+  // Verlet list (cut + buffer), full, sorted k per atom: nb[s*npad + i]
+  int maxnb;
+  int* nb;
+  int* nbcount;
+  // tight list (cut), full: tnb[s*npad + i] = k | code << 24
+  int* tnb;
+  int* tcount;
+  double* bbox;       // [nblk][6] block bounding boxes (centre, half extent)
+  // PME
+  void* grid;         // complex interleaved, x fastest, real type per prec
+  const void* qfac;   // K1*K2*K3 real
+  // solver vectors [6][npad] FP64 (component-major)
+  double* ef;
+  double* mu;
+  double* res;
+  double* h;
+  double* hreal;
+  double* pp;
+  double* zr;
+  double* partial;  // [4][nblk][2]
+  SolveState* st;
+  int* err;         // device error flag (list overflow ...)
+};
+
+struct Work {  // one 64-atom block of one system
+  int sys;
+  int blk;
+};
+
+}  // namespace thp
+
+struct thp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  void* serve = nullptr;
+  int n_cu = 0;
+};
+
+struct thp_sys {
+  thp_ctx* ctx = nullptr;
+  int n = 0, npad = 0, nblk = 0;
+  int prec = THP_PREC_FP64;
+  double box[3] = {0, 0, 0};
+  double aewald = 0, cut = 0, buffer = 0;
+  int nfft[3] = {0, 0, 0};
+  int order = 5;
+  int segs = 4;  // waves per 64-atom block in the pair kernels
+  std::vector<int> perm;   // sorted slot -> original index
+  std::vector<int> iperm;  // original index -> sorted slot
+  int* iperm_dev = nullptr;
+  double* cphi = nullptr;  // [10][npad] scratch for efld0_recip
+  std::vector<double> hx, hy, hz;       // sorted-order coordinates (host copy)
+  std::vector<double> bx0, by0, bz0;    // coordinates at the last list build
+  bool list_valid = false;
+  bool tight_valid = false;
+  int maxnb = 0;
+  // device allocations
+  std::vector<void*> allocs;
+  int64_t bytes = 0;
+  thp::SysView view;           // host copy of the view
+  thp::SysView* d_view = nullptr;
+  thp::Work* d_work = nullptr;  // single-system work table [nblk]
+  // block neighbor lists
+  double* bbox = nullptr;  // [nblk][6] centre + half-extent
+  int* bcount = nullptr;   // [nblk]
+  int* blist = nullptr;    // [nblk][maxbl]
+  int maxbl = 0;
+  // rocFFT
+  rocfft_plan fwd = nullptr, bwd = nullptr;
+  rocfft_execution_info info = nullptr;
+  void* fft_work = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};

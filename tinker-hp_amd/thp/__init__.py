@@ -1,0 +1,388 @@
+"""Python host binding of libthp, the MI355X-native AMOEBA induced-dipole solver.
+
+The product is the C ABI in include/thp.h (libthp.so, hand-written HIP
+kernels for gfx950).  This module binds it with ctypes and mirrors the
+reference routine names of the polarization path:
+
+    Solver.nblist()          <- nblist / mlistcell      (v1.2/source/nblist.f:17-247)
+    Solver.efld0_direct()    <- efld0_direct            (v1.2/source/newinduce_pme.f:948)
+    Solver.efld0_recip()     <- efld0_recip             (v1.2/source/newinduce_pme.f:1671)
+    Solver.tmatxb_pme(mu)    <- tmatxb_pme              (v1.2/source/newinduce_pme.f:1392)
+    Solver.tmatxbrecip(mu)   <- tmatxbrecip             (v1.2/source/newinduce_pme.f:1919)
+    Solver.grid_uind_site()  <- grid_uind_site          (v1.2/source/pmestuff.f:280)
+    Solver.fphi_uind_site()  <- fphi_uind_site          (v1.2/source/pmestuff.f:746)
+    Solver.newinduce_pme2()  <- newinduce_pme2          (v1.2/source/newinduce_pme2.f:16)
+
+There is no CPU fallback: every call goes to the GPU through libthp and
+raises ThpError when the library, the device or a kernel fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libthp.so")
+
+THP_OK = 0
+THP_PREC_FP64 = 0
+THP_PREC_MIXED = 1
+ERRORS = {
+    -1: "THP_ERR_ARG", -2: "THP_ERR_HIP", -3: "THP_ERR_FFT", -4: "THP_ERR_NOMEM",
+    -5: "THP_ERR_LIST", -6: "THP_ERR_NOCONV", -7: "THP_ERR_NAN", -8: "THP_ERR_STATE",
+    -9: "THP_ERR_BUSY",
+}
+COULOMB = 332.063714  # v1.2/source/MOD_units.f:81
+DEBYE = 4.80321       # v1.2/source/MOD_units.f:82
+
+
+class ThpError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("box", C.c_double * 3),
+        ("x", _dp), ("y", _dp), ("z", _dp), ("rpole", _dp), ("polarity", _dp), ("thole", _dp),
+        ("nscaled", C.c_int), ("scaled_ik", _ip), ("scaled_dpu", _dp),
+        ("aewald", C.c_double), ("cut", C.c_double), ("buffer", C.c_double),
+        ("nfft", C.c_int * 3), ("order", C.c_int), ("precision", C.c_int),
+    ]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("poleps", C.c_double), ("politer", C.c_int), ("guess", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [
+        ("niter", C.c_int), ("status", C.c_int), ("resnrm", C.c_double), ("ep", C.c_double),
+        ("n_indefinite", C.c_int), ("t_field_ms", C.c_double), ("t_solve_ms", C.c_double),
+    ]
+
+
+class ServeOpts(C.Structure):
+    _fields_ = [
+        ("max_systems", C.c_int), ("hbm_budget", C.c_int64), ("iters_per_graph", C.c_int),
+        ("autotune", C.c_int), ("poleps", C.c_double), ("politer", C.c_int),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("ticket", C.c_uint64), ("system_id", C.c_int), ("status", C.c_int), ("niter", C.c_int),
+        ("ep", C.c_double), ("latency_ms", C.c_double), ("resnrm", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def load_lib(path: str | None = None) -> C.CDLL:
+    """Load libthp.so (built by __graft_entry__.build()); fail loudly if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ThpError(-8, f"{p} not built; run __graft_entry__.build()")
+    lib = C.CDLL(p)
+    vp = C.c_void_p
+    sig = {
+        "thp_init": (C.c_int, [C.POINTER(vp), C.c_int]),
+        "thp_finalize": (C.c_int, [vp]),
+        "thp_last_error": (C.c_char_p, [vp]),
+        "thp_version": (C.c_char_p, []),
+        "thp_system_create": (C.c_int, [vp, C.POINTER(SystemDesc), C.POINTER(vp)]),
+        "thp_system_update_coords": (C.c_int, [vp, _dp, _dp, _dp]),
+        "thp_system_destroy": (C.c_int, [vp]),
+        "thp_system_footprint": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "thp_nblist": (C.c_int, [vp, C.c_int]),
+        "thp_nblist_get": (C.c_int, [vp, C.POINTER(C.c_int64), _ip, C.c_int64]),
+        "thp_efld0_direct": (C.c_int, [vp, _dp]),
+        "thp_efld0_recip": (C.c_int, [vp, _dp]),
+        "thp_tmatxb": (C.c_int, [vp, _dp, _dp, C.c_int]),
+        "thp_tmatxb_recip": (C.c_int, [vp, _dp, _dp]),
+        "thp_grid_uind": (C.c_int, [vp, _dp, _dp, _dp]),
+        "thp_fphi_uind": (C.c_int, [vp, _dp, _dp, _dp]),
+        "thp_pme_convolve": (C.c_int, [vp]),
+        "thp_induce": (C.c_int, [vp, C.POINTER(SolveOpts), _dp, _dp, _dp, C.POINTER(SolveStats)]),
+        "thp_tmatxb_dev": (C.c_int, [vp, vp, vp, C.c_int]),
+        "thp_induce_dev": (C.c_int, [vp, C.POINTER(SolveOpts), vp, _dp, C.POINTER(SolveStats)]),
+        "thp_serve_start": (C.c_int, [vp, C.POINTER(ServeOpts)]),
+        "thp_serve_add_system": (C.c_int, [vp, C.POINTER(SystemDesc), _ip]),
+        "thp_serve_submit": (C.c_int, [vp, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_uint64)]),
+        "thp_serve_poll": (C.c_int, [vp, C.c_uint64, C.POINTER(Result), C.c_double]),
+        "thp_serve_get_dipoles": (C.c_int, [vp, C.c_int, _dp]),
+        "thp_serve_stop": (C.c_int, [vp]),
+        "thp_make_system": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_double, C.c_uint64,
+                                      _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _dp]),
+        "thp_pick_grid": (C.c_int, [C.c_double]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _d(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a: np.ndarray):
+    return a.ctypes.data_as(_ip)
+
+
+# --------------------------------------------------------------- systems
+@dataclass
+class System:
+    """Inputs of one polarization solve (caller order, FP64)."""
+    n: int
+    box: np.ndarray
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    rpole: np.ndarray        # (n, 13)
+    polarity: np.ndarray
+    thole: np.ndarray
+    scaled_ik: np.ndarray    # (m, 2) int32
+    scaled_dpu: np.ndarray   # (m, 3)
+    aewald: float = 0.4
+    cut: float = 7.0
+    buffer: float = 2.0
+    nfft: tuple = (24, 24, 24)
+    order: int = 5
+    name: str = ""
+    extra: dict = field(default_factory=dict)
+
+    def with_coords(self, x, y, z):
+        s = System(**{k: getattr(self, k) for k in self.__dataclass_fields__})
+        s.x, s.y, s.z = (np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z))
+        return s
+
+    def desc(self, precision: int) -> SystemDesc:
+        d = SystemDesc()
+        d.n = self.n
+        for k in range(3):
+            d.box[k] = float(self.box[k])
+            d.nfft[k] = int(self.nfft[k])
+        self._keep = [np.ascontiguousarray(a) for a in (
+            self.x, self.y, self.z, self.rpole, self.polarity, self.thole)]
+        self._keep_i = np.ascontiguousarray(self.scaled_ik, dtype=np.int32)
+        self._keep_s = np.ascontiguousarray(self.scaled_dpu, dtype=np.float64)
+        d.x, d.y, d.z, d.rpole, d.polarity, d.thole = (_d(a) for a in self._keep)
+        d.nscaled = len(self._keep_i)
+        d.scaled_ik = _i(self._keep_i)
+        d.scaled_dpu = _d(self._keep_s)
+        d.aewald, d.cut, d.buffer = self.aewald, self.cut, self.buffer
+        d.order = self.order
+        d.precision = precision
+        return d
+
+
+def make_system(n: int, box: float, n_solute: int = 0, blob_radius: float = 0.0, seed: int = 1,
+                nfft=None, cut: float = 7.0, buffer: float = 2.0, aewald: float = 0.4,
+                name: str = "") -> System:
+    """Seeded water box (and optional solute blob) from libthp's generator."""
+    lib = load_lib()
+    x, y, z = (np.zeros(n) for _ in range(3))
+    rpole = np.zeros((n, 13))
+    pol, th = np.zeros(n), np.zeros(n)
+    ik = np.zeros((3 * n, 2), dtype=np.int32)
+    dpu = np.zeros((3 * n, 3))
+    ns = C.c_int(0)
+    rc = lib.thp_make_system(n, box, n_solute, blob_radius, seed, _d(x), _d(y), _d(z), _d(rpole),
+                             _d(pol), _d(th), C.byref(ns), _i(ik), _d(dpu))
+    if rc != THP_OK:
+        raise ThpError(rc, "thp_make_system")
+    if nfft is None:
+        k = lib.thp_pick_grid(box)
+        nfft = (k, k, k)
+    m = ns.value
+    return System(n=n, box=np.array([box, box, box], dtype=np.float64), x=x, y=y, z=z, rpole=rpole,
+                  polarity=pol, thole=th, scaled_ik=ik[:m].copy(), scaled_dpu=dpu[:m].copy(),
+                  aewald=aewald, cut=cut, buffer=buffer, nfft=tuple(int(v) for v in nfft), name=name)
+
+
+# The workloads (water recipe, seeded); sizes as named by the benchmark.
+def water_1944(seed=1):
+    """648 waters, 18.643 A box, 24^3 grid (the CPU-runnable case)."""
+    return make_system(1944, 18.643, seed=seed, nfft=(24, 24, 24), name="water-1944")
+
+
+def water_23559(seed=1):
+    """7,853 waters, 62.23 A box, 64^3 grid (the 24k-atom headline box)."""
+    return make_system(23559, 62.23, seed=seed, nfft=(64, 64, 64), name="water-23559")
+
+
+def solute_96000(seed=1):
+    """96,000 atoms, 100 A box, 20,000-atom solute blob of radius 30 A, 108^3."""
+    return make_system(96000, 100.0, n_solute=20000, blob_radius=30.0, seed=seed,
+                       nfft=(108, 108, 108), name="solute-96000")
+
+
+def water_1068000(seed=1):
+    """356,000 waters, 220 A box, 240^3 grid."""
+    return make_system(1068000, 220.0, seed=seed, nfft=(240, 240, 240), name="water-1068000")
+
+
+def serving_system(n: int, solute_frac: float, seed: int) -> System:
+    """Serving-stream system: water box at 0.1 atoms/A^3 with a solute blob.
+    Buffer min(2, L/2 - cut - 0.05) keeps cut+buffer inside half the box."""
+    L = (n / 0.1) ** (1.0 / 3.0)
+    ns = int(round(solute_frac * n))
+    R = (3.0 * ns / (4.0 * np.pi * 0.177)) ** (1.0 / 3.0) if ns > 0 else 0.0
+    buf = min(2.0, 0.5 * L - 7.0 - 0.05)
+    if buf <= 0:
+        raise ValueError(f"box {L:.2f} too small for a 7 A cutoff")
+    return make_system(n, L, n_solute=ns, blob_radius=R, seed=seed, buffer=buf,
+                       name=f"serve-{n}-{seed}")
+
+
+def tiny_water(nmol: int, box: float, nfft: int, seed=1, cut=None, buffer=None):
+    """Small water box for parity tests (cut+buffer fitted to the box)."""
+    c = cut if cut is not None else min(7.0, 0.5 * box - 0.6)
+    b = buffer if buffer is not None else min(2.0, 0.5 * box - c - 0.05)
+    return make_system(3 * nmol, box, seed=seed, nfft=(nfft,) * 3, cut=c, buffer=b,
+                       name=f"water-{3 * nmol}")
+
+
+# ---------------------------------------------------------------- solver
+class Context:
+    def __init__(self, device: int = 0):
+        self.lib = load_lib()
+        self.h = C.c_void_p()
+        rc = self.lib.thp_init(C.byref(self.h), device)
+        if rc != THP_OK:
+            msg = self.lib.thp_last_error(self.h).decode() if self.h else ""
+            raise ThpError(rc, msg)
+
+    def check(self, rc, what=""):
+        if rc != THP_OK:
+            raise ThpError(rc, f"{what}: {self.lib.thp_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            self.lib.thp_finalize(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Solver:
+    """One resident system on the GPU; methods mirror the reference routines."""
+
+    def __init__(self, ctx: Context, system: System, precision: int = THP_PREC_FP64):
+        self.ctx, self.sys, self.precision = ctx, system, precision
+        self.lib = ctx.lib
+        self.h = C.c_void_p()
+        d = system.desc(precision)
+        ctx.check(self.lib.thp_system_create(ctx.h, C.byref(d), C.byref(self.h)), "thp_system_create")
+
+    def close(self):
+        if self.h:
+            self.lib.thp_system_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc, what):
+        self.ctx.check(rc, what)
+
+    def footprint(self) -> int:
+        b = C.c_int64(0)
+        self._c(self.lib.thp_system_footprint(self.h, C.byref(b)), "footprint")
+        return b.value
+
+    def update_coords(self, x, y, z):
+        x, y, z = (np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z))
+        self._c(self.lib.thp_system_update_coords(self.h, _d(x), _d(y), _d(z)), "update_coords")
+
+    def nblist(self, force_rebuild=True):
+        self._c(self.lib.thp_nblist(self.h, 1 if force_rebuild else 0), "nblist")
+
+    def nblist_pairs(self) -> np.ndarray:
+        cnt = C.c_int64(0)
+        self._c(self.lib.thp_nblist_get(self.h, C.byref(cnt), None, 0), "nblist_get")
+        ik = np.zeros((cnt.value, 2), dtype=np.int32)
+        self._c(self.lib.thp_nblist_get(self.h, C.byref(cnt), _i(ik), cnt.value), "nblist_get")
+        return ik
+
+    def efld0_direct(self) -> np.ndarray:
+        ef = np.zeros((self.sys.n, 2, 3))
+        self._c(self.lib.thp_efld0_direct(self.h, _d(ef)), "efld0_direct")
+        return ef
+
+    def efld0_recip(self) -> np.ndarray:
+        cphi = np.zeros((self.sys.n, 10))
+        self._c(self.lib.thp_efld0_recip(self.h, _d(cphi)), "efld0_recip")
+        return cphi
+
+    def tmatxb_pme(self, mu, dodiag=True) -> np.ndarray:
+        mu = np.ascontiguousarray(mu, dtype=np.float64).reshape(self.sys.n, 2, 3)
+        efi = np.zeros_like(mu)
+        self._c(self.lib.thp_tmatxb(self.h, _d(mu), _d(efi), 1 if dodiag else 0), "tmatxb")
+        return efi
+
+    def tmatxbrecip(self, mu) -> np.ndarray:
+        mu = np.ascontiguousarray(mu, dtype=np.float64).reshape(self.sys.n, 2, 3)
+        out = np.zeros_like(mu)
+        self._c(self.lib.thp_tmatxb_recip(self.h, _d(mu), _d(out)), "tmatxb_recip")
+        return out
+
+    def grid_uind_site(self, fuind, fuinp) -> np.ndarray:
+        fd = np.ascontiguousarray(fuind, dtype=np.float64).reshape(self.sys.n, 3)
+        fp = np.ascontiguousarray(fuinp, dtype=np.float64).reshape(self.sys.n, 3)
+        k1, k2, k3 = self.sys.nfft
+        g = np.zeros((k3, k2, k1, 2))
+        self._c(self.lib.thp_grid_uind(self.h, _d(fd), _d(fp), _d(g)), "grid_uind")
+        return g
+
+    def fphi_uind_site(self, qgrid=None):
+        p1 = np.zeros((self.sys.n, 3))
+        p2 = np.zeros((self.sys.n, 3))
+        g = None if qgrid is None else np.ascontiguousarray(qgrid, dtype=np.float64)
+        self._c(self.lib.thp_fphi_uind(self.h, None if g is None else _d(g), _d(p1), _d(p2)), "fphi_uind")
+        return p1, p2
+
+    def pme_convolve(self):
+        self._c(self.lib.thp_pme_convolve(self.h), "pme_convolve")
+
+    def newinduce_pme2(self, poleps=1e-8, politer=500, mu0=None):
+        """Induced dipoles uind/uinp (n,3) each, E_pol (kcal/mol) and stats."""
+        n = self.sys.n
+        uind, uinp = np.zeros((n, 3)), np.zeros((n, 3))
+        guess = 0
+        if mu0 is not None:
+            mu0 = np.asarray(mu0).reshape(n, 2, 3)
+            uind[:] = mu0[:, 0]
+            uinp[:] = mu0[:, 1]
+            guess = 1
+        o = SolveOpts(poleps=poleps, politer=politer, guess=guess)
+        st = SolveStats()
+        ep = C.c_double(0)
+        self._c(self.lib.thp_induce(self.h, C.byref(o), _d(uind), _d(uinp), C.byref(ep), C.byref(st)),
+                "induce")
+        mu = np.stack([uind, uinp], axis=1)
+        return mu, ep.value, st
+
+    induce = newinduce_pme2
