@@ -184,35 +184,47 @@ void conversion_mats(const double box[3], const int nf[3], double ctf[10][10], d
   }
 }
 
-// Spatial order: Morton key of ~2.5 A cells of the wrapped coordinates,
-// ties broken by the original index (deterministic).
+// Spatial order: recursive median bisection of the wrapped coordinates
+// along the longest extent, with left halves holding a multiple of 64 atoms,
+// so that every 64-atom block (one wavefront) is a compact brick.  Ties
+// are broken by the original index, so the order is deterministic.
+void bisect(std::vector<int>& idx, size_t lo, size_t hi, const std::vector<double>& w) {
+  const size_t n = hi - lo;
+  if (n <= (size_t)kBlock) return;
+  double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+  for (size_t q = lo; q < hi; ++q)
+    for (int d = 0; d < 3; ++d) {
+      mn[d] = std::min(mn[d], w[3 * idx[q] + d]);
+      mx[d] = std::max(mx[d], w[3 * idx[q] + d]);
+    }
+  int d = 0;
+  for (int e = 1; e < 3; ++e)
+    if (mx[e] - mn[e] > mx[d] - mn[d]) d = e;
+  size_t nl = (size_t)kBlock * std::max<size_t>(1, (n + kBlock) / (2 * kBlock));
+  auto less = [&](int a, int b) { return w[3 * a + d] < w[3 * b + d] || (w[3 * a + d] == w[3 * b + d] && a < b); };
+  std::nth_element(idx.begin() + lo, idx.begin() + lo + nl, idx.begin() + hi, less);
+  bisect(idx, lo, lo + nl, w);
+  bisect(idx, lo + nl, hi, w);
+}
+
 std::vector<int> spatial_order(int n, const double* x, const double* y, const double* z, const double box[3]) {
-  auto spread_bits = [](uint64_t v) {
-    v &= 0x1fffff;
-    v = (v | v << 32) & 0x1f00000000ffffull;
-    v = (v | v << 16) & 0x1f0000ff0000ffull;
-    v = (v | v << 8) & 0x100f00f00f00f00full;
-    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
-    v = (v | v << 2) & 0x1249249249249249ull;
-    return v;
-  };
-  int nc[3];
-  for (int d = 0; d < 3; ++d) nc[d] = std::max(1, std::min(1 << 20, int(box[d] / 2.5)));
-  std::vector<std::pair<uint64_t, int>> key(n);
+  std::vector<double> w((size_t)3 * n);
   for (int i = 0; i < n; ++i) {
     const double p[3] = {x[i], y[i], z[i]};
-    uint64_t c[3];
-    for (int d = 0; d < 3; ++d) {
-      double f = p[d] / box[d];
-      f -= std::floor(f);
-      c[d] = (uint64_t)std::min(nc[d] - 1, int(f * nc[d]));
-    }
-    key[i] = {spread_bits(c[0]) | spread_bits(c[1]) << 1 | spread_bits(c[2]) << 2, i};
+    for (int d = 0; d < 3; ++d) w[3 * i + d] = p[d] - box[d] * std::floor(p[d] / box[d]);
   }
-  std::sort(key.begin(), key.end());
   std::vector<int> perm(n);
-  for (int i = 0; i < n; ++i) perm[i] = key[i].second;
+  std::iota(perm.begin(), perm.end(), 0);
+  bisect(perm, 0, (size_t)n, w);
   return perm;
+}
+
+// spreading brick edge: the largest B in 8..4 with K % B == 0 or >= 4, so
+// every brick (the last one included) is at least 4 points wide
+int pick_brick(int K) {
+  for (int B = 8; B >= 4; --B)
+    if (K % B == 0 || K % B >= 4) return B;
+  return 4;
 }
 
 thp_status upload_view(thp_sys* s) {
@@ -242,26 +254,26 @@ thp_status launch_check(thp_ctx* ctx, const char* what) {
   return THP_OK;
 }
 
-thp_status pair_tmatxb(thp_sys* s, int vin, int vout, int dodiag, int check_conv) {
+template <typename real, int KIND>
+void launch_pair(thp_sys* s, int vin, int vout, int dodiag, int check_conv) {
   hipStream_t st = s->ctx->stream;
-  const int S = s->segs;
-  if (s->prec == THP_PREC_FP64) {
-    if (S == 1) hipLaunchKernelGGL((k_tmatxb<double, 1>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work, vin, vout, dodiag, check_conv);
-    else if (S == 2) hipLaunchKernelGGL((k_tmatxb<double, 2>), g1(s->nblk), dim3(128), 0, st, s->d_view, s->d_work, vin, vout, dodiag, check_conv);
-    else hipLaunchKernelGGL((k_tmatxb<double, 4>), g1(s->nblk), dim3(256), 0, st, s->d_view, s->d_work, vin, vout, dodiag, check_conv);
-  } else {
-    if (S == 1) hipLaunchKernelGGL((k_tmatxb<float, 1>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work, vin, vout, dodiag, check_conv);
-    else if (S == 2) hipLaunchKernelGGL((k_tmatxb<float, 2>), g1(s->nblk), dim3(128), 0, st, s->d_view, s->d_work, vin, vout, dodiag, check_conv);
-    else hipLaunchKernelGGL((k_tmatxb<float, 4>), g1(s->nblk), dim3(256), 0, st, s->d_view, s->d_work, vin, vout, dodiag, check_conv);
+  switch (s->segs) {
+    case 1: hipLaunchKernelGGL((k_pair_union<real, KIND, 1>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work, vin, vout, dodiag, check_conv); break;
+    case 2: hipLaunchKernelGGL((k_pair_union<real, KIND, 2>), g1(s->nblk), dim3(128), 0, st, s->d_view, s->d_work, vin, vout, dodiag, check_conv); break;
+    default: hipLaunchKernelGGL((k_pair_union<real, KIND, 4>), g1(s->nblk), dim3(256), 0, st, s->d_view, s->d_work, vin, vout, dodiag, check_conv); break;
   }
-  return launch_check(s->ctx, "k_tmatxb");
+}
+
+thp_status pair_tmatxb(thp_sys* s, int vin, int vout, int dodiag, int check_conv) {
+  if (s->prec == THP_PREC_FP64) launch_pair<double, 1>(s, vin, vout, dodiag, check_conv);
+  else launch_pair<float, 1>(s, vin, vout, dodiag, check_conv);
+  return launch_check(s->ctx, "k_pair_union<tmatxb>");
 }
 
 thp_status pair_efld0(thp_sys* s) {
-  hipStream_t st = s->ctx->stream;
-  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_efld0<double, 4>), g1(s->nblk), dim3(256), 0, st, s->d_view, s->d_work);
-  else hipLaunchKernelGGL((k_efld0<float, 4>), g1(s->nblk), dim3(256), 0, st, s->d_view, s->d_work);
-  return launch_check(s->ctx, "k_efld0");
+  if (s->prec == THP_PREC_FP64) launch_pair<double, 0>(s, 0, 0, 0, 0);
+  else launch_pair<float, 0>(s, 0, 0, 0, 0);
+  return launch_check(s->ctx, "k_pair_union<efld0>");
 }
 
 size_t grid_points(const thp_sys* s) { return (size_t)s->nfft[0] * s->nfft[1] * s->nfft[2]; }
@@ -275,9 +287,30 @@ thp_status zero_grid(thp_sys* s) {
 }
 
 thp_status spread_uind(thp_sys* s, int vin, int frac, int check_conv) {
-  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_spread_uind<double>), g1(s->nblk), dim3(64), 0, s->ctx->stream, s->d_view, s->d_work, vin, frac, check_conv);
-  else hipLaunchKernelGGL((k_spread_uind<float>), g1(s->nblk), dim3(64), 0, s->ctx->stream, s->d_view, s->d_work, vin, frac, check_conv);
-  return launch_check(s->ctx, "k_spread_uind");
+  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_spread_brick<double, 0>), g1(s->nbricks), dim3(64 * kSpreadWaves), 0, s->ctx->stream, s->d_view, s->d_brickwork, vin, frac, check_conv);
+  else hipLaunchKernelGGL((k_spread_brick<float, 0>), g1(s->nbricks), dim3(64 * kSpreadWaves), 0, s->ctx->stream, s->d_view, s->d_brickwork, vin, frac, check_conv);
+  return launch_check(s->ctx, "k_spread_brick<uind>");
+}
+
+thp_status spread_mpole(thp_sys* s) {
+  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_spread_brick<double, 1>), g1(s->nbricks), dim3(64 * kSpreadWaves), 0, s->ctx->stream, s->d_view, s->d_brickwork, 0, 0, 0);
+  else hipLaunchKernelGGL((k_spread_brick<float, 1>), g1(s->nbricks), dim3(64 * kSpreadWaves), 0, s->ctx->stream, s->d_view, s->d_brickwork, 0, 0, 0);
+  return launch_check(s->ctx, "k_spread_brick<mpole>");
+}
+
+// per coordinate set: B-spline values, grid nodes and brick bins
+thp_status pme_prep(thp_sys* s) {
+  hipStream_t st = s->ctx->stream;
+  HIPCHK(s->ctx, hipMemsetAsync(s->view.bin_count, 0, sizeof(int) * s->nbricks, st));
+  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_pme_prep<double>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
+  else hipLaunchKernelGGL((k_pme_prep<float>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
+  RET(launch_check(s->ctx, "k_pme_prep"));
+  hipLaunchKernelGGL(k_pme_scan, g1(1), dim3(1024), 0, st, s->d_view, (const int*)s->d_sysid);
+  RET(launch_check(s->ctx, "k_pme_scan"));
+  hipLaunchKernelGGL(k_pme_fill, g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
+  RET(launch_check(s->ctx, "k_pme_fill"));
+  hipLaunchKernelGGL(k_pme_binsort, g1(s->nbricks), dim3(256), 0, st, s->d_view, s->d_brickwork);
+  return launch_check(s->ctx, "k_pme_binsort");
 }
 
 thp_status convolve(thp_sys* s, int check_conv) {
@@ -294,8 +327,9 @@ thp_status convolve(thp_sys* s, int check_conv) {
 
 template <int MODE>
 thp_status launch_gather_uind(thp_sys* s, int vin, int vout, int frac_out) {
-  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_uind<double, MODE>), g1(s->nblk), dim3(64), 0, s->ctx->stream, s->d_view, s->d_work, vin, vout, frac_out);
-  else hipLaunchKernelGGL((k_gather_uind<float, MODE>), g1(s->nblk), dim3(64), 0, s->ctx->stream, s->d_view, s->d_work, vin, vout, frac_out);
+  const int nb = s->view.gat_blocks;
+  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_uind<double, MODE>), g1(nb), dim3(256), 0, s->ctx->stream, s->d_view, s->d_gatwork, vin, vout, frac_out);
+  else hipLaunchKernelGGL((k_gather_uind<float, MODE>), g1(nb), dim3(256), 0, s->ctx->stream, s->d_view, s->d_gatwork, vin, vout, frac_out);
   return launch_check(s->ctx, "k_gather_uind");
 }
 
@@ -303,10 +337,7 @@ thp_status launch_gather_uind(thp_sys* s, int vin, int vout, int frac_out) {
 thp_status field(thp_sys* s, double* cphi_dev, int add_to_ef, int do_direct) {
   hipStream_t st = s->ctx->stream;
   if (do_direct) RET(pair_efld0(s));
-  RET(zero_grid(s));
-  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_spread_mpole<double>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
-  else hipLaunchKernelGGL((k_spread_mpole<float>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
-  RET(launch_check(s->ctx, "k_spread_mpole"));
+  RET(spread_mpole(s));
   RET(convolve(s, 0));
   if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_mpole<double>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work, cphi_dev, add_to_ef);
   else hipLaunchKernelGGL((k_gather_mpole<float>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work, cphi_dev, add_to_ef);
@@ -335,12 +366,95 @@ thp_status gather_out(thp_sys* s, double* host, const double* src_dev, int ncomp
   return THP_OK;
 }
 
+// bounding boxes and candidate block pairs (cut + buffer); valid until an
+// atom moves more than buffer/2
+thp_status build_block_lists(thp_sys* s) {
+  hipStream_t st = s->ctx->stream;
+  hipLaunchKernelGGL(k_bbox, g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
+  RET(launch_check(s->ctx, "k_bbox"));
+  hipLaunchKernelGGL((k_blockpairs<false>), g1(s->nblk), dim3(256), 0, st, s->d_view, s->d_work, s->bcount, (int*)nullptr, 0);
+  RET(launch_check(s->ctx, "k_blockpairs"));
+  std::vector<int> bc(s->nblk);
+  HIPCHK(s->ctx, hipMemcpyAsync(bc.data(), s->bcount, sizeof(int) * s->nblk, hipMemcpyDeviceToHost, st));
+  RET(sync(s));
+  const int maxbl = *std::max_element(bc.begin(), bc.end());
+  if (maxbl > s->maxbl) {
+    dfree(s, s->blist);
+    s->blist = nullptr;
+    RET(dalloc(s, &s->blist, (size_t)s->nblk * maxbl));
+    s->maxbl = maxbl;
+  }
+  hipLaunchKernelGGL((k_blockpairs<true>), g1(s->nblk), dim3(256), 0, st, s->d_view, s->d_work, s->bcount, s->blist, s->maxbl);
+  RET(launch_check(s->ctx, "k_blockpairs"));
+  s->bx0 = s->hx;
+  s->by0 = s->hy;
+  s->bz0 = s->hz;
+  s->list_valid = true;
+  s->tight_valid = false;
+  s->verlet_valid = false;
+  return THP_OK;
+}
+
+// block-union lists for the current coordinates (cut + skin, unscaled
+// pairs); single pass, capacities grown and the pass repeated on overflow
+thp_status build_union(thp_sys* s) {
+  hipStream_t st = s->ctx->stream;
+  hipLaunchKernelGGL(k_bbox, g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
+  RET(launch_check(s->ctx, "k_bbox"));
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    if (s->view.umax == 0) {
+      s->view.umax = 2048;
+      s->view.lmax = 384;
+      RET(dalloc(s, &s->view.ulist, (size_t)s->nblk * s->view.umax));
+      RET(dalloc(s, &s->view.llist, (size_t)s->view.lmax * s->npad));
+      RET(upload_view(s));
+    }
+    hipLaunchKernelGGL(k_union, g1(s->nblk), dim3(64 * kUnionWaves), 0, st, s->d_view, s->d_work, (const int*)s->bcount,
+                       (const int*)s->blist, s->maxbl);
+    RET(launch_check(s->ctx, "k_union"));
+    int err = 0;
+    HIPCHK(s->ctx, hipMemcpyAsync(&err, s->view.err, sizeof(int), hipMemcpyDeviceToHost, st));
+    RET(sync(s));
+    if (!err) {
+      s->ux0 = s->hx;
+      s->uy0 = s->hy;
+      s->uz0 = s->hz;
+      return THP_OK;
+    }
+    // grow to the sizes the pass counted, then repeat it
+    std::vector<int> uc(s->nblk), lc(s->npad);
+    HIPCHK(s->ctx, hipMemcpy(uc.data(), s->view.ucount, sizeof(int) * s->nblk, hipMemcpyDeviceToHost));
+    HIPCHK(s->ctx, hipMemcpy(lc.data(), s->view.lcount, sizeof(int) * s->npad, hipMemcpyDeviceToHost));
+    int umax = 0, lmax = 0;
+    for (int b = 0; b < s->nblk; ++b) umax = std::max(umax, uc[b]);
+    for (int q = 0; q < s->n; ++q) lmax = std::max(lmax, lc[q]);
+    if (umax > 65535) return fail(s->ctx, THP_ERR_LIST, "block union of %d atoms exceeds 16-bit indexing", umax);
+    dfree(s, s->view.ulist);
+    dfree(s, s->view.llist);
+    s->view.umax = std::max(s->view.umax, (umax + 63) / 64 * 64);
+    s->view.lmax = std::max(s->view.lmax, (lmax + 31) / 32 * 32);
+    RET(dalloc(s, &s->view.ulist, (size_t)s->nblk * s->view.umax));
+    RET(dalloc(s, &s->view.llist, (size_t)s->view.lmax * s->npad));
+    HIPCHK(s->ctx, hipMemsetAsync(s->view.err, 0, sizeof(int), st));
+    RET(upload_view(s));
+  }
+  return fail(s->ctx, THP_ERR_LIST, "block-union list capacity");
+}
+
 thp_status ensure_lists(thp_sys* s) {
-  if (!s->list_valid) RET(thp_nblist(s, 1));
+  if (!s->list_valid) RET(build_block_lists(s));
   if (!s->tight_valid) {
-    hipLaunchKernelGGL(k_tight, g1(s->nblk), dim3(64), 0, s->ctx->stream, s->d_view, s->d_work);
-    RET(launch_check(s->ctx, "k_tight"));
+    RET(build_union(s));
     s->tight_valid = true;
+  }
+  return THP_OK;
+}
+
+thp_status ensure_pme(thp_sys* s) {
+  if (!s->pme_valid) {
+    RET(pme_prep(s));
+    RET(check_err_flag(s));
+    s->pme_valid = true;
   }
   return THP_OK;
 }
@@ -412,6 +526,7 @@ thp_status thp_system_create(thp_ctx* ctx, const thp_system_desc* d, thp_sys** o
     s->nfft[k] = d->nfft[k];
   }
   s->aewald = d->aewald;
+  if (const char* e = getenv("THP_SEGS")) s->segs = std::max(1, std::min(4, atoi(e)));
   s->cut = d->cut;
   s->buffer = d->buffer;
   s->order = d->order;
@@ -487,6 +602,8 @@ thp_status thp_system_create(thp_ctx* ctx, const thp_system_desc* d, thp_sys** o
   v.n = n;
   v.npad = np;
   v.nblk = s->nblk;
+  v.red_blocks = std::max(1, std::min(256, (n + 255) / 256));
+  v.gat_blocks = std::max(1, std::min(4096, (n + kGatherAtoms - 1) / kGatherAtoms));
   v.prec = s->prec;
   for (int k = 0; k < 3; ++k) {
     v.box[k] = s->box[k];
@@ -495,6 +612,8 @@ thp_status thp_system_create(thp_ctx* ctx, const thp_system_desc* d, thp_sys** o
   }
   v.aewald = s->aewald;
   v.cut2 = s->cut * s->cut;
+  s->skin = std::min(0.5, 0.5 * s->buffer);
+  v.ulim2 = (s->cut + s->skin) * (s->cut + s->skin);
   v.mbuf2 = mb * mb;
   v.term_self = (4.0 / 3.0) * s->aewald * s->aewald * s->aewald / std::sqrt(M_PI);
   v.ncode = (int)codes.size();
@@ -525,11 +644,11 @@ thp_status thp_system_create(thp_ctx* ctx, const thp_system_desc* d, thp_sys** o
   v.scale_tab = dtab; v.excl_off = deoff; v.excl_k = dek; v.excl_code = dec;
   ALLOC(s->iperm_dev, n);
   H2D(s->iperm_dev, s->iperm.data(), sizeof(int) * n);
-  ALLOC(v.nbcount, np); ALLOC(v.tcount, np); ALLOC(v.bbox, (size_t)s->nblk * 6);
+  ALLOC(v.nbcount, np); ALLOC(v.lcount, np); ALLOC(v.ucount, s->nblk); ALLOC(v.bbox, (size_t)s->nblk * 6);
   ALLOC(s->bcount, s->nblk);
   ALLOC(v.ef, (size_t)6 * np); ALLOC(v.mu, (size_t)6 * np); ALLOC(v.res, (size_t)6 * np);
   ALLOC(v.h, (size_t)6 * np); ALLOC(v.hreal, (size_t)6 * np); ALLOC(v.pp, (size_t)6 * np);
-  ALLOC(v.zr, (size_t)6 * np); ALLOC(v.partial, (size_t)8 * s->nblk);
+  ALLOC(v.zr, (size_t)6 * np); ALLOC(v.partial, partial_doubles(v.gat_blocks, v.red_blocks));
   ALLOC(s->cphi, (size_t)10 * np);
   ALLOC(v.st, 1); ALLOC(v.err, 1);
   if (hipMemsetAsync(v.err, 0, sizeof(int), st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "memset"));
@@ -577,12 +696,53 @@ thp_status thp_system_create(thp_ctx* ctx, const thp_system_desc* d, thp_sys** o
     }
   }
 
+  // ---- brick bins for owner-computes spreading
+  for (int k = 0; k < 3; ++k) {
+    v.brick[k] = pick_brick(s->nfft[k]);
+    v.nbrick[k] = (s->nfft[k] + v.brick[k] - 1) / v.brick[k];
+  }
+  s->nbricks = v.nbrick[0] * v.nbrick[1] * v.nbrick[2];
+  if (s->prec == THP_PREC_FP64) {
+    double* th;
+    ALLOC(th, (size_t)np * kTheta);
+    v.theta = th;
+  } else {
+    float* th;
+    ALLOC(th, (size_t)np * kTheta);
+    v.theta = th;
+  }
+  ALLOC(v.g0, (size_t)4 * np);
+  ALLOC(v.bin_count, s->nbricks);
+  ALLOC(v.bin_start, s->nbricks + 1);
+  ALLOC(v.bin_list, n);
+  {
+    std::vector<Work> bw(s->nbricks);
+    for (int b = 0; b < s->nbricks; ++b) bw[b] = {0, b};
+    ALLOC(s->d_brickwork, s->nbricks);
+    H2D(s->d_brickwork, bw.data(), sizeof(Work) * s->nbricks);
+    if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "sync"));
+  }
+
   // ---- view + work table
   ALLOC(s->d_view, 1);
   std::vector<Work> wk(s->nblk);
   for (int b = 0; b < s->nblk; ++b) wk[b] = {0, b};
   ALLOC(s->d_work, s->nblk);
   H2D(s->d_work, wk.data(), sizeof(Work) * s->nblk);
+  ALLOC(s->d_redwork, v.red_blocks);
+  H2D(s->d_redwork, wk.data(), sizeof(Work) * v.red_blocks);
+  {
+    std::vector<Work> gw(v.gat_blocks);
+    for (int b = 0; b < v.gat_blocks; ++b) gw[b] = {0, b};
+    ALLOC(s->d_gatwork, v.gat_blocks);
+    H2D(s->d_gatwork, gw.data(), sizeof(Work) * v.gat_blocks);
+    if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "sync"));
+  }
+  ALLOC(s->d_sysid, 1);
+  {
+    const int zero = 0;
+    H2D(s->d_sysid, &zero, sizeof(int));
+  }
   for (int e = 0; e < 4; ++e)
     if (hipEventCreate(&s->ev[e]) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "event"));
   if (upload_view(s) != THP_OK) return cleanup(THP_ERR_HIP);
@@ -616,7 +776,7 @@ thp_status thp_system_update_coords(thp_sys* s, const double* x, const double* y
   if (!s || !x || !y || !z) return THP_ERR_ARG;
   const int n = s->n, np = s->npad;
   std::vector<double> hx(np, 0.0), hy(np, 0.0), hz(np, 0.0);
-  double maxd2 = 0.0;
+  double maxd2 = 0.0, maxu2 = 0.0;
   for (int q = 0; q < n; ++q) {
     const int a = s->perm[q];
     hx[q] = x[a];
@@ -625,6 +785,10 @@ thp_status thp_system_update_coords(thp_sys* s, const double* x, const double* y
     if (s->list_valid) {
       const double dx = hx[q] - s->bx0[q], dy = hy[q] - s->by0[q], dz = hz[q] - s->bz0[q];
       maxd2 = std::max(maxd2, dx * dx + dy * dy + dz * dz);
+    }
+    if (s->tight_valid) {
+      const double dx = hx[q] - s->ux0[q], dy = hy[q] - s->uy0[q], dz = hz[q] - s->uz0[q];
+      maxu2 = std::max(maxu2, dx * dx + dy * dy + dz * dz);
     }
   }
   hipStream_t st = s->ctx->stream;
@@ -635,32 +799,23 @@ thp_status thp_system_update_coords(thp_sys* s, const double* x, const double* y
   s->hx.assign(hx.begin(), hx.begin() + n);
   s->hy.assign(hy.begin(), hy.begin() + n);
   s->hz.assign(hz.begin(), hz.begin() + n);
-  // the Verlet list stays valid while no atom moved more than buffer/2
-  if (s->list_valid && maxd2 > 0.25 * s->buffer * s->buffer) s->list_valid = false;
-  s->tight_valid = false;
+  // block candidates (cut + buffer) stay valid while no atom moved more than
+  // (buffer - skin)/2 since their build, the block-union lists (cut + skin)
+  // while no atom moved more than skin/2
+  const double lim_b = 0.5 * (s->buffer - s->skin), lim_u = 0.5 * s->skin;
+  if (s->list_valid && maxd2 > lim_b * lim_b) s->list_valid = false;
+  if (!s->list_valid || maxu2 > lim_u * lim_u) s->tight_valid = false;
+  s->verlet_valid = false;
+  s->pme_valid = false;
   return THP_OK;
 }
 
 thp_status thp_nblist(thp_sys* s, int force_rebuild) {
   if (!s) return THP_ERR_ARG;
-  if (s->list_valid && !force_rebuild) return THP_OK;
+  if (!s->list_valid || force_rebuild) RET(build_block_lists(s));
+  if (s->verlet_valid) return THP_OK;
+  // per-atom Verlet list (mlistcell): every k with r^2 <= (cut+buffer)^2
   hipStream_t st = s->ctx->stream;
-  hipLaunchKernelGGL(k_bbox, g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
-  RET(launch_check(s->ctx, "k_bbox"));
-  hipLaunchKernelGGL((k_blockpairs<false>), g1(s->nblk), dim3(256), 0, st, s->d_view, s->d_work, s->bcount, (int*)nullptr, 0);
-  RET(launch_check(s->ctx, "k_blockpairs"));
-  std::vector<int> bc(s->nblk);
-  HIPCHK(s->ctx, hipMemcpyAsync(bc.data(), s->bcount, sizeof(int) * s->nblk, hipMemcpyDeviceToHost, st));
-  RET(sync(s));
-  const int maxbl = *std::max_element(bc.begin(), bc.end());
-  if (maxbl > s->maxbl) {
-    dfree(s, s->blist);
-    s->blist = nullptr;
-    RET(dalloc(s, &s->blist, (size_t)s->nblk * maxbl));
-    s->maxbl = maxbl;
-  }
-  hipLaunchKernelGGL((k_blockpairs<true>), g1(s->nblk), dim3(256), 0, st, s->d_view, s->d_work, s->bcount, s->blist, s->maxbl);
-  RET(launch_check(s->ctx, "k_blockpairs"));
   hipLaunchKernelGGL((k_nblist_atoms<false>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work, (const int*)s->bcount, (const int*)s->blist, s->maxbl);
   RET(launch_check(s->ctx, "k_nblist_atoms"));
   std::vector<int> cnt(s->npad);
@@ -671,10 +826,8 @@ thp_status thp_nblist(thp_sys* s, int force_rebuild) {
   maxnb = std::max(32, (maxnb + 31) / 32 * 32);
   if (maxnb > s->maxnb) {
     dfree(s, s->view.nb);
-    dfree(s, s->view.tnb);
-    s->view.nb = s->view.tnb = nullptr;
+    s->view.nb = nullptr;
     RET(dalloc(s, &s->view.nb, (size_t)maxnb * s->npad));
-    RET(dalloc(s, &s->view.tnb, (size_t)maxnb * s->npad));
     s->maxnb = maxnb;
     s->view.maxnb = maxnb;
     RET(upload_view(s));
@@ -682,11 +835,7 @@ thp_status thp_nblist(thp_sys* s, int force_rebuild) {
   hipLaunchKernelGGL((k_nblist_atoms<true>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work, (const int*)s->bcount, (const int*)s->blist, s->maxbl);
   RET(launch_check(s->ctx, "k_nblist_atoms"));
   RET(check_err_flag(s));
-  s->bx0 = s->hx;
-  s->by0 = s->hy;
-  s->bz0 = s->hz;
-  s->list_valid = true;
-  s->tight_valid = false;
+  s->verlet_valid = true;
   return THP_OK;
 }
 
@@ -724,6 +873,7 @@ thp_status thp_efld0_direct(thp_sys* s, double* ef) {
 
 thp_status thp_efld0_recip(thp_sys* s, double* cphi) {
   if (!s || !cphi) return THP_ERR_ARG;
+  RET(ensure_pme(s));
   RET(field(s, s->cphi, 0, 0));
   return gather_out(s, cphi, s->cphi, 10);
 }
@@ -749,8 +899,8 @@ thp_status thp_tmatxb_dev(thp_sys* s, const double* mu, double* efi, int dodiag)
 
 thp_status thp_tmatxb_recip(thp_sys* s, const double* mu, double* dipfield) {
   if (!s || !mu || !dipfield) return THP_ERR_ARG;
+  RET(ensure_pme(s));
   RET(scatter_in(s, s->view.pp, mu, 6));
-  RET(zero_grid(s));
   RET(spread_uind(s, 1, 0, 0));
   RET(convolve(s, 0));
   RET(launch_gather_uind<0>(s, 1, 3, 0));
@@ -765,8 +915,8 @@ thp_status thp_grid_uind(thp_sys* s, const double* fuind, const double* fuinp, d
       f[(size_t)a * 6 + c] = fuind[(size_t)a * 3 + c];
       f[(size_t)a * 6 + 3 + c] = fuinp[(size_t)a * 3 + c];
     }
+  RET(ensure_pme(s));
   RET(scatter_in(s, s->view.pp, f.data(), 6));
-  RET(zero_grid(s));
   RET(spread_uind(s, 1, 1, 0));
   if (qgrid) {
     const size_t cnt = 2 * grid_points(s);
@@ -791,6 +941,7 @@ thp_status thp_pme_convolve(thp_sys* s) {
 
 thp_status thp_fphi_uind(thp_sys* s, const double* qgrid, double* phi1, double* phi2) {
   if (!s || !phi1 || !phi2) return THP_ERR_ARG;
+  RET(ensure_pme(s));
   if (qgrid) {
     const size_t cnt = 2 * grid_points(s);
     if (s->prec == THP_PREC_FP64) {
@@ -818,17 +969,20 @@ thp_status thp_fphi_uind(thp_sys* s, const double* qgrid, double* phi1, double* 
 // ============================================================ PCG driver
 namespace thp {
 
-// Enqueue one PCG iteration (newinduce_pme2.f:442-569) on the context stream.
-thp_status enqueue_iteration(thp_sys* s) {
+int upd_threads(const thp_sys*) { return 256; }
+
+// Enqueue PCG iteration `it` (1-based; newinduce_pme2.f:442-569).
+thp_status enqueue_iteration(thp_sys* s, int it) {
   RET(pair_tmatxb(s, 1 /*pp*/, 3 /*hreal*/, 1, 1));
   RET(spread_uind(s, 1, 0, 1));
   RET(convolve(s, 1));
   RET(launch_gather_uind<2>(s, 1, 3, 0));
   hipStream_t st = s->ctx->stream;
-  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_cg_update1<double>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
-  else hipLaunchKernelGGL((k_cg_update1<float>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
+  const int G = s->view.red_blocks, T = upd_threads(s);
+  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_cg_update1<double>), g1(G), dim3(T), 0, st, s->d_view, s->d_redwork, it);
+  else hipLaunchKernelGGL((k_cg_update1<float>), g1(G), dim3(T), 0, st, s->d_view, s->d_redwork, it);
   RET(launch_check(s->ctx, "k_cg_update1"));
-  hipLaunchKernelGGL(k_cg_update2, g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
+  hipLaunchKernelGGL(k_cg_update2, g1(G), dim3(T), 0, st, s->d_view, s->d_redwork, it);
   return launch_check(s->ctx, "k_cg_update2");
 }
 
@@ -841,17 +995,16 @@ thp_status enqueue_setup(thp_sys* s, const thp_solve_opts* o, bool warm) {
   init.politer = o->politer;
   HIPCHK(s->ctx, hipMemcpyAsync(s->view.st, &init, sizeof init, hipMemcpyHostToDevice, st));
   RET(ensure_lists(s));
+  RET(ensure_pme(s));
   RET(field(s, nullptr, 1, 1));
   if (!warm) {
     hipLaunchKernelGGL(k_guess, g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
     RET(launch_check(s->ctx, "k_guess"));
   }
   RET(pair_tmatxb(s, 0 /*mu*/, 3 /*hreal*/, 1, 0));
-  RET(zero_grid(s));
   RET(spread_uind(s, 0, 0, 0));
   RET(convolve(s, 0));
-  RET(launch_gather_uind<1>(s, 0, 3, 0));
-  return zero_grid(s);
+  return launch_gather_uind<1>(s, 0, 3, 0);
 }
 
 thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_stats* stats) {
@@ -860,16 +1013,18 @@ thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_s
   RET(enqueue_setup(s, o, warm));
   HIPCHK(s->ctx, hipEventRecord(s->ev[1], st));
   int conv = 0;
-  for (int it = 0; it < o->politer; ++it) {
-    RET(enqueue_iteration(s));
+  for (int it = 1; it <= o->politer; ++it) {
+    RET(enqueue_iteration(s, it));
     // the reference reads the residual norm back on the host every
     // iteration (newinduce_pme2gpu.f:1040-1041)
     HIPCHK(s->ctx, hipMemcpyAsync(&conv, &s->view.st->conv, sizeof(int), hipMemcpyDeviceToHost, st));
     RET(sync(s));
     if (conv) break;
   }
-  hipLaunchKernelGGL(k_energy, g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
+  hipLaunchKernelGGL(k_energy, g1(s->view.red_blocks), dim3(upd_threads(s)), 0, st, s->d_view, s->d_redwork);
   RET(launch_check(s->ctx, "k_energy"));
+  hipLaunchKernelGGL(k_finish, g1(1), dim3(64), 0, st, s->d_view, (const int*)s->d_sysid);
+  RET(launch_check(s->ctx, "k_finish"));
   HIPCHK(s->ctx, hipEventRecord(s->ev[2], st));
   SolveState fin;
   HIPCHK(s->ctx, hipMemcpyAsync(&fin, s->view.st, sizeof fin, hipMemcpyDeviceToHost, st));
@@ -884,7 +1039,7 @@ thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_s
     stats->n_indefinite = fin.n_indef;
     stats->t_field_ms = t1;
     stats->t_solve_ms = t2;
-    stats->status = fin.conv == kConverged ? THP_OK : fin.conv == kNonFinite ? THP_ERR_NAN : fin.conv == kZeroCurv ? THP_OK : THP_ERR_NOCONV;
+    stats->status = fin.conv == kConverged || fin.conv == kZeroCurv ? THP_OK : fin.conv == kNonFinite ? THP_ERR_NAN : THP_ERR_NOCONV;
   }
   return THP_OK;
 }

@@ -16,8 +16,7 @@ constexpr int kBlock = 64;  // atoms per block = one wavefront
 constexpr int kOrder = 5;   // PME B-spline order handled by the kernels
 constexpr double kCoulomb = 332.063714;  // v1.2/source/MOD_units.f:81
 constexpr double kTinyPol = 1.0e-5;      // v1.2/source/MOD_polar.f:36
-constexpr int kCodeShift = 24;           // tight-list entry = k | code << 24
-constexpr int kMaxAtoms = 1 << kCodeShift;
+constexpr int kMaxAtoms = 1 << 24;
 
 // PCG termination codes held in SolveState::conv
 enum : int { kRunning = 0, kConverged = 1, kMaxIter = 2, kNonFinite = 3, kZeroCurv = 4 };
@@ -35,16 +34,18 @@ struct SolveState {
   int politer;
   int conv;
   int n_indef;
-  unsigned int counter[4];  // last-block detection, one per reducing kernel
 };
 
 // Everything a kernel needs about one system; lives in device memory so a
 // batched launch can reach many systems through one array.
 struct SysView {
   int n, npad, nblk;
+  int red_blocks;  // blocks per system in the PCG vector kernels (partials)
+  int gat_blocks;  // blocks per system in the PME gather kernels (partials)
   int prec;
   double box[3];
   double aewald, cut2, mbuf2, term_self;
+  double ulim2;       // (cut + skin)^2 of the block-union lists
   double afrac[3];  // nfft_k / box_k (orthorhombic fractional scaling)
   int nfft[3];
   int ncode;
@@ -66,11 +67,21 @@ struct SysView {
   int maxnb;
   int* nb;
   int* nbcount;
-  // tight list (cut), full: tnb[s*npad + i] = k | code << 24
-  int* tnb;
-  int* tcount;
   double* bbox;       // [nblk][6] block bounding boxes (centre, half extent)
+  // block-union lists (cut, unscaled pairs)
+  int umax, lmax;
+  int* ulist;              // [nblk][umax] union atoms of each block
+  int* ucount;             // [nblk]
+  unsigned short* llist;   // [lmax][npad] local partner indices into the block's union
+  int* lcount;             // [npad]
   // PME
+  int brick[3];       // spreading brick edge per axis (4..8 points)
+  int nbrick[3];
+  void* theta;        // [npad][48] B-spline values per atom (real type)
+  int* g0;            // [npad][4] first grid node per axis + home brick
+  int* bin_count;     // [nbricks]
+  int* bin_start;     // [nbricks + 1]
+  int* bin_list;      // [n] atoms grouped by home brick, sorted
   void* grid;         // complex interleaved, x fastest, real type per prec
   const void* qfac;   // K1*K2*K3 real
   // solver vectors [6][npad] FP64 (component-major)
@@ -81,7 +92,7 @@ struct SysView {
   double* hreal;
   double* pp;
   double* zr;
-  double* partial;  // [4][nblk][2]
+  double* partial;  // partial sums, see part_* in the kernels
   SolveState* st;
   int* err;         // device error flag (list overflow ...)
 };
@@ -109,15 +120,21 @@ struct thp_sys {
   double aewald = 0, cut = 0, buffer = 0;
   int nfft[3] = {0, 0, 0};
   int order = 5;
-  int segs = 4;  // waves per 64-atom block in the pair kernels
+  int segs = 8;  // waves per 64-atom block in the pair kernels (1,2,4,8,16)
   std::vector<int> perm;   // sorted slot -> original index
   std::vector<int> iperm;  // original index -> sorted slot
   int* iperm_dev = nullptr;
   double* cphi = nullptr;  // [10][npad] scratch for efld0_recip
   std::vector<double> hx, hy, hz;       // sorted-order coordinates (host copy)
-  std::vector<double> bx0, by0, bz0;    // coordinates at the last list build
+  std::vector<double> bx0, by0, bz0;    // coordinates at the last block-list build
+  std::vector<double> ux0, uy0, uz0;    // coordinates at the last block-union build
+  double skin = 0.5;                    // skin of the block-union lists
   bool list_valid = false;
-  bool tight_valid = false;
+  bool tight_valid = false;   // block-union lists match the coordinates
+  bool verlet_valid = false;  // per-atom Verlet list matches the coordinates
+  bool pme_valid = false;   // B-spline values and brick bins match the coordinates
+  int nbricks = 0;
+  thp::Work* d_brickwork = nullptr;  // [nbricks]
   int maxnb = 0;
   // device allocations
   std::vector<void*> allocs;
@@ -125,6 +142,9 @@ struct thp_sys {
   thp::SysView view;           // host copy of the view
   thp::SysView* d_view = nullptr;
   thp::Work* d_work = nullptr;  // single-system work table [nblk]
+  thp::Work* d_redwork = nullptr;  // vector-kernel work table [red_blocks]
+  thp::Work* d_gatwork = nullptr;  // gather work table [gat_blocks]
+  int* d_sysid = nullptr;          // {0}
   // block neighbor lists
   double* bbox = nullptr;  // [nblk][6] centre + half-extent
   int* bcount = nullptr;   // [nblk]

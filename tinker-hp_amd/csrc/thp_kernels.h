@@ -22,6 +22,20 @@
 namespace thp {
 
 // ------------------------------------------------------------------ utils
+// Arrays reached through a SysView are generic pointers; casting them to the
+// global address space lets the compiler emit global_load/global_store
+// instead of flat accesses.  Hot kernels copy the SysView fields they need
+// into locals once, so loop bodies hold no reloads of the view.
+#define GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const GLOBAL T* gp(const T* p) {
+  return (const GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ GLOBAL T* gw(T* p) {
+  return (GLOBAL T*)p;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -44,12 +58,19 @@ __device__ __forceinline__ double wave_min(double v) {
 __device__ __forceinline__ double image_exact(double xr, double cell) {
 #pragma clang fp contract(off)
   const double cell2 = 0.5 * cell;
-  if (fabs(xr) > cell2) {
-    const double cel = copysign(cell, xr);
-    const double fl = floor(fabs(xr) / cell);
-    const double prod = cel * fl;
-    xr = xr - prod;
-    if (fabs(xr) > cell2) xr = xr - copysign(cell, xr);
+  const double ax = fabs(xr);
+  if (ax > cell2) {
+    if (__builtin_expect(ax < 0.9999 * cell, 1)) {
+      // floor(|xr|/cell) is exactly 0 here, so the reference's first step
+      // leaves xr unchanged and only the second shift applies
+      xr = xr - copysign(cell, xr);
+    } else {
+      const double cel = copysign(cell, xr);
+      const double fl = floor(ax / cell);
+      const double prod = cel * fl;
+      xr = xr - prod;
+      if (fabs(xr) > cell2) xr = xr - copysign(cell, xr);
+    }
   }
   return xr;
 }
@@ -87,29 +108,64 @@ template <typename T> __device__ __forceinline__ T sqrt_t(T v);
 template <> __device__ __forceinline__ float sqrt_t(float v) { return sqrtf(v); }
 template <> __device__ __forceinline__ double sqrt_t(double v) { return sqrt(v); }
 
-// last-block detection: returns true in exactly one block per (kernel,
-// system) after every block of that system has published its partials.
-__device__ __forceinline__ bool last_block(unsigned int* counter, int nblocks) {
-  __shared__ int is_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned int prev = atomicAdd(counter, 1u);
-    is_last = (prev == (unsigned int)(nblocks - 1));
-  }
-  __syncthreads();
-  if (is_last) {
-    __threadfence();
-    if (threadIdx.x == 0) *counter = 0u;
-  }
-  return is_last != 0;
-}
-
 // fixed-order sum of partial[q][0..nblk)[r] by one 64-lane wave
 __device__ __forceinline__ double sum_partials(const double* part, int nblk, int r) {
   double acc = 0.0;
   for (int b = threadIdx.x; b < nblk; b += 64) acc += part[b * 2 + r];
   return wave_sum(acc);
+}
+
+// fixed-order sum of partial[0..count)[r] by a whole workgroup (any
+// multiple of 64 threads); every thread receives the sum
+__device__ __forceinline__ double block_sum_partials(const double* part, int count, int r, double* scratch) {
+  const GLOBAL double* P = gp(part);
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < count; b += blockDim.x) acc += P[b * 2 + r];
+  acc = wave_sum(acc);
+  const int nw = blockDim.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  double s = 0.0;
+  for (int q = 0; q < nw; ++q) s += scratch[q];
+  return s;
+}
+
+// partial-sum buffers: gather kernels write gat_blocks pairs (gg, init),
+// vector kernels write red_blocks pairs (rz ping-pong, ene, ep)
+__device__ __forceinline__ double* part_gg(const SysView& sv) { return sv.partial; }
+__device__ __forceinline__ double* part_init(const SysView& sv) { return sv.partial + (size_t)2 * sv.gat_blocks; }
+__device__ __forceinline__ double* part_rz(const SysView& sv, int parity) {
+  return sv.partial + (size_t)4 * sv.gat_blocks + (size_t)2 * sv.red_blocks * parity;
+}
+__device__ __forceinline__ double* part_ene(const SysView& sv) {
+  return sv.partial + (size_t)4 * sv.gat_blocks + (size_t)4 * sv.red_blocks;
+}
+__device__ __forceinline__ double* part_ep(const SysView& sv) {
+  return sv.partial + (size_t)4 * sv.gat_blocks + (size_t)6 * sv.red_blocks;
+}
+inline size_t partial_doubles(int gat_blocks, int red_blocks) { return (size_t)4 * gat_blocks + (size_t)8 * red_blocks; }
+
+// sum of two values over a 256-thread block, written as partial `slot`
+__device__ __forceinline__ void block_partial2(double a, double b, double* part, int slot) {
+  __shared__ double red[2][4];
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a;
+    red[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    double s0 = 0.0, s1 = 0.0;
+    for (int q = 0; q < nw; ++q) {
+      s0 += red[0][q];
+      s1 += red[1][q];
+    }
+    part[slot * 2] = s0;
+    part[slot * 2 + 1] = s1;
+  }
 }
 
 // ===================================================== neighbor lists
@@ -206,72 +262,48 @@ __global__ void __launch_bounds__(64) k_nblist_atoms(const SysView* views, const
                                                       const int* bcount, const int* blist, int maxbl) {
   const Work w = work[blockIdx.x];
   const SysView& sv = views[w.sys];
+  const int n = sv.n, maxnb = sv.maxnb;
+  const size_t np = sv.npad;
+  const double bx = sv.box[0], by = sv.box[1], bz = sv.box[2], mbuf2 = sv.mbuf2;
+  const GLOBAL double* X = gp(sv.x);
+  const GLOBAL double* Y = gp(sv.y);
+  const GLOBAL double* Z = gp(sv.z);
+  GLOBAL int* NB = gw(sv.nb);
+  const GLOBAL int* BL = gp(blist) + (size_t)w.blk * maxbl;
   const int lane = threadIdx.x;
   const int i = w.blk * kBlock + lane;
-  const bool ok = i < sv.n;
+  const bool ok = i < n;
   __shared__ double sx[kBlock], sy[kBlock], sz[kBlock];
-  const double xi = ok ? sv.x[i] : 0.0, yi = ok ? sv.y[i] : 0.0, zi = ok ? sv.z[i] : 0.0;
+  const double xi = ok ? X[i] : 0.0, yi = ok ? Y[i] : 0.0, zi = ok ? Z[i] : 0.0;
   int cnt = 0;
-  const int nbl = bcount[w.blk];
+  bool overflow = false;
+  const int nbl = gp(bcount)[w.blk];
   for (int q = 0; q < nbl; ++q) {
-    const int J = blist[(size_t)w.blk * maxbl + q];
-    const int k0 = J * kBlock;
+    const int k0 = BL[q] * kBlock;
     const int kk = k0 + lane;
     __syncthreads();
-    sx[lane] = kk < sv.n ? sv.x[kk] : 0.0;
-    sy[lane] = kk < sv.n ? sv.y[kk] : 0.0;
-    sz[lane] = kk < sv.n ? sv.z[kk] : 0.0;
+    sx[lane] = kk < n ? X[kk] : 0.0;
+    sy[lane] = kk < n ? Y[kk] : 0.0;
+    sz[lane] = kk < n ? Z[kk] : 0.0;
     __syncthreads();
     if (!ok) continue;
-    const int mmax = min(kBlock, sv.n - k0);
+    const int mmax = min(kBlock, n - k0);
     for (int m = 0; m < mmax; ++m) {
       const int k = k0 + m;
-      if (k == i) continue;
-      const double xr = image_exact(xi - sx[m], sv.box[0]);
-      const double yr = image_exact(yi - sy[m], sv.box[1]);
-      const double zr = image_exact(zi - sz[m], sv.box[2]);
-      if (r2_exact(xr, yr, zr) <= sv.mbuf2) {
+      const double xr = image_exact(xi - sx[m], bx);
+      const double yr = image_exact(yi - sy[m], by);
+      const double zr = image_exact(zi - sz[m], bz);
+      if (r2_exact(xr, yr, zr) <= mbuf2 && k != i) {
         if (FILL) {
-          if (cnt < sv.maxnb) sv.nb[(size_t)cnt * sv.npad + i] = k;
-          else *sv.err = THP_ERR_LIST;
+          if (cnt < maxnb) NB[(size_t)cnt * np + i] = k;
+          else overflow = true;
         }
         ++cnt;
       }
     }
   }
-  if (ok) sv.nbcount[i] = cnt;
-}
-
-// Tight list: entries of the Verlet list with r^2 <= cut^2 (the test of
-// efld0_direct / tmatxb_pme, newinduce_pme.f:1160 and :1516), tagged with
-// the code of the pair's (d,p,u) scale triple (0 = all ones).  One thread
-// per atom.
-__global__ void __launch_bounds__(64) k_tight(const SysView* views, const Work* work) {
-  const Work w = work[blockIdx.x];
-  const SysView& sv = views[w.sys];
-  const int i = w.blk * kBlock + threadIdx.x;
-  if (i >= sv.n) return;
-  const double xi = sv.x[i], yi = sv.y[i], zi = sv.z[i];
-  const int cnt = sv.nbcount[i];
-  const int e0 = sv.excl_off[i], e1 = sv.excl_off[i + 1];
-  int t = 0;
-  for (int s = 0; s < cnt; ++s) {
-    const int k = sv.nb[(size_t)s * sv.npad + i];
-    const double xr = image_exact(sv.x[k] - xi, sv.box[0]);
-    const double yr = image_exact(sv.y[k] - yi, sv.box[1]);
-    const double zr = image_exact(sv.z[k] - zi, sv.box[2]);
-    if (r2_exact(xr, yr, zr) <= sv.cut2) {
-      int code = 0;
-      for (int e = e0; e < e1; ++e)
-        if (sv.excl_k[e] == k) {
-          code = sv.excl_code[e];
-          break;
-        }
-      sv.tnb[(size_t)t * sv.npad + i] = k | (code << kCodeShift);
-      ++t;
-    }
-  }
-  sv.tcount[i] = t;
+  if (ok) gw(sv.nbcount)[i] = cnt;
+  if (FILL && overflow) *gw(sv.err) = THP_ERR_LIST;
 }
 
 // ===================================================== pair kernels
@@ -313,166 +345,391 @@ __device__ __forceinline__ void thole_l(real r, real pdi, real pdk, real thi, re
   }
 }
 
-// displacement x_k - x_i evaluated in FP64 (minimum image), then rounded
-// to the pair arithmetic type
+// Field at i due to the permanent multipoles of k (fid and fip of
+// efld0_direct, newinduce_pme.f:1116-1264), added to acc[0..5].
 template <typename real>
-__device__ __forceinline__ void disp(const SysView& sv, int k, double xi, double yi, double zi,
-                                     real& xr, real& yr, real& zr) {
-  double dx = sv.x[k] - xi, dy = sv.y[k] - yi, dz = sv.z[k] - zi;
-  dx = image_exact(dx, sv.box[0]);
-  dy = image_exact(dy, sv.box[1]);
-  dz = image_exact(dz, sv.box[2]);
-  xr = real(dx);
-  yr = real(dy);
-  zr = real(dz);
+__device__ __forceinline__ void efld0_pair(const SysView& sv, int k, real xr, real yr, real zr, real pdi,
+                                           real thi, real dsc, real psc, double* acc) {
+  const size_t np = sv.npad;
+  const real r2 = xr * xr + yr * yr + zr * zr;
+  const real rr1 = rsqrt_t(r2);
+  const real r = r2 * rr1;
+  const real rr2 = rr1 * rr1;
+  const real rr3 = rr2 * rr1;
+  const real rr5 = real(3) * rr2 * rr3;
+  const real rr7 = real(5) * rr2 * rr5;
+  const real ck = real(sv.rpole[0 * np + k]);
+  const real dkx = real(sv.rpole[1 * np + k]), dky = real(sv.rpole[2 * np + k]), dkz = real(sv.rpole[3 * np + k]);
+  const real qkxx = real(sv.rpole[4 * np + k]), qkxy = real(sv.rpole[5 * np + k]);
+  const real qkxz = real(sv.rpole[6 * np + k]), qkyy = real(sv.rpole[7 * np + k]);
+  const real qkyz = real(sv.rpole[8 * np + k]), qkzz = real(sv.rpole[9 * np + k]);
+  const real dkr = dkx * xr + dky * yr + dkz * zr;
+  const real qkx = qkxx * xr + qkxy * yr + qkxz * zr;
+  const real qky = qkxy * xr + qkyy * yr + qkyz * zr;
+  const real qkz = qkxz * xr + qkyz * yr + qkzz * zr;
+  const real qkr = qkx * xr + qky * yr + qkz * zr;
+  real bn[3];
+  ewald_bn<real, 3>(r, r2, real(sv.aewald), rr1, bn);
+  real l[3];
+  thole_l<real, 1>(r, pdi, real(sv.pdamp[k]), thi, real(sv.thole[k]), l);
+#pragma unroll
+  for (int pass = 0; pass < 2; ++pass) {
+    const real sc = pass == 0 ? dsc : psc;
+    const real dmp3 = bn[0] - (real(1) - sc * l[0]) * rr3;
+    const real dmp5 = bn[1] - (real(1) - sc * l[1]) * rr5;
+    const real dmp7 = bn[2] - (real(1) - sc * l[2]) * rr7;
+    const real ai = dmp3 * ck - dmp5 * dkr + dmp7 * qkr;
+    const real d5 = real(2) * dmp5;
+    acc[pass * 3 + 0] += double(-xr * ai - dmp3 * dkx + d5 * qkx);
+    acc[pass * 3 + 1] += double(-yr * ai - dmp3 * dky + d5 * qky);
+    acc[pass * 3 + 2] += double(-zr * ai - dmp3 * dkz + d5 * qkz);
+  }
 }
 
-// efld0_direct: permanent field (d and p) at each atom from its tight
-// list.  newinduce_pme.f:1116-1264; field at i due to k (fid/fip).
-template <typename real, int S>
-__global__ void __launch_bounds__(64 * S) k_efld0(const SysView* views, const Work* work) {
+// -E at i due to the dipoles u_k of both rhs (efi -= dlocal . u_k of
+// tmatxb_pme, newinduce_pme.f:1548-1608), added to acc[0..5].
+template <typename real>
+__device__ __forceinline__ void tmatxb_pair(const SysView& sv, const double* in, int k, real xr, real yr, real zr,
+                                            real pdi, real thi, real usc, double* acc) {
+  const size_t np = sv.npad;
+  const real r2 = xr * xr + yr * yr + zr * zr;
+  const real rr1 = rsqrt_t(r2);
+  const real r = r2 * rr1;
+  const real rr2 = rr1 * rr1;
+  const real rr3 = rr2 * rr1;
+  const real rr5 = real(3) * rr2 * rr3;
+  real bn[2];
+  ewald_bn<real, 2>(r, r2, real(sv.aewald), rr1, bn);
+  real l[3];
+  thole_l<real, 0>(r, pdi, real(sv.pdamp[k]), thi, real(sv.thole[k]), l);
+  const real dmp3 = bn[0] - (real(1) - usc * l[0]) * rr3;
+  const real dmp5 = bn[1] - (real(1) - usc * l[1]) * rr5;
+#pragma unroll
+  for (int rhs = 0; rhs < 2; ++rhs) {
+    const real ux = real(in[(size_t)(rhs * 3 + 0) * np + k]);
+    const real uy = real(in[(size_t)(rhs * 3 + 1) * np + k]);
+    const real uz = real(in[(size_t)(rhs * 3 + 2) * np + k]);
+    const real ur = dmp5 * (xr * ux + yr * uy + zr * uz);
+    acc[rhs * 3 + 0] += double(dmp3 * ux - ur * xr);
+    acc[rhs * 3 + 1] += double(dmp3 * uy - ur * yr);
+    acc[rhs * 3 + 2] += double(dmp3 * uz - ur * zr);
+  }
+}
+
+// Scaled (excluded) pairs of atom i, evaluated in FP64 in every precision
+// mode: their bn - (1 - s*lambda)/r^n terms cancel strongly at bonded
+// distances.  Mirrors the reference's separate correction pass over
+// scaled pairs (GPU/source/tmatxb_pmecu.f:568-706, tmatxb_pmegpu.f:1435-1545).
+template <int KIND>
+__device__ __forceinline__ void scaled_pairs(const SysView& sv, const double* in, int i, double xi, double yi,
+                                             double zi, double* acc) {
+  const int e0 = sv.excl_off[i], e1 = sv.excl_off[i + 1];
+  const double pdi = sv.pdamp[i], thi = sv.thole[i];
+  for (int e = e0; e < e1; ++e) {
+    const int k = sv.excl_k[e];
+    const double xr = image_exact(sv.x[k] - xi, sv.box[0]);
+    const double yr = image_exact(sv.y[k] - yi, sv.box[1]);
+    const double zr = image_exact(sv.z[k] - zi, sv.box[2]);
+    if (!(r2_exact(xr, yr, zr) <= sv.cut2)) continue;
+    const double* sc = sv.scale_tab + sv.excl_code[e] * 3;
+    if (KIND == 0) efld0_pair<double>(sv, k, xr, yr, zr, pdi, thi, sc[0], sc[1], acc);
+    else tmatxb_pair<double>(sv, in, k, xr, yr, zr, pdi, thi, sc[2], acc);
+  }
+}
+
+// ===================================================== block-union lists
+// For each 64-atom block, the union U of all atoms within cut + skin of at
+// least one atom of the block (unscaled pairs), in increasing order, and
+// for each atom its partners as 16-bit indices into U.  The pair kernels
+// apply the exact cutoff test r^2 <= cut^2 (newinduce_pme.f:1160) to every
+// entry, so the lists stay valid while no atom moves more than skin/2.
+// Built from the block candidate list by 4 wavefronts: each tests the
+// block's 64 atoms against a quarter of the candidate blocks (staged in
+// LDS) and stores its lanes' 64-bit hit masks in LDS; wavefront 0 then walks
+// the candidates in order, ORs the masks across lanes and assigns local
+// indices by popcount.  Overflow of the preallocated capacities is
+// reported through the error flag (the host then grows them and rebuilds).
+__device__ __forceinline__ unsigned long long wave_or64(unsigned long long m) {
+  unsigned lo = (unsigned)m, hi = (unsigned)(m >> 32);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    lo |= __shfl_xor(lo, off, 64);
+    hi |= __shfl_xor(hi, off, 64);
+  }
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+constexpr int kUnionWaves = 4;
+constexpr int kUnionCand = 96;  // candidate blocks per LDS pass
+
+__global__ void __launch_bounds__(64 * kUnionWaves) k_union(const SysView* views, const Work* work,
+                                                            const int* bcount, const int* blist, int maxbl) {
   const Work w = work[blockIdx.x];
   const SysView& sv = views[w.sys];
+  const int n = sv.n, umax = sv.umax, lmax = sv.lmax;
+  const size_t np = sv.npad;
+  const double bx = sv.box[0], by = sv.box[1], bz = sv.box[2], lim2 = sv.ulim2;
+  const GLOBAL double* X = gp(sv.x);
+  const GLOBAL double* Y = gp(sv.y);
+  const GLOBAL double* Z = gp(sv.z);
+  const GLOBAL double* BB = gp(sv.bbox);
+  const GLOBAL int* BL = gp(blist) + (size_t)w.blk * maxbl;
+  GLOBAL int* UL = gw(sv.ulist) + (size_t)w.blk * umax;
+  GLOBAL unsigned short* LL = gw(sv.llist);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int i = w.blk * kBlock + lane;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  if (i < sv.n) {
-    const double xi = sv.x[i], yi = sv.y[i], zi = sv.z[i];
-    const real pdi = real(sv.pdamp[i]), thi = real(sv.thole[i]);
-    const real aew = real(sv.aewald);
-    const int cnt = sv.tcount[i];
-    const int s0 = (cnt * wv) / S, s1 = (cnt * (wv + 1)) / S;
-    const size_t np = sv.npad;
-    for (int s = s0; s < s1; ++s) {
-      const int e = sv.tnb[(size_t)s * np + i];
-      const int k = e & (kMaxAtoms - 1);
-      const int code = e >> kCodeShift;
-      real xr, yr, zr;
-      disp<real>(sv, k, xi, yi, zi, xr, yr, zr);
-      const real r2 = xr * xr + yr * yr + zr * zr;
-      const real rr1 = rsqrt_t(r2);
-      const real r = r2 * rr1;
-      const real rr2 = rr1 * rr1;
-      const real rr3 = rr2 * rr1;
-      const real rr5 = real(3) * rr2 * rr3;
-      const real rr7 = real(5) * rr2 * rr5;
-      const real ck = real(sv.rpole[0 * np + k]);
-      const real dkx = real(sv.rpole[1 * np + k]), dky = real(sv.rpole[2 * np + k]), dkz = real(sv.rpole[3 * np + k]);
-      const real qkxx = real(sv.rpole[4 * np + k]), qkxy = real(sv.rpole[5 * np + k]);
-      const real qkxz = real(sv.rpole[6 * np + k]), qkyy = real(sv.rpole[7 * np + k]);
-      const real qkyz = real(sv.rpole[8 * np + k]), qkzz = real(sv.rpole[9 * np + k]);
-      const real dkr = dkx * xr + dky * yr + dkz * zr;
-      const real qkx = qkxx * xr + qkxy * yr + qkxz * zr;
-      const real qky = qkxy * xr + qkyy * yr + qkyz * zr;
-      const real qkz = qkxz * xr + qkyz * yr + qkzz * zr;
-      const real qkr = qkx * xr + qky * yr + qkz * zr;
-      real bn[3];
-      ewald_bn<real, 3>(r, r2, aew, rr1, bn);
-      real l[3];
-      thole_l<real, 1>(r, pdi, real(sv.pdamp[k]), thi, real(sv.thole[k]), l);
-      real dsc = real(1), psc = real(1);
-      if (code) {
-        dsc = real(sv.scale_tab[code * 3 + 0]);
-        psc = real(sv.scale_tab[code * 3 + 1]);
-      }
+  const bool ok = i < n;
+  __shared__ double sx[kUnionWaves][kBlock], sy[kUnionWaves][kBlock], sz[kUnionWaves][kBlock];
+  __shared__ unsigned long long smask[kUnionCand][kBlock];
+  __shared__ int s_k0[kUnionCand];
+  const double xi = ok ? X[i] : 0.0, yi = ok ? Y[i] : 0.0, zi = ok ? Z[i] : 0.0;
+  const int e0 = ok ? gp(sv.excl_off)[i] : 0, e1 = ok ? gp(sv.excl_off)[i + 1] : 0;
+  int ex[8];
 #pragma unroll
-      for (int pass = 0; pass < 2; ++pass) {
-        const real sc = pass == 0 ? dsc : psc;
-        const real dmp3 = bn[0] - (real(1) - sc * l[0]) * rr3;
-        const real dmp5 = bn[1] - (real(1) - sc * l[1]) * rr5;
-        const real dmp7 = bn[2] - (real(1) - sc * l[2]) * rr7;
-        const real ai = dmp3 * ck - dmp5 * dkr + dmp7 * qkr;
-        const real d5 = real(2) * dmp5;
-        acc[pass * 3 + 0] += double(-xr * ai - dmp3 * dkx + d5 * qkx);
-        acc[pass * 3 + 1] += double(-yr * ai - dmp3 * dky + d5 * qky);
-        acc[pass * 3 + 2] += double(-zr * ai - dmp3 * dkz + d5 * qkz);
+  for (int e = 0; e < 8; ++e) ex[e] = e0 + e < e1 ? gp(sv.excl_k)[e0 + e] : -1;
+  // this block's current bounding box: candidates farther than cut + skin
+  // are skipped
+  const double cxI = BB[w.blk * 6], cyI = BB[w.blk * 6 + 1], czI = BB[w.blk * 6 + 2];
+  const double hxI = BB[w.blk * 6 + 3], hyI = BB[w.blk * 6 + 4], hzI = BB[w.blk * 6 + 5];
+  int ucount = 0, lcount = 0;
+  bool overflow = false;
+  const int nbl = gp(bcount)[w.blk];
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int p0 = 0; p0 < nbl; p0 += kUnionCand) {
+    const int pn = min(kUnionCand, nbl - p0);
+    // phase 1: masks of candidates p0+wv, p0+wv+4, ...
+    for (int q = wv; q < pn; q += kUnionWaves) {
+      const int J = BL[p0 + q];
+      const int k0 = J * kBlock;
+      unsigned long long mask = 0ull;
+      double gx = BB[J * 6] - cxI, gy = BB[J * 6 + 1] - cyI, gz = BB[J * 6 + 2] - czI;
+      gx -= bx * rint(gx / bx);
+      gy -= by * rint(gy / by);
+      gz -= bz * rint(gz / bz);
+      gx = fmax(0.0, fabs(gx) - hxI - BB[J * 6 + 3]);
+      gy = fmax(0.0, fabs(gy) - hyI - BB[J * 6 + 4]);
+      gz = fmax(0.0, fabs(gz) - hzI - BB[J * 6 + 5]);
+      const bool near = gx * gx + gy * gy + gz * gz <= lim2 * (1.0 + 1e-9) + 1e-9;
+      if (near) {
+        const int kk = k0 + lane;
+        sx[wv][lane] = kk < n ? X[kk] : 0.0;
+        sy[wv][lane] = kk < n ? Y[kk] : 0.0;
+        sz[wv][lane] = kk < n ? Z[kk] : 0.0;
+        // a single wavefront writes and reads its own staging row
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        if (ok) {
+          const int mmax = min(kBlock, n - k0);
+          for (int m = 0; m < mmax; ++m) {
+            const double xr = image_exact(sx[wv][m] - xi, bx);
+            const double yr = image_exact(sy[wv][m] - yi, by);
+            const double zr = image_exact(sz[wv][m] - zi, bz);
+            if (r2_exact(xr, yr, zr) <= lim2) mask |= 1ull << m;
+          }
+          if (k0 <= i && i < k0 + kBlock) mask &= ~(1ull << (i - k0));
+#pragma unroll
+          for (int e = 0; e < 8; ++e)
+            if (ex[e] >= k0 && ex[e] < k0 + kBlock) mask &= ~(1ull << (ex[e] - k0));
+          for (int e = e0 + 8; e < e1; ++e) {
+            const int kx = gp(sv.excl_k)[e];
+            if (kx >= k0 && kx < k0 + kBlock) mask &= ~(1ull << (kx - k0));
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+      smask[q][lane] = mask;
+      if (lane == 0) s_k0[q] = k0;
+    }
+    __syncthreads();
+    // phase 2: wavefront 0 assigns union slots and local indices in order
+    if (wv == 0) {
+      for (int q = 0; q < pn; ++q) {
+        const unsigned long long mask = smask[q][lane];
+        const unsigned long long U = wave_or64(mask);
+        if (U == 0ull) continue;
+        if ((U >> lane) & 1ull) {
+          const int pos = ucount + __popcll(U & below);
+          if (pos < umax) UL[pos] = s_k0[q] + lane;
+          else overflow = true;
+        }
+        unsigned long long mm = mask;
+        while (mm) {
+          const int m = __ffsll((long long)mm) - 1;
+          mm &= mm - 1ull;
+          const int loc = ucount + __popcll(U & ((1ull << m) - 1ull));
+          if (lcount < lmax) LL[(size_t)lcount * np + i] = (unsigned short)loc;
+          else overflow = true;
+          ++lcount;
+        }
+        ucount += __popcll(U);
       }
     }
-  }
-  __shared__ double red[S > 1 ? S : 1][6][64];
-  if (S > 1) {
-#pragma unroll
-    for (int c = 0; c < 6; ++c) red[wv][c][lane] = acc[c];
     __syncthreads();
   }
-  if (wv == 0 && i < sv.n) {
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      double v = acc[c];
-      for (int q = 1; q < S; ++q) v += red[q][c][lane];
-      sv.ef[(size_t)c * sv.npad + i] = v;
-    }
+  if (wv == 0) {
+    if (ok) gw(sv.lcount)[i] = lcount;
+    if (lane == 0) gw(sv.ucount)[w.blk] = ucount;
+    if (overflow) *gw(sv.err) = THP_ERR_LIST;
   }
 }
 
-// tmatxb_pme: out = diag(1/alpha) in - E_real(in), the Thole-damped Ewald
-// real-space dipole field over the tight list (newinduce_pme.f:1516-1661).
-template <typename real, int S>
-__global__ void __launch_bounds__(64 * S) k_tmatxb(const SysView* views, const Work* work, int vin,
-                                                    int vout, int dodiag, int check_conv) {
+// Pair kernels over the block-union lists.  A workgroup of S wavefronts
+// owns one 64-atom block; the union atoms' data are staged in LDS in
+// chunks, each lane walks its local list (split in S contiguous segments,
+// one per wavefront), and the S partial fields are summed in LDS in a fixed
+// order.  KIND 0: efld0_direct (permanent field, d and p); KIND 1:
+// tmatxb_pme (-E_real of the dipoles vin, plus diag(1/alpha) vin).
+template <typename real, int KIND>
+struct UnionStage {
+  // positions are kept in FP64 so that displacements are exact before
+  // rounding to the pair type
+  static constexpr int NM = KIND == 0 ? 10 : 6;                 // moments per atom
+  static constexpr int BYTES = 24 + NM * (int)sizeof(real) + 2 * (int)sizeof(real);
+  static constexpr int CHUNK = (32 * 1024) / BYTES;              // atoms per LDS chunk
+};
+
+template <typename real, int KIND, int S>
+__global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, const Work* work, int vin, int vout,
+                                                        int dodiag, int check_conv) {
+  using ST = UnionStage<real, KIND>;
+  constexpr int C = ST::CHUNK;
+  constexpr int NM = ST::NM;
+  constexpr int NB = 8;  // list entries fetched per batch
+  __shared__ double s_pos[3][C];
+  __shared__ real s_m[NM][C];
+  __shared__ real s_pd[C], s_th[C];
+  __shared__ double red[S > 1 ? S * 6 * 64 : 1];
   const Work w = work[blockIdx.x];
   const SysView& sv = views[w.sys];
   if (check_conv && sv.st->conv) return;
+  const int n = sv.n;
+  const size_t np = sv.npad;
+  const double bx = sv.box[0], by = sv.box[1], bz = sv.box[2], cut2 = sv.cut2;
+  const real aew = real(sv.aewald);
+  const GLOBAL double* X = gp(sv.x);
+  const GLOBAL double* Y = gp(sv.y);
+  const GLOBAL double* Z = gp(sv.z);
+  const GLOBAL double* PD = gp(sv.pdamp);
+  const GLOBAL double* TH = gp(sv.thole);
+  const GLOBAL double* RP = gp(sv.rpole);
+  const double* in_g = vsel(sv, vin);
+  const GLOBAL double* IN = gp(in_g);
+  const GLOBAL unsigned short* LL = gp(sv.llist);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int i = w.blk * kBlock + lane;
-  const double* in = vsel(sv, vin);
+  const bool ok = i < n;
+  const int ucount = gp(sv.ucount)[w.blk];
+  const GLOBAL int* UL = gp(sv.ulist) + (size_t)w.blk * sv.umax;
+  const double xi = ok ? X[i] : 0.0, yi = ok ? Y[i] : 0.0, zi = ok ? Z[i] : 0.0;
+  const real pdi = ok ? real(PD[i]) : real(0), thi = ok ? real(TH[i]) : real(0);
+  const int cnt = ok ? gp(sv.lcount)[i] : 0;
+  const int s_end = (cnt * (wv + 1)) / S;
+  int s = (cnt * wv) / S;
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  const size_t np = sv.npad;
-  if (i < sv.n) {
-    const double xi = sv.x[i], yi = sv.y[i], zi = sv.z[i];
-    const real pdi = real(sv.pdamp[i]), thi = real(sv.thole[i]);
-    const real aew = real(sv.aewald);
-    const int cnt = sv.tcount[i];
-    const int s0 = (cnt * wv) / S, s1 = (cnt * (wv + 1)) / S;
-    for (int s = s0; s < s1; ++s) {
-      const int e = sv.tnb[(size_t)s * np + i];
-      const int k = e & (kMaxAtoms - 1);
-      const int code = e >> kCodeShift;
-      real xr, yr, zr;
-      disp<real>(sv, k, xi, yi, zi, xr, yr, zr);
-      const real r2 = xr * xr + yr * yr + zr * zr;
-      const real rr1 = rsqrt_t(r2);
-      const real r = r2 * rr1;
-      const real rr2 = rr1 * rr1;
-      const real rr3 = rr2 * rr1;
-      const real rr5 = real(3) * rr2 * rr3;
-      real bn[2];
-      ewald_bn<real, 2>(r, r2, aew, rr1, bn);
-      real l[3];
-      thole_l<real, 0>(r, pdi, real(sv.pdamp[k]), thi, real(sv.thole[k]), l);
-      real usc = real(1);
-      if (code) usc = real(sv.scale_tab[code * 3 + 2]);
-      const real dmp3 = bn[0] - (real(1) - usc * l[0]) * rr3;
-      const real dmp5 = bn[1] - (real(1) - usc * l[1]) * rr5;
+  for (int c0 = 0; c0 < ucount; c0 += C) {
+    const int cn = min(C, ucount - c0);
+    __syncthreads();
+    for (int q = threadIdx.x; q < cn; q += 64 * S) {
+      const int kk = UL[c0 + q];
+      s_pos[0][q] = X[kk];
+      s_pos[1][q] = Y[kk];
+      s_pos[2][q] = Z[kk];
+      s_pd[q] = real(PD[kk]);
+      s_th[q] = real(TH[kk]);
 #pragma unroll
-      for (int rhs = 0; rhs < 2; ++rhs) {
-        const real ux = real(in[(size_t)(rhs * 3 + 0) * np + k]);
-        const real uy = real(in[(size_t)(rhs * 3 + 1) * np + k]);
-        const real uz = real(in[(size_t)(rhs * 3 + 2) * np + k]);
-        const real ur = dmp5 * (xr * ux + yr * uy + zr * uz);
-        // efi -= dlocal . u_k with dlocal = -dmp3 I + dmp5 r r^T
-        acc[rhs * 3 + 0] += double(dmp3 * ux - ur * xr);
-        acc[rhs * 3 + 1] += double(dmp3 * uy - ur * yr);
-        acc[rhs * 3 + 2] += double(dmp3 * uz - ur * zr);
+      for (int m = 0; m < NM; ++m) s_m[m][q] = real(KIND == 0 ? RP[(size_t)m * np + kk] : IN[(size_t)m * np + kk]);
+    }
+    __syncthreads();
+    const int c1 = c0 + cn;
+    while (s < s_end) {
+      int loc[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) loc[b] = s + b < s_end ? int(LL[(size_t)(s + b) * np + i]) : 0x7fffffff;
+      bool done = false;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        if (loc[b] >= c1) {  // end of this lane's list or of the staged chunk
+          done = true;
+          break;
+        }
+        ++s;
+        const int q = loc[b] - c0;
+        const double dx = image_exact(s_pos[0][q] - xi, bx);
+        const double dy = image_exact(s_pos[1][q] - yi, by);
+        const double dz = image_exact(s_pos[2][q] - zi, bz);
+        if (!(r2_exact(dx, dy, dz) <= cut2)) continue;  // list entry within the skin only
+        const real xr = real(dx), yr = real(dy), zr = real(dz);
+        const real r2 = xr * xr + yr * yr + zr * zr;
+        const real rr1 = rsqrt_t(r2);
+        const real r = r2 * rr1;
+        const real rr2 = rr1 * rr1;
+        const real rr3 = rr2 * rr1;
+        const real rr5 = real(3) * rr2 * rr3;
+        if (KIND == 0) {
+          const real rr7 = real(5) * rr2 * rr5;
+          const real ck = s_m[0][q], dkx = s_m[1][q], dky = s_m[2][q], dkz = s_m[3][q];
+          const real qkxx = s_m[4][q], qkxy = s_m[5][q], qkxz = s_m[6][q], qkyy = s_m[7][q], qkyz = s_m[8][q],
+                     qkzz = s_m[9][q];
+          const real dkr = dkx * xr + dky * yr + dkz * zr;
+          const real qkx = qkxx * xr + qkxy * yr + qkxz * zr;
+          const real qky = qkxy * xr + qkyy * yr + qkyz * zr;
+          const real qkz = qkxz * xr + qkyz * yr + qkzz * zr;
+          const real qkr = qkx * xr + qky * yr + qkz * zr;
+          real bn[3];
+          ewald_bn<real, 3>(r, r2, aew, rr1, bn);
+          real l[3];
+          thole_l<real, 1>(r, pdi, s_pd[q], thi, s_th[q], l);
+          // unscaled pairs: dmp_n = bn_n - (1 - lambda_n) rr_n, the same for d and p
+          const real dmp3 = bn[0] - (real(1) - l[0]) * rr3;
+          const real dmp5 = bn[1] - (real(1) - l[1]) * rr5;
+          const real dmp7 = bn[2] - (real(1) - l[2]) * rr7;
+          const real ai = dmp3 * ck - dmp5 * dkr + dmp7 * qkr;
+          const real d5 = real(2) * dmp5;
+          acc[0] += double(-xr * ai - dmp3 * dkx + d5 * qkx);
+          acc[1] += double(-yr * ai - dmp3 * dky + d5 * qky);
+          acc[2] += double(-zr * ai - dmp3 * dkz + d5 * qkz);
+        } else {
+          real bn[2];
+          ewald_bn<real, 2>(r, r2, aew, rr1, bn);
+          real l[3];
+          thole_l<real, 0>(r, pdi, s_pd[q], thi, s_th[q], l);
+          const real dmp3 = bn[0] - (real(1) - l[0]) * rr3;
+          const real dmp5 = bn[1] - (real(1) - l[1]) * rr5;
+#pragma unroll
+          for (int rhs = 0; rhs < 2; ++rhs) {
+            const real ux = s_m[rhs * 3 + 0][q], uy = s_m[rhs * 3 + 1][q], uz = s_m[rhs * 3 + 2][q];
+            const real ur = dmp5 * (xr * ux + yr * uy + zr * uz);
+            acc[rhs * 3 + 0] += double(dmp3 * ux - ur * xr);
+            acc[rhs * 3 + 1] += double(dmp3 * uy - ur * yr);
+            acc[rhs * 3 + 2] += double(dmp3 * uz - ur * zr);
+          }
+        }
       }
+      if (done) break;
     }
   }
-  __shared__ double red[S > 1 ? S : 1][6][64];
+  if (KIND == 0) {  // unscaled pairs give identical d and p fields
+    acc[3] = acc[0];
+    acc[4] = acc[1];
+    acc[5] = acc[2];
+  }
+  if (ok && wv == S - 1) scaled_pairs<KIND == 0 ? 0 : 1>(sv, in_g, i, xi, yi, zi, acc);
+  // fixed-order sum over the S wavefronts
   if (S > 1) {
 #pragma unroll
-    for (int c = 0; c < 6; ++c) red[wv][c][lane] = acc[c];
+    for (int c = 0; c < 6; ++c) red[(wv * 6 + c) * 64 + lane] = acc[c];
     __syncthreads();
   }
-  if (wv == 0 && i < sv.n) {
-    const double pol = sv.pol[i];
+  if (wv == 0 && ok) {
+    const double pol = gp(sv.pol)[i];
     const double ipol = pol == 0.0 ? 1.0 / kTinyPol : 1.0 / pol;
-    double* out = vsel(sv, vout);
+    GLOBAL double* OUT = gw(KIND == 0 ? sv.ef : vsel(sv, vout));
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       double v = acc[c];
-      for (int q = 1; q < S; ++q) v += red[q][c][lane];
-      if (dodiag) v += in[(size_t)c * np + i] * ipol;
-      out[(size_t)c * np + i] = v;
+      for (int q = 1; q < S; ++q) v += red[(q * 6 + c) * 64 + lane];
+      if (KIND == 1 && dodiag) v += IN[(size_t)c * np + i] * ipol;
+      OUT[(size_t)c * np + i] = v;
     }
   }
 }
@@ -540,10 +797,6 @@ __device__ __forceinline__ int bspline_axis(double xc, double rc, int nfft, real
   return g0;
 }
 
-template <typename real> struct Cplx { real re, im; };
-
-__device__ __forceinline__ void atomic_add_real(double* p, double v) { unsafeAtomicAdd(p, v); }
-__device__ __forceinline__ void atomic_add_real(float* p, float v) { unsafeAtomicAdd(p, v); }
 
 // zero a system's grid (a fill kernel, safe inside captured graphs)
 template <typename real>
@@ -553,122 +806,319 @@ __global__ void __launch_bounds__(256) k_zero_grid(const SysView* views, int sys
     g[q] = real(0);
 }
 
-// grid_uind_site (pmestuff.f:280-400): fractional d dipoles into the real
-// part and p dipoles into the imaginary part; one thread per atom.
+// ---------------------------------------------------------------------
+// Owner-computes spreading.  The grid is cut into bricks of B[d] <= 8
+// points per axis (every brick at least 4 wide, so a 5-point B-spline
+// support spans at most two consecutive bricks).  Once per solve each atom
+// gets its first grid node g0 and its B-spline values theta (3 levels x 5
+// nodes x 3 axes) and is binned by the brick that holds g0; bins are sorted
+// by atom index.  A spreading pass then runs one wavefront per brick: lane
+// (py,pz) owns one x-pencil of the brick, walks the atoms of the 8 bins
+// that can reach the brick in a fixed order and accumulates its pencil in
+// registers, then stores it.  No atomics, no prior zeroing, deterministic.
+constexpr int kTheta = 48;  // per-atom theta row: [axis][level][node] = 3*3*5 = 45, padded
+
 template <typename real>
-__global__ void __launch_bounds__(64) k_spread_uind(const SysView* views, const Work* work, int vin,
-                                                     int frac_in, int check_conv) {
+__device__ __forceinline__ const real* theta_row(const SysView& sv, int i) {
+  return static_cast<const real*>(sv.theta) + (size_t)i * kTheta;
+}
+__device__ __forceinline__ int th_idx(int axis, int level, int node) { return axis * 15 + level * 5 + node; }
+
+// per-solve: g0, theta and home bin of every atom; bin counts
+template <typename real>
+__global__ void __launch_bounds__(64) k_pme_prep(const SysView* views, const Work* work) {
   const Work w = work[blockIdx.x];
   const SysView& sv = views[w.sys];
-  if (check_conv && sv.st->conv) return;
   const int i = w.blk * kBlock + threadIdx.x;
   if (i >= sv.n) return;
-  const size_t np = sv.npad;
-  const double* in = vsel(sv, vin);
-  real fd[3], fp[3];
+  const double xyz[3] = {sv.x[i], sv.y[i], sv.z[i]};
+  real* th = static_cast<real*>(sv.theta) + (size_t)i * kTheta;
+  int g0[3], bin[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const double sc = frac_in ? 1.0 : sv.afrac[c];
-    fd[c] = real(sc * in[(size_t)c * np + i]);
-    fp[c] = real(sc * in[(size_t)(3 + c) * np + i]);
+  for (int d = 0; d < 3; ++d) {
+    real t[3][kOrder];
+    g0[d] = bspline_axis<real, 3>(xyz[d], 1.0 / sv.box[d], sv.nfft[d], t);
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+      for (int m = 0; m < kOrder; ++m) th[th_idx(d, l, m)] = t[l][m];
+    bin[d] = g0[d] / sv.brick[d];
   }
-  real t1[2][kOrder], t2[2][kOrder], t3[2][kOrder];
-  const int g1 = bspline_axis<real, 2>(sv.x[i], 1.0 / sv.box[0], sv.nfft[0], t1);
-  const int g2 = bspline_axis<real, 2>(sv.y[i], 1.0 / sv.box[1], sv.nfft[1], t2);
-  const int g3 = bspline_axis<real, 2>(sv.z[i], 1.0 / sv.box[2], sv.nfft[2], t3);
-  real* grid = static_cast<real*>(sv.grid);
-  const int n1 = sv.nfft[0], n2 = sv.nfft[1], n3 = sv.nfft[2];
-#pragma unroll
-  for (int mk = 0; mk < kOrder; ++mk) {
-    int k = g3 + mk;
-    if (k >= n3) k -= n3;
-    const real v0 = t3[0][mk], v1 = t3[1][mk];
-#pragma unroll
-    for (int mj = 0; mj < kOrder; ++mj) {
-      int j = g2 + mj;
-      if (j >= n2) j -= n2;
-      const real u0 = t2[0][mj], u1 = t2[1][mj];
-      const real term01 = fd[1] * u1 * v0 + fd[2] * u0 * v1;
-      const real term11 = fd[0] * u0 * v0;
-      const real term02 = fp[1] * u1 * v0 + fp[2] * u0 * v1;
-      const real term12 = fp[0] * u0 * v0;
-      real* row = grid + ((size_t)k * n2 + j) * n1 * 2;
-#pragma unroll
-      for (int mi = 0; mi < kOrder; ++mi) {
-        int ii = g1 + mi;
-        if (ii >= n1) ii -= n1;
-        const real t0 = t1[0][mi], tt1 = t1[1][mi];
-        atomic_add_real(row + 2 * ii, term01 * t0 + term11 * tt1);
-        atomic_add_real(row + 2 * ii + 1, term02 * t0 + term12 * tt1);
-      }
-    }
-  }
+  const int b = (bin[2] * sv.nbrick[1] + bin[1]) * sv.nbrick[0] + bin[0];
+  sv.g0[4 * i] = g0[0];
+  sv.g0[4 * i + 1] = g0[1];
+  sv.g0[4 * i + 2] = g0[2];
+  sv.g0[4 * i + 3] = b;
+  atomicAdd(&sv.bin_count[b], 1);
 }
 
-// grid_mpole_site (pmestuff.f:163-274) after cmp_to_fmp_site; real part only.
-template <typename real>
-__global__ void __launch_bounds__(64) k_spread_mpole(const SysView* views, const Work* work) {
+// exclusive scan of the bin counts (one workgroup per system)
+__global__ void __launch_bounds__(1024) k_pme_scan(const SysView* views, const int* sys_ids) {
+  const SysView& sv = views[sys_ids[blockIdx.x]];
+  const int nb = sv.nbrick[0] * sv.nbrick[1] * sv.nbrick[2];
+  __shared__ int carry;
+  __shared__ int wsum[16];
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += 1024) {
+    const int q = base + threadIdx.x;
+    const int v = q < nb ? sv.bin_count[q] : 0;
+    // inclusive scan within the wave
+    int x = v;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(x, off, 64);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    int pre = carry;
+    for (int u = 0; u < wv; ++u) pre += wsum[u];
+    if (q < nb) {
+      sv.bin_start[q] = pre + x - v;
+      sv.bin_count[q] = 0;  // reused as the fill cursor
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int t = 0;
+      for (int u = 0; u < 16; ++u) t += wsum[u];
+      carry += t;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sv.bin_start[nb] = carry;
+}
+
+__global__ void __launch_bounds__(64) k_pme_fill(const SysView* views, const Work* work) {
   const Work w = work[blockIdx.x];
   const SysView& sv = views[w.sys];
   const int i = w.blk * kBlock + threadIdx.x;
   if (i >= sv.n) return;
-  const size_t np = sv.npad;
-  // cmp order (m, dx,dy,dz, qxx,qyy,qzz, 2qxy, 2qxz, 2qyz) from the stored
-  // (q, d, qxx,qxy,qxz,qyy,qyz,qzz)
-  double cmp[10];
-  cmp[0] = sv.rpole[0 * np + i];
-  cmp[1] = sv.rpole[1 * np + i];
-  cmp[2] = sv.rpole[2 * np + i];
-  cmp[3] = sv.rpole[3 * np + i];
-  cmp[4] = sv.rpole[4 * np + i];
-  cmp[5] = sv.rpole[7 * np + i];
-  cmp[6] = sv.rpole[9 * np + i];
-  cmp[7] = 2.0 * sv.rpole[5 * np + i];
-  cmp[8] = 2.0 * sv.rpole[6 * np + i];
-  cmp[9] = 2.0 * sv.rpole[8 * np + i];
-  real fmp[10];
-  fmp[0] = real(sv.ctf[0][0] * cmp[0]);
-#pragma unroll
-  for (int j = 1; j < 4; ++j) {
-    double a = 0.0;
-#pragma unroll
-    for (int k = 1; k < 4; ++k) a += sv.ctf[j][k] * cmp[k];
-    fmp[j] = real(a);
+  const int b = sv.g0[4 * i + 3];
+  const int pos = atomicAdd(&sv.bin_count[b], 1);
+  sv.bin_list[sv.bin_start[b] + pos] = i;
+}
+
+// sort each bin by atom index (bitonic, in LDS) so that spreading order
+// does not depend on the fill order
+constexpr int kMaxBin = 2048;
+__global__ void __launch_bounds__(256) k_pme_binsort(const SysView* views, const Work* bwork) {
+  const Work w = bwork[blockIdx.x];
+  const SysView& sv = views[w.sys];
+  const int b = w.blk;
+  const int s0 = sv.bin_start[b], cnt = sv.bin_start[b + 1] - s0;
+  if (cnt <= 1) return;
+  if (cnt > kMaxBin) {
+    if (threadIdx.x == 0) *sv.err = THP_ERR_LIST;
+    return;
   }
-#pragma unroll
-  for (int j = 4; j < 10; ++j) {
-    double a = 0.0;
-#pragma unroll
-    for (int k = 4; k < 10; ++k) a += sv.ctf[j][k] * cmp[k];
-    fmp[j] = real(a);
-  }
-  real t1[3][kOrder], t2[3][kOrder], t3[3][kOrder];
-  const int g1 = bspline_axis<real, 3>(sv.x[i], 1.0 / sv.box[0], sv.nfft[0], t1);
-  const int g2 = bspline_axis<real, 3>(sv.y[i], 1.0 / sv.box[1], sv.nfft[1], t2);
-  const int g3 = bspline_axis<real, 3>(sv.z[i], 1.0 / sv.box[2], sv.nfft[2], t3);
-  real* grid = static_cast<real*>(sv.grid);
-  const int n1 = sv.nfft[0], n2 = sv.nfft[1], n3 = sv.nfft[2];
-#pragma unroll
-  for (int mk = 0; mk < kOrder; ++mk) {
-    int k = g3 + mk;
-    if (k >= n3) k -= n3;
-    const real v0 = t3[0][mk], v1 = t3[1][mk], v2 = t3[2][mk];
-#pragma unroll
-    for (int mj = 0; mj < kOrder; ++mj) {
-      int j = g2 + mj;
-      if (j >= n2) j -= n2;
-      const real u0 = t2[0][mj], u1 = t2[1][mj], u2 = t2[2][mj];
-      const real term0 = fmp[0] * u0 * v0 + fmp[2] * u1 * v0 + fmp[3] * u0 * v1 + fmp[5] * u2 * v0 +
-                         fmp[6] * u0 * v2 + fmp[9] * u1 * v1;
-      const real term1 = fmp[1] * u0 * v0 + fmp[7] * u1 * v0 + fmp[8] * u0 * v1;
-      const real term2 = fmp[4] * u0 * v0;
-      real* row = grid + ((size_t)k * n2 + j) * n1 * 2;
-#pragma unroll
-      for (int mi = 0; mi < kOrder; ++mi) {
-        int ii = g1 + mi;
-        if (ii >= n1) ii -= n1;
-        atomic_add_real(row + 2 * ii, term0 * t1[0][mi] + term1 * t1[1][mi] + term2 * t1[2][mi]);
+  __shared__ int key[kMaxBin];
+  int n2 = 1;
+  while (n2 < cnt) n2 <<= 1;
+  for (int q = threadIdx.x; q < n2; q += blockDim.x) key[q] = q < cnt ? sv.bin_list[s0 + q] : 0x7fffffff;
+  __syncthreads();
+  for (int size = 2; size <= n2; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int q = threadIdx.x; q < n2; q += blockDim.x) {
+        const int partner = q ^ stride;
+        if (partner > q) {
+          const bool up = (q & size) == 0;
+          const int a = key[q], c = key[partner];
+          if ((a > c) == up) {
+            key[q] = c;
+            key[partner] = a;
+          }
+        }
       }
+      __syncthreads();
+    }
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) sv.bin_list[s0 + q] = key[q];
+}
+
+// One workgroup of 4 wavefronts per brick.  KIND 0: grid_uind_site
+// (pmestuff.f:280-400), fractional d dipoles into the real part and p
+// dipoles into the imaginary part.  KIND 1: grid_mpole_site
+// (pmestuff.f:163-274) after cmp_to_fmp_site, real part only (imaginary
+// part cleared).  The atoms of each bin are staged in LDS 64 at a time
+// (one coalesced round trip per 64 atoms); wavefront v accumulates the
+// staged atoms v, v+4, ... into its lanes' pencils, and the four partial
+// pencils are summed in LDS in a fixed order before the store.
+constexpr int kSpreadWaves = 4;
+template <typename real, int KIND>
+__global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_brick(const SysView* views, const Work* bwork, int vin,
+                                                                     int frac_in, int check_conv) {
+  constexpr int NL = KIND == 0 ? 2 : 3;   // spline levels used
+  constexpr int NF = KIND == 0 ? 6 : 10;  // moments per atom
+  constexpr int NT = 3 * NL * kOrder;
+  __shared__ int s_lx[64], s_ly[64], s_lz[64];
+  __shared__ real s_th[64][NT];
+  __shared__ real s_f[64][NF];
+  __shared__ real s_red[kSpreadWaves - 1][2][8][64];
+  const Work w = bwork[blockIdx.x];
+  const SysView& sv = views[w.sys];
+  if (check_conv && sv.st->conv) return;
+  const int nb0 = sv.nbrick[0], nb1 = sv.nbrick[1], nb2 = sv.nbrick[2];
+  const int tb = w.blk;
+  const int t0 = tb % nb0, t1 = (tb / nb0) % nb1, t2 = tb / (nb0 * nb1);
+  const int K0 = sv.nfft[0], K1 = sv.nfft[1], K2 = sv.nfft[2];
+  const int B0 = sv.brick[0], B1 = sv.brick[1], B2 = sv.brick[2];
+  const int sx = t0 * B0, sy = t1 * B1, sz = t2 * B2;
+  const int ex = t0 == nb0 - 1 ? K0 : sx + B0;
+  const int ey = t1 == nb1 - 1 ? K1 : sy + B1;
+  const int ez = t2 == nb2 - 1 ? K2 : sz + B2;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int py = lane & 7, pz = lane >> 3;
+  const bool own = sy + py < ey && sz + pz < ez;
+  const size_t np = sv.npad;
+  const GLOBAL double* IN = gp(vsel(sv, vin));
+  const GLOBAL double* RP = gp(sv.rpole);
+  const GLOBAL real* THETA = gp(static_cast<const real*>(sv.theta));
+  const GLOBAL int* G0 = gp(sv.g0);
+  const GLOBAL int* BS = gp(sv.bin_start);
+  const GLOBAL int* BLIST = gp(sv.bin_list);
+  const double af0 = frac_in ? 1.0 : sv.afrac[0], af1 = frac_in ? 1.0 : sv.afrac[1], af2 = frac_in ? 1.0 : sv.afrac[2];
+  double ctf[10][10];
+  if (KIND == 1) {
+#pragma unroll
+    for (int a = 0; a < 10; ++a)
+#pragma unroll
+      for (int b = 0; b < 10; ++b) ctf[a][b] = sv.ctf[a][b];
+  }
+  real acc0[8], acc1[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) acc0[q] = acc1[q] = real(0);
+  for (int bz = 0; bz < 2; ++bz)
+    for (int by = 0; by < 2; ++by)
+      for (int bx = 0; bx < 2; ++bx) {
+        const int hb0 = (t0 - bx + nb0) % nb0, hb1 = (t1 - by + nb1) % nb1, hb2 = (t2 - bz + nb2) % nb2;
+        const int hb = (hb2 * nb1 + hb1) * nb0 + hb0;
+        const int a0 = BS[hb], a1 = BS[hb + 1];
+        for (int c0 = a0; c0 < a1; c0 += 64) {
+          const int cnt = min(64, a1 - c0);
+          __syncthreads();
+          if (threadIdx.x < cnt) {
+            const int i = BLIST[c0 + lane];
+            const int gx = G0[4 * i], gy = G0[4 * i + 1], gz = G0[4 * i + 2];
+            // local offset of the first spline node relative to the brick start;
+            // bins of the previous brick give [-B,-1] (wrapped at the box edge)
+            int lx = gx - sx, ly = gy - sy, lz = gz - sz;
+            if (bx && lx > 0) lx -= K0;
+            if (by && ly > 0) ly -= K1;
+            if (bz && lz > 0) lz -= K2;
+            s_lx[lane] = lx;
+            s_ly[lane] = ly;
+            s_lz[lane] = lz;
+            const GLOBAL real* th = THETA + (size_t)i * kTheta;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+              for (int l = 0; l < NL; ++l)
+#pragma unroll
+                for (int m = 0; m < kOrder; ++m) s_th[lane][(d * NL + l) * kOrder + m] = th[th_idx(d, l, m)];
+            if (KIND == 0) {
+              s_f[lane][0] = real(af0 * IN[0 * np + i]);
+              s_f[lane][1] = real(af1 * IN[1 * np + i]);
+              s_f[lane][2] = real(af2 * IN[2 * np + i]);
+              s_f[lane][3] = real(af0 * IN[3 * np + i]);
+              s_f[lane][4] = real(af1 * IN[4 * np + i]);
+              s_f[lane][5] = real(af2 * IN[5 * np + i]);
+            } else {
+              double cmp[10];
+              cmp[0] = RP[0 * np + i];
+              cmp[1] = RP[1 * np + i];
+              cmp[2] = RP[2 * np + i];
+              cmp[3] = RP[3 * np + i];
+              cmp[4] = RP[4 * np + i];
+              cmp[5] = RP[7 * np + i];
+              cmp[6] = RP[9 * np + i];
+              cmp[7] = 2.0 * RP[5 * np + i];
+              cmp[8] = 2.0 * RP[6 * np + i];
+              cmp[9] = 2.0 * RP[8 * np + i];
+              s_f[lane][0] = real(ctf[0][0] * cmp[0]);
+#pragma unroll
+              for (int j = 1; j < 4; ++j) {
+                double s = 0.0;
+#pragma unroll
+                for (int kk = 1; kk < 4; ++kk) s += ctf[j][kk] * cmp[kk];
+                s_f[lane][j] = real(s);
+              }
+#pragma unroll
+              for (int j = 4; j < 10; ++j) {
+                double s = 0.0;
+#pragma unroll
+                for (int kk = 4; kk < 10; ++kk) s += ctf[j][kk] * cmp[kk];
+                s_f[lane][j] = real(s);
+              }
+            }
+          }
+          __syncthreads();
+          for (int q = wv; q < cnt; q += kSpreadWaves) {
+            const int lx = s_lx[q], ly = s_ly[q], lz = s_lz[q];
+            if (lx < -4 || ly < -4 || lz < -4) continue;  // support ends before this brick
+            const int my = py - ly, mz = pz - lz;
+            if (!(own && my >= 0 && my < kOrder && mz >= 0 && mz < kOrder)) continue;
+            const real* th = s_th[q];
+            const real* f = s_f[q];
+            if (KIND == 0) {
+              const real u0 = th[(1 * NL + 0) * kOrder + my], u1 = th[(1 * NL + 1) * kOrder + my];
+              const real v0 = th[(2 * NL + 0) * kOrder + mz], v1 = th[(2 * NL + 1) * kOrder + mz];
+              const real term01 = f[1] * u1 * v0 + f[2] * u0 * v1;
+              const real term11 = f[0] * u0 * v0;
+              const real term02 = f[4] * u1 * v0 + f[5] * u0 * v1;
+              const real term12 = f[3] * u0 * v0;
+#pragma unroll
+              for (int xq = 0; xq < 8; ++xq) {
+                const int m = xq - lx;
+                if (m >= 0 && m < kOrder) {
+                  const real ta = th[m], tb1 = th[kOrder + m];
+                  acc0[xq] += term01 * ta + term11 * tb1;
+                  acc1[xq] += term02 * ta + term12 * tb1;
+                }
+              }
+            } else {
+              const real u0 = th[(1 * NL + 0) * kOrder + my], u1 = th[(1 * NL + 1) * kOrder + my],
+                         u2 = th[(1 * NL + 2) * kOrder + my];
+              const real v0 = th[(2 * NL + 0) * kOrder + mz], v1 = th[(2 * NL + 1) * kOrder + mz],
+                         v2 = th[(2 * NL + 2) * kOrder + mz];
+              const real term0 =
+                  f[0] * u0 * v0 + f[2] * u1 * v0 + f[3] * u0 * v1 + f[5] * u2 * v0 + f[6] * u0 * v2 + f[9] * u1 * v1;
+              const real term1 = f[1] * u0 * v0 + f[7] * u1 * v0 + f[8] * u0 * v1;
+              const real term2 = f[4] * u0 * v0;
+#pragma unroll
+              for (int xq = 0; xq < 8; ++xq) {
+                const int m = xq - lx;
+                if (m >= 0 && m < kOrder)
+                  acc0[xq] += term0 * th[m] + term1 * th[kOrder + m] + term2 * th[2 * kOrder + m];
+              }
+            }
+          }
+        }
+      }
+  // fixed-order sum of the four wavefronts' pencils
+  if (wv > 0) {
+#pragma unroll
+    for (int xq = 0; xq < 8; ++xq) {
+      s_red[wv - 1][0][xq][lane] = acc0[xq];
+      s_red[wv - 1][1][xq][lane] = acc1[xq];
+    }
+  }
+  __syncthreads();
+  if (wv > 0 || !own) return;
+#pragma unroll
+  for (int v = 0; v < kSpreadWaves - 1; ++v)
+#pragma unroll
+    for (int xq = 0; xq < 8; ++xq) {
+      acc0[xq] += s_red[v][0][xq][lane];
+      acc1[xq] += s_red[v][1][xq][lane];
+    }
+  GLOBAL real* row = gw(static_cast<real*>(sv.grid)) + ((size_t)(sz + pz) * K1 + (sy + py)) * K0 * 2;
+#pragma unroll
+  for (int xq = 0; xq < 8; ++xq) {
+    const int x = sx + xq;
+    if (x < ex) {
+      row[2 * x] = acc0[xq];
+      row[2 * x + 1] = acc1[xq];
     }
   }
 }
@@ -690,134 +1140,121 @@ __global__ void __launch_bounds__(256) k_conv(const SysView* views, int sys, siz
 
 // fphi_uind_site first derivatives (pmestuff.f:746-1037, fdip_phi(2:4) of
 // both grids) and the fractional->Cartesian field (newinduce_pme.f:2103-2110).
-// Accumulation of the gathered sums is FP64.
+// 32 lanes share one atom: lane r < 25 reads stencil row (mj, mk) = (r%5,
+// r/5), five consecutive complex points, and the six row sums are reduced
+// over the 32 lanes by butterfly shuffles.  The sums are FP64.
+constexpr int kGatherAtoms = 8;  // atoms per 256-thread block
+
 template <typename real>
-__device__ __forceinline__ void gather_uind(const SysView& sv, int i, double p1[3], double p2[3]) {
-  real t1[2][kOrder], t2[2][kOrder], t3[2][kOrder];
-  const int g1 = bspline_axis<real, 2>(sv.x[i], 1.0 / sv.box[0], sv.nfft[0], t1);
-  const int g2 = bspline_axis<real, 2>(sv.y[i], 1.0 / sv.box[1], sv.nfft[1], t2);
-  const int g3 = bspline_axis<real, 2>(sv.z[i], 1.0 / sv.box[2], sv.nfft[2], t3);
-  const real* grid = static_cast<const real*>(sv.grid);
-  const int n1 = sv.nfft[0], n2 = sv.nfft[1], n3 = sv.nfft[2];
-  double a[6] = {0, 0, 0, 0, 0, 0};
+__device__ __forceinline__ void gather_uind_rows(const GLOBAL real* grid, const GLOBAL real* theta,
+                                                 const GLOBAL int* g0, int n1, int n2, int n3, int i, int r,
+                                                 double a[6]) {
 #pragma unroll
-  for (int mk = 0; mk < kOrder; ++mk) {
-    int k = g3 + mk;
-    if (k >= n3) k -= n3;
-    const real v0 = t3[0][mk], v1 = t3[1][mk];
-    real tu00_1 = 0, tu10_1 = 0, tu01_1 = 0, tu00_2 = 0, tu10_2 = 0, tu01_2 = 0;
+  for (int c = 0; c < 6; ++c) a[c] = 0.0;
+  if (r >= kOrder * kOrder) return;
+  const int mj = r % kOrder, mk = r / kOrder;
+  const int gx = g0[4 * i], gy = g0[4 * i + 1], gz = g0[4 * i + 2];
+  const GLOBAL real* th = theta + (size_t)i * kTheta;
+  int k = gz + mk;
+  if (k >= n3) k -= n3;
+  int j = gy + mj;
+  if (j >= n2) j -= n2;
+  const GLOBAL real* row = grid + ((size_t)k * n2 + j) * n1 * 2;
+  real t0_1 = 0, t1_1 = 0, t0_2 = 0, t1_2 = 0;
 #pragma unroll
-    for (int mj = 0; mj < kOrder; ++mj) {
-      int j = g2 + mj;
-      if (j >= n2) j -= n2;
-      const real u0 = t2[0][mj], u1 = t2[1][mj];
-      const real* row = grid + ((size_t)k * n2 + j) * n1 * 2;
-      real t0_1 = 0, t1_1 = 0, t0_2 = 0, t1_2 = 0;
-#pragma unroll
-      for (int mi = 0; mi < kOrder; ++mi) {
-        int ii = g1 + mi;
-        if (ii >= n1) ii -= n1;
-        const real q1 = row[2 * ii], q2 = row[2 * ii + 1];
-        t0_1 += q1 * t1[0][mi];
-        t1_1 += q1 * t1[1][mi];
-        t0_2 += q2 * t1[0][mi];
-        t1_2 += q2 * t1[1][mi];
-      }
-      tu00_1 += t0_1 * u0;
-      tu10_1 += t1_1 * u0;
-      tu01_1 += t0_1 * u1;
-      tu00_2 += t0_2 * u0;
-      tu10_2 += t1_2 * u0;
-      tu01_2 += t0_2 * u1;
-    }
-    a[0] += double(tu10_1 * v0);
-    a[1] += double(tu01_1 * v0);
-    a[2] += double(tu00_1 * v1);
-    a[3] += double(tu10_2 * v0);
-    a[4] += double(tu01_2 * v0);
-    a[5] += double(tu00_2 * v1);
+  for (int mi = 0; mi < kOrder; ++mi) {
+    int ii = gx + mi;
+    if (ii >= n1) ii -= n1;
+    const real q1 = row[2 * ii], q2 = row[2 * ii + 1];
+    const real ta = th[th_idx(0, 0, mi)], tb = th[th_idx(0, 1, mi)];
+    t0_1 += q1 * ta;
+    t1_1 += q1 * tb;
+    t0_2 += q2 * ta;
+    t1_2 += q2 * tb;
   }
-  p1[0] = a[0];
-  p1[1] = a[1];
-  p1[2] = a[2];
-  p2[0] = a[3];
-  p2[1] = a[4];
-  p2[2] = a[5];
+  const real u0 = th[th_idx(1, 0, mj)], u1 = th[th_idx(1, 1, mj)];
+  const real v0 = th[th_idx(2, 0, mk)], v1 = th[th_idx(2, 1, mk)];
+  a[0] = double(t1_1 * u0 * v0);
+  a[1] = double(t0_1 * u1 * v0);
+  a[2] = double(t0_1 * u0 * v1);
+  a[3] = double(t1_2 * u0 * v0);
+  a[4] = double(t0_2 * u1 * v0);
+  a[5] = double(t0_2 * u0 * v1);
 }
 
 // MODE 0: write the reciprocal dipole field of vin into vout (frac_out: the
 //         fractional derivatives fdip_phi(2:4) instead).
 // MODE 1: PCG start (newinduce_pme2.f:406-438): h = hreal + dip - term*mu,
-//         res = ef - h, zr = diag*res, pp = zr, ggold = sum res.zr.
-// MODE 2: PCG iteration (:452-473): h = hreal + dip - term*pp, gg = pp.h.
+//         res = ef - h, zr = diag*res, pp = zr; partial sums of res.zr
+//         (ggold of the first iteration) into rz partial buffer 0.
+// MODE 2: PCG iteration (:452-473): h = hreal + dip - term*pp; partial
+//         sums of pp.h (gg).
+// Launched on the gather work table: block (sys, g) of G = gat_blocks
+// blocks handles atoms g*8 + a, stepping by 8*G.
 template <typename real, int MODE>
-__global__ void __launch_bounds__(64) k_gather_uind(const SysView* views, const Work* work, int vin, int vout,
-                                                     int frac_out) {
+__global__ void __launch_bounds__(256) k_gather_uind(const SysView* views, const Work* work, int vin, int vout,
+                                                      int frac_out) {
   const Work w = work[blockIdx.x];
   const SysView& sv = views[w.sys];
   if (MODE == 2 && sv.st->conv) return;
-  const int i = w.blk * kBlock + threadIdx.x;
+  const int r = threadIdx.x & 31;
+  const int G = sv.gat_blocks;
   const size_t np = sv.npad;
-  const double* in = vsel(sv, vin);
+  const GLOBAL real* GRID = gp(static_cast<const real*>(sv.grid));
+  const GLOBAL real* THETA = gp(static_cast<const real*>(sv.theta));
+  const GLOBAL int* G0 = gp(sv.g0);
+  const int n1 = sv.nfft[0], n2 = sv.nfft[1], n3 = sv.nfft[2];
   double q[2] = {0.0, 0.0};
-  if (i < sv.n) {
-    double p1[3], p2[3];
-    gather_uind<real>(sv, i, p1, p2);
-    double dip[6];
+  for (int base = w.blk * kGatherAtoms; base < sv.n; base += G * kGatherAtoms) {
+    const int i = base + (threadIdx.x >> 5);
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    if (i < sv.n) gather_uind_rows<real>(GRID, THETA, G0, n1, n2, n3, i, r, a);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const double sc = (MODE == 0 && frac_out) ? 1.0 : sv.afrac[c];
-      dip[c] = sc * p1[c];
-      dip[3 + c] = sc * p2[c];
+    for (int c = 0; c < 6; ++c) {
+#pragma unroll
+      for (int off = 16; off > 0; off >>= 1) a[c] += __shfl_xor(a[c], off, 32);
     }
-    if (MODE == 0) {
-      double* out = vsel(sv, vout);
+    if (r == 0 && i < sv.n) {
+      double dip[6];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) out[(size_t)c * np + i] = dip[c];
-    } else {
-      const double term = sv.term_self;
-      const double diag = sv.pol[i] == 0.0 ? kTinyPol : sv.pol[i];
+      for (int c = 0; c < 3; ++c) {
+        const double sc = (MODE == 0 && frac_out) ? 1.0 : sv.afrac[c];
+        dip[c] = sc * a[c];
+        dip[3 + c] = sc * a[3 + c];
+      }
+      if (MODE == 0) {
+        double* out = vsel(sv, vout);
 #pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        const size_t o = (size_t)c * np + i;
-        const double u = in[o];
-        const double h = sv.hreal[o] + dip[c] - term * u;
-        sv.h[o] = h;
-        if (MODE == 1) {
-          const double r = sv.ef[o] - h;
-          const double z = diag * r;
-          sv.res[o] = r;
-          sv.zr[o] = z;
-          sv.pp[o] = z;
-          q[c / 3] += r * z;
-        } else {
-          q[c / 3] += u * h;
+        for (int c = 0; c < 6; ++c) out[(size_t)c * np + i] = dip[c];
+      } else {
+        const GLOBAL double* IN = gp(vsel(sv, vin));
+        const double term = sv.term_self;
+        const double pl = gp(sv.pol)[i];
+        const double diag = pl == 0.0 ? kTinyPol : pl;
+        const GLOBAL double* HR = gp(sv.hreal);
+        GLOBAL double* H = gw(sv.h);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          const size_t o = (size_t)c * np + i;
+          const double u = IN[o];
+          const double h = HR[o] + dip[c] - term * u;
+          H[o] = h;
+          if (MODE == 1) {
+            const double rr = gp(sv.ef)[o] - h;
+            const double z = diag * rr;
+            gw(sv.res)[o] = rr;
+            gw(sv.zr)[o] = z;
+            gw(sv.pp)[o] = z;
+            q[c / 3] += rr * z;
+          } else {
+            q[c / 3] += u * h;
+          }
         }
       }
     }
   }
   if (MODE == 0) return;
-  const double s0 = wave_sum(q[0]), s1 = wave_sum(q[1]);
-  double* part = sv.partial;  // slot 0
-  if (threadIdx.x == 0) {
-    part[w.blk * 2] = s0;
-    part[w.blk * 2 + 1] = s1;
-  }
-  if (last_block(&sv.st->counter[0], sv.nblk)) {
-    const double a0 = sum_partials(part, sv.nblk, 0), a1 = sum_partials(part, sv.nblk, 1);
-    if (threadIdx.x == 0) {
-      SolveState* st = sv.st;
-      if (MODE == 1) {
-        st->ggold[0] = a0;
-        st->ggold[1] = a1;
-      } else {
-        st->gg[0] = a0;
-        st->gg[1] = a1;
-        if (a0 <= 0.0 || a1 <= 0.0) st->n_indef += 1;
-        if (a0 == 0.0 || a1 == 0.0) st->conv = kZeroCurv;  // reference returns early
-      }
-    }
-  }
+  block_partial2(q[0], q[1], MODE == 1 ? part_init(sv) : part_gg(sv), w.blk);
 }
 
 // fphi_mpole_site (pmestuff.f:453-631, potential and derivatives through
@@ -833,9 +1270,16 @@ __global__ void __launch_bounds__(64) k_gather_mpole(const SysView* views, const
   if (i >= sv.n) return;
   const size_t np = sv.npad;
   real t1[3][kOrder], t2[3][kOrder], t3[3][kOrder];
-  const int g1 = bspline_axis<real, 3>(sv.x[i], 1.0 / sv.box[0], sv.nfft[0], t1);
-  const int g2 = bspline_axis<real, 3>(sv.y[i], 1.0 / sv.box[1], sv.nfft[1], t2);
-  const int g3 = bspline_axis<real, 3>(sv.z[i], 1.0 / sv.box[2], sv.nfft[2], t3);
+  const real* th = theta_row<real>(sv, i);
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int m = 0; m < kOrder; ++m) {
+      t1[l][m] = th[th_idx(0, l, m)];
+      t2[l][m] = th[th_idx(1, l, m)];
+      t3[l][m] = th[th_idx(2, l, m)];
+    }
+  const int g1 = sv.g0[4 * i], g2 = sv.g0[4 * i + 1], g3 = sv.g0[4 * i + 2];
   const real* grid = static_cast<const real*>(sv.grid);
   const int n1 = sv.nfft[0], n2 = sv.nfft[1], n3 = sv.nfft[2];
   double tuv[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -911,6 +1355,12 @@ __global__ void __launch_bounds__(64) k_gather_mpole(const SysView* views, const
 }
 
 // ===================================================== PCG vector kernels
+// Reductions: each reducing kernel writes one pair of FP64 partial sums
+// per block (G = red_blocks blocks per system); the kernels that need a
+// scalar re-sum the G partials in a fixed order themselves.  A kernel
+// boundary orders these accesses, so no fences or atomics are needed, and
+// every block obtains bit-identical scalars.
+
 // direct guess mu = alpha*ef (polgsf = 0, newinduce_pme2.f:180-188)
 __global__ void __launch_bounds__(64) k_guess(const SysView* views, const Work* work) {
   const Work w = work[blockIdx.x];
@@ -922,119 +1372,151 @@ __global__ void __launch_bounds__(64) k_guess(const SysView* views, const Work* 
   for (int c = 0; c < 6; ++c) sv.mu[(size_t)c * sv.npad + i] = a * sv.ef[(size_t)c * sv.npad + i];
 }
 
-// pcg_b (newinduce_pme2.f:504-541): alpha = ggold/gg; mu += alpha pp;
-// res -= alpha h; zr = diag*res; partial ggnew and ene.  Also clears the
-// system's PME grid for the next spreading pass.
+// pcg_b (newinduce_pme2.f:496-541) for iteration `it` (1-based):
+// alpha = ggold/gg with ggold = sum of rz partials of iteration it-1;
+// mu += alpha pp; res -= alpha h; zr = diag*res; partial ggnew (into rz
+// buffer it&1) and ene.
 template <typename real>
-__global__ void __launch_bounds__(64) k_cg_update1(const SysView* views, const Work* work) {
-  const Work w = work[blockIdx.x];
-  const SysView& sv = views[w.sys];
-  if (sv.st->conv) return;
-  const int i = w.blk * kBlock + threadIdx.x;
-  const size_t np = sv.npad;
-  const SolveState* st = sv.st;
-  const double alpha[2] = {st->ggold[0] / st->gg[0], st->ggold[1] / st->gg[1]};
-  double gn[2] = {0, 0}, en[2] = {0, 0};
-  if (i < sv.n) {
-    const double diag = sv.pol[i] == 0.0 ? kTinyPol : sv.pol[i];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      const size_t o = (size_t)c * np + i;
-      const double a = alpha[c / 3];
-      const double m = sv.mu[o] + a * sv.pp[o];
-      const double r = sv.res[o] - a * sv.h[o];
-      const double z = diag * r;
-      sv.mu[o] = m;
-      sv.res[o] = r;
-      sv.zr[o] = z;
-      gn[c / 3] += r * z;
-      en[c / 3] += -0.5 * (m * (r + sv.ef[o]));
-    }
-  }
-  // clear this block's share of the grid (it was consumed by the gather)
-  {
-    const size_t total = (size_t)2 * sv.nfft[0] * sv.nfft[1] * sv.nfft[2];
-    const size_t chunk = (total + sv.nblk - 1) / sv.nblk;
-    const size_t b0 = (size_t)w.blk * chunk, b1 = min(total, b0 + chunk);
-    real* g = static_cast<real*>(sv.grid);
-    for (size_t q = b0 + threadIdx.x; q < b1; q += 64) g[q] = real(0);
-  }
-  double* part = sv.partial + (size_t)2 * sv.nblk;  // slot 1 (ggnew), slot 2 (ene)
-  const double s0 = wave_sum(gn[0]), s1 = wave_sum(gn[1]);
-  const double e0 = wave_sum(en[0]), e1 = wave_sum(en[1]);
-  if (threadIdx.x == 0) {
-    part[w.blk * 2] = s0;
-    part[w.blk * 2 + 1] = s1;
-    part[2 * sv.nblk + w.blk * 2] = e0;
-    part[2 * sv.nblk + w.blk * 2 + 1] = e1;
-  }
-  if (last_block(&sv.st->counter[1], sv.nblk)) {
-    const double a0 = sum_partials(part, sv.nblk, 0), a1 = sum_partials(part, sv.nblk, 1);
-    const double b0 = sum_partials(part + 2 * sv.nblk, sv.nblk, 0);
-    const double b1 = sum_partials(part + 2 * sv.nblk, sv.nblk, 1);
-    if (threadIdx.x == 0) {
-      sv.st->ggnew[0] = a0;
-      sv.st->ggnew[1] = a1;
-      sv.st->ene[0] = b0;
-      sv.st->ene[1] = b1;
-    }
-  }
-}
-
-// pcg_newDirection + convergence (newinduce_pme2.f:543-568):
-// pp = zr + ggnew/ggold pp; last block: ggold = ggnew, residual norm
-// sqrt(ggnew/(3 npolar)), stop when max over rhs < poleps.
-__global__ void __launch_bounds__(64) k_cg_update2(const SysView* views, const Work* work) {
+__global__ void __launch_bounds__(256) k_cg_update1(const SysView* views, const Work* work, int it) {
   const Work w = work[blockIdx.x];
   const SysView& sv = views[w.sys];
   SolveState* st = sv.st;
   if (st->conv) return;
-  const int i = w.blk * kBlock + threadIdx.x;
+  const int G = sv.red_blocks, P = sv.gat_blocks;
+  __shared__ double sc[4];
+  __shared__ double scratch[8];
+  {
+    const double* po = it == 1 ? part_init(sv) : part_rz(sv, (it - 1) & 1);
+    const int npo = it == 1 ? P : G;
+    const double* pg = part_gg(sv);
+    const double go0 = block_sum_partials(po, npo, 0, scratch), go1 = block_sum_partials(po, npo, 1, scratch);
+    const double g0 = block_sum_partials(pg, P, 0, scratch), g1 = block_sum_partials(pg, P, 1, scratch);
+    if (threadIdx.x == 0) {
+      sc[0] = go0 / g0;
+      sc[1] = go1 / g1;
+      sc[2] = g0;
+      sc[3] = g1;
+      if (w.blk == 0) {
+        st->gg[0] = g0;
+        st->gg[1] = g1;
+        if (g0 <= 0.0 || g1 <= 0.0) st->n_indef += 1;
+      }
+    }
+  }
+  __syncthreads();
+  if (sc[2] == 0.0 || sc[3] == 0.0) {  // the reference returns before updating mu
+    if (threadIdx.x == 0) st->conv = kZeroCurv;
+    return;
+  }
+  const double alpha[2] = {sc[0], sc[1]};
   const size_t np = sv.npad;
-  const double beta[2] = {st->ggnew[0] / st->ggold[0], st->ggnew[1] / st->ggold[1]};
-  if (i < sv.n) {
+  const int n = sv.n;
+  GLOBAL double* MU = gw(sv.mu);
+  GLOBAL double* RES = gw(sv.res);
+  GLOBAL double* ZR = gw(sv.zr);
+  const GLOBAL double* PP = gp(sv.pp);
+  const GLOBAL double* H = gp(sv.h);
+  const GLOBAL double* EF = gp(sv.ef);
+  const GLOBAL double* POL = gp(sv.pol);
+  double gn[2] = {0, 0}, en[2] = {0, 0};
+  for (int i = w.blk * blockDim.x + threadIdx.x; i < n; i += G * blockDim.x) {
+    const double pl = POL[i];
+    const double diag = pl == 0.0 ? kTinyPol : pl;
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       const size_t o = (size_t)c * np + i;
-      sv.pp[o] = sv.zr[o] + beta[c / 3] * sv.pp[o];
+      const double a = alpha[c / 3];
+      const double m = MU[o] + a * PP[o];
+      const double r = RES[o] - a * H[o];
+      const double z = diag * r;
+      MU[o] = m;
+      RES[o] = r;
+      ZR[o] = z;
+      gn[c / 3] += r * z;
+      en[c / 3] += -0.5 * (m * (r + EF[o]));
     }
   }
-  if (last_block(&st->counter[2], sv.nblk)) {
-    if (threadIdx.x == 0) {
-      const double g0 = st->ggnew[0], g1 = st->ggnew[1];
-      st->ggold[0] = g0;
-      st->ggold[1] = g1;
-      st->niter += 1;
-      const double n3 = 3.0 * double(sv.n);
-      const double rn = fmax(sqrt(g0 / n3), sqrt(g1 / n3));
-      st->resnrm = rn;
-      if (!isfinite(g0) || !isfinite(g1)) st->conv = kNonFinite;
-      else if (rn < st->poleps) st->conv = kConverged;
-      else if (st->niter >= st->politer) st->conv = kMaxIter;
-    }
-  }
+  block_partial2(gn[0], gn[1], part_rz(sv, it & 1), w.blk);
+  block_partial2(en[0], en[1], part_ene(sv), w.blk);
 }
 
-// E_pol = -1/2 f sum mu_d . E_p (epolar0c: real + recip + self)
-__global__ void __launch_bounds__(64) k_energy(const SysView* views, const Work* work) {
+// pcg_newDirection + convergence (newinduce_pme2.f:543-568) for iteration
+// `it` (only its parity and whether it is the first matter, so captured
+// launches can be replayed): pp = zr + ggnew/ggold pp; residual norm sqrt(ggnew/(3 npolar)),
+// stop when the max over both rhs is below poleps.
+__global__ void __launch_bounds__(256) k_cg_update2(const SysView* views, const Work* work, int it) {
   const Work w = work[blockIdx.x];
   const SysView& sv = views[w.sys];
-  const int i = w.blk * kBlock + threadIdx.x;
+  SolveState* st = sv.st;
+  if (st->conv) return;
+  const int G = sv.red_blocks;
+  __shared__ double sc[3];
+  __shared__ double scratch[8];
+  {
+    const double* pn = part_rz(sv, it & 1);
+    const double* po = it == 1 ? part_init(sv) : part_rz(sv, (it - 1) & 1);
+    const int npo = it == 1 ? sv.gat_blocks : G;
+    const double gn0 = block_sum_partials(pn, G, 0, scratch), gn1 = block_sum_partials(pn, G, 1, scratch);
+    const double go0 = block_sum_partials(po, npo, 0, scratch), go1 = block_sum_partials(po, npo, 1, scratch);
+    if (threadIdx.x == 0) {
+      sc[0] = gn0 / go0;
+      sc[1] = gn1 / go1;
+      const double n3 = 3.0 * double(sv.n);
+      const double rn = fmax(sqrt(gn0 / n3), sqrt(gn1 / n3));
+      int conv = kRunning;
+      if (!isfinite(gn0) || !isfinite(gn1)) conv = kNonFinite;
+      else if (rn < st->poleps) conv = kConverged;
+      sc[2] = conv;
+      if (w.blk == 0) {  // the only writer (and reader) of these fields
+        st->ggnew[0] = gn0;
+        st->ggnew[1] = gn1;
+        st->resnrm = rn;
+        st->niter += 1;
+      }
+    }
+  }
+  __syncthreads();
+  const double beta[2] = {sc[0], sc[1]};
+  const size_t np = sv.npad;
+  const int n = sv.n;
+  GLOBAL double* PP = gw(sv.pp);
+  const GLOBAL double* ZR = gp(sv.zr);
+  for (int i = w.blk * blockDim.x + threadIdx.x; i < n; i += G * blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const size_t o = (size_t)c * np + i;
+      PP[o] = ZR[o] + beta[c / 3] * PP[o];
+    }
+  }
+  // every block computed the same decision; the flag is read by the next
+  // iteration's kernels (a block of this kernel that sees it set skips the
+  // direction update, which is no longer used)
+  if (threadIdx.x == 0 && sc[2] != kRunning) st->conv = int(sc[2]);
+}
+
+// E_pol = -1/2 f sum mu_d . E_p (epolar0c: real + recip + self); partials
+// into the energy buffer, summed by k_finish.
+__global__ void __launch_bounds__(256) k_energy(const SysView* views, const Work* work) {
+  const Work w = work[blockIdx.x];
+  const SysView& sv = views[w.sys];
   const size_t np = sv.npad;
   double e = 0.0;
-  if (i < sv.n) {
+  for (int i = w.blk * blockDim.x + threadIdx.x; i < sv.n; i += sv.red_blocks * blockDim.x) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) e += sv.mu[(size_t)c * np + i] * sv.ef[(size_t)(3 + c) * np + i];
   }
-  const double s = wave_sum(e);
-  double* part = sv.partial + (size_t)6 * sv.nblk;  // slot 3
+  block_partial2(e, 0.0, part_ep(sv), w.blk);
+}
+
+// one wave per system: final scalars into the SolveState
+__global__ void __launch_bounds__(64) k_finish(const SysView* views, const int* sys_ids) {
+  const SysView& sv = views[sys_ids[blockIdx.x]];
+  const double e = sum_partials(part_ep(sv), sv.red_blocks, 0);
+  const double e0 = sum_partials(part_ene(sv), sv.red_blocks, 0), e1 = sum_partials(part_ene(sv), sv.red_blocks, 1);
   if (threadIdx.x == 0) {
-    part[w.blk * 2] = s;
-    part[w.blk * 2 + 1] = 0.0;
-  }
-  if (last_block(&sv.st->counter[3], sv.nblk)) {
-    const double a = sum_partials(part, sv.nblk, 0);
-    if (threadIdx.x == 0) sv.st->ep = -0.5 * kCoulomb * a;
+    sv.st->ep = -0.5 * kCoulomb * e;
+    sv.st->ene[0] = e0;
+    sv.st->ene[1] = e1;
   }
 }
 
