@@ -142,6 +142,15 @@ thp_status thp_tmatxb_dev(thp_sys* sys, const double* mu, double* efi, int dodia
 thp_status thp_induce_dev(thp_sys* sys, const thp_solve_opts* opts, double* mu_dev /* [n][2][3] */,
                           double* ep, thp_solve_stats* stats);
 
+/* ------------------------------------------------------------ measurement */
+/* Measurement hook, no reference counterpart: average device time (ms) of
+ * one launch of a path kernel over `reps` back-to-back launches, timed with
+ * HIP events on the library's stream.  kernel: 0 = real-space matvec
+ * (tmatxb), 1 = induced-dipole spreading, 2 = induced-dipole gather.  Also
+ * returns the number of unscaled pairs within the cutoff (i<k), the unit
+ * of the matvec's algorithmic FLOP count. */
+thp_status thp_time_kernel(thp_sys* sys, int kernel, int reps, double* ms_per_launch, int64_t* pairs_in_cut);
+
 /* --------------------------------------------------------------- serving */
 typedef struct {
   int max_systems;            /* resident systems (slots) */

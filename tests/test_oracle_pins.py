@@ -1,0 +1,241 @@
+"""Pin the FP64 CPU oracle against mathematics it does not restate.
+
+Each check is chosen so that a plausible mistake in the oracle (a dropped
+term, a wrong sign or index, a transposed operand, a wrong normalisation)
+fails: closed forms, finite-difference identities, a brute-force neighbor
+search, the textbook Ewald structure-factor sum, alpha-invariance of the
+full Ewald field, a dense linear solve and the energy decomposition.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import ewald_ref as E
+import thp
+from oracle import oracle as O
+
+
+# ------------------------------------------------------------ B-splines
+@pytest.mark.parametrize("w", [0.0, 1e-9, 0.1234, 0.5, 0.77, 1.0 - 1e-12])
+def test_bsplgen_matches_cardinal_bspline(w):
+    # bsplgen (pmestuff.f:67-156): values and 3 derivatives at the 5 nodes
+    np.testing.assert_allclose(O.bsplgen(w), E.bspline_nodes(w), rtol=0, atol=1e-13)
+
+
+def test_bsplgen_partition_of_unity_and_symmetry():
+    for w in np.linspace(0, 0.999, 37):
+        th = O.bsplgen(w)
+        assert abs(th[:, 0].sum() - 1.0) < 1e-14       # sum of values = 1
+        assert abs(th[:, 1].sum()) < 1e-14             # sum of first derivatives = 0
+        th2 = O.bsplgen(1.0 - w) if w > 0 else None
+        if th2 is not None:                            # mirror symmetry
+            np.testing.assert_allclose(th[::-1, 0], th2[:, 0], atol=1e-14)
+
+
+# ----------------------------------------------------------- PME moduli
+@pytest.mark.parametrize("K", [18, 24, 64, 108])
+def test_bsmod_raw_is_dft_modulus(K):
+    # moduli/dftmod (kewald.f:672-770) before the zeta factor: |DFT of the
+    # spline samples|^2; the samples M5(4..0) sit at indices 1..5
+    samples = np.zeros(K)
+    for j in range(1, 6):
+        samples[j] = E.cardinal_bspline(5 - j, 5)
+    ref = np.abs(np.fft.fft(samples)) ** 2
+    got = O.bsmod(K, zeta=False)
+    tiny = ref < 1e-7
+    np.testing.assert_allclose(got[~tiny], ref[~tiny], rtol=1e-12)
+    # Euler exponential-spline fix (kewald.f:768-772): vanishing moduli are
+    # replaced by the mean of their neighbours (odd order, even K: Nyquist)
+    for i in np.nonzero(tiny)[0]:
+        assert abs(got[i] - 0.5 * (ref[i - 1] + ref[(i + 1) % K])) < 1e-14
+
+
+@pytest.mark.parametrize("K", [24, 64])
+def test_bsmod_zeta_factor_closed_form(K):
+    raw, full = O.bsmod(K, zeta=False), O.bsmod(K)
+    for i in range(K):
+        k = i if i < K // 2 else i - K     # dftmod's wave index (kewald.f:786-788)
+        if abs(k) == K // 2:
+            continue                       # S(5) = 0 exactly at the Nyquist index
+        z = E.zeta_closed(k, K, 5)
+        assert abs(full[i] / raw[i] - z * z) < 1e-8 * z * z
+
+
+# ------------------------------------------------------------------ FFT
+def test_fft3d_matches_numpy():
+    rng = np.random.default_rng(3)
+    g = rng.normal(size=(6, 10, 8)) + 1j * rng.normal(size=(6, 10, 8))
+    np.testing.assert_allclose(O.fft3d(g, -1), np.fft.fftn(g), rtol=0, atol=1e-11)
+    np.testing.assert_allclose(O.fft3d(g, +1), np.fft.ifftn(g) * g.size, rtol=0, atol=1e-11)
+
+
+# -------------------------------------------------------------- damping
+@pytest.mark.parametrize("r", [0.9, 2.5, 4.0, 6.9])
+def test_dampewald_recursion_is_the_radial_derivative(r):
+    # dampewald (damping.f:17-56): bn_{k} = -(1/r) d/dr bn_{k-1}, bn_0 = erfc(a r)/r
+    a = 0.4
+    b = O.dampewald(a, r)
+    assert abs(b[0] - math.erfc(a * r) / r) < 1e-15
+    h = 1e-3 * r
+    for k, idx in [(1, 2), (2, 4), (3, 6)]:
+        prev = lambda x: O.dampewald(a, x)[2 * (k - 1)]  # noqa: E731
+        d1 = (prev(r + h) - prev(r - h)) / (2 * h)
+        d2 = (prev(r + h / 2) - prev(r - h / 2)) / h
+        fd = -(4 * d2 - d1) / 3 / r  # Richardson: O(h^4)
+        assert abs(b[idx] - fd) < 1e-8 * abs(b[idx])
+
+
+def test_thole_lambdas_follow_the_field_hierarchy():
+    # lambda_{k+2} = lambda_k - r/k d lambda_k/dr for a radially smeared
+    # interaction; pins damptholed's lambda5/lambda7 (damping.f:142-215)
+    pdi, pdk, a = 0.837 ** (1 / 6), 0.496 ** (1 / 6), 0.39
+    for r in [0.6, 0.96, 1.5, 2.2, 3.0]:
+        lam = lambda x: O.dampthole(pdi, pdk, a, x, 7)  # noqa: E731
+        l3, l5, l7 = lam(r)[2], lam(r)[4], lam(r)[6]
+        h = 1e-5
+        d3 = (lam(r + h)[2] - lam(r - h)[2]) / (2 * h)
+        d5 = (lam(r + h)[4] - lam(r - h)[4]) / (2 * h)
+        assert abs(l5 - (l3 - r / 3 * d3)) < 1e-8
+        assert abs(l7 - (l5 - r / 5 * d5)) < 1e-8
+        # lambda3 is the charge of Thole's density rho(u) = 3a/(4 pi) exp(-a u^3)
+        # inside the scaled radius u = r/(alpha_i alpha_k)^(1/6)
+        u = r / (pdi * pdk)
+        s = np.linspace(0, u, 20001)
+        enc = np.trapezoid(4 * np.pi * s ** 2 * 3 * a / (4 * np.pi) * np.exp(-a * s ** 3), s)
+        assert abs(l3 - enc) < 1e-7
+
+
+# --------------------------------------------------------- neighbor list
+def brute_pairs(s, mb):
+    """O(N^2) enumeration with the reference's operation order:
+    xr = xi - xk, orthogonal image, r2 = (x*x + y*y) + z*z, r2 <= mb^2."""
+    x, y, z = s.x, s.y, s.z
+    L = s.box
+    out = []
+    for i in range(s.n - 1):
+        d = [x[i] - x[i + 1:], y[i] - y[i + 1:], z[i] - z[i + 1:]]
+        for c in range(3):
+            v = d[c]
+            big = np.abs(v) > 0.5 * L[c]
+            cel = np.copysign(L[c], v)
+            v2 = np.where(big, v - cel * np.floor(np.abs(v) / L[c]), v)
+            big2 = big & (np.abs(v2) > 0.5 * L[c])
+            d[c] = np.where(big2, v2 - np.copysign(L[c], v2), v2)
+        r2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]
+        ks = np.nonzero(r2 <= mb * mb)[0] + i + 1
+        out.extend((i, k) for k in ks)
+    return np.array(out, dtype=np.int32).reshape(-1, 2)
+
+
+def test_nblist_equals_brute_force_pair_set():
+    s = thp.water_1944()
+    ref = brute_pairs(s, s.cut + s.buffer)
+    got = O.Oracle(s).nblist_pairs()
+    assert len(got) == len(ref) == 891263
+    assert np.array_equal(got, ref)
+
+
+# ------------------------------------------------ Ewald: textbook reference
+def _few_site_system(seed, K):
+    """eight sites with charges, dipoles and quadrupoles, no exclusions"""
+    return thp.make_system(8, 16.0, n_solute=8, blob_radius=4.0, seed=seed, nfft=(K, K, K),
+                           cut=7.9, buffer=0.05)
+
+
+def test_pme_recip_field_matches_structure_factor_sum():
+    # efld0_recip (newinduce_pme.f:1671-1917) against the Ewald sum over
+    # reciprocal vectors; the difference is PME interpolation error, so it
+    # must be small on a fine grid and shrink fast under refinement (a sign
+    # or factor error in any moment would not)
+    err = {}
+    for K in (48, 96):
+        s = _few_site_system(5, K)
+        cphi = O.Oracle(s).efld0_recip()
+        pos = np.stack([s.x, s.y, s.z], 1)
+        q, d, Q = E.rpole_to_parts(s.rpole)
+        ref = E.recip_grad_multipoles(pos, q, d, Q, s.box, s.aewald, mmax=14)
+        err[K] = np.abs(cphi[:, 1:4] - ref).max() / np.abs(ref).max()
+    assert err[96] < 1e-5 and err[48] / err[96] > 8, err
+
+
+def test_pme_recip_dipole_field_matches_structure_factor_sum():
+    # tmatxbrecip (newinduce_pme.f:1919-2128), d and p grids separately
+    rng = np.random.default_rng(2)
+    err = {}
+    for K in (48, 96):
+        s = _few_site_system(7, K)
+        mu = rng.normal(size=(s.n, 2, 3)) * 0.2 if K == 48 else mu  # noqa: F821
+        got = O.Oracle(s).tmatxbrecip(mu)
+        pos = np.stack([s.x, s.y, s.z], 1)
+        zq, zQ = np.zeros(s.n), np.zeros((s.n, 3, 3))
+        e = 0.0
+        for rhs in range(2):
+            ref = E.recip_grad_multipoles(pos, zq, mu[:, rhs], zQ, s.box, s.aewald, mmax=14)
+            e = max(e, np.abs(got[:, rhs] - ref).max() / np.abs(ref).max())
+        err[K] = e
+    assert err[96] < 2e-6 and err[48] / err[96] > 8, err
+
+
+def _alpha_case():
+    # twelve sites in a 20 A box, cutoff 9.9 A, 64^3 grid (0.31 A spacing)
+    return thp.make_system(12, 20.0, n_solute=12, blob_radius=5.0, seed=11, nfft=(64, 64, 64),
+                           cut=9.9, buffer=0.05)
+
+
+def test_total_permanent_field_is_alpha_invariant():
+    # direct + reciprocal + self (newinduce_pme2.f:111-147) does not depend
+    # on the Ewald coefficient; the residual spread is PME interpolation
+    # error (~1e-4 here), while a wrong self term alone would move the
+    # field by ~1e-1 of its scale between these alphas
+    s = _alpha_case()
+    fields = [O.Oracle(s, aewald=a).efld0_total() for a in (0.40, 0.45, 0.50)]
+    scale = np.abs(fields[0]).max()
+    for f in fields[1:]:
+        assert np.abs(f - fields[0]).max() < 3e-4 * scale
+
+
+def test_total_dipole_matvec_is_alpha_invariant():
+    # tmatxb + tmatxbrecip - self (newinduce_pme2.f:406-424)
+    s = _alpha_case()
+    rng = np.random.default_rng(4)
+    mu = rng.normal(size=(s.n, 2, 3)) * 0.3
+    outs = [O.Oracle(s, aewald=a).matvec(mu) for a in (0.40, 0.45, 0.50)]
+    scale = np.abs(outs[0]).max()
+    for o in outs[1:]:
+        assert np.abs(o - outs[0]).max() < 1e-5 * scale
+
+
+# ----------------------------------------------------- solver and energy
+def test_pcg_solution_matches_dense_solve():
+    s = thp.tiny_water(27, 20.3, 24)
+    orc = O.Oracle(s)
+    n6 = 6 * s.n
+    A = np.zeros((n6, n6))
+    e = np.zeros(n6)
+    for j in range(n6):
+        e[:] = 0.0
+        e[j] = 1.0
+        A[:, j] = orc.matvec(e.reshape(s.n, 2, 3)).reshape(-1)
+    assert np.abs(A - A.T).max() < 1e-12 * np.abs(A).max()       # (alpha^-1 - T) symmetric
+    assert np.linalg.eigvalsh(0.5 * (A + A.T)).min() > 0          # positive definite here
+    res = orc.induce(poleps=1e-12, politer=200)
+    mu = np.linalg.solve(A, res["ef"].reshape(-1)).reshape(s.n, 2, 3)
+    assert np.abs(res["mu"] - mu).max() < 1e-9 * np.abs(mu).max()
+
+
+def test_energy_identity_matches_pairwise_decomposition():
+    s = thp.tiny_water(27, 20.3, 24)
+    orc = O.Oracle(s)
+    res = orc.induce(poleps=1e-10)
+    er, erc, es = orc.epolar_decomp(res["mu"][:, 0])
+    total = er + erc + es
+    assert abs(total - res["ep"]) < 1e-10 * abs(res["ep"])
+
+
+def test_cg_energy_functional_decreases():
+    s = thp.water_1944()
+    res = O.Oracle(s).induce(poleps=1e-8)
+    e = res["ene_hist"][:, 0]
+    assert np.all(np.diff(e) <= 1e-9 * np.abs(e).max())
+    assert res["status"] == 0 and res["niter"] == 22

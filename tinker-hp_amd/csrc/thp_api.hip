@@ -211,7 +211,10 @@ std::vector<int> spatial_order(int n, const double* x, const double* y, const do
   std::vector<double> w((size_t)3 * n);
   for (int i = 0; i < n; ++i) {
     const double p[3] = {x[i], y[i], z[i]};
-    for (int d = 0; d < 3; ++d) w[3 * i + d] = p[d] - box[d] * std::floor(p[d] / box[d]);
+    for (int d = 0; d < 3; ++d) {
+      const double v = p[d] - box[d] * std::floor(p[d] / box[d]);
+      w[3 * i + d] = std::isfinite(v) ? v : 0.0;  // ordering only; keeps the comparator a strict order
+    }
   }
   std::vector<int> perm(n);
   std::iota(perm.begin(), perm.end(), 0);
@@ -812,8 +815,10 @@ thp_status thp_system_update_coords(thp_sys* s, const double* x, const double* y
 
 thp_status thp_nblist(thp_sys* s, int force_rebuild) {
   if (!s) return THP_ERR_ARG;
-  if (!s->list_valid || force_rebuild) RET(build_block_lists(s));
-  if (s->verlet_valid) return THP_OK;
+  if (s->verlet_valid && !force_rebuild) return THP_OK;
+  // the Verlet list is the pair set of the current coordinates (mlistcell):
+  // the block candidates are rebuilt from them first
+  RET(build_block_lists(s));
   // per-atom Verlet list (mlistcell): every k with r^2 <= (cut+buffer)^2
   hipStream_t st = s->ctx->stream;
   hipLaunchKernelGGL((k_nblist_atoms<false>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work, (const int*)s->bcount, (const int*)s->blist, s->maxbl);
@@ -1074,6 +1079,49 @@ thp_status thp_induce(thp_sys* s, const thp_solve_opts* o, double* uind, double*
     }
   if (ep) *ep = sp->ep;
   return sp->status == THP_ERR_NAN ? fail(s->ctx, THP_ERR_NAN, "non-finite residual") : THP_OK;
+}
+
+thp_status thp_time_kernel(thp_sys* s, int kernel, int reps, double* ms, int64_t* pairs) {
+  if (!s || !ms || reps <= 0 || kernel < 0 || kernel > 2) return THP_ERR_ARG;
+  RET(ensure_lists(s));
+  RET(ensure_pme(s));
+  hipStream_t st = s->ctx->stream;
+  auto launch = [&]() -> thp_status {
+    if (kernel == 0) return pair_tmatxb(s, 1, 3, 1, 0);
+    if (kernel == 1) return spread_uind(s, 1, 0, 0);
+    return launch_gather_uind<0>(s, 1, 3, 0);
+  };
+  RET(launch());  // warm
+  HIPCHK(s->ctx, hipEventRecord(s->ev[0], st));
+  for (int r = 0; r < reps; ++r) RET(launch());
+  HIPCHK(s->ctx, hipEventRecord(s->ev[1], st));
+  HIPCHK(s->ctx, hipEventSynchronize(s->ev[1]));
+  float t = 0;
+  HIPCHK(s->ctx, hipEventElapsedTime(&t, s->ev[0], s->ev[1]));
+  *ms = double(t) / reps;
+  if (pairs) {
+    // unscaled pairs within the cutoff, from the block-union lists
+    int64_t cnt = 0;
+    std::vector<int> lc(s->npad);
+    HIPCHK(s->ctx, hipMemcpy(lc.data(), s->view.lcount, sizeof(int) * s->npad, hipMemcpyDeviceToHost));
+    std::vector<unsigned short> ll((size_t)s->view.lmax * s->npad);
+    std::vector<int> ul((size_t)s->nblk * s->view.umax);
+    HIPCHK(s->ctx, hipMemcpy(ll.data(), s->view.llist, sizeof(unsigned short) * ll.size(), hipMemcpyDeviceToHost));
+    HIPCHK(s->ctx, hipMemcpy(ul.data(), s->view.ulist, sizeof(int) * ul.size(), hipMemcpyDeviceToHost));
+    const double c2 = s->cut * s->cut;
+    for (int q = 0; q < s->n; ++q) {
+      const int b = q / kBlock;
+      for (int t2 = 0; t2 < lc[q]; ++t2) {
+        const int k = ul[(size_t)b * s->view.umax + ll[(size_t)t2 * s->npad + q]];
+        if (k <= q) continue;
+        double d[3] = {s->hx[k] - s->hx[q], s->hy[k] - s->hy[q], s->hz[k] - s->hz[q]};
+        for (int c = 0; c < 3; ++c) d[c] -= s->box[c] * std::rint(d[c] / s->box[c]);
+        if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] <= c2) ++cnt;
+      }
+    }
+    *pairs = cnt;
+  }
+  return THP_OK;
 }
 
 thp_status thp_induce_dev(thp_sys* s, const thp_solve_opts* o, double* mu_dev, double* ep, thp_solve_stats* stats) {

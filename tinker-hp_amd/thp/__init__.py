@@ -124,6 +124,7 @@ def load_lib(path: str | None = None) -> C.CDLL:
         "thp_serve_poll": (C.c_int, [vp, C.c_uint64, C.POINTER(Result), C.c_double]),
         "thp_serve_get_dipoles": (C.c_int, [vp, C.c_int, _dp]),
         "thp_serve_stop": (C.c_int, [vp]),
+        "thp_time_kernel": (C.c_int, [vp, C.c_int, C.c_int, _dp, C.POINTER(C.c_int64)]),
         "thp_make_system": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_double, C.c_uint64,
                                       _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _dp]),
         "thp_pick_grid": (C.c_int, [C.c_double]),
@@ -386,3 +387,11 @@ class Solver:
         return mu, ep.value, st
 
     induce = newinduce_pme2
+
+    def time_kernel(self, kernel: int, reps: int = 20):
+        """(ms per launch, unscaled pairs within the cutoff) of a path kernel:
+        0 = real-space matvec, 1 = spreading, 2 = gather."""
+        ms = C.c_double(0)
+        pairs = C.c_int64(0)
+        self._c(self.lib.thp_time_kernel(self.h, kernel, reps, C.byref(ms), C.byref(pairs)), "time_kernel")
+        return ms.value, pairs.value
