@@ -1,0 +1,159 @@
+"""Benchmark: AMOEBA polarization solves/s on the 23,559-atom water box.
+
+One step = one polarization solve (permanent field: direct + reciprocal +
+self; PCG to poleps = 1e-8 in the reference criterion; E_pol) of a
+resident system.  Legs:
+  plain  - FP64 pair/grid/FFT, one system per launch, a host read of the
+           convergence flag after every iteration (the reference's pattern);
+  mixed  - FP32 pair/grid/FFT with FP64 accumulation and CG scalars;
+  serve  - the batched serving loop (mixed), when libthp provides it.
+`value` is the fastest leg's solves/s.  The roofline object is measured live
+on the dominant kernel (the real-space matvec) with HIP events on the
+library's stream; cpu_baseline times the FP64 CPU oracle on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serve]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "tinker-hp_amd")]
+
+METRIC = "AMOEBA polarization solves/s at 24k atoms; batched p50/p99 latency vs plain"
+FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
+FLOP_PER_PAIR_TMATXB = 120  # real-space matvec, per in-cutoff pair, both directions, 2 rhs
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_leg(ctx, thp, system, precision, steps, warmup, poleps):
+    solver = thp.Solver(ctx, system, precision)
+    solver.nblist()
+    for _ in range(warmup):
+        solver.newinduce_pme2(poleps=poleps)
+    t0 = time.perf_counter()
+    last = None
+    for _ in range(steps):
+        last = solver.newinduce_pme2(poleps=poleps)
+    dt = time.perf_counter() - t0
+    return solver, dt / steps, last
+
+
+def cpu_baseline(system, poleps):
+    from oracle import oracle as O
+    cores = max(1, min(16, os.cpu_count() or 1))
+    O.set_threads(cores)
+    orc = O.Oracle(system)
+    orc.nblist_pairs()
+    t0 = time.perf_counter()
+    res = orc.induce(poleps=poleps)
+    dt = time.perf_counter() - t0
+    return res, dt, cores
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--poleps", type=float, default=1e-8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import thp
+
+    system = thp.water_23559()
+    log(f"bench: workload {system.name} n={system.n} box={system.box[0]} nfft={system.nfft}")
+    ctx = thp.Context(0)
+
+    legs = {}
+    plain, t_plain, r_plain = time_leg(ctx, thp, system, thp.THP_PREC_FP64, args.steps, args.warmup, args.poleps)
+    legs["plain"] = dict(solves_per_s=1.0 / t_plain, ms_per_solve=1e3 * t_plain, niter=r_plain[2].niter, ep=r_plain[1])
+    log(f"bench: plain fp64 {legs['plain']}")
+    mixed, t_mixed, r_mixed = time_leg(ctx, thp, system, thp.THP_PREC_MIXED, args.steps, args.warmup, args.poleps)
+    legs["mixed"] = dict(solves_per_s=1.0 / t_mixed, ms_per_solve=1e3 * t_mixed, niter=r_mixed[2].niter, ep=r_mixed[1])
+    log(f"bench: mixed {legs['mixed']}")
+
+    serve = None
+    if not args.no_serve:
+        try:
+            import thp.serve as tserve
+            serve = tserve.bench_serving(ctx, system, steps=max(args.steps, 8), warmup=args.warmup,
+                                         poleps=args.poleps)
+            legs["serve"] = serve
+            log(f"bench: serve {serve}")
+        except (ImportError, AttributeError, thp.ThpError) as e:
+            log(f"bench: serving leg unavailable: {e}")
+
+    # live roofline of the dominant kernel (mixed real-space matvec)
+    ms, pairs = mixed.time_kernel(0, reps=30)
+    flops = FLOP_PER_PAIR_TMATXB * pairs
+    achieved = flops / (ms * 1e-3) / 1e12
+    roofline = dict(bound="mfma", achieved=round(achieved, 4), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=round(achieved / FP32_PEAK_TFLOPS, 5), traffic=None,
+                    kernel="k_pair_union<float,tmatxb>", ms_per_launch=round(ms, 5), pairs_in_cut=int(pairs),
+                    flop_per_pair=FLOP_PER_PAIR_TMATXB, compute="FP32 VALU (no MFMA: no dense contraction)")
+    log(f"bench: roofline {roofline}")
+
+    cpu = None
+    err = {}
+    if not args.no_cpu:
+        res, dt, cores = cpu_baseline(system, args.poleps)
+        cpu = dict(value=round(1.0 / dt, 4), unit="solves/s", cores=cores, kind="port",
+                   sample=f"1 FP64 solve of {system.name} (oracle, OpenMP {cores} threads, list build excluded)",
+                   seconds=round(dt, 3))
+        for name, r in (("plain", r_plain), ("mixed", r_mixed)):
+            err[name] = dict(
+                ep_rel=abs(r[1] - res["ep"]) / abs(res["ep"]),
+                mu_rms_debye=float(np.sqrt(np.mean((r[0] - res["mu"]) ** 2)) * thp.DEBYE))
+        log(f"bench: cpu {cpu} errors {err}")
+
+    best = max(legs, key=lambda k: legs[k]["solves_per_s"])
+    value = legs[best]["solves_per_s"]
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 / value, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/f64" if best != "plain" else "f64",
+        "data": "synthetic (seeded water recipe)",
+        "config": {
+            "workload": f"{system.name}: 7,853 waters, 62.23 A box, PME 64^3 order 5, alpha 0.4, cutoff 7 A, "
+                        f"PCG to 1e-8",
+            "leg": best,
+            "plain_fp64_solves_per_s": round(legs["plain"]["solves_per_s"], 3),
+            "mixed_solves_per_s": round(legs["mixed"]["solves_per_s"], 3),
+            "speedup_vs_plain": round(value / legs["plain"]["solves_per_s"], 3),
+            "p50_ms": serve["p50_ms"] if serve else None,
+            "p99_ms": serve["p99_ms"] if serve else None,
+            "niter": legs[best]["niter"],
+            "errors_vs_fp64_oracle": err or None,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    mixed.close()
+    plain.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -16,9 +16,7 @@
 
 using namespace thp;
 
-namespace {
-
-thp_status fail(thp_ctx* ctx, thp_status code, const char* fmt, ...) {
+thp_status thp::fail(thp_ctx* ctx, thp_status code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -27,6 +25,10 @@ thp_status fail(thp_ctx* ctx, thp_status code, const char* fmt, ...) {
   if (ctx) ctx->err = buf;
   return code;
 }
+
+size_t thp::grid_points(const thp_sys* s) { return (size_t)s->nfft[0] * s->nfft[1] * s->nfft[2]; }
+
+namespace {
 
 #define HIPCHK(ctx, call)                                                                 \
   do {                                                                                   \
@@ -231,6 +233,7 @@ int pick_brick(int K) {
 }
 
 thp_status upload_view(thp_sys* s) {
+  ++s->view_version;
   HIPCHK(s->ctx, hipMemcpyAsync(s->d_view, &s->view, sizeof(SysView), hipMemcpyHostToDevice, s->ctx->stream));
   return THP_OK;
 }
@@ -258,28 +261,23 @@ thp_status launch_check(thp_ctx* ctx, const char* what) {
 }
 
 template <typename real, int KIND>
-void launch_pair(thp_sys* s, int vin, int vout, int dodiag, int check_conv) {
-  hipStream_t st = s->ctx->stream;
-  switch (s->segs) {
-    case 1: hipLaunchKernelGGL((k_pair_union<real, KIND, 1>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work, vin, vout, dodiag, check_conv); break;
-    case 2: hipLaunchKernelGGL((k_pair_union<real, KIND, 2>), g1(s->nblk), dim3(128), 0, st, s->d_view, s->d_work, vin, vout, dodiag, check_conv); break;
-    default: hipLaunchKernelGGL((k_pair_union<real, KIND, 4>), g1(s->nblk), dim3(256), 0, st, s->d_view, s->d_work, vin, vout, dodiag, check_conv); break;
+void launch_pair(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int check_conv) {
+  hipStream_t st = ctx->stream;
+  const dim3 g = g1(b.n_blk);
+  switch (b.segs) {
+    case 1: hipLaunchKernelGGL((k_pair_union<real, KIND, 1>), g, dim3(64), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
+    case 2: hipLaunchKernelGGL((k_pair_union<real, KIND, 2>), g, dim3(128), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
+    case 8: hipLaunchKernelGGL((k_pair_union<real, KIND, 8>), g, dim3(512), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
+    case 16: hipLaunchKernelGGL((k_pair_union<real, KIND, 16>), g, dim3(1024), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
+    default: hipLaunchKernelGGL((k_pair_union<real, KIND, 4>), g, dim3(256), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
   }
 }
 
-thp_status pair_tmatxb(thp_sys* s, int vin, int vout, int dodiag, int check_conv) {
-  if (s->prec == THP_PREC_FP64) launch_pair<double, 1>(s, vin, vout, dodiag, check_conv);
-  else launch_pair<float, 1>(s, vin, vout, dodiag, check_conv);
-  return launch_check(s->ctx, "k_pair_union<tmatxb>");
+thp_status pair_efld0(thp_ctx* ctx, const Batch& b) {
+  if (b.prec == THP_PREC_FP64) launch_pair<double, 0>(ctx, b, 0, 0, 0, 0);
+  else launch_pair<float, 0>(ctx, b, 0, 0, 0, 0);
+  return launch_check(ctx, "k_pair_union<efld0>");
 }
-
-thp_status pair_efld0(thp_sys* s) {
-  if (s->prec == THP_PREC_FP64) launch_pair<double, 0>(s, 0, 0, 0, 0);
-  else launch_pair<float, 0>(s, 0, 0, 0, 0);
-  return launch_check(s->ctx, "k_pair_union<efld0>");
-}
-
-size_t grid_points(const thp_sys* s) { return (size_t)s->nfft[0] * s->nfft[1] * s->nfft[2]; }
 
 thp_status zero_grid(thp_sys* s) {
   const size_t cnt = 2 * grid_points(s);
@@ -289,62 +287,48 @@ thp_status zero_grid(thp_sys* s) {
   return launch_check(s->ctx, "k_zero_grid");
 }
 
-thp_status spread_uind(thp_sys* s, int vin, int frac, int check_conv) {
-  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_spread_brick<double, 0>), g1(s->nbricks), dim3(64 * kSpreadWaves), 0, s->ctx->stream, s->d_view, s->d_brickwork, vin, frac, check_conv);
-  else hipLaunchKernelGGL((k_spread_brick<float, 0>), g1(s->nbricks), dim3(64 * kSpreadWaves), 0, s->ctx->stream, s->d_view, s->d_brickwork, vin, frac, check_conv);
-  return launch_check(s->ctx, "k_spread_brick<uind>");
+thp_status spread_uind(thp_ctx* ctx, const Batch& b, int vin, int frac, int check_conv) {
+  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_spread_brick<double, 0>), g1(b.n_brick), dim3(64 * kSpreadWaves), 0, ctx->stream, b.d_views, b.d_brick, vin, frac, check_conv);
+  else hipLaunchKernelGGL((k_spread_brick<float, 0>), g1(b.n_brick), dim3(64 * kSpreadWaves), 0, ctx->stream, b.d_views, b.d_brick, vin, frac, check_conv);
+  return launch_check(ctx, "k_spread_brick<uind>");
 }
 
-thp_status spread_mpole(thp_sys* s) {
-  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_spread_brick<double, 1>), g1(s->nbricks), dim3(64 * kSpreadWaves), 0, s->ctx->stream, s->d_view, s->d_brickwork, 0, 0, 0);
-  else hipLaunchKernelGGL((k_spread_brick<float, 1>), g1(s->nbricks), dim3(64 * kSpreadWaves), 0, s->ctx->stream, s->d_view, s->d_brickwork, 0, 0, 0);
-  return launch_check(s->ctx, "k_spread_brick<mpole>");
+thp_status spread_mpole(thp_ctx* ctx, const Batch& b) {
+  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_spread_brick<double, 1>), g1(b.n_brick), dim3(64 * kSpreadWaves), 0, ctx->stream, b.d_views, b.d_brick, 0, 0, 0);
+  else hipLaunchKernelGGL((k_spread_brick<float, 1>), g1(b.n_brick), dim3(64 * kSpreadWaves), 0, ctx->stream, b.d_views, b.d_brick, 0, 0, 0);
+  return launch_check(ctx, "k_spread_brick<mpole>");
 }
 
-// per coordinate set: B-spline values, grid nodes and brick bins
-thp_status pme_prep(thp_sys* s) {
-  hipStream_t st = s->ctx->stream;
-  HIPCHK(s->ctx, hipMemsetAsync(s->view.bin_count, 0, sizeof(int) * s->nbricks, st));
-  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_pme_prep<double>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
-  else hipLaunchKernelGGL((k_pme_prep<float>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
-  RET(launch_check(s->ctx, "k_pme_prep"));
-  hipLaunchKernelGGL(k_pme_scan, g1(1), dim3(1024), 0, st, s->d_view, (const int*)s->d_sysid);
-  RET(launch_check(s->ctx, "k_pme_scan"));
-  hipLaunchKernelGGL(k_pme_fill, g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
-  RET(launch_check(s->ctx, "k_pme_fill"));
-  hipLaunchKernelGGL(k_pme_binsort, g1(s->nbricks), dim3(256), 0, st, s->d_view, s->d_brickwork);
-  return launch_check(s->ctx, "k_pme_binsort");
-}
-
-thp_status convolve(thp_sys* s, int check_conv) {
-  void* buf[1] = {s->view.grid};
-  FFTCHK(s->ctx, rocfft_execute(s->fwd, buf, nullptr, s->info));
-  const size_t np = grid_points(s);
-  const int nb = (int)std::min<size_t>(4096, (np + 255) / 256);
-  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_conv<double>), g1(nb), dim3(256), 0, s->ctx->stream, s->d_view, 0, np, check_conv);
-  else hipLaunchKernelGGL((k_conv<float>), g1(nb), dim3(256), 0, s->ctx->stream, s->d_view, 0, np, check_conv);
-  RET(launch_check(s->ctx, "k_conv"));
-  FFTCHK(s->ctx, rocfft_execute(s->bwd, buf, nullptr, s->info));
+thp_status convolve(thp_ctx* ctx, const Batch& b, int check_conv) {
+  for (const FftExec& f : b.fft) {
+    void* buf[1] = {f.base};
+    FFTCHK(ctx, rocfft_execute(f.fwd, buf, nullptr, f.info));
+  }
+  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_conv<double>), g1(b.n_conv), dim3(256), 0, ctx->stream, b.d_views, b.d_conv, check_conv);
+  else hipLaunchKernelGGL((k_conv<float>), g1(b.n_conv), dim3(256), 0, ctx->stream, b.d_views, b.d_conv, check_conv);
+  RET(launch_check(ctx, "k_conv"));
+  for (const FftExec& f : b.fft) {
+    void* buf[1] = {f.base};
+    FFTCHK(ctx, rocfft_execute(f.bwd, buf, nullptr, f.info));
+  }
   return THP_OK;
 }
 
 template <int MODE>
-thp_status launch_gather_uind(thp_sys* s, int vin, int vout, int frac_out) {
-  const int nb = s->view.gat_blocks;
-  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_uind<double, MODE>), g1(nb), dim3(256), 0, s->ctx->stream, s->d_view, s->d_gatwork, vin, vout, frac_out);
-  else hipLaunchKernelGGL((k_gather_uind<float, MODE>), g1(nb), dim3(256), 0, s->ctx->stream, s->d_view, s->d_gatwork, vin, vout, frac_out);
-  return launch_check(s->ctx, "k_gather_uind");
+thp_status launch_gather_uind(thp_ctx* ctx, const Batch& b, int vin, int vout, int frac_out) {
+  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_uind<double, MODE>), g1(b.n_gat), dim3(256), 0, ctx->stream, b.d_views, b.d_gat, vin, vout, frac_out);
+  else hipLaunchKernelGGL((k_gather_uind<float, MODE>), g1(b.n_gat), dim3(256), 0, ctx->stream, b.d_views, b.d_gat, vin, vout, frac_out);
+  return launch_check(ctx, "k_gather_uind");
 }
 
 // permanent field: ef = E_direct - grad phi_recip + self (newinduce_pme2.f:111-147)
-thp_status field(thp_sys* s, double* cphi_dev, int add_to_ef, int do_direct) {
-  hipStream_t st = s->ctx->stream;
-  if (do_direct) RET(pair_efld0(s));
-  RET(spread_mpole(s));
-  RET(convolve(s, 0));
-  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_mpole<double>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work, cphi_dev, add_to_ef);
-  else hipLaunchKernelGGL((k_gather_mpole<float>), g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work, cphi_dev, add_to_ef);
-  return launch_check(s->ctx, "k_gather_mpole");
+thp_status field(thp_ctx* ctx, const Batch& b, double* cphi_dev, int add_to_ef, int do_direct) {
+  if (do_direct) RET(pair_efld0(ctx, b));
+  RET(spread_mpole(ctx, b));
+  RET(convolve(ctx, b, 0));
+  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_mpole<double>), g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, cphi_dev, add_to_ef);
+  else hipLaunchKernelGGL((k_gather_mpole<float>), g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, cphi_dev, add_to_ef);
+  return launch_check(ctx, "k_gather_mpole");
 }
 
 thp_status scatter_in(thp_sys* s, double* dst_dev, const double* host, int ncomp) {
@@ -444,6 +428,20 @@ thp_status build_union(thp_sys* s) {
   return fail(s->ctx, THP_ERR_LIST, "block-union list capacity");
 }
 
+thp_status ensure_pme(thp_sys* s) {
+  if (!s->pme_valid) {
+    RET(b_pme_prep(s->ctx, s->one));
+    RET(check_err_flag(s));
+    s->pme_valid = true;
+  }
+  return THP_OK;
+}
+
+}  // namespace
+
+// ================================================ batched launch sequences
+namespace thp {
+
 thp_status ensure_lists(thp_sys* s) {
   if (!s->list_valid) RET(build_block_lists(s));
   if (!s->tight_valid) {
@@ -453,16 +451,85 @@ thp_status ensure_lists(thp_sys* s) {
   return THP_OK;
 }
 
-thp_status ensure_pme(thp_sys* s) {
-  if (!s->pme_valid) {
-    RET(pme_prep(s));
-    RET(check_err_flag(s));
-    s->pme_valid = true;
-  }
-  return THP_OK;
+// per coordinate set: B-spline values, grid nodes and brick bins
+thp_status b_pme_prep(thp_ctx* ctx, const Batch& b) {
+  hipStream_t st = ctx->stream;
+  hipLaunchKernelGGL(k_zero_bins, g1((b.n_brick + 255) / 256), dim3(256), 0, st, b.d_views, b.d_brick, b.n_brick);
+  RET(launch_check(ctx, "k_zero_bins"));
+  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_pme_prep<double>), g1(b.n_blk), dim3(64), 0, st, b.d_views, b.d_blk);
+  else hipLaunchKernelGGL((k_pme_prep<float>), g1(b.n_blk), dim3(64), 0, st, b.d_views, b.d_blk);
+  RET(launch_check(ctx, "k_pme_prep"));
+  hipLaunchKernelGGL(k_pme_scan, g1(b.nsys), dim3(1024), 0, st, b.d_views, (const int*)b.d_ids);
+  RET(launch_check(ctx, "k_pme_scan"));
+  hipLaunchKernelGGL(k_pme_fill, g1(b.n_blk), dim3(64), 0, st, b.d_views, b.d_blk);
+  RET(launch_check(ctx, "k_pme_fill"));
+  hipLaunchKernelGGL(k_pme_binsort, g1(b.n_brick), dim3(256), 0, st, b.d_views, b.d_brick);
+  return launch_check(ctx, "k_pme_binsort");
 }
 
-}  // namespace
+thp_status b_init_state(thp_ctx* ctx, const Batch& b, double poleps, int politer, const int* d_warm) {
+  hipLaunchKernelGGL(k_init_state, g1((b.nsys + 63) / 64), dim3(64), 0, ctx->stream, b.d_views, (const int*)b.d_ids,
+                     b.nsys, poleps, politer, d_warm);
+  return launch_check(ctx, "k_init_state");
+}
+
+thp_status b_pair_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int check_conv) {
+  if (b.prec == THP_PREC_FP64) launch_pair<double, 1>(ctx, b, vin, vout, dodiag, check_conv);
+  else launch_pair<float, 1>(ctx, b, vin, vout, dodiag, check_conv);
+  return launch_check(ctx, "k_pair_union<tmatxb>");
+}
+
+// Field, guess and first residual (newinduce_pme2.f:111-196, 406-440);
+// the solve states are initialised and the B-spline tables current.
+thp_status b_setup(thp_ctx* ctx, const Batch& b) {
+  RET(field(ctx, b, nullptr, 1, 1));
+  hipLaunchKernelGGL(k_guess, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk);
+  RET(launch_check(ctx, "k_guess"));
+  RET(b_pair_tmatxb(ctx, b, 0 /*mu*/, 3 /*hreal*/, 1, 0));
+  RET(spread_uind(ctx, b, 0, 0, 0));
+  RET(convolve(ctx, b, 0));
+  return launch_gather_uind<1>(ctx, b, 0, 3, 0);
+}
+
+// One PCG iteration of every running system (newinduce_pme2.f:442-569);
+// each system's iteration number lives in its SolveState.
+thp_status b_iteration(thp_ctx* ctx, const Batch& b) {
+  RET(b_pair_tmatxb(ctx, b, 1 /*pp*/, 3 /*hreal*/, 1, 1));
+  RET(spread_uind(ctx, b, 1, 0, 1));
+  RET(convolve(ctx, b, 1));
+  RET(launch_gather_uind<2>(ctx, b, 1, 3, 0));
+  hipStream_t st = ctx->stream;
+  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_cg_update1<double>), g1(b.n_red), dim3(256), 0, st, b.d_views, b.d_red);
+  else hipLaunchKernelGGL((k_cg_update1<float>), g1(b.n_red), dim3(256), 0, st, b.d_views, b.d_red);
+  RET(launch_check(ctx, "k_cg_update1"));
+  hipLaunchKernelGGL(k_cg_update2, g1(b.n_red), dim3(256), 0, st, b.d_views, b.d_red);
+  return launch_check(ctx, "k_cg_update2");
+}
+
+// E_pol of the finished systems (only_done) or of all
+thp_status b_finish(thp_ctx* ctx, const Batch& b, int only_done) {
+  hipLaunchKernelGGL(k_energy, g1(b.n_red), dim3(256), 0, ctx->stream, b.d_views, b.d_red, only_done);
+  RET(launch_check(ctx, "k_energy"));
+  hipLaunchKernelGGL(k_finish, g1(b.nsys), dim3(64), 0, ctx->stream, b.d_views, (const int*)b.d_ids, only_done);
+  return launch_check(ctx, "k_finish");
+}
+
+thp_status b_collect(thp_ctx* ctx, const Batch& b, StateOut* d_out) {
+  hipLaunchKernelGGL(k_collect, g1((b.nsys + 63) / 64), dim3(64), 0, ctx->stream, b.d_views, (const int*)b.d_ids, b.nsys, d_out);
+  return launch_check(ctx, "k_collect");
+}
+
+void append_work(const thp_sys* s, int index, std::vector<Work>& blk, std::vector<Work>& red, std::vector<Work>& gat,
+                 std::vector<Work>& brick, std::vector<Work>& conv) {
+  for (int q = 0; q < s->nblk; ++q) blk.push_back({index, q});
+  for (int q = 0; q < s->view.red_blocks; ++q) red.push_back({index, q});
+  for (int q = 0; q < s->view.gat_blocks; ++q) gat.push_back({index, q});
+  for (int q = 0; q < s->nbricks; ++q) brick.push_back({index, q});
+  const int nconv = (int)((grid_points(s) + kConvChunk - 1) / kConvChunk);
+  for (int q = 0; q < nconv; ++q) conv.push_back({index, q});
+}
+
+}  // namespace thp
 
 // ====================================================================== API
 extern "C" {
@@ -492,6 +559,9 @@ thp_status thp_init(thp_ctx** out, int device) {
 thp_status thp_finalize(thp_ctx* ctx) {
   if (!ctx) return THP_ERR_ARG;
   if (ctx->serve) thp_serve_stop(ctx);
+  // systems the caller did not destroy are released here; their handles die
+  // with the context
+  while (!ctx->systems.empty()) thp_system_destroy(*ctx->systems.begin());
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return THP_OK;
@@ -500,6 +570,12 @@ thp_status thp_finalize(thp_ctx* ctx) {
 const char* thp_last_error(thp_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 thp_status thp_system_create(thp_ctx* ctx, const thp_system_desc* d, thp_sys** out) {
+  return create_system(ctx, d, out, true);
+}
+
+}  // extern "C"
+
+thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** out, bool own_grid) {
   if (!ctx || !d || !out) return THP_ERR_ARG;
   if (d->n <= 0 || d->n >= kMaxAtoms) return fail(ctx, THP_ERR_ARG, "n = %d out of range", d->n);
   if (d->order != kOrder) return fail(ctx, THP_ERR_ARG, "only B-spline order %d is implemented", kOrder);
@@ -520,6 +596,8 @@ thp_status thp_system_create(thp_ctx* ctx, const thp_system_desc* d, thp_sys** o
 
   thp_sys* s = new thp_sys;
   s->ctx = ctx;
+  s->own_grid = own_grid;
+  ctx->systems.insert(s);
   s->n = d->n;
   s->npad = (d->n + kBlock - 1) / kBlock * kBlock;
   s->nblk = s->npad / kBlock;
@@ -529,7 +607,6 @@ thp_status thp_system_create(thp_ctx* ctx, const thp_system_desc* d, thp_sys** o
     s->nfft[k] = d->nfft[k];
   }
   s->aewald = d->aewald;
-  if (const char* e = getenv("THP_SEGS")) s->segs = std::max(1, std::min(4, atoi(e)));
   s->cut = d->cut;
   s->buffer = d->buffer;
   s->order = d->order;
@@ -663,15 +740,15 @@ thp_status thp_system_create(thp_ctx* ctx, const thp_system_desc* d, thp_sys** o
   const size_t npts = grid_points(s);
   const auto q = qfac_table(s->box, s->nfft, s->aewald, s->order);
   if (s->prec == THP_PREC_FP64) {
-    double *g, *qf;
-    ALLOC(g, 2 * npts);
+    double *g = nullptr, *qf;
+    if (own_grid) ALLOC(g, 2 * npts);
     ALLOC(qf, npts);
     H2D(qf, q.data(), sizeof(double) * npts);
     v.grid = g;
     v.qfac = qf;
   } else {
-    float *g, *qf;
-    ALLOC(g, 2 * npts);
+    float *g = nullptr, *qf;
+    if (own_grid) ALLOC(g, 2 * npts);
     ALLOC(qf, npts);
     std::vector<float> qs(q.begin(), q.end());
     H2D(qf, qs.data(), sizeof(float) * npts);
@@ -679,7 +756,7 @@ thp_status thp_system_create(thp_ctx* ctx, const thp_system_desc* d, thp_sys** o
     v.grid = g;
     v.qfac = qf;
   }
-  {
+  if (own_grid) {
     const size_t len[3] = {(size_t)s->nfft[0], (size_t)s->nfft[1], (size_t)s->nfft[2]};
     const rocfft_precision pr = s->prec == THP_PREC_FP64 ? rocfft_precision_double : rocfft_precision_single;
     if (rocfft_plan_create(&s->fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward, pr, 3, len, 1, nullptr) != rocfft_status_success ||
@@ -741,10 +818,40 @@ thp_status thp_system_create(thp_ctx* ctx, const thp_system_desc* d, thp_sys** o
     H2D(s->d_gatwork, gw.data(), sizeof(Work) * v.gat_blocks);
     if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "sync"));
   }
-  ALLOC(s->d_sysid, 1);
   {
-    const int zero = 0;
-    H2D(s->d_sysid, &zero, sizeof(int));
+    const int nconv = (int)((npts + kConvChunk - 1) / kConvChunk);
+    std::vector<Work> cw(nconv);
+    for (int b = 0; b < nconv; ++b) cw[b] = {0, b};
+    ALLOC(s->d_convwork, nconv);
+    H2D(s->d_convwork, cw.data(), sizeof(Work) * nconv);
+    s->nconv = nconv;
+    if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "sync"));
+  }
+  ALLOC(s->d_sysid, 2);
+  {
+    const int ids[2] = {0, 1};  // {system index 0, warm-start flag 1}
+    H2D(s->d_sysid, ids, sizeof ids);
+    if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "sync"));
+  }
+  s->d_one = s->d_sysid + 1;
+  {
+    Batch& b = s->one;
+    b.prec = s->prec;
+    b.nsys = 1;
+    b.segs = ctx->pair_segs > 0 ? ctx->pair_segs : kDefaultSegs;
+    b.d_views = s->d_view;
+    b.d_blk = s->d_work;
+    b.n_blk = s->nblk;
+    b.d_red = s->d_redwork;
+    b.n_red = v.red_blocks;
+    b.d_gat = s->d_gatwork;
+    b.n_gat = v.gat_blocks;
+    b.d_brick = s->d_brickwork;
+    b.n_brick = s->nbricks;
+    b.d_conv = s->d_convwork;
+    b.n_conv = s->nconv;
+    b.d_ids = s->d_sysid;
+    if (own_grid) b.fft.push_back({s->fwd, s->bwd, s->info, v.grid});
   }
   for (int e = 0; e < 4; ++e)
     if (hipEventCreate(&s->ev[e]) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "event"));
@@ -756,9 +863,14 @@ thp_status thp_system_create(thp_ctx* ctx, const thp_system_desc* d, thp_sys** o
   return THP_OK;
 }
 
+extern "C" {
+
 thp_status thp_system_destroy(thp_sys* s) {
   if (!s) return THP_ERR_ARG;
-  if (s->ctx && s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
+  if (s->ctx) {
+    if (s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
+    s->ctx->systems.erase(s);
+  }
   if (s->fwd) rocfft_plan_destroy(s->fwd);
   if (s->bwd) rocfft_plan_destroy(s->bwd);
   if (s->info) rocfft_execution_info_destroy(s->info);
@@ -872,14 +984,14 @@ thp_status thp_nblist_get(thp_sys* s, int64_t* npairs, int* ik, int64_t cap) {
 thp_status thp_efld0_direct(thp_sys* s, double* ef) {
   if (!s || !ef) return THP_ERR_ARG;
   RET(ensure_lists(s));
-  RET(pair_efld0(s));
+  RET(pair_efld0(s->ctx, s->one));
   return gather_out(s, ef, s->view.ef, 6);
 }
 
 thp_status thp_efld0_recip(thp_sys* s, double* cphi) {
   if (!s || !cphi) return THP_ERR_ARG;
   RET(ensure_pme(s));
-  RET(field(s, s->cphi, 0, 0));
+  RET(field(s->ctx, s->one, s->cphi, 0, 0));
   return gather_out(s, cphi, s->cphi, 10);
 }
 
@@ -887,7 +999,7 @@ thp_status thp_tmatxb(thp_sys* s, const double* mu, double* efi, int dodiag) {
   if (!s || !mu || !efi) return THP_ERR_ARG;
   RET(ensure_lists(s));
   RET(scatter_in(s, s->view.pp, mu, 6));
-  RET(pair_tmatxb(s, 1 /*pp*/, 3 /*hreal*/, dodiag, 0));
+  RET(b_pair_tmatxb(s->ctx, s->one, 1 /*pp*/, 3 /*hreal*/, dodiag, 0));
   return gather_out(s, efi, s->view.hreal, 6);
 }
 
@@ -896,7 +1008,7 @@ thp_status thp_tmatxb_dev(thp_sys* s, const double* mu, double* efi, int dodiag)
   RET(ensure_lists(s));
   hipLaunchKernelGGL(k_scatter_in, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream, s->view.pp, mu, s->iperm_dev, s->n, s->npad, 6);
   RET(launch_check(s->ctx, "k_scatter_in"));
-  RET(pair_tmatxb(s, 1, 3, dodiag, 0));
+  RET(b_pair_tmatxb(s->ctx, s->one, 1, 3, dodiag, 0));
   hipLaunchKernelGGL(k_gather_out, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream, efi, s->view.hreal, s->iperm_dev, s->n, s->npad, 6);
   RET(launch_check(s->ctx, "k_gather_out"));
   return sync(s);
@@ -906,9 +1018,9 @@ thp_status thp_tmatxb_recip(thp_sys* s, const double* mu, double* dipfield) {
   if (!s || !mu || !dipfield) return THP_ERR_ARG;
   RET(ensure_pme(s));
   RET(scatter_in(s, s->view.pp, mu, 6));
-  RET(spread_uind(s, 1, 0, 0));
-  RET(convolve(s, 0));
-  RET(launch_gather_uind<0>(s, 1, 3, 0));
+  RET(spread_uind(s->ctx, s->one, 1, 0, 0));
+  RET(convolve(s->ctx, s->one, 0));
+  RET(launch_gather_uind<0>(s->ctx, s->one, 1, 3, 0));
   return gather_out(s, dipfield, s->view.hreal, 6);
 }
 
@@ -922,7 +1034,7 @@ thp_status thp_grid_uind(thp_sys* s, const double* fuind, const double* fuinp, d
     }
   RET(ensure_pme(s));
   RET(scatter_in(s, s->view.pp, f.data(), 6));
-  RET(spread_uind(s, 1, 1, 0));
+  RET(spread_uind(s->ctx, s->one, 1, 1, 0));
   if (qgrid) {
     const size_t cnt = 2 * grid_points(s);
     if (s->prec == THP_PREC_FP64) {
@@ -940,7 +1052,7 @@ thp_status thp_grid_uind(thp_sys* s, const double* fuind, const double* fuinp, d
 
 thp_status thp_pme_convolve(thp_sys* s) {
   if (!s) return THP_ERR_ARG;
-  RET(convolve(s, 0));
+  RET(convolve(s->ctx, s->one, 0));
   return sync(s);
 }
 
@@ -958,7 +1070,7 @@ thp_status thp_fphi_uind(thp_sys* s, const double* qgrid, double* phi1, double* 
       RET(sync(s));
     }
   }
-  RET(launch_gather_uind<0>(s, 1, 3, 1));
+  RET(launch_gather_uind<0>(s->ctx, s->one, 1, 3, 1));
   std::vector<double> out((size_t)s->n * 6);
   RET(gather_out(s, out.data(), s->view.hreal, 6));
   for (int a = 0; a < s->n; ++a)
@@ -974,70 +1086,35 @@ thp_status thp_fphi_uind(thp_sys* s, const double* qgrid, double* phi1, double* 
 // ============================================================ PCG driver
 namespace thp {
 
-int upd_threads(const thp_sys*) { return 256; }
-
-// Enqueue PCG iteration `it` (1-based; newinduce_pme2.f:442-569).
-thp_status enqueue_iteration(thp_sys* s, int it) {
-  RET(pair_tmatxb(s, 1 /*pp*/, 3 /*hreal*/, 1, 1));
-  RET(spread_uind(s, 1, 0, 1));
-  RET(convolve(s, 1));
-  RET(launch_gather_uind<2>(s, 1, 3, 0));
-  hipStream_t st = s->ctx->stream;
-  const int G = s->view.red_blocks, T = upd_threads(s);
-  if (s->prec == THP_PREC_FP64) hipLaunchKernelGGL((k_cg_update1<double>), g1(G), dim3(T), 0, st, s->d_view, s->d_redwork, it);
-  else hipLaunchKernelGGL((k_cg_update1<float>), g1(G), dim3(T), 0, st, s->d_view, s->d_redwork, it);
-  RET(launch_check(s->ctx, "k_cg_update1"));
-  hipLaunchKernelGGL(k_cg_update2, g1(G), dim3(T), 0, st, s->d_view, s->d_redwork, it);
-  return launch_check(s->ctx, "k_cg_update2");
-}
-
-// Field, guess and first residual (newinduce_pme2.f:111-196, 406-440).
-thp_status enqueue_setup(thp_sys* s, const thp_solve_opts* o, bool warm) {
-  hipStream_t st = s->ctx->stream;
-  SolveState init;
-  memset(&init, 0, sizeof init);
-  init.poleps = o->poleps;
-  init.politer = o->politer;
-  HIPCHK(s->ctx, hipMemcpyAsync(s->view.st, &init, sizeof init, hipMemcpyHostToDevice, st));
-  RET(ensure_lists(s));
-  RET(ensure_pme(s));
-  RET(field(s, nullptr, 1, 1));
-  if (!warm) {
-    hipLaunchKernelGGL(k_guess, g1(s->nblk), dim3(64), 0, st, s->d_view, s->d_work);
-    RET(launch_check(s->ctx, "k_guess"));
-  }
-  RET(pair_tmatxb(s, 0 /*mu*/, 3 /*hreal*/, 1, 0));
-  RET(spread_uind(s, 0, 0, 0));
-  RET(convolve(s, 0));
-  return launch_gather_uind<1>(s, 0, 3, 0);
-}
-
+// One solve of one system: the reference reads the residual norm back on
+// the host after every iteration (newinduce_pme2gpu.f:1040-1041), and so
+// does this one-system path.
 thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_stats* stats) {
   hipStream_t st = s->ctx->stream;
-  HIPCHK(s->ctx, hipEventRecord(s->ev[0], st));
-  RET(enqueue_setup(s, o, warm));
-  HIPCHK(s->ctx, hipEventRecord(s->ev[1], st));
+  thp_ctx* ctx = s->ctx;
+  const Batch& b = s->one;
+  HIPCHK(ctx, hipEventRecord(s->ev[0], st));
+  RET(ensure_lists(s));
+  RET(ensure_pme(s));
+  RET(b_init_state(ctx, b, o->poleps, o->politer, warm ? s->d_one : nullptr));
+  RET(b_setup(ctx, b));
+  HIPCHK(ctx, hipEventRecord(s->ev[1], st));
   int conv = 0;
   for (int it = 1; it <= o->politer; ++it) {
-    RET(enqueue_iteration(s, it));
-    // the reference reads the residual norm back on the host every
-    // iteration (newinduce_pme2gpu.f:1040-1041)
-    HIPCHK(s->ctx, hipMemcpyAsync(&conv, &s->view.st->conv, sizeof(int), hipMemcpyDeviceToHost, st));
+    RET(b_iteration(ctx, b));
+    HIPCHK(ctx, hipMemcpyAsync(&conv, &s->view.st->conv, sizeof(int), hipMemcpyDeviceToHost, st));
     RET(sync(s));
     if (conv) break;
   }
-  hipLaunchKernelGGL(k_energy, g1(s->view.red_blocks), dim3(upd_threads(s)), 0, st, s->d_view, s->d_redwork);
-  RET(launch_check(s->ctx, "k_energy"));
-  hipLaunchKernelGGL(k_finish, g1(1), dim3(64), 0, st, s->d_view, (const int*)s->d_sysid);
-  RET(launch_check(s->ctx, "k_finish"));
-  HIPCHK(s->ctx, hipEventRecord(s->ev[2], st));
+  RET(b_finish(ctx, b, 0));
+  HIPCHK(ctx, hipEventRecord(s->ev[2], st));
   SolveState fin;
-  HIPCHK(s->ctx, hipMemcpyAsync(&fin, s->view.st, sizeof fin, hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(&fin, s->view.st, sizeof fin, hipMemcpyDeviceToHost, st));
   RET(sync(s));
   if (stats) {
     float t1 = 0, t2 = 0;
-    hipEventElapsedTime(&t1, s->ev[0], s->ev[1]);
-    hipEventElapsedTime(&t2, s->ev[1], s->ev[2]);
+    (void)hipEventElapsedTime(&t1, s->ev[0], s->ev[1]);
+    (void)hipEventElapsedTime(&t2, s->ev[1], s->ev[2]);
     stats->niter = fin.niter;
     stats->resnrm = fin.resnrm;
     stats->ep = fin.ep;
@@ -1087,9 +1164,9 @@ thp_status thp_time_kernel(thp_sys* s, int kernel, int reps, double* ms, int64_t
   RET(ensure_pme(s));
   hipStream_t st = s->ctx->stream;
   auto launch = [&]() -> thp_status {
-    if (kernel == 0) return pair_tmatxb(s, 1, 3, 1, 0);
-    if (kernel == 1) return spread_uind(s, 1, 0, 0);
-    return launch_gather_uind<0>(s, 1, 3, 0);
+    if (kernel == 0) return b_pair_tmatxb(s->ctx, s->one, 1, 3, 1, 0);
+    if (kernel == 1) return spread_uind(s->ctx, s->one, 1, 0, 0);
+    return launch_gather_uind<0>(s->ctx, s->one, 1, 3, 0);
   };
   RET(launch());  // warm
   HIPCHK(s->ctx, hipEventRecord(s->ev[0], st));

@@ -5,6 +5,7 @@
 #include <rocfft/rocfft.h>
 
 #include <cstdint>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -30,10 +31,12 @@ struct SolveState {
   double resnrm;
   double poleps;
   double ep;
-  int niter;
+  int niter;    // completed iterations (written by k_cg_update2 block 0)
+  int it_cur;   // iteration in progress (written by k_cg_update1 block 0)
   int politer;
   int conv;
   int n_indef;
+  int warm;     // 1: mu holds the starting dipoles (no direct guess)
 };
 
 // Everything a kernel needs about one system; lives in device memory so a
@@ -97,9 +100,42 @@ struct SysView {
   int* err;         // device error flag (list overflow ...)
 };
 
-struct Work {  // one 64-atom block of one system
+struct Work {  // one unit of a launch: a block (brick, chunk ...) of one system
   int sys;
   int blk;
+};
+
+constexpr int kConvChunk = 2048;
+constexpr int kDefaultSegs = 8;   // pair-kernel wavefronts per block unless tuned  // grid points per block of the convolution
+
+// One rocFFT execution: forward and backward plans over `count` contiguous
+// grids starting at `base` (count 1 for a single system).
+struct FftExec {
+  rocfft_plan fwd = nullptr, bwd = nullptr;
+  rocfft_execution_info info = nullptr;
+  void* base = nullptr;
+};
+
+// A set of systems launched together.  Every kernel reaches its systems
+// through d_views[w.sys] and walks one of the work tables; a single system
+// is a batch of one.  All systems of a batch share one precision.
+struct Batch {
+  int prec = THP_PREC_FP64;
+  int nsys = 0;
+  int segs = 4;                 // wavefronts per 64-atom block in the pair kernels
+  SysView* d_views = nullptr;   // [nsys]
+  Work* d_blk = nullptr;        // 64-atom blocks
+  int n_blk = 0;
+  Work* d_red = nullptr;        // PCG vector-kernel blocks
+  int n_red = 0;
+  Work* d_gat = nullptr;        // PME gather blocks
+  int n_gat = 0;
+  Work* d_brick = nullptr;      // spreading bricks
+  int n_brick = 0;
+  Work* d_conv = nullptr;       // convolution chunks of kConvChunk points
+  int n_conv = 0;
+  int* d_ids = nullptr;         // [nsys] = 0..nsys-1
+  std::vector<FftExec> fft;
 };
 
 }  // namespace thp
@@ -110,6 +146,8 @@ struct thp_ctx {
   std::string err;
   void* serve = nullptr;
   int n_cu = 0;
+  int pair_segs = 0;           // 0: default; else forced waves per block (1..16)
+  std::set<thp_sys*> systems;  // live systems; thp_finalize releases what is left
 };
 
 struct thp_sys {
@@ -120,7 +158,7 @@ struct thp_sys {
   double aewald = 0, cut = 0, buffer = 0;
   int nfft[3] = {0, 0, 0};
   int order = 5;
-  int segs = 8;  // waves per 64-atom block in the pair kernels (1,2,4,8,16)
+  bool own_grid = true;  // false: the grid and FFT plans come from a batch (serving)
   std::vector<int> perm;   // sorted slot -> original index
   std::vector<int> iperm;  // original index -> sorted slot
   int* iperm_dev = nullptr;
@@ -141,10 +179,15 @@ struct thp_sys {
   int64_t bytes = 0;
   thp::SysView view;           // host copy of the view
   thp::SysView* d_view = nullptr;
+  int view_version = 0;        // bumped whenever the device view is rewritten
   thp::Work* d_work = nullptr;  // single-system work table [nblk]
   thp::Work* d_redwork = nullptr;  // vector-kernel work table [red_blocks]
   thp::Work* d_gatwork = nullptr;  // gather work table [gat_blocks]
-  int* d_sysid = nullptr;          // {0}
+  int* d_sysid = nullptr;          // {0, 1}
+  int* d_one = nullptr;            // d_sysid + 1
+  thp::Work* d_convwork = nullptr; // convolution work table
+  int nconv = 0;
+  thp::Batch one;                  // this system as a batch of one
   // block neighbor lists
   double* bbox = nullptr;  // [nblk][6] centre + half-extent
   int* bcount = nullptr;   // [nblk]
@@ -156,3 +199,28 @@ struct thp_sys {
   void* fft_work = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
+
+// ------------------------------------------------ host helpers (thp_api.hip)
+namespace thp {
+thp_status fail(thp_ctx* ctx, thp_status code, const char* fmt, ...);
+thp_status create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** out, bool own_grid);
+thp_status ensure_lists(thp_sys* s);
+size_t grid_points(const thp_sys* s);
+// per-system work tables with sys = `index` (appended)
+void append_work(const thp_sys* s, int index, std::vector<Work>& blk, std::vector<Work>& red,
+                 std::vector<Work>& gat, std::vector<Work>& brick, std::vector<Work>& conv);
+// batched launch sequences (enqueued on ctx->stream)
+thp_status b_init_state(thp_ctx* ctx, const Batch& b, double poleps, int politer, const int* d_warm);
+thp_status b_pme_prep(thp_ctx* ctx, const Batch& b);
+thp_status b_setup(thp_ctx* ctx, const Batch& b);
+thp_status b_iteration(thp_ctx* ctx, const Batch& b);
+thp_status b_finish(thp_ctx* ctx, const Batch& b, int only_done);
+thp_status b_pair_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int check_conv);
+// copy each system's SolveState and error flag into out[nsys] (device)
+struct StateOut {
+  SolveState st;
+  int err;
+  int pad;
+};
+thp_status b_collect(thp_ctx* ctx, const Batch& b, StateOut* d_out);
+}  // namespace thp

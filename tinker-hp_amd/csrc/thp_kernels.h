@@ -173,6 +173,7 @@ __device__ __forceinline__ void block_partial2(double a, double b, double* part,
 // image), stored as centre + half extent.  One wavefront per block.
 __global__ void __launch_bounds__(64) k_bbox(const SysView* views, const Work* work) {
   const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
   const int lane = threadIdx.x;
   const int i0 = w.blk * kBlock;
@@ -215,6 +216,7 @@ template <bool FILL>
 __global__ void __launch_bounds__(256) k_blockpairs(const SysView* views, const Work* work,
                                                     int* bcount, int* blist, int maxbl) {
   const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
   const double* bb = sv.bbox;
   const int I = w.blk;
@@ -261,6 +263,7 @@ template <bool FILL>
 __global__ void __launch_bounds__(64) k_nblist_atoms(const SysView* views, const Work* work,
                                                       const int* bcount, const int* blist, int maxbl) {
   const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
   const int n = sv.n, maxnb = sv.maxnb;
   const size_t np = sv.npad;
@@ -465,6 +468,7 @@ constexpr int kUnionCand = 96;  // candidate blocks per LDS pass
 __global__ void __launch_bounds__(64 * kUnionWaves) k_union(const SysView* views, const Work* work,
                                                             const int* bcount, const int* blist, int maxbl) {
   const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
   const int n = sv.n, umax = sv.umax, lmax = sv.lmax;
   const size_t np = sv.npad;
@@ -600,6 +604,7 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
   __shared__ real s_pd[C], s_th[C];
   __shared__ double red[S > 1 ? S * 6 * 64 : 1];
   const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
   if (check_conv && sv.st->conv) return;
   const int n = sv.n;
@@ -665,7 +670,7 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
         const real rr2 = rr1 * rr1;
         const real rr3 = rr2 * rr1;
         const real rr5 = real(3) * rr2 * rr3;
-        if (KIND == 0) {
+        if constexpr (KIND == 0) {
           const real rr7 = real(5) * rr2 * rr5;
           const real ck = s_m[0][q], dkx = s_m[1][q], dky = s_m[2][q], dkz = s_m[3][q];
           const real qkxx = s_m[4][q], qkxy = s_m[5][q], qkxz = s_m[6][q], qkyy = s_m[7][q], qkyz = s_m[8][q],
@@ -828,6 +833,7 @@ __device__ __forceinline__ int th_idx(int axis, int level, int node) { return ax
 template <typename real>
 __global__ void __launch_bounds__(64) k_pme_prep(const SysView* views, const Work* work) {
   const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
   const int i = w.blk * kBlock + threadIdx.x;
   if (i >= sv.n) return;
@@ -892,6 +898,7 @@ __global__ void __launch_bounds__(1024) k_pme_scan(const SysView* views, const i
 
 __global__ void __launch_bounds__(64) k_pme_fill(const SysView* views, const Work* work) {
   const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
   const int i = w.blk * kBlock + threadIdx.x;
   if (i >= sv.n) return;
@@ -905,6 +912,7 @@ __global__ void __launch_bounds__(64) k_pme_fill(const SysView* views, const Wor
 constexpr int kMaxBin = 2048;
 __global__ void __launch_bounds__(256) k_pme_binsort(const SysView* views, const Work* bwork) {
   const Work w = bwork[blockIdx.x];
+  if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
   const int b = w.blk;
   const int s0 = sv.bin_start[b], cnt = sv.bin_start[b + 1] - s0;
@@ -956,6 +964,7 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_brick(const SysVie
   __shared__ real s_f[64][NF];
   __shared__ real s_red[kSpreadWaves - 1][2][8][64];
   const Work w = bwork[blockIdx.x];
+  if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
   if (check_conv && sv.st->conv) return;
   const int nb0 = sv.nbrick[0], nb1 = sv.nbrick[1], nb2 = sv.nbrick[2];
@@ -1126,12 +1135,16 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_brick(const SysVie
 // scalar-sum convolution between the forward and backward transforms
 // (qfac, newinduce_pme.f:1790-1861)
 template <typename real>
-__global__ void __launch_bounds__(256) k_conv(const SysView* views, int sys, size_t npts, int check_conv) {
-  const SysView& sv = views[sys];
+__global__ void __launch_bounds__(256) k_conv(const SysView* views, const Work* work, int check_conv) {
+  const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
+  const SysView& sv = views[w.sys];
   if (check_conv && sv.st->conv) return;
-  real* g = static_cast<real*>(sv.grid);
-  const real* q = static_cast<const real*>(sv.qfac);
-  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < npts; p += (size_t)gridDim.x * blockDim.x) {
+  const size_t npts = (size_t)sv.nfft[0] * sv.nfft[1] * sv.nfft[2];
+  GLOBAL real* g = gw(static_cast<real*>(sv.grid));
+  const GLOBAL real* q = gp(static_cast<const real*>(sv.qfac));
+  const size_t p0 = (size_t)w.blk * kConvChunk, p1 = min(npts, p0 + kConvChunk);
+  for (size_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
     const real f = q[p];
     g[2 * p] *= f;
     g[2 * p + 1] *= f;
@@ -1195,6 +1208,7 @@ template <typename real, int MODE>
 __global__ void __launch_bounds__(256) k_gather_uind(const SysView* views, const Work* work, int vin, int vout,
                                                       int frac_out) {
   const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
   if (MODE == 2 && sv.st->conv) return;
   const int r = threadIdx.x & 31;
@@ -1265,6 +1279,7 @@ template <typename real>
 __global__ void __launch_bounds__(64) k_gather_mpole(const SysView* views, const Work* work, double* cphi_out,
                                                       int add_to_ef) {
   const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
   const int i = w.blk * kBlock + threadIdx.x;
   if (i >= sv.n) return;
@@ -1364,7 +1379,9 @@ __global__ void __launch_bounds__(64) k_gather_mpole(const SysView* views, const
 // direct guess mu = alpha*ef (polgsf = 0, newinduce_pme2.f:180-188)
 __global__ void __launch_bounds__(64) k_guess(const SysView* views, const Work* work) {
   const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
+  if (sv.st->warm) return;
   const int i = w.blk * kBlock + threadIdx.x;
   if (i >= sv.n) return;
   const double a = sv.pol[i];
@@ -1372,16 +1389,20 @@ __global__ void __launch_bounds__(64) k_guess(const SysView* views, const Work* 
   for (int c = 0; c < 6; ++c) sv.mu[(size_t)c * sv.npad + i] = a * sv.ef[(size_t)c * sv.npad + i];
 }
 
-// pcg_b (newinduce_pme2.f:496-541) for iteration `it` (1-based):
+// pcg_b (newinduce_pme2.f:496-541) for iteration it = niter + 1 (the
+// iteration number lives in the solve state, so one launch can carry
+// systems at different iterations and captured launches can be replayed):
 // alpha = ggold/gg with ggold = sum of rz partials of iteration it-1;
 // mu += alpha pp; res -= alpha h; zr = diag*res; partial ggnew (into rz
 // buffer it&1) and ene.
 template <typename real>
-__global__ void __launch_bounds__(256) k_cg_update1(const SysView* views, const Work* work, int it) {
+__global__ void __launch_bounds__(256) k_cg_update1(const SysView* views, const Work* work) {
   const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
   SolveState* st = sv.st;
   if (st->conv) return;
+  const int it = st->niter + 1;
   const int G = sv.red_blocks, P = sv.gat_blocks;
   __shared__ double sc[4];
   __shared__ double scratch[8];
@@ -1396,9 +1417,10 @@ __global__ void __launch_bounds__(256) k_cg_update1(const SysView* views, const 
       sc[1] = go1 / g1;
       sc[2] = g0;
       sc[3] = g1;
-      if (w.blk == 0) {
+      if (w.blk == 0) {  // the only writer of these fields in this kernel
         st->gg[0] = g0;
         st->gg[1] = g1;
+        st->it_cur = it;
         if (g0 <= 0.0 || g1 <= 0.0) st->n_indef += 1;
       }
     }
@@ -1440,15 +1462,16 @@ __global__ void __launch_bounds__(256) k_cg_update1(const SysView* views, const 
   block_partial2(en[0], en[1], part_ene(sv), w.blk);
 }
 
-// pcg_newDirection + convergence (newinduce_pme2.f:543-568) for iteration
-// `it` (only its parity and whether it is the first matter, so captured
-// launches can be replayed): pp = zr + ggnew/ggold pp; residual norm sqrt(ggnew/(3 npolar)),
+// pcg_newDirection + convergence (newinduce_pme2.f:543-568) for the
+// iteration started by k_cg_update1 (it_cur): pp = zr + ggnew/ggold pp; residual norm sqrt(ggnew/(3 npolar)),
 // stop when the max over both rhs is below poleps.
-__global__ void __launch_bounds__(256) k_cg_update2(const SysView* views, const Work* work, int it) {
+__global__ void __launch_bounds__(256) k_cg_update2(const SysView* views, const Work* work) {
   const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
   SolveState* st = sv.st;
   if (st->conv) return;
+  const int it = st->it_cur;
   const int G = sv.red_blocks;
   __shared__ double sc[3];
   __shared__ double scratch[8];
@@ -1466,12 +1489,13 @@ __global__ void __launch_bounds__(256) k_cg_update2(const SysView* views, const 
       int conv = kRunning;
       if (!isfinite(gn0) || !isfinite(gn1)) conv = kNonFinite;
       else if (rn < st->poleps) conv = kConverged;
+      else if (it >= st->politer) conv = kMaxIter;
       sc[2] = conv;
       if (w.blk == 0) {  // the only writer (and reader) of these fields
         st->ggnew[0] = gn0;
         st->ggnew[1] = gn1;
         st->resnrm = rn;
-        st->niter += 1;
+        st->niter = it;
       }
     }
   }
@@ -1496,9 +1520,11 @@ __global__ void __launch_bounds__(256) k_cg_update2(const SysView* views, const 
 
 // E_pol = -1/2 f sum mu_d . E_p (epolar0c: real + recip + self); partials
 // into the energy buffer, summed by k_finish.
-__global__ void __launch_bounds__(256) k_energy(const SysView* views, const Work* work) {
+__global__ void __launch_bounds__(256) k_energy(const SysView* views, const Work* work, int only_done) {
   const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
+  if (only_done && sv.st->conv == kRunning) return;
   const size_t np = sv.npad;
   double e = 0.0;
   for (int i = w.blk * blockDim.x + threadIdx.x; i < sv.n; i += sv.red_blocks * blockDim.x) {
@@ -1509,8 +1535,9 @@ __global__ void __launch_bounds__(256) k_energy(const SysView* views, const Work
 }
 
 // one wave per system: final scalars into the SolveState
-__global__ void __launch_bounds__(64) k_finish(const SysView* views, const int* sys_ids) {
+__global__ void __launch_bounds__(64) k_finish(const SysView* views, const int* sys_ids, int only_done) {
   const SysView& sv = views[sys_ids[blockIdx.x]];
+  if (only_done && sv.st->conv == kRunning) return;
   const double e = sum_partials(part_ep(sv), sv.red_blocks, 0);
   const double e0 = sum_partials(part_ene(sv), sv.red_blocks, 0), e1 = sum_partials(part_ene(sv), sv.red_blocks, 1);
   if (threadIdx.x == 0) {
@@ -1518,6 +1545,48 @@ __global__ void __launch_bounds__(64) k_finish(const SysView* views, const int* 
     sv.st->ene[0] = e0;
     sv.st->ene[1] = e1;
   }
+}
+
+// start of a solve: fresh SolveState of every system of the launch
+// (warm[s] != 0: mu already holds the starting dipoles); a kernel rather
+// than a host copy, so that it can sit in a captured sequence
+__global__ void k_init_state(const SysView* views, const int* sys_ids, int nsys, double poleps, int politer,
+                             const int* warm) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nsys) return;
+  SolveState* st = views[sys_ids[q]].st;
+  SolveState z;
+  z.ggold[0] = z.ggold[1] = z.gg[0] = z.gg[1] = z.ggnew[0] = z.ggnew[1] = 0.0;
+  z.ene[0] = z.ene[1] = 0.0;
+  z.resnrm = 0.0;
+  z.poleps = poleps;
+  z.ep = 0.0;
+  z.niter = 0;
+  z.it_cur = 0;
+  z.politer = politer;
+  z.conv = kRunning;
+  z.n_indef = 0;
+  z.warm = warm ? warm[q] : 0;
+  *st = z;
+}
+
+// bin counters of the brick bins, zeroed before k_pme_prep counts into them
+__global__ void k_zero_bins(const SysView* views, const Work* bwork, int nwork) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nwork) return;
+  const Work w = bwork[q];
+  if (w.sys < 0) return;
+  views[w.sys].bin_count[w.blk] = 0;
+}
+
+// solve states and error flags of every system of the launch into out[]
+__global__ void k_collect(const SysView* views, const int* sys_ids, int nsys, StateOut* out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nsys) return;
+  const SysView& sv = views[sys_ids[q]];
+  out[q].st = *sv.st;
+  out[q].err = *sv.err;
+  out[q].pad = 0;
 }
 
 // permutation helpers between caller order and sorted order, [n][6] <-> [6][npad]

@@ -19,6 +19,7 @@ raises ThpError when the library, the device or a kernel fails.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 import os
 from dataclasses import dataclass, field
 
@@ -264,6 +265,7 @@ class Context:
     def __init__(self, device: int = 0):
         self.lib = load_lib()
         self.h = C.c_void_p()
+        self._solvers = weakref.WeakSet()
         rc = self.lib.thp_init(C.byref(self.h), device)
         if rc != THP_OK:
             msg = self.lib.thp_last_error(self.h).decode() if self.h else ""
@@ -275,6 +277,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for s in list(self._solvers):
+                s.close()
             self.lib.thp_finalize(self.h)
             self.h = C.c_void_p()
 
@@ -294,11 +298,13 @@ class Solver:
         self.h = C.c_void_p()
         d = system.desc(precision)
         ctx.check(self.lib.thp_system_create(ctx.h, C.byref(d), C.byref(self.h)), "thp_system_create")
+        ctx._solvers.add(self)
 
     def close(self):
-        if self.h:
+        # a finalized context has already released its systems
+        if self.h and self.ctx.h:
             self.lib.thp_system_destroy(self.h)
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
     def __del__(self):
         try:
