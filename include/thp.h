@@ -12,8 +12,9 @@
  *     global frame, quadrupole in the internal /3 convention), vectors over
  *     both right-hand sides as [n][2][3] (rhs 0 = d, rhs 1 = p), cphi[n][10].
  *   - boxes are orthorhombic (three edge lengths).
- *   - a thp_ctx is used from one host thread at a time, except
- *     thp_serve_submit / thp_serve_poll, which may be called from any thread.
+ *   - a thp_ctx is used from one host thread at a time.  While its serving
+ *     loop runs (thp_serve_start .. thp_serve_stop) only thp_serve_* calls
+ *     may be made on it, and those from any thread.
  *
  * Reference interfaces replaced (paths relative to the reference tree):
  *   thp_init/thp_finalize ...... C_init_env, C_get_cell
@@ -152,35 +153,67 @@ thp_status thp_induce_dev(thp_sys* sys, const thp_solve_opts* opts, double* mu_d
 thp_status thp_time_kernel(thp_sys* sys, int kernel, int reps, double* ms_per_launch, int64_t* pairs_in_cut);
 
 /* --------------------------------------------------------------- serving */
+/* Serving loop, no single reference counterpart: it replaces an MD driver
+ * that calls newinduce_pme2 once per system and step
+ * (v1.2/source/newinduce_pme2.f:16-283) with a persistent worker thread
+ * that solves many independent systems at once.  Requests are admitted as
+ * slots (max_systems) and device memory (hbm_budget) free up; the solves
+ * in flight share every kernel launch (each kernel walks a work table over
+ * all of them, systems at different CG iterations included) and their
+ * reciprocal grids of equal size are transformed by one batched FFT.
+ * Each system keeps its coordinates, lists, tables and last dipoles
+ * resident; the dipoles seed the next solve of the same system (the
+ * polgsf = 1 branch, newinduce_pme2.f:189-196). */
 typedef struct {
-  int max_systems;            /* resident systems (slots) */
-  int64_t hbm_budget;         /* device bytes the loop may hold; 0 = 80% of free */
-  int iters_per_graph;        /* CG iterations per captured graph launch */
-  int autotune;               /* 1: pick pair-tile/grid-block shapes by measurement */
-  double poleps;
-  int politer;
+  int max_systems;            /* solves in flight at once (slots), >= 1 */
+  int64_t hbm_budget;         /* device bytes for resident systems; 0 = 80% of free memory */
+  int iters_per_graph;        /* CG iterations per launch chunk between status reads, >= 1 */
+  int autotune;               /* 1: pick the pair kernel's wavefronts per block by measurement */
+  double poleps;              /* PCG tolerance of every solve */
+  int politer;                /* iteration cap of every solve */
+  int warm_start;             /* 1: start from the system's previous dipoles when it has them */
+  int use_graphs;             /* 1: replay each chunk as a captured HIP graph */
 } thp_serve_opts;
 
 typedef struct {
   uint64_t ticket;
   int system_id;
-  int status;                 /* THP_OK / THP_ERR_NOCONV / THP_ERR_NAN ... */
+  int status;                 /* THP_OK / THP_ERR_NOCONV / THP_ERR_NAN / THP_ERR_LIST ... */
   int niter;
-  double ep;
+  double ep;                  /* polarization energy, kcal/mol */
   double latency_ms;          /* submit -> result */
   double resnrm;
+  double queue_ms;            /* submit -> start of the solve */
+  int warm;                   /* 1 if started from the previous dipoles */
 } thp_result;
 
+typedef struct {
+  int64_t solves;             /* results published */
+  int64_t chunks;             /* launch chunks run */
+  double mean_active;         /* mean solves in flight per chunk */
+  int64_t graph_launches;     /* chunks replayed as graphs */
+  int64_t evictions;          /* resident systems released for memory */
+  int64_t resident_bytes;     /* device bytes held by resident systems */
+  int resident_systems;
+  int pair_segs;              /* wavefronts per block of the last chunk's pair kernel */
+} thp_serve_stats;
+
 thp_status thp_serve_start(thp_ctx* ctx, const thp_serve_opts* opts);
-/* register a system with the loop (its data stays resident while admitted) */
+/* register a system with the loop (host copy; device data is created when
+ * it is first solved and stays resident until evicted for memory) */
 thp_status thp_serve_add_system(thp_ctx* ctx, const thp_system_desc* desc, int* system_id);
-/* request a solve of system_id at new coordinates (NULL: unchanged) */
+/* request a solve of system_id at new coordinates (NULL: unchanged);
+ * requests of one system run in submission order */
 thp_status thp_serve_submit(thp_ctx* ctx, int system_id, const double* x, const double* y,
                             const double* z, uint64_t* ticket);
-/* 1 if ready (result filled), 0 if not yet; wait_ms<0 blocks */
+/* 1 if ready (result filled, ticket released), 0 if not yet; wait_ms < 0
+ * blocks until ready */
 thp_status thp_serve_poll(thp_ctx* ctx, uint64_t ticket, thp_result* result, double wait_ms);
-/* copy the last induced dipoles of a system [n][2][3] (original order) */
+/* dipoles [n][2][3] (original order) of the system's last finished solve;
+ * waits for a solve of that system in flight */
 thp_status thp_serve_get_dipoles(thp_ctx* ctx, int system_id, double* mu);
+thp_status thp_serve_stats_get(thp_ctx* ctx, thp_serve_stats* stats);
+/* stops the worker; solves not finished are reported THP_ERR_STATE */
 thp_status thp_serve_stop(thp_ctx* ctx);
 
 /* ------------------------------------------------------ synthetic inputs */

@@ -406,6 +406,29 @@ thp_status build_union(thp_sys* s) {
       s->ux0 = s->hx;
       s->uy0 = s->hy;
       s->uz0 = s->hz;
+      s->coef_valid = false;
+      // coefficient storage for the longest list
+      std::vector<int> lc(s->npad);
+      HIPCHK(s->ctx, hipMemcpy(lc.data(), s->view.lcount, sizeof(int) * s->npad, hipMemcpyDeviceToHost));
+      int cmax = 0;
+      for (int q = 0; q < s->n; ++q) cmax = std::max(cmax, lc[q]);
+      cmax = std::max(32, (cmax + 31) / 32 * 32);
+      if (cmax > s->view.cmax) {
+        dfree(s, s->view.pcoef);
+        s->view.pcoef = nullptr;
+        const size_t words = (size_t)cmax * s->npad * 2;
+        if (s->prec == THP_PREC_FP64) {
+          double* p = nullptr;
+          RET(dalloc(s, &p, words));
+          s->view.pcoef = p;
+        } else {
+          float* p = nullptr;
+          RET(dalloc(s, &p, words));
+          s->view.pcoef = p;
+        }
+        s->view.cmax = cmax;
+        RET(upload_view(s));
+      }
       return THP_OK;
     }
     // grow to the sizes the pass counted, then repeat it
@@ -451,6 +474,17 @@ thp_status ensure_lists(thp_sys* s) {
   return THP_OK;
 }
 
+// pair coefficients for the current coordinates (the permanent-field pass
+// writes them; ef is rewritten as a side effect)
+thp_status ensure_coef(thp_sys* s) {
+  RET(ensure_lists(s));
+  if (!s->coef_valid) {
+    RET(pair_efld0(s->ctx, s->one));
+    s->coef_valid = true;
+  }
+  return THP_OK;
+}
+
 // per coordinate set: B-spline values, grid nodes and brick bins
 thp_status b_pme_prep(thp_ctx* ctx, const Batch& b) {
   hipStream_t st = ctx->stream;
@@ -473,10 +507,25 @@ thp_status b_init_state(thp_ctx* ctx, const Batch& b, double poleps, int politer
   return launch_check(ctx, "k_init_state");
 }
 
+template <typename real>
+void launch_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int check_conv) {
+  hipStream_t st = ctx->stream;
+  const dim3 g = g1(b.n_blk);
+  switch (b.segs) {
+    case 1: hipLaunchKernelGGL((k_tmatxb_cl<real, 1>), g, dim3(64), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
+    case 2: hipLaunchKernelGGL((k_tmatxb_cl<real, 2>), g, dim3(128), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
+    case 8: hipLaunchKernelGGL((k_tmatxb_cl<real, 8>), g, dim3(512), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
+    case 16: hipLaunchKernelGGL((k_tmatxb_cl<real, 16>), g, dim3(1024), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
+    default: hipLaunchKernelGGL((k_tmatxb_cl<real, 4>), g, dim3(256), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
+  }
+}
+
+// real-space matvec; needs the pair coefficients of the current
+// coordinates (written by the permanent-field pass)
 thp_status b_pair_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int check_conv) {
-  if (b.prec == THP_PREC_FP64) launch_pair<double, 1>(ctx, b, vin, vout, dodiag, check_conv);
-  else launch_pair<float, 1>(ctx, b, vin, vout, dodiag, check_conv);
-  return launch_check(ctx, "k_pair_union<tmatxb>");
+  if (b.prec == THP_PREC_FP64) launch_tmatxb<double>(ctx, b, vin, vout, dodiag, check_conv);
+  else launch_tmatxb<float>(ctx, b, vin, vout, dodiag, check_conv);
+  return launch_check(ctx, "k_tmatxb_cl");
 }
 
 // Field, guess and first residual (newinduce_pme2.f:111-196, 406-440);
@@ -519,6 +568,8 @@ thp_status b_collect(thp_ctx* ctx, const Batch& b, StateOut* d_out) {
   return launch_check(ctx, "k_collect");
 }
 
+thp_status download_mu(thp_sys* s, double* host) { return gather_out(s, host, s->view.mu, 6); }
+
 void append_work(const thp_sys* s, int index, std::vector<Work>& blk, std::vector<Work>& red, std::vector<Work>& gat,
                  std::vector<Work>& brick, std::vector<Work>& conv) {
   for (int q = 0; q < s->nblk; ++q) blk.push_back({index, q});
@@ -552,6 +603,11 @@ thp_status thp_init(thp_ctx** out, int device) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
   std::call_once(g_fft_once, [] { rocfft_setup(); });
+  // THP_SEGS forces the pair kernels' wavefronts per 64-atom block
+  if (const char* e = getenv("THP_SEGS")) {
+    const int v = atoi(e);
+    if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) ctx->pair_segs = v;
+  }
   *out = ctx;
   return THP_OK;
 }
@@ -692,7 +748,9 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
   }
   v.aewald = s->aewald;
   v.cut2 = s->cut * s->cut;
+  v.cut = s->cut;
   s->skin = std::min(0.5, 0.5 * s->buffer);
+  v.skin = s->skin;
   v.ulim2 = (s->cut + s->skin) * (s->cut + s->skin);
   v.mbuf2 = mb * mb;
   v.term_self = (4.0 / 3.0) * s->aewald * s->aewald * s->aewald / std::sqrt(M_PI);
@@ -922,6 +980,7 @@ thp_status thp_system_update_coords(thp_sys* s, const double* x, const double* y
   if (!s->list_valid || maxu2 > lim_u * lim_u) s->tight_valid = false;
   s->verlet_valid = false;
   s->pme_valid = false;
+  s->coef_valid = false;
   return THP_OK;
 }
 
@@ -985,6 +1044,7 @@ thp_status thp_efld0_direct(thp_sys* s, double* ef) {
   if (!s || !ef) return THP_ERR_ARG;
   RET(ensure_lists(s));
   RET(pair_efld0(s->ctx, s->one));
+  s->coef_valid = true;
   return gather_out(s, ef, s->view.ef, 6);
 }
 
@@ -997,7 +1057,7 @@ thp_status thp_efld0_recip(thp_sys* s, double* cphi) {
 
 thp_status thp_tmatxb(thp_sys* s, const double* mu, double* efi, int dodiag) {
   if (!s || !mu || !efi) return THP_ERR_ARG;
-  RET(ensure_lists(s));
+  RET(ensure_coef(s));
   RET(scatter_in(s, s->view.pp, mu, 6));
   RET(b_pair_tmatxb(s->ctx, s->one, 1 /*pp*/, 3 /*hreal*/, dodiag, 0));
   return gather_out(s, efi, s->view.hreal, 6);
@@ -1005,7 +1065,7 @@ thp_status thp_tmatxb(thp_sys* s, const double* mu, double* efi, int dodiag) {
 
 thp_status thp_tmatxb_dev(thp_sys* s, const double* mu, double* efi, int dodiag) {
   if (!s || !mu || !efi) return THP_ERR_ARG;
-  RET(ensure_lists(s));
+  RET(ensure_coef(s));
   hipLaunchKernelGGL(k_scatter_in, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream, s->view.pp, mu, s->iperm_dev, s->n, s->npad, 6);
   RET(launch_check(s->ctx, "k_scatter_in"));
   RET(b_pair_tmatxb(s->ctx, s->one, 1, 3, dodiag, 0));
@@ -1098,6 +1158,7 @@ thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_s
   RET(ensure_pme(s));
   RET(b_init_state(ctx, b, o->poleps, o->politer, warm ? s->d_one : nullptr));
   RET(b_setup(ctx, b));
+  s->coef_valid = true;
   HIPCHK(ctx, hipEventRecord(s->ev[1], st));
   int conv = 0;
   for (int it = 1; it <= o->politer; ++it) {
@@ -1160,7 +1221,7 @@ thp_status thp_induce(thp_sys* s, const thp_solve_opts* o, double* uind, double*
 
 thp_status thp_time_kernel(thp_sys* s, int kernel, int reps, double* ms, int64_t* pairs) {
   if (!s || !ms || reps <= 0 || kernel < 0 || kernel > 2) return THP_ERR_ARG;
-  RET(ensure_lists(s));
+  RET(ensure_coef(s));
   RET(ensure_pme(s));
   hipStream_t st = s->ctx->stream;
   auto launch = [&]() -> thp_status {

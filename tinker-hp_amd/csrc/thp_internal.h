@@ -48,6 +48,7 @@ struct SysView {
   int prec;
   double box[3];
   double aewald, cut2, mbuf2, term_self;
+  double cut, skin;   // real-space cutoff; skin of the block-union lists
   double ulim2;       // (cut + skin)^2 of the block-union lists
   double afrac[3];  // nfft_k / box_k (orthorhombic fractional scaling)
   int nfft[3];
@@ -77,6 +78,11 @@ struct SysView {
   int* ucount;             // [nblk]
   unsigned short* llist;   // [lmax][npad] local partner indices into the block's union
   int* lcount;             // [npad]
+  // per-solve pair coefficients of the list entries: (dmp3, dmp5) of the
+  // real-space dipole field, zero beyond the cutoff; [cmax][npad] pairs of
+  // the pair type, written by the permanent-field pass
+  void* pcoef;
+  int cmax;
   // PME
   int brick[3];       // spreading brick edge per axis (4..8 points)
   int nbrick[3];
@@ -171,6 +177,7 @@ struct thp_sys {
   bool tight_valid = false;   // block-union lists match the coordinates
   bool verlet_valid = false;  // per-atom Verlet list matches the coordinates
   bool pme_valid = false;   // B-spline values and brick bins match the coordinates
+  bool coef_valid = false;  // pair coefficients match the coordinates and lists
   int nbricks = 0;
   thp::Work* d_brickwork = nullptr;  // [nbricks]
   int maxnb = 0;
@@ -223,4 +230,6 @@ struct StateOut {
   int pad;
 };
 thp_status b_collect(thp_ctx* ctx, const Batch& b, StateOut* d_out);
+// the system's dipoles [n][2][3] in caller order into host memory (synchronous)
+thp_status download_mu(thp_sys* s, double* host);
 }  // namespace thp

@@ -13,9 +13,10 @@
 //  * pair and grid arithmetic run in `real` (double for the plain FP64
 //    path, float for the mixed path); per-atom fields, CG vectors and CG
 //    scalars are FP64 everywhere.
-//  * reductions use per-block partials plus a last-block finalisation that
-//    sums them in a fixed order and writes the FP64 scalars to the device
-//    SolveState, so the PCG loop needs no host synchronisation.
+//  * reductions write per-block FP64 partials; the next kernel re-sums them
+//    in a fixed order (a kernel boundary orders the accesses), so the PCG
+//    scalars live on the device, need no atomics or fences, and are
+//    bit-identical from run to run.
 #pragma once
 #include "thp_internal.h"
 
@@ -151,6 +152,9 @@ __device__ __forceinline__ void block_partial2(double a, double b, double* part,
   __shared__ double red[2][4];
   a = wave_sum(a);
   b = wave_sum(b);
+  // a previous call's thread 0 may still be reading red: wait for it before
+  // the wavefronts overwrite their slots
+  __syncthreads();
   if ((threadIdx.x & 63) == 0) {
     red[0][threadIdx.x >> 6] = a;
     red[1][threadIdx.x >> 6] = b;
@@ -424,7 +428,7 @@ __device__ __forceinline__ void tmatxb_pair(const SysView& sv, const double* in,
 // distances.  Mirrors the reference's separate correction pass over
 // scaled pairs (GPU/source/tmatxb_pmecu.f:568-706, tmatxb_pmegpu.f:1435-1545).
 template <int KIND>
-__device__ __forceinline__ void scaled_pairs(const SysView& sv, const double* in, int i, double xi, double yi,
+__device__ __attribute__((noinline)) void scaled_pairs(const SysView& sv, const double* in, int i, double xi, double yi,
                                              double zi, double* acc) {
   const int e0 = sv.excl_off[i], e1 = sv.excl_off[i + 1];
   const double pdi = sv.pdamp[i], thi = sv.thole[i];
@@ -583,6 +587,28 @@ __global__ void __launch_bounds__(64 * kUnionWaves) k_union(const SysView* views
 // one per wavefront), and the S partial fields are summed in LDS in a fixed
 // order.  KIND 0: efld0_direct (permanent field, d and p); KIND 1:
 // tmatxb_pme (-E_real of the dipoles vin, plus diag(1/alpha) vin).
+// (dmp3, dmp5) of one list entry, as a two-wide vector (one 8- or 16-byte
+// access)
+template <typename real>
+struct Vec2;
+template <>
+struct Vec2<float> {
+  typedef float type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Vec2<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+template <typename real>
+using Coef2 = typename Vec2<real>::type;
+template <typename real>
+__device__ __forceinline__ Coef2<real> coef2(real a, real b) {
+  Coef2<real> v;
+  v.x = a;
+  v.y = b;
+  return v;
+}
+
 template <typename real, int KIND>
 struct UnionStage {
   // positions are kept in FP64 so that displacements are exact before
@@ -620,6 +646,7 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
   const double* in_g = vsel(sv, vin);
   const GLOBAL double* IN = gp(in_g);
   const GLOBAL unsigned short* LL = gp(sv.llist);
+  GLOBAL Coef2<real>* PC = gw(static_cast<Coef2<real>*>(sv.pcoef));
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int i = w.blk * kBlock + lane;
   const bool ok = i < n;
@@ -657,12 +684,16 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
           done = true;
           break;
         }
+        const size_t slot = (size_t)s * np + i;
         ++s;
         const int q = loc[b] - c0;
         const double dx = image_exact(s_pos[0][q] - xi, bx);
         const double dy = image_exact(s_pos[1][q] - yi, by);
         const double dz = image_exact(s_pos[2][q] - zi, bz);
-        if (!(r2_exact(dx, dy, dz) <= cut2)) continue;  // list entry within the skin only
+        if (!(r2_exact(dx, dy, dz) <= cut2)) {  // list entry within the skin only
+          if constexpr (KIND == 0) PC[slot] = coef2<real>(real(0), real(0));
+          continue;
+        }
         const real xr = real(dx), yr = real(dy), zr = real(dz);
         const real r2 = xr * xr + yr * yr + zr * zr;
         const real rr1 = rsqrt_t(r2);
@@ -688,6 +719,7 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
           const real dmp3 = bn[0] - (real(1) - l[0]) * rr3;
           const real dmp5 = bn[1] - (real(1) - l[1]) * rr5;
           const real dmp7 = bn[2] - (real(1) - l[2]) * rr7;
+          PC[slot] = coef2<real>(dmp3, dmp5);
           const real ai = dmp3 * ck - dmp5 * dkr + dmp7 * qkr;
           const real d5 = real(2) * dmp5;
           acc[0] += double(-xr * ai - dmp3 * dkx + d5 * qkx);
@@ -734,6 +766,147 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
       double v = acc[c];
       for (int q = 1; q < S; ++q) v += red[(q * 6 + c) * 64 + lane];
       if (KIND == 1 && dodiag) v += IN[(size_t)c * np + i] * ipol;
+      OUT[(size_t)c * np + i] = v;
+    }
+  }
+}
+
+// Real-space dipole field over the block-union lists with the per-solve
+// coefficients of the permanent-field pass (tmatxb_pme,
+// newinduce_pme.f:1548-1608: efi -= dlocal . u_k with dlocal = -dmp3 I +
+// dmp5 r r^T).  Union atoms are staged in LDS as positions relative to the
+// block centre plus their dipoles; an entry costs one LDS gather, its
+// coefficient pair (coalesced across the wavefront) and ~25 flops.  Entries
+// beyond the cutoff carry zero coefficients.  Blocks whose union could
+// reach across half the box take the minimum image of each displacement.
+template <typename real>
+struct ClStage {
+  static constexpr int CHUNK = (40 * 1024) / (9 * (int)sizeof(real));  // atoms per LDS chunk
+};
+
+template <typename real, int S>
+__global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, const Work* work, int vin, int vout,
+                                                       int dodiag, int check_conv) {
+  constexpr int C = ClStage<real>::CHUNK;
+  constexpr int NB = 8;  // list entries fetched per batch
+  __shared__ real s_p[3][C];
+  __shared__ real s_m[6][C];
+  __shared__ double red[S > 1 ? S * 6 * 64 : 1];
+  const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
+  const SysView& sv = views[w.sys];
+  if (check_conv && sv.st->conv) return;
+  const int n = sv.n;
+  const size_t np = sv.npad;
+  const double bx = sv.box[0], by = sv.box[1], bz = sv.box[2];
+  const GLOBAL double* X = gp(sv.x);
+  const GLOBAL double* Y = gp(sv.y);
+  const GLOBAL double* Z = gp(sv.z);
+  const double* in_g = vsel(sv, vin);
+  const GLOBAL double* IN = gp(in_g);
+  const GLOBAL unsigned short* LL = gp(sv.llist);
+  const GLOBAL Coef2<real>* PC = gp(static_cast<const Coef2<real>*>(sv.pcoef));
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = w.blk * kBlock + lane;
+  const bool ok = i < n;
+  const int ucount = gp(sv.ucount)[w.blk];
+  const GLOBAL int* UL = gp(sv.ulist) + (size_t)w.blk * sv.umax;
+  const GLOBAL double* BB = gp(sv.bbox) + (size_t)w.blk * 6;
+  const double cx = BB[0], cy = BB[1], cz = BB[2];
+  // union atoms lie within cut + skin of a block atom; the block's atoms
+  // moved at most skin/2 since its box was taken
+  const double lim = sv.cut + 2.0 * sv.skin;
+  const bool wrap = !(BB[3] + lim < 0.5 * bx && BB[4] + lim < 0.5 * by && BB[5] + lim < 0.5 * bz);
+  const real lx = real(bx), ly = real(by), lz = real(bz);
+  const real ilx = real(1.0 / bx), ily = real(1.0 / by), ilz = real(1.0 / bz);
+  const double xi = ok ? X[i] : 0.0, yi = ok ? Y[i] : 0.0, zi = ok ? Z[i] : 0.0;
+  real pix, piy, piz;
+  {
+    double d = xi - cx, e = yi - cy, f = zi - cz;
+    d -= bx * rint(d / bx);
+    e -= by * rint(e / by);
+    f -= bz * rint(f / bz);
+    pix = real(d);
+    piy = real(e);
+    piz = real(f);
+  }
+  const int cnt = ok ? gp(sv.lcount)[i] : 0;
+  const int s_end = (cnt * (wv + 1)) / S;
+  int s = (cnt * wv) / S;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int c0 = 0; c0 < ucount; c0 += C) {
+    const int cn = min(C, ucount - c0);
+    __syncthreads();
+    for (int q = threadIdx.x; q < cn; q += 64 * S) {
+      const int kk = UL[c0 + q];
+      double d = X[kk] - cx, e = Y[kk] - cy, f = Z[kk] - cz;
+      d -= bx * rint(d / bx);
+      e -= by * rint(e / by);
+      f -= bz * rint(f / bz);
+      s_p[0][q] = real(d);
+      s_p[1][q] = real(e);
+      s_p[2][q] = real(f);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) s_m[m][q] = real(IN[(size_t)m * np + kk]);
+    }
+    __syncthreads();
+    const int c1 = c0 + cn;
+    while (s < s_end) {
+      int loc[NB];
+      Coef2<real> cf[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        loc[b] = 0x7fffffff;
+        cf[b] = coef2<real>(real(0), real(0));
+        if (s + b < s_end) {
+          loc[b] = int(LL[(size_t)(s + b) * np + i]);
+          cf[b] = PC[(size_t)(s + b) * np + i];
+        }
+      }
+      bool done = false;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        if (loc[b] >= c1) {  // end of this lane's list or of the staged chunk
+          done = true;
+          break;
+        }
+        ++s;
+        const int q = loc[b] - c0;
+        real dx = s_p[0][q] - pix, dy = s_p[1][q] - piy, dz = s_p[2][q] - piz;
+        if (wrap) {
+          dx -= lx * rint(dx * ilx);
+          dy -= ly * rint(dy * ily);
+          dz -= lz * rint(dz * ilz);
+        }
+        const real d3 = cf[b].x, d5 = cf[b].y;
+#pragma unroll
+        for (int rhs = 0; rhs < 2; ++rhs) {
+          const real ux = s_m[rhs * 3 + 0][q], uy = s_m[rhs * 3 + 1][q], uz = s_m[rhs * 3 + 2][q];
+          const real ur = d5 * (dx * ux + dy * uy + dz * uz);
+          acc[rhs * 3 + 0] += double(d3 * ux - ur * dx);
+          acc[rhs * 3 + 1] += double(d3 * uy - ur * dy);
+          acc[rhs * 3 + 2] += double(d3 * uz - ur * dz);
+        }
+      }
+      if (done) break;
+    }
+  }
+  if (ok && wv == S - 1) scaled_pairs<1>(sv, in_g, i, xi, yi, zi, acc);
+  // fixed-order sum over the S wavefronts
+  if (S > 1) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) red[(wv * 6 + c) * 64 + lane] = acc[c];
+    __syncthreads();
+  }
+  if (wv == 0 && ok) {
+    const double pol = gp(sv.pol)[i];
+    const double ipol = pol == 0.0 ? 1.0 / kTinyPol : 1.0 / pol;
+    GLOBAL double* OUT = gw(vsel(sv, vout));
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double v = acc[c];
+      for (int q = 1; q < S; ++q) v += red[(q * 6 + c) * 64 + lane];
+      if (dodiag) v += IN[(size_t)c * np + i] * ipol;
       OUT[(size_t)c * np + i] = v;
     }
   }

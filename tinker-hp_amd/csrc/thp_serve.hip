@@ -1,11 +1,893 @@
-// Serving loop (continuous batching of independent solves).
+// Serving loop: continuous batching of independent polarization solves.
+//
+// One worker thread owns the device work of a context while the loop runs.
+// Per pass it (1) admits queued requests into free slots, creating the
+// device data of systems that are not resident (evicting idle ones when the
+// HBM budget is short), (2) runs the solve set-up (B-spline tables,
+// permanent field, guess or warm start, first residual) of the admitted
+// requests as one batch, (3) runs `iters_per_graph` PCG iterations of every
+// solve in flight as one batch (eagerly or as a replayed HIP graph), and
+// (4) reads every solve state back in one copy and retires the finished
+// solves.  Kernels walk work tables of (system, unit) entries, so solves at
+// different iterations share each launch; reciprocal grids of equal size
+// sit side by side in a pool and go through one batched rocFFT plan.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
 #include "thp_internal.h"
 
-extern "C" {
-thp_status thp_serve_start(thp_ctx* ctx, const thp_serve_opts*) { return ctx ? THP_ERR_STATE : THP_ERR_ARG; }
-thp_status thp_serve_add_system(thp_ctx* ctx, const thp_system_desc*, int*) { return ctx ? THP_ERR_STATE : THP_ERR_ARG; }
-thp_status thp_serve_submit(thp_ctx* ctx, int, const double*, const double*, const double*, uint64_t*) { return ctx ? THP_ERR_STATE : THP_ERR_ARG; }
-thp_status thp_serve_poll(thp_ctx* ctx, uint64_t, thp_result*, double) { return ctx ? THP_ERR_STATE : THP_ERR_ARG; }
-thp_status thp_serve_get_dipoles(thp_ctx* ctx, int, double*) { return ctx ? THP_ERR_STATE : THP_ERR_ARG; }
-thp_status thp_serve_stop(thp_ctx* ctx) { return ctx ? THP_OK : THP_ERR_ARG; }
+using namespace thp;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
 }
+
+// host copy of a registered system, enough to (re)create its device data
+struct ServeSystem {
+  int id = 0;
+  thp_system_desc desc{};
+  std::vector<double> x, y, z, rpole, pol, thole, dpu;
+  std::vector<int> ik;
+  thp_sys* s = nullptr;      // device data when resident
+  int64_t bytes = 0;         // device footprint once created (estimate before)
+  bool busy = false;         // a solve of it is in flight
+  bool has_mu = false;       // device mu holds its last solution
+  uint64_t last_use = 0;
+};
+
+struct Request {
+  uint64_t ticket = 0;
+  int sid = 0;
+  bool has_xyz = false;
+  std::vector<double> x, y, z;
+  Clock::time_point t_submit;
+};
+
+struct InFlight {
+  Request req;
+  ServeSystem* sys = nullptr;
+  Clock::time_point t_start;
+  int warm = 0;
+};
+
+struct DipoleCopy {  // thp_serve_get_dipoles handed to the worker
+  int sid = 0;
+  double* out = nullptr;
+  bool done = false;
+  thp_status rc = THP_OK;
+};
+
+// reciprocal grids of one size, side by side, and batched FFT plans
+struct GridPool {
+  int K[3] = {0, 0, 0};
+  int cap = 0;
+  void* base = nullptr;
+  size_t bytes_per_grid = 0;
+  std::map<int, FftExec> plans;  // by number of grids
+  std::vector<void*> work;       // rocFFT work buffers of the plans
+};
+
+// device and pinned host buffers of one batch layout
+struct BatchBuf {
+  Batch b;
+  SysView* views = nullptr;
+  Work* tables = nullptr;
+  int* ids = nullptr;
+  int* warm = nullptr;
+  size_t cap_views = 0, cap_tables = 0;
+  // pinned staging
+  SysView* h_views = nullptr;
+  Work* h_tables = nullptr;
+  int* h_ids = nullptr;
+  int* h_warm = nullptr;
+  size_t hcap_views = 0, hcap_tables = 0;
+  // graph of the current layout
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  int graph_iters = 0;
+};
+
+struct Serve {
+  thp_ctx* ctx = nullptr;
+  thp_serve_opts o{};
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request> queue;
+  std::unordered_map<uint64_t, thp_result> done;
+  std::vector<std::unique_ptr<ServeSystem>> systems;
+  std::deque<DipoleCopy*> copies;
+  uint64_t next_ticket = 1;
+  bool stop = false;
+  thp_status fatal = THP_OK;
+  std::string fatal_msg;
+  thp_serve_stats stats{};
+  double active_sum = 0.0;
+  std::thread worker;
+  // worker-owned
+  std::vector<InFlight> active;
+  int64_t resident = 0;
+  int64_t budget = 0;
+  uint64_t use_clock = 0;
+  std::map<std::tuple<int, int, int>, GridPool> pools;
+  int pool_generation = 0;
+  BatchBuf setup, iter;
+  bool iter_dirty = true;
+  int iter_generation = -1;
+  std::vector<int> iter_versions;  // view versions the iteration batch was built from
+  StateOut* d_out = nullptr;
+  StateOut* h_out = nullptr;
+  int out_cap = 0;
+  bool graphs_ok = true;
+  std::map<std::pair<int, int>, int> tuned;  // (log2 atoms, systems) -> pair waves per block
+};
+
+#define SRET(call)                   \
+  do {                               \
+    thp_status r_ = (call);          \
+    if (r_ != THP_OK) return r_;     \
+  } while (0)
+
+#define SHIP(ctx, call)                                                                                   \
+  do {                                                                                                   \
+    hipError_t e_ = (call);                                                                              \
+    if (e_ != hipSuccess) return fail((ctx), THP_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+// device footprint of a system before it exists (per-atom arrays, lists,
+// B-spline tables, scalar-sum table); replaced by the real figure once created
+int64_t estimate_bytes(const thp_system_desc& d) {
+  const int64_t np = (d.n + kBlock - 1) / kBlock * kBlock;
+  const int64_t pts = (int64_t)d.nfft[0] * d.nfft[1] * d.nfft[2];
+  const int64_t real = d.precision == THP_PREC_FP64 ? 8 : 4;
+  return np * 1792 + pts * real + (256 << 10);
+}
+
+thp_status grow_device(thp_ctx* ctx, void** p, size_t* cap, size_t need, size_t elem) {
+  if (need <= *cap) return THP_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  const size_t c = std::max(need, *cap * 2);
+  SHIP(ctx, hipMalloc(p, c * elem));
+  *cap = c;
+  return THP_OK;
+}
+
+thp_status grow_pinned(thp_ctx* ctx, void** p, size_t* cap, size_t need, size_t elem) {
+  if (need <= *cap) return THP_OK;
+  if (*p) (void)hipHostFree(*p);
+  *p = nullptr;
+  const size_t c = std::max(need, *cap * 2);
+  SHIP(ctx, hipHostMalloc(p, c * elem, hipHostMallocDefault));
+  *cap = c;
+  return THP_OK;
+}
+
+void drop_graph(BatchBuf& bb) {
+  if (bb.exec) (void)hipGraphExecDestroy(bb.exec);
+  if (bb.graph) (void)hipGraphDestroy(bb.graph);
+  bb.exec = nullptr;
+  bb.graph = nullptr;
+  bb.graph_iters = 0;
+}
+
+void free_batchbuf(BatchBuf& bb) {
+  drop_graph(bb);
+  for (void* p : {(void*)bb.views, (void*)bb.tables, (void*)bb.ids, (void*)bb.warm})
+    if (p) (void)hipFree(p);
+  for (void* p : {(void*)bb.h_views, (void*)bb.h_tables, (void*)bb.h_ids, (void*)bb.h_warm})
+    if (p) (void)hipHostFree(p);
+  bb = BatchBuf();
+}
+
+void free_pool(GridPool& gp) {
+  for (auto& kv : gp.plans) {
+    if (kv.second.fwd) rocfft_plan_destroy(kv.second.fwd);
+    if (kv.second.bwd) rocfft_plan_destroy(kv.second.bwd);
+    if (kv.second.info) rocfft_execution_info_destroy(kv.second.info);
+  }
+  for (void* w : gp.work) (void)hipFree(w);
+  if (gp.base) (void)hipFree(gp.base);
+  gp = GridPool();
+}
+
+// the batched plan over `count` grids of a pool (created on first use)
+thp_status pool_plan(Serve* sv, GridPool& gp, int count, int prec, FftExec* out) {
+  auto it = gp.plans.find(count);
+  if (it != gp.plans.end()) {
+    *out = it->second;
+    out->base = gp.base;
+    return THP_OK;
+  }
+  thp_ctx* ctx = sv->ctx;
+  FftExec f;
+  const size_t len[3] = {(size_t)gp.K[0], (size_t)gp.K[1], (size_t)gp.K[2]};
+  const rocfft_precision pr = prec == THP_PREC_FP64 ? rocfft_precision_double : rocfft_precision_single;
+  if (rocfft_plan_create(&f.fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward, pr, 3, len, (size_t)count,
+                         nullptr) != rocfft_status_success ||
+      rocfft_plan_create(&f.bwd, rocfft_placement_inplace, rocfft_transform_type_complex_inverse, pr, 3, len, (size_t)count,
+                         nullptr) != rocfft_status_success)
+    return fail(ctx, THP_ERR_FFT, "rocfft_plan_create (%d grids of %dx%dx%d) failed", count, gp.K[0], gp.K[1], gp.K[2]);
+  if (rocfft_execution_info_create(&f.info) != rocfft_status_success) return fail(ctx, THP_ERR_FFT, "execution info");
+  rocfft_execution_info_set_stream(f.info, ctx->stream);
+  size_t w1 = 0, w2 = 0;
+  rocfft_plan_get_work_buffer_size(f.fwd, &w1);
+  rocfft_plan_get_work_buffer_size(f.bwd, &w2);
+  const size_t wb = std::max(w1, w2);
+  if (wb) {
+    void* w = nullptr;
+    SHIP(ctx, hipMalloc(&w, wb));
+    gp.work.push_back(w);
+    rocfft_execution_info_set_work_buffer(f.info, w, wb);
+  }
+  gp.plans[count] = f;
+  *out = f;
+  out->base = gp.base;
+  return THP_OK;
+}
+
+// Build a batch over `members` (views, work tables, grid slots, FFT plans)
+// into bb.  Grids are scratch between kernels of one pass, so each batch
+// takes pool slots 0..m-1 of each grid size afresh.
+thp_status build_batch(Serve* sv, BatchBuf& bb, const std::vector<thp_sys*>& members, const std::vector<int>& warm) {
+  thp_ctx* ctx = sv->ctx;
+  const int m = (int)members.size();
+  std::vector<Work> blk, red, gat, brick, conv;
+  for (int q = 0; q < m; ++q) append_work(members[q], q, blk, red, gat, brick, conv);
+  // grid slots per size
+  std::map<std::tuple<int, int, int>, std::vector<int>> groups;
+  for (int q = 0; q < m; ++q) groups[{members[q]->nfft[0], members[q]->nfft[1], members[q]->nfft[2]}].push_back(q);
+  const int prec = members[0]->prec;
+  const size_t real = prec == THP_PREC_FP64 ? 8 : 4;
+  for (auto& g : groups) {
+    GridPool& gp = sv->pools[g.first];
+    const int need = (int)g.second.size();
+    if (gp.cap < need) {
+      // grow: nothing in flight reads the old grids (the stream is idle
+      // between passes), and every graph that captured them is dropped
+      SHIP(ctx, hipStreamSynchronize(ctx->stream));
+      free_pool(gp);
+      gp.K[0] = std::get<0>(g.first);
+      gp.K[1] = std::get<1>(g.first);
+      gp.K[2] = std::get<2>(g.first);
+      gp.cap = std::max(need, 2);
+      while (gp.cap < need) gp.cap *= 2;
+      gp.bytes_per_grid = 2 * real * (size_t)gp.K[0] * gp.K[1] * gp.K[2];
+      SHIP(ctx, hipMalloc(&gp.base, gp.bytes_per_grid * gp.cap));
+      ++sv->pool_generation;
+      drop_graph(sv->setup);
+      drop_graph(sv->iter);
+      sv->iter_dirty = true;
+    }
+  }
+  // views
+  SRET(grow_pinned(ctx, (void**)&bb.h_views, &bb.hcap_views, m, sizeof(SysView)));
+  const size_t nt = blk.size() + red.size() + gat.size() + brick.size() + conv.size();
+  SRET(grow_pinned(ctx, (void**)&bb.h_tables, &bb.hcap_tables, nt, sizeof(Work)));
+  size_t cap_ids = bb.cap_views;
+  SRET(grow_device(ctx, (void**)&bb.views, &bb.cap_views, m, sizeof(SysView)));
+  SRET(grow_device(ctx, (void**)&bb.tables, &bb.cap_tables, nt, sizeof(Work)));
+  if (!bb.ids || cap_ids < bb.cap_views) {
+    if (bb.ids) (void)hipFree(bb.ids);
+    if (bb.warm) (void)hipFree(bb.warm);
+    if (bb.h_ids) (void)hipHostFree(bb.h_ids);
+    if (bb.h_warm) (void)hipHostFree(bb.h_warm);
+    SHIP(ctx, hipMalloc((void**)&bb.ids, bb.cap_views * sizeof(int)));
+    SHIP(ctx, hipMalloc((void**)&bb.warm, bb.cap_views * sizeof(int)));
+    SHIP(ctx, hipHostMalloc((void**)&bb.h_ids, bb.cap_views * sizeof(int), hipHostMallocDefault));
+    SHIP(ctx, hipHostMalloc((void**)&bb.h_warm, bb.cap_views * sizeof(int), hipHostMallocDefault));
+  }
+  Batch& b = bb.b;
+  b = Batch();
+  b.prec = prec;
+  b.nsys = m;
+  b.fft.clear();
+  for (auto& g : groups) {
+    GridPool& gp = sv->pools[g.first];
+    for (size_t slot = 0; slot < g.second.size(); ++slot) {
+      const int q = g.second[slot];
+      bb.h_views[q] = members[q]->view;
+      bb.h_views[q].grid = static_cast<char*>(gp.base) + slot * gp.bytes_per_grid;
+    }
+    FftExec f;
+    SRET(pool_plan(sv, gp, (int)g.second.size(), prec, &f));
+    b.fft.push_back(f);
+  }
+  for (int q = 0; q < m; ++q) {
+    bb.h_ids[q] = q;
+    bb.h_warm[q] = q < (int)warm.size() ? warm[q] : 0;
+  }
+  size_t off = 0;
+  auto put = [&](const std::vector<Work>& v, Work** dptr, int* cnt) {
+    std::copy(v.begin(), v.end(), bb.h_tables + off);
+    *dptr = bb.tables + off;
+    *cnt = (int)v.size();
+    off += v.size();
+  };
+  put(blk, &b.d_blk, &b.n_blk);
+  put(red, &b.d_red, &b.n_red);
+  put(gat, &b.d_gat, &b.n_gat);
+  put(brick, &b.d_brick, &b.n_brick);
+  put(conv, &b.d_conv, &b.n_conv);
+  b.d_views = bb.views;
+  b.d_ids = bb.ids;
+  hipStream_t st = ctx->stream;
+  SHIP(ctx, hipMemcpyAsync(bb.views, bb.h_views, sizeof(SysView) * m, hipMemcpyHostToDevice, st));
+  SHIP(ctx, hipMemcpyAsync(bb.tables, bb.h_tables, sizeof(Work) * nt, hipMemcpyHostToDevice, st));
+  SHIP(ctx, hipMemcpyAsync(bb.ids, bb.h_ids, sizeof(int) * m, hipMemcpyHostToDevice, st));
+  SHIP(ctx, hipMemcpyAsync(bb.warm, bb.h_warm, sizeof(int) * m, hipMemcpyHostToDevice, st));
+  drop_graph(bb);
+  return THP_OK;
+}
+
+thp_status release_system(Serve* sv, ServeSystem* ss) {
+  if (!ss->s) return THP_OK;
+  sv->resident -= ss->bytes;
+  thp_system_destroy(ss->s);
+  ss->s = nullptr;
+  ss->has_mu = false;
+  sv->stats.evictions += 1;
+  return THP_OK;
+}
+
+// make a system resident within the budget; false in *ok when it cannot
+// fit now (idle residents are released, least recently used first)
+thp_status admit(Serve* sv, ServeSystem* ss, const std::vector<ServeSystem*>& pinned, bool* ok) {
+  *ok = true;
+  if (ss->s) return THP_OK;
+  const int64_t need = ss->bytes;
+  if (need > sv->budget) return fail(sv->ctx, THP_ERR_NOMEM, "system of %d atoms needs %lld bytes, over the budget %lld",
+                                     ss->desc.n, (long long)need, (long long)sv->budget);
+  while (sv->resident + need > sv->budget) {
+    ServeSystem* victim = nullptr;
+    for (auto& u : sv->systems) {
+      ServeSystem* c = u.get();
+      if (!c->s || c->busy || c == ss) continue;
+      if (std::find(pinned.begin(), pinned.end(), c) != pinned.end()) continue;
+      if (!victim || c->last_use < victim->last_use) victim = c;
+    }
+    if (!victim) {
+      *ok = false;
+      return THP_OK;
+    }
+    SRET(release_system(sv, victim));
+  }
+  thp_sys* s = nullptr;
+  SRET(create_system(sv->ctx, &ss->desc, &s, false));
+  ss->s = s;
+  ss->bytes = s->bytes;
+  sv->resident += ss->bytes;
+  ss->has_mu = false;
+  return THP_OK;
+}
+
+int bucket_log2(int64_t v) {
+  int b = 0;
+  while ((int64_t(1) << (b + 1)) <= v) ++b;
+  return b;
+}
+
+// pair kernel shape for the iteration batch: measured once per (size,
+// batch) class over the candidates and remembered in the context
+thp_status tune_pair(Serve* sv, Batch& b, const std::vector<thp_sys*>& members) {
+  thp_ctx* ctx = sv->ctx;
+  int64_t atoms = 0;
+  for (thp_sys* s : members) atoms += s->n;
+  const auto key = std::make_pair(bucket_log2(atoms), std::min(b.nsys, 64));
+  auto it = sv->tuned.find(key);
+  if (it != sv->tuned.end()) {
+    b.segs = it->second;
+    return THP_OK;
+  }
+  if (ctx->pair_segs > 0 || !sv->o.autotune) {
+    b.segs = ctx->pair_segs > 0 ? ctx->pair_segs : kDefaultSegs;
+    sv->tuned[key] = b.segs;
+    return THP_OK;
+  }
+  // every candidate rewrites hreal from pp, which the next iteration does
+  // again before anything reads hreal
+  static const int cand[] = {2, 4, 8, 16};
+  hipEvent_t e0, e1;
+  SHIP(ctx, hipEventCreate(&e0));
+  SHIP(ctx, hipEventCreate(&e1));
+  float best = 1e30f;
+  int best_s = kDefaultSegs;
+  for (int c : cand) {
+    b.segs = c;
+    SRET(b_pair_tmatxb(ctx, b, 1, 3, 1, 1));  // warm
+    SHIP(ctx, hipEventRecord(e0, ctx->stream));
+    for (int r = 0; r < 3; ++r) SRET(b_pair_tmatxb(ctx, b, 1, 3, 1, 1));
+    SHIP(ctx, hipEventRecord(e1, ctx->stream));
+    SHIP(ctx, hipEventSynchronize(e1));
+    float t = 0;
+    SHIP(ctx, hipEventElapsedTime(&t, e0, e1));
+    if (t < best) {
+      best = t;
+      best_s = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b.segs = best_s;
+  sv->tuned[key] = best_s;
+  return THP_OK;
+}
+
+thp_status run_chunk(Serve* sv, int iters) {
+  thp_ctx* ctx = sv->ctx;
+  BatchBuf& bb = sv->iter;
+  if (sv->o.use_graphs && sv->graphs_ok) {
+    if (!bb.exec || bb.graph_iters != iters) {
+      drop_graph(bb);
+      hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+      thp_status r = e == hipSuccess ? THP_OK : THP_ERR_HIP;
+      for (int k = 0; k < iters && r == THP_OK; ++k) r = b_iteration(ctx, bb.b);
+      hipGraph_t g = nullptr;
+      if (e == hipSuccess) {
+        hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+        if (e2 != hipSuccess) r = THP_ERR_HIP;
+      }
+      hipGraphExec_t x = nullptr;
+      if (r == THP_OK && g && hipGraphInstantiate(&x, g, nullptr, nullptr, 0) == hipSuccess) {
+        bb.graph = g;
+        bb.exec = x;
+        bb.graph_iters = iters;
+      } else {
+        // capture is not available for this sequence: launch eagerly from now on
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        sv->graphs_ok = false;
+      }
+    }
+    if (bb.exec) {
+      SHIP(ctx, hipGraphLaunch(bb.exec, ctx->stream));
+      sv->stats.graph_launches += 1;
+      return THP_OK;
+    }
+  }
+  for (int k = 0; k < iters; ++k) SRET(b_iteration(ctx, bb.b));
+  return THP_OK;
+}
+
+thp_status handle_copies(Serve* sv, std::deque<DipoleCopy*>& pending) {
+  for (auto it = pending.begin(); it != pending.end();) {
+    DipoleCopy* c = *it;
+    ServeSystem* ss = c->sid >= 0 && c->sid < (int)sv->systems.size() ? sv->systems[c->sid].get() : nullptr;
+    if (ss && ss->busy) {
+      ++it;
+      continue;
+    }
+    thp_status rc = THP_OK;
+    if (!ss) rc = THP_ERR_ARG;
+    else if (!ss->s || !ss->has_mu) rc = fail(sv->ctx, THP_ERR_STATE, "system %d has no resident solution", c->sid);
+    else rc = download_mu(ss->s, c->out);
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      c->rc = rc;
+      c->done = true;
+    }
+    sv->cv_done.notify_all();
+    it = pending.erase(it);
+  }
+  return THP_OK;
+}
+
+void publish(Serve* sv, const InFlight& f, const StateOut& so, Clock::time_point now) {
+  thp_result r{};
+  r.ticket = f.req.ticket;
+  r.system_id = f.req.sid;
+  r.niter = so.st.niter;
+  r.ep = so.st.ep;
+  r.resnrm = so.st.resnrm;
+  r.latency_ms = ms_between(f.req.t_submit, now);
+  r.queue_ms = ms_between(f.req.t_submit, f.t_start);
+  r.warm = f.warm;
+  if (so.err) r.status = so.err;
+  else if (so.st.conv == kConverged || so.st.conv == kZeroCurv) r.status = THP_OK;
+  else if (so.st.conv == kNonFinite) r.status = THP_ERR_NAN;
+  else r.status = THP_ERR_NOCONV;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  sv->done[r.ticket] = r;
+  sv->stats.solves += 1;
+}
+
+void publish_error(Serve* sv, const Request& q, thp_status code, Clock::time_point t_start) {
+  thp_result r{};
+  r.ticket = q.ticket;
+  r.system_id = q.sid;
+  r.status = code;
+  const auto now = Clock::now();
+  r.latency_ms = ms_between(q.t_submit, now);
+  r.queue_ms = ms_between(q.t_submit, t_start);
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->done[r.ticket] = r;
+    sv->stats.solves += 1;
+  }
+  sv->cv_done.notify_all();
+}
+
+// one pass of the loop; returns a fatal status only for device failures
+thp_status pass(Serve* sv, std::vector<Request>& starting) {
+  thp_ctx* ctx = sv->ctx;
+  const auto t_pass = Clock::now();
+  // ---- admission: residency, coordinates, lists
+  std::vector<InFlight> admitted;
+  std::vector<ServeSystem*> pinned;
+  for (auto& a : sv->active) pinned.push_back(a.sys);
+  std::vector<Request> deferred;
+  for (auto& q : starting) {
+    ServeSystem* ss = sv->systems[q.sid].get();
+    bool ok = true;
+    const thp_status ra = admit(sv, ss, pinned, &ok);
+    if (ra == THP_ERR_HIP) return ra;
+    if (ra != THP_OK) {
+      publish_error(sv, q, ra, t_pass);
+      continue;
+    }
+    if (!ok) {
+      deferred.push_back(std::move(q));
+      continue;
+    }
+    pinned.push_back(ss);
+    ss->last_use = ++sv->use_clock;
+    thp_status rc = THP_OK;
+    if (q.has_xyz) rc = thp_system_update_coords(ss->s, q.x.data(), q.y.data(), q.z.data());
+    if (rc == THP_OK) rc = ensure_lists(ss->s);
+    if (rc != THP_OK) {
+      if (rc == THP_ERR_HIP) return rc;
+      publish_error(sv, q, rc, t_pass);
+      continue;
+    }
+    InFlight f;
+    f.req = std::move(q);
+    f.sys = ss;
+    f.t_start = t_pass;
+    f.warm = sv->o.warm_start && ss->has_mu ? 1 : 0;
+    ss->busy = true;
+    admitted.push_back(std::move(f));
+  }
+  if (!deferred.empty()) {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (auto it = deferred.rbegin(); it != deferred.rend(); ++it) sv->queue.push_front(std::move(*it));
+  }
+  // ---- set-up of the admitted solves as one batch
+  if (!admitted.empty()) {
+    std::vector<thp_sys*> mem;
+    std::vector<int> warm;
+    for (auto& f : admitted) {
+      mem.push_back(f.sys->s);
+      warm.push_back(f.warm);
+    }
+    SRET(build_batch(sv, sv->setup, mem, warm));
+    Batch& b = sv->setup.b;
+    b.segs = ctx->pair_segs > 0 ? ctx->pair_segs : kDefaultSegs;
+    SRET(b_init_state(ctx, b, sv->o.poleps, sv->o.politer, sv->setup.warm));
+    SRET(b_pme_prep(ctx, b));
+    SRET(b_setup(ctx, b));
+    for (auto& f : admitted) {
+      f.sys->s->pme_valid = true;
+      f.sys->s->coef_valid = true;
+      sv->active.push_back(std::move(f));
+    }
+    sv->iter_dirty = true;
+  }
+  if (sv->active.empty()) return THP_OK;
+  // ---- PCG iterations of everything in flight
+  std::vector<thp_sys*> mem;
+  for (auto& a : sv->active) mem.push_back(a.sys->s);
+  if (!sv->iter_dirty) {
+    for (size_t q = 0; q < mem.size(); ++q)
+      if (q >= sv->iter_versions.size() || sv->iter_versions[q] != mem[q]->view_version) sv->iter_dirty = true;
+  }
+  if (sv->iter_dirty || sv->iter_generation != sv->pool_generation) {
+    SRET(build_batch(sv, sv->iter, mem, {}));
+    SRET(tune_pair(sv, sv->iter.b, mem));
+    sv->iter_dirty = false;
+    sv->iter_generation = sv->pool_generation;
+    sv->iter_versions.clear();
+    for (thp_sys* s : mem) sv->iter_versions.push_back(s->view_version);
+  }
+  const int iters = std::max(1, sv->o.iters_per_graph);
+  SRET(run_chunk(sv, iters));
+  sv->stats.chunks += 1;
+  sv->active_sum += (double)sv->active.size();
+  sv->stats.pair_segs = sv->iter.b.segs;
+  // ---- energies of the finished solves, states back in one copy
+  Batch& b = sv->iter.b;
+  SRET(b_finish(ctx, b, 1));
+  if (sv->out_cap < b.nsys) {
+    if (sv->d_out) (void)hipFree(sv->d_out);
+    if (sv->h_out) (void)hipHostFree(sv->h_out);
+    sv->out_cap = std::max(b.nsys, 2 * sv->out_cap);
+    SHIP(ctx, hipMalloc((void**)&sv->d_out, sizeof(StateOut) * sv->out_cap));
+    SHIP(ctx, hipHostMalloc((void**)&sv->h_out, sizeof(StateOut) * sv->out_cap, hipHostMallocDefault));
+  }
+  SRET(b_collect(ctx, b, sv->d_out));
+  SHIP(ctx, hipMemcpyAsync(sv->h_out, sv->d_out, sizeof(StateOut) * b.nsys, hipMemcpyDeviceToHost, ctx->stream));
+  SHIP(ctx, hipStreamSynchronize(ctx->stream));
+  const auto now = Clock::now();
+  std::vector<InFlight> still;
+  for (size_t q = 0; q < sv->active.size(); ++q) {
+    const StateOut& so = sv->h_out[q];
+    InFlight& f = sv->active[q];
+    if (so.err == 0 && so.st.conv == kRunning) {
+      still.push_back(std::move(f));
+      continue;
+    }
+    f.sys->busy = false;
+    f.sys->has_mu = so.err == 0 && (so.st.conv == kConverged || so.st.conv == kZeroCurv);
+    f.sys->last_use = ++sv->use_clock;
+    publish(sv, f, so, now);
+  }
+  if (still.size() != sv->active.size()) {
+    sv->active.swap(still);
+    sv->iter_dirty = true;
+    sv->cv_done.notify_all();
+  }
+  return THP_OK;
+}
+
+void worker_main(Serve* sv) {
+  thp_ctx* ctx = sv->ctx;
+  (void)hipSetDevice(ctx->device);
+  std::deque<DipoleCopy*> copies;
+  for (;;) {
+    std::vector<Request> starting;
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      sv->cv_work.wait(lk, [&] {
+        return sv->stop || !sv->active.empty() || !sv->copies.empty() || !copies.empty() ||
+               (!sv->queue.empty() && (int)sv->active.size() < sv->o.max_systems);
+      });
+      if (sv->stop) break;
+      while (!sv->copies.empty()) {
+        copies.push_back(sv->copies.front());
+        sv->copies.pop_front();
+      }
+      // FIFO admission into free slots; one solve per system at a time
+      int free_slots = sv->o.max_systems - (int)sv->active.size();
+      std::vector<int> taken;
+      for (auto it = sv->queue.begin(); it != sv->queue.end() && free_slots > 0;) {
+        ServeSystem* ss = sv->systems[it->sid].get();
+        if (ss->busy || std::find(taken.begin(), taken.end(), it->sid) != taken.end()) {
+          taken.push_back(it->sid);  // keeps later requests of this system behind it
+          ++it;
+          continue;
+        }
+        taken.push_back(it->sid);
+        starting.push_back(std::move(*it));
+        it = sv->queue.erase(it);
+        --free_slots;
+      }
+    }
+    thp_status rc = handle_copies(sv, copies);
+    if (rc == THP_OK) rc = pass(sv, starting);
+    if (rc != THP_OK) {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      sv->fatal = rc;
+      sv->fatal_msg = ctx->err;
+      sv->stop = true;
+      break;
+    }
+    if (sv->active.empty() && starting.empty() && copies.empty()) {
+      // nothing ran in this pass: wait for work instead of spinning
+      std::unique_lock<std::mutex> lk(sv->mu);
+      sv->cv_work.wait_for(lk, std::chrono::milliseconds(1));
+    }
+  }
+  // cancel what did not finish
+  const auto now = Clock::now();
+  for (auto& f : sv->active) {
+    f.sys->busy = false;
+    publish_error(sv, f.req, sv->fatal != THP_OK ? sv->fatal : THP_ERR_STATE, f.t_start);
+  }
+  sv->active.clear();
+  std::lock_guard<std::mutex> lk(sv->mu);
+  for (auto& q : sv->queue) {
+    thp_result r{};
+    r.ticket = q.ticket;
+    r.system_id = q.sid;
+    r.status = sv->fatal != THP_OK ? sv->fatal : THP_ERR_STATE;
+    r.latency_ms = ms_between(q.t_submit, now);
+    sv->done[r.ticket] = r;
+  }
+  sv->queue.clear();
+  for (DipoleCopy* c : copies) {
+    c->rc = THP_ERR_STATE;
+    c->done = true;
+  }
+  for (DipoleCopy* c : sv->copies) {
+    c->rc = THP_ERR_STATE;
+    c->done = true;
+  }
+  sv->copies.clear();
+  sv->cv_done.notify_all();
+}
+
+Serve* serve_of(thp_ctx* ctx) { return ctx ? static_cast<Serve*>(ctx->serve) : nullptr; }
+
+}  // namespace
+
+extern "C" {
+
+thp_status thp_serve_start(thp_ctx* ctx, const thp_serve_opts* opts) {
+  if (!ctx || !opts) return THP_ERR_ARG;
+  if (ctx->serve) return fail(ctx, THP_ERR_STATE, "serving loop already running");
+  if (opts->max_systems < 1 || opts->iters_per_graph < 1 || !(opts->poleps > 0) || opts->politer < 1 ||
+      opts->hbm_budget < 0)
+    return fail(ctx, THP_ERR_ARG, "bad serve options");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, THP_ERR_HIP, "hipSetDevice");
+  Serve* sv = new Serve;
+  sv->ctx = ctx;
+  sv->o = *opts;
+  if (opts->hbm_budget > 0) sv->budget = opts->hbm_budget;
+  else {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+      delete sv;
+      return fail(ctx, THP_ERR_HIP, "hipMemGetInfo");
+    }
+    sv->budget = (int64_t)(0.8 * (double)fr);
+  }
+  ctx->serve = sv;
+  sv->worker = std::thread(worker_main, sv);
+  return THP_OK;
+}
+
+thp_status thp_serve_add_system(thp_ctx* ctx, const thp_system_desc* d, int* system_id) {
+  Serve* sv = serve_of(ctx);
+  if (!sv) return ctx ? fail(ctx, THP_ERR_STATE, "serving loop not started") : THP_ERR_ARG;
+  if (!d || !system_id || d->n <= 0 || d->n >= kMaxAtoms || !d->x || !d->y || !d->z || !d->rpole || !d->polarity ||
+      !d->thole || d->nscaled < 0 || (d->nscaled > 0 && (!d->scaled_ik || !d->scaled_dpu)))
+    return THP_ERR_ARG;
+  auto ss = std::make_unique<ServeSystem>();
+  const int n = d->n;
+  ss->x.assign(d->x, d->x + n);
+  ss->y.assign(d->y, d->y + n);
+  ss->z.assign(d->z, d->z + n);
+  ss->rpole.assign(d->rpole, d->rpole + (size_t)13 * n);
+  ss->pol.assign(d->polarity, d->polarity + n);
+  ss->thole.assign(d->thole, d->thole + n);
+  if (d->nscaled > 0) {
+    ss->ik.assign(d->scaled_ik, d->scaled_ik + (size_t)2 * d->nscaled);
+    ss->dpu.assign(d->scaled_dpu, d->scaled_dpu + (size_t)3 * d->nscaled);
+  }
+  ss->desc = *d;
+  ss->desc.x = ss->x.data();
+  ss->desc.y = ss->y.data();
+  ss->desc.z = ss->z.data();
+  ss->desc.rpole = ss->rpole.data();
+  ss->desc.polarity = ss->pol.data();
+  ss->desc.thole = ss->thole.data();
+  ss->desc.scaled_ik = ss->ik.empty() ? nullptr : ss->ik.data();
+  ss->desc.scaled_dpu = ss->dpu.empty() ? nullptr : ss->dpu.data();
+  ss->bytes = estimate_bytes(ss->desc);
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (!sv->systems.empty() && sv->systems[0]->desc.precision != d->precision)
+    return THP_ERR_ARG;  // one precision per loop
+  ss->id = (int)sv->systems.size();
+  *system_id = ss->id;
+  sv->systems.push_back(std::move(ss));
+  return THP_OK;
+}
+
+thp_status thp_serve_submit(thp_ctx* ctx, int sid, const double* x, const double* y, const double* z,
+                            uint64_t* ticket) {
+  Serve* sv = serve_of(ctx);
+  if (!sv) return ctx ? THP_ERR_STATE : THP_ERR_ARG;
+  if (!ticket || (x && (!y || !z))) return THP_ERR_ARG;
+  Request q;
+  q.sid = sid;
+  q.t_submit = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (sv->stop) return sv->fatal != THP_OK ? sv->fatal : THP_ERR_STATE;
+    if (sid < 0 || sid >= (int)sv->systems.size()) return THP_ERR_ARG;
+  }
+  if (x) {
+    int n = 0;
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      n = sv->systems[sid]->desc.n;
+    }
+    q.has_xyz = true;
+    q.x.assign(x, x + n);
+    q.y.assign(y, y + n);
+    q.z.assign(z, z + n);
+  }
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    q.ticket = sv->next_ticket++;
+    *ticket = q.ticket;
+    sv->queue.push_back(std::move(q));
+  }
+  sv->cv_work.notify_one();
+  return THP_OK;
+}
+
+thp_status thp_serve_poll(thp_ctx* ctx, uint64_t ticket, thp_result* result, double wait_ms) {
+  Serve* sv = serve_of(ctx);
+  if (!sv) return ctx ? THP_ERR_STATE : THP_ERR_ARG;
+  if (!result) return THP_ERR_ARG;
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto ready = [&] { return sv->done.count(ticket) > 0 || sv->stop; };
+  if (wait_ms < 0) sv->cv_done.wait(lk, ready);
+  else if (wait_ms > 0) sv->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(wait_ms), ready);
+  auto it = sv->done.find(ticket);
+  if (it == sv->done.end()) return sv->stop ? (sv->fatal != THP_OK ? sv->fatal : THP_ERR_STATE) : 0;
+  *result = it->second;
+  sv->done.erase(it);
+  return 1;
+}
+
+thp_status thp_serve_get_dipoles(thp_ctx* ctx, int sid, double* mu) {
+  Serve* sv = serve_of(ctx);
+  if (!sv) return ctx ? THP_ERR_STATE : THP_ERR_ARG;
+  if (!mu) return THP_ERR_ARG;
+  DipoleCopy c;
+  c.sid = sid;
+  c.out = mu;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (sv->stop) return THP_ERR_STATE;
+    if (sid < 0 || sid >= (int)sv->systems.size()) return THP_ERR_ARG;
+    sv->copies.push_back(&c);
+  }
+  sv->cv_work.notify_one();
+  std::unique_lock<std::mutex> lk(sv->mu);
+  sv->cv_done.wait(lk, [&] { return c.done; });
+  return c.rc;
+}
+
+thp_status thp_serve_stats_get(thp_ctx* ctx, thp_serve_stats* out) {
+  Serve* sv = serve_of(ctx);
+  if (!sv) return ctx ? THP_ERR_STATE : THP_ERR_ARG;
+  if (!out) return THP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  *out = sv->stats;
+  out->mean_active = sv->stats.chunks ? sv->active_sum / (double)sv->stats.chunks : 0.0;
+  out->resident_bytes = sv->resident;
+  int nres = 0;
+  for (auto& s : sv->systems) nres += s->s ? 1 : 0;
+  out->resident_systems = nres;
+  return THP_OK;
+}
+
+thp_status thp_serve_stop(thp_ctx* ctx) {
+  if (!ctx) return THP_ERR_ARG;
+  Serve* sv = serve_of(ctx);
+  if (!sv) return THP_OK;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stop = true;
+  }
+  sv->cv_work.notify_all();
+  if (sv->worker.joinable()) sv->worker.join();
+  (void)hipStreamSynchronize(ctx->stream);
+  free_batchbuf(sv->setup);
+  free_batchbuf(sv->iter);
+  for (auto& kv : sv->pools) free_pool(kv.second);
+  if (sv->d_out) (void)hipFree(sv->d_out);
+  if (sv->h_out) (void)hipHostFree(sv->h_out);
+  for (auto& s : sv->systems)
+    if (s->s) thp_system_destroy(s->s);
+  const thp_status fatal = sv->fatal;
+  if (fatal != THP_OK) ctx->err = sv->fatal_msg;
+  ctx->serve = nullptr;
+  delete sv;
+  return fatal;
+}
+
+}  // extern "C"

@@ -75,6 +75,7 @@ class ServeOpts(C.Structure):
     _fields_ = [
         ("max_systems", C.c_int), ("hbm_budget", C.c_int64), ("iters_per_graph", C.c_int),
         ("autotune", C.c_int), ("poleps", C.c_double), ("politer", C.c_int),
+        ("warm_start", C.c_int), ("use_graphs", C.c_int),
     ]
 
 
@@ -82,6 +83,15 @@ class Result(C.Structure):
     _fields_ = [
         ("ticket", C.c_uint64), ("system_id", C.c_int), ("status", C.c_int), ("niter", C.c_int),
         ("ep", C.c_double), ("latency_ms", C.c_double), ("resnrm", C.c_double),
+        ("queue_ms", C.c_double), ("warm", C.c_int),
+    ]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [
+        ("solves", C.c_int64), ("chunks", C.c_int64), ("mean_active", C.c_double),
+        ("graph_launches", C.c_int64), ("evictions", C.c_int64), ("resident_bytes", C.c_int64),
+        ("resident_systems", C.c_int), ("pair_segs", C.c_int),
     ]
 
 
@@ -124,6 +134,7 @@ def load_lib(path: str | None = None) -> C.CDLL:
         "thp_serve_submit": (C.c_int, [vp, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_uint64)]),
         "thp_serve_poll": (C.c_int, [vp, C.c_uint64, C.POINTER(Result), C.c_double]),
         "thp_serve_get_dipoles": (C.c_int, [vp, C.c_int, _dp]),
+        "thp_serve_stats_get": (C.c_int, [vp, C.POINTER(ServeStats)]),
         "thp_serve_stop": (C.c_int, [vp]),
         "thp_time_kernel": (C.c_int, [vp, C.c_int, C.c_int, _dp, C.POINTER(C.c_int64)]),
         "thp_make_system": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_double, C.c_uint64,

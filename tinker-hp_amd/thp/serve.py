@@ -1,0 +1,186 @@
+"""Serving loop front end: continuous batching of independent solves.
+
+`Server` wraps thp_serve_* (a worker thread inside libthp owns the device):
+systems are registered once (`add`), solves are requested at new
+coordinates (`submit`) and collected by ticket (`poll`).  `bench_serving`
+drives it closed-loop on copies of one workload and reports solves/s and
+p50/p99 latency; `poisson_stream` replays a Poisson arrival stream of
+mixed-size systems.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import (THP_OK, THP_PREC_MIXED, Result, ServeOpts, ServeStats, System, ThpError, _d,
+               serving_system)
+
+
+class Server:
+    def __init__(self, ctx, max_systems=8, hbm_budget=0, iters_per_graph=4, autotune=1, poleps=1e-8,
+                 politer=500, warm_start=1, use_graphs=0):
+        self.ctx, self.lib = ctx, ctx.lib
+        o = ServeOpts(max_systems=max_systems, hbm_budget=hbm_budget, iters_per_graph=iters_per_graph,
+                      autotune=autotune, poleps=poleps, politer=politer, warm_start=warm_start,
+                      use_graphs=use_graphs)
+        ctx.check(self.lib.thp_serve_start(ctx.h, C.byref(o)), "thp_serve_start")
+        self.systems: list[System] = []
+        self._open = True
+
+    def add(self, system: System, precision: int = THP_PREC_MIXED) -> int:
+        d = system.desc(precision)
+        sid = C.c_int(-1)
+        self.ctx.check(self.lib.thp_serve_add_system(self.ctx.h, C.byref(d), C.byref(sid)), "thp_serve_add_system")
+        self.systems.append(system)
+        return sid.value
+
+    def submit(self, sid: int, x=None, y=None, z=None) -> int:
+        t = C.c_uint64(0)
+        if x is None:
+            rc = self.lib.thp_serve_submit(self.ctx.h, sid, None, None, None, C.byref(t))
+        else:
+            x, y, z = (np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z))
+            rc = self.lib.thp_serve_submit(self.ctx.h, sid, _d(x), _d(y), _d(z), C.byref(t))
+        self.ctx.check(rc, "thp_serve_submit")
+        return t.value
+
+    def poll(self, ticket: int, wait_ms: float = -1.0):
+        r = Result()
+        rc = self.lib.thp_serve_poll(self.ctx.h, ticket, C.byref(r), wait_ms)
+        if rc < 0:
+            raise ThpError(rc, "thp_serve_poll")
+        return r if rc == 1 else None
+
+    def dipoles(self, sid: int) -> np.ndarray:
+        n = self.systems[sid].n
+        mu = np.zeros((n, 2, 3))
+        self.ctx.check(self.lib.thp_serve_get_dipoles(self.ctx.h, sid, _d(mu)), "thp_serve_get_dipoles")
+        return mu
+
+    def stats(self) -> ServeStats:
+        s = ServeStats()
+        self.ctx.check(self.lib.thp_serve_stats_get(self.ctx.h, C.byref(s)), "thp_serve_stats_get")
+        return s
+
+    def stop(self):
+        if self._open:
+            self._open = False
+            rc = self.lib.thp_serve_stop(self.ctx.h)
+            if rc != THP_OK:
+                raise ThpError(rc, self.lib.thp_last_error(self.ctx.h).decode())
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.stop()
+
+
+def _pct(v, q):
+    return float(np.percentile(np.asarray(v), q)) if len(v) else None
+
+
+def bench_serving(ctx, system: System, steps: int = 20, warmup: int = 2, poleps: float = 1e-8, copies: int = 8,
+                  jitter: float = 0.01, warm_start: int = 1, iters_per_graph: int = 4, use_graphs: int = 0,
+                  seed: int = 11) -> dict:
+    """Closed loop over `copies` resident systems with the geometry of
+    `system` (each a separate system; coordinates perturbed by N(0, jitter)
+    per request): `copies` requests stay outstanding, each result triggers
+    the next request of that system.  `warmup` rounds per system are not
+    timed.  Returns solves/s over the timed requests and latency
+    percentiles."""
+    rng = np.random.default_rng(seed)
+    srv = Server(ctx, max_systems=copies, iters_per_graph=iters_per_graph, poleps=poleps,
+                 warm_start=warm_start, use_graphs=use_graphs)
+    try:
+        sids = [srv.add(system) for _ in range(copies)]
+
+        def coords():
+            return tuple(v + rng.normal(0.0, jitter, system.n) for v in (system.x, system.y, system.z))
+
+        # untimed rounds: device data creation, lists, plans, tuning
+        for _ in range(max(1, warmup)):
+            ts = [srv.submit(s, *coords()) for s in sids]
+            for t in ts:
+                r = srv.poll(t)
+                if r.status != THP_OK:
+                    raise ThpError(r.status, "serving warm-up solve")
+        total = steps * copies
+        lat, nit, eps = [], [], []
+        outstanding = {}
+        t0 = time.perf_counter()
+        issued = 0
+        for s in sids:
+            outstanding[srv.submit(s, *coords())] = s
+            issued += 1
+        while outstanding:
+            # results arrive in completion order; wait on the oldest ticket
+            t = min(outstanding)
+            r = srv.poll(t)
+            s = outstanding.pop(t)
+            if r.status != THP_OK:
+                raise ThpError(r.status, "serving solve")
+            lat.append(r.latency_ms)
+            nit.append(r.niter)
+            eps.append(r.ep)
+            if issued < total:
+                outstanding[srv.submit(s, *coords())] = s
+                issued += 1
+        dt = time.perf_counter() - t0
+        st = srv.stats()
+        return dict(solves_per_s=total / dt, ms_per_solve=1e3 * dt / total, solves=total, copies=copies,
+                    niter=float(np.mean(nit)), ep=float(eps[0]), p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99),
+                    mean_active=st.mean_active, pair_segs=st.pair_segs, warm_start=warm_start,
+                    iters_per_graph=iters_per_graph, graphs=int(st.graph_launches > 0))
+    finally:
+        srv.stop()
+
+
+def poisson_stream(ctx, n_requests=2000, rate_per_s=100.0, resubmits=10, n_min=500, n_max=30000, max_systems=16,
+                   poleps=1e-8, seed=5, jitter=0.01, warm_start=1, iters_per_graph=4, use_graphs=0):
+    """Poisson arrivals of solves of independent mixed-size systems: sizes
+    log-uniform in [n_min, n_max], 0-30% solute; each system is requested
+    `resubmits` times at coordinates perturbed by N(0, jitter).  Returns
+    throughput and latency percentiles (submit -> result)."""
+    rng = np.random.default_rng(seed)
+    n_sys = max(1, n_requests // resubmits)
+    srv = Server(ctx, max_systems=max_systems, iters_per_graph=iters_per_graph, poleps=poleps,
+                 warm_start=warm_start, use_graphs=use_graphs)
+    try:
+        systems = []
+        for k in range(n_sys):
+            n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+            systems.append(serving_system(n, float(rng.uniform(0.0, 0.3)), seed=1000 + k))
+        sids = [srv.add(s) for s in systems]
+        order = np.repeat(np.arange(n_sys), resubmits)
+        rng.shuffle(order)
+        order = order[:n_requests]
+        gaps = rng.exponential(1.0 / rate_per_s, len(order))
+        tickets = []
+        t0 = time.perf_counter()
+        t_next = t0
+        for q, k in enumerate(order):
+            t_next += gaps[q]
+            delay = t_next - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            s = systems[k]
+            x, y, z = (v + rng.normal(0.0, jitter, s.n) for v in (s.x, s.y, s.z))
+            tickets.append(srv.submit(sids[k], x, y, z))
+        lat, bad, atoms = [], 0, 0
+        for t in tickets:
+            r = srv.poll(t)
+            if r.status != THP_OK:
+                bad += 1
+            lat.append(r.latency_ms)
+        dt = time.perf_counter() - t0
+        st = srv.stats()
+        for k in order:
+            atoms += systems[k].n
+        return dict(requests=len(order), failed=bad, seconds=dt, solves_per_s=len(order) / dt,
+                    offered_rate=rate_per_s, p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99),
+                    mean_active=st.mean_active, evictions=st.evictions, mean_atoms=atoms / len(order))
+    finally:
+        srv.stop()
