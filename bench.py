@@ -70,6 +70,8 @@ def main():
     ap.add_argument("--poleps", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-stream", action="store_true")
+    ap.add_argument("--stream-requests", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -101,6 +103,18 @@ def main():
             log(f"bench: serve {serve}")
         except (ImportError, AttributeError, thp.ThpError) as e:
             log(f"bench: serving leg unavailable: {e}")
+
+    # Poisson stream of mixed-size systems at 1.5x the plain one-system rate
+    stream = None
+    if not args.no_serve and not args.no_stream:
+        try:
+            import thp.serve as tserve
+            rate = 1.5 * legs["plain"]["solves_per_s"]
+            stream = tserve.poisson_stream(ctx, n_requests=args.stream_requests, rate_per_s=rate,
+                                           poleps=args.poleps)
+            log(f"bench: stream {stream}")
+        except (ImportError, AttributeError, thp.ThpError) as e:
+            log(f"bench: stream leg unavailable: {e}")
 
     # live roofline of the dominant kernel (mixed real-space matvec)
     ms, pairs = mixed.time_kernel(0, reps=30)
@@ -150,6 +164,8 @@ def main():
             "speedup_vs_plain": round(value / legs["plain"]["solves_per_s"], 3),
             "p50_ms": serve["p50_ms"] if serve else None,
             "p99_ms": serve["p99_ms"] if serve else None,
+            "serve": serve,
+            "stream": stream,
             "niter": legs[best]["niter"],
             "errors_vs_fp64_oracle": err or None,
         },

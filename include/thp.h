@@ -196,6 +196,14 @@ typedef struct {
   int64_t resident_bytes;     /* device bytes held by resident systems */
   int resident_systems;
   int pair_segs;              /* wavefronts per block of the last chunk's pair kernel */
+  int64_t batch_builds;       /* iteration-batch layouts built (tables, graph) */
+  double host_ms;             /* worker time spent enqueueing and admitting */
+  double wait_ms;             /* worker time spent waiting for the device */
+  double chunk_ms;            /*   of host_ms: enqueueing the iteration chunks */
+  double admit_ms;            /*   of host_ms: admission (residency, coordinates, lists) */
+  double setup_ms;            /*   of host_ms: enqueueing the set-up batches */
+  double coords_ms;           /*   of admit_ms: coordinate updates */
+  double lists_ms;            /*   of admit_ms: list checks and rebuilds */
 } thp_serve_stats;
 
 thp_status thp_serve_start(thp_ctx* ctx, const thp_serve_opts* opts);
@@ -207,7 +215,7 @@ thp_status thp_serve_add_system(thp_ctx* ctx, const thp_system_desc* desc, int* 
 thp_status thp_serve_submit(thp_ctx* ctx, int system_id, const double* x, const double* y,
                             const double* z, uint64_t* ticket);
 /* 1 if ready (result filled, ticket released), 0 if not yet; wait_ms < 0
- * blocks until ready */
+ * blocks until ready.  ticket 0 takes any finished request (the oldest). */
 thp_status thp_serve_poll(thp_ctx* ctx, uint64_t ticket, thp_result* result, double wait_ms);
 /* dipoles [n][2][3] (original order) of the system's last finished solve;
  * waits for a solve of that system in flight */

@@ -129,9 +129,11 @@ def test_continuous_admission_more_requests_than_slots(ctx):
 
 def test_hbm_budget_evicts_idle_systems(ctx):
     systems = [thp.serving_system(3000, 0.0, seed=60 + k) for k in range(4)]
-    one = thp.Solver(ctx, systems[0], MIXED)
-    fp = one.footprint()
-    one.close()
+    # footprint of one resident serving system (its grid lives in the pool)
+    with Server(ctx, max_systems=1) as srv:
+        srv.poll(srv.submit(srv.add(systems[0])))
+        fp = srv.stats().resident_bytes
+    assert fp > 0
     with Server(ctx, max_systems=1, hbm_budget=int(2.5 * fp), warm_start=1) as srv:
         sids = [srv.add(s) for s in systems]
         res = [srv.poll(srv.submit(sid)) for sid in sids + sids]

@@ -273,9 +273,17 @@ void launch_pair(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, in
   }
 }
 
+// direct permanent field + the pair coefficients and compacted lists that
+// the matvec of the same coordinates uses
 thp_status pair_efld0(thp_ctx* ctx, const Batch& b) {
-  if (b.prec == THP_PREC_FP64) launch_pair<double, 0>(ctx, b, 0, 0, 0, 0);
-  else launch_pair<float, 0>(ctx, b, 0, 0, 0, 0);
+  if (b.prec == THP_PREC_FP64) {
+    launch_pair<double, 0>(ctx, b, 0, 0, 0, 0);
+    hipLaunchKernelGGL(k_compact_lists<double>, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk);
+  } else {
+    launch_pair<float, 0>(ctx, b, 0, 0, 0, 0);
+    hipLaunchKernelGGL(k_compact_lists<float>, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk);
+  }
+  hipLaunchKernelGGL(k_scaled<0>, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, 0, 0, 0);
   return launch_check(ctx, "k_pair_union<efld0>");
 }
 
@@ -287,16 +295,32 @@ thp_status zero_grid(thp_sys* s) {
   return launch_check(s->ctx, "k_zero_grid");
 }
 
+constexpr int spread_threads(int prec) { return 64 * (prec == THP_PREC_FP64 ? SpreadTileCfg<double>::WAVES : SpreadTileCfg<float>::WAVES); }
+
+// grid_uind_site: tiles of the home atoms, then the merge into the grid
 thp_status spread_uind(thp_ctx* ctx, const Batch& b, int vin, int frac, int check_conv) {
-  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_spread_brick<double, 0>), g1(b.n_brick), dim3(64 * kSpreadWaves), 0, ctx->stream, b.d_views, b.d_brick, vin, frac, check_conv);
-  else hipLaunchKernelGGL((k_spread_brick<float, 0>), g1(b.n_brick), dim3(64 * kSpreadWaves), 0, ctx->stream, b.d_views, b.d_brick, vin, frac, check_conv);
-  return launch_check(ctx, "k_spread_brick<uind>");
+  const dim3 t(spread_threads(b.prec));
+  if (b.prec == THP_PREC_FP64) {
+    hipLaunchKernelGGL((k_spread_tile<double, 0>), g1(b.n_brick), t, 0, ctx->stream, b.d_views, b.d_brick, vin, frac, check_conv);
+    hipLaunchKernelGGL((k_spread_merge<double, 0>), g1(b.n_conv), dim3(256), 0, ctx->stream, b.d_views, b.d_conv, check_conv);
+  } else {
+    hipLaunchKernelGGL((k_spread_tile<float, 0>), g1(b.n_brick), t, 0, ctx->stream, b.d_views, b.d_brick, vin, frac, check_conv);
+    hipLaunchKernelGGL((k_spread_merge<float, 0>), g1(b.n_conv), dim3(256), 0, ctx->stream, b.d_views, b.d_conv, check_conv);
+  }
+  return launch_check(ctx, "k_spread_tile<uind>");
 }
 
+// grid_mpole_site (real part, imaginary part cleared)
 thp_status spread_mpole(thp_ctx* ctx, const Batch& b) {
-  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_spread_brick<double, 1>), g1(b.n_brick), dim3(64 * kSpreadWaves), 0, ctx->stream, b.d_views, b.d_brick, 0, 0, 0);
-  else hipLaunchKernelGGL((k_spread_brick<float, 1>), g1(b.n_brick), dim3(64 * kSpreadWaves), 0, ctx->stream, b.d_views, b.d_brick, 0, 0, 0);
-  return launch_check(ctx, "k_spread_brick<mpole>");
+  const dim3 t(spread_threads(b.prec));
+  if (b.prec == THP_PREC_FP64) {
+    hipLaunchKernelGGL((k_spread_tile<double, 1>), g1(b.n_brick), t, 0, ctx->stream, b.d_views, b.d_brick, 0, 0, 0);
+    hipLaunchKernelGGL((k_spread_merge<double, 1>), g1(b.n_conv), dim3(256), 0, ctx->stream, b.d_views, b.d_conv, 0);
+  } else {
+    hipLaunchKernelGGL((k_spread_tile<float, 1>), g1(b.n_brick), t, 0, ctx->stream, b.d_views, b.d_brick, 0, 0, 0);
+    hipLaunchKernelGGL((k_spread_merge<float, 1>), g1(b.n_conv), dim3(256), 0, ctx->stream, b.d_views, b.d_conv, 0);
+  }
+  return launch_check(ctx, "k_spread_tile<mpole>");
 }
 
 thp_status convolve(thp_ctx* ctx, const Batch& b, int check_conv) {
@@ -316,9 +340,9 @@ thp_status convolve(thp_ctx* ctx, const Batch& b, int check_conv) {
 
 template <int MODE>
 thp_status launch_gather_uind(thp_ctx* ctx, const Batch& b, int vin, int vout, int frac_out) {
-  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_uind<double, MODE>), g1(b.n_gat), dim3(256), 0, ctx->stream, b.d_views, b.d_gat, vin, vout, frac_out);
-  else hipLaunchKernelGGL((k_gather_uind<float, MODE>), g1(b.n_gat), dim3(256), 0, ctx->stream, b.d_views, b.d_gat, vin, vout, frac_out);
-  return launch_check(ctx, "k_gather_uind");
+  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_tile<double, MODE>), g1(b.n_gat), dim3(64), 0, ctx->stream, b.d_views, b.d_gat, vin, vout, frac_out);
+  else hipLaunchKernelGGL((k_gather_tile<float, MODE>), g1(b.n_gat), dim3(64), 0, ctx->stream, b.d_views, b.d_gat, vin, vout, frac_out);
+  return launch_check(ctx, "k_gather_tile");
 }
 
 // permanent field: ef = E_direct - grad phi_recip + self (newinduce_pme2.f:111-147)
@@ -415,16 +439,24 @@ thp_status build_union(thp_sys* s) {
       cmax = std::max(32, (cmax + 31) / 32 * 32);
       if (cmax > s->view.cmax) {
         dfree(s, s->view.pcoef);
-        s->view.pcoef = nullptr;
+        dfree(s, s->view.tcoef);
+        dfree(s, s->view.tlist);
+        s->view.pcoef = s->view.tcoef = nullptr;
+        s->view.tlist = nullptr;
+        RET(dalloc(s, &s->view.tlist, (size_t)cmax * s->npad));
         const size_t words = (size_t)cmax * s->npad * 2;
         if (s->prec == THP_PREC_FP64) {
-          double* p = nullptr;
+          double *p = nullptr, *t = nullptr;
           RET(dalloc(s, &p, words));
+          RET(dalloc(s, &t, words));
           s->view.pcoef = p;
+          s->view.tcoef = t;
         } else {
-          float* p = nullptr;
+          float *p = nullptr, *t = nullptr;
           RET(dalloc(s, &p, words));
+          RET(dalloc(s, &t, words));
           s->view.pcoef = p;
+          s->view.tcoef = t;
         }
         s->view.cmax = cmax;
         RET(upload_view(s));
@@ -485,6 +517,65 @@ thp_status ensure_coef(thp_sys* s) {
   return THP_OK;
 }
 
+// New coordinates (caller order).  Host work: sorted copy and the
+// displacement checks that decide which lists stay valid.  With `pinned`
+// (3*npad doubles of pinned host memory, untouched until the stream passes
+// the copies) the upload is enqueued without waiting; otherwise it is
+// synchronous.
+thp_status update_coords(thp_sys* s, const double* x, const double* y, const double* z, double* pinned) {
+  const int n = s->n, np = s->npad;
+  // sorted copy in ordinary host memory (pinned memory is only written,
+  // never read back, from the host)
+  std::vector<double>& hx = s->hx;
+  std::vector<double>& hy = s->hy;
+  std::vector<double>& hz = s->hz;
+  hx.resize(n);
+  hy.resize(n);
+  hz.resize(n);
+  double maxd2 = 0.0, maxu2 = 0.0;
+  for (int q = 0; q < n; ++q) {
+    const int a = s->perm[q];
+    hx[q] = x[a];
+    hy[q] = y[a];
+    hz[q] = z[a];
+    if (s->list_valid) {
+      const double dx = hx[q] - s->bx0[q], dy = hy[q] - s->by0[q], dz = hz[q] - s->bz0[q];
+      maxd2 = std::max(maxd2, dx * dx + dy * dy + dz * dz);
+    }
+    if (s->tight_valid) {
+      const double dx = hx[q] - s->ux0[q], dy = hy[q] - s->uy0[q], dz = hz[q] - s->uz0[q];
+      maxu2 = std::max(maxu2, dx * dx + dy * dy + dz * dz);
+    }
+  }
+  std::vector<double> local;
+  double* buf = pinned;
+  if (!buf) {
+    local.resize((size_t)3 * np);
+    buf = local.data();
+  }
+  for (int c = 0; c < 3; ++c) {
+    const std::vector<double>& src = c == 0 ? hx : c == 1 ? hy : hz;
+    double* dst = buf + (size_t)c * np;
+    std::memcpy(dst, src.data(), sizeof(double) * n);
+    std::memset(dst + n, 0, sizeof(double) * (np - n));
+  }
+  hipStream_t st = s->ctx->stream;
+  HIPCHK(s->ctx, hipMemcpyAsync((void*)s->view.x, buf, sizeof(double) * np, hipMemcpyHostToDevice, st));
+  HIPCHK(s->ctx, hipMemcpyAsync((void*)s->view.y, buf + np, sizeof(double) * np, hipMemcpyHostToDevice, st));
+  HIPCHK(s->ctx, hipMemcpyAsync((void*)s->view.z, buf + 2 * (size_t)np, sizeof(double) * np, hipMemcpyHostToDevice, st));
+  if (!pinned) RET(sync(s));
+  // block candidates (cut + buffer) stay valid while no atom moved more than
+  // (buffer - skin)/2 since their build, the block-union lists (cut + skin)
+  // while no atom moved more than skin/2
+  const double lim_b = 0.5 * (s->buffer - s->skin), lim_u = 0.5 * s->skin;
+  if (s->list_valid && maxd2 > lim_b * lim_b) s->list_valid = false;
+  if (!s->list_valid || maxu2 > lim_u * lim_u) s->tight_valid = false;
+  s->verlet_valid = false;
+  s->pme_valid = false;
+  s->coef_valid = false;
+  return THP_OK;
+}
+
 // per coordinate set: B-spline values, grid nodes and brick bins
 thp_status b_pme_prep(thp_ctx* ctx, const Batch& b) {
   hipStream_t st = ctx->stream;
@@ -525,6 +616,7 @@ void launch_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, 
 thp_status b_pair_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int check_conv) {
   if (b.prec == THP_PREC_FP64) launch_tmatxb<double>(ctx, b, vin, vout, dodiag, check_conv);
   else launch_tmatxb<float>(ctx, b, vin, vout, dodiag, check_conv);
+  hipLaunchKernelGGL(k_scaled<1>, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, vin, vout, check_conv);
   return launch_check(ctx, "k_tmatxb_cl");
 }
 
@@ -739,7 +831,12 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
   v.npad = np;
   v.nblk = s->nblk;
   v.red_blocks = std::max(1, std::min(256, (n + 255) / 256));
-  v.gat_blocks = std::max(1, std::min(4096, (n + kGatherAtoms - 1) / kGatherAtoms));
+  for (int k = 0; k < 3; ++k) {
+    v.brick[k] = pick_brick(s->nfft[k]);
+    v.nbrick[k] = (s->nfft[k] + v.brick[k] - 1) / v.brick[k];
+  }
+  s->nbricks = v.nbrick[0] * v.nbrick[1] * v.nbrick[2];
+  v.gat_blocks = s->nbricks;  // one gather workgroup (and partial pair) per brick
   v.prec = s->prec;
   for (int k = 0; k < 3; ++k) {
     v.box[k] = s->box[k];
@@ -782,7 +879,7 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
   v.scale_tab = dtab; v.excl_off = deoff; v.excl_k = dek; v.excl_code = dec;
   ALLOC(s->iperm_dev, n);
   H2D(s->iperm_dev, s->iperm.data(), sizeof(int) * n);
-  ALLOC(v.nbcount, np); ALLOC(v.lcount, np); ALLOC(v.ucount, s->nblk); ALLOC(v.bbox, (size_t)s->nblk * 6);
+  ALLOC(v.nbcount, np); ALLOC(v.lcount, np); ALLOC(v.tcount, np); ALLOC(v.ucount, s->nblk); ALLOC(v.bbox, (size_t)s->nblk * 6);
   ALLOC(s->bcount, s->nblk);
   ALLOC(v.ef, (size_t)6 * np); ALLOC(v.mu, (size_t)6 * np); ALLOC(v.res, (size_t)6 * np);
   ALLOC(v.h, (size_t)6 * np); ALLOC(v.hreal, (size_t)6 * np); ALLOC(v.pp, (size_t)6 * np);
@@ -790,7 +887,16 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
   ALLOC(s->cphi, (size_t)10 * np);
   ALLOC(v.st, 1); ALLOC(v.err, 1);
   if (hipMemsetAsync(v.err, 0, sizeof(int), st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "memset"));
-  if (hipMemsetAsync(v.st, 0, sizeof(SolveState), st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "memset"));
+  {
+    // no solve in progress: every PCG kernel skips the system until
+    // k_init_state starts one
+    SolveState idle;
+    memset(&idle, 0, sizeof idle);
+    idle.conv = kConverged;
+    idle.finalized = 1;
+    if (hipMemcpy(v.st, &idle, sizeof idle, hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(fail(ctx, THP_ERR_HIP, "solve state"));
+  }
   for (double* vec : {v.ef, v.mu, v.res, v.h, v.hreal, v.pp, v.zr})
     if (hipMemsetAsync(vec, 0, sizeof(double) * 6 * np, st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "memset"));
 
@@ -834,12 +940,19 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
     }
   }
 
-  // ---- brick bins for owner-computes spreading
-  for (int k = 0; k < 3; ++k) {
-    v.brick[k] = pick_brick(s->nfft[k]);
-    v.nbrick[k] = (s->nfft[k] + v.brick[k] - 1) / v.brick[k];
+  // ---- brick bins and spreading tiles
+  if (own_grid) {
+    const size_t tw = (size_t)s->nbricks * kTile3 * 2;
+    if (s->prec == THP_PREC_FP64) {
+      double* t;
+      ALLOC(t, tw);
+      v.tiles = t;
+    } else {
+      float* t;
+      ALLOC(t, tw);
+      v.tiles = t;
+    }
   }
-  s->nbricks = v.nbrick[0] * v.nbrick[1] * v.nbrick[2];
   if (s->prec == THP_PREC_FP64) {
     double* th;
     ALLOC(th, (size_t)np * kTheta);
@@ -869,13 +982,7 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
   H2D(s->d_work, wk.data(), sizeof(Work) * s->nblk);
   ALLOC(s->d_redwork, v.red_blocks);
   H2D(s->d_redwork, wk.data(), sizeof(Work) * v.red_blocks);
-  {
-    std::vector<Work> gw(v.gat_blocks);
-    for (int b = 0; b < v.gat_blocks; ++b) gw[b] = {0, b};
-    ALLOC(s->d_gatwork, v.gat_blocks);
-    H2D(s->d_gatwork, gw.data(), sizeof(Work) * v.gat_blocks);
-    if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "sync"));
-  }
+  s->d_gatwork = s->d_brickwork;  // gather workgroups are bricks
   {
     const int nconv = (int)((npts + kConvChunk - 1) / kConvChunk);
     std::vector<Work> cw(nconv);
@@ -947,41 +1054,7 @@ thp_status thp_system_footprint(const thp_sys* s, int64_t* bytes) {
 
 thp_status thp_system_update_coords(thp_sys* s, const double* x, const double* y, const double* z) {
   if (!s || !x || !y || !z) return THP_ERR_ARG;
-  const int n = s->n, np = s->npad;
-  std::vector<double> hx(np, 0.0), hy(np, 0.0), hz(np, 0.0);
-  double maxd2 = 0.0, maxu2 = 0.0;
-  for (int q = 0; q < n; ++q) {
-    const int a = s->perm[q];
-    hx[q] = x[a];
-    hy[q] = y[a];
-    hz[q] = z[a];
-    if (s->list_valid) {
-      const double dx = hx[q] - s->bx0[q], dy = hy[q] - s->by0[q], dz = hz[q] - s->bz0[q];
-      maxd2 = std::max(maxd2, dx * dx + dy * dy + dz * dz);
-    }
-    if (s->tight_valid) {
-      const double dx = hx[q] - s->ux0[q], dy = hy[q] - s->uy0[q], dz = hz[q] - s->uz0[q];
-      maxu2 = std::max(maxu2, dx * dx + dy * dy + dz * dz);
-    }
-  }
-  hipStream_t st = s->ctx->stream;
-  HIPCHK(s->ctx, hipMemcpyAsync((void*)s->view.x, hx.data(), sizeof(double) * np, hipMemcpyHostToDevice, st));
-  HIPCHK(s->ctx, hipMemcpyAsync((void*)s->view.y, hy.data(), sizeof(double) * np, hipMemcpyHostToDevice, st));
-  HIPCHK(s->ctx, hipMemcpyAsync((void*)s->view.z, hz.data(), sizeof(double) * np, hipMemcpyHostToDevice, st));
-  RET(sync(s));
-  s->hx.assign(hx.begin(), hx.begin() + n);
-  s->hy.assign(hy.begin(), hy.begin() + n);
-  s->hz.assign(hz.begin(), hz.begin() + n);
-  // block candidates (cut + buffer) stay valid while no atom moved more than
-  // (buffer - skin)/2 since their build, the block-union lists (cut + skin)
-  // while no atom moved more than skin/2
-  const double lim_b = 0.5 * (s->buffer - s->skin), lim_u = 0.5 * s->skin;
-  if (s->list_valid && maxd2 > lim_b * lim_b) s->list_valid = false;
-  if (!s->list_valid || maxu2 > lim_u * lim_u) s->tight_valid = false;
-  s->verlet_valid = false;
-  s->pme_valid = false;
-  s->coef_valid = false;
-  return THP_OK;
+  return update_coords(s, x, y, z, nullptr);
 }
 
 thp_status thp_nblist(thp_sys* s, int force_rebuild) {

@@ -37,6 +37,7 @@ struct SolveState {
   int conv;
   int n_indef;
   int warm;     // 1: mu holds the starting dipoles (no direct guess)
+  int finalized;  // 1 once k_finish has written ep of a finished solve
 };
 
 // Everything a kernel needs about one system; lives in device memory so a
@@ -83,6 +84,11 @@ struct SysView {
   // the pair type, written by the permanent-field pass
   void* pcoef;
   int cmax;
+  // per-solve compacted lists: the in-cutoff entries of llist, in order,
+  // with their coefficients
+  unsigned short* tlist;   // [cmax][npad]
+  void* tcoef;             // [cmax][npad] coefficient pairs of tlist
+  int* tcount;             // [npad]
   // PME
   int brick[3];       // spreading brick edge per axis (4..8 points)
   int nbrick[3];
@@ -92,6 +98,7 @@ struct SysView {
   int* bin_start;     // [nbricks + 1]
   int* bin_list;      // [n] atoms grouped by home brick, sorted
   void* grid;         // complex interleaved, x fastest, real type per prec
+  void* tiles;        // [nbricks][12^3][2] spreading tiles (real type), scratch
   const void* qfac;   // K1*K2*K3 real
   // solver vectors [6][npad] FP64 (component-major)
   double* ef;
@@ -112,6 +119,8 @@ struct Work {  // one unit of a launch: a block (brick, chunk ...) of one system
 };
 
 constexpr int kConvChunk = 2048;
+constexpr int kTile = 12;                      // spreading tile edge: brick (<= 8) + order - 1
+constexpr int kTile3 = kTile * kTile * kTile;  // points per tile
 constexpr int kDefaultSegs = 8;   // pair-kernel wavefronts per block unless tuned  // grid points per block of the convolution
 
 // One rocFFT execution: forward and backward plans over `count` contiguous
@@ -212,6 +221,7 @@ namespace thp {
 thp_status fail(thp_ctx* ctx, thp_status code, const char* fmt, ...);
 thp_status create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** out, bool own_grid);
 thp_status ensure_lists(thp_sys* s);
+thp_status update_coords(thp_sys* s, const double* x, const double* y, const double* z, double* pinned);
 size_t grid_points(const thp_sys* s);
 // per-system work tables with sys = `index` (appended)
 void append_work(const thp_sys* s, int index, std::vector<Work>& blk, std::vector<Work>& red,

@@ -428,7 +428,7 @@ __device__ __forceinline__ void tmatxb_pair(const SysView& sv, const double* in,
 // distances.  Mirrors the reference's separate correction pass over
 // scaled pairs (GPU/source/tmatxb_pmecu.f:568-706, tmatxb_pmegpu.f:1435-1545).
 template <int KIND>
-__device__ __attribute__((noinline)) void scaled_pairs(const SysView& sv, const double* in, int i, double xi, double yi,
+__device__ __forceinline__ void scaled_pairs(const SysView& sv, const double* in, int i, double xi, double yi,
                                              double zi, double* acc) {
   const int e0 = sv.excl_off[i], e1 = sv.excl_off[i + 1];
   const double pdi = sv.pdamp[i], thi = sv.thole[i];
@@ -442,6 +442,28 @@ __device__ __attribute__((noinline)) void scaled_pairs(const SysView& sv, const 
     if (KIND == 0) efld0_pair<double>(sv, k, xr, yr, zr, pdi, thi, sc[0], sc[1], acc);
     else tmatxb_pair<double>(sv, in, k, xr, yr, zr, pdi, thi, sc[2], acc);
   }
+}
+
+// Scaled pairs of every atom of the launch, added to the output of the
+// unscaled pass that precedes it (KIND 0: ef, both rhs; KIND 1: the matvec
+// output vector vout of input vin).  A kernel of its own keeps the FP64
+// damping code out of the register budget of the pair kernels.
+template <int KIND>
+__global__ void __launch_bounds__(64) k_scaled(const SysView* views, const Work* work, int vin, int vout,
+                                               int check_conv) {
+  const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
+  const SysView& sv = views[w.sys];
+  if (check_conv && sv.st->conv) return;
+  const int i = w.blk * kBlock + threadIdx.x;
+  if (i >= sv.n) return;
+  if (sv.excl_off[i] == sv.excl_off[i + 1]) return;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  scaled_pairs<KIND>(sv, vsel(sv, vin), i, sv.x[i], sv.y[i], sv.z[i], acc);
+  double* out = KIND == 0 ? sv.ef : vsel(sv, vout);
+  const size_t np = sv.npad;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) out[(size_t)c * np + i] += acc[c];
 }
 
 // ===================================================== block-union lists
@@ -750,7 +772,6 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
     acc[4] = acc[1];
     acc[5] = acc[2];
   }
-  if (ok && wv == S - 1) scaled_pairs<KIND == 0 ? 0 : 1>(sv, in_g, i, xi, yi, zi, acc);
   // fixed-order sum over the S wavefronts
   if (S > 1) {
 #pragma unroll
@@ -771,27 +792,96 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
   }
 }
 
+// Per-solve list compaction after the permanent-field pass: each atom's
+// entries with non-zero coefficients (those within the cutoff) are copied,
+// in order, to the front of the compacted list and coefficient arrays.
+// Loads are issued 16 entries at a time ahead of the stores.
+template <typename real>
+__global__ void __launch_bounds__(64) k_compact_lists(const SysView* views, const Work* work) {
+  constexpr int NB = 16;
+  const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
+  const SysView& sv = views[w.sys];
+  const int i = w.blk * kBlock + threadIdx.x;
+  if (i >= sv.n) return;
+  const size_t np = sv.npad;
+  const GLOBAL unsigned short* LL = gp(sv.llist);
+  const GLOBAL Coef2<real>* PC = gp(static_cast<const Coef2<real>*>(sv.pcoef));
+  GLOBAL unsigned short* TL = gw(sv.tlist);
+  GLOBAL Coef2<real>* TC = gw(static_cast<Coef2<real>*>(sv.tcoef));
+  const int cnt = gp(sv.lcount)[i];
+  int t = 0;
+  for (int s0 = 0; s0 < cnt; s0 += NB) {
+    unsigned short l[NB];
+    Coef2<real> c[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      l[b] = 0;
+      c[b] = coef2<real>(real(0), real(0));
+      if (s0 + b < cnt) {
+        l[b] = LL[(size_t)(s0 + b) * np + i];
+        c[b] = PC[(size_t)(s0 + b) * np + i];
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      if (c[b].x != real(0) || c[b].y != real(0)) {
+        TL[(size_t)t * np + i] = l[b];
+        TC[(size_t)t * np + i] = c[b];
+        ++t;
+      }
+    }
+  }
+  gw(sv.tcount)[i] = t;
+}
+
 // Real-space dipole field over the block-union lists with the per-solve
 // coefficients of the permanent-field pass (tmatxb_pme,
 // newinduce_pme.f:1548-1608: efi -= dlocal . u_k with dlocal = -dmp3 I +
 // dmp5 r r^T).  Union atoms are staged in LDS as positions relative to the
-// block centre plus their dipoles; an entry costs one LDS gather, its
-// coefficient pair (coalesced across the wavefront) and ~25 flops.  Entries
-// beyond the cutoff carry zero coefficients.  Blocks whose union could
+// block centre plus their dipoles; an entry of the per-solve compacted list
+// costs one LDS gather, its coefficient pair (coalesced across the
+// wavefront) and ~25 flops.  Blocks whose union could
 // reach across half the box take the minimum image of each displacement.
 template <typename real>
-struct ClStage {
-  static constexpr int CHUNK = (40 * 1024) / (9 * (int)sizeof(real));  // atoms per LDS chunk
+struct __attribute__((aligned(4 * sizeof(real)))) Quad {
+  real a, b, c, d;
 };
+
+template <typename real>
+struct ClStage {
+  // per union atom: (x, y, z, ud_x), (ud_y, ud_z, up_x, up_y), up_z
+  static constexpr int BYTES = 9 * (int)sizeof(real);
+  static constexpr int CHUNK = (48 * 1024) / BYTES;  // atoms per LDS chunk
+};
+
+template <typename real, int NB>
+__device__ __forceinline__ void load_entries(const GLOBAL unsigned short* LL, const GLOBAL Coef2<real>* PC, size_t np,
+                                             int i, int s, int s_end, int* loc, Coef2<real>* cf) {
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    loc[b] = 0x7fffffff;
+    cf[b] = coef2<real>(real(0), real(0));
+    if (s + b < s_end) {
+      loc[b] = int(LL[(size_t)(s + b) * np + i]);
+      cf[b] = PC[(size_t)(s + b) * np + i];
+    }
+  }
+}
 
 template <typename real, int S>
 __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, const Work* work, int vin, int vout,
                                                        int dodiag, int check_conv) {
   constexpr int C = ClStage<real>::CHUNK;
-  constexpr int NB = 8;  // list entries fetched per batch
-  __shared__ real s_p[3][C];
-  __shared__ real s_m[6][C];
-  __shared__ double red[S > 1 ? S * 6 * 64 : 1];
+  constexpr int NB = 8;  // list entries per batch; the next batch is in flight meanwhile
+  // the staging arrays and the final cross-wavefront sum share one buffer
+  constexpr int STAGE_BYTES = C * (2 * (int)sizeof(Quad<real>) + (int)sizeof(real));
+  constexpr int RED_BYTES = S * 6 * 64 * (int)sizeof(double);
+  __shared__ __attribute__((aligned(16))) unsigned char smem[STAGE_BYTES > RED_BYTES ? STAGE_BYTES : RED_BYTES];
+  Quad<real>* s_a = reinterpret_cast<Quad<real>*>(smem);
+  Quad<real>* s_b = s_a + C;
+  real* s_c = reinterpret_cast<real*>(s_b + C);
+  double* red = reinterpret_cast<double*>(smem);
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
@@ -804,8 +894,8 @@ __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, cons
   const GLOBAL double* Z = gp(sv.z);
   const double* in_g = vsel(sv, vin);
   const GLOBAL double* IN = gp(in_g);
-  const GLOBAL unsigned short* LL = gp(sv.llist);
-  const GLOBAL Coef2<real>* PC = gp(static_cast<const Coef2<real>*>(sv.pcoef));
+  const GLOBAL unsigned short* LL = gp(sv.tlist);
+  const GLOBAL Coef2<real>* PC = gp(static_cast<const Coef2<real>*>(sv.tcoef));
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int i = w.blk * kBlock + lane;
   const bool ok = i < n;
@@ -830,7 +920,7 @@ __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, cons
     piy = real(e);
     piz = real(f);
   }
-  const int cnt = ok ? gp(sv.lcount)[i] : 0;
+  const int cnt = ok ? gp(sv.tcount)[i] : 0;
   const int s_end = (cnt * (wv + 1)) / S;
   int s = (cnt * wv) / S;
   double acc[6] = {0, 0, 0, 0, 0, 0};
@@ -843,26 +933,17 @@ __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, cons
       d -= bx * rint(d / bx);
       e -= by * rint(e / by);
       f -= bz * rint(f / bz);
-      s_p[0][q] = real(d);
-      s_p[1][q] = real(e);
-      s_p[2][q] = real(f);
-#pragma unroll
-      for (int m = 0; m < 6; ++m) s_m[m][q] = real(IN[(size_t)m * np + kk]);
+      s_a[q] = Quad<real>{real(d), real(e), real(f), real(IN[0 * np + kk])};
+      s_b[q] = Quad<real>{real(IN[1 * np + kk]), real(IN[2 * np + kk]), real(IN[3 * np + kk]), real(IN[4 * np + kk])};
+      s_c[q] = real(IN[5 * np + kk]);
     }
     __syncthreads();
     const int c1 = c0 + cn;
-    while (s < s_end) {
-      int loc[NB];
-      Coef2<real> cf[NB];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        loc[b] = 0x7fffffff;
-        cf[b] = coef2<real>(real(0), real(0));
-        if (s + b < s_end) {
-          loc[b] = int(LL[(size_t)(s + b) * np + i]);
-          cf[b] = PC[(size_t)(s + b) * np + i];
-        }
-      }
+    int loc[NB], nloc[NB];
+    Coef2<real> cf[NB], ncf[NB];
+    load_entries<real, NB>(LL, PC, np, i, s, s_end, loc, cf);
+    while (loc[0] < c1) {
+      load_entries<real, NB>(LL, PC, np, i, s + NB, s_end, nloc, ncf);
       bool done = false;
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
@@ -872,28 +953,37 @@ __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, cons
         }
         ++s;
         const int q = loc[b] - c0;
-        real dx = s_p[0][q] - pix, dy = s_p[1][q] - piy, dz = s_p[2][q] - piz;
+        const Quad<real> A = s_a[q];
+        const Quad<real> B = s_b[q];
+        const real upz = s_c[q];
+        real dx = A.a - pix, dy = A.b - piy, dz = A.c - piz;
         if (wrap) {
           dx -= lx * rint(dx * ilx);
           dy -= ly * rint(dy * ily);
           dz -= lz * rint(dz * ilz);
         }
         const real d3 = cf[b].x, d5 = cf[b].y;
-#pragma unroll
-        for (int rhs = 0; rhs < 2; ++rhs) {
-          const real ux = s_m[rhs * 3 + 0][q], uy = s_m[rhs * 3 + 1][q], uz = s_m[rhs * 3 + 2][q];
-          const real ur = d5 * (dx * ux + dy * uy + dz * uz);
-          acc[rhs * 3 + 0] += double(d3 * ux - ur * dx);
-          acc[rhs * 3 + 1] += double(d3 * uy - ur * dy);
-          acc[rhs * 3 + 2] += double(d3 * uz - ur * dz);
-        }
+        const real urd = d5 * (dx * A.d + dy * B.a + dz * B.b);
+        const real urp = d5 * (dx * B.c + dy * B.d + dz * upz);
+        acc[0] += double(d3 * A.d - urd * dx);
+        acc[1] += double(d3 * B.a - urd * dy);
+        acc[2] += double(d3 * B.b - urd * dz);
+        acc[3] += double(d3 * B.c - urp * dx);
+        acc[4] += double(d3 * B.d - urp * dy);
+        acc[5] += double(d3 * upz - urp * dz);
       }
       if (done) break;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        loc[b] = nloc[b];
+        cf[b] = ncf[b];
+      }
     }
   }
-  if (ok && wv == S - 1) scaled_pairs<1>(sv, in_g, i, xi, yi, zi, acc);
-  // fixed-order sum over the S wavefronts
+  // fixed-order sum over the S wavefronts (the staging buffer is free once
+  // every wavefront has passed the barrier)
   if (S > 1) {
+    __syncthreads();
 #pragma unroll
     for (int c = 0; c < 6; ++c) red[(wv * 6 + c) * 64 + lane] = acc[c];
     __syncthreads();
@@ -1305,6 +1395,207 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_brick(const SysVie
   }
 }
 
+// ---------------------------------------------------------------------
+// Tile spreading.  One workgroup per brick takes the atoms whose first
+// spline node lies in the brick (its "home" bin, sorted by atom index); the
+// 5x5x5 support of such an atom lies inside the brick's tile of
+// (B+4)^3 points.  Each wavefront owns a private LDS copy of the tile and
+// adds whole stencils into it, atom after atom, one stencil point per lane
+// (two passes of 64 lanes for 125 points): program order within a
+// wavefront makes the read-add-write sequences of consecutive atoms exact,
+// with no atomics.  The wavefront tiles are summed in a fixed order and
+// written to the brick's slot of the tile buffer; k_spread_merge then adds,
+// for every grid point, its own brick's tile and the overhang of the
+// previous brick along each axis (at most 8 tiles, fixed order).  Bricks
+// are at least 4 points wide, so no tile reaches two bricks ahead.
+
+template <typename real>
+struct SpreadTileCfg {
+  static constexpr int WAVES = sizeof(real) == 4 ? 4 : 2;  // private tiles per workgroup
+};
+
+template <typename real, int KIND>
+__global__ void __launch_bounds__(64 * SpreadTileCfg<real>::WAVES) k_spread_tile(const SysView* views, const Work* bwork,
+                                                                                  int vin, int frac_in, int check_conv) {
+  constexpr int W = SpreadTileCfg<real>::WAVES;
+  constexpr int NL = KIND == 0 ? 2 : 3;   // spline levels used
+  constexpr int NF = KIND == 0 ? 6 : 10;  // moments per atom
+  constexpr int NT = 3 * NL * kOrder;
+  __shared__ real s_tile[W][kTile3 * 2];
+  __shared__ real s_th[64][NT];
+  __shared__ real s_f[64][NF];
+  __shared__ int s_off[64];
+  const Work w = bwork[blockIdx.x];
+  if (w.sys < 0) return;
+  const SysView& sv = views[w.sys];
+  if (check_conv && sv.st->conv) return;
+  const int nb0 = sv.nbrick[0], nb1 = sv.nbrick[1];
+  const int tb = w.blk;
+  const int t0 = tb % nb0, t1 = (tb / nb0) % nb1, t2 = tb / (nb0 * nb1);
+  const int sx = t0 * sv.brick[0], sy = t1 * sv.brick[1], sz = t2 * sv.brick[2];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const size_t np = sv.npad;
+  const GLOBAL double* IN = gp(vsel(sv, vin));
+  const GLOBAL double* RP = gp(sv.rpole);
+  const GLOBAL real* THETA = gp(static_cast<const real*>(sv.theta));
+  const GLOBAL int* G0 = gp(sv.g0);
+  const int a0 = gp(sv.bin_start)[tb], a1 = gp(sv.bin_start)[tb + 1];
+  const GLOBAL int* BLIST = gp(sv.bin_list);
+  const double af0 = frac_in ? 1.0 : sv.afrac[0], af1 = frac_in ? 1.0 : sv.afrac[1], af2 = frac_in ? 1.0 : sv.afrac[2];
+  real* my = s_tile[wv];
+  for (int p = lane; p < kTile3 * 2; p += 64) my[p] = real(0);
+  // stencil point of this lane in each pass
+  int pmx[2], pmy[2], pmz[2], poff[2];
+#pragma unroll
+  for (int ps = 0; ps < 2; ++ps) {
+    const int l = lane + 64 * ps;
+    pmx[ps] = l % kOrder;
+    pmy[ps] = (l / kOrder) % kOrder;
+    pmz[ps] = l / (kOrder * kOrder);
+    poff[ps] = (pmz[ps] * kTile + pmy[ps]) * kTile + pmx[ps];
+  }
+  for (int c0 = a0; c0 < a1; c0 += 64) {
+    const int cnt = min(64, a1 - c0);
+    __syncthreads();
+    for (int q = threadIdx.x; q < cnt; q += 64 * W) {
+      const int i = BLIST[c0 + q];
+      const int lx = G0[4 * i] - sx, ly = G0[4 * i + 1] - sy, lz = G0[4 * i + 2] - sz;
+      s_off[q] = (lz * kTile + ly) * kTile + lx;
+      const GLOBAL real* th = THETA + (size_t)i * kTheta;
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int l = 0; l < NL; ++l)
+#pragma unroll
+          for (int m = 0; m < kOrder; ++m) s_th[q][(d * NL + l) * kOrder + m] = th[th_idx(d, l, m)];
+      if (KIND == 0) {
+        s_f[q][0] = real(af0 * IN[0 * np + i]);
+        s_f[q][1] = real(af1 * IN[1 * np + i]);
+        s_f[q][2] = real(af2 * IN[2 * np + i]);
+        s_f[q][3] = real(af0 * IN[3 * np + i]);
+        s_f[q][4] = real(af1 * IN[4 * np + i]);
+        s_f[q][5] = real(af2 * IN[5 * np + i]);
+      } else {
+        double cmp[10];
+        cmp[0] = RP[0 * np + i];
+        cmp[1] = RP[1 * np + i];
+        cmp[2] = RP[2 * np + i];
+        cmp[3] = RP[3 * np + i];
+        cmp[4] = RP[4 * np + i];
+        cmp[5] = RP[7 * np + i];
+        cmp[6] = RP[9 * np + i];
+        cmp[7] = 2.0 * RP[5 * np + i];
+        cmp[8] = 2.0 * RP[6 * np + i];
+        cmp[9] = 2.0 * RP[8 * np + i];
+        s_f[q][0] = real(sv.ctf[0][0] * cmp[0]);
+#pragma unroll
+        for (int j = 1; j < 4; ++j) {
+          double s = 0.0;
+#pragma unroll
+          for (int kk = 1; kk < 4; ++kk) s += sv.ctf[j][kk] * cmp[kk];
+          s_f[q][j] = real(s);
+        }
+#pragma unroll
+        for (int j = 4; j < 10; ++j) {
+          double s = 0.0;
+#pragma unroll
+          for (int kk = 4; kk < 10; ++kk) s += sv.ctf[j][kk] * cmp[kk];
+          s_f[q][j] = real(s);
+        }
+      }
+    }
+    __syncthreads();
+    for (int q = wv; q < cnt; q += W) {
+      const real* th = s_th[q];
+      const real* f = s_f[q];
+      const int base = s_off[q];
+#pragma unroll
+      for (int ps = 0; ps < 2; ++ps) {
+        if (ps == 1 && lane >= kOrder * kOrder * kOrder - 64) break;
+        const int mx = pmx[ps], my_ = pmy[ps], mz = pmz[ps];
+        const int t = 2 * (base + poff[ps]);
+        if (KIND == 0) {
+          const real ta = th[(0 * NL + 0) * kOrder + mx], tb1 = th[(0 * NL + 1) * kOrder + mx];
+          const real u0 = th[(1 * NL + 0) * kOrder + my_], u1 = th[(1 * NL + 1) * kOrder + my_];
+          const real v0 = th[(2 * NL + 0) * kOrder + mz], v1 = th[(2 * NL + 1) * kOrder + mz];
+          const real g1 = ta * (f[1] * u1 * v0 + f[2] * u0 * v1) + tb1 * (f[0] * u0 * v0);
+          const real g2 = ta * (f[4] * u1 * v0 + f[5] * u0 * v1) + tb1 * (f[3] * u0 * v0);
+          my[t] += g1;
+          my[t + 1] += g2;
+        } else {
+          const real t0v = th[(0 * NL + 0) * kOrder + mx], t1v = th[(0 * NL + 1) * kOrder + mx],
+                     t2v = th[(0 * NL + 2) * kOrder + mx];
+          const real u0 = th[(1 * NL + 0) * kOrder + my_], u1 = th[(1 * NL + 1) * kOrder + my_],
+                     u2 = th[(1 * NL + 2) * kOrder + my_];
+          const real v0 = th[(2 * NL + 0) * kOrder + mz], v1 = th[(2 * NL + 1) * kOrder + mz],
+                     v2 = th[(2 * NL + 2) * kOrder + mz];
+          const real term0 =
+              f[0] * u0 * v0 + f[2] * u1 * v0 + f[3] * u0 * v1 + f[5] * u2 * v0 + f[6] * u0 * v2 + f[9] * u1 * v1;
+          const real term1 = f[1] * u0 * v0 + f[7] * u1 * v0 + f[8] * u0 * v1;
+          const real term2 = f[4] * u0 * v0;
+          my[t] += term0 * t0v + term1 * t1v + term2 * t2v;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // fixed-order sum of the wavefront tiles into the brick's tile slot
+  GLOBAL real* OUT = gw(static_cast<real*>(sv.tiles)) + (size_t)tb * kTile3 * 2;
+  for (int p = threadIdx.x; p < kTile3 * 2; p += 64 * W) {
+    real v = s_tile[0][p];
+#pragma unroll
+    for (int q = 1; q < W; ++q) v += s_tile[q][p];
+    OUT[p] = v;
+  }
+}
+
+// grid point = own brick's tile + overhang of the previous brick per axis
+// (fixed order); KIND 1 clears the imaginary part
+template <typename real, int KIND>
+__global__ void __launch_bounds__(256) k_spread_merge(const SysView* views, const Work* work, int check_conv) {
+  const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
+  const SysView& sv = views[w.sys];
+  if (check_conv && sv.st->conv) return;
+  const int K0 = sv.nfft[0], K1 = sv.nfft[1], K2 = sv.nfft[2];
+  const int B0 = sv.brick[0], B1 = sv.brick[1], B2 = sv.brick[2];
+  const int nb0 = sv.nbrick[0], nb1 = sv.nbrick[1], nb2 = sv.nbrick[2];
+  const size_t npts = (size_t)K0 * K1 * K2;
+  const GLOBAL real* T = gp(static_cast<const real*>(sv.tiles));
+  GLOBAL real* G = gw(static_cast<real*>(sv.grid));
+  const size_t p0 = (size_t)w.blk * kConvChunk, p1 = min(npts, p0 + kConvChunk);
+  for (size_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    const int x = int(p % K0), y = int((p / K0) % K1), z = int(p / ((size_t)K0 * K1));
+    const int bx = min(x / B0, nb0 - 1), by = min(y / B1, nb1 - 1), bz = min(z / B2, nb2 - 1);
+    const int ox = x - bx * B0, oy = y - by * B1, oz = z - bz * B2;
+    real re = real(0), im = real(0);
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz) {
+      if (dz && oz >= kOrder - 1) break;
+      const int cz = dz ? (bz + nb2 - 1) % nb2 : bz;
+      const int tz = dz ? oz + (cz == nb2 - 1 ? K2 - (nb2 - 1) * B2 : B2) : oz;
+#pragma unroll
+      for (int dy = 0; dy < 2; ++dy) {
+        if (dy && oy >= kOrder - 1) break;
+        const int cy = dy ? (by + nb1 - 1) % nb1 : by;
+        const int ty = dy ? oy + (cy == nb1 - 1 ? K1 - (nb1 - 1) * B1 : B1) : oy;
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx) {
+          if (dx && ox >= kOrder - 1) break;
+          const int cx = dx ? (bx + nb0 - 1) % nb0 : bx;
+          const int tx = dx ? ox + (cx == nb0 - 1 ? K0 - (nb0 - 1) * B0 : B0) : ox;
+          const size_t tile = ((size_t)cz * nb1 + cy) * nb0 + cx;
+          const size_t t = (tile * kTile3 + ((size_t)(tz * kTile + ty) * kTile + tx)) * 2;
+          re += T[t];
+          if (KIND == 0) im += T[t + 1];
+        }
+      }
+    }
+    G[2 * p] = re;
+    G[2 * p + 1] = im;
+  }
+}
+
 // scalar-sum convolution between the forward and backward transforms
 // (qfac, newinduce_pme.f:1790-1861)
 template <typename real>
@@ -1442,6 +1733,130 @@ __global__ void __launch_bounds__(256) k_gather_uind(const SysView* views, const
   }
   if (MODE == 0) return;
   block_partial2(q[0], q[1], MODE == 1 ? part_init(sv) : part_gg(sv), w.blk);
+}
+
+// Tile gather: fphi_uind_site first derivatives (pmestuff.f:746-1037,
+// fdip_phi(2:4) of both grids) and the fractional->Cartesian field
+// (newinduce_pme.f:2103-2110).  One wavefront per brick loads the brick's
+// (B+4)^3 tile of the grid into LDS once; each lane then gathers one home
+// atom's 5x5x5 stencil from LDS (rows summed in the grid type, rows
+// accumulated in FP64).  MODE as in k_gather_uind; the partial sums are
+// one pair per brick.
+template <typename real, int MODE>
+__global__ void __launch_bounds__(64) k_gather_tile(const SysView* views, const Work* bwork, int vin, int vout,
+                                                     int frac_out) {
+  __shared__ real s_g[kTile3 * 2];
+  const Work w = bwork[blockIdx.x];
+  if (w.sys < 0) return;
+  const SysView& sv = views[w.sys];
+  if (MODE == 2 && sv.st->conv) return;
+  const int K0 = sv.nfft[0], K1 = sv.nfft[1], K2 = sv.nfft[2];
+  const int B0 = sv.brick[0], B1 = sv.brick[1], B2 = sv.brick[2];
+  const int nb0 = sv.nbrick[0], nb1 = sv.nbrick[1], nb2 = sv.nbrick[2];
+  const int tb = w.blk;
+  const int t0 = tb % nb0, t1 = (tb / nb0) % nb1, t2 = tb / (nb0 * nb1);
+  const int sx = t0 * B0, sy = t1 * B1, sz = t2 * B2;
+  const int e0 = (t0 == nb0 - 1 ? K0 - sx : B0) + kOrder - 1;
+  const int e1 = (t1 == nb1 - 1 ? K1 - sy : B1) + kOrder - 1;
+  const int e2 = (t2 == nb2 - 1 ? K2 - sz : B2) + kOrder - 1;
+  const int lane = threadIdx.x;
+  const GLOBAL real* GRID = gp(static_cast<const real*>(sv.grid));
+  for (int p = lane; p < kTile3; p += 64) {
+    const int ox = p % kTile, oy = (p / kTile) % kTile, oz = p / (kTile * kTile);
+    if (ox < e0 && oy < e1 && oz < e2) {
+      int gx = sx + ox, gy = sy + oy, gz = sz + oz;
+      if (gx >= K0) gx -= K0;
+      if (gy >= K1) gy -= K1;
+      if (gz >= K2) gz -= K2;
+      const size_t g = ((size_t)gz * K1 + gy) * K0 + gx;
+      s_g[2 * p] = GRID[2 * g];
+      s_g[2 * p + 1] = GRID[2 * g + 1];
+    }
+  }
+  __syncthreads();
+  const size_t np = sv.npad;
+  const GLOBAL real* THETA = gp(static_cast<const real*>(sv.theta));
+  const GLOBAL int* G0 = gp(sv.g0);
+  const int a0 = gp(sv.bin_start)[tb], a1 = gp(sv.bin_start)[tb + 1];
+  const GLOBAL int* BLIST = gp(sv.bin_list);
+  double qs[2] = {0.0, 0.0};
+  for (int c0 = a0; c0 < a1; c0 += 64) {
+    if (c0 + lane >= a1) break;
+    const int i = BLIST[c0 + lane];
+    const int lx = G0[4 * i] - sx, ly = G0[4 * i + 1] - sy, lz = G0[4 * i + 2] - sz;
+    const GLOBAL real* th = THETA + (size_t)i * kTheta;
+    real tx0[kOrder], tx1[kOrder], ty0[kOrder], ty1[kOrder];
+#pragma unroll
+    for (int m = 0; m < kOrder; ++m) {
+      tx0[m] = th[th_idx(0, 0, m)];
+      tx1[m] = th[th_idx(0, 1, m)];
+      ty0[m] = th[th_idx(1, 0, m)];
+      ty1[m] = th[th_idx(1, 1, m)];
+    }
+    double a[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll 1
+    for (int mz = 0; mz < kOrder; ++mz) {
+      const real v0 = th[th_idx(2, 0, mz)], v1 = th[th_idx(2, 1, mz)];
+#pragma unroll
+      for (int my = 0; my < kOrder; ++my) {
+        const real u0 = ty0[my], u1 = ty1[my];
+        const real* row = s_g + 2 * (((lz + mz) * kTile + (ly + my)) * kTile + lx);
+        real t0_1 = 0, t1_1 = 0, t0_2 = 0, t1_2 = 0;
+#pragma unroll
+        for (int mx = 0; mx < kOrder; ++mx) {
+          const real q1 = row[2 * mx], q2 = row[2 * mx + 1];
+          t0_1 += q1 * tx0[mx];
+          t1_1 += q1 * tx1[mx];
+          t0_2 += q2 * tx0[mx];
+          t1_2 += q2 * tx1[mx];
+        }
+        a[0] += double(t1_1 * u0 * v0);
+        a[1] += double(t0_1 * u1 * v0);
+        a[2] += double(t0_1 * u0 * v1);
+        a[3] += double(t1_2 * u0 * v0);
+        a[4] += double(t0_2 * u1 * v0);
+        a[5] += double(t0_2 * u0 * v1);
+      }
+    }
+    double dip[6];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double sc = (MODE == 0 && frac_out) ? 1.0 : sv.afrac[c];
+      dip[c] = sc * a[c];
+      dip[3 + c] = sc * a[3 + c];
+    }
+    if (MODE == 0) {
+      double* out = vsel(sv, vout);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) out[(size_t)c * np + i] = dip[c];
+    } else {
+      const GLOBAL double* IN = gp(vsel(sv, vin));
+      const double term = sv.term_self;
+      const double pl = gp(sv.pol)[i];
+      const double diag = pl == 0.0 ? kTinyPol : pl;
+      const GLOBAL double* HR = gp(sv.hreal);
+      GLOBAL double* H = gw(sv.h);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const size_t o = (size_t)c * np + i;
+        const double u = IN[o];
+        const double h = HR[o] + dip[c] - term * u;
+        H[o] = h;
+        if (MODE == 1) {
+          const double rr = gp(sv.ef)[o] - h;
+          const double z = diag * rr;
+          gw(sv.res)[o] = rr;
+          gw(sv.zr)[o] = z;
+          gw(sv.pp)[o] = z;
+          qs[c / 3] += rr * z;
+        } else {
+          qs[c / 3] += u * h;
+        }
+      }
+    }
+  }
+  if (MODE == 0) return;
+  block_partial2(qs[0], qs[1], MODE == 1 ? part_init(sv) : part_gg(sv), w.blk);
 }
 
 // fphi_mpole_site (pmestuff.f:453-631, potential and derivatives through
@@ -1697,7 +2112,7 @@ __global__ void __launch_bounds__(256) k_energy(const SysView* views, const Work
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
-  if (only_done && sv.st->conv == kRunning) return;
+  if (only_done && (sv.st->conv == kRunning || sv.st->finalized)) return;
   const size_t np = sv.npad;
   double e = 0.0;
   for (int i = w.blk * blockDim.x + threadIdx.x; i < sv.n; i += sv.red_blocks * blockDim.x) {
@@ -1710,13 +2125,14 @@ __global__ void __launch_bounds__(256) k_energy(const SysView* views, const Work
 // one wave per system: final scalars into the SolveState
 __global__ void __launch_bounds__(64) k_finish(const SysView* views, const int* sys_ids, int only_done) {
   const SysView& sv = views[sys_ids[blockIdx.x]];
-  if (only_done && sv.st->conv == kRunning) return;
+  if (only_done && (sv.st->conv == kRunning || sv.st->finalized)) return;
   const double e = sum_partials(part_ep(sv), sv.red_blocks, 0);
   const double e0 = sum_partials(part_ene(sv), sv.red_blocks, 0), e1 = sum_partials(part_ene(sv), sv.red_blocks, 1);
   if (threadIdx.x == 0) {
     sv.st->ep = -0.5 * kCoulomb * e;
     sv.st->ene[0] = e0;
     sv.st->ene[1] = e1;
+    sv.st->finalized = 1;
   }
 }
 
@@ -1740,6 +2156,7 @@ __global__ void k_init_state(const SysView* views, const int* sys_ids, int nsys,
   z.conv = kRunning;
   z.n_indef = 0;
   z.warm = warm ? warm[q] : 0;
+  z.finalized = 0;
   *st = z;
 }
 

@@ -41,10 +41,13 @@ struct ServeSystem {
   std::vector<double> x, y, z, rpole, pol, thole, dpu;
   std::vector<int> ik;
   thp_sys* s = nullptr;      // device data when resident
+  double* staging = nullptr; // pinned host copy of sorted coordinates (3*npad), when resident
   int64_t bytes = 0;         // device footprint once created (estimate before)
   bool busy = false;         // a solve of it is in flight
   bool has_mu = false;       // device mu holds its last solution
   uint64_t last_use = 0;
+  int member = -1;           // position in the iteration batch, -1 if not a member
+  int idle_passes = 0;       // passes without a solve in flight while a member
 };
 
 struct Request {
@@ -60,6 +63,7 @@ struct InFlight {
   ServeSystem* sys = nullptr;
   Clock::time_point t_start;
   int warm = 0;
+  int bidx = -1;  // index in the iteration batch of the last chunk that carried it
 };
 
 struct DipoleCopy {  // thp_serve_get_dipoles handed to the worker
@@ -74,7 +78,9 @@ struct GridPool {
   int K[3] = {0, 0, 0};
   int cap = 0;
   void* base = nullptr;
-  size_t bytes_per_grid = 0;
+  size_t bytes_per_grid = 0;   // stride of the grids (FFT batch distance)
+  void* tiles = nullptr;       // spreading tiles of each slot, after the grids
+  size_t bytes_per_tiles = 0;
   std::map<int, FftExec> plans;  // by number of grids
   std::vector<void*> work;       // rocFFT work buffers of the plans
 };
@@ -123,6 +129,11 @@ struct Serve {
   std::map<std::tuple<int, int, int>, GridPool> pools;
   int pool_generation = 0;
   BatchBuf setup, iter;
+  // The iteration batch: systems with a solve in flight plus recently idle
+  // ones (they leave after kIdlePasses passes without work), so that the
+  // layout, its tables and its captured graph survive the usual
+  // finish-and-resubmit turnover.  Idle members cost only early exits.
+  std::vector<ServeSystem*> members;
   bool iter_dirty = true;
   int iter_generation = -1;
   std::vector<int> iter_versions;  // view versions the iteration batch was built from
@@ -264,7 +275,11 @@ thp_status build_batch(Serve* sv, BatchBuf& bb, const std::vector<thp_sys*>& mem
       gp.cap = std::max(need, 2);
       while (gp.cap < need) gp.cap *= 2;
       gp.bytes_per_grid = 2 * real * (size_t)gp.K[0] * gp.K[1] * gp.K[2];
-      SHIP(ctx, hipMalloc(&gp.base, gp.bytes_per_grid * gp.cap));
+      const thp_sys* s0 = members[g.second[0]];
+      gp.bytes_per_tiles = ((2 * real * (size_t)s0->nbricks * kTile3 + 255) / 256) * 256;
+      const size_t grids = ((gp.bytes_per_grid * gp.cap + 255) / 256) * 256;
+      SHIP(ctx, hipMalloc(&gp.base, grids + gp.bytes_per_tiles * gp.cap));
+      gp.tiles = static_cast<char*>(gp.base) + grids;
       ++sv->pool_generation;
       drop_graph(sv->setup);
       drop_graph(sv->iter);
@@ -299,6 +314,7 @@ thp_status build_batch(Serve* sv, BatchBuf& bb, const std::vector<thp_sys*>& mem
       const int q = g.second[slot];
       bb.h_views[q] = members[q]->view;
       bb.h_views[q].grid = static_cast<char*>(gp.base) + slot * gp.bytes_per_grid;
+      bb.h_views[q].tiles = static_cast<char*>(gp.tiles) + slot * gp.bytes_per_tiles;
     }
     FftExec f;
     SRET(pool_plan(sv, gp, (int)g.second.size(), prec, &f));
@@ -331,11 +347,22 @@ thp_status build_batch(Serve* sv, BatchBuf& bb, const std::vector<thp_sys*>& mem
   return THP_OK;
 }
 
+void drop_member(Serve* sv, ServeSystem* ss) {
+  if (ss->member < 0) return;
+  sv->members.erase(sv->members.begin() + ss->member);
+  ss->member = -1;
+  for (size_t q = 0; q < sv->members.size(); ++q) sv->members[q]->member = (int)q;
+  sv->iter_dirty = true;
+}
+
 thp_status release_system(Serve* sv, ServeSystem* ss) {
   if (!ss->s) return THP_OK;
+  drop_member(sv, ss);
   sv->resident -= ss->bytes;
   thp_system_destroy(ss->s);
   ss->s = nullptr;
+  if (ss->staging) (void)hipHostFree(ss->staging);
+  ss->staging = nullptr;
   ss->has_mu = false;
   sv->stats.evictions += 1;
   return THP_OK;
@@ -365,6 +392,10 @@ thp_status admit(Serve* sv, ServeSystem* ss, const std::vector<ServeSystem*>& pi
   }
   thp_sys* s = nullptr;
   SRET(create_system(sv->ctx, &ss->desc, &s, false));
+  if (hipHostMalloc((void**)&ss->staging, sizeof(double) * 3 * s->npad, hipHostMallocDefault) != hipSuccess) {
+    thp_system_destroy(s);
+    return fail(sv->ctx, THP_ERR_NOMEM, "pinned staging for system %d", ss->id);
+  }
   ss->s = s;
   ss->bytes = s->bytes;
   sv->resident += ss->bytes;
@@ -518,14 +549,113 @@ void publish_error(Serve* sv, const Request& q, thp_status code, Clock::time_poi
   sv->cv_done.notify_all();
 }
 
-// one pass of the loop; returns a fatal status only for device failures
+constexpr int kIdlePasses = 4;
+
+// Retire the solves of the last chunk from the states read back into
+// h_out (indexed by each solve's batch index at launch).
+void retire(Serve* sv) {
+  const auto now = Clock::now();
+  std::vector<InFlight> still;
+  for (auto& f : sv->active) {
+    if (f.bidx < 0) {
+      still.push_back(std::move(f));
+      continue;
+    }
+    const StateOut& so = sv->h_out[f.bidx];
+    if (so.err == 0 && so.st.conv == kRunning) {
+      still.push_back(std::move(f));
+      continue;
+    }
+    f.sys->busy = false;
+    f.sys->has_mu = so.err == 0 && (so.st.conv == kConverged || so.st.conv == kZeroCurv);
+    f.sys->last_use = ++sv->use_clock;
+    publish(sv, f, so, now);
+  }
+  if (still.size() != sv->active.size()) {
+    sv->active.swap(still);
+    sv->cv_done.notify_all();
+  }
+}
+
+// iteration-batch membership: add systems with a solve in flight, drop
+// members idle for kIdlePasses passes
+void update_members(Serve* sv) {
+  for (auto& f : sv->active)
+    if (f.sys->member < 0) {
+      f.sys->member = (int)sv->members.size();
+      sv->members.push_back(f.sys);
+      sv->iter_dirty = true;
+    }
+  std::vector<ServeSystem*> keep;
+  for (ServeSystem* m : sv->members) {
+    m->idle_passes = m->busy ? 0 : m->idle_passes + 1;
+    if (m->idle_passes > kIdlePasses) {
+      m->member = -1;
+      sv->iter_dirty = true;
+    } else {
+      keep.push_back(m);
+    }
+  }
+  if (keep.size() != sv->members.size()) {
+    sv->members.swap(keep);
+    for (size_t q = 0; q < sv->members.size(); ++q) sv->members[q]->member = (int)q;
+  }
+}
+
+// One pass of the loop: the iteration chunk of the batch is enqueued first;
+// admission (host work, asynchronous coordinate uploads) and the set-up
+// batch of the admitted requests follow while the device runs it; then one
+// wait, and the finished solves retire.  Returns a fatal status only for
+// device failures.
 thp_status pass(Serve* sv, std::vector<Request>& starting) {
   thp_ctx* ctx = sv->ctx;
   const auto t_pass = Clock::now();
-  // ---- admission: residency, coordinates, lists
+  // ---- PCG iterations of everything in flight
+  bool chunk = false;
+  if (!sv->active.empty()) {
+    update_members(sv);
+    std::vector<thp_sys*> mem;
+    for (ServeSystem* m : sv->members) mem.push_back(m->s);
+    if (!sv->iter_dirty) {
+      for (size_t q = 0; q < mem.size(); ++q)
+        if (q >= sv->iter_versions.size() || sv->iter_versions[q] != mem[q]->view_version) sv->iter_dirty = true;
+    }
+    if (sv->iter_dirty || sv->iter_generation != sv->pool_generation) {
+      SRET(build_batch(sv, sv->iter, mem, {}));
+      SRET(tune_pair(sv, sv->iter.b, mem));
+      sv->iter_dirty = false;
+      sv->iter_generation = sv->pool_generation;
+      sv->iter_versions.clear();
+      for (thp_sys* s : mem) sv->iter_versions.push_back(s->view_version);
+      sv->stats.batch_builds += 1;
+    }
+    const int iters = std::max(1, sv->o.iters_per_graph);
+    SRET(run_chunk(sv, iters));
+    sv->stats.chunks += 1;
+    sv->active_sum += (double)sv->active.size();
+    sv->stats.pair_segs = sv->iter.b.segs;
+    // energies of the solves that finished, states back in one copy
+    Batch& b = sv->iter.b;
+    SRET(b_finish(ctx, b, 1));
+    if (sv->out_cap < b.nsys) {
+      SHIP(ctx, hipStreamSynchronize(ctx->stream));
+      if (sv->d_out) (void)hipFree(sv->d_out);
+      if (sv->h_out) (void)hipHostFree(sv->h_out);
+      sv->out_cap = std::max(b.nsys, 2 * sv->out_cap);
+      SHIP(ctx, hipMalloc((void**)&sv->d_out, sizeof(StateOut) * sv->out_cap));
+      SHIP(ctx, hipHostMalloc((void**)&sv->h_out, sizeof(StateOut) * sv->out_cap, hipHostMallocDefault));
+    }
+    SRET(b_collect(ctx, b, sv->d_out));
+    SHIP(ctx, hipMemcpyAsync(sv->h_out, sv->d_out, sizeof(StateOut) * b.nsys, hipMemcpyDeviceToHost, ctx->stream));
+    for (auto& f : sv->active) f.bidx = f.sys->member;
+    chunk = true;
+  }
+  // ---- admission: residency, coordinates, lists (overlaps the chunk)
+  const auto t_adm = Clock::now();
+  Clock::time_point t_set;
   std::vector<InFlight> admitted;
   std::vector<ServeSystem*> pinned;
-  for (auto& a : sv->active) pinned.push_back(a.sys);
+  for (auto& f : sv->active) pinned.push_back(f.sys);
   std::vector<Request> deferred;
   for (auto& q : starting) {
     ServeSystem* ss = sv->systems[q.sid].get();
@@ -543,8 +673,15 @@ thp_status pass(Serve* sv, std::vector<Request>& starting) {
     pinned.push_back(ss);
     ss->last_use = ++sv->use_clock;
     thp_status rc = THP_OK;
-    if (q.has_xyz) rc = thp_system_update_coords(ss->s, q.x.data(), q.y.data(), q.z.data());
+    const auto tc0 = Clock::now();
+    if (q.has_xyz) rc = update_coords(ss->s, q.x.data(), q.y.data(), q.z.data(), ss->staging);
+    const auto tc1 = Clock::now();
     if (rc == THP_OK) rc = ensure_lists(ss->s);
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      sv->stats.coords_ms += ms_between(tc0, tc1);
+      sv->stats.lists_ms += ms_between(tc1, Clock::now());
+    }
     if (rc != THP_OK) {
       if (rc == THP_ERR_HIP) return rc;
       publish_error(sv, q, rc, t_pass);
@@ -563,6 +700,7 @@ thp_status pass(Serve* sv, std::vector<Request>& starting) {
     for (auto it = deferred.rbegin(); it != deferred.rend(); ++it) sv->queue.push_front(std::move(*it));
   }
   // ---- set-up of the admitted solves as one batch
+  t_set = Clock::now();
   if (!admitted.empty()) {
     std::vector<thp_sys*> mem;
     std::vector<int> warm;
@@ -576,65 +714,24 @@ thp_status pass(Serve* sv, std::vector<Request>& starting) {
     SRET(b_init_state(ctx, b, sv->o.poleps, sv->o.politer, sv->setup.warm));
     SRET(b_pme_prep(ctx, b));
     SRET(b_setup(ctx, b));
-    for (auto& f : admitted) {
-      f.sys->s->pme_valid = true;
-      f.sys->s->coef_valid = true;
-      sv->active.push_back(std::move(f));
-    }
-    sv->iter_dirty = true;
   }
-  if (sv->active.empty()) return THP_OK;
-  // ---- PCG iterations of everything in flight
-  std::vector<thp_sys*> mem;
-  for (auto& a : sv->active) mem.push_back(a.sys->s);
-  if (!sv->iter_dirty) {
-    for (size_t q = 0; q < mem.size(); ++q)
-      if (q >= sv->iter_versions.size() || sv->iter_versions[q] != mem[q]->view_version) sv->iter_dirty = true;
-  }
-  if (sv->iter_dirty || sv->iter_generation != sv->pool_generation) {
-    SRET(build_batch(sv, sv->iter, mem, {}));
-    SRET(tune_pair(sv, sv->iter.b, mem));
-    sv->iter_dirty = false;
-    sv->iter_generation = sv->pool_generation;
-    sv->iter_versions.clear();
-    for (thp_sys* s : mem) sv->iter_versions.push_back(s->view_version);
-  }
-  const int iters = std::max(1, sv->o.iters_per_graph);
-  SRET(run_chunk(sv, iters));
-  sv->stats.chunks += 1;
-  sv->active_sum += (double)sv->active.size();
-  sv->stats.pair_segs = sv->iter.b.segs;
-  // ---- energies of the finished solves, states back in one copy
-  Batch& b = sv->iter.b;
-  SRET(b_finish(ctx, b, 1));
-  if (sv->out_cap < b.nsys) {
-    if (sv->d_out) (void)hipFree(sv->d_out);
-    if (sv->h_out) (void)hipHostFree(sv->h_out);
-    sv->out_cap = std::max(b.nsys, 2 * sv->out_cap);
-    SHIP(ctx, hipMalloc((void**)&sv->d_out, sizeof(StateOut) * sv->out_cap));
-    SHIP(ctx, hipHostMalloc((void**)&sv->h_out, sizeof(StateOut) * sv->out_cap, hipHostMallocDefault));
-  }
-  SRET(b_collect(ctx, b, sv->d_out));
-  SHIP(ctx, hipMemcpyAsync(sv->h_out, sv->d_out, sizeof(StateOut) * b.nsys, hipMemcpyDeviceToHost, ctx->stream));
+  const auto t_wait = Clock::now();
   SHIP(ctx, hipStreamSynchronize(ctx->stream));
-  const auto now = Clock::now();
-  std::vector<InFlight> still;
-  for (size_t q = 0; q < sv->active.size(); ++q) {
-    const StateOut& so = sv->h_out[q];
-    InFlight& f = sv->active[q];
-    if (so.err == 0 && so.st.conv == kRunning) {
-      still.push_back(std::move(f));
-      continue;
-    }
-    f.sys->busy = false;
-    f.sys->has_mu = so.err == 0 && (so.st.conv == kConverged || so.st.conv == kZeroCurv);
-    f.sys->last_use = ++sv->use_clock;
-    publish(sv, f, so, now);
+  const auto t_done = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stats.host_ms += ms_between(t_pass, t_wait);
+    sv->stats.wait_ms += ms_between(t_wait, t_done);
+    sv->stats.chunk_ms += ms_between(t_pass, t_adm);
+    sv->stats.admit_ms += ms_between(t_adm, t_set);
+    sv->stats.setup_ms += ms_between(t_set, t_wait);
   }
-  if (still.size() != sv->active.size()) {
-    sv->active.swap(still);
-    sv->iter_dirty = true;
-    sv->cv_done.notify_all();
+  if (chunk) retire(sv);
+  for (auto& f : admitted) {
+    f.sys->s->pme_valid = true;
+    f.sys->s->coef_valid = true;
+    f.bidx = -1;
+    sv->active.push_back(std::move(f));
   }
   return THP_OK;
 }
@@ -822,10 +919,18 @@ thp_status thp_serve_poll(thp_ctx* ctx, uint64_t ticket, thp_result* result, dou
   if (!sv) return ctx ? THP_ERR_STATE : THP_ERR_ARG;
   if (!result) return THP_ERR_ARG;
   std::unique_lock<std::mutex> lk(sv->mu);
-  auto ready = [&] { return sv->done.count(ticket) > 0 || sv->stop; };
+  // ticket 0: any finished request (the oldest ticket among them)
+  auto find = [&] {
+    if (ticket != 0) return sv->done.find(ticket);
+    auto best = sv->done.end();
+    for (auto it = sv->done.begin(); it != sv->done.end(); ++it)
+      if (best == sv->done.end() || it->first < best->first) best = it;
+    return best;
+  };
+  auto ready = [&] { return find() != sv->done.end() || sv->stop; };
   if (wait_ms < 0) sv->cv_done.wait(lk, ready);
   else if (wait_ms > 0) sv->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(wait_ms), ready);
-  auto it = sv->done.find(ticket);
+  auto it = find();
   if (it == sv->done.end()) return sv->stop ? (sv->fatal != THP_OK ? sv->fatal : THP_ERR_STATE) : 0;
   *result = it->second;
   sv->done.erase(it);

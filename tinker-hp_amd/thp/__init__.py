@@ -91,7 +91,10 @@ class ServeStats(C.Structure):
     _fields_ = [
         ("solves", C.c_int64), ("chunks", C.c_int64), ("mean_active", C.c_double),
         ("graph_launches", C.c_int64), ("evictions", C.c_int64), ("resident_bytes", C.c_int64),
-        ("resident_systems", C.c_int), ("pair_segs", C.c_int),
+        ("resident_systems", C.c_int), ("pair_segs", C.c_int), ("batch_builds", C.c_int64),
+        ("host_ms", C.c_double), ("wait_ms", C.c_double), ("chunk_ms", C.c_double),
+        ("admit_ms", C.c_double), ("setup_ms", C.c_double), ("coords_ms", C.c_double),
+        ("lists_ms", C.c_double),
     ]
 
 

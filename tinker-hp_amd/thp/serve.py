@@ -83,67 +83,78 @@ def _pct(v, q):
 
 
 def bench_serving(ctx, system: System, steps: int = 20, warmup: int = 2, poleps: float = 1e-8, copies: int = 8,
-                  jitter: float = 0.01, warm_start: int = 1, iters_per_graph: int = 4, use_graphs: int = 0,
-                  seed: int = 11) -> dict:
+                  jitter: float = 0.01, warm_start: int = 1, iters_per_graph: int = 4, use_graphs: int = 1,
+                  variants: int = 4, seed: int = 11) -> dict:
     """Closed loop over `copies` resident systems with the geometry of
-    `system` (each a separate system; coordinates perturbed by N(0, jitter)
-    per request): `copies` requests stay outstanding, each result triggers
-    the next request of that system.  `warmup` rounds per system are not
-    timed.  Returns solves/s over the timed requests and latency
-    percentiles."""
+    `system` (each registered as its own system): `copies` requests stay
+    outstanding and each result triggers the next request of that system.
+    Requests cycle through `variants` coordinate sets per system, each the
+    reference geometry perturbed by N(0, jitter), generated before the timed
+    region.  `warmup` rounds per system are not timed.  Returns solves/s over
+    the timed requests and latency percentiles."""
     rng = np.random.default_rng(seed)
     srv = Server(ctx, max_systems=copies, iters_per_graph=iters_per_graph, poleps=poleps,
                  warm_start=warm_start, use_graphs=use_graphs)
     try:
         sids = [srv.add(system) for _ in range(copies)]
+        coords = [[tuple(np.ascontiguousarray(v + rng.normal(0.0, jitter, system.n))
+                         for v in (system.x, system.y, system.z)) for _ in range(variants)] for _ in sids]
+        nxt = [0] * copies
 
-        def coords():
-            return tuple(v + rng.normal(0.0, jitter, system.n) for v in (system.x, system.y, system.z))
+        def submit(k):
+            xyz = coords[k][nxt[k] % variants]
+            nxt[k] += 1
+            return srv.submit(sids[k], *xyz)
 
         # untimed rounds: device data creation, lists, plans, tuning
         for _ in range(max(1, warmup)):
-            ts = [srv.submit(s, *coords()) for s in sids]
+            ts = [submit(k) for k in range(copies)]
             for t in ts:
                 r = srv.poll(t)
                 if r.status != THP_OK:
                     raise ThpError(r.status, "serving warm-up solve")
         total = steps * copies
         lat, nit, eps = [], [], []
-        outstanding = {}
+        owner = {}
         t0 = time.perf_counter()
         issued = 0
-        for s in sids:
-            outstanding[srv.submit(s, *coords())] = s
+        for k in range(copies):
+            owner[submit(k)] = k
             issued += 1
-        while outstanding:
-            # results arrive in completion order; wait on the oldest ticket
-            t = min(outstanding)
-            r = srv.poll(t)
-            s = outstanding.pop(t)
+        while owner:
+            r = srv.poll(0)  # any finished request
+            k = owner.pop(r.ticket)
             if r.status != THP_OK:
                 raise ThpError(r.status, "serving solve")
             lat.append(r.latency_ms)
             nit.append(r.niter)
             eps.append(r.ep)
             if issued < total:
-                outstanding[srv.submit(s, *coords())] = s
+                owner[submit(k)] = k
                 issued += 1
         dt = time.perf_counter() - t0
         st = srv.stats()
         return dict(solves_per_s=total / dt, ms_per_solve=1e3 * dt / total, solves=total, copies=copies,
                     niter=float(np.mean(nit)), ep=float(eps[0]), p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99),
                     mean_active=st.mean_active, pair_segs=st.pair_segs, warm_start=warm_start,
-                    iters_per_graph=iters_per_graph, graphs=int(st.graph_launches > 0))
+                    iters_per_graph=iters_per_graph, graphs=int(st.graph_launches > 0),
+                    batch_builds=st.batch_builds, worker_host_ms=round(st.host_ms, 1),
+                    worker_wait_ms=round(st.wait_ms, 1), worker_chunk_ms=round(st.chunk_ms, 1),
+                    worker_admit_ms=round(st.admit_ms, 1), worker_setup_ms=round(st.setup_ms, 1),
+                    worker_coords_ms=round(st.coords_ms, 1), worker_lists_ms=round(st.lists_ms, 1))
     finally:
         srv.stop()
 
 
 def poisson_stream(ctx, n_requests=2000, rate_per_s=100.0, resubmits=10, n_min=500, n_max=30000, max_systems=16,
-                   poleps=1e-8, seed=5, jitter=0.01, warm_start=1, iters_per_graph=4, use_graphs=0):
+                   poleps=1e-8, seed=5, jitter=0.01, warm_start=1, iters_per_graph=4, use_graphs=1):
     """Poisson arrivals of solves of independent mixed-size systems: sizes
     log-uniform in [n_min, n_max], 0-30% solute; each system is requested
-    `resubmits` times at coordinates perturbed by N(0, jitter).  Returns
-    throughput and latency percentiles (submit -> result)."""
+    `resubmits` times at coordinates perturbed by N(0, jitter).  Systems are
+    registered and solved once before the stream (device data resident, as
+    for a server that has seen them), and every request's coordinates are
+    generated before the clock starts.  Returns throughput and latency
+    percentiles (submit -> result)."""
     rng = np.random.default_rng(seed)
     n_sys = max(1, n_requests // resubmits)
     srv = Server(ctx, max_systems=max_systems, iters_per_graph=iters_per_graph, poleps=poleps,
@@ -154,9 +165,14 @@ def poisson_stream(ctx, n_requests=2000, rate_per_s=100.0, resubmits=10, n_min=5
             n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
             systems.append(serving_system(n, float(rng.uniform(0.0, 0.3)), seed=1000 + k))
         sids = [srv.add(s) for s in systems]
+        for r in [srv.poll(t) for t in [srv.submit(sid) for sid in sids]]:
+            if r.status != THP_OK:
+                raise ThpError(r.status, "stream warm-up solve")
         order = np.repeat(np.arange(n_sys), resubmits)
         rng.shuffle(order)
         order = order[:n_requests]
+        coords = [tuple(np.ascontiguousarray(v + rng.normal(0.0, jitter, systems[k].n))
+                        for v in (systems[k].x, systems[k].y, systems[k].z)) for k in order]
         gaps = rng.exponential(1.0 / rate_per_s, len(order))
         tickets = []
         t0 = time.perf_counter()
@@ -166,21 +182,23 @@ def poisson_stream(ctx, n_requests=2000, rate_per_s=100.0, resubmits=10, n_min=5
             delay = t_next - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
-            s = systems[k]
-            x, y, z = (v + rng.normal(0.0, jitter, s.n) for v in (s.x, s.y, s.z))
-            tickets.append(srv.submit(sids[k], x, y, z))
-        lat, bad, atoms = [], 0, 0
+            tickets.append(srv.submit(sids[k], *coords[q]))
+        t_sub = time.perf_counter() - t0
+        lat, bad, atoms, nit = [], 0, 0, []
         for t in tickets:
             r = srv.poll(t)
             if r.status != THP_OK:
                 bad += 1
             lat.append(r.latency_ms)
+            nit.append(r.niter)
         dt = time.perf_counter() - t0
         st = srv.stats()
         for k in order:
             atoms += systems[k].n
-        return dict(requests=len(order), failed=bad, seconds=dt, solves_per_s=len(order) / dt,
-                    offered_rate=rate_per_s, p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99),
-                    mean_active=st.mean_active, evictions=st.evictions, mean_atoms=atoms / len(order))
+        return dict(requests=len(order), systems=n_sys, failed=bad, seconds=round(dt, 3),
+                    solves_per_s=len(order) / dt, offered_rate=rate_per_s, submit_seconds=round(t_sub, 3),
+                    p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99), niter=float(np.mean(nit)),
+                    mean_active=st.mean_active, evictions=st.evictions, mean_atoms=atoms / len(order),
+                    max_systems=max_systems)
     finally:
         srv.stop()
