@@ -72,6 +72,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--stream-requests", type=int, default=2000)
+    ap.add_argument("--copies", type=int, default=16, help="resident 24k systems in the serving leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -98,7 +99,8 @@ def main():
         try:
             import thp.serve as tserve
             serve = tserve.bench_serving(ctx, system, steps=max(args.steps, 8), warmup=args.warmup,
-                                         poleps=args.poleps)
+                                         poleps=args.poleps, copies=args.copies, iters_per_graph=4,
+                                         use_graphs=1)
             legs["serve"] = serve
             log(f"bench: serve {serve}")
         except (ImportError, AttributeError, thp.ThpError) as e:

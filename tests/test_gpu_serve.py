@@ -174,3 +174,27 @@ def test_graph_replay_matches_eager(ctx, iters):
             assert a.ep == b.ep and a.niter == b.niter
     for a, b in zip(m0, m1):
         assert np.array_equal(a, b)
+
+
+def test_coordinates_past_the_skin_rebuild_lists(ctx):
+    # the lists allow 0.25 A of motion (skin/2) before the block-union lists
+    # and 0.75 A before the block candidates must be rebuilt; the serving
+    # loop detects it on the device, rebuilds and repeats the set-up
+    s = thp.serving_system(3000, 0.1, seed=41)
+    rng = np.random.default_rng(12)
+    moves = [(s.x + 0.3, s.y, s.z), (s.x + 1.2, s.y - 0.9, s.z),
+             tuple(v + rng.normal(0.0, 0.2, s.n) for v in (s.x, s.y, s.z))]
+    with Server(ctx, max_systems=2, warm_start=1) as srv:
+        sid = srv.add(s)
+        assert srv.poll(srv.submit(sid)).status == thp.THP_OK
+        results = []
+        for x, y, z in moves:
+            r = srv.poll(srv.submit(sid, x, y, z))
+            results.append((r, srv.dipoles(sid)))
+    for (r, mu), (x, y, z) in zip(results, moves):
+        assert r.status == thp.THP_OK
+        one = thp.Solver(ctx, s.with_coords(x, y, z), MIXED)
+        m1, e1, _ = one.newinduce_pme2()
+        one.close()
+        assert abs(r.ep - e1) <= E_TOL * abs(e1), (r.ep, e1)
+        assert _rms_debye(mu, m1) <= MU_TOL_D

@@ -275,15 +275,15 @@ void launch_pair(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, in
 
 // direct permanent field + the pair coefficients and compacted lists that
 // the matvec of the same coordinates uses
-thp_status pair_efld0(thp_ctx* ctx, const Batch& b) {
+thp_status pair_efld0(thp_ctx* ctx, const Batch& b, int gate) {
   if (b.prec == THP_PREC_FP64) {
-    launch_pair<double, 0>(ctx, b, 0, 0, 0, 0);
-    hipLaunchKernelGGL(k_compact_lists<double>, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk);
+    launch_pair<double, 0>(ctx, b, 0, 0, 0, gate);
+    hipLaunchKernelGGL(k_compact_lists<double>, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, gate);
   } else {
-    launch_pair<float, 0>(ctx, b, 0, 0, 0, 0);
-    hipLaunchKernelGGL(k_compact_lists<float>, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk);
+    launch_pair<float, 0>(ctx, b, 0, 0, 0, gate);
+    hipLaunchKernelGGL(k_compact_lists<float>, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, gate);
   }
-  hipLaunchKernelGGL(k_scaled<0>, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, 0, 0, 0);
+  hipLaunchKernelGGL(k_scaled<0>, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, 0, 0, gate);
   return launch_check(ctx, "k_pair_union<efld0>");
 }
 
@@ -311,14 +311,14 @@ thp_status spread_uind(thp_ctx* ctx, const Batch& b, int vin, int frac, int chec
 }
 
 // grid_mpole_site (real part, imaginary part cleared)
-thp_status spread_mpole(thp_ctx* ctx, const Batch& b) {
+thp_status spread_mpole(thp_ctx* ctx, const Batch& b, int gate) {
   const dim3 t(spread_threads(b.prec));
   if (b.prec == THP_PREC_FP64) {
-    hipLaunchKernelGGL((k_spread_tile<double, 1>), g1(b.n_brick), t, 0, ctx->stream, b.d_views, b.d_brick, 0, 0, 0);
-    hipLaunchKernelGGL((k_spread_merge<double, 1>), g1(b.n_conv), dim3(256), 0, ctx->stream, b.d_views, b.d_conv, 0);
+    hipLaunchKernelGGL((k_spread_tile<double, 1>), g1(b.n_brick), t, 0, ctx->stream, b.d_views, b.d_brick, 0, 0, gate);
+    hipLaunchKernelGGL((k_spread_merge<double, 1>), g1(b.n_conv), dim3(256), 0, ctx->stream, b.d_views, b.d_conv, gate);
   } else {
-    hipLaunchKernelGGL((k_spread_tile<float, 1>), g1(b.n_brick), t, 0, ctx->stream, b.d_views, b.d_brick, 0, 0, 0);
-    hipLaunchKernelGGL((k_spread_merge<float, 1>), g1(b.n_conv), dim3(256), 0, ctx->stream, b.d_views, b.d_conv, 0);
+    hipLaunchKernelGGL((k_spread_tile<float, 1>), g1(b.n_brick), t, 0, ctx->stream, b.d_views, b.d_brick, 0, 0, gate);
+    hipLaunchKernelGGL((k_spread_merge<float, 1>), g1(b.n_conv), dim3(256), 0, ctx->stream, b.d_views, b.d_conv, gate);
   }
   return launch_check(ctx, "k_spread_tile<mpole>");
 }
@@ -339,19 +339,19 @@ thp_status convolve(thp_ctx* ctx, const Batch& b, int check_conv) {
 }
 
 template <int MODE>
-thp_status launch_gather_uind(thp_ctx* ctx, const Batch& b, int vin, int vout, int frac_out) {
-  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_tile<double, MODE>), g1(b.n_gat), dim3(64), 0, ctx->stream, b.d_views, b.d_gat, vin, vout, frac_out);
-  else hipLaunchKernelGGL((k_gather_tile<float, MODE>), g1(b.n_gat), dim3(64), 0, ctx->stream, b.d_views, b.d_gat, vin, vout, frac_out);
+thp_status launch_gather_uind(thp_ctx* ctx, const Batch& b, int vin, int vout, int frac_out, int gate) {
+  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_tile<double, MODE>), g1(b.n_gat), dim3(64), 0, ctx->stream, b.d_views, b.d_gat, vin, vout, frac_out, gate);
+  else hipLaunchKernelGGL((k_gather_tile<float, MODE>), g1(b.n_gat), dim3(64), 0, ctx->stream, b.d_views, b.d_gat, vin, vout, frac_out, gate);
   return launch_check(ctx, "k_gather_tile");
 }
 
 // permanent field: ef = E_direct - grad phi_recip + self (newinduce_pme2.f:111-147)
-thp_status field(thp_ctx* ctx, const Batch& b, double* cphi_dev, int add_to_ef, int do_direct) {
-  if (do_direct) RET(pair_efld0(ctx, b));
-  RET(spread_mpole(ctx, b));
-  RET(convolve(ctx, b, 0));
-  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_mpole<double>), g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, cphi_dev, add_to_ef);
-  else hipLaunchKernelGGL((k_gather_mpole<float>), g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, cphi_dev, add_to_ef);
+thp_status field(thp_ctx* ctx, const Batch& b, double* cphi_dev, int add_to_ef, int do_direct, int gate) {
+  if (do_direct) RET(pair_efld0(ctx, b, gate));
+  RET(spread_mpole(ctx, b, gate));
+  RET(convolve(ctx, b, gate));
+  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_mpole<double>), g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, cphi_dev, add_to_ef, gate);
+  else hipLaunchKernelGGL((k_gather_mpole<float>), g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, cphi_dev, add_to_ef, gate);
   return launch_check(ctx, "k_gather_mpole");
 }
 
@@ -431,6 +431,14 @@ thp_status build_union(thp_sys* s) {
       s->uy0 = s->hy;
       s->uz0 = s->hz;
       s->coef_valid = false;
+      {
+        // reference coordinates of the skin check of k_load_coords
+        std::vector<double> r((size_t)3 * s->npad, 0.0);
+        std::copy(s->hx.begin(), s->hx.end(), r.begin());
+        std::copy(s->hy.begin(), s->hy.end(), r.begin() + s->npad);
+        std::copy(s->hz.begin(), s->hz.end(), r.begin() + 2 * (size_t)s->npad);
+        HIPCHK(s->ctx, hipMemcpy((void*)s->view.refu, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice));
+      }
       // coefficient storage for the longest list
       std::vector<int> lc(s->npad);
       HIPCHK(s->ctx, hipMemcpy(lc.data(), s->view.lcount, sizeof(int) * s->npad, hipMemcpyDeviceToHost));
@@ -485,7 +493,7 @@ thp_status build_union(thp_sys* s) {
 
 thp_status ensure_pme(thp_sys* s) {
   if (!s->pme_valid) {
-    RET(b_pme_prep(s->ctx, s->one));
+    RET(b_pme_prep(s->ctx, s->one, kGateNone));
     RET(check_err_flag(s));
     s->pme_valid = true;
   }
@@ -511,7 +519,7 @@ thp_status ensure_lists(thp_sys* s) {
 thp_status ensure_coef(thp_sys* s) {
   RET(ensure_lists(s));
   if (!s->coef_valid) {
-    RET(pair_efld0(s->ctx, s->one));
+    RET(pair_efld0(s->ctx, s->one, kGateNone));
     s->coef_valid = true;
   }
   return THP_OK;
@@ -577,24 +585,36 @@ thp_status update_coords(thp_sys* s, const double* x, const double* y, const dou
 }
 
 // per coordinate set: B-spline values, grid nodes and brick bins
-thp_status b_pme_prep(thp_ctx* ctx, const Batch& b) {
+thp_status b_pme_prep(thp_ctx* ctx, const Batch& b, int gate) {
   hipStream_t st = ctx->stream;
-  hipLaunchKernelGGL(k_zero_bins, g1((b.n_brick + 255) / 256), dim3(256), 0, st, b.d_views, b.d_brick, b.n_brick);
+  hipLaunchKernelGGL(k_zero_bins, g1((b.n_brick + 255) / 256), dim3(256), 0, st, b.d_views, b.d_brick, b.n_brick, gate);
   RET(launch_check(ctx, "k_zero_bins"));
-  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_pme_prep<double>), g1(b.n_blk), dim3(64), 0, st, b.d_views, b.d_blk);
-  else hipLaunchKernelGGL((k_pme_prep<float>), g1(b.n_blk), dim3(64), 0, st, b.d_views, b.d_blk);
+  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_pme_prep<double>), g1(b.n_blk), dim3(64), 0, st, b.d_views, b.d_blk, gate);
+  else hipLaunchKernelGGL((k_pme_prep<float>), g1(b.n_blk), dim3(64), 0, st, b.d_views, b.d_blk, gate);
   RET(launch_check(ctx, "k_pme_prep"));
-  hipLaunchKernelGGL(k_pme_scan, g1(b.nsys), dim3(1024), 0, st, b.d_views, (const int*)b.d_ids);
+  hipLaunchKernelGGL(k_pme_scan, g1(b.nsys), dim3(1024), 0, st, b.d_views, (const int*)b.d_ids, gate);
   RET(launch_check(ctx, "k_pme_scan"));
-  hipLaunchKernelGGL(k_pme_fill, g1(b.n_blk), dim3(64), 0, st, b.d_views, b.d_blk);
+  hipLaunchKernelGGL(k_pme_fill, g1(b.n_blk), dim3(64), 0, st, b.d_views, b.d_blk, gate);
   RET(launch_check(ctx, "k_pme_fill"));
-  hipLaunchKernelGGL(k_pme_binsort, g1(b.n_brick), dim3(256), 0, st, b.d_views, b.d_brick);
+  hipLaunchKernelGGL(k_pme_binsort, g1(b.n_brick), dim3(256), 0, st, b.d_views, b.d_brick, gate);
   return launch_check(ctx, "k_pme_binsort");
 }
 
-thp_status b_init_state(thp_ctx* ctx, const Batch& b, double poleps, int politer, const int* d_warm) {
+// caller-order coordinates of the requested systems into their sorted slots
+thp_status b_load_coords(thp_ctx* ctx, const Batch& b, int gate) {
+  hipLaunchKernelGGL(k_load_coords, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, gate);
+  return launch_check(ctx, "k_load_coords");
+}
+
+thp_status b_end_setup(thp_ctx* ctx, const Batch& b) {
+  hipLaunchKernelGGL(k_end_setup, g1((b.nsys + 63) / 64), dim3(64), 0, ctx->stream, b.d_views, (const int*)b.d_ids, b.nsys);
+  return launch_check(ctx, "k_end_setup");
+}
+
+thp_status b_init_state(thp_ctx* ctx, const Batch& b, double poleps, int politer, const int* d_warm,
+                        const int* d_req) {
   hipLaunchKernelGGL(k_init_state, g1((b.nsys + 63) / 64), dim3(64), 0, ctx->stream, b.d_views, (const int*)b.d_ids,
-                     b.nsys, poleps, politer, d_warm);
+                     b.nsys, poleps, politer, d_warm, d_req);
   return launch_check(ctx, "k_init_state");
 }
 
@@ -622,23 +642,23 @@ thp_status b_pair_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int do
 
 // Field, guess and first residual (newinduce_pme2.f:111-196, 406-440);
 // the solve states are initialised and the B-spline tables current.
-thp_status b_setup(thp_ctx* ctx, const Batch& b) {
-  RET(field(ctx, b, nullptr, 1, 1));
-  hipLaunchKernelGGL(k_guess, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk);
+thp_status b_setup(thp_ctx* ctx, const Batch& b, int gate) {
+  RET(field(ctx, b, nullptr, 1, 1, gate));
+  hipLaunchKernelGGL(k_guess, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, gate);
   RET(launch_check(ctx, "k_guess"));
-  RET(b_pair_tmatxb(ctx, b, 0 /*mu*/, 3 /*hreal*/, 1, 0));
-  RET(spread_uind(ctx, b, 0, 0, 0));
-  RET(convolve(ctx, b, 0));
-  return launch_gather_uind<1>(ctx, b, 0, 3, 0);
+  RET(b_pair_tmatxb(ctx, b, 0 /*mu*/, 3 /*hreal*/, 1, gate));
+  RET(spread_uind(ctx, b, 0, 0, gate));
+  RET(convolve(ctx, b, gate));
+  return launch_gather_uind<1>(ctx, b, 0, 3, 0, gate);
 }
 
 // One PCG iteration of every running system (newinduce_pme2.f:442-569);
 // each system's iteration number lives in its SolveState.
 thp_status b_iteration(thp_ctx* ctx, const Batch& b) {
-  RET(b_pair_tmatxb(ctx, b, 1 /*pp*/, 3 /*hreal*/, 1, 1));
-  RET(spread_uind(ctx, b, 1, 0, 1));
-  RET(convolve(ctx, b, 1));
-  RET(launch_gather_uind<2>(ctx, b, 1, 3, 0));
+  RET(b_pair_tmatxb(ctx, b, 1 /*pp*/, 3 /*hreal*/, 1, kGateIter));
+  RET(spread_uind(ctx, b, 1, 0, kGateIter));
+  RET(convolve(ctx, b, kGateIter));
+  RET(launch_gather_uind<2>(ctx, b, 1, 3, 0, kGateIter));
   hipStream_t st = ctx->stream;
   if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_cg_update1<double>), g1(b.n_red), dim3(256), 0, st, b.d_views, b.d_red);
   else hipLaunchKernelGGL((k_cg_update1<float>), g1(b.n_red), dim3(256), 0, st, b.d_views, b.d_red);
@@ -661,6 +681,38 @@ thp_status b_collect(thp_ctx* ctx, const Batch& b, StateOut* d_out) {
 }
 
 thp_status download_mu(thp_sys* s, double* host) { return gather_out(s, host, s->view.mu, 6); }
+
+// After device-side coordinate loading: bring the sorted coordinates back to
+// the host copy and decide which lists stay valid (as update_coords does).
+thp_status sync_host_coords(thp_sys* s) {
+  const int n = s->n;
+  std::vector<double> x(n), y(n), z(n);
+  HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+  HIPCHK(s->ctx, hipMemcpy(x.data(), s->view.x, sizeof(double) * n, hipMemcpyDeviceToHost));
+  HIPCHK(s->ctx, hipMemcpy(y.data(), s->view.y, sizeof(double) * n, hipMemcpyDeviceToHost));
+  HIPCHK(s->ctx, hipMemcpy(z.data(), s->view.z, sizeof(double) * n, hipMemcpyDeviceToHost));
+  double maxd2 = 0.0, maxu2 = 0.0;
+  for (int q = 0; q < n; ++q) {
+    if (s->list_valid) {
+      const double dx = x[q] - s->bx0[q], dy = y[q] - s->by0[q], dz = z[q] - s->bz0[q];
+      maxd2 = std::max(maxd2, dx * dx + dy * dy + dz * dz);
+    }
+    if (s->tight_valid) {
+      const double dx = x[q] - s->ux0[q], dy = y[q] - s->uy0[q], dz = z[q] - s->uz0[q];
+      maxu2 = std::max(maxu2, dx * dx + dy * dy + dz * dz);
+    }
+  }
+  s->hx.swap(x);
+  s->hy.swap(y);
+  s->hz.swap(z);
+  const double lim_b = 0.5 * (s->buffer - s->skin), lim_u = 0.5 * s->skin;
+  if (s->list_valid && maxd2 > lim_b * lim_b) s->list_valid = false;
+  if (!s->list_valid || maxu2 > lim_u * lim_u) s->tight_valid = false;
+  s->verlet_valid = false;
+  s->pme_valid = false;
+  s->coef_valid = false;
+  return THP_OK;
+}
 
 void append_work(const thp_sys* s, int index, std::vector<Work>& blk, std::vector<Work>& red, std::vector<Work>& gat,
                  std::vector<Work>& brick, std::vector<Work>& conv) {
@@ -848,6 +900,7 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
   v.cut = s->cut;
   s->skin = std::min(0.5, 0.5 * s->buffer);
   v.skin = s->skin;
+  v.lim_u2 = 0.25 * s->skin * s->skin;
   v.ulim2 = (s->cut + s->skin) * (s->cut + s->skin);
   v.mbuf2 = mb * mb;
   v.term_self = (4.0 / 3.0) * s->aewald * s->aewald * s->aewald / std::sqrt(M_PI);
@@ -879,6 +932,24 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
   v.scale_tab = dtab; v.excl_off = deoff; v.excl_k = dek; v.excl_code = dec;
   ALLOC(s->iperm_dev, n);
   H2D(s->iperm_dev, s->iperm.data(), sizeof(int) * n);
+  {
+    // device-side coordinate loading (caller order -> sorted slots)
+    double *raw, *refu;
+    int* pd;
+    ALLOC(raw, (size_t)3 * n);
+    ALLOC(refu, (size_t)3 * np);
+    ALLOC(pd, np);
+    H2D(raw, d->x, sizeof(double) * n);
+    H2D(raw + n, d->y, sizeof(double) * n);
+    H2D(raw + 2 * (size_t)n, d->z, sizeof(double) * n);
+    std::vector<int> pp(np, 0);
+    for (int q = 0; q < n; ++q) pp[q] = s->perm[q];
+    H2D(pd, pp.data(), sizeof(int) * np);
+    if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "sync"));
+    v.raw = raw;
+    v.refu = refu;
+    v.perm = pd;
+  }
   ALLOC(v.nbcount, np); ALLOC(v.lcount, np); ALLOC(v.tcount, np); ALLOC(v.ucount, s->nblk); ALLOC(v.bbox, (size_t)s->nblk * 6);
   ALLOC(s->bcount, s->nblk);
   ALLOC(v.ef, (size_t)6 * np); ALLOC(v.mu, (size_t)6 * np); ALLOC(v.res, (size_t)6 * np);
@@ -1116,7 +1187,7 @@ thp_status thp_nblist_get(thp_sys* s, int64_t* npairs, int* ik, int64_t cap) {
 thp_status thp_efld0_direct(thp_sys* s, double* ef) {
   if (!s || !ef) return THP_ERR_ARG;
   RET(ensure_lists(s));
-  RET(pair_efld0(s->ctx, s->one));
+  RET(pair_efld0(s->ctx, s->one, kGateNone));
   s->coef_valid = true;
   return gather_out(s, ef, s->view.ef, 6);
 }
@@ -1124,7 +1195,7 @@ thp_status thp_efld0_direct(thp_sys* s, double* ef) {
 thp_status thp_efld0_recip(thp_sys* s, double* cphi) {
   if (!s || !cphi) return THP_ERR_ARG;
   RET(ensure_pme(s));
-  RET(field(s->ctx, s->one, s->cphi, 0, 0));
+  RET(field(s->ctx, s->one, s->cphi, 0, 0, kGateNone));
   return gather_out(s, cphi, s->cphi, 10);
 }
 
@@ -1153,7 +1224,7 @@ thp_status thp_tmatxb_recip(thp_sys* s, const double* mu, double* dipfield) {
   RET(scatter_in(s, s->view.pp, mu, 6));
   RET(spread_uind(s->ctx, s->one, 1, 0, 0));
   RET(convolve(s->ctx, s->one, 0));
-  RET(launch_gather_uind<0>(s->ctx, s->one, 1, 3, 0));
+  RET(launch_gather_uind<0>(s->ctx, s->one, 1, 3, 0, kGateNone));
   return gather_out(s, dipfield, s->view.hreal, 6);
 }
 
@@ -1203,7 +1274,7 @@ thp_status thp_fphi_uind(thp_sys* s, const double* qgrid, double* phi1, double* 
       RET(sync(s));
     }
   }
-  RET(launch_gather_uind<0>(s->ctx, s->one, 1, 3, 1));
+  RET(launch_gather_uind<0>(s->ctx, s->one, 1, 3, 1, kGateNone));
   std::vector<double> out((size_t)s->n * 6);
   RET(gather_out(s, out.data(), s->view.hreal, 6));
   for (int a = 0; a < s->n; ++a)
@@ -1229,8 +1300,8 @@ thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_s
   HIPCHK(ctx, hipEventRecord(s->ev[0], st));
   RET(ensure_lists(s));
   RET(ensure_pme(s));
-  RET(b_init_state(ctx, b, o->poleps, o->politer, warm ? s->d_one : nullptr));
-  RET(b_setup(ctx, b));
+  RET(b_init_state(ctx, b, o->poleps, o->politer, warm ? s->d_one : nullptr, nullptr));
+  RET(b_setup(ctx, b, kGateNone));
   s->coef_valid = true;
   HIPCHK(ctx, hipEventRecord(s->ev[1], st));
   int conv = 0;
@@ -1300,7 +1371,7 @@ thp_status thp_time_kernel(thp_sys* s, int kernel, int reps, double* ms, int64_t
   auto launch = [&]() -> thp_status {
     if (kernel == 0) return b_pair_tmatxb(s->ctx, s->one, 1, 3, 1, 0);
     if (kernel == 1) return spread_uind(s->ctx, s->one, 1, 0, 0);
-    return launch_gather_uind<0>(s->ctx, s->one, 1, 3, 0);
+    return launch_gather_uind<0>(s->ctx, s->one, 1, 3, 0, kGateNone);
   };
   RET(launch());  // warm
   HIPCHK(s->ctx, hipEventRecord(s->ev[0], st));

@@ -22,6 +22,10 @@ constexpr int kMaxAtoms = 1 << 24;
 // PCG termination codes held in SolveState::conv
 enum : int { kRunning = 0, kConverged = 1, kMaxIter = 2, kNonFinite = 3, kZeroCurv = 4 };
 
+// Which systems of a launch a kernel works on: all, those iterating (a
+// solve in progress and its set-up done), or those in set-up.
+enum : int { kGateNone = 0, kGateIter = 1, kGateSetup = 2 };
+
 // Device-resident state of one PCG solve (all scalars FP64 on the device).
 struct SolveState {
   double ggold[2];
@@ -38,6 +42,8 @@ struct SolveState {
   int n_indef;
   int warm;     // 1: mu holds the starting dipoles (no direct guess)
   int finalized;  // 1 once k_finish has written ep of a finished solve
+  int in_setup;   // 1 between k_init_state (requested) and k_end_setup
+  int stale;      // 1 if the coordinates moved past the block-union lists' skin
 };
 
 // Everything a kernel needs about one system; lives in device memory so a
@@ -60,6 +66,13 @@ struct SysView {
   const double* x;
   const double* y;
   const double* z;
+  // device-side coordinate loading: caller-order coordinates [3][n], the
+  // sorted slot -> caller index map, and the sorted coordinates at the last
+  // block-union build [3][npad] with the squared displacement they allow
+  const double* raw;
+  const int* perm;
+  const double* refu;
+  double lim_u2;
   const double* rpole;  // [10][npad]: q, dx,dy,dz, qxx,qxy,qxz,qyy,qyz,qzz
   const double* pol;
   const double* pdamp;
@@ -227,9 +240,14 @@ size_t grid_points(const thp_sys* s);
 void append_work(const thp_sys* s, int index, std::vector<Work>& blk, std::vector<Work>& red,
                  std::vector<Work>& gat, std::vector<Work>& brick, std::vector<Work>& conv);
 // batched launch sequences (enqueued on ctx->stream)
-thp_status b_init_state(thp_ctx* ctx, const Batch& b, double poleps, int politer, const int* d_warm);
-thp_status b_pme_prep(thp_ctx* ctx, const Batch& b);
-thp_status b_setup(thp_ctx* ctx, const Batch& b);
+// d_req (optional): per-system request flags; requested systems start a
+// solve in set-up state (kGateSetup kernels work on them until b_end_setup)
+thp_status b_init_state(thp_ctx* ctx, const Batch& b, double poleps, int politer, const int* d_warm,
+                        const int* d_req);
+thp_status b_load_coords(thp_ctx* ctx, const Batch& b, int gate);
+thp_status b_pme_prep(thp_ctx* ctx, const Batch& b, int gate);
+thp_status b_setup(thp_ctx* ctx, const Batch& b, int gate);
+thp_status b_end_setup(thp_ctx* ctx, const Batch& b);
 thp_status b_iteration(thp_ctx* ctx, const Batch& b);
 thp_status b_finish(thp_ctx* ctx, const Batch& b, int only_done);
 thp_status b_pair_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int check_conv);
@@ -242,4 +260,6 @@ struct StateOut {
 thp_status b_collect(thp_ctx* ctx, const Batch& b, StateOut* d_out);
 // the system's dipoles [n][2][3] in caller order into host memory (synchronous)
 thp_status download_mu(thp_sys* s, double* host);
+// host copy of the device's sorted coordinates and list validity (synchronous)
+thp_status sync_host_coords(thp_sys* s);
 }  // namespace thp

@@ -83,6 +83,12 @@ __device__ __forceinline__ double r2_exact(double xr, double yr, double zr) {
   return (a + b) + c;
 }
 
+__device__ __forceinline__ bool gated_out(const SolveState* st, int gate) {
+  if (gate == kGateNone) return false;
+  if (gate == kGateIter) return st->conv != kRunning || st->in_setup;
+  return !st->in_setup;
+}
+
 __device__ __forceinline__ double* vsel(const SysView& sv, int which) {
   switch (which) {
     case 0: return sv.mu;
@@ -450,11 +456,11 @@ __device__ __forceinline__ void scaled_pairs(const SysView& sv, const double* in
 // damping code out of the register budget of the pair kernels.
 template <int KIND>
 __global__ void __launch_bounds__(64) k_scaled(const SysView* views, const Work* work, int vin, int vout,
-                                               int check_conv) {
+                                               int gate) {
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
-  if (check_conv && sv.st->conv) return;
+  if (gated_out(sv.st, gate)) return;
   const int i = w.blk * kBlock + threadIdx.x;
   if (i >= sv.n) return;
   if (sv.excl_off[i] == sv.excl_off[i + 1]) return;
@@ -642,7 +648,7 @@ struct UnionStage {
 
 template <typename real, int KIND, int S>
 __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, const Work* work, int vin, int vout,
-                                                        int dodiag, int check_conv) {
+                                                        int dodiag, int gate) {
   using ST = UnionStage<real, KIND>;
   constexpr int C = ST::CHUNK;
   constexpr int NM = ST::NM;
@@ -654,7 +660,7 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
-  if (check_conv && sv.st->conv) return;
+  if (gated_out(sv.st, gate)) return;
   const int n = sv.n;
   const size_t np = sv.npad;
   const double bx = sv.box[0], by = sv.box[1], bz = sv.box[2], cut2 = sv.cut2;
@@ -797,11 +803,12 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
 // in order, to the front of the compacted list and coefficient arrays.
 // Loads are issued 16 entries at a time ahead of the stores.
 template <typename real>
-__global__ void __launch_bounds__(64) k_compact_lists(const SysView* views, const Work* work) {
+__global__ void __launch_bounds__(64) k_compact_lists(const SysView* views, const Work* work, int gate) {
   constexpr int NB = 16;
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
+  if (gated_out(sv.st, gate)) return;
   const int i = w.blk * kBlock + threadIdx.x;
   if (i >= sv.n) return;
   const size_t np = sv.npad;
@@ -871,7 +878,7 @@ __device__ __forceinline__ void load_entries(const GLOBAL unsigned short* LL, co
 
 template <typename real, int S>
 __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, const Work* work, int vin, int vout,
-                                                       int dodiag, int check_conv) {
+                                                       int dodiag, int gate) {
   constexpr int C = ClStage<real>::CHUNK;
   constexpr int NB = 8;  // list entries per batch; the next batch is in flight meanwhile
   // the staging arrays and the final cross-wavefront sum share one buffer
@@ -885,7 +892,7 @@ __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, cons
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
-  if (check_conv && sv.st->conv) return;
+  if (gated_out(sv.st, gate)) return;
   const int n = sv.n;
   const size_t np = sv.npad;
   const double bx = sv.box[0], by = sv.box[1], bz = sv.box[2];
@@ -1094,10 +1101,11 @@ __device__ __forceinline__ int th_idx(int axis, int level, int node) { return ax
 
 // per-solve: g0, theta and home bin of every atom; bin counts
 template <typename real>
-__global__ void __launch_bounds__(64) k_pme_prep(const SysView* views, const Work* work) {
+__global__ void __launch_bounds__(64) k_pme_prep(const SysView* views, const Work* work, int gate) {
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
+  if (gated_out(sv.st, gate)) return;
   const int i = w.blk * kBlock + threadIdx.x;
   if (i >= sv.n) return;
   const double xyz[3] = {sv.x[i], sv.y[i], sv.z[i]};
@@ -1122,8 +1130,9 @@ __global__ void __launch_bounds__(64) k_pme_prep(const SysView* views, const Wor
 }
 
 // exclusive scan of the bin counts (one workgroup per system)
-__global__ void __launch_bounds__(1024) k_pme_scan(const SysView* views, const int* sys_ids) {
+__global__ void __launch_bounds__(1024) k_pme_scan(const SysView* views, const int* sys_ids, int gate) {
   const SysView& sv = views[sys_ids[blockIdx.x]];
+  if (gated_out(sv.st, gate)) return;
   const int nb = sv.nbrick[0] * sv.nbrick[1] * sv.nbrick[2];
   __shared__ int carry;
   __shared__ int wsum[16];
@@ -1159,10 +1168,11 @@ __global__ void __launch_bounds__(1024) k_pme_scan(const SysView* views, const i
   if (threadIdx.x == 0) sv.bin_start[nb] = carry;
 }
 
-__global__ void __launch_bounds__(64) k_pme_fill(const SysView* views, const Work* work) {
+__global__ void __launch_bounds__(64) k_pme_fill(const SysView* views, const Work* work, int gate) {
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
+  if (gated_out(sv.st, gate)) return;
   const int i = w.blk * kBlock + threadIdx.x;
   if (i >= sv.n) return;
   const int b = sv.g0[4 * i + 3];
@@ -1173,10 +1183,11 @@ __global__ void __launch_bounds__(64) k_pme_fill(const SysView* views, const Wor
 // sort each bin by atom index (bitonic, in LDS) so that spreading order
 // does not depend on the fill order
 constexpr int kMaxBin = 2048;
-__global__ void __launch_bounds__(256) k_pme_binsort(const SysView* views, const Work* bwork) {
+__global__ void __launch_bounds__(256) k_pme_binsort(const SysView* views, const Work* bwork, int gate) {
   const Work w = bwork[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
+  if (gated_out(sv.st, gate)) return;
   const int b = w.blk;
   const int s0 = sv.bin_start[b], cnt = sv.bin_start[b + 1] - s0;
   if (cnt <= 1) return;
@@ -1218,7 +1229,7 @@ __global__ void __launch_bounds__(256) k_pme_binsort(const SysView* views, const
 constexpr int kSpreadWaves = 4;
 template <typename real, int KIND>
 __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_brick(const SysView* views, const Work* bwork, int vin,
-                                                                     int frac_in, int check_conv) {
+                                                                     int frac_in, int gate) {
   constexpr int NL = KIND == 0 ? 2 : 3;   // spline levels used
   constexpr int NF = KIND == 0 ? 6 : 10;  // moments per atom
   constexpr int NT = 3 * NL * kOrder;
@@ -1229,7 +1240,7 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_brick(const SysVie
   const Work w = bwork[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
-  if (check_conv && sv.st->conv) return;
+  if (gated_out(sv.st, gate)) return;
   const int nb0 = sv.nbrick[0], nb1 = sv.nbrick[1], nb2 = sv.nbrick[2];
   const int tb = w.blk;
   const int t0 = tb % nb0, t1 = (tb / nb0) % nb1, t2 = tb / (nb0 * nb1);
@@ -1416,7 +1427,7 @@ struct SpreadTileCfg {
 
 template <typename real, int KIND>
 __global__ void __launch_bounds__(64 * SpreadTileCfg<real>::WAVES) k_spread_tile(const SysView* views, const Work* bwork,
-                                                                                  int vin, int frac_in, int check_conv) {
+                                                                                  int vin, int frac_in, int gate) {
   constexpr int W = SpreadTileCfg<real>::WAVES;
   constexpr int NL = KIND == 0 ? 2 : 3;   // spline levels used
   constexpr int NF = KIND == 0 ? 6 : 10;  // moments per atom
@@ -1428,7 +1439,7 @@ __global__ void __launch_bounds__(64 * SpreadTileCfg<real>::WAVES) k_spread_tile
   const Work w = bwork[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
-  if (check_conv && sv.st->conv) return;
+  if (gated_out(sv.st, gate)) return;
   const int nb0 = sv.nbrick[0], nb1 = sv.nbrick[1];
   const int tb = w.blk;
   const int t0 = tb % nb0, t1 = (tb / nb0) % nb1, t2 = tb / (nb0 * nb1);
@@ -1552,11 +1563,11 @@ __global__ void __launch_bounds__(64 * SpreadTileCfg<real>::WAVES) k_spread_tile
 // grid point = own brick's tile + overhang of the previous brick per axis
 // (fixed order); KIND 1 clears the imaginary part
 template <typename real, int KIND>
-__global__ void __launch_bounds__(256) k_spread_merge(const SysView* views, const Work* work, int check_conv) {
+__global__ void __launch_bounds__(256) k_spread_merge(const SysView* views, const Work* work, int gate) {
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
-  if (check_conv && sv.st->conv) return;
+  if (gated_out(sv.st, gate)) return;
   const int K0 = sv.nfft[0], K1 = sv.nfft[1], K2 = sv.nfft[2];
   const int B0 = sv.brick[0], B1 = sv.brick[1], B2 = sv.brick[2];
   const int nb0 = sv.nbrick[0], nb1 = sv.nbrick[1], nb2 = sv.nbrick[2];
@@ -1599,11 +1610,11 @@ __global__ void __launch_bounds__(256) k_spread_merge(const SysView* views, cons
 // scalar-sum convolution between the forward and backward transforms
 // (qfac, newinduce_pme.f:1790-1861)
 template <typename real>
-__global__ void __launch_bounds__(256) k_conv(const SysView* views, const Work* work, int check_conv) {
+__global__ void __launch_bounds__(256) k_conv(const SysView* views, const Work* work, int gate) {
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
-  if (check_conv && sv.st->conv) return;
+  if (gated_out(sv.st, gate)) return;
   const size_t npts = (size_t)sv.nfft[0] * sv.nfft[1] * sv.nfft[2];
   GLOBAL real* g = gw(static_cast<real*>(sv.grid));
   const GLOBAL real* q = gp(static_cast<const real*>(sv.qfac));
@@ -1744,12 +1755,12 @@ __global__ void __launch_bounds__(256) k_gather_uind(const SysView* views, const
 // one pair per brick.
 template <typename real, int MODE>
 __global__ void __launch_bounds__(64) k_gather_tile(const SysView* views, const Work* bwork, int vin, int vout,
-                                                     int frac_out) {
+                                                     int frac_out, int gate) {
   __shared__ real s_g[kTile3 * 2];
   const Work w = bwork[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
-  if (MODE == 2 && sv.st->conv) return;
+  if (gated_out(sv.st, gate)) return;
   const int K0 = sv.nfft[0], K1 = sv.nfft[1], K2 = sv.nfft[2];
   const int B0 = sv.brick[0], B1 = sv.brick[1], B2 = sv.brick[2];
   const int nb0 = sv.nbrick[0], nb1 = sv.nbrick[1], nb2 = sv.nbrick[2];
@@ -1865,10 +1876,11 @@ __global__ void __launch_bounds__(64) k_gather_tile(const SysView* views, const 
 // (newinduce_pme2.f:134-147); cphi (optional) receives [10][npad].
 template <typename real>
 __global__ void __launch_bounds__(64) k_gather_mpole(const SysView* views, const Work* work, double* cphi_out,
-                                                      int add_to_ef) {
+                                                      int add_to_ef, int gate) {
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
+  if (gated_out(sv.st, gate)) return;
   const int i = w.blk * kBlock + threadIdx.x;
   if (i >= sv.n) return;
   const size_t np = sv.npad;
@@ -1965,11 +1977,11 @@ __global__ void __launch_bounds__(64) k_gather_mpole(const SysView* views, const
 // every block obtains bit-identical scalars.
 
 // direct guess mu = alpha*ef (polgsf = 0, newinduce_pme2.f:180-188)
-__global__ void __launch_bounds__(64) k_guess(const SysView* views, const Work* work) {
+__global__ void __launch_bounds__(64) k_guess(const SysView* views, const Work* work, int gate) {
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
-  if (sv.st->warm) return;
+  if (gated_out(sv.st, gate) || sv.st->warm) return;
   const int i = w.blk * kBlock + threadIdx.x;
   if (i >= sv.n) return;
   const double a = sv.pol[i];
@@ -1989,7 +2001,7 @@ __global__ void __launch_bounds__(256) k_cg_update1(const SysView* views, const 
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
   SolveState* st = sv.st;
-  if (st->conv) return;
+  if (st->conv || st->in_setup) return;
   const int it = st->niter + 1;
   const int G = sv.red_blocks, P = sv.gat_blocks;
   __shared__ double sc[4];
@@ -2058,7 +2070,7 @@ __global__ void __launch_bounds__(256) k_cg_update2(const SysView* views, const 
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
   SolveState* st = sv.st;
-  if (st->conv) return;
+  if (st->conv || st->in_setup) return;
   const int it = st->it_cur;
   const int G = sv.red_blocks;
   __shared__ double sc[3];
@@ -2112,7 +2124,7 @@ __global__ void __launch_bounds__(256) k_energy(const SysView* views, const Work
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
-  if (only_done && (sv.st->conv == kRunning || sv.st->finalized)) return;
+  if (only_done && (sv.st->conv == kRunning || sv.st->finalized || sv.st->in_setup)) return;
   const size_t np = sv.npad;
   double e = 0.0;
   for (int i = w.blk * blockDim.x + threadIdx.x; i < sv.n; i += sv.red_blocks * blockDim.x) {
@@ -2125,7 +2137,7 @@ __global__ void __launch_bounds__(256) k_energy(const SysView* views, const Work
 // one wave per system: final scalars into the SolveState
 __global__ void __launch_bounds__(64) k_finish(const SysView* views, const int* sys_ids, int only_done) {
   const SysView& sv = views[sys_ids[blockIdx.x]];
-  if (only_done && (sv.st->conv == kRunning || sv.st->finalized)) return;
+  if (only_done && (sv.st->conv == kRunning || sv.st->finalized || sv.st->in_setup)) return;
   const double e = sum_partials(part_ep(sv), sv.red_blocks, 0);
   const double e0 = sum_partials(part_ene(sv), sv.red_blocks, 0), e1 = sum_partials(part_ene(sv), sv.red_blocks, 1);
   if (threadIdx.x == 0) {
@@ -2140,9 +2152,10 @@ __global__ void __launch_bounds__(64) k_finish(const SysView* views, const int* 
 // (warm[s] != 0: mu already holds the starting dipoles); a kernel rather
 // than a host copy, so that it can sit in a captured sequence
 __global__ void k_init_state(const SysView* views, const int* sys_ids, int nsys, double poleps, int politer,
-                             const int* warm) {
+                             const int* warm, const int* req) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nsys) return;
+  if (req && !req[q]) return;  // only the requested systems start a solve
   SolveState* st = views[sys_ids[q]].st;
   SolveState z;
   z.ggold[0] = z.ggold[1] = z.gg[0] = z.gg[1] = z.ggnew[0] = z.ggnew[1] = 0.0;
@@ -2157,15 +2170,46 @@ __global__ void k_init_state(const SysView* views, const int* sys_ids, int nsys,
   z.n_indef = 0;
   z.warm = warm ? warm[q] : 0;
   z.finalized = 0;
+  z.in_setup = req ? 1 : 0;
+  z.stale = 0;
   *st = z;
 }
 
+// end of the set-up phase: the systems set up in this launch iterate from now
+__global__ void k_end_setup(const SysView* views, const int* sys_ids, int nsys) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nsys) return;
+  SolveState* st = views[sys_ids[q]].st;
+  if (st->in_setup) st->in_setup = 0;
+}
+
+// caller-order coordinates -> sorted slots, with the skin check of the
+// block-union lists (stale: the host rebuilds the lists and repeats the set-up)
+__global__ void __launch_bounds__(64) k_load_coords(const SysView* views, const Work* work, int gate) {
+  const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
+  const SysView& sv = views[w.sys];
+  if (gated_out(sv.st, gate)) return;
+  const int i = w.blk * kBlock + threadIdx.x;
+  if (i >= sv.n) return;
+  const int n = sv.n;
+  const size_t np = sv.npad;
+  const int a = sv.perm[i];
+  const double x = sv.raw[a], y = sv.raw[(size_t)n + a], z = sv.raw[2 * (size_t)n + a];
+  const_cast<double*>(sv.x)[i] = x;
+  const_cast<double*>(sv.y)[i] = y;
+  const_cast<double*>(sv.z)[i] = z;
+  const double dx = x - sv.refu[i], dy = y - sv.refu[np + i], dz = z - sv.refu[2 * np + i];
+  if (dx * dx + dy * dy + dz * dz > sv.lim_u2) sv.st->stale = 1;
+}
+
 // bin counters of the brick bins, zeroed before k_pme_prep counts into them
-__global__ void k_zero_bins(const SysView* views, const Work* bwork, int nwork) {
+__global__ void k_zero_bins(const SysView* views, const Work* bwork, int nwork, int gate) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nwork) return;
   const Work w = bwork[q];
   if (w.sys < 0) return;
+  if (gated_out(views[w.sys].st, gate)) return;
   views[w.sys].bin_count[w.blk] = 0;
 }
 

@@ -3,14 +3,19 @@
 // One worker thread owns the device work of a context while the loop runs.
 // Per pass it (1) admits queued requests into free slots, creating the
 // device data of systems that are not resident (evicting idle ones when the
-// HBM budget is short), (2) runs the solve set-up (B-spline tables,
-// permanent field, guess or warm start, first residual) of the admitted
-// requests as one batch, (3) runs `iters_per_graph` PCG iterations of every
-// solve in flight as one batch (eagerly or as a replayed HIP graph), and
-// (4) reads every solve state back in one copy and retires the finished
-// solves.  Kernels walk work tables of (system, unit) entries, so solves at
-// different iterations share each launch; reciprocal grids of equal size
-// sit side by side in a pool and go through one batched rocFFT plan.
+// HBM budget is short) and uploading the requests' coordinates as given,
+// (2) runs one launch sequence over the batch of member systems: the
+// set-up of the admitted solves (coordinates into sorted order, B-spline
+// tables, permanent field and pair coefficients, guess or warm start, first
+// residual; kernels gated to the systems in set-up), then
+// `iters_per_graph` PCG iterations of every solve in flight, then the
+// energies of the solves that finished, and (3) reads every solve state
+// back in one copy and retires the finished solves.  The sequence is
+// captured once per batch layout as a HIP graph and replayed, so a pass
+// costs the host a few copies and one launch.  Kernels walk work tables of
+// (system, unit) entries, so solves at different stages share each launch;
+// reciprocal grids of equal size sit side by side in a pool and go through
+// one batched rocFFT plan.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -41,7 +46,6 @@ struct ServeSystem {
   std::vector<double> x, y, z, rpole, pol, thole, dpu;
   std::vector<int> ik;
   thp_sys* s = nullptr;      // device data when resident
-  double* staging = nullptr; // pinned host copy of sorted coordinates (3*npad), when resident
   int64_t bytes = 0;         // device footprint once created (estimate before)
   bool busy = false;         // a solve of it is in flight
   bool has_mu = false;       // device mu holds its last solution
@@ -53,8 +57,8 @@ struct ServeSystem {
 struct Request {
   uint64_t ticket = 0;
   int sid = 0;
-  bool has_xyz = false;
-  std::vector<double> x, y, z;
+  double* xyz = nullptr;  // pinned [3][n] caller-order coordinates, or null (unchanged)
+  size_t xyz_cap = 0;     // doubles held by xyz (a pool buffer)
   Clock::time_point t_submit;
 };
 
@@ -63,7 +67,7 @@ struct InFlight {
   ServeSystem* sys = nullptr;
   Clock::time_point t_start;
   int warm = 0;
-  int bidx = -1;  // index in the iteration batch of the last chunk that carried it
+  bool restart = false;  // set-up repeated after a list rebuild (stale coordinates)
 };
 
 struct DipoleCopy {  // thp_serve_get_dipoles handed to the worker
@@ -99,9 +103,11 @@ struct BatchBuf {
   int* h_ids = nullptr;
   int* h_warm = nullptr;
   size_t hcap_views = 0, hcap_tables = 0;
-  // graph of the current layout
-  hipGraphExec_t exec = nullptr;
-  hipGraph_t graph = nullptr;
+  int* req = nullptr;    // per-system set-up request flags (device)
+  int* h_req = nullptr;  // pinned staging of req
+  // graphs of the current layout: set-up + iterations, iterations only
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipGraph_t graph[2] = {nullptr, nullptr};
   int graph_iters = 0;
 };
 
@@ -142,7 +148,33 @@ struct Serve {
   int out_cap = 0;
   bool graphs_ok = true;
   std::map<std::pair<int, int>, int> tuned;  // (log2 atoms, systems) -> pair waves per block
+  std::multimap<size_t, double*> pinned_pool;  // free request coordinate buffers, by capacity (under mu)
+  std::vector<InFlight> restarts;              // set-ups to repeat after list rebuilds (worker)
 };
+
+// request coordinate buffers: pinned, recycled through the pool
+double* take_pinned(Serve* sv, size_t doubles, size_t* cap) {
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->pinned_pool.lower_bound(doubles);
+    if (it != sv->pinned_pool.end() && it->first <= 2 * doubles) {
+      double* p = it->second;
+      *cap = it->first;
+      sv->pinned_pool.erase(it);
+      return p;
+    }
+  }
+  double* p = nullptr;
+  if (hipHostMalloc((void**)&p, sizeof(double) * doubles, hipHostMallocDefault) != hipSuccess) return nullptr;
+  *cap = doubles;
+  return p;
+}
+
+void give_pinned(Serve* sv, double* p, size_t cap) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  sv->pinned_pool.emplace(cap, p);
+}
 
 #define SRET(call)                   \
   do {                               \
@@ -186,18 +218,20 @@ thp_status grow_pinned(thp_ctx* ctx, void** p, size_t* cap, size_t need, size_t 
 }
 
 void drop_graph(BatchBuf& bb) {
-  if (bb.exec) (void)hipGraphExecDestroy(bb.exec);
-  if (bb.graph) (void)hipGraphDestroy(bb.graph);
-  bb.exec = nullptr;
-  bb.graph = nullptr;
+  for (int v = 0; v < 2; ++v) {
+    if (bb.exec[v]) (void)hipGraphExecDestroy(bb.exec[v]);
+    if (bb.graph[v]) (void)hipGraphDestroy(bb.graph[v]);
+    bb.exec[v] = nullptr;
+    bb.graph[v] = nullptr;
+  }
   bb.graph_iters = 0;
 }
 
 void free_batchbuf(BatchBuf& bb) {
   drop_graph(bb);
-  for (void* p : {(void*)bb.views, (void*)bb.tables, (void*)bb.ids, (void*)bb.warm})
+  for (void* p : {(void*)bb.views, (void*)bb.tables, (void*)bb.ids, (void*)bb.warm, (void*)bb.req})
     if (p) (void)hipFree(p);
-  for (void* p : {(void*)bb.h_views, (void*)bb.h_tables, (void*)bb.h_ids, (void*)bb.h_warm})
+  for (void* p : {(void*)bb.h_views, (void*)bb.h_tables, (void*)bb.h_ids, (void*)bb.h_warm, (void*)bb.h_req})
     if (p) (void)hipHostFree(p);
   bb = BatchBuf();
 }
@@ -294,14 +328,16 @@ thp_status build_batch(Serve* sv, BatchBuf& bb, const std::vector<thp_sys*>& mem
   SRET(grow_device(ctx, (void**)&bb.views, &bb.cap_views, m, sizeof(SysView)));
   SRET(grow_device(ctx, (void**)&bb.tables, &bb.cap_tables, nt, sizeof(Work)));
   if (!bb.ids || cap_ids < bb.cap_views) {
-    if (bb.ids) (void)hipFree(bb.ids);
-    if (bb.warm) (void)hipFree(bb.warm);
-    if (bb.h_ids) (void)hipHostFree(bb.h_ids);
-    if (bb.h_warm) (void)hipHostFree(bb.h_warm);
+    for (void* p : {(void*)bb.ids, (void*)bb.warm, (void*)bb.req})
+      if (p) (void)hipFree(p);
+    for (void* p : {(void*)bb.h_ids, (void*)bb.h_warm, (void*)bb.h_req})
+      if (p) (void)hipHostFree(p);
     SHIP(ctx, hipMalloc((void**)&bb.ids, bb.cap_views * sizeof(int)));
     SHIP(ctx, hipMalloc((void**)&bb.warm, bb.cap_views * sizeof(int)));
+    SHIP(ctx, hipMalloc((void**)&bb.req, bb.cap_views * sizeof(int)));
     SHIP(ctx, hipHostMalloc((void**)&bb.h_ids, bb.cap_views * sizeof(int), hipHostMallocDefault));
     SHIP(ctx, hipHostMalloc((void**)&bb.h_warm, bb.cap_views * sizeof(int), hipHostMallocDefault));
+    SHIP(ctx, hipHostMalloc((void**)&bb.h_req, bb.cap_views * sizeof(int), hipHostMallocDefault));
   }
   Batch& b = bb.b;
   b = Batch();
@@ -361,8 +397,6 @@ thp_status release_system(Serve* sv, ServeSystem* ss) {
   sv->resident -= ss->bytes;
   thp_system_destroy(ss->s);
   ss->s = nullptr;
-  if (ss->staging) (void)hipHostFree(ss->staging);
-  ss->staging = nullptr;
   ss->has_mu = false;
   sv->stats.evictions += 1;
   return THP_OK;
@@ -392,10 +426,6 @@ thp_status admit(Serve* sv, ServeSystem* ss, const std::vector<ServeSystem*>& pi
   }
   thp_sys* s = nullptr;
   SRET(create_system(sv->ctx, &ss->desc, &s, false));
-  if (hipHostMalloc((void**)&ss->staging, sizeof(double) * 3 * s->npad, hipHostMallocDefault) != hipSuccess) {
-    thp_system_destroy(s);
-    return fail(sv->ctx, THP_ERR_NOMEM, "pinned staging for system %d", ss->id);
-  }
   ss->s = s;
   ss->bytes = s->bytes;
   sv->resident += ss->bytes;
@@ -455,24 +485,42 @@ thp_status tune_pair(Serve* sv, Batch& b, const std::vector<thp_sys*>& members) 
   return THP_OK;
 }
 
-thp_status run_chunk(Serve* sv, int iters) {
+// The launch sequence of one pass over the member batch: with set-up
+// (variant 1) the requested systems go through their set-up first; then
+// `iters` PCG iterations of everything in flight, the energies of the
+// solves that finished, and the state collection.
+thp_status enqueue_pass(Serve* sv, int variant, int iters) {
+  thp_ctx* ctx = sv->ctx;
+  Batch& b = sv->iter.b;
+  if (variant) {
+    SRET(b_init_state(ctx, b, sv->o.poleps, sv->o.politer, sv->iter.warm, sv->iter.req));
+    SRET(b_load_coords(ctx, b, kGateSetup));
+    SRET(b_pme_prep(ctx, b, kGateSetup));
+    SRET(b_setup(ctx, b, kGateSetup));
+    SRET(b_end_setup(ctx, b));
+  }
+  for (int k = 0; k < iters; ++k) SRET(b_iteration(ctx, b));
+  SRET(b_finish(ctx, b, 1));
+  return b_collect(ctx, b, sv->d_out);
+}
+
+// Run one pass, replaying the layout's captured graph of the variant when
+// capture works (the graph reads the request flags and coordinates from
+// the batch's device buffers, written before the launch).
+thp_status run_pass(Serve* sv, int variant, int iters) {
   thp_ctx* ctx = sv->ctx;
   BatchBuf& bb = sv->iter;
+  if (bb.graph_iters != iters) drop_graph(bb);
   if (sv->o.use_graphs && sv->graphs_ok) {
-    if (!bb.exec || bb.graph_iters != iters) {
-      drop_graph(bb);
+    if (!bb.exec[variant]) {
       hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
-      thp_status r = e == hipSuccess ? THP_OK : THP_ERR_HIP;
-      for (int k = 0; k < iters && r == THP_OK; ++k) r = b_iteration(ctx, bb.b);
+      thp_status r = e == hipSuccess ? enqueue_pass(sv, variant, iters) : THP_ERR_HIP;
       hipGraph_t g = nullptr;
-      if (e == hipSuccess) {
-        hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
-        if (e2 != hipSuccess) r = THP_ERR_HIP;
-      }
+      if (e == hipSuccess && hipStreamEndCapture(ctx->stream, &g) != hipSuccess) r = THP_ERR_HIP;
       hipGraphExec_t x = nullptr;
       if (r == THP_OK && g && hipGraphInstantiate(&x, g, nullptr, nullptr, 0) == hipSuccess) {
-        bb.graph = g;
-        bb.exec = x;
+        bb.graph[variant] = g;
+        bb.exec[variant] = x;
         bb.graph_iters = iters;
       } else {
         // capture is not available for this sequence: launch eagerly from now on
@@ -481,14 +529,13 @@ thp_status run_chunk(Serve* sv, int iters) {
         sv->graphs_ok = false;
       }
     }
-    if (bb.exec) {
-      SHIP(ctx, hipGraphLaunch(bb.exec, ctx->stream));
+    if (bb.exec[variant]) {
+      SHIP(ctx, hipGraphLaunch(bb.exec[variant], ctx->stream));
       sv->stats.graph_launches += 1;
       return THP_OK;
     }
   }
-  for (int k = 0; k < iters; ++k) SRET(b_iteration(ctx, bb.b));
-  return THP_OK;
+  return enqueue_pass(sv, variant, iters);
 }
 
 thp_status handle_copies(Serve* sv, std::deque<DipoleCopy*>& pending) {
@@ -551,32 +598,6 @@ void publish_error(Serve* sv, const Request& q, thp_status code, Clock::time_poi
 
 constexpr int kIdlePasses = 4;
 
-// Retire the solves of the last chunk from the states read back into
-// h_out (indexed by each solve's batch index at launch).
-void retire(Serve* sv) {
-  const auto now = Clock::now();
-  std::vector<InFlight> still;
-  for (auto& f : sv->active) {
-    if (f.bidx < 0) {
-      still.push_back(std::move(f));
-      continue;
-    }
-    const StateOut& so = sv->h_out[f.bidx];
-    if (so.err == 0 && so.st.conv == kRunning) {
-      still.push_back(std::move(f));
-      continue;
-    }
-    f.sys->busy = false;
-    f.sys->has_mu = so.err == 0 && (so.st.conv == kConverged || so.st.conv == kZeroCurv);
-    f.sys->last_use = ++sv->use_clock;
-    publish(sv, f, so, now);
-  }
-  if (still.size() != sv->active.size()) {
-    sv->active.swap(still);
-    sv->cv_done.notify_all();
-  }
-}
-
 // iteration-batch membership: add systems with a solve in flight, drop
 // members idle for kIdlePasses passes
 void update_members(Serve* sv) {
@@ -602,67 +623,28 @@ void update_members(Serve* sv) {
   }
 }
 
-// One pass of the loop: the iteration chunk of the batch is enqueued first;
-// admission (host work, asynchronous coordinate uploads) and the set-up
-// batch of the admitted requests follow while the device runs it; then one
-// wait, and the finished solves retire.  Returns a fatal status only for
-// device failures.
+// One pass of the loop: admission (residency, coordinate uploads, lists of
+// new systems), one launch sequence over the member batch, one wait, then
+// the finished solves retire and solves whose coordinates outran their
+// lists get the lists rebuilt and their set-up repeated in the next pass.
+// Returns a fatal status only for device failures.
 thp_status pass(Serve* sv, std::vector<Request>& starting) {
   thp_ctx* ctx = sv->ctx;
   const auto t_pass = Clock::now();
-  // ---- PCG iterations of everything in flight
-  bool chunk = false;
-  if (!sv->active.empty()) {
-    update_members(sv);
-    std::vector<thp_sys*> mem;
-    for (ServeSystem* m : sv->members) mem.push_back(m->s);
-    if (!sv->iter_dirty) {
-      for (size_t q = 0; q < mem.size(); ++q)
-        if (q >= sv->iter_versions.size() || sv->iter_versions[q] != mem[q]->view_version) sv->iter_dirty = true;
-    }
-    if (sv->iter_dirty || sv->iter_generation != sv->pool_generation) {
-      SRET(build_batch(sv, sv->iter, mem, {}));
-      SRET(tune_pair(sv, sv->iter.b, mem));
-      sv->iter_dirty = false;
-      sv->iter_generation = sv->pool_generation;
-      sv->iter_versions.clear();
-      for (thp_sys* s : mem) sv->iter_versions.push_back(s->view_version);
-      sv->stats.batch_builds += 1;
-    }
-    const int iters = std::max(1, sv->o.iters_per_graph);
-    SRET(run_chunk(sv, iters));
-    sv->stats.chunks += 1;
-    sv->active_sum += (double)sv->active.size();
-    sv->stats.pair_segs = sv->iter.b.segs;
-    // energies of the solves that finished, states back in one copy
-    Batch& b = sv->iter.b;
-    SRET(b_finish(ctx, b, 1));
-    if (sv->out_cap < b.nsys) {
-      SHIP(ctx, hipStreamSynchronize(ctx->stream));
-      if (sv->d_out) (void)hipFree(sv->d_out);
-      if (sv->h_out) (void)hipHostFree(sv->h_out);
-      sv->out_cap = std::max(b.nsys, 2 * sv->out_cap);
-      SHIP(ctx, hipMalloc((void**)&sv->d_out, sizeof(StateOut) * sv->out_cap));
-      SHIP(ctx, hipHostMalloc((void**)&sv->h_out, sizeof(StateOut) * sv->out_cap, hipHostMallocDefault));
-    }
-    SRET(b_collect(ctx, b, sv->d_out));
-    SHIP(ctx, hipMemcpyAsync(sv->h_out, sv->d_out, sizeof(StateOut) * b.nsys, hipMemcpyDeviceToHost, ctx->stream));
-    for (auto& f : sv->active) f.bidx = f.sys->member;
-    chunk = true;
-  }
-  // ---- admission: residency, coordinates, lists (overlaps the chunk)
-  const auto t_adm = Clock::now();
-  Clock::time_point t_set;
-  std::vector<InFlight> admitted;
+  // ---- admission
   std::vector<ServeSystem*> pinned;
   for (auto& f : sv->active) pinned.push_back(f.sys);
+  for (auto& f : sv->restarts) pinned.push_back(f.sys);
   std::vector<Request> deferred;
+  std::vector<InFlight> admitted;
+  std::vector<std::pair<double*, size_t>> uploads;  // pinned buffers in flight this pass
   for (auto& q : starting) {
     ServeSystem* ss = sv->systems[q.sid].get();
     bool ok = true;
     const thp_status ra = admit(sv, ss, pinned, &ok);
     if (ra == THP_ERR_HIP) return ra;
     if (ra != THP_OK) {
+      give_pinned(sv, q.xyz, q.xyz_cap);
       publish_error(sv, q, ra, t_pass);
       continue;
     }
@@ -672,11 +654,18 @@ thp_status pass(Serve* sv, std::vector<Request>& starting) {
     }
     pinned.push_back(ss);
     ss->last_use = ++sv->use_clock;
-    thp_status rc = THP_OK;
     const auto tc0 = Clock::now();
-    if (q.has_xyz) rc = update_coords(ss->s, q.x.data(), q.y.data(), q.z.data(), ss->staging);
+    thp_status rc = THP_OK;
+    if (q.xyz) {
+      const size_t n = (size_t)ss->s->n;
+      if (hipMemcpyAsync((void*)ss->s->view.raw, q.xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream) !=
+          hipSuccess)
+        return fail(ctx, THP_ERR_HIP, "coordinate upload");
+      uploads.emplace_back(q.xyz, q.xyz_cap);
+      q.xyz = nullptr;
+    }
     const auto tc1 = Clock::now();
-    if (rc == THP_OK) rc = ensure_lists(ss->s);
+    rc = ensure_lists(ss->s);  // new systems only; others keep theirs until the skin check fails
     {
       std::lock_guard<std::mutex> lk(sv->mu);
       sv->stats.coords_ms += ms_between(tc0, tc1);
@@ -699,39 +688,106 @@ thp_status pass(Serve* sv, std::vector<Request>& starting) {
     std::lock_guard<std::mutex> lk(sv->mu);
     for (auto it = deferred.rbegin(); it != deferred.rend(); ++it) sv->queue.push_front(std::move(*it));
   }
-  // ---- set-up of the admitted solves as one batch
-  t_set = Clock::now();
-  if (!admitted.empty()) {
-    std::vector<thp_sys*> mem;
-    std::vector<int> warm;
-    for (auto& f : admitted) {
-      mem.push_back(f.sys->s);
-      warm.push_back(f.warm);
-    }
-    SRET(build_batch(sv, sv->setup, mem, warm));
-    Batch& b = sv->setup.b;
-    b.segs = ctx->pair_segs > 0 ? ctx->pair_segs : kDefaultSegs;
-    SRET(b_init_state(ctx, b, sv->o.poleps, sv->o.politer, sv->setup.warm));
-    SRET(b_pme_prep(ctx, b));
-    SRET(b_setup(ctx, b));
+  for (auto& f : sv->restarts) admitted.push_back(std::move(f));
+  sv->restarts.clear();
+  for (auto& f : admitted) sv->active.push_back(std::move(f));
+  const auto t_set = Clock::now();
+  if (sv->active.empty()) {
+    for (auto& u : uploads) give_pinned(sv, u.first, u.second);
+    return THP_OK;
   }
+  // ---- the member batch: layout, request flags
+  update_members(sv);
+  std::vector<thp_sys*> mem;
+  for (ServeSystem* m : sv->members) mem.push_back(m->s);
+  if (!sv->iter_dirty) {
+    for (size_t q = 0; q < mem.size(); ++q)
+      if (q >= sv->iter_versions.size() || sv->iter_versions[q] != mem[q]->view_version) sv->iter_dirty = true;
+  }
+  if (sv->out_cap < (int)mem.size()) {
+    SHIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (sv->d_out) (void)hipFree(sv->d_out);
+    if (sv->h_out) (void)hipHostFree(sv->h_out);
+    sv->out_cap = std::max((int)mem.size(), 2 * sv->out_cap);
+    SHIP(ctx, hipMalloc((void**)&sv->d_out, sizeof(StateOut) * sv->out_cap));
+    SHIP(ctx, hipHostMalloc((void**)&sv->h_out, sizeof(StateOut) * sv->out_cap, hipHostMallocDefault));
+    sv->iter_dirty = true;  // the graphs hold the old pointer
+  }
+  if (sv->iter_dirty || sv->iter_generation != sv->pool_generation) {
+    SRET(build_batch(sv, sv->iter, mem, {}));
+    SRET(tune_pair(sv, sv->iter.b, mem));
+    sv->iter_dirty = false;
+    sv->iter_generation = sv->pool_generation;
+    sv->iter_versions.clear();
+    for (thp_sys* s : mem) sv->iter_versions.push_back(s->view_version);
+    sv->stats.batch_builds += 1;
+  }
+  BatchBuf& bb = sv->iter;
+  const int m = (int)mem.size();
+  for (int q = 0; q < m; ++q) {
+    bb.h_req[q] = 0;
+    bb.h_warm[q] = 0;
+  }
+  for (auto& f : admitted) {
+    bb.h_req[f.sys->member] = 1;
+    bb.h_warm[f.sys->member] = f.warm;
+  }
+  const int variant = admitted.empty() ? 0 : 1;
+  if (variant) {
+    SHIP(ctx, hipMemcpyAsync(bb.req, bb.h_req, sizeof(int) * m, hipMemcpyHostToDevice, ctx->stream));
+    SHIP(ctx, hipMemcpyAsync(bb.warm, bb.h_warm, sizeof(int) * m, hipMemcpyHostToDevice, ctx->stream));
+  }
+  // ---- one launch sequence for the pass
+  const auto t_chunk = Clock::now();
+  const int iters = std::max(1, sv->o.iters_per_graph);
+  SRET(run_pass(sv, variant, iters));
+  sv->stats.chunks += 1;
+  sv->active_sum += (double)sv->active.size();
+  sv->stats.pair_segs = bb.b.segs;
+  SHIP(ctx, hipMemcpyAsync(sv->h_out, sv->d_out, sizeof(StateOut) * m, hipMemcpyDeviceToHost, ctx->stream));
   const auto t_wait = Clock::now();
   SHIP(ctx, hipStreamSynchronize(ctx->stream));
   const auto t_done = Clock::now();
+  for (auto& u : uploads) give_pinned(sv, u.first, u.second);
   {
     std::lock_guard<std::mutex> lk(sv->mu);
     sv->stats.host_ms += ms_between(t_pass, t_wait);
     sv->stats.wait_ms += ms_between(t_wait, t_done);
-    sv->stats.chunk_ms += ms_between(t_pass, t_adm);
-    sv->stats.admit_ms += ms_between(t_adm, t_set);
-    sv->stats.setup_ms += ms_between(t_set, t_wait);
+    sv->stats.admit_ms += ms_between(t_pass, t_set);
+    sv->stats.setup_ms += ms_between(t_set, t_chunk);
+    sv->stats.chunk_ms += ms_between(t_chunk, t_wait);
   }
-  if (chunk) retire(sv);
-  for (auto& f : admitted) {
-    f.sys->s->pme_valid = true;
-    f.sys->s->coef_valid = true;
-    f.bidx = -1;
-    sv->active.push_back(std::move(f));
+  // ---- retire finished solves; rebuild lists of stale ones
+  const auto now = Clock::now();
+  std::vector<InFlight> still;
+  for (auto& f : sv->active) {
+    const StateOut& so = sv->h_out[f.sys->member];
+    if (so.st.stale) {
+      // coordinates moved past the skin of the lists the set-up used
+      thp_status rc = sync_host_coords(f.sys->s);
+      if (rc == THP_OK) rc = ensure_lists(f.sys->s);
+      if (rc == THP_ERR_HIP) return rc;
+      if (rc != THP_OK) {
+        f.sys->busy = false;
+        publish_error(sv, f.req, rc, f.t_start);
+        continue;
+      }
+      f.restart = true;
+      sv->restarts.push_back(std::move(f));
+      continue;
+    }
+    if (so.err == 0 && so.st.conv == kRunning) {
+      still.push_back(std::move(f));
+      continue;
+    }
+    f.sys->busy = false;
+    f.sys->has_mu = so.err == 0 && (so.st.conv == kConverged || so.st.conv == kZeroCurv);
+    f.sys->last_use = ++sv->use_clock;
+    publish(sv, f, so, now);
+  }
+  if (still.size() != sv->active.size()) {
+    sv->active.swap(still);
+    sv->cv_done.notify_all();
   }
   return THP_OK;
 }
@@ -745,8 +801,8 @@ void worker_main(Serve* sv) {
     {
       std::unique_lock<std::mutex> lk(sv->mu);
       sv->cv_work.wait(lk, [&] {
-        return sv->stop || !sv->active.empty() || !sv->copies.empty() || !copies.empty() ||
-               (!sv->queue.empty() && (int)sv->active.size() < sv->o.max_systems);
+        return sv->stop || !sv->active.empty() || !sv->restarts.empty() || !sv->copies.empty() || !copies.empty() ||
+               (!sv->queue.empty() && (int)(sv->active.size() + sv->restarts.size()) < sv->o.max_systems);
       });
       if (sv->stop) break;
       while (!sv->copies.empty()) {
@@ -754,7 +810,7 @@ void worker_main(Serve* sv) {
         sv->copies.pop_front();
       }
       // FIFO admission into free slots; one solve per system at a time
-      int free_slots = sv->o.max_systems - (int)sv->active.size();
+      int free_slots = sv->o.max_systems - (int)(sv->active.size() + sv->restarts.size());
       std::vector<int> taken;
       for (auto it = sv->queue.begin(); it != sv->queue.end() && free_slots > 0;) {
         ServeSystem* ss = sv->systems[it->sid].get();
@@ -778,7 +834,7 @@ void worker_main(Serve* sv) {
       sv->stop = true;
       break;
     }
-    if (sv->active.empty() && starting.empty() && copies.empty()) {
+    if (sv->active.empty() && sv->restarts.empty() && starting.empty() && copies.empty()) {
       // nothing ran in this pass: wait for work instead of spinning
       std::unique_lock<std::mutex> lk(sv->mu);
       sv->cv_work.wait_for(lk, std::chrono::milliseconds(1));
@@ -786,13 +842,16 @@ void worker_main(Serve* sv) {
   }
   // cancel what did not finish
   const auto now = Clock::now();
-  for (auto& f : sv->active) {
-    f.sys->busy = false;
-    publish_error(sv, f.req, sv->fatal != THP_OK ? sv->fatal : THP_ERR_STATE, f.t_start);
+  for (auto* list : {&sv->active, &sv->restarts}) {
+    for (auto& f : *list) {
+      f.sys->busy = false;
+      publish_error(sv, f.req, sv->fatal != THP_OK ? sv->fatal : THP_ERR_STATE, f.t_start);
+    }
+    list->clear();
   }
-  sv->active.clear();
   std::lock_guard<std::mutex> lk(sv->mu);
   for (auto& q : sv->queue) {
+    if (q.xyz) sv->pinned_pool.emplace(q.xyz_cap, q.xyz);
     thp_result r{};
     r.ticket = q.ticket;
     r.system_id = q.sid;
@@ -894,15 +953,18 @@ thp_status thp_serve_submit(thp_ctx* ctx, int sid, const double* x, const double
     if (sid < 0 || sid >= (int)sv->systems.size()) return THP_ERR_ARG;
   }
   if (x) {
-    int n = 0;
+    // caller-order coordinates into a pinned buffer on the caller's thread;
+    // the worker uploads it as is
+    size_t n = 0;
     {
       std::lock_guard<std::mutex> lk(sv->mu);
-      n = sv->systems[sid]->desc.n;
+      n = (size_t)sv->systems[sid]->desc.n;
     }
-    q.has_xyz = true;
-    q.x.assign(x, x + n);
-    q.y.assign(y, y + n);
-    q.z.assign(z, z + n);
+    q.xyz = take_pinned(sv, 3 * n, &q.xyz_cap);
+    if (!q.xyz) return THP_ERR_NOMEM;
+    std::memcpy(q.xyz, x, sizeof(double) * n);
+    std::memcpy(q.xyz + n, y, sizeof(double) * n);
+    std::memcpy(q.xyz + 2 * n, z, sizeof(double) * n);
   }
   {
     std::lock_guard<std::mutex> lk(sv->mu);
@@ -986,8 +1048,10 @@ thp_status thp_serve_stop(thp_ctx* ctx) {
   for (auto& kv : sv->pools) free_pool(kv.second);
   if (sv->d_out) (void)hipFree(sv->d_out);
   if (sv->h_out) (void)hipHostFree(sv->h_out);
-  for (auto& s : sv->systems)
+  for (auto& s : sv->systems) {
     if (s->s) thp_system_destroy(s->s);
+  }
+  for (auto& kv : sv->pinned_pool) (void)hipHostFree(kv.second);
   const thp_status fatal = sv->fatal;
   if (fatal != THP_OK) ctx->err = sv->fatal_msg;
   ctx->serve = nullptr;
