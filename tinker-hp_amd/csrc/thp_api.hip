@@ -618,17 +618,23 @@ thp_status b_init_state(thp_ctx* ctx, const Batch& b, double poleps, int politer
   return launch_check(ctx, "k_init_state");
 }
 
-template <typename real>
-void launch_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int check_conv) {
+template <typename real, int NB>
+void launch_tmatxb_nb(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int gate) {
   hipStream_t st = ctx->stream;
   const dim3 g = g1(b.n_blk);
   switch (b.segs) {
-    case 1: hipLaunchKernelGGL((k_tmatxb_cl<real, 1>), g, dim3(64), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
-    case 2: hipLaunchKernelGGL((k_tmatxb_cl<real, 2>), g, dim3(128), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
-    case 8: hipLaunchKernelGGL((k_tmatxb_cl<real, 8>), g, dim3(512), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
-    case 16: hipLaunchKernelGGL((k_tmatxb_cl<real, 16>), g, dim3(1024), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
-    default: hipLaunchKernelGGL((k_tmatxb_cl<real, 4>), g, dim3(256), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
+    case 1: hipLaunchKernelGGL((k_tmatxb_cl<real, 1, NB>), g, dim3(64), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, gate); break;
+    case 2: hipLaunchKernelGGL((k_tmatxb_cl<real, 2, NB>), g, dim3(128), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, gate); break;
+    case 8: hipLaunchKernelGGL((k_tmatxb_cl<real, 8, NB>), g, dim3(512), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, gate); break;
+    case 16: hipLaunchKernelGGL((k_tmatxb_cl<real, 16, NB>), g, dim3(1024), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, gate); break;
+    default: hipLaunchKernelGGL((k_tmatxb_cl<real, 4, NB>), g, dim3(256), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, gate); break;
   }
+}
+
+template <typename real>
+void launch_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int gate) {
+  if (b.nb == 16) launch_tmatxb_nb<real, 16>(ctx, b, vin, vout, dodiag, gate);
+  else launch_tmatxb_nb<real, 8>(ctx, b, vin, vout, dodiag, gate);
 }
 
 // real-space matvec; needs the pair coefficients of the current
@@ -714,6 +720,24 @@ thp_status sync_host_coords(thp_sys* s) {
   return THP_OK;
 }
 
+// Work table in XCD order.  MI355X hands workgroup k to XCD k % 8 (each XCD
+// has its own L2): the list is cut into 8 contiguous ranges (spatially
+// compact runs of blocks or bricks of the same systems) and interleaved so
+// that range x runs on XCD x and its neighbouring data stay in that L2.
+// Padding entries (sys = -1) keep the ranges aligned; kernels skip them.
+std::vector<Work> xcd_order(const std::vector<Work>& list) {
+  constexpr size_t kXcd = 8;
+  const size_t n = list.size();
+  const size_t per = (n + kXcd - 1) / kXcd;
+  std::vector<Work> t(per * kXcd, Work{-1, 0});
+  for (size_t x = 0; x < kXcd; ++x)
+    for (size_t j = 0; j < per; ++j) {
+      const size_t src = x * per + j;
+      if (src < n) t[j * kXcd + x] = list[src];
+    }
+  return t;
+}
+
 void append_work(const thp_sys* s, int index, std::vector<Work>& blk, std::vector<Work>& red, std::vector<Work>& gat,
                  std::vector<Work>& brick, std::vector<Work>& conv) {
   for (int q = 0; q < s->nblk; ++q) blk.push_back({index, q});
@@ -751,6 +775,11 @@ thp_status thp_init(thp_ctx** out, int device) {
   if (const char* e = getenv("THP_SEGS")) {
     const int v = atoi(e);
     if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) ctx->pair_segs = v;
+  }
+  // THP_NB forces the matvec's list entries per prefetched batch
+  if (const char* e = getenv("THP_NB")) {
+    const int v = atoi(e);
+    if (v == 8 || v == 16) ctx->pair_nb = v;
   }
   *out = ctx;
   return THP_OK;
@@ -1045,7 +1074,8 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
     if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "sync"));
   }
 
-  // ---- view + work table
+  // ---- view + work tables (natural order for the list builds, XCD order
+  // for the path kernels)
   ALLOC(s->d_view, 1);
   std::vector<Work> wk(s->nblk);
   for (int b = 0; b < s->nblk; ++b) wk[b] = {0, b};
@@ -1053,7 +1083,20 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
   H2D(s->d_work, wk.data(), sizeof(Work) * s->nblk);
   ALLOC(s->d_redwork, v.red_blocks);
   H2D(s->d_redwork, wk.data(), sizeof(Work) * v.red_blocks);
-  s->d_gatwork = s->d_brickwork;  // gather workgroups are bricks
+  {
+    const std::vector<Work> wx = xcd_order(wk);
+    std::vector<Work> bn(s->nbricks);
+    for (int b = 0; b < s->nbricks; ++b) bn[b] = {0, b};
+    const std::vector<Work> bx = xcd_order(bn);
+    ALLOC(s->d_workx, wx.size());
+    ALLOC(s->d_brickx, bx.size());
+    H2D(s->d_workx, wx.data(), sizeof(Work) * wx.size());
+    H2D(s->d_brickx, bx.data(), sizeof(Work) * bx.size());
+    s->nworkx = (int)wx.size();
+    s->nbrickx = (int)bx.size();
+    if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "sync"));
+  }
+  s->d_gatwork = s->d_brickx;  // gather workgroups are bricks
   {
     const int nconv = (int)((npts + kConvChunk - 1) / kConvChunk);
     std::vector<Work> cw(nconv);
@@ -1075,15 +1118,16 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
     b.prec = s->prec;
     b.nsys = 1;
     b.segs = ctx->pair_segs > 0 ? ctx->pair_segs : kDefaultSegs;
+    b.nb = ctx->pair_nb > 0 ? ctx->pair_nb : 8;
     b.d_views = s->d_view;
-    b.d_blk = s->d_work;
-    b.n_blk = s->nblk;
+    b.d_blk = s->d_workx;
+    b.n_blk = s->nworkx;
     b.d_red = s->d_redwork;
     b.n_red = v.red_blocks;
     b.d_gat = s->d_gatwork;
-    b.n_gat = v.gat_blocks;
-    b.d_brick = s->d_brickwork;
-    b.n_brick = s->nbricks;
+    b.n_gat = s->nbrickx;
+    b.d_brick = s->d_brickx;
+    b.n_brick = s->nbrickx;
     b.d_conv = s->d_convwork;
     b.n_conv = s->nconv;
     b.d_ids = s->d_sysid;

@@ -151,6 +151,7 @@ struct Batch {
   int prec = THP_PREC_FP64;
   int nsys = 0;
   int segs = 4;                 // wavefronts per 64-atom block in the pair kernels
+  int nb = 8;                   // list entries per prefetched batch in the matvec (8 or 16)
   SysView* d_views = nullptr;   // [nsys]
   Work* d_blk = nullptr;        // 64-atom blocks
   int n_blk = 0;
@@ -175,6 +176,7 @@ struct thp_ctx {
   void* serve = nullptr;
   int n_cu = 0;
   int pair_segs = 0;           // 0: default; else forced waves per block (1..16)
+  int pair_nb = 0;             // 0: default; else forced matvec entries per batch (8, 16)
   std::set<thp_sys*> systems;  // live systems; thp_finalize releases what is left
 };
 
@@ -215,6 +217,9 @@ struct thp_sys {
   int* d_sysid = nullptr;          // {0, 1}
   int* d_one = nullptr;            // d_sysid + 1
   thp::Work* d_convwork = nullptr; // convolution work table
+  thp::Work* d_workx = nullptr;    // block table in XCD order (path kernels)
+  thp::Work* d_brickx = nullptr;   // brick table in XCD order
+  int nworkx = 0, nbrickx = 0;
   int nconv = 0;
   thp::Batch one;                  // this system as a batch of one
   // block neighbor lists
@@ -236,6 +241,9 @@ thp_status create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** out, 
 thp_status ensure_lists(thp_sys* s);
 thp_status update_coords(thp_sys* s, const double* x, const double* y, const double* z, double* pinned);
 size_t grid_points(const thp_sys* s);
+// work table reordered so that workgroup k's entry runs on XCD k % 8 next to
+// its spatial neighbours (padding entries have sys = -1)
+std::vector<Work> xcd_order(const std::vector<Work>& list);
 // per-system work tables with sys = `index` (appended)
 void append_work(const thp_sys* s, int index, std::vector<Work>& blk, std::vector<Work>& red,
                  std::vector<Work>& gat, std::vector<Work>& brick, std::vector<Work>& conv);

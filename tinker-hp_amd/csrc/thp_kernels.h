@@ -876,11 +876,11 @@ __device__ __forceinline__ void load_entries(const GLOBAL unsigned short* LL, co
   }
 }
 
-template <typename real, int S>
+template <typename real, int S, int NB>
 __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, const Work* work, int vin, int vout,
                                                        int dodiag, int gate) {
   constexpr int C = ClStage<real>::CHUNK;
-  constexpr int NB = 8;  // list entries per batch; the next batch is in flight meanwhile
+  // NB: list entries per batch; the next batch is in flight meanwhile
   // the staging arrays and the final cross-wavefront sum share one buffer
   constexpr int STAGE_BYTES = C * (2 * (int)sizeof(Quad<real>) + (int)sizeof(real));
   constexpr int RED_BYTES = S * 6 * 64 * (int)sizeof(double);

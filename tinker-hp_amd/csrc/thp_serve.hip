@@ -141,6 +141,7 @@ struct Serve {
   // finish-and-resubmit turnover.  Idle members cost only early exits.
   std::vector<ServeSystem*> members;
   bool iter_dirty = true;
+  int layout_age = 0;  // passes run with the current iteration-batch layout
   int iter_generation = -1;
   std::vector<int> iter_versions;  // view versions the iteration batch was built from
   StateOut* d_out = nullptr;
@@ -290,6 +291,9 @@ thp_status build_batch(Serve* sv, BatchBuf& bb, const std::vector<thp_sys*>& mem
   const int m = (int)members.size();
   std::vector<Work> blk, red, gat, brick, conv;
   for (int q = 0; q < m; ++q) append_work(members[q], q, blk, red, gat, brick, conv);
+  blk = xcd_order(blk);
+  brick = xcd_order(brick);
+  gat = xcd_order(gat);
   // grid slots per size
   std::map<std::tuple<int, int, int>, std::vector<int>> groups;
   for (int q = 0; q < m; ++q) groups[{members[q]->nfft[0], members[q]->nfft[1], members[q]->nfft[2]}].push_back(q);
@@ -445,43 +449,53 @@ thp_status tune_pair(Serve* sv, Batch& b, const std::vector<thp_sys*>& members) 
   thp_ctx* ctx = sv->ctx;
   int64_t atoms = 0;
   for (thp_sys* s : members) atoms += s->n;
-  const auto key = std::make_pair(bucket_log2(atoms), std::min(b.nsys, 64));
+  const auto key = std::make_pair(bucket_log2(atoms), bucket_log2(b.nsys));
   auto it = sv->tuned.find(key);
   if (it != sv->tuned.end()) {
-    b.segs = it->second;
+    b.segs = it->second / 100;
+    b.nb = it->second % 100;
     return THP_OK;
   }
-  if (ctx->pair_segs > 0 || !sv->o.autotune) {
+  if ((ctx->pair_segs > 0 && ctx->pair_nb > 0) || !sv->o.autotune) {
     b.segs = ctx->pair_segs > 0 ? ctx->pair_segs : kDefaultSegs;
-    sv->tuned[key] = b.segs;
+    b.nb = ctx->pair_nb > 0 ? ctx->pair_nb : 8;
+    sv->tuned[key] = b.segs * 100 + b.nb;
     return THP_OK;
   }
   // every candidate rewrites hreal from pp, which the next iteration does
   // again before anything reads hreal
-  static const int cand[] = {2, 4, 8, 16};
+  static const int cand_s[] = {2, 4, 8, 16};
+  static const int cand_nb[] = {8, 16};
   hipEvent_t e0, e1;
   SHIP(ctx, hipEventCreate(&e0));
   SHIP(ctx, hipEventCreate(&e1));
   float best = 1e30f;
-  int best_s = kDefaultSegs;
-  for (int c : cand) {
-    b.segs = c;
-    SRET(b_pair_tmatxb(ctx, b, 1, 3, 1, 1));  // warm
-    SHIP(ctx, hipEventRecord(e0, ctx->stream));
-    for (int r = 0; r < 3; ++r) SRET(b_pair_tmatxb(ctx, b, 1, 3, 1, 1));
-    SHIP(ctx, hipEventRecord(e1, ctx->stream));
-    SHIP(ctx, hipEventSynchronize(e1));
-    float t = 0;
-    SHIP(ctx, hipEventElapsedTime(&t, e0, e1));
-    if (t < best) {
-      best = t;
-      best_s = c;
+  int best_s = kDefaultSegs, best_nb = 8;
+  for (int cs : cand_s) {
+    if (ctx->pair_segs > 0 && cs != ctx->pair_segs) continue;
+    for (int cn : cand_nb) {
+      if (ctx->pair_nb > 0 && cn != ctx->pair_nb) continue;
+      b.segs = cs;
+      b.nb = cn;
+      SRET(b_pair_tmatxb(ctx, b, 1, 3, 1, kGateIter));  // warm
+      SHIP(ctx, hipEventRecord(e0, ctx->stream));
+      for (int r = 0; r < 3; ++r) SRET(b_pair_tmatxb(ctx, b, 1, 3, 1, kGateIter));
+      SHIP(ctx, hipEventRecord(e1, ctx->stream));
+      SHIP(ctx, hipEventSynchronize(e1));
+      float t = 0;
+      SHIP(ctx, hipEventElapsedTime(&t, e0, e1));
+      if (t < best) {
+        best = t;
+        best_s = cs;
+        best_nb = cn;
+      }
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   b.segs = best_s;
-  sv->tuned[key] = best_s;
+  b.nb = best_nb;
+  sv->tuned[key] = best_s * 100 + best_nb;
   return THP_OK;
 }
 
@@ -511,7 +525,9 @@ thp_status run_pass(Serve* sv, int variant, int iters) {
   thp_ctx* ctx = sv->ctx;
   BatchBuf& bb = sv->iter;
   if (bb.graph_iters != iters) drop_graph(bb);
-  if (sv->o.use_graphs && sv->graphs_ok) {
+  // capture a layout once it has survived a pass: with membership changing
+  // every pass (streams of many distinct systems) eager launches are cheaper
+  if (sv->o.use_graphs && sv->graphs_ok && sv->layout_age >= 1) {
     if (!bb.exec[variant]) {
       hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
       thp_status r = e == hipSuccess ? enqueue_pass(sv, variant, iters) : THP_ERR_HIP;
@@ -721,6 +737,9 @@ thp_status pass(Serve* sv, std::vector<Request>& starting) {
     sv->iter_versions.clear();
     for (thp_sys* s : mem) sv->iter_versions.push_back(s->view_version);
     sv->stats.batch_builds += 1;
+    sv->layout_age = 0;
+  } else {
+    ++sv->layout_age;
   }
   BatchBuf& bb = sv->iter;
   const int m = (int)mem.size();
