@@ -44,6 +44,44 @@ def time_leg(ctx, thp, system, precision, steps, warmup, poleps):
     return solver, dt / steps, last
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "matvec_traffic_water23559.md")
+
+
+def kernel_probe(thp):
+    """Launch the matvec, spreading and gather of the mixed 24k system 10
+    times each, then one device-to-device copy of 256 MiB whose counter
+    readings calibrate FETCH_SIZE and WRITE_SIZE."""
+    import ctypes as C
+    ctx = thp.Context(0)
+    sv = thp.Solver(ctx, thp.water_23559(), thp.THP_PREC_MIXED)
+    for k, name in ((0, "matvec"), (1, "spread"), (2, "gather")):
+        ms, pairs = sv.time_kernel(k, reps=10)
+        log(f"probe: {name} {1e3 * ms:.1f} us per launch")
+    hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    nbytes = 256 << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) or hip.hipDeviceSynchronize():
+        raise RuntimeError("device copy failed")
+    log(f"probe: copied {nbytes} bytes device to device")
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    sv.close()
+    ctx.close()
+
+
+def measured_traffic():
+    """HBM bytes per matvec launch from the committed PMC summary, or None."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(system, poleps, seconds=10.0):
     """FP64 CPU oracle on the host cores: repeated full solves (field, guess,
     PCG to poleps, energy) of the same system for about `seconds`."""
@@ -73,6 +111,8 @@ def main():
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--stream-requests", type=int, default=2000)
     ap.add_argument("--copies", type=int, default=16, help="resident 24k systems in the serving leg")
+    ap.add_argument("--kernel-probe", action="store_true",
+                    help="only launch the path kernels and a copy of known size (for rocprofv3 --pmc runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -81,6 +121,10 @@ def main():
 
     import numpy as np
     import thp
+
+    if args.kernel_probe:
+        kernel_probe(thp)
+        return
 
     system = thp.water_23559()
     log(f"bench: workload {system.name} n={system.n} box={system.box[0]} nfft={system.nfft}")
@@ -123,7 +167,7 @@ def main():
     flops = FLOP_PER_PAIR_TMATXB * pairs
     achieved = flops / (ms * 1e-3) / 1e12
     roofline = dict(bound="mfma", achieved=round(achieved, 4), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=round(achieved / FP32_PEAK_TFLOPS, 5), traffic=None,
+                    frac=round(achieved / FP32_PEAK_TFLOPS, 5), traffic=measured_traffic(),
                     kernel="k_tmatxb_cl<float,S> (real-space matvec)", ms_per_launch=round(ms, 5),
                     pairs_in_cut=int(pairs), flop_per_pair=FLOP_PER_PAIR_TMATXB,
                     compute="FP32 VALU (no MFMA: no dense contraction)")
