@@ -276,13 +276,8 @@ void launch_pair(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, in
 // direct permanent field + the pair coefficients and compacted lists that
 // the matvec of the same coordinates uses
 thp_status pair_efld0(thp_ctx* ctx, const Batch& b, int gate) {
-  if (b.prec == THP_PREC_FP64) {
-    launch_pair<double, 0>(ctx, b, 0, 0, 0, gate);
-    hipLaunchKernelGGL(k_compact_lists<double>, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, gate);
-  } else {
-    launch_pair<float, 0>(ctx, b, 0, 0, 0, gate);
-    hipLaunchKernelGGL(k_compact_lists<float>, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, gate);
-  }
+  if (b.prec == THP_PREC_FP64) launch_pair<double, 0>(ctx, b, 0, 0, 0, gate);
+  else launch_pair<float, 0>(ctx, b, 0, 0, 0, gate);
   hipLaunchKernelGGL(k_scaled<0>, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, 0, 0, gate);
   return launch_check(ctx, "k_pair_union<efld0>");
 }
@@ -642,7 +637,6 @@ void launch_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, 
 thp_status b_pair_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int check_conv) {
   if (b.prec == THP_PREC_FP64) launch_tmatxb<double>(ctx, b, vin, vout, dodiag, check_conv);
   else launch_tmatxb<float>(ctx, b, vin, vout, dodiag, check_conv);
-  hipLaunchKernelGGL(k_scaled<1>, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, vin, vout, check_conv);
   return launch_check(ctx, "k_tmatxb_cl");
 }
 
@@ -959,6 +953,11 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
   }
   v.x = dx; v.y = dy; v.z = dz; v.rpole = drp; v.pol = dpol; v.pdamp = dpd; v.thole = dth;
   v.scale_tab = dtab; v.excl_off = deoff; v.excl_k = dek; v.excl_code = dec;
+  {
+    double* sco;
+    ALLOC(sco, 2 * std::max<size_t>(1, ek.size()));
+    v.scoef = sco;
+  }
   ALLOC(s->iperm_dev, n);
   H2D(s->iperm_dev, s->iperm.data(), sizeof(int) * n);
   {

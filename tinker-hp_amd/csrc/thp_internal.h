@@ -81,6 +81,7 @@ struct SysView {
   const int* excl_off;      // [n+1]
   const int* excl_k;
   const int* excl_code;
+  double* scoef;            // [nexcl][2] per-solve (dmp3, dmp5) of the scaled pairs, matvec u-scale
   // Verlet list (cut + buffer), full, sorted k per atom: nb[s*npad + i]
   int maxnb;
   int* nb;

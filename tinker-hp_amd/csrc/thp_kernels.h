@@ -362,7 +362,8 @@ __device__ __forceinline__ void thole_l(real r, real pdi, real pdk, real thi, re
 // efld0_direct, newinduce_pme.f:1116-1264), added to acc[0..5].
 template <typename real>
 __device__ __forceinline__ void efld0_pair(const SysView& sv, int k, real xr, real yr, real zr, real pdi,
-                                           real thi, real dsc, real psc, double* acc) {
+                                           real thi, real dsc, real psc, double* acc, real usc = real(1),
+                                           real* ucoef = nullptr) {
   const size_t np = sv.npad;
   const real r2 = xr * xr + yr * yr + zr * zr;
   const real rr1 = rsqrt_t(r2);
@@ -385,6 +386,10 @@ __device__ __forceinline__ void efld0_pair(const SysView& sv, int k, real xr, re
   ewald_bn<real, 3>(r, r2, real(sv.aewald), rr1, bn);
   real l[3];
   thole_l<real, 1>(r, pdi, real(sv.pdamp[k]), thi, real(sv.thole[k]), l);
+  if (ucoef) {  // the matvec's coefficients of this pair (u-scale)
+    ucoef[0] = bn[0] - (real(1) - usc * l[0]) * rr3;
+    ucoef[1] = bn[1] - (real(1) - usc * l[1]) * rr5;
+  }
 #pragma unroll
   for (int pass = 0; pass < 2; ++pass) {
     const real sc = pass == 0 ? dsc : psc;
@@ -443,9 +448,12 @@ __device__ __forceinline__ void scaled_pairs(const SysView& sv, const double* in
     const double xr = image_exact(sv.x[k] - xi, sv.box[0]);
     const double yr = image_exact(sv.y[k] - yi, sv.box[1]);
     const double zr = image_exact(sv.z[k] - zi, sv.box[2]);
-    if (!(r2_exact(xr, yr, zr) <= sv.cut2)) continue;
+    if (!(r2_exact(xr, yr, zr) <= sv.cut2)) {
+      if (KIND == 0) sv.scoef[2 * e] = sv.scoef[2 * e + 1] = 0.0;
+      continue;
+    }
     const double* sc = sv.scale_tab + sv.excl_code[e] * 3;
-    if (KIND == 0) efld0_pair<double>(sv, k, xr, yr, zr, pdi, thi, sc[0], sc[1], acc);
+    if (KIND == 0) efld0_pair<double>(sv, k, xr, yr, zr, pdi, thi, sc[0], sc[1], acc, sc[2], sv.scoef + 2 * e);
     else tmatxb_pair<double>(sv, in, k, xr, yr, zr, pdi, thi, sc[2], acc);
   }
 }
@@ -683,8 +691,10 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
   const double xi = ok ? X[i] : 0.0, yi = ok ? Y[i] : 0.0, zi = ok ? Z[i] : 0.0;
   const real pdi = ok ? real(PD[i]) : real(0), thi = ok ? real(TH[i]) : real(0);
   const int cnt = ok ? gp(sv.lcount)[i] : 0;
+  const int s_begin = (cnt * wv) / S;
   const int s_end = (cnt * (wv + 1)) / S;
-  int s = (cnt * wv) / S;
+  int s = s_begin;
+  int kept = 0;  // entries of this wavefront's segment within the cutoff
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (int c0 = 0; c0 < ucount; c0 += C) {
     const int cn = min(C, ucount - c0);
@@ -722,6 +732,7 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
           if constexpr (KIND == 0) PC[slot] = coef2<real>(real(0), real(0));
           continue;
         }
+        ++kept;
         const real xr = real(dx), yr = real(dy), zr = real(dz);
         const real r2 = xr * xr + yr * yr + zr * zr;
         const real rr1 = rsqrt_t(r2);
@@ -773,10 +784,47 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
       if (done) break;
     }
   }
-  if (KIND == 0) {  // unscaled pairs give identical d and p fields
+  if constexpr (KIND == 0) {
+    // unscaled pairs give identical d and p fields
     acc[3] = acc[0];
     acc[4] = acc[1];
     acc[5] = acc[2];
+    // compacted list of the in-cutoff entries for this solve's matvecs:
+    // per-lane offsets of the S segments from their counts, then each
+    // wavefront copies its own segment's kept entries, in order
+    __shared__ int s_kept[S][64];
+    s_kept[wv][lane] = kept;
+    __syncthreads();
+    int t = 0;
+    for (int q = 0; q < wv; ++q) t += s_kept[q][lane];
+    if (wv == 0 && ok) {
+      int tot = 0;
+      for (int q = 0; q < S; ++q) tot += s_kept[q][lane];
+      gw(sv.tcount)[i] = tot;
+    }
+    GLOBAL unsigned short* TL = gw(sv.tlist);
+    GLOBAL Coef2<real>* TC = gw(static_cast<Coef2<real>*>(sv.tcoef));
+    for (int s0 = s_begin; s0 < s_end; s0 += NB) {
+      unsigned short l[NB];
+      Coef2<real> c[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        l[b] = 0;
+        c[b] = coef2<real>(real(0), real(0));
+        if (s0 + b < s_end) {
+          l[b] = LL[(size_t)(s0 + b) * np + i];
+          c[b] = PC[(size_t)(s0 + b) * np + i];
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        if (c[b].x != real(0) || c[b].y != real(0)) {
+          TL[(size_t)t * np + i] = l[b];
+          TC[(size_t)t * np + i] = c[b];
+          ++t;
+        }
+      }
+    }
   }
   // fixed-order sum over the S wavefronts
   if (S > 1) {
@@ -796,50 +844,6 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
       OUT[(size_t)c * np + i] = v;
     }
   }
-}
-
-// Per-solve list compaction after the permanent-field pass: each atom's
-// entries with non-zero coefficients (those within the cutoff) are copied,
-// in order, to the front of the compacted list and coefficient arrays.
-// Loads are issued 16 entries at a time ahead of the stores.
-template <typename real>
-__global__ void __launch_bounds__(64) k_compact_lists(const SysView* views, const Work* work, int gate) {
-  constexpr int NB = 16;
-  const Work w = work[blockIdx.x];
-  if (w.sys < 0) return;
-  const SysView& sv = views[w.sys];
-  if (gated_out(sv.st, gate)) return;
-  const int i = w.blk * kBlock + threadIdx.x;
-  if (i >= sv.n) return;
-  const size_t np = sv.npad;
-  const GLOBAL unsigned short* LL = gp(sv.llist);
-  const GLOBAL Coef2<real>* PC = gp(static_cast<const Coef2<real>*>(sv.pcoef));
-  GLOBAL unsigned short* TL = gw(sv.tlist);
-  GLOBAL Coef2<real>* TC = gw(static_cast<Coef2<real>*>(sv.tcoef));
-  const int cnt = gp(sv.lcount)[i];
-  int t = 0;
-  for (int s0 = 0; s0 < cnt; s0 += NB) {
-    unsigned short l[NB];
-    Coef2<real> c[NB];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      l[b] = 0;
-      c[b] = coef2<real>(real(0), real(0));
-      if (s0 + b < cnt) {
-        l[b] = LL[(size_t)(s0 + b) * np + i];
-        c[b] = PC[(size_t)(s0 + b) * np + i];
-      }
-    }
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      if (c[b].x != real(0) || c[b].y != real(0)) {
-        TL[(size_t)t * np + i] = l[b];
-        TC[(size_t)t * np + i] = c[b];
-        ++t;
-      }
-    }
-  }
-  gw(sv.tcount)[i] = t;
 }
 
 // Real-space dipole field over the block-union lists with the per-solve
@@ -984,6 +988,27 @@ __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, cons
       for (int b = 0; b < NB; ++b) {
         loc[b] = nloc[b];
         cf[b] = ncf[b];
+      }
+    }
+  }
+  // scaled pairs with their per-solve FP64 coefficients (tmatxb u-scale)
+  if (ok && wv == S - 1) {
+    const int e0 = sv.excl_off[i], e1 = sv.excl_off[i + 1];
+    for (int e = e0; e < e1; ++e) {
+      const double d3 = sv.scoef[2 * e], d5 = sv.scoef[2 * e + 1];
+      if (d3 == 0.0 && d5 == 0.0) continue;  // beyond the cutoff
+      const int kx = sv.excl_k[e];
+      const double xr = image_exact(X[kx] - xi, bx);
+      const double yr = image_exact(Y[kx] - yi, by);
+      const double zr = image_exact(Z[kx] - zi, bz);
+#pragma unroll
+      for (int rhs = 0; rhs < 2; ++rhs) {
+        const double ux = IN[(size_t)(rhs * 3 + 0) * np + kx], uy = IN[(size_t)(rhs * 3 + 1) * np + kx],
+                     uz = IN[(size_t)(rhs * 3 + 2) * np + kx];
+        const double ur = d5 * (xr * ux + yr * uy + zr * uz);
+        acc[rhs * 3 + 0] += d3 * ux - ur * xr;
+        acc[rhs * 3 + 1] += d3 * uy - ur * yr;
+        acc[rhs * 3 + 2] += d3 * uz - ur * zr;
       }
     }
   }
