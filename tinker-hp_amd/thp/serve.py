@@ -84,14 +84,17 @@ def _pct(v, q):
 
 def bench_serving(ctx, system: System, steps: int = 20, warmup: int = 2, poleps: float = 1e-8, copies: int = 8,
                   jitter: float = 0.01, warm_start: int = 1, iters_per_graph: int = 4, use_graphs: int = 1,
-                  variants: int = 4, seed: int = 11) -> dict:
+                  variants: int = 4, depth: int = 2, seed: int = 11) -> dict:
     """Closed loop over `copies` resident systems with the geometry of
-    `system` (each registered as its own system): `copies` requests stay
-    outstanding and each result triggers the next request of that system.
-    Requests cycle through `variants` coordinate sets per system, each the
-    reference geometry perturbed by N(0, jitter), generated before the timed
-    region.  `warmup` rounds per system are not timed.  Returns solves/s over
-    the timed requests and latency percentiles."""
+    `system` (each registered as its own system): every system keeps `depth`
+    requests outstanding (they run one after the other, each warm-started
+    from the previous one) and each result triggers the next request of that
+    system.  Requests cycle through `variants` coordinate sets per system,
+    each the reference geometry perturbed by N(0, jitter), generated before
+    the timed region.  `warmup` rounds per system are not timed.  Returns
+    solves/s over the timed requests and latency percentiles (submit ->
+    result, so with depth 2 a request's latency includes the solve ahead of
+    it)."""
     rng = np.random.default_rng(seed)
     srv = Server(ctx, max_systems=copies, iters_per_graph=iters_per_graph, poleps=poleps,
                  warm_start=warm_start, use_graphs=use_graphs)
@@ -118,9 +121,11 @@ def bench_serving(ctx, system: System, steps: int = 20, warmup: int = 2, poleps:
         owner = {}
         t0 = time.perf_counter()
         issued = 0
-        for k in range(copies):
-            owner[submit(k)] = k
-            issued += 1
+        for _ in range(max(1, depth)):
+            for k in range(copies):
+                if issued < total:
+                    owner[submit(k)] = k
+                    issued += 1
         while owner:
             r = srv.poll(0)  # any finished request
             k = owner.pop(r.ticket)
@@ -137,7 +142,7 @@ def bench_serving(ctx, system: System, steps: int = 20, warmup: int = 2, poleps:
         return dict(solves_per_s=total / dt, ms_per_solve=1e3 * dt / total, solves=total, copies=copies,
                     niter=float(np.mean(nit)), ep=float(eps[0]), p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99),
                     mean_active=st.mean_active, pair_segs=st.pair_segs, warm_start=warm_start,
-                    iters_per_graph=iters_per_graph, graphs=int(st.graph_launches > 0),
+                    iters_per_graph=iters_per_graph, graphs=int(st.graph_launches > 0), depth=depth,
                     batch_builds=st.batch_builds, worker_host_ms=round(st.host_ms, 1),
                     worker_wait_ms=round(st.wait_ms, 1), worker_chunk_ms=round(st.chunk_ms, 1),
                     worker_admit_ms=round(st.admit_ms, 1), worker_setup_ms=round(st.setup_ms, 1),
