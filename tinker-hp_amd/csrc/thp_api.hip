@@ -439,6 +439,7 @@ thp_status build_union(thp_sys* s) {
       HIPCHK(s->ctx, hipMemcpy(lc.data(), s->view.lcount, sizeof(int) * s->npad, hipMemcpyDeviceToHost));
       int cmax = 0;
       for (int q = 0; q < s->n; ++q) cmax = std::max(cmax, lc[q]);
+      static_assert(32 % kPack == 0, "list storage holds whole lane packs");
       cmax = std::max(32, (cmax + 31) / 32 * 32);
       if (cmax > s->view.cmax) {
         dfree(s, s->view.pcoef);

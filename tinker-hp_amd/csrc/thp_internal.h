@@ -14,6 +14,7 @@
 namespace thp {
 
 constexpr int kBlock = 64;  // atoms per block = one wavefront
+constexpr int kPack = 8;    // compacted list entries per lane pack (16 B of indices)
 constexpr int kOrder = 5;   // PME B-spline order handled by the kernels
 constexpr double kCoulomb = 332.063714;  // v1.2/source/MOD_units.f:81
 constexpr double kTinyPol = 1.0e-5;      // v1.2/source/MOD_polar.f:36
@@ -99,9 +100,11 @@ struct SysView {
   void* pcoef;
   int cmax;
   // per-solve compacted lists: the in-cutoff entries of llist, in order,
-  // with their coefficients
-  unsigned short* tlist;   // [cmax][npad]
-  void* tcoef;             // [cmax][npad] coefficient pairs of tlist
+  // with their coefficients, in lane packs of kPack entries: entry e of
+  // atom i sits at ((e / kPack) * npad + i) * kPack + e % kPack, and the
+  // last pack of an atom is padded with index 0 and zero coefficients
+  unsigned short* tlist;   // [cmax / kPack][npad][kPack]
+  void* tcoef;             // [cmax / kPack][npad][kPack] coefficient pairs of tlist
   int* tcount;             // [npad]
   // PME
   int brick[3];       // spreading brick edge per axis (4..8 points)
@@ -135,7 +138,7 @@ struct Work {  // one unit of a launch: a block (brick, chunk ...) of one system
 constexpr int kConvChunk = 2048;
 constexpr int kTile = 12;                      // spreading tile edge: brick (<= 8) + order - 1
 constexpr int kTile3 = kTile * kTile * kTile;  // points per tile
-constexpr int kDefaultSegs = 8;   // pair-kernel wavefronts per block unless tuned  // grid points per block of the convolution
+constexpr int kDefaultSegs = 4;   // pair-kernel wavefronts per block unless tuned
 
 // One rocFFT execution: forward and backward plans over `count` contiguous
 // grids starting at `base` (count 1 for a single system).

@@ -819,10 +819,19 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
         if (c[b].x != real(0) || c[b].y != real(0)) {
-          TL[(size_t)t * np + i] = l[b];
-          TC[(size_t)t * np + i] = c[b];
+          const size_t at = ((size_t)(t / kPack) * np + i) * kPack + t % kPack;
+          TL[at] = l[b];
+          TC[at] = c[b];
           ++t;
         }
+      }
+    }
+    // the last wavefront ends at the lane's total: pad its last pack
+    if (wv == S - 1 && ok) {
+      for (; t % kPack; ++t) {
+        const size_t at = ((size_t)(t / kPack) * np + i) * kPack + t % kPack;
+        TL[at] = 0;
+        TC[at] = coef2<real>(real(0), real(0));
       }
     }
   }
@@ -850,10 +859,15 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
 // coefficients of the permanent-field pass (tmatxb_pme,
 // newinduce_pme.f:1548-1608: efi -= dlocal . u_k with dlocal = -dmp3 I +
 // dmp5 r r^T).  Union atoms are staged in LDS as positions relative to the
-// block centre plus their dipoles; an entry of the per-solve compacted list
-// costs one LDS gather, its coefficient pair (coalesced across the
-// wavefront) and ~25 flops.  Blocks whose union could
+// block centre plus their dipoles.  Each lane walks its atom's compacted
+// list one lane pack at a time (kPack entries: a 16-byte index load and
+// kPack coefficient pairs, contiguous per lane and coalesced across the
+// wavefront) with the next packs already in flight; an entry costs one LDS
+// gather and ~25 flops.  A pack's contributions are summed in the pair type
+// and each pack sum is added to the FP64 field.  Blocks whose union could
 // reach across half the box take the minimum image of each displacement.
+// Unions larger than one LDS chunk are walked chunk by chunk, with the
+// entries outside the staged chunk masked.
 template <typename real>
 struct __attribute__((aligned(4 * sizeof(real)))) Quad {
   real a, b, c, d;
@@ -863,28 +877,134 @@ template <typename real>
 struct ClStage {
   // per union atom: (x, y, z, ud_x), (ud_y, ud_z, up_x, up_y), up_z
   static constexpr int BYTES = 9 * (int)sizeof(real);
-  static constexpr int CHUNK = (48 * 1024) / BYTES;  // atoms per LDS chunk
+  static constexpr int CHUNK = (sizeof(real) == 4 ? 48 * 1024 : 64 * 1024) / BYTES;  // atoms per LDS chunk
 };
 
-template <typename real, int NB>
-__device__ __forceinline__ void load_entries(const GLOBAL unsigned short* LL, const GLOBAL Coef2<real>* PC, size_t np,
-                                             int i, int s, int s_end, int* loc, Coef2<real>* cf) {
+// one lane pack of the compacted list
+template <typename real>
+struct LanePack {
+  unsigned int ix[kPack / 2];  // two 16-bit union positions per word, low half first
+  Coef2<real> cf[kPack];
+};
+
+template <typename real>
+__device__ __forceinline__ void load_pack(const GLOBAL unsigned short* TL, const GLOBAL Coef2<real>* TC, size_t np,
+                                          int i, int b, LanePack<real>& p) {
+  typedef unsigned int U4 __attribute__((ext_vector_type(4)));
+  typedef real R4 __attribute__((ext_vector_type(4)));
+  static_assert(kPack == 8, "one 16-byte index load per pack");
+  const size_t at = (size_t)b * np + i;
+  const U4 v = reinterpret_cast<const GLOBAL U4*>(TL)[at];
+  p.ix[0] = v.x;
+  p.ix[1] = v.y;
+  p.ix[2] = v.z;
+  p.ix[3] = v.w;
+  const GLOBAL R4* c = reinterpret_cast<const GLOBAL R4*>(TC + at * kPack);
 #pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    loc[b] = 0x7fffffff;
-    cf[b] = coef2<real>(real(0), real(0));
-    if (s + b < s_end) {
-      loc[b] = int(LL[(size_t)(s + b) * np + i]);
-      cf[b] = PC[(size_t)(s + b) * np + i];
-    }
+  for (int k = 0; k < kPack / 2; ++k) {
+    const R4 u = c[k];
+    p.cf[2 * k] = coef2<real>(u.x, u.y);
+    p.cf[2 * k + 1] = coef2<real>(u.z, u.w);
   }
+}
+
+// geometry of the lane's atom and of the staged union
+template <typename real>
+struct ClGeom {
+  const Quad<real>* s_a;
+  const Quad<real>* s_b;
+  const real* s_c;
+  real pix, piy, piz;  // the lane's atom relative to the block centre
+  real lx, ly, lz, ilx, ily, ilz;
+};
+
+template <typename real, bool MASK, bool WRAP>
+__device__ __forceinline__ void apply_pack(const LanePack<real>& p, int c0, int cn, const ClGeom<real>& g,
+                                           double* acc) {
+  real f0 = real(0), f1 = real(0), f2 = real(0), f3 = real(0), f4 = real(0), f5 = real(0);
+#pragma unroll
+  for (int e = 0; e < kPack; ++e) {
+    const unsigned int word = p.ix[e >> 1];
+    int q = int((e & 1) ? (word >> 16) : (word & 0xffffu));
+    real d3 = p.cf[e].x, d5 = p.cf[e].y;
+    if (MASK) {
+      q -= c0;
+      const bool in = (unsigned int)q < (unsigned int)cn;
+      q = in ? q : 0;
+      d3 = in ? d3 : real(0);
+      d5 = in ? d5 : real(0);
+    }
+    const Quad<real> A = g.s_a[q];
+    const Quad<real> B = g.s_b[q];
+    const real upz = g.s_c[q];
+    real dx = A.a - g.pix, dy = A.b - g.piy, dz = A.c - g.piz;
+    if (WRAP) {
+      dx -= g.lx * rint(dx * g.ilx);
+      dy -= g.ly * rint(dy * g.ily);
+      dz -= g.lz * rint(dz * g.ilz);
+    }
+    const real urd = d5 * (dx * A.d + dy * B.a + dz * B.b);
+    const real urp = d5 * (dx * B.c + dy * B.d + dz * upz);
+    f0 = (f0 + d3 * A.d) - urd * dx;
+    f1 = (f1 + d3 * B.a) - urd * dy;
+    f2 = (f2 + d3 * B.b) - urd * dz;
+    f3 = (f3 + d3 * B.c) - urp * dx;
+    f4 = (f4 + d3 * B.d) - urp * dy;
+    f5 = (f5 + d3 * upz) - urp * dz;
+  }
+  acc[0] += double(f0);
+  acc[1] += double(f1);
+  acc[2] += double(f2);
+  acc[3] += double(f3);
+  acc[4] += double(f4);
+  acc[5] += double(f5);
+}
+
+// whole union staged: packs [b0, b1) of the lane, NP per step, the next
+// NP in flight (loads past the segment re-read its last pack)
+template <typename real, int NP, bool WRAP>
+__device__ __forceinline__ void walk_packs(const GLOBAL unsigned short* TL, const GLOBAL Coef2<real>* TC, size_t np,
+                                           int i, int b0, int b1, const ClGeom<real>& g, double* acc) {
+  if (b0 >= b1) return;
+  LanePack<real> cur[NP], nxt[NP];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) load_pack<real>(TL, TC, np, i, min(b0 + k, b1 - 1), cur[k]);
+  for (int b = b0; b < b1; b += NP) {
+#pragma unroll
+    for (int k = 0; k < NP; ++k) load_pack<real>(TL, TC, np, i, min(b + NP + k, b1 - 1), nxt[k]);
+#pragma unroll
+    for (int k = 0; k < NP; ++k)
+      if (k == 0 || b + k < b1) apply_pack<real, false, WRAP>(cur[k], 0, 0, g, acc);
+#pragma unroll
+    for (int k = 0; k < NP; ++k) cur[k] = nxt[k];
+  }
+}
+
+// one chunk [c0, c0 + cn) of a union larger than the LDS stage.  Lists are
+// sorted by union position, so the walk stops at the first pack that starts
+// beyond the chunk; the last pack walked may straddle into the next chunk
+// and is walked again there (masked).
+template <typename real, bool WRAP>
+__device__ __forceinline__ void walk_chunk(const GLOBAL unsigned short* TL, const GLOBAL Coef2<real>* TC, size_t np,
+                                           int i, int b1, int c0, int cn, int& resume, const ClGeom<real>& g,
+                                           double* acc) {
+  int last = -1;
+  for (int b = resume; b < b1; ++b) {
+    LanePack<real> p;
+    load_pack<real>(TL, TC, np, i, b, p);
+    if (int(p.ix[0] & 0xffffu) >= c0 + cn) break;
+    apply_pack<real, true, WRAP>(p, c0, cn, g, acc);
+    last = b;
+  }
+  if (last >= 0) resume = last;
 }
 
 template <typename real, int S, int NB>
 __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, const Work* work, int vin, int vout,
                                                        int dodiag, int gate) {
   constexpr int C = ClStage<real>::CHUNK;
-  // NB: list entries per batch; the next batch is in flight meanwhile
+  static_assert(NB % kPack == 0, "whole lane packs per step");
+  constexpr int NP = NB / kPack;
   // the staging arrays and the final cross-wavefront sum share one buffer
   constexpr int STAGE_BYTES = C * (2 * (int)sizeof(Quad<real>) + (int)sizeof(real));
   constexpr int RED_BYTES = S * 6 * 64 * (int)sizeof(double);
@@ -900,13 +1020,14 @@ __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, cons
   const int n = sv.n;
   const size_t np = sv.npad;
   const double bx = sv.box[0], by = sv.box[1], bz = sv.box[2];
+  const double ibx = 1.0 / bx, iby = 1.0 / by, ibz = 1.0 / bz;
   const GLOBAL double* X = gp(sv.x);
   const GLOBAL double* Y = gp(sv.y);
   const GLOBAL double* Z = gp(sv.z);
   const double* in_g = vsel(sv, vin);
   const GLOBAL double* IN = gp(in_g);
-  const GLOBAL unsigned short* LL = gp(sv.tlist);
-  const GLOBAL Coef2<real>* PC = gp(static_cast<const Coef2<real>*>(sv.tcoef));
+  const GLOBAL unsigned short* TL = gp(sv.tlist);
+  const GLOBAL Coef2<real>* TC = gp(static_cast<const Coef2<real>*>(sv.tcoef));
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int i = w.blk * kBlock + lane;
   const bool ok = i < n;
@@ -915,25 +1036,35 @@ __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, cons
   const GLOBAL double* BB = gp(sv.bbox) + (size_t)w.blk * 6;
   const double cx = BB[0], cy = BB[1], cz = BB[2];
   // union atoms lie within cut + skin of a block atom; the block's atoms
-  // moved at most skin/2 since its box was taken
+  // moved at most skin/2 since its box was taken.  Away from the wrap, the
+  // image nearest the block centre is the pair's image, and rounding
+  // d / L through the reciprocal cannot change it.
   const double lim = sv.cut + 2.0 * sv.skin;
   const bool wrap = !(BB[3] + lim < 0.5 * bx && BB[4] + lim < 0.5 * by && BB[5] + lim < 0.5 * bz);
-  const real lx = real(bx), ly = real(by), lz = real(bz);
-  const real ilx = real(1.0 / bx), ily = real(1.0 / by), ilz = real(1.0 / bz);
+  ClGeom<real> g;
+  g.s_a = s_a;
+  g.s_b = s_b;
+  g.s_c = s_c;
+  g.lx = real(bx);
+  g.ly = real(by);
+  g.lz = real(bz);
+  g.ilx = real(ibx);
+  g.ily = real(iby);
+  g.ilz = real(ibz);
   const double xi = ok ? X[i] : 0.0, yi = ok ? Y[i] : 0.0, zi = ok ? Z[i] : 0.0;
-  real pix, piy, piz;
   {
     double d = xi - cx, e = yi - cy, f = zi - cz;
-    d -= bx * rint(d / bx);
-    e -= by * rint(e / by);
-    f -= bz * rint(f / bz);
-    pix = real(d);
-    piy = real(e);
-    piz = real(f);
+    d -= bx * rint(d * ibx);
+    e -= by * rint(e * iby);
+    f -= bz * rint(f * ibz);
+    g.pix = real(d);
+    g.piy = real(e);
+    g.piz = real(f);
   }
   const int cnt = ok ? gp(sv.tcount)[i] : 0;
-  const int s_end = (cnt * (wv + 1)) / S;
-  int s = (cnt * wv) / S;
+  const int npk = (cnt + kPack - 1) / kPack;
+  const int b0 = (npk * wv) / S, b1 = (npk * (wv + 1)) / S;
+  int resume = b0;
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (int c0 = 0; c0 < ucount; c0 += C) {
     const int cn = min(C, ucount - c0);
@@ -941,54 +1072,24 @@ __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, cons
     for (int q = threadIdx.x; q < cn; q += 64 * S) {
       const int kk = UL[c0 + q];
       double d = X[kk] - cx, e = Y[kk] - cy, f = Z[kk] - cz;
-      d -= bx * rint(d / bx);
-      e -= by * rint(e / by);
-      f -= bz * rint(f / bz);
+      d -= bx * rint(d * ibx);
+      e -= by * rint(e * iby);
+      f -= bz * rint(f * ibz);
       s_a[q] = Quad<real>{real(d), real(e), real(f), real(IN[0 * np + kk])};
       s_b[q] = Quad<real>{real(IN[1 * np + kk]), real(IN[2 * np + kk]), real(IN[3 * np + kk]), real(IN[4 * np + kk])};
       s_c[q] = real(IN[5 * np + kk]);
     }
     __syncthreads();
-    const int c1 = c0 + cn;
-    int loc[NB], nloc[NB];
-    Coef2<real> cf[NB], ncf[NB];
-    load_entries<real, NB>(LL, PC, np, i, s, s_end, loc, cf);
-    while (loc[0] < c1) {
-      load_entries<real, NB>(LL, PC, np, i, s + NB, s_end, nloc, ncf);
-      bool done = false;
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        if (loc[b] >= c1) {  // end of this lane's list or of the staged chunk
-          done = true;
-          break;
-        }
-        ++s;
-        const int q = loc[b] - c0;
-        const Quad<real> A = s_a[q];
-        const Quad<real> B = s_b[q];
-        const real upz = s_c[q];
-        real dx = A.a - pix, dy = A.b - piy, dz = A.c - piz;
-        if (wrap) {
-          dx -= lx * rint(dx * ilx);
-          dy -= ly * rint(dy * ily);
-          dz -= lz * rint(dz * ilz);
-        }
-        const real d3 = cf[b].x, d5 = cf[b].y;
-        const real urd = d5 * (dx * A.d + dy * B.a + dz * B.b);
-        const real urp = d5 * (dx * B.c + dy * B.d + dz * upz);
-        acc[0] += double(d3 * A.d - urd * dx);
-        acc[1] += double(d3 * B.a - urd * dy);
-        acc[2] += double(d3 * B.b - urd * dz);
-        acc[3] += double(d3 * B.c - urp * dx);
-        acc[4] += double(d3 * B.d - urp * dy);
-        acc[5] += double(d3 * upz - urp * dz);
-      }
-      if (done) break;
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        loc[b] = nloc[b];
-        cf[b] = ncf[b];
-      }
+    if (ucount <= C) {
+      if (wrap)
+        walk_packs<real, NP, true>(TL, TC, np, i, b0, b1, g, acc);
+      else
+        walk_packs<real, NP, false>(TL, TC, np, i, b0, b1, g, acc);
+    } else {
+      if (wrap)
+        walk_chunk<real, true>(TL, TC, np, i, b1, c0, cn, resume, g, acc);
+      else
+        walk_chunk<real, false>(TL, TC, np, i, b1, c0, cn, resume, g, acc);
     }
   }
   // scaled pairs with their per-solve FP64 coefficients (tmatxb u-scale)

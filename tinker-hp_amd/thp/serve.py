@@ -180,6 +180,7 @@ def poisson_stream(ctx, n_requests=2000, rate_per_s=100.0, resubmits=10, n_min=5
                         for v in (systems[k].x, systems[k].y, systems[k].z)) for k in order]
         gaps = rng.exponential(1.0 / rate_per_s, len(order))
         tickets = []
+        st0 = srv.stats()
         t0 = time.perf_counter()
         t_next = t0
         for q, k in enumerate(order):
@@ -204,6 +205,12 @@ def poisson_stream(ctx, n_requests=2000, rate_per_s=100.0, resubmits=10, n_min=5
                     solves_per_s=len(order) / dt, offered_rate=rate_per_s, submit_seconds=round(t_sub, 3),
                     p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99), niter=float(np.mean(nit)),
                     mean_active=st.mean_active, evictions=st.evictions, mean_atoms=atoms / len(order),
-                    max_systems=max_systems)
+                    max_systems=max_systems, passes=st.chunks - st0.chunks,
+                    batch_builds=st.batch_builds - st0.batch_builds,
+                    graph_launches=st.graph_launches - st0.graph_launches,
+                    worker_host_ms=round(st.host_ms - st0.host_ms, 1), worker_wait_ms=round(st.wait_ms - st0.wait_ms, 1),
+                    worker_admit_ms=round(st.admit_ms - st0.admit_ms, 1),
+                    worker_setup_ms=round(st.setup_ms - st0.setup_ms, 1),
+                    worker_chunk_ms=round(st.chunk_ms - st0.chunk_ms, 1))
     finally:
         srv.stop()
