@@ -38,20 +38,11 @@ namespace {
                   __FILE__, __LINE__);                                                   \
   } while (0)
 
-#define FFTCHK(ctx, call)                                                                \
-  do {                                                                                   \
-    rocfft_status s_ = (call);                                                           \
-    if (s_ != rocfft_status_success)                                                     \
-      return fail((ctx), THP_ERR_FFT, "%s failed: rocfft status %d", #call, (int)s_);   \
-  } while (0)
-
 #define RET(call)                      \
   do {                                 \
     thp_status r_ = (call);            \
     if (r_ != THP_OK) return r_;       \
   } while (0)
-
-std::once_flag g_fft_once;
 
 template <typename T>
 thp_status dalloc(thp_sys* s, T** p, size_t count) {
@@ -318,19 +309,23 @@ thp_status spread_mpole(thp_ctx* ctx, const Batch& b, int gate) {
   return launch_check(ctx, "k_spread_tile<mpole>");
 }
 
-thp_status convolve(thp_ctx* ctx, const Batch& b, int check_conv) {
-  for (const FftExec& f : b.fft) {
-    void* buf[1] = {f.base};
-    FFTCHK(ctx, rocfft_execute(f.fwd, buf, nullptr, f.info));
-  }
-  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_conv<double>), g1(b.n_conv), dim3(256), 0, ctx->stream, b.d_views, b.d_conv, check_conv);
-  else hipLaunchKernelGGL((k_conv<float>), g1(b.n_conv), dim3(256), 0, ctx->stream, b.d_views, b.d_conv, check_conv);
-  RET(launch_check(ctx, "k_conv"));
-  for (const FftExec& f : b.fft) {
-    void* buf[1] = {f.base};
-    FFTCHK(ctx, rocfft_execute(f.bwd, buf, nullptr, f.info));
-  }
-  return THP_OK;
+// the grids' convolution with qfac: x and y forward passes, the fused z
+// pass (forward, qfac, backward), y and x backward passes
+template <typename real>
+void launch_convolve(thp_ctx* ctx, const Batch& b, int gate) {
+  const dim3 g = g1(b.n_fft), t(256);
+  hipStream_t st = ctx->stream;
+  hipLaunchKernelGGL((k_fft_axis<real, 0, 0>), g, t, 0, st, b.d_views, b.d_fft, gate);
+  hipLaunchKernelGGL((k_fft_axis<real, 1, 0>), g, t, 0, st, b.d_views, b.d_fft, gate);
+  hipLaunchKernelGGL((k_fft_axis<real, 2, 2>), g, t, 0, st, b.d_views, b.d_fft, gate);
+  hipLaunchKernelGGL((k_fft_axis<real, 1, 1>), g, t, 0, st, b.d_views, b.d_fft, gate);
+  hipLaunchKernelGGL((k_fft_axis<real, 0, 1>), g, t, 0, st, b.d_views, b.d_fft, gate);
+}
+
+thp_status convolve(thp_ctx* ctx, const Batch& b, int gate) {
+  if (b.prec == THP_PREC_FP64) launch_convolve<double>(ctx, b, gate);
+  else launch_convolve<float>(ctx, b, gate);
+  return launch_check(ctx, "k_fft_axis");
 }
 
 template <int MODE>
@@ -734,13 +729,15 @@ std::vector<Work> xcd_order(const std::vector<Work>& list) {
 }
 
 void append_work(const thp_sys* s, int index, std::vector<Work>& blk, std::vector<Work>& red, std::vector<Work>& gat,
-                 std::vector<Work>& brick, std::vector<Work>& conv) {
+                 std::vector<Work>& brick, std::vector<Work>& conv, std::vector<Work>& fft) {
   for (int q = 0; q < s->nblk; ++q) blk.push_back({index, q});
   for (int q = 0; q < s->view.red_blocks; ++q) red.push_back({index, q});
   for (int q = 0; q < s->view.gat_blocks; ++q) gat.push_back({index, q});
   for (int q = 0; q < s->nbricks; ++q) brick.push_back({index, q});
   const int nconv = (int)((grid_points(s) + kConvChunk - 1) / kConvChunk);
   for (int q = 0; q < nconv; ++q) conv.push_back({index, q});
+  const int ng = std::max(fft_groups(s->nfft, 0), std::max(fft_groups(s->nfft, 1), fft_groups(s->nfft, 2)));
+  for (int q = 0; q < ng; ++q) fft.push_back({index, q});
 }
 
 }  // namespace thp
@@ -765,7 +762,6 @@ thp_status thp_init(thp_ctx** out, int device) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
-  std::call_once(g_fft_once, [] { rocfft_setup(); });
   // THP_SEGS forces the pair kernels' wavefronts per 64-atom block
   if (const char* e = getenv("THP_SEGS")) {
     const int v = atoi(e);
@@ -806,6 +802,12 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
   for (int k = 0; k < 3; ++k) {
     if (!(d->box[k] > 0)) return fail(ctx, THP_ERR_ARG, "box edge %d must be positive", k);
     if (d->nfft[k] < 2 * kOrder) return fail(ctx, THP_ERR_ARG, "nfft[%d] = %d too small", k, d->nfft[k]);
+    // the grid FFT takes 2,3,5-smooth lengths (kewald.f:67-73 picks only those)
+    int r = d->nfft[k];
+    for (int f : {2, 3, 5})
+      while (r % f == 0) r /= f;
+    if (r != 1 || d->nfft[k] > kFftMaxLen)
+      return fail(ctx, THP_ERR_ARG, "nfft[%d] = %d: the FFT takes 2,3,5-smooth lengths up to %d", k, d->nfft[k], kFftMaxLen);
   }
   if (!(d->cut > 0) || d->buffer < 0 || !(d->aewald > 0)) return fail(ctx, THP_ERR_ARG, "bad cut/buffer/aewald");
   // nblist.f:76-83: (cut + buffer) must not exceed half the box
@@ -1000,7 +1002,7 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
   for (double* vec : {v.ef, v.mu, v.res, v.h, v.hreal, v.pp, v.zr})
     if (hipMemsetAsync(vec, 0, sizeof(double) * 6 * np, st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "memset"));
 
-  // ---- PME grid, scalar-sum table and rocFFT plans
+  // ---- PME grid, scalar-sum table and FFT twiddles
   const size_t npts = grid_points(s);
   const auto q = qfac_table(s->box, s->nfft, s->aewald, s->order);
   if (s->prec == THP_PREC_FP64) {
@@ -1020,23 +1022,27 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
     v.grid = g;
     v.qfac = qf;
   }
-  if (own_grid) {
-    const size_t len[3] = {(size_t)s->nfft[0], (size_t)s->nfft[1], (size_t)s->nfft[2]};
-    const rocfft_precision pr = s->prec == THP_PREC_FP64 ? rocfft_precision_double : rocfft_precision_single;
-    if (rocfft_plan_create(&s->fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward, pr, 3, len, 1, nullptr) != rocfft_status_success ||
-        rocfft_plan_create(&s->bwd, rocfft_placement_inplace, rocfft_transform_type_complex_inverse, pr, 3, len, 1, nullptr) != rocfft_status_success)
-      return cleanup(fail(ctx, THP_ERR_FFT, "rocfft_plan_create failed"));
-    size_t w1 = 0, w2 = 0;
-    rocfft_plan_get_work_buffer_size(s->fwd, &w1);
-    rocfft_plan_get_work_buffer_size(s->bwd, &w2);
-    if (rocfft_execution_info_create(&s->info) != rocfft_status_success) return cleanup(fail(ctx, THP_ERR_FFT, "execution info"));
-    rocfft_execution_info_set_stream(s->info, st);
-    const size_t wb = std::max(w1, w2);
-    if (wb) {
-      char* wp;
-      ALLOC(wp, wb);
-      s->fft_work = wp;
-      rocfft_execution_info_set_work_buffer(s->info, wp, wb);
+  {
+    // twiddles exp(-2 pi i m / K) of the three axes, from FP64
+    std::vector<double> tw;
+    for (int ax = 0; ax < 3; ++ax)
+      for (int m = 0; m < s->nfft[ax]; ++m) {
+        const double ang = -2.0 * M_PI * (double)m / (double)s->nfft[ax];
+        tw.push_back(std::cos(ang));
+        tw.push_back(std::sin(ang));
+      }
+    if (s->prec == THP_PREC_FP64) {
+      double* t;
+      ALLOC(t, tw.size());
+      H2D(t, tw.data(), sizeof(double) * tw.size());
+      v.twid = t;
+    } else {
+      float* t;
+      ALLOC(t, tw.size());
+      std::vector<float> tf(tw.begin(), tw.end());
+      H2D(t, tf.data(), sizeof(float) * tf.size());
+      if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "sync"));
+      v.twid = t;
     }
   }
 
@@ -1104,6 +1110,12 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
     ALLOC(s->d_convwork, nconv);
     H2D(s->d_convwork, cw.data(), sizeof(Work) * nconv);
     s->nconv = nconv;
+    const int nfftw = std::max(fft_groups(s->nfft, 0), std::max(fft_groups(s->nfft, 1), fft_groups(s->nfft, 2)));
+    std::vector<Work> fw(nfftw);
+    for (int b = 0; b < nfftw; ++b) fw[b] = {0, b};
+    ALLOC(s->d_fftwork, nfftw);
+    H2D(s->d_fftwork, fw.data(), sizeof(Work) * nfftw);
+    s->nfftw = nfftw;
     if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "sync"));
   }
   ALLOC(s->d_sysid, 2);
@@ -1131,7 +1143,8 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
     b.d_conv = s->d_convwork;
     b.n_conv = s->nconv;
     b.d_ids = s->d_sysid;
-    if (own_grid) b.fft.push_back({s->fwd, s->bwd, s->info, v.grid});
+    b.d_fft = s->d_fftwork;
+    b.n_fft = s->nfftw;
   }
   for (int e = 0; e < 4; ++e)
     if (hipEventCreate(&s->ev[e]) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "event"));
@@ -1151,9 +1164,6 @@ thp_status thp_system_destroy(thp_sys* s) {
     if (s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
     s->ctx->systems.erase(s);
   }
-  if (s->fwd) rocfft_plan_destroy(s->fwd);
-  if (s->bwd) rocfft_plan_destroy(s->bwd);
-  if (s->info) rocfft_execution_info_destroy(s->info);
   for (void* p : s->allocs) (void)hipFree(p);
   for (auto& e : s->ev)
     if (e) (void)hipEventDestroy(e);

@@ -2,7 +2,6 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
-#include <rocfft/rocfft.h>
 
 #include <cstdint>
 #include <set>
@@ -117,6 +116,7 @@ struct SysView {
   void* grid;         // complex interleaved, x fastest, real type per prec
   void* tiles;        // [nbricks][12^3][2] spreading tiles (real type), scratch
   const void* qfac;   // K1*K2*K3 real
+  const void* twid;   // [K1 + K2 + K3] complex exp(-2 pi i m / K) per axis (real type)
   // solver vectors [6][npad] FP64 (component-major)
   double* ef;
   double* mu;
@@ -136,17 +136,28 @@ struct Work {  // one unit of a launch: a block (brick, chunk ...) of one system
 };
 
 constexpr int kConvChunk = 2048;
+// grid FFT: complex values per LDS buffer of a workgroup, longest axis
+constexpr int kFftBuf = 2048;
+constexpr int kFftMaxLen = 512;
+
+// lines per FFT workgroup along an axis of length K: the largest power of
+// two up to 32 whose padded lines (K + 1 values each) fit one LDS buffer
+__host__ __device__ inline int fft_group_lines(int K) {
+  int fl = 32;
+  while (fl > 1 && fl * (K + 1) > kFftBuf) fl >>= 1;
+  return fl;
+}
+
+// workgroups of one FFT pass along `axis` of a K0 x K1 x K2 grid
+__host__ __device__ inline int fft_groups(const int* nfft, int axis) {
+  const int fl = fft_group_lines(nfft[axis]);
+  const int inner = axis == 0 ? nfft[1] : nfft[0];
+  const int outer = axis == 2 ? nfft[1] : nfft[2];
+  return outer * ((inner + fl - 1) / fl);
+}
 constexpr int kTile = 12;                      // spreading tile edge: brick (<= 8) + order - 1
 constexpr int kTile3 = kTile * kTile * kTile;  // points per tile
 constexpr int kDefaultSegs = 4;   // pair-kernel wavefronts per block unless tuned
-
-// One rocFFT execution: forward and backward plans over `count` contiguous
-// grids starting at `base` (count 1 for a single system).
-struct FftExec {
-  rocfft_plan fwd = nullptr, bwd = nullptr;
-  rocfft_execution_info info = nullptr;
-  void* base = nullptr;
-};
 
 // A set of systems launched together.  Every kernel reaches its systems
 // through d_views[w.sys] and walks one of the work tables; a single system
@@ -165,10 +176,11 @@ struct Batch {
   int n_gat = 0;
   Work* d_brick = nullptr;      // spreading bricks
   int n_brick = 0;
-  Work* d_conv = nullptr;       // convolution chunks of kConvChunk points
+  Work* d_conv = nullptr;       // grid chunks of kConvChunk points
   int n_conv = 0;
+  Work* d_fft = nullptr;        // FFT line groups (the most of the three axes)
+  int n_fft = 0;
   int* d_ids = nullptr;         // [nsys] = 0..nsys-1
-  std::vector<FftExec> fft;
 };
 
 }  // namespace thp
@@ -220,7 +232,9 @@ struct thp_sys {
   thp::Work* d_gatwork = nullptr;  // gather work table [gat_blocks]
   int* d_sysid = nullptr;          // {0, 1}
   int* d_one = nullptr;            // d_sysid + 1
-  thp::Work* d_convwork = nullptr; // convolution work table
+  thp::Work* d_convwork = nullptr; // grid-chunk work table
+  thp::Work* d_fftwork = nullptr;  // FFT line-group work table
+  int nfftw = 0;
   thp::Work* d_workx = nullptr;    // block table in XCD order (path kernels)
   thp::Work* d_brickx = nullptr;   // brick table in XCD order
   int nworkx = 0, nbrickx = 0;
@@ -232,9 +246,6 @@ struct thp_sys {
   int* blist = nullptr;    // [nblk][maxbl]
   int maxbl = 0;
   // rocFFT
-  rocfft_plan fwd = nullptr, bwd = nullptr;
-  rocfft_execution_info info = nullptr;
-  void* fft_work = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
@@ -250,7 +261,8 @@ size_t grid_points(const thp_sys* s);
 std::vector<Work> xcd_order(const std::vector<Work>& list);
 // per-system work tables with sys = `index` (appended)
 void append_work(const thp_sys* s, int index, std::vector<Work>& blk, std::vector<Work>& red,
-                 std::vector<Work>& gat, std::vector<Work>& brick, std::vector<Work>& conv);
+                 std::vector<Work>& gat, std::vector<Work>& brick, std::vector<Work>& conv,
+                 std::vector<Work>& fft);
 // batched launch sequences (enqueued on ctx->stream)
 // d_req (optional): per-system request flags; requested systems start a
 // solve in set-up state (kGateSetup kernels work on them until b_end_setup)

@@ -1733,22 +1733,189 @@ __global__ void __launch_bounds__(256) k_spread_merge(const SysView* views, cons
   }
 }
 
-// scalar-sum convolution between the forward and backward transforms
-// (qfac, newinduce_pme.f:1790-1861)
+// ===================================================== 3-D FFT
+// Batched multi-size complex FFT of the PME grids, one axis per launch: the
+// unnormalised forward then backward transforms of fft2d_frontmpi/backmpi
+// (fft_mpi.f90:65-160), whose normalisation sits in qfac.  A workgroup
+// transforms a group of lines of one system's grid along one axis in LDS
+// with self-sorting (Stockham) radix-4/2/3/5 stages; systems of a launch
+// may have different grid sizes.  The z pass fuses the forward transform,
+// the scalar-sum product with qfac (newinduce_pme.f:1790-1861) and the
+// backward transform, so a convolution is five passes over the grid.
+// Lines of a group are consecutive along the fastest other index (y for the
+// x pass, x for the y and z passes), so every global access of a group is
+// a run of consecutive complex values.
+
 template <typename real>
-__global__ void __launch_bounds__(256) k_conv(const SysView* views, const Work* work, int gate) {
+__device__ __forceinline__ Coef2<real> cmul(Coef2<real> a, Coef2<real> b) {
+  return coef2<real>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// s * i * a
+template <typename real>
+__device__ __forceinline__ Coef2<real> cmul_si(Coef2<real> a, real s) {
+  return coef2<real>(-s * a.y, s * a.x);
+}
+
+// X_k = sum_q x_q w^(qk), w = exp(s 2 pi i / R)
+template <typename real, int R>
+__device__ __forceinline__ void dft_small(Coef2<real>* x, real s) {
+  if constexpr (R == 2) {
+    const Coef2<real> a = x[0], b = x[1];
+    x[0] = a + b;
+    x[1] = a - b;
+  } else if constexpr (R == 4) {
+    const Coef2<real> t0 = x[0] + x[2], t1 = x[0] - x[2], t2 = x[1] + x[3], t3 = x[1] - x[3];
+    const Coef2<real> it3 = cmul_si<real>(t3, s);
+    x[0] = t0 + t2;
+    x[2] = t0 - t2;
+    x[1] = t1 + it3;
+    x[3] = t1 - it3;
+  } else if constexpr (R == 3) {
+    const real h = real(0.86602540378443864676);  // sin(2 pi / 3)
+    const Coef2<real> t1 = x[1] + x[2], t2 = x[1] - x[2];
+    const Coef2<real> m = x[0] - real(0.5) * t1;
+    const Coef2<real> it2 = cmul_si<real>(h * t2, s);
+    x[0] = x[0] + t1;
+    x[1] = m + it2;
+    x[2] = m - it2;
+  } else {
+    static_assert(R == 5, "radix 2, 3, 4 or 5");
+    const real c1 = real(0.30901699437494742410), c2 = real(-0.80901699437494742410);  // cos(2 pi/5), cos(4 pi/5)
+    const real s1 = real(0.95105651629515357212), s2 = real(0.58778525229247312917);   // sin(2 pi/5), sin(4 pi/5)
+    const Coef2<real> t1 = x[1] + x[4], t2 = x[2] + x[3], t3 = x[1] - x[4], t4 = x[2] - x[3];
+    const Coef2<real> a1 = x[0] + c1 * t1 + c2 * t2, a2 = x[0] + c2 * t1 + c1 * t2;
+    const Coef2<real> b1 = cmul_si<real>(s1 * t3 + s2 * t4, s), b2 = cmul_si<real>(s2 * t3 - s1 * t4, s);
+    x[0] = x[0] + t1 + t2;
+    x[1] = a1 + b1;
+    x[4] = a1 - b1;
+    x[2] = a2 + b2;
+    x[3] = a2 - b2;
+  }
+}
+
+// one Stockham stage of radix R over the group's lines (line stride kp in
+// LDS): butterfly i of a line reads src[i + q K/R], applies the twiddles
+// w^(q k K/(pR)) with k = i mod p, and writes dst[(i - k) R + k + q p]
+template <typename real, int R>
+__device__ __forceinline__ void fft_stage(const Coef2<real>* src, Coef2<real>* dst, const Coef2<real>* tw, int K,
+                                          int kp, int p, int nl, int tid, int tpl_shift, real s) {
+  const int T = K / R;
+  const int step = K / (p * R);
+  const int a = tid >> tpl_shift;
+  if (a >= nl) return;
+  const Coef2<real>* in = src + a * kp;
+  Coef2<real>* out = dst + a * kp;
+  for (int i = tid & ((1 << tpl_shift) - 1); i < T; i += 1 << tpl_shift) {
+    const int k = i % p;
+    Coef2<real> x[R];
+    x[0] = in[i];
+#pragma unroll
+    for (int q = 1; q < R; ++q) {
+      Coef2<real> w = tw[q * k * step];
+      w.y *= -s;  // tw holds exp(-2 pi i m / K): conjugated for the backward sign
+      x[q] = cmul<real>(in[i + q * T], coef2<real>(w.x, w.y));
+    }
+    dft_small<real, R>(x, s);
+    const int j = (i - k) * R + k;
+#pragma unroll
+    for (int q = 0; q < R; ++q) out[j + q * p] = x[q];
+  }
+}
+
+// whole transform of the group's lines; returns the buffer holding it
+template <typename real>
+__device__ int fft_lines_lds(Coef2<real>* b0, Coef2<real>* b1, int cur, const Coef2<real>* tw, int K, int kp, int nl,
+                             int tid, int tpl_shift, real s) {
+  int rem = K, p = 1;
+  while (rem > 1) {
+    const int R = rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2 : rem % 3 == 0 ? 3 : 5;
+    const Coef2<real>* src = cur ? b1 : b0;
+    Coef2<real>* dst = cur ? b0 : b1;
+    switch (R) {
+      case 4: fft_stage<real, 4>(src, dst, tw, K, kp, p, nl, tid, tpl_shift, s); break;
+      case 2: fft_stage<real, 2>(src, dst, tw, K, kp, p, nl, tid, tpl_shift, s); break;
+      case 3: fft_stage<real, 3>(src, dst, tw, K, kp, p, nl, tid, tpl_shift, s); break;
+      default: fft_stage<real, 5>(src, dst, tw, K, kp, p, nl, tid, tpl_shift, s); break;
+    }
+    __syncthreads();
+    cur ^= 1;
+    p *= R;
+    rem /= R;
+  }
+  return cur;
+}
+
+// MODE 0 forward, 1 backward, 2 forward * qfac * backward (z pass only)
+template <typename real, int AXIS, int MODE>
+__global__ void __launch_bounds__(256) k_fft_axis(const SysView* views, const Work* work, int gate) {
+  static_assert(MODE != 2 || AXIS == 2, "the convolution is fused into the z pass");
+  __shared__ Coef2<real> s_buf[2][kFftBuf];
+  __shared__ Coef2<real> s_tw[kFftMaxLen];
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
   if (gated_out(sv.st, gate)) return;
-  const size_t npts = (size_t)sv.nfft[0] * sv.nfft[1] * sv.nfft[2];
-  GLOBAL real* g = gw(static_cast<real*>(sv.grid));
-  const GLOBAL real* q = gp(static_cast<const real*>(sv.qfac));
-  const size_t p0 = (size_t)w.blk * kConvChunk, p1 = min(npts, p0 + kConvChunk);
-  for (size_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-    const real f = q[p];
-    g[2 * p] *= f;
-    g[2 * p + 1] *= f;
+  const int K0 = sv.nfft[0], K1 = sv.nfft[1], K2 = sv.nfft[2];
+  const int K = sv.nfft[AXIS];
+  const int kp = K + 1;  // padded line stride in LDS
+  const int fl = fft_group_lines(K);
+  const int lg = 31 - __clz(fl);
+  const int inner_n = AXIS == 0 ? K1 : K0;
+  const int outer_n = AXIS == 2 ? K1 : K2;
+  const int ngi = (inner_n + fl - 1) / fl;
+  if (w.blk >= outer_n * ngi) return;
+  const int outer = w.blk / ngi;
+  const int in0 = (w.blk - outer * ngi) * fl;
+  const int nl = min(fl, inner_n - in0);
+  // complex element (line a, position t) of the group: base + t * ts + a * as
+  size_t base, ts;
+  if (AXIS == 0) {
+    base = ((size_t)outer * K1 + in0) * K0;
+    ts = 1;
+  } else if (AXIS == 1) {
+    base = (size_t)outer * K1 * K0 + in0;
+    ts = (size_t)K0;
+  } else {
+    base = (size_t)outer * K0 + in0;
+    ts = (size_t)K0 * K1;
+  }
+  GLOBAL Coef2<real>* G = gw(static_cast<Coef2<real>*>(sv.grid));
+  const GLOBAL Coef2<real>* TW =
+      gp(static_cast<const Coef2<real>*>(sv.twid)) + (AXIS == 0 ? 0 : AXIS == 1 ? K0 : K0 + K1);
+  const int tid = threadIdx.x;
+  for (int m = tid; m < K; m += 256) s_tw[m] = TW[m];
+  if (AXIS == 0) {
+    // the group's lines are one contiguous run of nl * K values
+    for (int f = tid; f < nl * K; f += 256) {
+      const int a = f / K, t = f - a * K;
+      s_buf[0][a * kp + t] = G[base + f];
+    }
+  } else {
+    const int a = tid & (fl - 1);
+    if (a < nl)
+      for (int t = tid >> lg; t < K; t += 256 >> lg) s_buf[0][a * kp + t] = G[base + t * ts + a];
+  }
+  __syncthreads();
+  const int tpl_shift = 8 - lg;  // threads per line = 256 / fl
+  int cur = fft_lines_lds<real>(s_buf[0], s_buf[1], 0, s_tw, K, kp, nl, tid, tpl_shift, MODE == 1 ? real(1) : real(-1));
+  if (MODE == 2) {
+    const GLOBAL real* Q = gp(static_cast<const real*>(sv.qfac));
+    const int a = tid & (fl - 1);
+    if (a < nl)
+      for (int t = tid >> lg; t < K; t += 256 >> lg) s_buf[cur][a * kp + t] *= Q[base + t * ts + a];
+    __syncthreads();
+    cur = fft_lines_lds<real>(s_buf[0], s_buf[1], cur, s_tw, K, kp, nl, tid, tpl_shift, real(1));
+  }
+  if (AXIS == 0) {
+    for (int f = tid; f < nl * K; f += 256) {
+      const int a = f / K, t = f - a * K;
+      G[base + f] = s_buf[cur][a * kp + t];
+    }
+  } else {
+    const int a = tid & (fl - 1);
+    if (a < nl)
+      for (int t = tid >> lg; t < K; t += 256 >> lg) G[base + t * ts + a] = s_buf[cur][a * kp + t];
   }
 }
 

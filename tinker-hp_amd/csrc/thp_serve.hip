@@ -77,16 +77,14 @@ struct DipoleCopy {  // thp_serve_get_dipoles handed to the worker
   thp_status rc = THP_OK;
 };
 
-// reciprocal grids of one size, side by side, and batched FFT plans
+// reciprocal grids of one size, side by side, with their spreading tiles
 struct GridPool {
   int K[3] = {0, 0, 0};
   int cap = 0;
   void* base = nullptr;
-  size_t bytes_per_grid = 0;   // stride of the grids (FFT batch distance)
+  size_t bytes_per_grid = 0;
   void* tiles = nullptr;       // spreading tiles of each slot, after the grids
   size_t bytes_per_tiles = 0;
-  std::map<int, FftExec> plans;  // by number of grids
-  std::vector<void*> work;       // rocFFT work buffers of the plans
 };
 
 // device and pinned host buffers of one batch layout
@@ -238,59 +236,18 @@ void free_batchbuf(BatchBuf& bb) {
 }
 
 void free_pool(GridPool& gp) {
-  for (auto& kv : gp.plans) {
-    if (kv.second.fwd) rocfft_plan_destroy(kv.second.fwd);
-    if (kv.second.bwd) rocfft_plan_destroy(kv.second.bwd);
-    if (kv.second.info) rocfft_execution_info_destroy(kv.second.info);
-  }
-  for (void* w : gp.work) (void)hipFree(w);
   if (gp.base) (void)hipFree(gp.base);
   gp = GridPool();
 }
 
-// the batched plan over `count` grids of a pool (created on first use)
-thp_status pool_plan(Serve* sv, GridPool& gp, int count, int prec, FftExec* out) {
-  auto it = gp.plans.find(count);
-  if (it != gp.plans.end()) {
-    *out = it->second;
-    out->base = gp.base;
-    return THP_OK;
-  }
-  thp_ctx* ctx = sv->ctx;
-  FftExec f;
-  const size_t len[3] = {(size_t)gp.K[0], (size_t)gp.K[1], (size_t)gp.K[2]};
-  const rocfft_precision pr = prec == THP_PREC_FP64 ? rocfft_precision_double : rocfft_precision_single;
-  if (rocfft_plan_create(&f.fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward, pr, 3, len, (size_t)count,
-                         nullptr) != rocfft_status_success ||
-      rocfft_plan_create(&f.bwd, rocfft_placement_inplace, rocfft_transform_type_complex_inverse, pr, 3, len, (size_t)count,
-                         nullptr) != rocfft_status_success)
-    return fail(ctx, THP_ERR_FFT, "rocfft_plan_create (%d grids of %dx%dx%d) failed", count, gp.K[0], gp.K[1], gp.K[2]);
-  if (rocfft_execution_info_create(&f.info) != rocfft_status_success) return fail(ctx, THP_ERR_FFT, "execution info");
-  rocfft_execution_info_set_stream(f.info, ctx->stream);
-  size_t w1 = 0, w2 = 0;
-  rocfft_plan_get_work_buffer_size(f.fwd, &w1);
-  rocfft_plan_get_work_buffer_size(f.bwd, &w2);
-  const size_t wb = std::max(w1, w2);
-  if (wb) {
-    void* w = nullptr;
-    SHIP(ctx, hipMalloc(&w, wb));
-    gp.work.push_back(w);
-    rocfft_execution_info_set_work_buffer(f.info, w, wb);
-  }
-  gp.plans[count] = f;
-  *out = f;
-  out->base = gp.base;
-  return THP_OK;
-}
-
-// Build a batch over `members` (views, work tables, grid slots, FFT plans)
+// Build a batch over `members` (views, work tables, grid slots)
 // into bb.  Grids are scratch between kernels of one pass, so each batch
 // takes pool slots 0..m-1 of each grid size afresh.
 thp_status build_batch(Serve* sv, BatchBuf& bb, const std::vector<thp_sys*>& members, const std::vector<int>& warm) {
   thp_ctx* ctx = sv->ctx;
   const int m = (int)members.size();
-  std::vector<Work> blk, red, gat, brick, conv;
-  for (int q = 0; q < m; ++q) append_work(members[q], q, blk, red, gat, brick, conv);
+  std::vector<Work> blk, red, gat, brick, conv, fft;
+  for (int q = 0; q < m; ++q) append_work(members[q], q, blk, red, gat, brick, conv, fft);
   blk = xcd_order(blk);
   brick = xcd_order(brick);
   gat = xcd_order(gat);
@@ -326,7 +283,7 @@ thp_status build_batch(Serve* sv, BatchBuf& bb, const std::vector<thp_sys*>& mem
   }
   // views
   SRET(grow_pinned(ctx, (void**)&bb.h_views, &bb.hcap_views, m, sizeof(SysView)));
-  const size_t nt = blk.size() + red.size() + gat.size() + brick.size() + conv.size();
+  const size_t nt = blk.size() + red.size() + gat.size() + brick.size() + conv.size() + fft.size();
   SRET(grow_pinned(ctx, (void**)&bb.h_tables, &bb.hcap_tables, nt, sizeof(Work)));
   size_t cap_ids = bb.cap_views;
   SRET(grow_device(ctx, (void**)&bb.views, &bb.cap_views, m, sizeof(SysView)));
@@ -347,7 +304,6 @@ thp_status build_batch(Serve* sv, BatchBuf& bb, const std::vector<thp_sys*>& mem
   b = Batch();
   b.prec = prec;
   b.nsys = m;
-  b.fft.clear();
   for (auto& g : groups) {
     GridPool& gp = sv->pools[g.first];
     for (size_t slot = 0; slot < g.second.size(); ++slot) {
@@ -356,9 +312,6 @@ thp_status build_batch(Serve* sv, BatchBuf& bb, const std::vector<thp_sys*>& mem
       bb.h_views[q].grid = static_cast<char*>(gp.base) + slot * gp.bytes_per_grid;
       bb.h_views[q].tiles = static_cast<char*>(gp.tiles) + slot * gp.bytes_per_tiles;
     }
-    FftExec f;
-    SRET(pool_plan(sv, gp, (int)g.second.size(), prec, &f));
-    b.fft.push_back(f);
   }
   for (int q = 0; q < m; ++q) {
     bb.h_ids[q] = q;
@@ -376,6 +329,7 @@ thp_status build_batch(Serve* sv, BatchBuf& bb, const std::vector<thp_sys*>& mem
   put(gat, &b.d_gat, &b.n_gat);
   put(brick, &b.d_brick, &b.n_brick);
   put(conv, &b.d_conv, &b.n_conv);
+  put(fft, &b.d_fft, &b.n_fft);
   b.d_views = bb.views;
   b.d_ids = bb.ids;
   hipStream_t st = ctx->stream;
