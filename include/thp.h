@@ -50,7 +50,7 @@ typedef int thp_status;
 #define THP_OK 0
 #define THP_ERR_ARG (-1)      /* invalid argument */
 #define THP_ERR_HIP (-2)      /* HIP runtime failure */
-#define THP_ERR_FFT (-3)      /* rocFFT failure */
+#define THP_ERR_FFT (-3)      /* grid FFT failure (reserved) */
 #define THP_ERR_NOMEM (-4)    /* device allocation failed / over budget */
 #define THP_ERR_LIST (-5)     /* cutoff+buffer exceeds half the box, or list capacity */
 #define THP_ERR_NOCONV (-6)   /* PCG did not reach poleps within politer */

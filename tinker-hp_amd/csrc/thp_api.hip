@@ -252,16 +252,12 @@ thp_status launch_check(thp_ctx* ctx, const char* what) {
 }
 
 template <typename real, int KIND>
-void launch_pair(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int check_conv) {
-  hipStream_t st = ctx->stream;
-  const dim3 g = g1(b.n_blk);
-  switch (b.segs) {
-    case 1: hipLaunchKernelGGL((k_pair_union<real, KIND, 1>), g, dim3(64), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
-    case 2: hipLaunchKernelGGL((k_pair_union<real, KIND, 2>), g, dim3(128), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
-    case 8: hipLaunchKernelGGL((k_pair_union<real, KIND, 8>), g, dim3(512), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
-    case 16: hipLaunchKernelGGL((k_pair_union<real, KIND, 16>), g, dim3(1024), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
-    default: hipLaunchKernelGGL((k_pair_union<real, KIND, 4>), g, dim3(256), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, check_conv); break;
-  }
+void launch_pair(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int gate) {
+  // the permanent-field pass carries ~300 flops and two exponentials per
+  // entry: eight wavefronts per block beat the matvec's tuned shape
+  constexpr int S = kEfld0Segs;
+  hipLaunchKernelGGL((k_pair_union<real, KIND, S>), g1(b.n_blk), dim3(64 * S), 0, ctx->stream, b.d_views, b.d_blk, vin,
+                     vout, dodiag, gate);
 }
 
 // direct permanent field + the pair coefficients and compacted lists that

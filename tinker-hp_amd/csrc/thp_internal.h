@@ -157,7 +157,8 @@ __host__ __device__ inline int fft_groups(const int* nfft, int axis) {
 }
 constexpr int kTile = 12;                      // spreading tile edge: brick (<= 8) + order - 1
 constexpr int kTile3 = kTile * kTile * kTile;  // points per tile
-constexpr int kDefaultSegs = 4;   // pair-kernel wavefronts per block unless tuned
+constexpr int kDefaultSegs = 4;   // matvec wavefronts per block unless tuned
+constexpr int kEfld0Segs = 8;     // permanent-field pass wavefronts per block
 
 // A set of systems launched together.  Every kernel reaches its systems
 // through d_views[w.sys] and walks one of the work tables; a single system
@@ -245,7 +246,6 @@ struct thp_sys {
   int* bcount = nullptr;   // [nblk]
   int* blist = nullptr;    // [nblk][maxbl]
   int maxbl = 0;
-  // rocFFT
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 

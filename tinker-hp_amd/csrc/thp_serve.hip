@@ -14,8 +14,8 @@
 // captured once per batch layout as a HIP graph and replayed, so a pass
 // costs the host a few copies and one launch.  Kernels walk work tables of
 // (system, unit) entries, so solves at different stages share each launch;
-// reciprocal grids of equal size sit side by side in a pool and go through
-// one batched rocFFT plan.
+// reciprocal grids of equal size sit side by side in a pool, and one set of
+// FFT launches transforms the grids of every size in the batch.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>

@@ -119,6 +119,7 @@ def bench_serving(ctx, system: System, steps: int = 20, warmup: int = 2, poleps:
         total = steps * copies
         lat, nit, eps = [], [], []
         owner = {}
+        st0 = srv.stats()
         t0 = time.perf_counter()
         issued = 0
         for _ in range(max(1, depth)):
@@ -139,16 +140,24 @@ def bench_serving(ctx, system: System, steps: int = 20, warmup: int = 2, poleps:
                 issued += 1
         dt = time.perf_counter() - t0
         st = srv.stats()
+        d = _stats_delta(st0, st)
         return dict(solves_per_s=total / dt, ms_per_solve=1e3 * dt / total, solves=total, copies=copies,
                     niter=float(np.mean(nit)), ep=float(eps[0]), p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99),
-                    mean_active=st.mean_active, pair_segs=st.pair_segs, warm_start=warm_start,
-                    iters_per_graph=iters_per_graph, graphs=int(st.graph_launches > 0), depth=depth,
-                    batch_builds=st.batch_builds, worker_host_ms=round(st.host_ms, 1),
-                    worker_wait_ms=round(st.wait_ms, 1), worker_chunk_ms=round(st.chunk_ms, 1),
-                    worker_admit_ms=round(st.admit_ms, 1), worker_setup_ms=round(st.setup_ms, 1),
-                    worker_coords_ms=round(st.coords_ms, 1), worker_lists_ms=round(st.lists_ms, 1))
+                    pair_segs=st.pair_segs, warm_start=warm_start, iters_per_graph=iters_per_graph,
+                    graphs=int(st.graph_launches > 0), depth=depth, **d)
     finally:
         srv.stop()
+
+
+def _stats_delta(a, b):
+    """Worker figures between two stats snapshots (the timed region)."""
+    passes = b.chunks - a.chunks
+    active = b.mean_active * b.chunks - a.mean_active * a.chunks
+    out = dict(passes=passes, mean_active=active / passes if passes else 0.0,
+               batch_builds=b.batch_builds - a.batch_builds, graph_launches=b.graph_launches - a.graph_launches)
+    for f in ("host_ms", "wait_ms", "chunk_ms", "admit_ms", "setup_ms", "coords_ms", "lists_ms"):
+        out["worker_" + f] = round(getattr(b, f) - getattr(a, f), 1)
+    return out
 
 
 def poisson_stream(ctx, n_requests=2000, rate_per_s=100.0, resubmits=10, n_min=500, n_max=30000, max_systems=16,
@@ -204,13 +213,7 @@ def poisson_stream(ctx, n_requests=2000, rate_per_s=100.0, resubmits=10, n_min=5
         return dict(requests=len(order), systems=n_sys, failed=bad, seconds=round(dt, 3),
                     solves_per_s=len(order) / dt, offered_rate=rate_per_s, submit_seconds=round(t_sub, 3),
                     p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99), niter=float(np.mean(nit)),
-                    mean_active=st.mean_active, evictions=st.evictions, mean_atoms=atoms / len(order),
-                    max_systems=max_systems, passes=st.chunks - st0.chunks,
-                    batch_builds=st.batch_builds - st0.batch_builds,
-                    graph_launches=st.graph_launches - st0.graph_launches,
-                    worker_host_ms=round(st.host_ms - st0.host_ms, 1), worker_wait_ms=round(st.wait_ms - st0.wait_ms, 1),
-                    worker_admit_ms=round(st.admit_ms - st0.admit_ms, 1),
-                    worker_setup_ms=round(st.setup_ms - st0.setup_ms, 1),
-                    worker_chunk_ms=round(st.chunk_ms - st0.chunk_ms, 1))
+                    evictions=st.evictions, mean_atoms=atoms / len(order),
+                    max_systems=max_systems, **_stats_delta(st0, st))
     finally:
         srv.stop()
