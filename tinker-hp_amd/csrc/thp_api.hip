@@ -251,22 +251,18 @@ thp_status launch_check(thp_ctx* ctx, const char* what) {
   return THP_OK;
 }
 
-template <typename real, int KIND>
-void launch_pair(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int gate) {
-  // the permanent-field pass carries ~300 flops and two exponentials per
-  // entry: eight wavefronts per block beat the matvec's tuned shape
-  constexpr int S = kEfld0Segs;
-  hipLaunchKernelGGL((k_pair_union<real, KIND, S>), g1(b.n_blk), dim3(64 * S), 0, ctx->stream, b.d_views, b.d_blk, vin,
-                     vout, dodiag, gate);
-}
-
 // direct permanent field + the pair coefficients and compacted lists that
-// the matvec of the same coordinates uses
+// the matvec of the same coordinates uses.  The pass carries ~300 flops and
+// two exponentials per entry: eight wavefronts per block beat the matvec's
+// tuned shape.
 thp_status pair_efld0(thp_ctx* ctx, const Batch& b, int gate) {
-  if (b.prec == THP_PREC_FP64) launch_pair<double, 0>(ctx, b, 0, 0, 0, gate);
-  else launch_pair<float, 0>(ctx, b, 0, 0, 0, gate);
+  constexpr int S = kEfld0Segs;
+  if (b.prec == THP_PREC_FP64)
+    hipLaunchKernelGGL((k_efld0<double, S>), g1(b.n_blk), dim3(64 * S), 0, ctx->stream, b.d_views, b.d_blk, gate);
+  else
+    hipLaunchKernelGGL((k_efld0<float, S>), g1(b.n_blk), dim3(64 * S), 0, ctx->stream, b.d_views, b.d_blk, gate);
   hipLaunchKernelGGL(k_scaled<0>, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, 0, 0, gate);
-  return launch_check(ctx, "k_pair_union<efld0>");
+  return launch_check(ctx, "k_efld0");
 }
 
 thp_status zero_grid(thp_sys* s) {
@@ -425,7 +421,7 @@ thp_status build_union(thp_sys* s) {
         std::copy(s->hz.begin(), s->hz.end(), r.begin() + 2 * (size_t)s->npad);
         HIPCHK(s->ctx, hipMemcpy((void*)s->view.refu, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice));
       }
-      // coefficient storage for the longest list
+      // compacted-list storage for the longest list
       std::vector<int> lc(s->npad);
       HIPCHK(s->ctx, hipMemcpy(lc.data(), s->view.lcount, sizeof(int) * s->npad, hipMemcpyDeviceToHost));
       int cmax = 0;
@@ -433,24 +429,19 @@ thp_status build_union(thp_sys* s) {
       static_assert(32 % kPack == 0, "list storage holds whole lane packs");
       cmax = std::max(32, (cmax + 31) / 32 * 32);
       if (cmax > s->view.cmax) {
-        dfree(s, s->view.pcoef);
         dfree(s, s->view.tcoef);
         dfree(s, s->view.tlist);
-        s->view.pcoef = s->view.tcoef = nullptr;
+        s->view.tcoef = nullptr;
         s->view.tlist = nullptr;
         RET(dalloc(s, &s->view.tlist, (size_t)cmax * s->npad));
         const size_t words = (size_t)cmax * s->npad * 2;
         if (s->prec == THP_PREC_FP64) {
-          double *p = nullptr, *t = nullptr;
-          RET(dalloc(s, &p, words));
+          double* t = nullptr;
           RET(dalloc(s, &t, words));
-          s->view.pcoef = p;
           s->view.tcoef = t;
         } else {
-          float *p = nullptr, *t = nullptr;
-          RET(dalloc(s, &p, words));
+          float* t = nullptr;
           RET(dalloc(s, &t, words));
-          s->view.pcoef = p;
           s->view.tcoef = t;
         }
         s->view.cmax = cmax;

@@ -93,13 +93,11 @@ struct SysView {
   int* ucount;             // [nblk]
   unsigned short* llist;   // [lmax][npad] local partner indices into the block's union
   int* lcount;             // [npad]
-  // per-solve pair coefficients of the list entries: (dmp3, dmp5) of the
-  // real-space dipole field, zero beyond the cutoff; [cmax][npad] pairs of
-  // the pair type, written by the permanent-field pass
-  void* pcoef;
-  int cmax;
-  // per-solve compacted lists: the in-cutoff entries of llist, in order,
-  // with their coefficients, in lane packs of kPack entries: entry e of
+  int cmax;  // longest local list, rounded up to whole lane packs
+  // per-solve compacted lists, written by the permanent-field pass: the
+  // in-cutoff entries of llist, in order, with the coefficients (dmp3, dmp5)
+  // of the real-space dipole field (pair type), in lane packs of kPack
+  // entries: entry e of
   // atom i sits at ((e / kPack) * npad + i) * kPack + e % kPack, and the
   // last pack of an atom is padded with index 0 and zero coefficients
   unsigned short* tlist;   // [cmax / kPack][npad][kPack]

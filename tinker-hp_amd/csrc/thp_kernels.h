@@ -617,12 +617,6 @@ __global__ void __launch_bounds__(64 * kUnionWaves) k_union(const SysView* views
   }
 }
 
-// Pair kernels over the block-union lists.  A workgroup of S wavefronts
-// owns one 64-atom block; the union atoms' data are staged in LDS in
-// chunks, each lane walks its local list (split in S contiguous segments,
-// one per wavefront), and the S partial fields are summed in LDS in a fixed
-// order.  KIND 0: efld0_direct (permanent field, d and p); KIND 1:
-// tmatxb_pme (-E_real of the dipoles vin, plus diag(1/alpha) vin).
 // (dmp3, dmp5) of one list entry, as a two-wide vector (one 8- or 16-byte
 // access)
 template <typename real>
@@ -645,26 +639,34 @@ __device__ __forceinline__ Coef2<real> coef2(real a, real b) {
   return v;
 }
 
-template <typename real, int KIND>
-struct UnionStage {
-  // positions are kept in FP64 so that displacements are exact before
-  // rounding to the pair type
-  static constexpr int NM = KIND == 0 ? 10 : 6;                 // moments per atom
-  static constexpr int BYTES = 24 + NM * (int)sizeof(real) + 2 * (int)sizeof(real);
-  static constexpr int CHUNK = (32 * 1024) / BYTES;              // atoms per LDS chunk
+// Permanent field over the block-union lists (efld0_direct,
+// newinduce_pme.f:1116-1264) and the per-solve compacted lists of the
+// matvec.  A workgroup of S wavefronts owns one 64-atom block; each lane's
+// local list is split in S contiguous segments, one per wavefront.  A first
+// pass counts each segment's entries within the cutoff (the exact FP64
+// test of efld0_direct), so that every wavefront knows where its entries
+// start in the lane's compacted list; the second pass evaluates the field
+// of those entries and writes each one, in list order, with its matvec
+// coefficients (dmp3, dmp5).  Union atoms are staged in LDS in chunks; the
+// S partial fields are summed in LDS in a fixed order.
+template <typename real>
+struct Efld0Stage {
+  // positions stay FP64 so that the cutoff test and the displacements are
+  // exact before rounding to the pair type
+  static constexpr int SMEM = 48 * 1024;
+  static constexpr int CHUNK = SMEM / (24 + 12 * (int)sizeof(real));  // xyz, 10 moments, pdamp, thole
+  static constexpr int CHUNK_POS = SMEM / 24;                           // xyz only (counting pass)
 };
 
-template <typename real, int KIND, int S>
-__global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, const Work* work, int vin, int vout,
-                                                        int dodiag, int gate) {
-  using ST = UnionStage<real, KIND>;
+template <typename real, int S>
+__global__ void __launch_bounds__(64 * S) k_efld0(const SysView* views, const Work* work, int gate) {
+  using ST = Efld0Stage<real>;
   constexpr int C = ST::CHUNK;
-  constexpr int NM = ST::NM;
+  constexpr int CP = ST::CHUNK_POS;
   constexpr int NB = 8;  // list entries fetched per batch
-  __shared__ double s_pos[3][C];
-  __shared__ real s_m[NM][C];
-  __shared__ real s_pd[C], s_th[C];
-  __shared__ double red[S > 1 ? S * 6 * 64 : 1];
+  static_assert(S * 6 * 64 * (int)sizeof(double) <= ST::SMEM, "the cross-wavefront sum fits the staging buffer");
+  __shared__ __attribute__((aligned(16))) unsigned char smem[ST::SMEM];
+  __shared__ int s_kept[S][64];
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
@@ -679,10 +681,7 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
   const GLOBAL double* PD = gp(sv.pdamp);
   const GLOBAL double* TH = gp(sv.thole);
   const GLOBAL double* RP = gp(sv.rpole);
-  const double* in_g = vsel(sv, vin);
-  const GLOBAL double* IN = gp(in_g);
   const GLOBAL unsigned short* LL = gp(sv.llist);
-  GLOBAL Coef2<real>* PC = gw(static_cast<Coef2<real>*>(sv.pcoef));
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int i = w.blk * kBlock + lane;
   const bool ok = i < n;
@@ -693,21 +692,77 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
   const int cnt = ok ? gp(sv.lcount)[i] : 0;
   const int s_begin = (cnt * wv) / S;
   const int s_end = (cnt * (wv + 1)) / S;
+
+  // ---- pass 1: entries of this wavefront's segment within the cutoff
+  {
+    double* px = reinterpret_cast<double*>(smem);
+    double* py = px + CP;
+    double* pz = py + CP;
+    int s = s_begin, kept = 0;
+    for (int c0 = 0; c0 < ucount; c0 += CP) {
+      const int cn = min(CP, ucount - c0);
+      __syncthreads();
+      for (int q = threadIdx.x; q < cn; q += 64 * S) {
+        const int kk = UL[c0 + q];
+        px[q] = X[kk];
+        py[q] = Y[kk];
+        pz[q] = Z[kk];
+      }
+      __syncthreads();
+      const int c1 = c0 + cn;
+      while (s < s_end) {
+        int loc[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) loc[b] = s + b < s_end ? int(LL[(size_t)(s + b) * np + i]) : 0x7fffffff;
+        bool done = false;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          if (loc[b] >= c1) {  // end of this segment or of the staged chunk
+            done = true;
+            break;
+          }
+          ++s;
+          const int q = loc[b] - c0;
+          const double dx = image_exact(px[q] - xi, bx);
+          const double dy = image_exact(py[q] - yi, by);
+          const double dz = image_exact(pz[q] - zi, bz);
+          kept += r2_exact(dx, dy, dz) <= cut2 ? 1 : 0;
+        }
+        if (done) break;
+      }
+    }
+    s_kept[wv][lane] = kept;
+  }
+  __syncthreads();
+  int t = 0;  // this segment's first position in the lane's compacted list
+  for (int q = 0; q < wv; ++q) t += s_kept[q][lane];
+  if (wv == 0 && ok) {
+    int tot = 0;
+    for (int q = 0; q < S; ++q) tot += s_kept[q][lane];
+    gw(sv.tcount)[i] = tot;
+  }
+
+  // ---- pass 2: field of the kept entries, compacted list written in order
+  double* s_pos = reinterpret_cast<double*>(smem);  // [3][C]
+  real* s_m = reinterpret_cast<real*>(s_pos + 3 * C);  // [10][C]
+  real* s_pd = s_m + 10 * C;
+  real* s_th = s_pd + C;
+  GLOBAL unsigned short* TL = gw(sv.tlist);
+  GLOBAL Coef2<real>* TC = gw(static_cast<Coef2<real>*>(sv.tcoef));
   int s = s_begin;
-  int kept = 0;  // entries of this wavefront's segment within the cutoff
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (int c0 = 0; c0 < ucount; c0 += C) {
     const int cn = min(C, ucount - c0);
     __syncthreads();
     for (int q = threadIdx.x; q < cn; q += 64 * S) {
       const int kk = UL[c0 + q];
-      s_pos[0][q] = X[kk];
-      s_pos[1][q] = Y[kk];
-      s_pos[2][q] = Z[kk];
+      s_pos[0 * C + q] = X[kk];
+      s_pos[1 * C + q] = Y[kk];
+      s_pos[2 * C + q] = Z[kk];
       s_pd[q] = real(PD[kk]);
       s_th[q] = real(TH[kk]);
 #pragma unroll
-      for (int m = 0; m < NM; ++m) s_m[m][q] = real(KIND == 0 ? RP[(size_t)m * np + kk] : IN[(size_t)m * np + kk]);
+      for (int m = 0; m < 10; ++m) s_m[m * C + q] = real(RP[(size_t)m * np + kk]);
     }
     __syncthreads();
     const int c1 = c0 + cn;
@@ -718,21 +773,16 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
       bool done = false;
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
-        if (loc[b] >= c1) {  // end of this lane's list or of the staged chunk
+        if (loc[b] >= c1) {
           done = true;
           break;
         }
-        const size_t slot = (size_t)s * np + i;
         ++s;
         const int q = loc[b] - c0;
-        const double dx = image_exact(s_pos[0][q] - xi, bx);
-        const double dy = image_exact(s_pos[1][q] - yi, by);
-        const double dz = image_exact(s_pos[2][q] - zi, bz);
-        if (!(r2_exact(dx, dy, dz) <= cut2)) {  // list entry within the skin only
-          if constexpr (KIND == 0) PC[slot] = coef2<real>(real(0), real(0));
-          continue;
-        }
-        ++kept;
+        const double dx = image_exact(s_pos[0 * C + q] - xi, bx);
+        const double dy = image_exact(s_pos[1 * C + q] - yi, by);
+        const double dz = image_exact(s_pos[2 * C + q] - zi, bz);
+        if (!(r2_exact(dx, dy, dz) <= cut2)) continue;  // list entry within the skin only
         const real xr = real(dx), yr = real(dy), zr = real(dz);
         const real r2 = xr * xr + yr * yr + zr * zr;
         const real rr1 = rsqrt_t(r2);
@@ -740,116 +790,63 @@ __global__ void __launch_bounds__(64 * S) k_pair_union(const SysView* views, con
         const real rr2 = rr1 * rr1;
         const real rr3 = rr2 * rr1;
         const real rr5 = real(3) * rr2 * rr3;
-        if constexpr (KIND == 0) {
-          const real rr7 = real(5) * rr2 * rr5;
-          const real ck = s_m[0][q], dkx = s_m[1][q], dky = s_m[2][q], dkz = s_m[3][q];
-          const real qkxx = s_m[4][q], qkxy = s_m[5][q], qkxz = s_m[6][q], qkyy = s_m[7][q], qkyz = s_m[8][q],
-                     qkzz = s_m[9][q];
-          const real dkr = dkx * xr + dky * yr + dkz * zr;
-          const real qkx = qkxx * xr + qkxy * yr + qkxz * zr;
-          const real qky = qkxy * xr + qkyy * yr + qkyz * zr;
-          const real qkz = qkxz * xr + qkyz * yr + qkzz * zr;
-          const real qkr = qkx * xr + qky * yr + qkz * zr;
-          real bn[3];
-          ewald_bn<real, 3>(r, r2, aew, rr1, bn);
-          real l[3];
-          thole_l<real, 1>(r, pdi, s_pd[q], thi, s_th[q], l);
-          // unscaled pairs: dmp_n = bn_n - (1 - lambda_n) rr_n, the same for d and p
-          const real dmp3 = bn[0] - (real(1) - l[0]) * rr3;
-          const real dmp5 = bn[1] - (real(1) - l[1]) * rr5;
-          const real dmp7 = bn[2] - (real(1) - l[2]) * rr7;
-          PC[slot] = coef2<real>(dmp3, dmp5);
-          const real ai = dmp3 * ck - dmp5 * dkr + dmp7 * qkr;
-          const real d5 = real(2) * dmp5;
-          acc[0] += double(-xr * ai - dmp3 * dkx + d5 * qkx);
-          acc[1] += double(-yr * ai - dmp3 * dky + d5 * qky);
-          acc[2] += double(-zr * ai - dmp3 * dkz + d5 * qkz);
-        } else {
-          real bn[2];
-          ewald_bn<real, 2>(r, r2, aew, rr1, bn);
-          real l[3];
-          thole_l<real, 0>(r, pdi, s_pd[q], thi, s_th[q], l);
-          const real dmp3 = bn[0] - (real(1) - l[0]) * rr3;
-          const real dmp5 = bn[1] - (real(1) - l[1]) * rr5;
-#pragma unroll
-          for (int rhs = 0; rhs < 2; ++rhs) {
-            const real ux = s_m[rhs * 3 + 0][q], uy = s_m[rhs * 3 + 1][q], uz = s_m[rhs * 3 + 2][q];
-            const real ur = dmp5 * (xr * ux + yr * uy + zr * uz);
-            acc[rhs * 3 + 0] += double(dmp3 * ux - ur * xr);
-            acc[rhs * 3 + 1] += double(dmp3 * uy - ur * yr);
-            acc[rhs * 3 + 2] += double(dmp3 * uz - ur * zr);
-          }
-        }
+        const real rr7 = real(5) * rr2 * rr5;
+        const real ck = s_m[0 * C + q], dkx = s_m[1 * C + q], dky = s_m[2 * C + q], dkz = s_m[3 * C + q];
+        const real qkxx = s_m[4 * C + q], qkxy = s_m[5 * C + q], qkxz = s_m[6 * C + q], qkyy = s_m[7 * C + q],
+                   qkyz = s_m[8 * C + q], qkzz = s_m[9 * C + q];
+        const real dkr = dkx * xr + dky * yr + dkz * zr;
+        const real qkx = qkxx * xr + qkxy * yr + qkxz * zr;
+        const real qky = qkxy * xr + qkyy * yr + qkyz * zr;
+        const real qkz = qkxz * xr + qkyz * yr + qkzz * zr;
+        const real qkr = qkx * xr + qky * yr + qkz * zr;
+        real bn[3];
+        ewald_bn<real, 3>(r, r2, aew, rr1, bn);
+        real l[3];
+        thole_l<real, 1>(r, pdi, s_pd[q], thi, s_th[q], l);
+        // unscaled pairs: dmp_n = bn_n - (1 - lambda_n) rr_n, the same for d and p
+        const real dmp3 = bn[0] - (real(1) - l[0]) * rr3;
+        const real dmp5 = bn[1] - (real(1) - l[1]) * rr5;
+        const real dmp7 = bn[2] - (real(1) - l[2]) * rr7;
+        const size_t at = ((size_t)(t / kPack) * np + i) * kPack + t % kPack;
+        TL[at] = (unsigned short)loc[b];
+        TC[at] = coef2<real>(dmp3, dmp5);
+        ++t;
+        const real ai = dmp3 * ck - dmp5 * dkr + dmp7 * qkr;
+        const real d5 = real(2) * dmp5;
+        acc[0] += double(-xr * ai - dmp3 * dkx + d5 * qkx);
+        acc[1] += double(-yr * ai - dmp3 * dky + d5 * qky);
+        acc[2] += double(-zr * ai - dmp3 * dkz + d5 * qkz);
       }
       if (done) break;
     }
   }
-  if constexpr (KIND == 0) {
-    // unscaled pairs give identical d and p fields
-    acc[3] = acc[0];
-    acc[4] = acc[1];
-    acc[5] = acc[2];
-    // compacted list of the in-cutoff entries for this solve's matvecs:
-    // per-lane offsets of the S segments from their counts, then each
-    // wavefront copies its own segment's kept entries, in order
-    __shared__ int s_kept[S][64];
-    s_kept[wv][lane] = kept;
-    __syncthreads();
-    int t = 0;
-    for (int q = 0; q < wv; ++q) t += s_kept[q][lane];
-    if (wv == 0 && ok) {
-      int tot = 0;
-      for (int q = 0; q < S; ++q) tot += s_kept[q][lane];
-      gw(sv.tcount)[i] = tot;
-    }
-    GLOBAL unsigned short* TL = gw(sv.tlist);
-    GLOBAL Coef2<real>* TC = gw(static_cast<Coef2<real>*>(sv.tcoef));
-    for (int s0 = s_begin; s0 < s_end; s0 += NB) {
-      unsigned short l[NB];
-      Coef2<real> c[NB];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        l[b] = 0;
-        c[b] = coef2<real>(real(0), real(0));
-        if (s0 + b < s_end) {
-          l[b] = LL[(size_t)(s0 + b) * np + i];
-          c[b] = PC[(size_t)(s0 + b) * np + i];
-        }
-      }
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        if (c[b].x != real(0) || c[b].y != real(0)) {
-          const size_t at = ((size_t)(t / kPack) * np + i) * kPack + t % kPack;
-          TL[at] = l[b];
-          TC[at] = c[b];
-          ++t;
-        }
-      }
-    }
-    // the last wavefront ends at the lane's total: pad its last pack
-    if (wv == S - 1 && ok) {
-      for (; t % kPack; ++t) {
-        const size_t at = ((size_t)(t / kPack) * np + i) * kPack + t % kPack;
-        TL[at] = 0;
-        TC[at] = coef2<real>(real(0), real(0));
-      }
+  // the last wavefront ends at the lane's total: pad its last pack
+  if (wv == S - 1 && ok) {
+    for (; t % kPack; ++t) {
+      const size_t at = ((size_t)(t / kPack) * np + i) * kPack + t % kPack;
+      TL[at] = 0;
+      TC[at] = coef2<real>(real(0), real(0));
     }
   }
-  // fixed-order sum over the S wavefronts
+  // unscaled pairs give identical d and p fields
+  acc[3] = acc[0];
+  acc[4] = acc[1];
+  acc[5] = acc[2];
+  // fixed-order sum over the S wavefronts (the staging buffer is free once
+  // every wavefront has passed the barrier)
+  double* red = reinterpret_cast<double*>(smem);
   if (S > 1) {
+    __syncthreads();
 #pragma unroll
     for (int c = 0; c < 6; ++c) red[(wv * 6 + c) * 64 + lane] = acc[c];
     __syncthreads();
   }
   if (wv == 0 && ok) {
-    const double pol = gp(sv.pol)[i];
-    const double ipol = pol == 0.0 ? 1.0 / kTinyPol : 1.0 / pol;
-    GLOBAL double* OUT = gw(KIND == 0 ? sv.ef : vsel(sv, vout));
+    GLOBAL double* OUT = gw(sv.ef);
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       double v = acc[c];
       for (int q = 1; q < S; ++q) v += red[(q * 6 + c) * 64 + lane];
-      if (KIND == 1 && dodiag) v += IN[(size_t)c * np + i] * ipol;
       OUT[(size_t)c * np + i] = v;
     }
   }
@@ -1738,7 +1735,7 @@ __global__ void __launch_bounds__(256) k_spread_merge(const SysView* views, cons
 // unnormalised forward then backward transforms of fft2d_frontmpi/backmpi
 // (fft_mpi.f90:65-160), whose normalisation sits in qfac.  A workgroup
 // transforms a group of lines of one system's grid along one axis in LDS
-// with self-sorting (Stockham) radix-4/2/3/5 stages; systems of a launch
+// with self-sorting (Stockham) radix-8/4/2/3/5 stages; systems of a launch
 // may have different grid sizes.  The z pass fuses the forward transform,
 // the scalar-sum product with qfac (newinduce_pme.f:1790-1861) and the
 // backward transform, so a convolution is five passes over the grid.
@@ -1794,6 +1791,30 @@ __device__ __forceinline__ void dft_small(Coef2<real>* x, real s) {
   }
 }
 
+// radix-8 DFT as two radix-4 DFTs: even outputs from x_q + x_(q+4), odd
+// outputs from (x_q - x_(q+4)) w^q
+template <typename real>
+__device__ __forceinline__ void dft8(Coef2<real>* x, real s) {
+  const real r = real(0.70710678118654752440);  // 1 / sqrt(2)
+  Coef2<real> e[4], o[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    e[q] = x[q] + x[q + 4];
+    o[q] = x[q] - x[q + 4];
+  }
+  // w = exp(s i pi / 4): w^1 = (1 + s i) r, w^2 = s i, w^3 = (-1 + s i) r
+  o[1] = coef2<real>(r * (o[1].x - s * o[1].y), r * (o[1].y + s * o[1].x));
+  o[2] = cmul_si<real>(o[2], s);
+  o[3] = coef2<real>(r * (-o[3].x - s * o[3].y), r * (-o[3].y + s * o[3].x));
+  dft_small<real, 4>(e, s);
+  dft_small<real, 4>(o, s);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    x[2 * m] = e[m];
+    x[2 * m + 1] = o[m];
+  }
+}
+
 // one Stockham stage of radix R over the group's lines (line stride kp in
 // LDS): butterfly i of a line reads src[i + q K/R], applies the twiddles
 // w^(q k K/(pR)) with k = i mod p, and writes dst[(i - k) R + k + q p]
@@ -1816,7 +1837,8 @@ __device__ __forceinline__ void fft_stage(const Coef2<real>* src, Coef2<real>* d
       w.y *= -s;  // tw holds exp(-2 pi i m / K): conjugated for the backward sign
       x[q] = cmul<real>(in[i + q * T], coef2<real>(w.x, w.y));
     }
-    dft_small<real, R>(x, s);
+    if constexpr (R == 8) dft8<real>(x, s);
+    else dft_small<real, R>(x, s);
     const int j = (i - k) * R + k;
 #pragma unroll
     for (int q = 0; q < R; ++q) out[j + q * p] = x[q];
@@ -1829,10 +1851,11 @@ __device__ int fft_lines_lds(Coef2<real>* b0, Coef2<real>* b1, int cur, const Co
                              int tid, int tpl_shift, real s) {
   int rem = K, p = 1;
   while (rem > 1) {
-    const int R = rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2 : rem % 3 == 0 ? 3 : 5;
+    const int R = rem % 8 == 0 ? 8 : rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2 : rem % 3 == 0 ? 3 : 5;
     const Coef2<real>* src = cur ? b1 : b0;
     Coef2<real>* dst = cur ? b0 : b1;
     switch (R) {
+      case 8: fft_stage<real, 8>(src, dst, tw, K, kp, p, nl, tid, tpl_shift, s); break;
       case 4: fft_stage<real, 4>(src, dst, tw, K, kp, p, nl, tid, tpl_shift, s); break;
       case 2: fft_stage<real, 2>(src, dst, tw, K, kp, p, nl, tid, tpl_shift, s); break;
       case 3: fft_stage<real, 3>(src, dst, tw, K, kp, p, nl, tid, tpl_shift, s); break;
