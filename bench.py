@@ -11,7 +11,8 @@ resident system.  Legs:
 on the dominant kernel (the real-space matvec) with HIP events on the
 library's stream; cpu_baseline times the FP64 CPU oracle on the host cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serve]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serve] [--no-stream]
+       python bench.py --serve-only | --kernel-probe    (profiling aids, no JSON line)
 """
 import argparse
 import json
@@ -48,13 +49,13 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "matvec_traffic_water23559.md")
 
 
 def kernel_probe(thp):
-    """Launch the matvec, spreading and gather of the mixed 24k system 10
-    times each, then one device-to-device copy of 256 MiB whose counter
-    readings calibrate FETCH_SIZE and WRITE_SIZE."""
+    """Launch the matvec, spreading, gather and permanent-field pass of the
+    mixed 24k system 10 times each, then one device-to-device copy of 256 MiB
+    whose counter readings calibrate FETCH_SIZE and WRITE_SIZE."""
     import ctypes as C
     ctx = thp.Context(0)
     sv = thp.Solver(ctx, thp.water_23559(), thp.THP_PREC_MIXED)
-    for k, name in ((0, "matvec"), (1, "spread"), (2, "gather")):
+    for k, name in ((0, "matvec"), (1, "spread"), (2, "gather"), (3, "efld0")):
         ms, pairs = sv.time_kernel(k, reps=10)
         log(f"probe: {name} {1e3 * ms:.1f} us per launch")
     hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
@@ -113,6 +114,8 @@ def main():
     ap.add_argument("--copies", type=int, default=16, help="resident 24k systems in the serving leg")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="only launch the path kernels and a copy of known size (for rocprofv3 --pmc runs)")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only run the serving leg and print its figures (for kernel profiles of the batched loop)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -124,6 +127,14 @@ def main():
 
     if args.kernel_probe:
         kernel_probe(thp)
+        return
+    if args.serve_only:
+        import thp.serve as tserve
+        ctx = thp.Context(0)
+        serve = tserve.bench_serving(ctx, thp.water_23559(), steps=max(args.steps, 8), warmup=args.warmup,
+                                     poleps=args.poleps, copies=args.copies, iters_per_graph=4, use_graphs=1)
+        log(f"bench: serve {serve}")
+        ctx.close()
         return
 
     system = thp.water_23559()

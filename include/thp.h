@@ -147,7 +147,8 @@ thp_status thp_induce_dev(thp_sys* sys, const thp_solve_opts* opts, double* mu_d
 /* Measurement hook, no reference counterpart: average device time (ms) of
  * one launch of a path kernel over `reps` back-to-back launches, timed with
  * HIP events on the library's stream.  kernel: 0 = real-space matvec
- * (tmatxb), 1 = induced-dipole spreading, 2 = induced-dipole gather.  Also
+ * (tmatxb), 1 = induced-dipole spreading, 2 = induced-dipole gather,
+ * 3 = direct permanent-field pass (efld0 with the matvec's lists).  Also
  * returns the number of unscaled pairs within the cutoff (i<k), the unit
  * of the matvec's algorithmic FLOP count. */
 thp_status thp_time_kernel(thp_sys* sys, int kernel, int reps, double* ms_per_launch, int64_t* pairs_in_cut);

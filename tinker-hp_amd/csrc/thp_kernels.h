@@ -404,43 +404,13 @@ __device__ __forceinline__ void efld0_pair(const SysView& sv, int k, real xr, re
   }
 }
 
-// -E at i due to the dipoles u_k of both rhs (efi -= dlocal . u_k of
-// tmatxb_pme, newinduce_pme.f:1548-1608), added to acc[0..5].
-template <typename real>
-__device__ __forceinline__ void tmatxb_pair(const SysView& sv, const double* in, int k, real xr, real yr, real zr,
-                                            real pdi, real thi, real usc, double* acc) {
-  const size_t np = sv.npad;
-  const real r2 = xr * xr + yr * yr + zr * zr;
-  const real rr1 = rsqrt_t(r2);
-  const real r = r2 * rr1;
-  const real rr2 = rr1 * rr1;
-  const real rr3 = rr2 * rr1;
-  const real rr5 = real(3) * rr2 * rr3;
-  real bn[2];
-  ewald_bn<real, 2>(r, r2, real(sv.aewald), rr1, bn);
-  real l[3];
-  thole_l<real, 0>(r, pdi, real(sv.pdamp[k]), thi, real(sv.thole[k]), l);
-  const real dmp3 = bn[0] - (real(1) - usc * l[0]) * rr3;
-  const real dmp5 = bn[1] - (real(1) - usc * l[1]) * rr5;
-#pragma unroll
-  for (int rhs = 0; rhs < 2; ++rhs) {
-    const real ux = real(in[(size_t)(rhs * 3 + 0) * np + k]);
-    const real uy = real(in[(size_t)(rhs * 3 + 1) * np + k]);
-    const real uz = real(in[(size_t)(rhs * 3 + 2) * np + k]);
-    const real ur = dmp5 * (xr * ux + yr * uy + zr * uz);
-    acc[rhs * 3 + 0] += double(dmp3 * ux - ur * xr);
-    acc[rhs * 3 + 1] += double(dmp3 * uy - ur * yr);
-    acc[rhs * 3 + 2] += double(dmp3 * uz - ur * zr);
-  }
-}
-
-// Scaled (excluded) pairs of atom i, evaluated in FP64 in every precision
-// mode: their bn - (1 - s*lambda)/r^n terms cancel strongly at bonded
-// distances.  Mirrors the reference's separate correction pass over
-// scaled pairs (GPU/source/tmatxb_pmecu.f:568-706, tmatxb_pmegpu.f:1435-1545).
-template <int KIND>
-__device__ __forceinline__ void scaled_pairs(const SysView& sv, const double* in, int i, double xi, double yi,
-                                             double zi, double* acc) {
+// Scaled (excluded) pairs of atom i in the permanent field, evaluated in
+// FP64 in every precision mode: their bn - (1 - s*lambda)/r^n terms cancel
+// strongly at bonded distances.  Mirrors the reference's separate
+// correction pass over scaled pairs (GPU/source/tmatxb_pmecu.f:568-706).
+// Also writes each scaled pair's matvec coefficients (u-scale) to scoef,
+// zero beyond the cutoff; the matvec applies them in FP64.
+__device__ __forceinline__ void scaled_pairs(const SysView& sv, int i, double xi, double yi, double zi, double* acc) {
   const int e0 = sv.excl_off[i], e1 = sv.excl_off[i + 1];
   const double pdi = sv.pdamp[i], thi = sv.thole[i];
   for (int e = e0; e < e1; ++e) {
@@ -449,22 +419,18 @@ __device__ __forceinline__ void scaled_pairs(const SysView& sv, const double* in
     const double yr = image_exact(sv.y[k] - yi, sv.box[1]);
     const double zr = image_exact(sv.z[k] - zi, sv.box[2]);
     if (!(r2_exact(xr, yr, zr) <= sv.cut2)) {
-      if (KIND == 0) sv.scoef[2 * e] = sv.scoef[2 * e + 1] = 0.0;
+      sv.scoef[2 * e] = sv.scoef[2 * e + 1] = 0.0;
       continue;
     }
     const double* sc = sv.scale_tab + sv.excl_code[e] * 3;
-    if (KIND == 0) efld0_pair<double>(sv, k, xr, yr, zr, pdi, thi, sc[0], sc[1], acc, sc[2], sv.scoef + 2 * e);
-    else tmatxb_pair<double>(sv, in, k, xr, yr, zr, pdi, thi, sc[2], acc);
+    efld0_pair<double>(sv, k, xr, yr, zr, pdi, thi, sc[0], sc[1], acc, sc[2], sv.scoef + 2 * e);
   }
 }
 
-// Scaled pairs of every atom of the launch, added to the output of the
-// unscaled pass that precedes it (KIND 0: ef, both rhs; KIND 1: the matvec
-// output vector vout of input vin).  A kernel of its own keeps the FP64
-// damping code out of the register budget of the pair kernels.
-template <int KIND>
-__global__ void __launch_bounds__(64) k_scaled(const SysView* views, const Work* work, int vin, int vout,
-                                               int gate) {
+// Scaled pairs of every atom of the launch, added to the permanent field
+// of the unscaled pass that precedes it.  A kernel of its own keeps the
+// FP64 damping code out of the register budget of the pair kernel.
+__global__ void __launch_bounds__(64) k_efld0_scaled(const SysView* views, const Work* work, int gate) {
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
@@ -473,11 +439,10 @@ __global__ void __launch_bounds__(64) k_scaled(const SysView* views, const Work*
   if (i >= sv.n) return;
   if (sv.excl_off[i] == sv.excl_off[i + 1]) return;
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  scaled_pairs<KIND>(sv, vsel(sv, vin), i, sv.x[i], sv.y[i], sv.z[i], acc);
-  double* out = KIND == 0 ? sv.ef : vsel(sv, vout);
+  scaled_pairs(sv, i, sv.x[i], sv.y[i], sv.z[i], acc);
   const size_t np = sv.npad;
 #pragma unroll
-  for (int c = 0; c < 6; ++c) out[(size_t)c * np + i] += acc[c];
+  for (int c = 0; c < 6; ++c) sv.ef[(size_t)c * np + i] += acc[c];
 }
 
 // ===================================================== block-union lists
@@ -1196,24 +1161,13 @@ __device__ __forceinline__ int bspline_axis(double xc, double rc, int nfft, real
 }
 
 
-// zero a system's grid (a fill kernel, safe inside captured graphs)
-template <typename real>
-__global__ void __launch_bounds__(256) k_zero_grid(const SysView* views, int sys, size_t count) {
-  real* g = static_cast<real*>(views[sys].grid);
-  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < count; q += (size_t)gridDim.x * blockDim.x)
-    g[q] = real(0);
-}
-
 // ---------------------------------------------------------------------
-// Owner-computes spreading.  The grid is cut into bricks of B[d] <= 8
-// points per axis (every brick at least 4 wide, so a 5-point B-spline
-// support spans at most two consecutive bricks).  Once per solve each atom
-// gets its first grid node g0 and its B-spline values theta (3 levels x 5
-// nodes x 3 axes) and is binned by the brick that holds g0; bins are sorted
-// by atom index.  A spreading pass then runs one wavefront per brick: lane
-// (py,pz) owns one x-pencil of the brick, walks the atoms of the 8 bins
-// that can reach the brick in a fixed order and accumulates its pencil in
-// registers, then stores it.  No atomics, no prior zeroing, deterministic.
+// Bricks and bins.  The grid is cut into bricks of B[d] <= 8 points per
+// axis (every brick at least 4 wide, so a 5-point B-spline support spans
+// at most two consecutive bricks).  Once per solve each atom gets its first
+// grid node g0 and its B-spline values theta (3 levels x 5 nodes x 3 axes)
+// and is binned by the brick that holds g0; bins are sorted by atom index.
+// Spreading and gathering then work brick by brick (below).
 constexpr int kTheta = 48;  // per-atom theta row: [axis][level][node] = 3*3*5 = 45, padded
 
 template <typename real>
@@ -1339,194 +1293,6 @@ __global__ void __launch_bounds__(256) k_pme_binsort(const SysView* views, const
       __syncthreads();
     }
   for (int q = threadIdx.x; q < cnt; q += blockDim.x) sv.bin_list[s0 + q] = key[q];
-}
-
-// One workgroup of 4 wavefronts per brick.  KIND 0: grid_uind_site
-// (pmestuff.f:280-400), fractional d dipoles into the real part and p
-// dipoles into the imaginary part.  KIND 1: grid_mpole_site
-// (pmestuff.f:163-274) after cmp_to_fmp_site, real part only (imaginary
-// part cleared).  The atoms of each bin are staged in LDS 64 at a time
-// (one coalesced round trip per 64 atoms); wavefront v accumulates the
-// staged atoms v, v+4, ... into its lanes' pencils, and the four partial
-// pencils are summed in LDS in a fixed order before the store.
-constexpr int kSpreadWaves = 4;
-template <typename real, int KIND>
-__global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_brick(const SysView* views, const Work* bwork, int vin,
-                                                                     int frac_in, int gate) {
-  constexpr int NL = KIND == 0 ? 2 : 3;   // spline levels used
-  constexpr int NF = KIND == 0 ? 6 : 10;  // moments per atom
-  constexpr int NT = 3 * NL * kOrder;
-  __shared__ int s_lx[64], s_ly[64], s_lz[64];
-  __shared__ real s_th[64][NT];
-  __shared__ real s_f[64][NF];
-  __shared__ real s_red[kSpreadWaves - 1][2][8][64];
-  const Work w = bwork[blockIdx.x];
-  if (w.sys < 0) return;
-  const SysView& sv = views[w.sys];
-  if (gated_out(sv.st, gate)) return;
-  const int nb0 = sv.nbrick[0], nb1 = sv.nbrick[1], nb2 = sv.nbrick[2];
-  const int tb = w.blk;
-  const int t0 = tb % nb0, t1 = (tb / nb0) % nb1, t2 = tb / (nb0 * nb1);
-  const int K0 = sv.nfft[0], K1 = sv.nfft[1], K2 = sv.nfft[2];
-  const int B0 = sv.brick[0], B1 = sv.brick[1], B2 = sv.brick[2];
-  const int sx = t0 * B0, sy = t1 * B1, sz = t2 * B2;
-  const int ex = t0 == nb0 - 1 ? K0 : sx + B0;
-  const int ey = t1 == nb1 - 1 ? K1 : sy + B1;
-  const int ez = t2 == nb2 - 1 ? K2 : sz + B2;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int py = lane & 7, pz = lane >> 3;
-  const bool own = sy + py < ey && sz + pz < ez;
-  const size_t np = sv.npad;
-  const GLOBAL double* IN = gp(vsel(sv, vin));
-  const GLOBAL double* RP = gp(sv.rpole);
-  const GLOBAL real* THETA = gp(static_cast<const real*>(sv.theta));
-  const GLOBAL int* G0 = gp(sv.g0);
-  const GLOBAL int* BS = gp(sv.bin_start);
-  const GLOBAL int* BLIST = gp(sv.bin_list);
-  const double af0 = frac_in ? 1.0 : sv.afrac[0], af1 = frac_in ? 1.0 : sv.afrac[1], af2 = frac_in ? 1.0 : sv.afrac[2];
-  double ctf[10][10];
-  if (KIND == 1) {
-#pragma unroll
-    for (int a = 0; a < 10; ++a)
-#pragma unroll
-      for (int b = 0; b < 10; ++b) ctf[a][b] = sv.ctf[a][b];
-  }
-  real acc0[8], acc1[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) acc0[q] = acc1[q] = real(0);
-  for (int bz = 0; bz < 2; ++bz)
-    for (int by = 0; by < 2; ++by)
-      for (int bx = 0; bx < 2; ++bx) {
-        const int hb0 = (t0 - bx + nb0) % nb0, hb1 = (t1 - by + nb1) % nb1, hb2 = (t2 - bz + nb2) % nb2;
-        const int hb = (hb2 * nb1 + hb1) * nb0 + hb0;
-        const int a0 = BS[hb], a1 = BS[hb + 1];
-        for (int c0 = a0; c0 < a1; c0 += 64) {
-          const int cnt = min(64, a1 - c0);
-          __syncthreads();
-          if (threadIdx.x < cnt) {
-            const int i = BLIST[c0 + lane];
-            const int gx = G0[4 * i], gy = G0[4 * i + 1], gz = G0[4 * i + 2];
-            // local offset of the first spline node relative to the brick start;
-            // bins of the previous brick give [-B,-1] (wrapped at the box edge)
-            int lx = gx - sx, ly = gy - sy, lz = gz - sz;
-            if (bx && lx > 0) lx -= K0;
-            if (by && ly > 0) ly -= K1;
-            if (bz && lz > 0) lz -= K2;
-            s_lx[lane] = lx;
-            s_ly[lane] = ly;
-            s_lz[lane] = lz;
-            const GLOBAL real* th = THETA + (size_t)i * kTheta;
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-#pragma unroll
-              for (int l = 0; l < NL; ++l)
-#pragma unroll
-                for (int m = 0; m < kOrder; ++m) s_th[lane][(d * NL + l) * kOrder + m] = th[th_idx(d, l, m)];
-            if (KIND == 0) {
-              s_f[lane][0] = real(af0 * IN[0 * np + i]);
-              s_f[lane][1] = real(af1 * IN[1 * np + i]);
-              s_f[lane][2] = real(af2 * IN[2 * np + i]);
-              s_f[lane][3] = real(af0 * IN[3 * np + i]);
-              s_f[lane][4] = real(af1 * IN[4 * np + i]);
-              s_f[lane][5] = real(af2 * IN[5 * np + i]);
-            } else {
-              double cmp[10];
-              cmp[0] = RP[0 * np + i];
-              cmp[1] = RP[1 * np + i];
-              cmp[2] = RP[2 * np + i];
-              cmp[3] = RP[3 * np + i];
-              cmp[4] = RP[4 * np + i];
-              cmp[5] = RP[7 * np + i];
-              cmp[6] = RP[9 * np + i];
-              cmp[7] = 2.0 * RP[5 * np + i];
-              cmp[8] = 2.0 * RP[6 * np + i];
-              cmp[9] = 2.0 * RP[8 * np + i];
-              s_f[lane][0] = real(ctf[0][0] * cmp[0]);
-#pragma unroll
-              for (int j = 1; j < 4; ++j) {
-                double s = 0.0;
-#pragma unroll
-                for (int kk = 1; kk < 4; ++kk) s += ctf[j][kk] * cmp[kk];
-                s_f[lane][j] = real(s);
-              }
-#pragma unroll
-              for (int j = 4; j < 10; ++j) {
-                double s = 0.0;
-#pragma unroll
-                for (int kk = 4; kk < 10; ++kk) s += ctf[j][kk] * cmp[kk];
-                s_f[lane][j] = real(s);
-              }
-            }
-          }
-          __syncthreads();
-          for (int q = wv; q < cnt; q += kSpreadWaves) {
-            const int lx = s_lx[q], ly = s_ly[q], lz = s_lz[q];
-            if (lx < -4 || ly < -4 || lz < -4) continue;  // support ends before this brick
-            const int my = py - ly, mz = pz - lz;
-            if (!(own && my >= 0 && my < kOrder && mz >= 0 && mz < kOrder)) continue;
-            const real* th = s_th[q];
-            const real* f = s_f[q];
-            if (KIND == 0) {
-              const real u0 = th[(1 * NL + 0) * kOrder + my], u1 = th[(1 * NL + 1) * kOrder + my];
-              const real v0 = th[(2 * NL + 0) * kOrder + mz], v1 = th[(2 * NL + 1) * kOrder + mz];
-              const real term01 = f[1] * u1 * v0 + f[2] * u0 * v1;
-              const real term11 = f[0] * u0 * v0;
-              const real term02 = f[4] * u1 * v0 + f[5] * u0 * v1;
-              const real term12 = f[3] * u0 * v0;
-#pragma unroll
-              for (int xq = 0; xq < 8; ++xq) {
-                const int m = xq - lx;
-                if (m >= 0 && m < kOrder) {
-                  const real ta = th[m], tb1 = th[kOrder + m];
-                  acc0[xq] += term01 * ta + term11 * tb1;
-                  acc1[xq] += term02 * ta + term12 * tb1;
-                }
-              }
-            } else {
-              const real u0 = th[(1 * NL + 0) * kOrder + my], u1 = th[(1 * NL + 1) * kOrder + my],
-                         u2 = th[(1 * NL + 2) * kOrder + my];
-              const real v0 = th[(2 * NL + 0) * kOrder + mz], v1 = th[(2 * NL + 1) * kOrder + mz],
-                         v2 = th[(2 * NL + 2) * kOrder + mz];
-              const real term0 =
-                  f[0] * u0 * v0 + f[2] * u1 * v0 + f[3] * u0 * v1 + f[5] * u2 * v0 + f[6] * u0 * v2 + f[9] * u1 * v1;
-              const real term1 = f[1] * u0 * v0 + f[7] * u1 * v0 + f[8] * u0 * v1;
-              const real term2 = f[4] * u0 * v0;
-#pragma unroll
-              for (int xq = 0; xq < 8; ++xq) {
-                const int m = xq - lx;
-                if (m >= 0 && m < kOrder)
-                  acc0[xq] += term0 * th[m] + term1 * th[kOrder + m] + term2 * th[2 * kOrder + m];
-              }
-            }
-          }
-        }
-      }
-  // fixed-order sum of the four wavefronts' pencils
-  if (wv > 0) {
-#pragma unroll
-    for (int xq = 0; xq < 8; ++xq) {
-      s_red[wv - 1][0][xq][lane] = acc0[xq];
-      s_red[wv - 1][1][xq][lane] = acc1[xq];
-    }
-  }
-  __syncthreads();
-  if (wv > 0 || !own) return;
-#pragma unroll
-  for (int v = 0; v < kSpreadWaves - 1; ++v)
-#pragma unroll
-    for (int xq = 0; xq < 8; ++xq) {
-      acc0[xq] += s_red[v][0][xq][lane];
-      acc1[xq] += s_red[v][1][xq][lane];
-    }
-  GLOBAL real* row = gw(static_cast<real*>(sv.grid)) + ((size_t)(sz + pz) * K1 + (sy + py)) * K0 * 2;
-#pragma unroll
-  for (int xq = 0; xq < 8; ++xq) {
-    const int x = sx + xq;
-    if (x < ex) {
-      row[2 * x] = acc0[xq];
-      row[2 * x + 1] = acc1[xq];
-    }
-  }
 }
 
 // ---------------------------------------------------------------------
@@ -1942,133 +1708,15 @@ __global__ void __launch_bounds__(256) k_fft_axis(const SysView* views, const Wo
   }
 }
 
-// fphi_uind_site first derivatives (pmestuff.f:746-1037, fdip_phi(2:4) of
-// both grids) and the fractional->Cartesian field (newinduce_pme.f:2103-2110).
-// 32 lanes share one atom: lane r < 25 reads stencil row (mj, mk) = (r%5,
-// r/5), five consecutive complex points, and the six row sums are reduced
-// over the 32 lanes by butterfly shuffles.  The sums are FP64.
-constexpr int kGatherAtoms = 8;  // atoms per 256-thread block
-
-template <typename real>
-__device__ __forceinline__ void gather_uind_rows(const GLOBAL real* grid, const GLOBAL real* theta,
-                                                 const GLOBAL int* g0, int n1, int n2, int n3, int i, int r,
-                                                 double a[6]) {
-#pragma unroll
-  for (int c = 0; c < 6; ++c) a[c] = 0.0;
-  if (r >= kOrder * kOrder) return;
-  const int mj = r % kOrder, mk = r / kOrder;
-  const int gx = g0[4 * i], gy = g0[4 * i + 1], gz = g0[4 * i + 2];
-  const GLOBAL real* th = theta + (size_t)i * kTheta;
-  int k = gz + mk;
-  if (k >= n3) k -= n3;
-  int j = gy + mj;
-  if (j >= n2) j -= n2;
-  const GLOBAL real* row = grid + ((size_t)k * n2 + j) * n1 * 2;
-  real t0_1 = 0, t1_1 = 0, t0_2 = 0, t1_2 = 0;
-#pragma unroll
-  for (int mi = 0; mi < kOrder; ++mi) {
-    int ii = gx + mi;
-    if (ii >= n1) ii -= n1;
-    const real q1 = row[2 * ii], q2 = row[2 * ii + 1];
-    const real ta = th[th_idx(0, 0, mi)], tb = th[th_idx(0, 1, mi)];
-    t0_1 += q1 * ta;
-    t1_1 += q1 * tb;
-    t0_2 += q2 * ta;
-    t1_2 += q2 * tb;
-  }
-  const real u0 = th[th_idx(1, 0, mj)], u1 = th[th_idx(1, 1, mj)];
-  const real v0 = th[th_idx(2, 0, mk)], v1 = th[th_idx(2, 1, mk)];
-  a[0] = double(t1_1 * u0 * v0);
-  a[1] = double(t0_1 * u1 * v0);
-  a[2] = double(t0_1 * u0 * v1);
-  a[3] = double(t1_2 * u0 * v0);
-  a[4] = double(t0_2 * u1 * v0);
-  a[5] = double(t0_2 * u0 * v1);
-}
-
-// MODE 0: write the reciprocal dipole field of vin into vout (frac_out: the
-//         fractional derivatives fdip_phi(2:4) instead).
-// MODE 1: PCG start (newinduce_pme2.f:406-438): h = hreal + dip - term*mu,
-//         res = ef - h, zr = diag*res, pp = zr; partial sums of res.zr
-//         (ggold of the first iteration) into rz partial buffer 0.
-// MODE 2: PCG iteration (:452-473): h = hreal + dip - term*pp; partial
-//         sums of pp.h (gg).
-// Launched on the gather work table: block (sys, g) of G = gat_blocks
-// blocks handles atoms g*8 + a, stepping by 8*G.
-template <typename real, int MODE>
-__global__ void __launch_bounds__(256) k_gather_uind(const SysView* views, const Work* work, int vin, int vout,
-                                                      int frac_out) {
-  const Work w = work[blockIdx.x];
-  if (w.sys < 0) return;
-  const SysView& sv = views[w.sys];
-  if (MODE == 2 && sv.st->conv) return;
-  const int r = threadIdx.x & 31;
-  const int G = sv.gat_blocks;
-  const size_t np = sv.npad;
-  const GLOBAL real* GRID = gp(static_cast<const real*>(sv.grid));
-  const GLOBAL real* THETA = gp(static_cast<const real*>(sv.theta));
-  const GLOBAL int* G0 = gp(sv.g0);
-  const int n1 = sv.nfft[0], n2 = sv.nfft[1], n3 = sv.nfft[2];
-  double q[2] = {0.0, 0.0};
-  for (int base = w.blk * kGatherAtoms; base < sv.n; base += G * kGatherAtoms) {
-    const int i = base + (threadIdx.x >> 5);
-    double a[6] = {0, 0, 0, 0, 0, 0};
-    if (i < sv.n) gather_uind_rows<real>(GRID, THETA, G0, n1, n2, n3, i, r, a);
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-#pragma unroll
-      for (int off = 16; off > 0; off >>= 1) a[c] += __shfl_xor(a[c], off, 32);
-    }
-    if (r == 0 && i < sv.n) {
-      double dip[6];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const double sc = (MODE == 0 && frac_out) ? 1.0 : sv.afrac[c];
-        dip[c] = sc * a[c];
-        dip[3 + c] = sc * a[3 + c];
-      }
-      if (MODE == 0) {
-        double* out = vsel(sv, vout);
-#pragma unroll
-        for (int c = 0; c < 6; ++c) out[(size_t)c * np + i] = dip[c];
-      } else {
-        const GLOBAL double* IN = gp(vsel(sv, vin));
-        const double term = sv.term_self;
-        const double pl = gp(sv.pol)[i];
-        const double diag = pl == 0.0 ? kTinyPol : pl;
-        const GLOBAL double* HR = gp(sv.hreal);
-        GLOBAL double* H = gw(sv.h);
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-          const size_t o = (size_t)c * np + i;
-          const double u = IN[o];
-          const double h = HR[o] + dip[c] - term * u;
-          H[o] = h;
-          if (MODE == 1) {
-            const double rr = gp(sv.ef)[o] - h;
-            const double z = diag * rr;
-            gw(sv.res)[o] = rr;
-            gw(sv.zr)[o] = z;
-            gw(sv.pp)[o] = z;
-            q[c / 3] += rr * z;
-          } else {
-            q[c / 3] += u * h;
-          }
-        }
-      }
-    }
-  }
-  if (MODE == 0) return;
-  block_partial2(q[0], q[1], MODE == 1 ? part_init(sv) : part_gg(sv), w.blk);
-}
-
 // Tile gather: fphi_uind_site first derivatives (pmestuff.f:746-1037,
 // fdip_phi(2:4) of both grids) and the fractional->Cartesian field
 // (newinduce_pme.f:2103-2110).  One wavefront per brick loads the brick's
 // (B+4)^3 tile of the grid into LDS once; each lane then gathers one home
 // atom's 5x5x5 stencil from LDS (rows summed in the grid type, rows
-// accumulated in FP64).  MODE as in k_gather_uind; the partial sums are
-// one pair per brick.
+// accumulated in FP64).  MODE 0 writes the field (fdip_phi or, with
+// frac_out, the fractional derivatives); MODE 1 also starts the PCG
+// (res = ef - h, zr, pp, newinduce_pme2.f:406-438) and MODE 2 also forms
+// p.h (newinduce_pme2.f:452-473), with one partial pair per brick.
 template <typename real, int MODE>
 __global__ void __launch_bounds__(64) k_gather_tile(const SysView* views, const Work* bwork, int vin, int vout,
                                                      int frac_out, int gate) {
