@@ -70,6 +70,19 @@ def test_field_and_matvec_entry_points_golden(ctx, prec, tol):
     assert rel(solver.tmatxbrecip(g["mu_probe"]), g["tmatxbrecip"]) < tol
 
 
+# grid lengths whose transforms take every radix of the grid FFT (8, 4, 2,
+# 3, 5), odd lengths and a different length per axis
+@pytest.mark.parametrize("nfft", [(30, 30, 30), (36, 36, 36), (45, 45, 45), (50, 50, 50), (30, 36, 45)])
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-11), (MIXED, 2e-5)])
+def test_reciprocal_terms_on_mixed_radix_grids(ctx, nfft, prec, tol):
+    s = thp.make_system(699, 19.5, seed=13, nfft=nfft, name=f"water-699-{nfft}")
+    orc = O.Oracle(s)
+    solver = thp.Solver(ctx, s, prec)
+    mu = np.random.default_rng(2).normal(size=(s.n, 2, 3)) * 0.05
+    assert rel(solver.tmatxbrecip(mu), orc.tmatxbrecip(mu)) < tol
+    assert rel(solver.efld0_recip(), orc.efld0_recip()) < tol
+
+
 @pytest.mark.parametrize("prec,tol", [(FP64, 1e-11), (MIXED, 2e-5)])
 def test_field_and_matvec_water1944(ctx, prec, tol):
     s = thp.water_1944()
