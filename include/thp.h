@@ -36,6 +36,10 @@
  *   thp_induce ................. newinduce_pme2 + inducepcg_pme2
  *                                (v1.2/source/newinduce_pme2.f:16-636;
  *                                 GPU/source/newinduce_pme2gpu.f:17-1137)
+ *   thp_set_predictor .......... polpred keyword + ulspred and the dipole
+ *                                history (v1.2/source/mdinit.f:93,185-190;
+ *                                 v1.2/source/newinduce_pme2.f:165-178,
+ *                                 253-264, 910-1010; kpolar.f:405-427)
  */
 #ifndef THP_H
 #define THP_H
@@ -60,6 +64,12 @@ typedef int thp_status;
 
 #define THP_PREC_FP64 0       /* FP64 pair, grid, FFT and vectors ("plain") */
 #define THP_PREC_MIXED 1      /* FP32 pair/grid/FFT, FP64 accumulation and CG */
+
+/* induced-dipole predictor (polpred) */
+#define THP_PRED_NONE 0
+#define THP_PRED_GEAR 1       /* Gear binomial coefficients */
+#define THP_PRED_ASPC 2       /* always stable predictor-corrector (Kolafa 2004) */
+#define THP_PRED_LSQR 3       /* least-squares fit of the stored solves */
 
 typedef struct thp_ctx thp_ctx;
 typedef struct thp_sys thp_sys;
@@ -137,6 +147,12 @@ thp_status thp_pme_convolve(thp_sys* sys);
 
 thp_status thp_induce(thp_sys* sys, const thp_solve_opts* opts, double* uind, double* uinp,
                       double* ep, thp_solve_stats* stats);
+/* Select the induced-dipole predictor (THP_PRED_*) and clear its history.
+ * While a predictor is set, every solve of the system stores its converged
+ * dipoles (the last 7 are kept); once 7 are stored, the next solve starts
+ * from their extrapolation instead of opts->guess, as use_pred does in the
+ * reference. */
+thp_status thp_set_predictor(thp_sys* sys, int method);
 
 /* device-pointer variants: mu, efi, ef, uind... are device addresses */
 thp_status thp_tmatxb_dev(thp_sys* sys, const double* mu, double* efi, int dodiag);

@@ -185,3 +185,78 @@ def dampthole(pdi, pdk, pgamma, r, order=7):
     d = np.zeros(9)
     load().orc_damptholed(pdi, pdk, pgamma, r, order, _d(d))
     return d
+
+
+# ------------------------------------------------------------ dipole predictor
+# Gear binomial and ASPC (Kolafa 2004) coefficients of kpolar.f:405-423,
+# newest history entry first.
+PRED_COEF = {
+    "GEAR": (6.0, -15.0, 20.0, -15.0, 6.0, -1.0, 0.0),
+    "ASPC": (22.0 / 7.0, -55.0 / 14.0, 55.0 / 21.0, -22.0 / 21.0, 5.0 / 21.0, -1.0 / 42.0, 0.0),
+}
+MAXUALT = 7  # MOD_uprior.f:34
+
+
+def ldlt_solve(a, b):
+    """x with a x = b for symmetric positive definite a through the
+    symmetric decomposition a = L D L^T, L unit lower triangular (cholesky.f,
+    Martin, Peters and Wilkinson 1965)."""
+    a = np.array(a, dtype=np.float64)
+    n = len(b)
+    L = np.eye(n)
+    d = np.zeros(n)
+    for i in range(n):
+        for j in range(i):
+            s = a[i, j]
+            for k in range(j):
+                s -= L[i, k] * d[k] * L[j, k]
+            L[i, j] = s / d[j]
+        s = a[i, i]
+        for k in range(i):
+            s -= L[i, k] * L[i, k] * d[k]
+        d[i] = s
+    y = np.array(b, dtype=np.float64)
+    for i in range(n):
+        for k in range(i):
+            y[i] -= L[i, k] * y[k]
+    x = y / d
+    for i in reversed(range(n)):
+        for k in range(i + 1, n):
+            x[i] -= L[k, i] * x[k]
+    return x
+
+
+class Predictor:
+    """History of converged dipoles and their extrapolation into the next
+    solve's guess: ulspred (newinduce_pme2.f:910-1010), its use
+    (newinduce_pme2.f:165-178) and the history update (:253-264).  The guess
+    applies the coefficients fitted on the d dipoles (bpred) to both d and p,
+    as the reference does."""
+
+    def __init__(self, method: str):
+        assert method in ("GEAR", "ASPC", "LSQR")
+        self.method = method
+        self.hist = []  # newest first, each (n, 2, 3)
+
+    def push(self, mu):
+        self.hist.insert(0, np.array(mu, dtype=np.float64).reshape(-1, 2, 3))
+        del self.hist[MAXUALT:]
+
+    def coefficients(self):
+        if self.method in PRED_COEF:
+            return np.array(PRED_COEF[self.method])
+        # least-squares fit of the newest d dipoles on the six older ones
+        u = [h[:, 0, :].reshape(-1) for h in self.hist]
+        c = np.array([[float(np.dot(u[k], u[m])) for m in range(MAXUALT)] for k in range(MAXUALT)])
+        b = c[0, 1:]
+        a = c[1:, 1:]
+        coef = np.zeros(MAXUALT)
+        coef[:-1] = ldlt_solve(a, b) if np.max(np.triu(a)) != 0.0 else b
+        return coef
+
+    def predict(self):
+        """Guess (n, 2, 3), or None while fewer than seven solves are stored."""
+        if len(self.hist) < MAXUALT:
+            return None
+        b = self.coefficients()
+        return sum(b[k] * self.hist[k] for k in range(MAXUALT - 1))

@@ -239,3 +239,57 @@ def test_cg_energy_functional_decreases():
     e = res["ene_hist"][:, 0]
     assert np.all(np.diff(e) <= 1e-9 * np.abs(e).max())
     assert res["status"] == 0 and res["niter"] == 22
+
+
+# ------------------------------------------------------------ dipole predictor
+def test_ldlt_solve_matches_dense_solve():
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 5, 6):
+        m = rng.normal(size=(n, n))
+        a = m @ m.T + n * np.eye(n)
+        b = rng.normal(size=n)
+        np.testing.assert_allclose(O.ldlt_solve(a, b), np.linalg.solve(a, b), rtol=1e-12, atol=1e-14)
+
+
+def _poly_history(degree, t, n=40, seed=4, h=0.25):
+    """dipoles u(s) = sum_j a_j s^j with random vectors a_j at the steps
+    s = (t-1) h .. (t-7) h (newest first), and u(t h)"""
+    rng = np.random.default_rng(seed)
+    a = rng.normal(size=(degree + 1, n, 2, 3))
+    u = lambda s: sum(a[j] * s ** j for j in range(degree + 1))
+    return [u((t - k) * h) for k in range(1, O.MAXUALT + 1)], u(t * h)
+
+
+def test_gear_predictor_is_exact_for_quintic_histories():
+    # the binomial coefficients are the sixth finite difference: polynomials
+    # of degree 5 are extrapolated exactly
+    hist, target = _poly_history(5, 0.3)
+    p = O.Predictor("GEAR")
+    for h in reversed(hist):
+        p.push(h)
+    np.testing.assert_allclose(p.predict(), target, rtol=1e-9, atol=1e-9)
+    assert O.Predictor("GEAR").predict() is None  # nothing stored yet
+
+
+def test_aspc_predictor_is_consistent_and_exact_for_linear_histories():
+    b = np.array(O.PRED_COEF["ASPC"])
+    k = np.arange(1, O.MAXUALT + 1)
+    assert abs(b.sum() - 1.0) < 1e-15 and abs((k * b).sum()) < 1e-14  # zeroth and first moments
+    hist, target = _poly_history(1, 2.0)
+    p = O.Predictor("ASPC")
+    for h in reversed(hist):
+        p.push(h)
+    np.testing.assert_allclose(p.predict(), target, rtol=1e-12, atol=1e-12)
+
+
+def test_lsqr_predictor_recovers_the_gear_recurrence():
+    # a degree-5 history satisfies the sixth-difference recurrence exactly,
+    # so the least-squares fit of the newest entry on the six older ones is
+    # the Gear coefficient set
+    hist, target = _poly_history(5, 1.0)
+    p = O.Predictor("LSQR")
+    for h in reversed(hist):
+        p.push(h)
+    np.testing.assert_allclose(p.coefficients()[:6], O.PRED_COEF["GEAR"][:6], rtol=1e-7)
+    assert p.coefficients()[6] == 0.0
+    np.testing.assert_allclose(p.predict(), target, rtol=1e-8, atol=1e-8)

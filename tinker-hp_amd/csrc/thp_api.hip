@@ -1326,6 +1326,32 @@ namespace thp {
 // One solve of one system: the reference reads the residual norm back on
 // the host after every iteration (newinduce_pme2gpu.f:1040-1041), and so
 // does this one-system path.
+// guess of the next solve from the stored solves (newinduce_pme2.f:165-178):
+// the least-squares coefficients are fitted on the device first
+thp_status predict_guess(thp_sys* s) {
+  thp_ctx* ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  if (s->pred == THP_PRED_LSQR) {
+    hipLaunchKernelGGL(k_pred_normal, g1(s->pred_blocks), dim3(256), 0, st, s->d_view, s->pred_hist, s->pred_newest,
+                       s->pred_part);
+    hipLaunchKernelGGL(k_pred_solve, g1(1), dim3(64), 0, st, s->pred_part, s->pred_blocks, s->pred_coef);
+  }
+  const int nb = (int)std::min<size_t>(1024, ((size_t)6 * s->npad + 255) / 256);
+  hipLaunchKernelGGL(k_pred_apply, g1(nb), dim3(256), 0, st, s->d_view, s->pred_hist, s->pred_coef, s->pred_newest);
+  return launch_check(ctx, "k_pred_apply");
+}
+
+// store the solve's dipoles as the newest history entry (newinduce_pme2.f:253-264)
+thp_status push_history(thp_sys* s) {
+  const int slot = s->pred_n == 0 ? 0 : (s->pred_newest + 1) % kMaxUalt;
+  const int nb = (int)std::min<size_t>(1024, ((size_t)6 * s->npad + 255) / 256);
+  hipLaunchKernelGGL(k_pred_push, g1(nb), dim3(256), 0, s->ctx->stream, s->d_view, s->pred_hist, slot);
+  RET(launch_check(s->ctx, "k_pred_push"));
+  s->pred_newest = slot;
+  s->pred_n = std::min(s->pred_n + 1, kMaxUalt);
+  return THP_OK;
+}
+
 thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_stats* stats) {
   hipStream_t st = s->ctx->stream;
   thp_ctx* ctx = s->ctx;
@@ -1333,6 +1359,12 @@ thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_s
   HIPCHK(ctx, hipEventRecord(s->ev[0], st));
   RET(ensure_lists(s));
   RET(ensure_pme(s));
+  // with a predictor and a full history the extrapolated guess replaces
+  // opts->guess, as use_pred precedes polgsf in the reference
+  if (s->pred != THP_PRED_NONE && s->pred_n == kMaxUalt) {
+    RET(predict_guess(s));
+    warm = true;
+  }
   RET(b_init_state(ctx, b, o->poleps, o->politer, warm ? s->d_one : nullptr, nullptr));
   RET(b_setup(ctx, b, kGateNone));
   s->coef_valid = true;
@@ -1345,6 +1377,7 @@ thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_s
     if (conv) break;
   }
   RET(b_finish(ctx, b, 0));
+  if (s->pred != THP_PRED_NONE) RET(push_history(s));
   HIPCHK(ctx, hipEventRecord(s->ev[2], st));
   SolveState fin;
   HIPCHK(ctx, hipMemcpyAsync(&fin, s->view.st, sizeof fin, hipMemcpyDeviceToHost, st));
@@ -1367,6 +1400,27 @@ thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_s
 }  // namespace thp
 
 extern "C" {
+
+thp_status thp_set_predictor(thp_sys* s, int method) {
+  using namespace thp;
+  if (!s) return THP_ERR_ARG;
+  if (method < THP_PRED_NONE || method > THP_PRED_LSQR) return fail(s->ctx, THP_ERR_ARG, "unknown predictor %d", method);
+  if (method != THP_PRED_NONE && !s->pred_hist) {
+    s->pred_blocks = (s->n + 255) / 256;
+    RET(dalloc(s, &s->pred_hist, (size_t)kMaxUalt * 6 * s->npad));
+    RET(dalloc(s, &s->pred_coef, (size_t)kMaxUalt));
+    RET(dalloc(s, &s->pred_part, (size_t)s->pred_blocks * kPredPairs));
+  }
+  // Gear binomial and ASPC coefficients, newest entry first (kpolar.f:405-423)
+  static const double gear[kMaxUalt] = {6.0, -15.0, 20.0, -15.0, 6.0, -1.0, 0.0};
+  static const double aspc[kMaxUalt] = {22.0 / 7.0, -55.0 / 14.0, 55.0 / 21.0, -22.0 / 21.0, 5.0 / 21.0, -1.0 / 42.0, 0.0};
+  if (method == THP_PRED_GEAR || method == THP_PRED_ASPC)
+    HIPCHK(s->ctx, hipMemcpy(s->pred_coef, method == THP_PRED_GEAR ? gear : aspc, sizeof gear, hipMemcpyHostToDevice));
+  s->pred = method;
+  s->pred_n = 0;
+  s->pred_newest = 0;
+  return THP_OK;
+}
 
 thp_status thp_induce(thp_sys* s, const thp_solve_opts* o, double* uind, double* uinp, double* ep,
                       thp_solve_stats* stats) {

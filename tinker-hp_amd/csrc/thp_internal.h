@@ -13,6 +13,8 @@
 namespace thp {
 
 constexpr int kBlock = 64;  // atoms per block = one wavefront
+constexpr int kMaxUalt = 7;  // stored solves of the dipole predictor (MOD_uprior.f:34)
+constexpr int kPredPairs = kMaxUalt * (kMaxUalt + 1) / 2;
 constexpr int kPack = 8;    // compacted list entries per lane pack (16 B of indices)
 constexpr int kOrder = 5;   // PME B-spline order handled by the kernels
 constexpr double kCoulomb = 332.063714;  // v1.2/source/MOD_units.f:81
@@ -208,6 +210,14 @@ struct thp_sys {
   std::vector<int> iperm;  // original index -> sorted slot
   int* iperm_dev = nullptr;
   double* cphi = nullptr;  // [10][npad] scratch for efld0_recip
+  // dipole predictor (thp_set_predictor)
+  int pred = 0;                  // THP_PRED_*
+  int pred_n = 0;                // stored solves, at most kMaxUalt (nualt)
+  int pred_newest = 0;           // ring slot of the newest stored solve
+  double* pred_hist = nullptr;   // [kMaxUalt][6][npad]
+  double* pred_coef = nullptr;   // [kMaxUalt], newest entry first
+  double* pred_part = nullptr;   // [pred_blocks][kPredPairs]
+  int pred_blocks = 0;
   std::vector<double> hx, hy, hz;       // sorted-order coordinates (host copy)
   std::vector<double> bx0, by0, bz0;    // coordinates at the last block-list build
   std::vector<double> ux0, uy0, uz0;    // coordinates at the last block-union build

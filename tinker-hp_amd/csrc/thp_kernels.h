@@ -1933,6 +1933,131 @@ __global__ void __launch_bounds__(64) k_gather_mpole(const SysView* views, const
   }
 }
 
+// ===================================================== dipole predictor
+// History of the converged dipoles of the last kMaxUalt solves and its
+// extrapolation into the next guess (ulspred, newinduce_pme2.f:910-1010;
+// the guess, :165-178; the history update, :253-264).  hist holds
+// kMaxUalt slots of [6][npad] FP64 dipoles used as a ring; entry k counts
+// back from the newest (k = 0).
+
+// copy the solve's dipoles into history slot `slot`
+__global__ void __launch_bounds__(256) k_pred_push(const SysView* views, double* hist, int slot) {
+  const SysView& sv = views[0];
+  const size_t m = (size_t)6 * sv.npad;
+  const GLOBAL double* src = gp(sv.mu);
+  GLOBAL double* dst = gw(hist) + (size_t)slot * m;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (size_t)gridDim.x * blockDim.x)
+    dst[q] = src[q];
+}
+
+// mu = sum_k coef[k] entry k over the kMaxUalt - 1 newest entries, the same
+// coefficients for d and p (newinduce_pme2.f:170-176)
+__global__ void __launch_bounds__(256) k_pred_apply(const SysView* views, const double* hist, const double* coef,
+                                                    int newest) {
+  const SysView& sv = views[0];
+  const size_t m = (size_t)6 * sv.npad;
+  double c[kMaxUalt - 1];
+#pragma unroll
+  for (int k = 0; k < kMaxUalt - 1; ++k) c[k] = coef[k];
+  const GLOBAL double* H = gp(hist);
+  GLOBAL double* MU = gw(sv.mu);
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (size_t)gridDim.x * blockDim.x) {
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < kMaxUalt - 1; ++k) v += c[k] * H[(size_t)((newest - k + kMaxUalt) % kMaxUalt) * m + q];
+    MU[q] = v;
+  }
+}
+
+// least-squares predictor, normal equations: per 256-atom block, the sums
+// c(k, m) = u_k . u_m over the block's atoms and the three d components,
+// 0 <= k <= m < kMaxUalt, packed by rows of the upper triangle
+__global__ void __launch_bounds__(256) k_pred_normal(const SysView* views, const double* hist, int newest,
+                                                     double* part) {
+  const SysView& sv = views[0];
+  const size_t m = (size_t)6 * sv.npad;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double acc[kPredPairs];
+#pragma unroll
+  for (int p = 0; p < kPredPairs; ++p) acc[p] = 0.0;
+  if (i < sv.n) {
+    const GLOBAL double* H = gp(hist);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double u[kMaxUalt];
+#pragma unroll
+      for (int k = 0; k < kMaxUalt; ++k) u[k] = H[(size_t)((newest - k + kMaxUalt) % kMaxUalt) * m + (size_t)c * sv.npad + i];
+      int p = 0;
+#pragma unroll
+      for (int k = 0; k < kMaxUalt; ++k)
+#pragma unroll
+        for (int q = k; q < kMaxUalt; ++q) acc[p++] += u[k] * u[q];
+    }
+  }
+  __shared__ double red[kPredPairs][4];
+#pragma unroll
+  for (int p = 0; p < kPredPairs; ++p) {
+    const double v = wave_sum(acc[p]);
+    if ((threadIdx.x & 63) == 0) red[p][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kPredPairs)
+    part[(size_t)blockIdx.x * kPredPairs + threadIdx.x] =
+        ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
+}
+
+// the fit of the newest entry on the kMaxUalt - 1 older ones: normal
+// equations summed over the block partials in block order, solved by the
+// symmetric decomposition A = L D L^T of cholesky.f (Martin, Peters and
+// Wilkinson 1965) when some A(k, m) is positive (newinduce_pme2.f:986-997);
+// coef[kMaxUalt - 1] = 0
+__global__ void __launch_bounds__(64) k_pred_solve(const double* part, int nblocks, double* coef) {
+  __shared__ double c[kPredPairs];
+  for (int p = threadIdx.x; p < kPredPairs; p += 64) {
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * kPredPairs + p];
+    c[p] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  constexpr int N = kMaxUalt - 1;
+  auto C = [&](int k, int q) {
+    if (k > q) {
+      const int t = k;
+      k = q;
+      q = t;
+    }
+    return c[k * kMaxUalt - k * (k - 1) / 2 + (q - k)];
+  };
+  double A[N][N], x[N];
+  double amax = 0.0;
+  for (int j = 0; j < N; ++j) {
+    x[j] = C(0, j + 1);
+    for (int l = 0; l < N; ++l) A[j][l] = C(j + 1, l + 1);
+    for (int l = j; l < N; ++l) amax = fmax(amax, A[j][l]);
+  }
+  if (amax != 0.0) {
+    double L[N][N], D[N];
+    for (int r = 0; r < N; ++r) {
+      for (int j = 0; j < r; ++j) {
+        double s = A[r][j];
+        for (int k = 0; k < j; ++k) s -= L[r][k] * D[k] * L[j][k];
+        L[r][j] = s / D[j];
+      }
+      double s = A[r][r];
+      for (int k = 0; k < r; ++k) s -= L[r][k] * L[r][k] * D[k];
+      D[r] = s;
+    }
+    for (int r = 0; r < N; ++r)
+      for (int k = 0; k < r; ++k) x[r] -= L[r][k] * x[k];
+    for (int r = 0; r < N; ++r) x[r] /= D[r];
+    for (int r = N - 1; r >= 0; --r)
+      for (int k = r + 1; k < N; ++k) x[r] -= L[k][r] * x[k];
+  }
+  for (int j = 0; j < N; ++j) coef[j] = x[j];
+  coef[N] = 0.0;
+}
+
 // ===================================================== PCG vector kernels
 // Reductions: each reducing kernel writes one pair of FP64 partial sums
 // per block (G = red_blocks blocks per system); the kernels that need a

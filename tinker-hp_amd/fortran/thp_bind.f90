@@ -120,6 +120,11 @@ module thp_bind
       real(c_double), intent(out) :: ep
       type(thp_solve_stats), intent(out) :: stats
     end function
+    integer(c_int) function thp_set_predictor(sys, method) bind(C, name="thp_set_predictor")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sys
+      integer(c_int), value :: method
+    end function
     integer(c_int) function thp_make_system(n, box, nsol, radius, seed, x, y, z, rpole, polarity, &
                                             thole, nscaled, ik, dpu) bind(C, name="thp_make_system")
       import :: c_int, c_double, c_int64_t

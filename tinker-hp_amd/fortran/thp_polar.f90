@@ -169,6 +169,26 @@ contains
     if (st%status /= THP_OK) write (*, '(a,i0)') ' NEWINDUCE_PME2  --  Warning, PCG not converged, status ', st%status
   end subroutine
 
+  ! the POLAR-PREDICT keyword (mdinit.f:185-190): 'NONE', 'GEAR', 'ASPC'
+  ! or 'LSQR'; the stored solves are cleared
+  subroutine polar_set_predictor(polpred)
+    character(len=*), intent(in) :: polpred
+    integer(c_int) :: m
+    select case (polpred)
+    case ('NONE')
+      m = 0
+    case ('GEAR')
+      m = 1
+    case ('ASPC')
+      m = 2
+    case ('LSQR')
+      m = 3
+    case default
+      error stop 'polar_set_predictor: unknown predictor'
+    end select
+    call fatal_status(thp_set_predictor(sys, m), 'thp_set_predictor')
+  end subroutine
+
   subroutine polar_finalize
     if (c_associated(sys)) call fatal_status(thp_system_destroy(sys), 'thp_system_destroy')
     if (c_associated(ctx)) call fatal_status(thp_finalize(ctx), 'thp_finalize')

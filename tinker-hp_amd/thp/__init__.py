@@ -31,6 +31,8 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libthp.so")
 THP_OK = 0
 THP_PREC_FP64 = 0
 THP_PREC_MIXED = 1
+# induced-dipole predictors (polpred)
+THP_PRED_NONE, THP_PRED_GEAR, THP_PRED_ASPC, THP_PRED_LSQR = 0, 1, 2, 3
 ERRORS = {
     -1: "THP_ERR_ARG", -2: "THP_ERR_HIP", -3: "THP_ERR_FFT", -4: "THP_ERR_NOMEM",
     -5: "THP_ERR_LIST", -6: "THP_ERR_NOCONV", -7: "THP_ERR_NAN", -8: "THP_ERR_STATE",
@@ -130,6 +132,7 @@ def load_lib(path: str | None = None) -> C.CDLL:
         "thp_fphi_uind": (C.c_int, [vp, _dp, _dp, _dp]),
         "thp_pme_convolve": (C.c_int, [vp]),
         "thp_induce": (C.c_int, [vp, C.POINTER(SolveOpts), _dp, _dp, _dp, C.POINTER(SolveStats)]),
+        "thp_set_predictor": (C.c_int, [vp, C.c_int]),
         "thp_tmatxb_dev": (C.c_int, [vp, vp, vp, C.c_int]),
         "thp_induce_dev": (C.c_int, [vp, C.POINTER(SolveOpts), vp, _dp, C.POINTER(SolveStats)]),
         "thp_serve_start": (C.c_int, [vp, C.POINTER(ServeOpts)]),
@@ -387,6 +390,10 @@ class Solver:
 
     def pme_convolve(self):
         self._c(self.lib.thp_pme_convolve(self.h), "pme_convolve")
+
+    def set_predictor(self, method: int):
+        """THP_PRED_NONE/GEAR/ASPC/LSQR; clears the stored solves."""
+        self._c(self.lib.thp_set_predictor(self.h, int(method)), "set_predictor")
 
     def newinduce_pme2(self, poleps=1e-8, politer=500, mu0=None):
         """Induced dipoles uind/uinp (n,3) each, E_pol (kcal/mol) and stats."""
