@@ -36,6 +36,10 @@
  *   thp_induce ................. newinduce_pme2 + inducepcg_pme2
  *                                (v1.2/source/newinduce_pme2.f:16-636;
  *                                 GPU/source/newinduce_pme2gpu.f:17-1137)
+ *   thp_set_solver ............. POLAR-ALG keyword (v1.2/source/prmkey.f:396-398)
+ *                                selecting inducepcg_pme2 or inducejac_pme2
+ *                                (v1.2/source/newinduce_pme2.f:210-217,
+ *                                 638-890; diis.f)
  *   thp_set_predictor .......... polpred keyword + ulspred and the dipole
  *                                history (v1.2/source/mdinit.f:93,185-190;
  *                                 v1.2/source/newinduce_pme2.f:165-178,
@@ -64,6 +68,10 @@ typedef int thp_status;
 
 #define THP_PREC_FP64 0       /* FP64 pair, grid, FFT and vectors ("plain") */
 #define THP_PREC_MIXED 1      /* FP32 pair/grid/FFT, FP64 accumulation and CG */
+
+/* induced-dipole solver (polalg) */
+#define THP_POLALG_PCG 1          /* preconditioned conjugate gradient (default) */
+#define THP_POLALG_JACOBI_DIIS 2  /* Jacobi iterations with DIIS extrapolation */
 
 /* induced-dipole predictor (polpred) */
 #define THP_PRED_NONE 0
@@ -147,6 +155,10 @@ thp_status thp_pme_convolve(thp_sys* sys);
 
 thp_status thp_induce(thp_sys* sys, const thp_solve_opts* opts, double* uind, double* uinp,
                       double* ep, thp_solve_stats* stats);
+/* Select the solver of thp_induce (THP_POLALG_*).  Jacobi/DIIS stops when
+ * the Jacobi increment max_rhs sqrt(|munew - mu|^2 / 3n) < poleps and keeps
+ * up to 25 subspace vectors on the device (2.4 kB per atom). */
+thp_status thp_set_solver(thp_sys* sys, int polalg);
 /* Select the induced-dipole predictor (THP_PRED_*) and clear its history.
  * While a predictor is set, every solve of the system stores its converged
  * dipoles (the last 7 are kept); once 7 are stored, the next solve starts

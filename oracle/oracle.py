@@ -260,3 +260,69 @@ class Predictor:
             return None
         b = self.coefficients()
         return sum(b[k] * self.hist[k] for k in range(MAXUALT - 1))
+
+
+# ------------------------------------------------------------ Jacobi / DIIS
+NDISMX = 25  # inducejac_pme2 (newinduce_pme2.f:664)
+
+
+def jacobi_diis(hfun, ef, alpha, poleps, politer, mu0=None):
+    """Jacobi iterations mu <- alpha (ef - h(mu)) with DIIS extrapolation:
+    inducejac_pme2 (newinduce_pme2.f:638-890) with diis, makeb and extrap
+    (diis.f), the subspace restarting from its ten newest vectors when it
+    reaches NDISMX.  The small linear system goes to numpy's LAPACK gesv,
+    the routine the reference calls.  Vectors are (n, 2, 3); the norm is
+    sqrt(sum over one rhs / (3 n)) as in the reference."""
+    shape = ef.shape
+    n = shape[0]
+    mu = alpha * ef if mu0 is None else np.array(mu0, dtype=np.float64).reshape(shape)
+    xd, ed = [], []  # subspace vectors, oldest first
+    bmat = np.zeros((NDISMX + 1, NDISMX + 1))
+    nmat = 1
+    rr = np.inf
+    it = 0
+    restarts = 0
+    for it in range(1, politer + 1):
+        munew = alpha * (ef - hfun(mu))
+        rnorm = ((munew - mu) ** 2).sum(axis=(0, 2))
+        xd.append(munew.reshape(-1).copy())
+        ed.append((munew - mu).reshape(-1))
+        if nmat >= NDISMX:  # restart: drop the ten oldest vectors
+            bmat[1:nmat - 10, 1:nmat - 10] = bmat[11:nmat, 11:nmat].copy()
+            xd, ed = xd[10:], ed[10:]
+            nmat -= 10
+            restarts += 1
+        # makeb for vector nmat (1-based), Lagrange row and column 0
+        if nmat == 1:
+            bmat[0, 0] = 0.0
+            bmat[0, 1] = bmat[1, 0] = 1.0
+            bmat[1, 1] = float(np.dot(ed[0], ed[0]))
+        else:
+            bmat[nmat, 0] = bmat[0, nmat] = 1.0
+            for j in range(1, nmat):
+                bmat[nmat, j] = bmat[j, nmat] = float(np.dot(ed[j - 1], ed[nmat - 1]))
+            bmat[nmat, nmat] = float(np.dot(ed[nmat - 1], ed[nmat - 1]))
+        nmat += 1
+        rhs = np.zeros(nmat)
+        rhs[0] = 1.0
+        cex = np.linalg.solve(bmat[:nmat, :nmat], rhs)
+        mu = sum(cex[j + 1] * xd[j] for j in range(nmat - 1)).reshape(shape)
+        rr = float(np.max(np.sqrt(rnorm / (3.0 * n))))
+        if rr < poleps:
+            break
+    return mu, it, rr, restarts
+
+
+def induce_jacobi_diis(orc, poleps=1e-8, politer=500, mu0=None):
+    """The reference's polalg = 2 solve of one system: h(mu) = -E_real(mu)
+    without the diagonal + the reciprocal dipole field - term mu
+    (newinduce_pme2.f:739-755).  Returns dict(mu, ep, niter, resnrm)."""
+    s = orc.sys
+    ef = orc.efld0_total()
+    alpha = np.asarray(s.polarity, dtype=np.float64)[:, None, None]
+    term = (4.0 / 3.0) * s.aewald ** 3 / np.sqrt(np.pi)
+    hfun = lambda mu: orc.tmatxb_pme(mu, dodiag=False) + orc.tmatxbrecip(mu) - term * mu
+    mu, it, rr, _ = jacobi_diis(hfun, ef, alpha, poleps, politer, mu0)
+    f = 332.063714  # electric (MOD_units.f:81)
+    ep = -0.5 * f * float(np.sum(mu[:, 0, :] * ef[:, 1, :]))
+    return dict(mu=mu, ep=ep, niter=it, resnrm=rr)

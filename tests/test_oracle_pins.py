@@ -293,3 +293,50 @@ def test_lsqr_predictor_recovers_the_gear_recurrence():
     np.testing.assert_allclose(p.coefficients()[:6], O.PRED_COEF["GEAR"][:6], rtol=1e-7)
     assert p.coefficients()[6] == 0.0
     np.testing.assert_allclose(p.predict(), target, rtol=1e-8, atol=1e-8)
+
+
+# ------------------------------------------------------------ Jacobi / DIIS
+def test_jacobi_diis_first_step_is_the_jacobi_iterate():
+    # one vector: B = [[0, 1], [1, e.e]] gives c = (-e.e, 1), so the
+    # extrapolation returns the Jacobi iterate itself
+    rng = np.random.default_rng(8)
+    n = 12
+    T = rng.normal(size=(6 * n, 6 * n)) * 0.02
+    T = -(T + T.T) / 2
+    alpha = rng.uniform(0.5, 1.5, size=(n, 1, 1))
+    ef = rng.normal(size=(n, 2, 3))
+    hfun = lambda mu: (T @ mu.reshape(-1)).reshape(n, 2, 3)
+    mu1, it, _, _ = O.jacobi_diis(hfun, ef, alpha, poleps=1e300, politer=5)
+    assert it == 1
+    np.testing.assert_allclose(mu1, alpha * (ef - hfun(alpha * ef)), rtol=1e-14, atol=1e-15)
+
+
+def test_jacobi_diis_with_restarts_matches_dense_solve():
+    # slowly converging Jacobi map (spectral radius near 1): the subspace
+    # fills up and restarts several times on the way
+    rng = np.random.default_rng(0)
+    n = 30
+    M = rng.normal(size=(6 * n, 6 * n)) * 0.05
+    T = -(M + M.T) / 2
+    alpha = np.ones((n, 1, 1))
+    ef = rng.normal(size=(n, 2, 3))
+    hfun = lambda mu: (T @ mu.reshape(-1)).reshape(n, 2, 3)
+    mu, it, rr, restarts = O.jacobi_diis(hfun, ef, alpha, poleps=1e-12, politer=300)
+    assert restarts >= 2 and rr < 1e-12
+    ref = np.linalg.solve(np.eye(6 * n) + T, ef.reshape(-1)).reshape(n, 2, 3)
+    assert np.abs(mu - ref).max() < 1e-10
+
+
+def test_jacobi_diis_solution_matches_dense_solve():
+    s = thp.tiny_water(27, 20.3, 24)
+    orc = O.Oracle(s)
+    n6 = 6 * s.n
+    A = np.zeros((n6, n6))
+    e = np.zeros(n6)
+    for j in range(n6):
+        e[:] = 0.0
+        e[j] = 1.0
+        A[:, j] = orc.matvec(e.reshape(s.n, 2, 3)).reshape(-1)
+    mu = np.linalg.solve(A, orc.efld0_total().reshape(-1)).reshape(s.n, 2, 3)
+    jd = O.induce_jacobi_diis(orc, poleps=1e-14, politer=200)
+    assert np.abs(jd["mu"] - mu).max() < 1e-12 * np.abs(mu).max()

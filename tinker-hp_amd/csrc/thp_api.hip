@@ -617,10 +617,15 @@ thp_status b_pair_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int do
 
 // Field, guess and first residual (newinduce_pme2.f:111-196, 406-440);
 // the solve states are initialised and the B-spline tables current.
-thp_status b_setup(thp_ctx* ctx, const Batch& b, int gate) {
+// permanent field and the starting dipoles (newinduce_pme2.f:111-196)
+thp_status b_setup_field(thp_ctx* ctx, const Batch& b, int gate) {
   RET(field(ctx, b, nullptr, 1, 1, gate));
   hipLaunchKernelGGL(k_guess, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, gate);
-  RET(launch_check(ctx, "k_guess"));
+  return launch_check(ctx, "k_guess");
+}
+
+thp_status b_setup(thp_ctx* ctx, const Batch& b, int gate) {
+  RET(b_setup_field(ctx, b, gate));
   RET(b_pair_tmatxb(ctx, b, 0 /*mu*/, 3 /*hreal*/, 1, gate));
   RET(spread_uind(ctx, b, 0, 0, gate));
   RET(convolve(ctx, b, gate));
@@ -1352,6 +1357,36 @@ thp_status push_history(thp_sys* s) {
   return THP_OK;
 }
 
+// One Jacobi/DIIS iteration (inducejac_pme2, newinduce_pme2.f:731-846)
+// with the host's mirror of the subspace bookkeeping of diis.f: the new
+// vector is stored at subspace position nmat, and a full subspace drops its
+// ten oldest vectors first.
+thp_status jacobi_iteration(thp_sys* s, int it, int* nmat, int* first) {
+  thp_ctx* ctx = s->ctx;
+  const Batch& b = s->one;
+  hipStream_t st = ctx->stream;
+  RET(b_pair_tmatxb(ctx, b, V_MU, V_HREAL, 0, kGateNone));
+  RET(spread_uind(ctx, b, V_MU, 0, kGateNone));
+  RET(convolve(ctx, b, kGateNone));
+  RET(launch_gather_uind<0>(ctx, b, V_MU, V_H, 0, kGateNone));
+  const int slot = (*first + *nmat - 1) % kDiisMax;
+  int shifted = 0;
+  if (*nmat >= kDiisMax) {
+    *first = (*first + 10) % kDiisMax;
+    *nmat -= 10;
+    shifted = 1;
+  }
+  const int nvec = *nmat;
+  hipLaunchKernelGGL(k_jac_step, g1(s->diis_blocks), dim3(256), 0, st, s->d_view, s->diis_x, s->diis_e, slot, *first,
+                     nvec, s->diis_part);
+  hipLaunchKernelGGL(k_jac_solve, g1(1), dim3(64), 0, st, s->d_view, s->diis, s->diis_part, s->diis_blocks, nvec, shifted,
+                     it);
+  const int nb = (int)std::min<size_t>(1024, ((size_t)6 * s->npad + 255) / 256);
+  hipLaunchKernelGGL(k_jac_extrap, g1(nb), dim3(256), 0, st, s->d_view, s->diis_x, s->diis, *first, nvec);
+  *nmat += 1;
+  return launch_check(ctx, "k_jac_extrap");
+}
+
 thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_stats* stats) {
   hipStream_t st = s->ctx->stream;
   thp_ctx* ctx = s->ctx;
@@ -1366,12 +1401,15 @@ thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_s
     warm = true;
   }
   RET(b_init_state(ctx, b, o->poleps, o->politer, warm ? s->d_one : nullptr, nullptr));
-  RET(b_setup(ctx, b, kGateNone));
+  const bool jacobi = s->polalg == THP_POLALG_JACOBI_DIIS;
+  if (jacobi) RET(b_setup_field(ctx, b, kGateNone));
+  else RET(b_setup(ctx, b, kGateNone));
   s->coef_valid = true;
   HIPCHK(ctx, hipEventRecord(s->ev[1], st));
-  int conv = 0;
+  int conv = 0, nmat = 1, first = 0;
   for (int it = 1; it <= o->politer; ++it) {
-    RET(b_iteration(ctx, b));
+    if (jacobi) RET(jacobi_iteration(s, it, &nmat, &first));
+    else RET(b_iteration(ctx, b));
     HIPCHK(ctx, hipMemcpyAsync(&conv, &s->view.st->conv, sizeof(int), hipMemcpyDeviceToHost, st));
     RET(sync(s));
     if (conv) break;
@@ -1400,6 +1438,22 @@ thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_s
 }  // namespace thp
 
 extern "C" {
+
+thp_status thp_set_solver(thp_sys* s, int polalg) {
+  using namespace thp;
+  if (!s) return THP_ERR_ARG;
+  if (polalg != THP_POLALG_PCG && polalg != THP_POLALG_JACOBI_DIIS)
+    return fail(s->ctx, THP_ERR_ARG, "unknown polalg %d (1: PCG, 2: Jacobi/DIIS)", polalg);
+  if (polalg == THP_POLALG_JACOBI_DIIS && !s->diis) {
+    s->diis_blocks = (s->n + 255) / 256;
+    RET(dalloc(s, &s->diis_x, (size_t)kDiisMax * 6 * s->npad));
+    RET(dalloc(s, &s->diis_e, (size_t)kDiisMax * 6 * s->npad));
+    RET(dalloc(s, &s->diis_part, (size_t)s->diis_blocks * kDiisParts));
+    RET(dalloc(s, &s->diis, 1));
+  }
+  s->polalg = polalg;
+  return THP_OK;
+}
 
 thp_status thp_set_predictor(thp_sys* s, int method) {
   using namespace thp;

@@ -15,6 +15,8 @@ namespace thp {
 constexpr int kBlock = 64;  // atoms per block = one wavefront
 constexpr int kMaxUalt = 7;  // stored solves of the dipole predictor (MOD_uprior.f:34)
 constexpr int kPredPairs = kMaxUalt * (kMaxUalt + 1) / 2;
+constexpr int kDiisMax = 25;              // DIIS subspace (ndismx, newinduce_pme2.f:664)
+constexpr int kDiisParts = 2 + kDiisMax;  // per-block partials of a Jacobi step
 constexpr int kPack = 8;    // compacted list entries per lane pack (16 B of indices)
 constexpr int kOrder = 5;   // PME B-spline order handled by the kernels
 constexpr double kCoulomb = 332.063714;  // v1.2/source/MOD_units.f:81
@@ -29,6 +31,13 @@ enum : int { kRunning = 0, kConverged = 1, kMaxIter = 2, kNonFinite = 3, kZeroCu
 enum : int { kGateNone = 0, kGateIter = 1, kGateSetup = 2 };
 
 // Device-resident state of one PCG solve (all scalars FP64 on the device).
+// Pulay's matrix (Lagrange row and column 0) and the extrapolation
+// coefficients of the Jacobi/DIIS solver
+struct DiisState {
+  double B[kDiisMax + 1][kDiisMax + 1];
+  double cex[kDiisMax + 1];
+};
+
 struct SolveState {
   double ggold[2];
   double gg[2];
@@ -218,6 +227,13 @@ struct thp_sys {
   double* pred_coef = nullptr;   // [kMaxUalt], newest entry first
   double* pred_part = nullptr;   // [pred_blocks][kPredPairs]
   int pred_blocks = 0;
+  // solver (thp_set_solver): 1 PCG, 2 Jacobi/DIIS
+  int polalg = 1;
+  double* diis_x = nullptr;     // [kDiisMax][6][npad] Jacobi iterates
+  double* diis_e = nullptr;     // [kDiisMax][6][npad] their increments
+  double* diis_part = nullptr;  // [diis_blocks][kDiisParts]
+  thp::DiisState* diis = nullptr;
+  int diis_blocks = 0;
   std::vector<double> hx, hy, hz;       // sorted-order coordinates (host copy)
   std::vector<double> bx0, by0, bz0;    // coordinates at the last block-list build
   std::vector<double> ux0, uy0, uz0;    // coordinates at the last block-union build

@@ -2058,6 +2058,170 @@ __global__ void __launch_bounds__(64) k_pred_solve(const double* part, int nbloc
   coef[N] = 0.0;
 }
 
+// ===================================================== Jacobi / DIIS solver
+// Jacobi iterations with DIIS extrapolation, the reference's polalg = 2
+// (inducejac_pme2, newinduce_pme2.f:638-890; diis, makeb and extrap,
+// diis.f).  The subspace's Jacobi iterates x_j and increments e_j live in
+// two device rings of kDiisMax [6][npad] FP64 vectors: subspace vector j
+// (1 = oldest) sits in slot (first + j - 1) mod kDiisMax.  The subspace size
+// and ring start do not depend on the data, so the host tracks them and
+// passes them in; Pulay's matrix and the extrapolation coefficients stay on
+// the device.
+
+// Jacobi step of every atom: h = -E_real(mu) (no diagonal) + the reciprocal
+// dipole field (in sv.h) - term mu, munew = alpha (ef - h)
+// (newinduce_pme2.f:739-760), stored with its increment e = munew - mu in
+// ring slot `slot`.  Per 256-atom block: the increment norms of both rhs
+// and the products e_j . e of the nvec - 1 older subspace vectors and e . e
+// (makeb, diis.f).
+__global__ void __launch_bounds__(256) k_jac_step(const SysView* views, double* xr, double* er, int slot, int first,
+                                                  int nvec, double* part) {
+  const SysView& sv = views[0];
+  const size_t np = sv.npad, m = (size_t)6 * np;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double rn0 = 0.0, rn1 = 0.0;
+  double dots[kDiisMax];
+#pragma unroll
+  for (int j = 0; j < kDiisMax; ++j) dots[j] = 0.0;
+  if (i < sv.n) {
+    const double a = gp(sv.pol)[i], term = sv.term_self;
+    double e[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const size_t o = (size_t)c * np + i;
+      const double u = gp(sv.mu)[o];
+      const double h = gp(sv.hreal)[o] + gp(sv.h)[o] - term * u;
+      const double nw = a * (gp(sv.ef)[o] - h);
+      e[c] = nw - u;
+      gw(xr)[(size_t)slot * m + o] = nw;
+      gw(er)[(size_t)slot * m + o] = e[c];
+    }
+    rn0 = (e[0] * e[0] + e[1] * e[1]) + e[2] * e[2];
+    rn1 = (e[3] * e[3] + e[4] * e[4]) + e[5] * e[5];
+#pragma unroll
+    for (int j = 0; j < kDiisMax - 1; ++j) {
+      if (j < nvec - 1) {
+        const GLOBAL double* ej = gp(er) + (size_t)((first + j) % kDiisMax) * m + i;
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) s += ej[(size_t)c * np] * e[c];
+        dots[j] = s;
+      }
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s += e[c] * e[c];
+#pragma unroll
+    for (int j = 0; j < kDiisMax; ++j)
+      if (j == nvec - 1) dots[j] = s;
+  }
+  __shared__ double red[kDiisParts][4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  {
+    const double v0 = wave_sum(rn0), v1 = wave_sum(rn1);
+    if (lane == 0) {
+      red[0][wv] = v0;
+      red[1][wv] = v1;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kDiisMax; ++j) {
+    if (j < nvec) {
+      const double v = wave_sum(dots[j]);
+      if (lane == 0) red[2 + j][wv] = v;
+    }
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < 2 + nvec; p += 256)
+    part[(size_t)blockIdx.x * kDiisParts + p] = ((red[p][0] + red[p][1]) + red[p][2]) + red[p][3];
+}
+
+// Pulay's matrix and the extrapolation coefficients (diis, makeb: diis.f;
+// the dgesv call of newinduce_pme2.f:807-812), then the convergence test of
+// the Jacobi increment, max over rhs of sqrt(|e|^2 / (3n)) < poleps
+// (:836-846).  One workgroup; the small system is solved by Gaussian
+// elimination with partial pivoting, as LAPACK's dgesv.
+__global__ void __launch_bounds__(64) k_jac_solve(const SysView* views, DiisState* ds, const double* part,
+                                                  int nblocks, int nvec, int shifted, int it) {
+  const SysView& sv = views[0];
+  __shared__ double sums[kDiisParts];
+  __shared__ double A[kDiisMax + 1][kDiisMax + 1], x[kDiisMax + 1];
+  for (int p = threadIdx.x; p < 2 + nvec; p += 64) {
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * kDiisParts + p];
+    sums[p] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double(*B)[kDiisMax + 1] = ds->B;
+  if (shifted) {  // the ten oldest vectors left the subspace
+    for (int j = 1; j < nvec; ++j)
+      for (int k = 1; k < nvec; ++k) B[j][k] = B[j + 10][k + 10];
+  }
+  if (nvec == 1) {
+    B[0][0] = 0.0;
+    B[0][1] = B[1][0] = 1.0;
+    B[1][1] = sums[2];
+  } else {
+    B[nvec][0] = B[0][nvec] = 1.0;
+    for (int j = 1; j < nvec; ++j) B[nvec][j] = B[j][nvec] = sums[2 + j - 1];
+    B[nvec][nvec] = sums[2 + nvec - 1];
+  }
+  const int nm = nvec + 1;
+  for (int r = 0; r < nm; ++r) {
+    for (int k = 0; k < nm; ++k) A[r][k] = B[r][k];
+    x[r] = r == 0 ? 1.0 : 0.0;
+  }
+  for (int k = 0; k < nm; ++k) {
+    int piv = k;
+    for (int r = k + 1; r < nm; ++r)
+      if (fabs(A[r][k]) > fabs(A[piv][k])) piv = r;
+    if (piv != k) {
+      for (int q = 0; q < nm; ++q) {
+        const double t = A[k][q];
+        A[k][q] = A[piv][q];
+        A[piv][q] = t;
+      }
+      const double t = x[k];
+      x[k] = x[piv];
+      x[piv] = t;
+    }
+    for (int r = k + 1; r < nm; ++r) {
+      const double f = A[r][k] / A[k][k];
+      for (int q = k + 1; q < nm; ++q) A[r][q] -= f * A[k][q];
+      x[r] -= f * x[k];
+    }
+  }
+  for (int r = nm - 1; r >= 0; --r) {
+    double s = x[r];
+    for (int q = r + 1; q < nm; ++q) s -= A[r][q] * x[q];
+    x[r] = s / A[r][r];
+  }
+  for (int r = 0; r < nm; ++r) ds->cex[r] = x[r];
+  SolveState* st = sv.st;
+  const double n3 = 3.0 * double(sv.n);
+  const double rr = fmax(sqrt(sums[0] / n3), sqrt(sums[1] / n3));
+  st->niter = it;
+  st->resnrm = rr;
+  if (!(rr == rr) || rr > 1e300) st->conv = kNonFinite;
+  else if (rr < st->poleps) st->conv = kConverged;
+  else if (it >= st->politer) st->conv = kMaxIter;
+}
+
+// mu = sum_j cex[j] x_j over the subspace, oldest first (extrap, diis.f)
+__global__ void __launch_bounds__(256) k_jac_extrap(const SysView* views, const double* xr, const DiisState* ds,
+                                                    int first, int nvec) {
+  const SysView& sv = views[0];
+  const size_t m = (size_t)6 * sv.npad;
+  const GLOBAL double* X = gp(xr);
+  GLOBAL double* MU = gw(sv.mu);
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (size_t)gridDim.x * blockDim.x) {
+    double v = 0.0;
+    for (int j = 1; j <= nvec; ++j) v += ds->cex[j] * X[(size_t)((first + j - 1) % kDiisMax) * m + q];
+    MU[q] = v;
+  }
+}
+
 // ===================================================== PCG vector kernels
 // Reductions: each reducing kernel writes one pair of FP64 partial sums
 // per block (G = red_blocks blocks per system); the kernels that need a

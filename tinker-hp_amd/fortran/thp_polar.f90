@@ -169,6 +169,12 @@ contains
     if (st%status /= THP_OK) write (*, '(a,i0)') ' NEWINDUCE_PME2  --  Warning, PCG not converged, status ', st%status
   end subroutine
 
+  ! the POLAR-ALG keyword (prmkey.f:396-398): 1 PCG, 2 Jacobi/DIIS
+  subroutine polar_set_solver(polalg)
+    integer, intent(in) :: polalg
+    call fatal_status(thp_set_solver(sys, int(polalg, c_int)), 'thp_set_solver')
+  end subroutine
+
   ! the POLAR-PREDICT keyword (mdinit.f:185-190): 'NONE', 'GEAR', 'ASPC'
   ! or 'LSQR'; the stored solves are cleared
   subroutine polar_set_predictor(polpred)

@@ -31,6 +31,8 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libthp.so")
 THP_OK = 0
 THP_PREC_FP64 = 0
 THP_PREC_MIXED = 1
+# induced-dipole solvers (polalg)
+THP_POLALG_PCG, THP_POLALG_JACOBI_DIIS = 1, 2
 # induced-dipole predictors (polpred)
 THP_PRED_NONE, THP_PRED_GEAR, THP_PRED_ASPC, THP_PRED_LSQR = 0, 1, 2, 3
 ERRORS = {
@@ -133,6 +135,7 @@ def load_lib(path: str | None = None) -> C.CDLL:
         "thp_pme_convolve": (C.c_int, [vp]),
         "thp_induce": (C.c_int, [vp, C.POINTER(SolveOpts), _dp, _dp, _dp, C.POINTER(SolveStats)]),
         "thp_set_predictor": (C.c_int, [vp, C.c_int]),
+        "thp_set_solver": (C.c_int, [vp, C.c_int]),
         "thp_tmatxb_dev": (C.c_int, [vp, vp, vp, C.c_int]),
         "thp_induce_dev": (C.c_int, [vp, C.POINTER(SolveOpts), vp, _dp, C.POINTER(SolveStats)]),
         "thp_serve_start": (C.c_int, [vp, C.POINTER(ServeOpts)]),
@@ -390,6 +393,10 @@ class Solver:
 
     def pme_convolve(self):
         self._c(self.lib.thp_pme_convolve(self.h), "pme_convolve")
+
+    def set_solver(self, polalg: int):
+        """THP_POLALG_PCG (1) or THP_POLALG_JACOBI_DIIS (2)."""
+        self._c(self.lib.thp_set_solver(self.h, int(polalg)), "set_solver")
 
     def set_predictor(self, method: int):
         """THP_PRED_NONE/GEAR/ASPC/LSQR; clears the stored solves."""
