@@ -127,6 +127,9 @@ const char* thp_version(void);
 /* ---------------------------------------------------------------- systems */
 thp_status thp_system_create(thp_ctx* ctx, const thp_system_desc* desc, thp_sys** sys);
 thp_status thp_system_update_coords(thp_sys* sys, const double* x, const double* y, const double* z);
+/* new global-frame multipoles rpole [n][13] (as rotpole leaves them each MD
+ * step, v1.2/source/rotpole.f); the next field and solve use them */
+thp_status thp_system_update_multipoles(thp_sys* sys, const double* rpole);
 thp_status thp_system_destroy(thp_sys* sys);
 /* device memory held by the system (bytes) */
 thp_status thp_system_footprint(const thp_sys* sys, int64_t* bytes);
@@ -243,6 +246,11 @@ thp_status thp_serve_add_system(thp_ctx* ctx, const thp_system_desc* desc, int* 
  * requests of one system run in submission order */
 thp_status thp_serve_submit(thp_ctx* ctx, int system_id, const double* x, const double* y,
                             const double* z, uint64_t* ticket);
+/* the same with new global-frame multipoles rpole [n][13] (NULL: unchanged),
+ * for requests that follow a trajectory (the multipoles rotate with the
+ * molecules) */
+thp_status thp_serve_submit_multipoles(thp_ctx* ctx, int system_id, const double* x, const double* y,
+                                       const double* z, const double* rpole, uint64_t* ticket);
 /* 1 if ready (result filled, ticket released), 0 if not yet; wait_ms < 0
  * blocks until ready.  ticket 0 takes any finished request (the oldest). */
 thp_status thp_serve_poll(thp_ctx* ctx, uint64_t ticket, thp_result* result, double wait_ms);

@@ -964,6 +964,15 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
     v.raw = raw;
     v.refu = refu;
     v.perm = pd;
+    v.raw_rpole = nullptr;
+    if (!own_grid) {
+      // serving systems take new multipoles with a request
+      double* rr;
+      ALLOC(rr, (size_t)13 * n);
+      H2D(rr, d->rpole, sizeof(double) * 13 * n);
+      if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "sync"));
+      v.raw_rpole = rr;
+    }
   }
   ALLOC(v.nbcount, np); ALLOC(v.lcount, np); ALLOC(v.tcount, np); ALLOC(v.ucount, s->nblk); ALLOC(v.bbox, (size_t)s->nblk * 6);
   ALLOC(s->bcount, s->nblk);
@@ -1164,6 +1173,24 @@ thp_status thp_system_footprint(const thp_sys* s, int64_t* bytes) {
 thp_status thp_system_update_coords(thp_sys* s, const double* x, const double* y, const double* z) {
   if (!s || !x || !y || !z) return THP_ERR_ARG;
   return update_coords(s, x, y, z, nullptr);
+}
+
+thp_status thp_system_update_multipoles(thp_sys* s, const double* rpole) {
+  if (!s || !rpole) return THP_ERR_ARG;
+  const int n = s->n;
+  const size_t np = s->npad;
+  std::vector<double> rp((size_t)10 * np, 0.0);
+  for (int q = 0; q < n; ++q) {
+    const double* r = rpole + (size_t)s->perm[q] * 13;
+    const double v[10] = {r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[8], r[9], r[12]};
+    for (int c = 0; c < 10; ++c) rp[(size_t)c * np + q] = v[c];
+  }
+  HIPCHK(s->ctx, hipMemcpy(const_cast<double*>(s->view.rpole), rp.data(), sizeof(double) * rp.size(),
+                           hipMemcpyHostToDevice));
+  if (s->view.raw_rpole)
+    HIPCHK(s->ctx, hipMemcpy(const_cast<double*>(s->view.raw_rpole), rpole, sizeof(double) * 13 * n,
+                             hipMemcpyHostToDevice));
+  return THP_OK;
 }
 
 thp_status thp_nblist(thp_sys* s, int force_rebuild) {

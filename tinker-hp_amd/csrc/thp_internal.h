@@ -81,6 +81,7 @@ struct SysView {
   // sorted slot -> caller index map, and the sorted coordinates at the last
   // block-union build [3][npad] with the squared displacement they allow
   const double* raw;
+  const double* raw_rpole;  // [n][13] caller-order multipoles (serving systems), else null
   const int* perm;
   const double* refu;
   double lim_u2;

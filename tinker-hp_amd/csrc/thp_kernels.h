@@ -2436,8 +2436,9 @@ __global__ void k_end_setup(const SysView* views, const int* sys_ids, int nsys) 
   if (st->in_setup) st->in_setup = 0;
 }
 
-// caller-order coordinates -> sorted slots, with the skin check of the
-// block-union lists (stale: the host rebuilds the lists and repeats the set-up)
+// caller-order coordinates (and multipoles, for serving systems) into the
+// sorted slots, with the skin check of the block-union lists (stale: the
+// host rebuilds the lists and repeats the set-up)
 __global__ void __launch_bounds__(64) k_load_coords(const SysView* views, const Work* work, int gate) {
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
@@ -2454,6 +2455,21 @@ __global__ void __launch_bounds__(64) k_load_coords(const SysView* views, const 
   const_cast<double*>(sv.z)[i] = z;
   const double dx = x - sv.refu[i], dy = y - sv.refu[np + i], dz = z - sv.refu[2 * np + i];
   if (dx * dx + dy * dy + dz * dz > sv.lim_u2) sv.st->stale = 1;
+  if (sv.raw_rpole) {
+    // the 10 used components of rpole(13) (q, d, Qxx Qxy Qxz Qyy Qyz Qzz)
+    const GLOBAL double* r = gp(sv.raw_rpole) + (size_t)a * 13;
+    GLOBAL double* RP = gw(const_cast<double*>(sv.rpole));
+    RP[0 * np + i] = r[0];
+    RP[1 * np + i] = r[1];
+    RP[2 * np + i] = r[2];
+    RP[3 * np + i] = r[3];
+    RP[4 * np + i] = r[4];
+    RP[5 * np + i] = r[5];
+    RP[6 * np + i] = r[6];
+    RP[7 * np + i] = r[8];
+    RP[8 * np + i] = r[9];
+    RP[9 * np + i] = r[12];
+  }
 }
 
 // bin counters of the brick bins, zeroed before k_pme_prep counts into them

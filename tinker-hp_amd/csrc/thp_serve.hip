@@ -59,6 +59,8 @@ struct Request {
   int sid = 0;
   double* xyz = nullptr;  // pinned [3][n] caller-order coordinates, or null (unchanged)
   size_t xyz_cap = 0;     // doubles held by xyz (a pool buffer)
+  double* rp = nullptr;   // pinned [n][13] caller-order multipoles, or null (unchanged)
+  size_t rp_cap = 0;
   Clock::time_point t_submit;
 };
 
@@ -615,6 +617,7 @@ thp_status pass(Serve* sv, std::vector<Request>& starting) {
     if (ra == THP_ERR_HIP) return ra;
     if (ra != THP_OK) {
       give_pinned(sv, q.xyz, q.xyz_cap);
+      give_pinned(sv, q.rp, q.rp_cap);
       publish_error(sv, q, ra, t_pass);
       continue;
     }
@@ -633,6 +636,14 @@ thp_status pass(Serve* sv, std::vector<Request>& starting) {
         return fail(ctx, THP_ERR_HIP, "coordinate upload");
       uploads.emplace_back(q.xyz, q.xyz_cap);
       q.xyz = nullptr;
+    }
+    if (q.rp) {
+      const size_t n = (size_t)ss->s->n;
+      if (hipMemcpyAsync((void*)ss->s->view.raw_rpole, q.rp, sizeof(double) * 13 * n, hipMemcpyHostToDevice,
+                         ctx->stream) != hipSuccess)
+        return fail(ctx, THP_ERR_HIP, "multipole upload");
+      uploads.emplace_back(q.rp, q.rp_cap);
+      q.rp = nullptr;
     }
     const auto tc1 = Clock::now();
     rc = ensure_lists(ss->s);  // new systems only; others keep theirs until the skin check fails
@@ -825,6 +836,7 @@ void worker_main(Serve* sv) {
   std::lock_guard<std::mutex> lk(sv->mu);
   for (auto& q : sv->queue) {
     if (q.xyz) sv->pinned_pool.emplace(q.xyz_cap, q.xyz);
+    if (q.rp) sv->pinned_pool.emplace(q.rp_cap, q.rp);
     thp_result r{};
     r.ticket = q.ticket;
     r.system_id = q.sid;
@@ -912,32 +924,37 @@ thp_status thp_serve_add_system(thp_ctx* ctx, const thp_system_desc* d, int* sys
   return THP_OK;
 }
 
-thp_status thp_serve_submit(thp_ctx* ctx, int sid, const double* x, const double* y, const double* z,
-                            uint64_t* ticket) {
+thp_status thp_serve_submit_multipoles(thp_ctx* ctx, int sid, const double* x, const double* y, const double* z,
+                                       const double* rpole, uint64_t* ticket) {
   Serve* sv = serve_of(ctx);
   if (!sv) return ctx ? THP_ERR_STATE : THP_ERR_ARG;
   if (!ticket || (x && (!y || !z))) return THP_ERR_ARG;
   Request q;
   q.sid = sid;
   q.t_submit = Clock::now();
+  size_t n = 0;
   {
     std::lock_guard<std::mutex> lk(sv->mu);
     if (sv->stop) return sv->fatal != THP_OK ? sv->fatal : THP_ERR_STATE;
     if (sid < 0 || sid >= (int)sv->systems.size()) return THP_ERR_ARG;
+    n = (size_t)sv->systems[sid]->desc.n;
   }
+  // caller-order arrays into pinned buffers on the caller's thread; the
+  // worker uploads them as they are
   if (x) {
-    // caller-order coordinates into a pinned buffer on the caller's thread;
-    // the worker uploads it as is
-    size_t n = 0;
-    {
-      std::lock_guard<std::mutex> lk(sv->mu);
-      n = (size_t)sv->systems[sid]->desc.n;
-    }
     q.xyz = take_pinned(sv, 3 * n, &q.xyz_cap);
     if (!q.xyz) return THP_ERR_NOMEM;
     std::memcpy(q.xyz, x, sizeof(double) * n);
     std::memcpy(q.xyz + n, y, sizeof(double) * n);
     std::memcpy(q.xyz + 2 * n, z, sizeof(double) * n);
+  }
+  if (rpole) {
+    q.rp = take_pinned(sv, 13 * n, &q.rp_cap);
+    if (!q.rp) {
+      give_pinned(sv, q.xyz, q.xyz_cap);
+      return THP_ERR_NOMEM;
+    }
+    std::memcpy(q.rp, rpole, sizeof(double) * 13 * n);
   }
   {
     std::lock_guard<std::mutex> lk(sv->mu);
@@ -947,6 +964,11 @@ thp_status thp_serve_submit(thp_ctx* ctx, int sid, const double* x, const double
   }
   sv->cv_work.notify_one();
   return THP_OK;
+}
+
+thp_status thp_serve_submit(thp_ctx* ctx, int sid, const double* x, const double* y, const double* z,
+                            uint64_t* ticket) {
+  return thp_serve_submit_multipoles(ctx, sid, x, y, z, nullptr, ticket);
 }
 
 thp_status thp_serve_poll(thp_ctx* ctx, uint64_t ticket, thp_result* result, double wait_ms) {

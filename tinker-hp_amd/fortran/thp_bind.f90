@@ -120,6 +120,11 @@ module thp_bind
       real(c_double), intent(out) :: ep
       type(thp_solve_stats), intent(out) :: stats
     end function
+    integer(c_int) function thp_system_update_multipoles(sys, rpole) bind(C, name="thp_system_update_multipoles")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: sys
+      real(c_double), intent(in) :: rpole(*)
+    end function
     integer(c_int) function thp_set_solver(sys, polalg) bind(C, name="thp_set_solver")
       import :: c_int, c_ptr
       type(c_ptr), value :: sys

@@ -169,6 +169,12 @@ contains
     if (st%status /= THP_OK) write (*, '(a,i0)') ' NEWINDUCE_PME2  --  Warning, PCG not converged, status ', st%status
   end subroutine
 
+  ! global-frame multipoles rpole(13,n) of this step, after rotpole
+  subroutine polar_update_multipoles(rpole)
+    real(c_double), intent(in) :: rpole(13, n)
+    call fatal_status(thp_system_update_multipoles(sys, rpole), 'thp_system_update_multipoles')
+  end subroutine
+
   ! the POLAR-ALG keyword (prmkey.f:396-398): 1 PCG, 2 Jacobi/DIIS
   subroutine polar_set_solver(polalg)
     integer, intent(in) :: polalg

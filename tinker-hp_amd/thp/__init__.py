@@ -122,6 +122,7 @@ def load_lib(path: str | None = None) -> C.CDLL:
         "thp_version": (C.c_char_p, []),
         "thp_system_create": (C.c_int, [vp, C.POINTER(SystemDesc), C.POINTER(vp)]),
         "thp_system_update_coords": (C.c_int, [vp, _dp, _dp, _dp]),
+        "thp_system_update_multipoles": (C.c_int, [vp, _dp]),
         "thp_system_destroy": (C.c_int, [vp]),
         "thp_system_footprint": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "thp_nblist": (C.c_int, [vp, C.c_int]),
@@ -141,6 +142,7 @@ def load_lib(path: str | None = None) -> C.CDLL:
         "thp_serve_start": (C.c_int, [vp, C.POINTER(ServeOpts)]),
         "thp_serve_add_system": (C.c_int, [vp, C.POINTER(SystemDesc), _ip]),
         "thp_serve_submit": (C.c_int, [vp, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_uint64)]),
+        "thp_serve_submit_multipoles": (C.c_int, [vp, C.c_int, _dp, _dp, _dp, _dp, C.POINTER(C.c_uint64)]),
         "thp_serve_poll": (C.c_int, [vp, C.c_uint64, C.POINTER(Result), C.c_double]),
         "thp_serve_get_dipoles": (C.c_int, [vp, C.c_int, _dp]),
         "thp_serve_stats_get": (C.c_int, [vp, C.POINTER(ServeStats)]),
@@ -191,6 +193,11 @@ class System:
     def with_coords(self, x, y, z):
         s = System(**{k: getattr(self, k) for k in self.__dataclass_fields__})
         s.x, s.y, s.z = (np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z))
+        return s
+
+    def with_multipoles(self, rpole):
+        s = System(**{k: getattr(self, k) for k in self.__dataclass_fields__})
+        s.rpole = np.ascontiguousarray(rpole, dtype=np.float64).reshape(self.n, 13)
         return s
 
     def desc(self, precision: int) -> SystemDesc:
@@ -343,6 +350,11 @@ class Solver:
     def update_coords(self, x, y, z):
         x, y, z = (np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z))
         self._c(self.lib.thp_system_update_coords(self.h, _d(x), _d(y), _d(z)), "update_coords")
+
+    def update_multipoles(self, rpole):
+        """New global-frame multipoles rpole (n, 13), e.g. after rotpole."""
+        rp = np.ascontiguousarray(rpole, dtype=np.float64).reshape(self.sys.n, 13)
+        self._c(self.lib.thp_system_update_multipoles(self.h, _d(rp)), "update_multipoles")
 
     def nblist(self, force_rebuild=True):
         self._c(self.lib.thp_nblist(self.h, 1 if force_rebuild else 0), "nblist")

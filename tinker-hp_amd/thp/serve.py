@@ -36,13 +36,19 @@ class Server:
         self.systems.append(system)
         return sid.value
 
-    def submit(self, sid: int, x=None, y=None, z=None) -> int:
+    def submit(self, sid: int, x=None, y=None, z=None, rpole=None) -> int:
+        """Request a solve at new coordinates and, optionally, new global-frame
+        multipoles rpole (n, 13); None keeps the current ones."""
         t = C.c_uint64(0)
-        if x is None:
-            rc = self.lib.thp_serve_submit(self.ctx.h, sid, None, None, None, C.byref(t))
-        else:
+        xyz = (None, None, None)
+        if x is not None:
             x, y, z = (np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z))
-            rc = self.lib.thp_serve_submit(self.ctx.h, sid, _d(x), _d(y), _d(z), C.byref(t))
+            xyz = (_d(x), _d(y), _d(z))
+        if rpole is None:
+            rc = self.lib.thp_serve_submit(self.ctx.h, sid, *xyz, C.byref(t))
+        else:
+            rp = np.ascontiguousarray(rpole, dtype=np.float64).reshape(self.systems[sid].n, 13)
+            rc = self.lib.thp_serve_submit_multipoles(self.ctx.h, sid, *xyz, _d(rp), C.byref(t))
         self.ctx.check(rc, "thp_serve_submit")
         return t.value
 
