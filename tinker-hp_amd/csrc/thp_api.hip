@@ -265,31 +265,32 @@ thp_status pair_efld0(thp_ctx* ctx, const Batch& b, int gate) {
   return launch_check(ctx, "k_efld0");
 }
 
-constexpr int spread_threads(int prec) { return 64 * (prec == THP_PREC_FP64 ? SpreadTileCfg<double>::WAVES : SpreadTileCfg<float>::WAVES); }
+// spreading tiles of the home atoms, then the merge into the grid.  Four
+// wavefronts per brick for one system (FP32), two for batches of several
+// systems and in FP64 (LDS).
+template <typename real, int KIND>
+void launch_spread(thp_ctx* ctx, const Batch& b, int vin, int frac, int gate) {
+  hipStream_t st = ctx->stream;
+  if (sizeof(real) == 4 && b.nsys < 4)
+    hipLaunchKernelGGL((k_spread_tile<real, KIND, 4>), g1(b.n_brick), dim3(256), 0, st, b.d_views, b.d_brick, vin, frac,
+                       gate);
+  else
+    hipLaunchKernelGGL((k_spread_tile<real, KIND, 2>), g1(b.n_brick), dim3(128), 0, st, b.d_views, b.d_brick, vin, frac,
+                       gate);
+  hipLaunchKernelGGL((k_spread_merge<real, KIND>), g1(b.n_conv), dim3(256), 0, st, b.d_views, b.d_conv, gate);
+}
 
-// grid_uind_site: tiles of the home atoms, then the merge into the grid
-thp_status spread_uind(thp_ctx* ctx, const Batch& b, int vin, int frac, int check_conv) {
-  const dim3 t(spread_threads(b.prec));
-  if (b.prec == THP_PREC_FP64) {
-    hipLaunchKernelGGL((k_spread_tile<double, 0>), g1(b.n_brick), t, 0, ctx->stream, b.d_views, b.d_brick, vin, frac, check_conv);
-    hipLaunchKernelGGL((k_spread_merge<double, 0>), g1(b.n_conv), dim3(256), 0, ctx->stream, b.d_views, b.d_conv, check_conv);
-  } else {
-    hipLaunchKernelGGL((k_spread_tile<float, 0>), g1(b.n_brick), t, 0, ctx->stream, b.d_views, b.d_brick, vin, frac, check_conv);
-    hipLaunchKernelGGL((k_spread_merge<float, 0>), g1(b.n_conv), dim3(256), 0, ctx->stream, b.d_views, b.d_conv, check_conv);
-  }
+// grid_uind_site
+thp_status spread_uind(thp_ctx* ctx, const Batch& b, int vin, int frac, int gate) {
+  if (b.prec == THP_PREC_FP64) launch_spread<double, 0>(ctx, b, vin, frac, gate);
+  else launch_spread<float, 0>(ctx, b, vin, frac, gate);
   return launch_check(ctx, "k_spread_tile<uind>");
 }
 
 // grid_mpole_site (real part, imaginary part cleared)
 thp_status spread_mpole(thp_ctx* ctx, const Batch& b, int gate) {
-  const dim3 t(spread_threads(b.prec));
-  if (b.prec == THP_PREC_FP64) {
-    hipLaunchKernelGGL((k_spread_tile<double, 1>), g1(b.n_brick), t, 0, ctx->stream, b.d_views, b.d_brick, 0, 0, gate);
-    hipLaunchKernelGGL((k_spread_merge<double, 1>), g1(b.n_conv), dim3(256), 0, ctx->stream, b.d_views, b.d_conv, gate);
-  } else {
-    hipLaunchKernelGGL((k_spread_tile<float, 1>), g1(b.n_brick), t, 0, ctx->stream, b.d_views, b.d_brick, 0, 0, gate);
-    hipLaunchKernelGGL((k_spread_merge<float, 1>), g1(b.n_conv), dim3(256), 0, ctx->stream, b.d_views, b.d_conv, gate);
-  }
+  if (b.prec == THP_PREC_FP64) launch_spread<double, 1>(ctx, b, 0, 0, gate);
+  else launch_spread<float, 1>(ctx, b, 0, 0, gate);
   return launch_check(ctx, "k_spread_tile<mpole>");
 }
 

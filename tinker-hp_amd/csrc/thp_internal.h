@@ -145,7 +145,7 @@ struct Work {  // one unit of a launch: a block (brick, chunk ...) of one system
   int blk;
 };
 
-constexpr int kConvChunk = 2048;
+constexpr int kConvChunk = 512;  // grid points per workgroup of the spreading merge
 // grid FFT: complex values per LDS buffer of a workgroup, longest axis
 constexpr int kFftBuf = 2048;
 constexpr int kFftMaxLen = 512;

@@ -1309,15 +1309,12 @@ __global__ void __launch_bounds__(256) k_pme_binsort(const SysView* views, const
 // previous brick along each axis (at most 8 tiles, fixed order).  Bricks
 // are at least 4 points wide, so no tile reaches two bricks ahead.
 
-template <typename real>
-struct SpreadTileCfg {
-  static constexpr int WAVES = sizeof(real) == 4 ? 4 : 2;  // private tiles per workgroup
-};
-
-template <typename real, int KIND>
-__global__ void __launch_bounds__(64 * SpreadTileCfg<real>::WAVES) k_spread_tile(const SysView* views, const Work* bwork,
-                                                                                  int vin, int frac_in, int gate) {
-  constexpr int W = SpreadTileCfg<real>::WAVES;
+// W wavefronts (private tiles) per workgroup: more tiles shorten one
+// brick's atom loop, fewer tiles cost less LDS and less zeroing and
+// summing per brick, which pays when a batch brings many bricks
+template <typename real, int KIND, int W>
+__global__ void __launch_bounds__(64 * W) k_spread_tile(const SysView* views, const Work* bwork, int vin, int frac_in,
+                                                        int gate) {
   constexpr int NL = KIND == 0 ? 2 : 3;   // spline levels used
   constexpr int NF = KIND == 0 ? 6 : 10;  // moments per atom
   constexpr int NT = 3 * NL * kOrder;
