@@ -265,18 +265,15 @@ thp_status pair_efld0(thp_ctx* ctx, const Batch& b, int gate) {
   return launch_check(ctx, "k_efld0");
 }
 
-// spreading tiles of the home atoms, then the merge into the grid.  Four
-// wavefronts per brick for one system (FP32), two for batches of several
-// systems and in FP64 (LDS).
+// spreading tiles of the home atoms (two wavefronts per brick), then the
+// merge into the grid
 template <typename real, int KIND>
 void launch_spread(thp_ctx* ctx, const Batch& b, int vin, int frac, int gate) {
   hipStream_t st = ctx->stream;
-  if (sizeof(real) == 4 && b.nsys < 4)
-    hipLaunchKernelGGL((k_spread_tile<real, KIND, 4>), g1(b.n_brick), dim3(256), 0, st, b.d_views, b.d_brick, vin, frac,
-                       gate);
-  else
-    hipLaunchKernelGGL((k_spread_tile<real, KIND, 2>), g1(b.n_brick), dim3(128), 0, st, b.d_views, b.d_brick, vin, frac,
-                       gate);
+  // one tile count for every launch: the order of a grid point's sums, and
+  // so a request's result, does not depend on the batch it shares
+  hipLaunchKernelGGL((k_spread_tile<real, KIND, 2>), g1(b.n_brick), dim3(128), 0, st, b.d_views, b.d_brick, vin, frac,
+                     gate);
   hipLaunchKernelGGL((k_spread_merge<real, KIND>), g1(b.n_conv), dim3(256), 0, st, b.d_views, b.d_conv, gate);
 }
 

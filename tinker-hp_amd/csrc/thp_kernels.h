@@ -1024,6 +1024,13 @@ __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, cons
     g.piz = real(f);
   }
   const int cnt = ok ? gp(sv.tcount)[i] : 0;
+  const int ex0 = ok ? gp(sv.excl_off)[i] : 0, ex1 = ok ? gp(sv.excl_off)[i + 1] : 0;
+  // partner and coefficients of this wavefront's first scaled pair, loaded
+  // ahead of the walk
+  const bool ex_first = ex0 + wv < ex1;
+  const int kx_first = ex_first ? gp(sv.excl_k)[ex0 + wv] : 0;
+  const double d3_first = ex_first ? gp(sv.scoef)[2 * (ex0 + wv)] : 0.0;
+  const double d5_first = ex_first ? gp(sv.scoef)[2 * (ex0 + wv) + 1] : 0.0;
   const int npk = (cnt + kPack - 1) / kPack;
   const int b0 = (npk * wv) / S, b1 = (npk * (wv + 1)) / S;
   int resume = b0;
@@ -1031,15 +1038,36 @@ __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, cons
   for (int c0 = 0; c0 < ucount; c0 += C) {
     const int cn = min(C, ucount - c0);
     __syncthreads();
-    for (int q = threadIdx.x; q < cn; q += 64 * S) {
-      const int kk = UL[c0 + q];
-      double d = X[kk] - cx, e = Y[kk] - cy, f = Z[kk] - cz;
-      d -= bx * rint(d * ibx);
-      e -= by * rint(e * iby);
-      f -= bz * rint(f * ibz);
-      s_a[q] = Quad<real>{real(d), real(e), real(f), real(IN[0 * np + kk])};
-      s_b[q] = Quad<real>{real(IN[1 * np + kk]), real(IN[2 * np + kk]), real(IN[3 * np + kk]), real(IN[4 * np + kk])};
-      s_c[q] = real(IN[5 * np + kk]);
+    // a thread stages its union atoms SB at a time with every load of a
+    // batch in flight before the batch's LDS writes (the tail of the last
+    // batch re-reads the chunk's last atom and writes nothing)
+    constexpr int SB = 4;
+    for (int q0 = threadIdx.x; q0 < cn; q0 += 64 * S * SB) {
+      int kk[SB];
+#pragma unroll
+      for (int u = 0; u < SB; ++u) kk[u] = UL[c0 + min(q0 + u * 64 * S, cn - 1)];
+      double px[SB], py[SB], pz[SB], mu[SB][6];
+#pragma unroll
+      for (int u = 0; u < SB; ++u) {
+        px[u] = X[kk[u]];
+        py[u] = Y[kk[u]];
+        pz[u] = Z[kk[u]];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) mu[u][c] = IN[(size_t)c * np + kk[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < SB; ++u) {
+        const int q = q0 + u * 64 * S;
+        if (q < cn) {
+          double d = px[u] - cx, e = py[u] - cy, f = pz[u] - cz;
+          d -= bx * rint(d * ibx);
+          e -= by * rint(e * iby);
+          f -= bz * rint(f * ibz);
+          s_a[q] = Quad<real>{real(d), real(e), real(f), real(mu[u][0])};
+          s_b[q] = Quad<real>{real(mu[u][1]), real(mu[u][2]), real(mu[u][3]), real(mu[u][4])};
+          s_c[q] = real(mu[u][5]);
+        }
+      }
     }
     __syncthreads();
     if (ucount <= C) {
@@ -1054,13 +1082,15 @@ __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, cons
         walk_chunk<real, false>(TL, TC, np, i, b1, c0, cn, resume, g, acc);
     }
   }
-  // scaled pairs with their per-solve FP64 coefficients (tmatxb u-scale)
-  if (ok && wv == S - 1) {
-    const int e0 = sv.excl_off[i], e1 = sv.excl_off[i + 1];
-    for (int e = e0; e < e1; ++e) {
-      const double d3 = sv.scoef[2 * e], d5 = sv.scoef[2 * e + 1];
-      if (d3 == 0.0 && d5 == 0.0) continue;  // beyond the cutoff
-      const int kx = sv.excl_k[e];
+  // scaled pairs with their per-solve FP64 coefficients (tmatxb u-scale),
+  // dealt round the wavefronts (entries ex0 + wv, ex0 + wv + S, ...) so that
+  // no wavefront walks a long chain of dependent loads.  Pairs beyond the
+  // cutoff have zero coefficients and add exact zeros.
+  if (ok) {
+    for (int e = ex0 + wv; e < ex1; e += S) {
+      const bool first = e == ex0 + wv;
+      const double d3 = first ? d3_first : sv.scoef[2 * e], d5 = first ? d5_first : sv.scoef[2 * e + 1];
+      const int kx = first ? kx_first : sv.excl_k[e];
       const double xr = image_exact(X[kx] - xi, bx);
       const double yr = image_exact(Y[kx] - yi, by);
       const double zr = image_exact(Z[kx] - zi, bz);
@@ -1318,7 +1348,8 @@ __global__ void __launch_bounds__(64 * W) k_spread_tile(const SysView* views, co
   constexpr int NL = KIND == 0 ? 2 : 3;   // spline levels used
   constexpr int NF = KIND == 0 ? 6 : 10;  // moments per atom
   constexpr int NT = 3 * NL * kOrder;
-  __shared__ real s_tile[W][kTile3 * 2];
+  // a wavefront's tile, plus a slot the idle lanes of the second pass write
+  __shared__ __attribute__((aligned(16))) real s_tile[W][kTile3 * 2 + 4];
   __shared__ real s_th[64][NT];
   __shared__ real s_f[64][NF];
   __shared__ int s_off[64];
@@ -1340,12 +1371,15 @@ __global__ void __launch_bounds__(64 * W) k_spread_tile(const SysView* views, co
   const GLOBAL int* BLIST = gp(sv.bin_list);
   const double af0 = frac_in ? 1.0 : sv.afrac[0], af1 = frac_in ? 1.0 : sv.afrac[1], af2 = frac_in ? 1.0 : sv.afrac[2];
   real* my = s_tile[wv];
-  for (int p = lane; p < kTile3 * 2; p += 64) my[p] = real(0);
-  // stencil point of this lane in each pass
+  typedef real RV __attribute__((ext_vector_type(16 / sizeof(real))));  // 16-byte vector
+  constexpr int NV = kTile3 * 2 * (int)sizeof(real) / 16;
+  for (int p = lane; p < NV; p += 64) reinterpret_cast<RV*>(my)[p] = RV(real(0));
+  // stencil point of this lane in each pass (lanes past the 125th point in
+  // the second pass take the last point's weights and write the spare slot)
   int pmx[2], pmy[2], pmz[2], poff[2];
 #pragma unroll
   for (int ps = 0; ps < 2; ++ps) {
-    const int l = lane + 64 * ps;
+    const int l = min(lane + 64 * ps, kOrder * kOrder * kOrder - 1);
     pmx[ps] = l % kOrder;
     pmy[ps] = (l / kOrder) % kOrder;
     pmz[ps] = l / (kOrder * kOrder);
@@ -1402,23 +1436,25 @@ __global__ void __launch_bounds__(64 * W) k_spread_tile(const SysView* views, co
       }
     }
     __syncthreads();
-    for (int q = wv; q < cnt; q += W) {
+    // The two passes of an atom touch distinct points: both are read before
+    // either is written.  The next atom's weights are read from LDS while
+    // this atom's tile reads are in flight, so an atom costs one LDS round
+    // trip.  Consecutive atoms of a wavefront may share points and follow
+    // each other in program order.
+    const bool live1 = lane < kOrder * kOrder * kOrder - 64;
+    auto weights = [&](int q, real (&g)[2][2], int& t0, int& t1) {
       const real* th = s_th[q];
       const real* f = s_f[q];
       const int base = s_off[q];
 #pragma unroll
       for (int ps = 0; ps < 2; ++ps) {
-        if (ps == 1 && lane >= kOrder * kOrder * kOrder - 64) break;
         const int mx = pmx[ps], my_ = pmy[ps], mz = pmz[ps];
-        const int t = 2 * (base + poff[ps]);
         if (KIND == 0) {
           const real ta = th[(0 * NL + 0) * kOrder + mx], tb1 = th[(0 * NL + 1) * kOrder + mx];
           const real u0 = th[(1 * NL + 0) * kOrder + my_], u1 = th[(1 * NL + 1) * kOrder + my_];
           const real v0 = th[(2 * NL + 0) * kOrder + mz], v1 = th[(2 * NL + 1) * kOrder + mz];
-          const real g1 = ta * (f[1] * u1 * v0 + f[2] * u0 * v1) + tb1 * (f[0] * u0 * v0);
-          const real g2 = ta * (f[4] * u1 * v0 + f[5] * u0 * v1) + tb1 * (f[3] * u0 * v0);
-          my[t] += g1;
-          my[t + 1] += g2;
+          g[ps][0] = ta * (f[1] * u1 * v0 + f[2] * u0 * v1) + tb1 * (f[0] * u0 * v0);
+          g[ps][1] = ta * (f[4] * u1 * v0 + f[5] * u0 * v1) + tb1 * (f[3] * u0 * v0);
         } else {
           const real t0v = th[(0 * NL + 0) * kOrder + mx], t1v = th[(0 * NL + 1) * kOrder + mx],
                      t2v = th[(0 * NL + 2) * kOrder + mx];
@@ -1430,18 +1466,47 @@ __global__ void __launch_bounds__(64 * W) k_spread_tile(const SysView* views, co
               f[0] * u0 * v0 + f[2] * u1 * v0 + f[3] * u0 * v1 + f[5] * u2 * v0 + f[6] * u0 * v2 + f[9] * u1 * v1;
           const real term1 = f[1] * u0 * v0 + f[7] * u1 * v0 + f[8] * u0 * v1;
           const real term2 = f[4] * u0 * v0;
-          my[t] += term0 * t0v + term1 * t1v + term2 * t2v;
+          g[ps][0] = term0 * t0v + term1 * t1v + term2 * t2v;
+          g[ps][1] = real(0);
         }
+      }
+      t0 = 2 * (base + poff[0]);
+      t1 = live1 ? 2 * (base + poff[1]) : 2 * kTile3;
+    };
+    if (wv < cnt) {
+      real g[2][2];
+      int t0, t1;
+      weights(wv, g, t0, t1);
+      for (int q = wv; q < cnt; q += W) {
+        typedef real R2 __attribute__((ext_vector_type(2)));
+        R2 a0 = *reinterpret_cast<const R2*>(my + t0);
+        R2 a1 = *reinterpret_cast<const R2*>(my + t1);
+        real gn[2][2];
+        int n0, n1;
+        weights(min(q + W, cnt - 1), gn, n0, n1);
+        a0.x += g[0][0];
+        a0.y += g[0][1];
+        a1.x += g[1][0];
+        a1.y += g[1][1];
+        *reinterpret_cast<R2*>(my + t0) = a0;
+        *reinterpret_cast<R2*>(my + t1) = a1;
+#pragma unroll
+        for (int ps = 0; ps < 2; ++ps) {
+          g[ps][0] = gn[ps][0];
+          g[ps][1] = gn[ps][1];
+        }
+        t0 = n0;
+        t1 = n1;
       }
     }
   }
   __syncthreads();
   // fixed-order sum of the wavefront tiles into the brick's tile slot
-  GLOBAL real* OUT = gw(static_cast<real*>(sv.tiles)) + (size_t)tb * kTile3 * 2;
-  for (int p = threadIdx.x; p < kTile3 * 2; p += 64 * W) {
-    real v = s_tile[0][p];
+  GLOBAL RV* OUT = reinterpret_cast<GLOBAL RV*>(gw(static_cast<real*>(sv.tiles)) + (size_t)tb * kTile3 * 2);
+  for (int p = threadIdx.x; p < NV; p += 64 * W) {
+    RV v = reinterpret_cast<const RV*>(s_tile[0])[p];
 #pragma unroll
-    for (int q = 1; q < W; ++q) v += s_tile[q][p];
+    for (int q = 1; q < W; ++q) v += reinterpret_cast<const RV*>(s_tile[q])[p];
     OUT[p] = v;
   }
 }
@@ -1732,17 +1797,33 @@ __global__ void __launch_bounds__(64) k_gather_tile(const SysView* views, const 
   const int e1 = (t1 == nb1 - 1 ? K1 - sy : B1) + kOrder - 1;
   const int e2 = (t2 == nb2 - 1 ? K2 - sz : B2) + kOrder - 1;
   const int lane = threadIdx.x;
-  const GLOBAL real* GRID = gp(static_cast<const real*>(sv.grid));
-  for (int p = lane; p < kTile3; p += 64) {
-    const int ox = p % kTile, oy = (p / kTile) % kTile, oz = p / (kTile * kTile);
-    if (ox < e0 && oy < e1 && oz < e2) {
-      int gx = sx + ox, gy = sy + oy, gz = sz + oz;
-      if (gx >= K0) gx -= K0;
-      if (gy >= K1) gy -= K1;
-      if (gz >= K2) gz -= K2;
-      const size_t g = ((size_t)gz * K1 + gy) * K0 + gx;
-      s_g[2 * p] = GRID[2 * g];
-      s_g[2 * p + 1] = GRID[2 * g + 1];
+  // tile load: a batch of loads in flight per lane before its LDS writes
+  // (points outside the brick's extent are never read and are left as 0)
+  {
+    typedef real R2 __attribute__((ext_vector_type(2)));
+    const GLOBAL R2* GRID2 = reinterpret_cast<const GLOBAL R2*>(gp(static_cast<const real*>(sv.grid)));
+    R2* s_g2 = reinterpret_cast<R2*>(s_g);
+    static_assert(kTile3 % 64 == 0, "whole passes over the tile");
+    constexpr int PER = kTile3 / 64;
+    constexpr int BATCH = sizeof(real) == 4 ? PER : PER / 3;
+    static_assert(PER % BATCH == 0, "whole batches");
+    for (int k0 = 0; k0 < PER; k0 += BATCH) {
+      R2 v[BATCH];
+#pragma unroll
+      for (int k = 0; k < BATCH; ++k) {
+        const int p = lane + 64 * (k0 + k);
+        const int ox = p % kTile, oy = (p / kTile) % kTile, oz = p / (kTile * kTile);
+        int gx = sx + ox, gy = sy + oy, gz = sz + oz;
+        if (gx >= K0) gx -= K0;
+        if (gy >= K1) gy -= K1;
+        if (gz >= K2) gz -= K2;
+        const bool in = ox < e0 && oy < e1 && oz < e2;
+        const size_t g = in ? ((size_t)gz * K1 + gy) * K0 + gx : 0;
+        v[k] = GRID2[g];
+        if (!in) v[k] = R2{real(0), real(0)};
+      }
+#pragma unroll
+      for (int k = 0; k < BATCH; ++k) s_g2[lane + 64 * (k0 + k)] = v[k];
     }
   }
   __syncthreads();
@@ -1757,18 +1838,26 @@ __global__ void __launch_bounds__(64) k_gather_tile(const SysView* views, const 
     const int i = BLIST[c0 + lane];
     const int lx = G0[4 * i] - sx, ly = G0[4 * i + 1] - sy, lz = G0[4 * i + 2] - sz;
     const GLOBAL real* th = THETA + (size_t)i * kTheta;
-    real tx0[kOrder], tx1[kOrder], ty0[kOrder], ty1[kOrder];
+    real tx0[kOrder], tx1[kOrder], ty0[kOrder], ty1[kOrder], tz0[kOrder], tz1[kOrder];
 #pragma unroll
     for (int m = 0; m < kOrder; ++m) {
       tx0[m] = th[th_idx(0, 0, m)];
       tx1[m] = th[th_idx(0, 1, m)];
       ty0[m] = th[th_idx(1, 0, m)];
       ty1[m] = th[th_idx(1, 1, m)];
+      tz0[m] = th[th_idx(2, 0, m)];
+      tz1[m] = th[th_idx(2, 1, m)];
     }
     double a[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll 1
     for (int mz = 0; mz < kOrder; ++mz) {
-      const real v0 = th[th_idx(2, 0, mz)], v1 = th[th_idx(2, 1, mz)];
+      real v0 = tz0[0], v1 = tz1[0];
+#pragma unroll
+      for (int m = 1; m < kOrder; ++m)
+        if (mz == m) {
+          v0 = tz0[m];
+          v1 = tz1[m];
+        }
 #pragma unroll
       for (int my = 0; my < kOrder; ++my) {
         const real u0 = ty0[my], u1 = ty1[my];
@@ -1808,14 +1897,24 @@ __global__ void __launch_bounds__(64) k_gather_tile(const SysView* views, const 
       const double diag = pl == 0.0 ? kTinyPol : pl;
       const GLOBAL double* HR = gp(sv.hreal);
       GLOBAL double* H = gw(sv.h);
+      // every load before the first store (the compiler cannot tell the
+      // output vectors from the inputs)
+      double uv[6], hv[6], ev[6];
 #pragma unroll
       for (int c = 0; c < 6; ++c) {
         const size_t o = (size_t)c * np + i;
-        const double u = IN[o];
-        const double h = HR[o] + dip[c] - term * u;
+        uv[c] = IN[o];
+        hv[c] = HR[o];
+        ev[c] = MODE == 1 ? gp(sv.ef)[o] : 0.0;
+      }
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const size_t o = (size_t)c * np + i;
+        const double u = uv[c];
+        const double h = hv[c] + dip[c] - term * u;
         H[o] = h;
         if (MODE == 1) {
-          const double rr = gp(sv.ef)[o] - h;
+          const double rr = ev[c] - h;
           const double z = diag * rr;
           gw(sv.res)[o] = rr;
           gw(sv.zr)[o] = z;
