@@ -105,9 +105,10 @@ struct BatchBuf {
   size_t hcap_views = 0, hcap_tables = 0;
   int* req = nullptr;    // per-system set-up request flags (device)
   int* h_req = nullptr;  // pinned staging of req
-  // graphs of the current layout: set-up + iterations, iterations only
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
-  hipGraph_t graph[2] = {nullptr, nullptr};
+  // graphs of the current layout (set-up + iterations, iterations only),
+  // each as a run of segment graphs launched back to back
+  std::vector<hipGraphExec_t> exec[2];
+  std::vector<hipGraph_t> graph[2];
   int graph_iters = 0;
 };
 
@@ -220,10 +221,10 @@ thp_status grow_pinned(thp_ctx* ctx, void** p, size_t* cap, size_t need, size_t 
 
 void drop_graph(BatchBuf& bb) {
   for (int v = 0; v < 2; ++v) {
-    if (bb.exec[v]) (void)hipGraphExecDestroy(bb.exec[v]);
-    if (bb.graph[v]) (void)hipGraphDestroy(bb.graph[v]);
-    bb.exec[v] = nullptr;
-    bb.graph[v] = nullptr;
+    for (hipGraphExec_t x : bb.exec[v]) (void)hipGraphExecDestroy(x);
+    for (hipGraph_t g : bb.graph[v]) (void)hipGraphDestroy(g);
+    bb.exec[v].clear();
+    bb.graph[v].clear();
   }
   bb.graph_iters = 0;
 }
@@ -458,20 +459,39 @@ thp_status tune_pair(Serve* sv, Batch& b, const std::vector<thp_sys*>& members) 
 // The launch sequence of one pass over the member batch: with set-up
 // (variant 1) the requested systems go through their set-up first; then
 // `iters` PCG iterations of everything in flight, the energies of the
-// solves that finished, and the state collection.
-thp_status enqueue_pass(Serve* sv, int variant, int iters) {
+// solves that finished, and the state collection.  The sequence is cut
+// into segments of at most about thirty launches (the set-up; two
+// iterations at a time, the last with the energies and the collection),
+// each captured as a graph of its own: replaying one graph of ~80 kernels
+// left the device idle for 0.5-1 ms after its 32nd kernel, while graphs
+// this short replay back to back without a gap.
+constexpr int kItersPerSegment = 2;
+
+int pass_segments(int variant, int iters) { return (variant ? 1 : 0) + (iters + kItersPerSegment - 1) / kItersPerSegment; }
+
+thp_status enqueue_segment(Serve* sv, int variant, int iters, int seg) {
   thp_ctx* ctx = sv->ctx;
   Batch& b = sv->iter.b;
   if (variant) {
-    SRET(b_init_state(ctx, b, sv->o.poleps, sv->o.politer, sv->iter.warm, sv->iter.req));
-    SRET(b_load_coords(ctx, b, kGateSetup));
-    SRET(b_pme_prep(ctx, b, kGateSetup));
-    SRET(b_setup(ctx, b, kGateSetup));
-    SRET(b_end_setup(ctx, b));
+    if (seg == 0) {
+      SRET(b_init_state(ctx, b, sv->o.poleps, sv->o.politer, sv->iter.warm, sv->iter.req));
+      SRET(b_load_coords(ctx, b, kGateSetup));
+      SRET(b_pme_prep(ctx, b, kGateSetup));
+      SRET(b_setup(ctx, b, kGateSetup));
+      return b_end_setup(ctx, b);
+    }
+    --seg;
   }
-  for (int k = 0; k < iters; ++k) SRET(b_iteration(ctx, b));
+  const int k0 = seg * kItersPerSegment, k1 = std::min(iters, k0 + kItersPerSegment);
+  for (int k = k0; k < k1; ++k) SRET(b_iteration(ctx, b));
+  if (k1 < iters) return THP_OK;
   SRET(b_finish(ctx, b, 1));
   return b_collect(ctx, b, sv->d_out);
+}
+
+thp_status enqueue_pass(Serve* sv, int variant, int iters) {
+  for (int s = 0; s < pass_segments(variant, iters); ++s) SRET(enqueue_segment(sv, variant, iters, s));
+  return THP_OK;
 }
 
 // Run one pass, replaying the layout's captured graph of the variant when
@@ -484,25 +504,36 @@ thp_status run_pass(Serve* sv, int variant, int iters) {
   // capture a layout once it has survived a pass: with membership changing
   // every pass (streams of many distinct systems) eager launches are cheaper
   if (sv->o.use_graphs && sv->graphs_ok && sv->layout_age >= 1) {
-    if (!bb.exec[variant]) {
-      hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
-      thp_status r = e == hipSuccess ? enqueue_pass(sv, variant, iters) : THP_ERR_HIP;
-      hipGraph_t g = nullptr;
-      if (e == hipSuccess && hipStreamEndCapture(ctx->stream, &g) != hipSuccess) r = THP_ERR_HIP;
-      hipGraphExec_t x = nullptr;
-      if (r == THP_OK && g && hipGraphInstantiate(&x, g, nullptr, nullptr, 0) == hipSuccess) {
-        bb.graph[variant] = g;
-        bb.exec[variant] = x;
+    if (bb.exec[variant].empty()) {
+      bool ok = true;
+      for (int s = 0; ok && s < pass_segments(variant, iters); ++s) {
+        hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+        thp_status r = e == hipSuccess ? enqueue_segment(sv, variant, iters, s) : THP_ERR_HIP;
+        hipGraph_t g = nullptr;
+        if (e == hipSuccess && hipStreamEndCapture(ctx->stream, &g) != hipSuccess) r = THP_ERR_HIP;
+        hipGraphExec_t x = nullptr;
+        if (r == THP_OK && g && hipGraphInstantiate(&x, g, nullptr, nullptr, 0) == hipSuccess) {
+          bb.graph[variant].push_back(g);
+          bb.exec[variant].push_back(x);
+        } else {
+          if (g) (void)hipGraphDestroy(g);
+          ok = false;
+        }
+      }
+      if (ok) {
         bb.graph_iters = iters;
       } else {
         // capture is not available for this sequence: launch eagerly from now on
-        if (g) (void)hipGraphDestroy(g);
+        for (hipGraphExec_t x : bb.exec[variant]) (void)hipGraphExecDestroy(x);
+        for (hipGraph_t g : bb.graph[variant]) (void)hipGraphDestroy(g);
+        bb.exec[variant].clear();
+        bb.graph[variant].clear();
         (void)hipGetLastError();
         sv->graphs_ok = false;
       }
     }
-    if (bb.exec[variant]) {
-      SHIP(ctx, hipGraphLaunch(bb.exec[variant], ctx->stream));
+    if (!bb.exec[variant].empty()) {
+      for (hipGraphExec_t x : bb.exec[variant]) SHIP(ctx, hipGraphLaunch(x, ctx->stream));
       sv->stats.graph_launches += 1;
       return THP_OK;
     }
