@@ -8,8 +8,10 @@ resident system.  Legs:
   mixed  - FP32 pair/grid/FFT with FP64 accumulation and CG scalars;
   serve  - the batched serving loop (mixed), when libthp provides it.
 `value` is the fastest leg's solves/s.  The roofline object is measured live
-on the dominant kernel (the real-space matvec) with HIP events on the
-library's stream; cpu_baseline times the FP64 CPU oracle on the host cores.
+on the dominant kernel of that leg: the real-space matvec, as one launch over
+the serving batch (the resident copies) when the serving leg is the fastest,
+timed with HIP events on the library's stream; cpu_baseline times the FP64
+CPU oracle on the host cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serve] [--no-stream]
        python bench.py --serve-only | --kernel-probe    (profiling aids, no JSON line)
@@ -48,16 +50,22 @@ def time_leg(ctx, thp, system, precision, steps, warmup, poleps):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "matvec_traffic_water23559.md")
 
 
-def kernel_probe(thp):
+def kernel_probe(thp, copies):
     """Launch the matvec, spreading, gather and permanent-field pass of the
-    mixed 24k system 10 times each, then one device-to-device copy of 256 MiB
-    whose counter readings calibrate FETCH_SIZE and WRITE_SIZE."""
+    mixed 24k system 10 times each, the matvec 10 times as one launch over
+    `copies` resident copies (the serving batch), then one device-to-device
+    copy of 256 MiB whose counter readings calibrate FETCH_SIZE and
+    WRITE_SIZE."""
     import ctypes as C
     ctx = thp.Context(0)
-    sv = thp.Solver(ctx, thp.water_23559(), thp.THP_PREC_MIXED)
+    system = thp.water_23559()
+    sv = thp.Solver(ctx, system, thp.THP_PREC_MIXED)
     for k, name in ((0, "matvec"), (1, "spread"), (2, "gather"), (3, "efld0")):
         ms, pairs = sv.time_kernel(k, reps=10)
         log(f"probe: {name} {1e3 * ms:.1f} us per launch")
+    batch = [sv] + [thp.Solver(ctx, system, thp.THP_PREC_MIXED) for _ in range(copies - 1)]
+    ms, pairs = thp.time_kernel_batch(batch, 0, reps=10)
+    log(f"probe: matvec over {copies} systems {1e3 * ms:.1f} us per launch")
     hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
     nbytes = 256 << 20
     src, dst = C.c_void_p(), C.c_void_p()
@@ -72,11 +80,13 @@ def kernel_probe(thp):
     ctx.close()
 
 
-def measured_traffic():
-    """HBM bytes per matvec launch from the committed PMC summary, or None."""
+def measured_traffic(nsys):
+    """HBM bytes per matvec launch over `nsys` systems from the committed PMC
+    summary, or None."""
+    key = "traffic_bytes_per_launch:" if nsys == 1 else f"traffic_bytes_per_launch_{nsys}:"
     try:
         for line in open(TRAFFIC_FILE):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key):
                 return int(line.split(":")[1].split()[0])
     except OSError:
         pass
@@ -126,7 +136,7 @@ def main():
     import thp
 
     if args.kernel_probe:
-        kernel_probe(thp)
+        kernel_probe(thp, args.copies)
         return
     if args.serve_only:
         import thp.serve as tserve
@@ -173,14 +183,21 @@ def main():
         except (ImportError, AttributeError, thp.ThpError) as e:
             log(f"bench: stream leg unavailable: {e}")
 
-    # live roofline of the dominant kernel (mixed real-space matvec)
-    ms, pairs = mixed.time_kernel(0, reps=30)
+    # live roofline of the dominant kernel (mixed real-space matvec): one
+    # launch over the serving batch when that leg is the fastest
+    best = max(legs, key=lambda k: legs[k]["solves_per_s"])
+    batch = [mixed]
+    if best == "serve":
+        batch += [thp.Solver(ctx, system, thp.THP_PREC_MIXED) for _ in range(args.copies - 1)]
+    ms, pairs = thp.time_kernel_batch(batch, 0, reps=30)
+    for s in batch[1:]:
+        s.close()
     flops = FLOP_PER_PAIR_TMATXB * pairs
     achieved = flops / (ms * 1e-3) / 1e12
     roofline = dict(bound="mfma", achieved=round(achieved, 4), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=round(achieved / FP32_PEAK_TFLOPS, 5), traffic=measured_traffic(),
-                    kernel="k_tmatxb_cl<float,S> (real-space matvec)", ms_per_launch=round(ms, 5),
-                    pairs_in_cut=int(pairs), flop_per_pair=FLOP_PER_PAIR_TMATXB,
+                    frac=round(achieved / FP32_PEAK_TFLOPS, 5), traffic=measured_traffic(len(batch)),
+                    kernel="k_tmatxb_cl<float,S> (real-space matvec)", systems_per_launch=len(batch),
+                    ms_per_launch=round(ms, 5), pairs_in_cut=int(pairs), flop_per_pair=FLOP_PER_PAIR_TMATXB,
                     compute="FP32 VALU (no MFMA: no dense contraction)")
     log(f"bench: roofline {roofline}")
 
@@ -197,7 +214,6 @@ def main():
                 mu_rms_debye=float(np.sqrt(np.mean((r[0] - res["mu"]) ** 2)) * thp.DEBYE))
         log(f"bench: cpu {cpu} errors {err}")
 
-    best = max(legs, key=lambda k: legs[k]["solves_per_s"])
     value = legs[best]["solves_per_s"]
     out = {
         "metric": METRIC,

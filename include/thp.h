@@ -184,6 +184,13 @@ thp_status thp_induce_dev(thp_sys* sys, const thp_solve_opts* opts, double* mu_d
  * of the matvec's algorithmic FLOP count. */
 thp_status thp_time_kernel(thp_sys* sys, int kernel, int reps, double* ms_per_launch, int64_t* pairs_in_cut);
 
+/* The same timing over one launch that covers several systems (distinct,
+ * one context, one precision), with the work tables laid out as the
+ * serving loop lays out its batches.  pairs_in_cut is the sum over the
+ * systems. */
+thp_status thp_time_kernel_batch(thp_sys* const* systems, int nsys, int kernel, int reps, double* ms_per_launch,
+                                 int64_t* pairs_in_cut);
+
 /* --------------------------------------------------------------- serving */
 /* Serving loop, no single reference counterpart: it replaces an MD driver
  * that calls newinduce_pme2 once per system and step

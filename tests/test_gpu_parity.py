@@ -312,3 +312,22 @@ def test_device_pointer_variants(ctx):
     finally:
         hip.hipFree(d_mu)
         hip.hipFree(d_out)
+
+
+def test_kernel_timing_over_a_batch(ctx):
+    """The batched timing hook covers every system of the launch: its pair
+    count is the sum of the systems' counts, and a system given twice is
+    refused."""
+    s1, _ = golden81()
+    s2 = thp.water_1944()
+    a, b = thp.Solver(ctx, s1, MIXED), thp.Solver(ctx, s2, MIXED)
+    ms_a, pa = a.time_kernel(0, reps=2)
+    ms_b, pb = b.time_kernel(0, reps=2)
+    for k in range(4):
+        ms, pairs = thp.time_kernel_batch([a, b], k, reps=2)
+        assert ms > 0.0
+        assert pairs == pa + pb
+    with pytest.raises(thp.ThpError):
+        thp.time_kernel_batch([a, a], 0, reps=1)
+    a.close()
+    b.close()

@@ -710,6 +710,31 @@ std::vector<Work> xcd_order(const std::vector<Work>& list) {
   return t;
 }
 
+// unscaled pairs within the cutoff (i<k), counted on the host from the
+// block-union lists: the unit of the matvec's algorithmic FLOP count
+thp_status count_pairs(thp_sys* s, int64_t* out) {
+  int64_t cnt = 0;
+  std::vector<int> lc(s->npad);
+  HIPCHK(s->ctx, hipMemcpy(lc.data(), s->view.lcount, sizeof(int) * s->npad, hipMemcpyDeviceToHost));
+  std::vector<unsigned short> ll((size_t)s->view.lmax * s->npad);
+  std::vector<int> ul((size_t)s->nblk * s->view.umax);
+  HIPCHK(s->ctx, hipMemcpy(ll.data(), s->view.llist, sizeof(unsigned short) * ll.size(), hipMemcpyDeviceToHost));
+  HIPCHK(s->ctx, hipMemcpy(ul.data(), s->view.ulist, sizeof(int) * ul.size(), hipMemcpyDeviceToHost));
+  const double c2 = s->cut * s->cut;
+  for (int q = 0; q < s->n; ++q) {
+    const int b = q / kBlock;
+    for (int t2 = 0; t2 < lc[q]; ++t2) {
+      const int k = ul[(size_t)b * s->view.umax + ll[(size_t)t2 * s->npad + q]];
+      if (k <= q) continue;
+      double d[3] = {s->hx[k] - s->hx[q], s->hy[k] - s->hy[q], s->hz[k] - s->hz[q]};
+      for (int c = 0; c < 3; ++c) d[c] -= s->box[c] * std::rint(d[c] / s->box[c]);
+      if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] <= c2) ++cnt;
+    }
+  }
+  *out = cnt;
+  return THP_OK;
+}
+
 void append_work(const thp_sys* s, int index, std::vector<Work>& blk, std::vector<Work>& red, std::vector<Work>& gat,
                  std::vector<Work>& brick, std::vector<Work>& conv, std::vector<Work>& fft) {
   for (int q = 0; q < s->nblk; ++q) blk.push_back({index, q});
@@ -1530,45 +1555,99 @@ thp_status thp_induce(thp_sys* s, const thp_solve_opts* o, double* uind, double*
 }
 
 thp_status thp_time_kernel(thp_sys* s, int kernel, int reps, double* ms, int64_t* pairs) {
-  if (!s || !ms || reps <= 0 || kernel < 0 || kernel > 3) return THP_ERR_ARG;
-  RET(ensure_coef(s));
-  RET(ensure_pme(s));
-  hipStream_t st = s->ctx->stream;
+  if (!s) return THP_ERR_ARG;
+  return thp_time_kernel_batch(&s, 1, kernel, reps, ms, pairs);
+}
+
+thp_status thp_time_kernel_batch(thp_sys* const* systems, int nsys, int kernel, int reps, double* ms, int64_t* pairs) {
+  if (!systems || nsys <= 0 || !ms || reps <= 0 || kernel < 0 || kernel > 3) return THP_ERR_ARG;
+  thp_sys* s0 = systems[0];
+  if (!s0) return THP_ERR_ARG;
+  thp_ctx* ctx = s0->ctx;
+  for (int q = 0; q < nsys; ++q) {
+    if (!systems[q] || systems[q]->ctx != ctx || systems[q]->prec != s0->prec)
+      return fail(ctx, THP_ERR_ARG, "batch members must share one context and one precision");
+    for (int r = 0; r < q; ++r)
+      if (systems[r] == systems[q]) return fail(ctx, THP_ERR_ARG, "system %d appears twice in the batch", q);
+  }
+  for (int q = 0; q < nsys; ++q) {
+    RET(ensure_coef(systems[q]));
+    RET(ensure_pme(systems[q]));
+  }
+  hipStream_t st = ctx->stream;
+  // a batch of one is the system's own; several get views and work tables
+  // laid out as the serving loop lays them out (XCD-ordered blocks)
+  Batch b = s0->one;
+  void* d_mem = nullptr;
+  if (nsys > 1) {
+    std::vector<Work> blk, red, gat, brick, conv, fft;
+    for (int q = 0; q < nsys; ++q) append_work(systems[q], q, blk, red, gat, brick, conv, fft);
+    blk = xcd_order(blk);
+    brick = xcd_order(brick);
+    gat = xcd_order(gat);
+    std::vector<SysView> views(nsys);
+    for (int q = 0; q < nsys; ++q) views[q] = systems[q]->view;
+    const size_t nt = blk.size() + red.size() + gat.size() + brick.size() + conv.size() + fft.size();
+    const size_t vbytes = ((sizeof(SysView) * nsys + 255) / 256) * 256;
+    const size_t ibytes = ((sizeof(int) * nsys + 255) / 256) * 256;
+    HIPCHK(ctx, hipMalloc(&d_mem, vbytes + ibytes + sizeof(Work) * nt));
+    SysView* d_views = static_cast<SysView*>(d_mem);
+    int* d_ids = reinterpret_cast<int*>(static_cast<char*>(d_mem) + vbytes);
+    Work* d_tab = reinterpret_cast<Work*>(static_cast<char*>(d_mem) + vbytes + ibytes);
+    std::vector<Work> tab;
+    tab.reserve(nt);
+    auto put = [&](const std::vector<Work>& v, Work** dptr, int* cnt) {
+      *dptr = d_tab + tab.size();
+      *cnt = (int)v.size();
+      tab.insert(tab.end(), v.begin(), v.end());
+    };
+    b.nsys = nsys;
+    put(blk, &b.d_blk, &b.n_blk);
+    put(red, &b.d_red, &b.n_red);
+    put(gat, &b.d_gat, &b.n_gat);
+    put(brick, &b.d_brick, &b.n_brick);
+    put(conv, &b.d_conv, &b.n_conv);
+    put(fft, &b.d_fft, &b.n_fft);
+    b.d_views = d_views;
+    b.d_ids = d_ids;
+    std::vector<int> ids(nsys);
+    for (int q = 0; q < nsys; ++q) ids[q] = q;
+    thp_status rc = THP_OK;
+    if (hipMemcpy(d_views, views.data(), sizeof(SysView) * nsys, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_ids, ids.data(), sizeof(int) * nsys, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_tab, tab.data(), sizeof(Work) * nt, hipMemcpyHostToDevice) != hipSuccess)
+      rc = fail(ctx, THP_ERR_HIP, "batch tables");
+    if (rc != THP_OK) {
+      (void)hipFree(d_mem);
+      return rc;
+    }
+  }
   auto launch = [&]() -> thp_status {
-    if (kernel == 0) return b_pair_tmatxb(s->ctx, s->one, 1, 3, 1, 0);
-    if (kernel == 1) return spread_uind(s->ctx, s->one, 1, 0, 0);
-    if (kernel == 3) return pair_efld0(s->ctx, s->one, kGateNone);  // rewrites the same lists
-    return launch_gather_uind<0>(s->ctx, s->one, 1, 3, 0, kGateNone);
+    if (kernel == 0) return b_pair_tmatxb(ctx, b, 1, 3, 1, 0);
+    if (kernel == 1) return spread_uind(ctx, b, 1, 0, 0);
+    if (kernel == 3) return pair_efld0(ctx, b, kGateNone);  // rewrites the same lists
+    return launch_gather_uind<0>(ctx, b, 1, 3, 0, kGateNone);
   };
-  RET(launch());  // warm
-  HIPCHK(s->ctx, hipEventRecord(s->ev[0], st));
-  for (int r = 0; r < reps; ++r) RET(launch());
-  HIPCHK(s->ctx, hipEventRecord(s->ev[1], st));
-  HIPCHK(s->ctx, hipEventSynchronize(s->ev[1]));
+  thp_status rc = launch();  // warm
+  if (rc == THP_OK && hipEventRecord(s0->ev[0], st) != hipSuccess) rc = fail(ctx, THP_ERR_HIP, "event");
+  for (int r = 0; rc == THP_OK && r < reps; ++r) rc = launch();
   float t = 0;
-  HIPCHK(s->ctx, hipEventElapsedTime(&t, s->ev[0], s->ev[1]));
+  if (rc == THP_OK && (hipEventRecord(s0->ev[1], st) != hipSuccess || hipEventSynchronize(s0->ev[1]) != hipSuccess ||
+                       hipEventElapsedTime(&t, s0->ev[0], s0->ev[1]) != hipSuccess))
+    rc = fail(ctx, THP_ERR_HIP, "kernel timing");
+  if (d_mem) {
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(d_mem);
+  }
+  RET(rc);
   *ms = double(t) / reps;
   if (pairs) {
-    // unscaled pairs within the cutoff, from the block-union lists
-    int64_t cnt = 0;
-    std::vector<int> lc(s->npad);
-    HIPCHK(s->ctx, hipMemcpy(lc.data(), s->view.lcount, sizeof(int) * s->npad, hipMemcpyDeviceToHost));
-    std::vector<unsigned short> ll((size_t)s->view.lmax * s->npad);
-    std::vector<int> ul((size_t)s->nblk * s->view.umax);
-    HIPCHK(s->ctx, hipMemcpy(ll.data(), s->view.llist, sizeof(unsigned short) * ll.size(), hipMemcpyDeviceToHost));
-    HIPCHK(s->ctx, hipMemcpy(ul.data(), s->view.ulist, sizeof(int) * ul.size(), hipMemcpyDeviceToHost));
-    const double c2 = s->cut * s->cut;
-    for (int q = 0; q < s->n; ++q) {
-      const int b = q / kBlock;
-      for (int t2 = 0; t2 < lc[q]; ++t2) {
-        const int k = ul[(size_t)b * s->view.umax + ll[(size_t)t2 * s->npad + q]];
-        if (k <= q) continue;
-        double d[3] = {s->hx[k] - s->hx[q], s->hy[k] - s->hy[q], s->hz[k] - s->hz[q]};
-        for (int c = 0; c < 3; ++c) d[c] -= s->box[c] * std::rint(d[c] / s->box[c]);
-        if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] <= c2) ++cnt;
-      }
+    *pairs = 0;
+    for (int q = 0; q < nsys; ++q) {
+      int64_t c = 0;
+      RET(count_pairs(systems[q], &c));
+      *pairs += c;
     }
-    *pairs = cnt;
   }
   return THP_OK;
 }

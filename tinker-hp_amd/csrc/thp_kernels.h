@@ -1735,38 +1735,76 @@ __global__ void __launch_bounds__(256) k_fft_axis(const SysView* views, const Wo
   const GLOBAL Coef2<real>* TW =
       gp(static_cast<const Coef2<real>*>(sv.twid)) + (AXIS == 0 ? 0 : AXIS == 1 ? K0 : K0 + K1);
   const int tid = threadIdx.x;
-  for (int m = tid; m < K; m += 256) s_tw[m] = TW[m];
-  if (AXIS == 0) {
-    // the group's lines are one contiguous run of nl * K values
-    for (int f = tid; f < nl * K; f += 256) {
+  // element u of this thread: LDS slot and grid index.  The x pass reads
+  // its group as one contiguous run; the y and z passes read runs of fl
+  // consecutive values, one per line.
+  constexpr int PER = kFftBuf / 256;  // elements per thread, at most
+  const int ra = tid & (fl - 1), rt = tid >> lg, rstep = 256 >> lg;
+  auto elem = [&](int u, int& slot, size_t& g) {
+    if (AXIS == 0) {
+      const int f = tid + 256 * u;
       const int a = f / K, t = f - a * K;
-      s_buf[0][a * kp + t] = G[base + f];
+      slot = a * kp + t;
+      g = base + f;
+      return f < nl * K;
+    } else {
+      const int t = rt + u * rstep;
+      slot = ra * kp + t;
+      g = base + t * ts + ra;
+      return ra < nl && t < K;
     }
-  } else {
-    const int a = tid & (fl - 1);
-    if (a < nl)
-      for (int t = tid >> lg; t < K; t += 256 >> lg) s_buf[0][a * kp + t] = G[base + t * ts + a];
+  };
+  // every global load of the group (values, twiddles, qfac) is in flight
+  // before the first LDS write; the loads are unconditional (elements past
+  // the group re-read its first one) so that none waits for another
+  Coef2<real> v[PER];
+  real qv[MODE == 2 ? PER : 1];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    int slot;
+    size_t g;
+    v[u] = G[elem(u, slot, g) ? g : base];
+  }
+  if constexpr (MODE == 2) {
+    const GLOBAL real* Q = gp(static_cast<const real*>(sv.qfac));
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      int slot;
+      size_t g;
+      qv[u] = Q[elem(u, slot, g) ? g : base];
+    }
+  }
+  static_assert(kFftMaxLen <= 2 * 256, "two twiddles per thread");
+  Coef2<real> tv[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) tv[j] = TW[min(tid + 256 * j, K - 1)];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+    if (tid + 256 * j < K) s_tw[tid + 256 * j] = tv[j];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    int slot;
+    size_t g;
+    if (elem(u, slot, g)) s_buf[0][slot] = v[u];
   }
   __syncthreads();
   const int tpl_shift = 8 - lg;  // threads per line = 256 / fl
   int cur = fft_lines_lds<real>(s_buf[0], s_buf[1], 0, s_tw, K, kp, nl, tid, tpl_shift, MODE == 1 ? real(1) : real(-1));
   if (MODE == 2) {
-    const GLOBAL real* Q = gp(static_cast<const real*>(sv.qfac));
-    const int a = tid & (fl - 1);
-    if (a < nl)
-      for (int t = tid >> lg; t < K; t += 256 >> lg) s_buf[cur][a * kp + t] *= Q[base + t * ts + a];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      int slot;
+      size_t g;
+      if (elem(u, slot, g)) s_buf[cur][slot] *= qv[u];
+    }
     __syncthreads();
     cur = fft_lines_lds<real>(s_buf[0], s_buf[1], cur, s_tw, K, kp, nl, tid, tpl_shift, real(1));
   }
-  if (AXIS == 0) {
-    for (int f = tid; f < nl * K; f += 256) {
-      const int a = f / K, t = f - a * K;
-      G[base + f] = s_buf[cur][a * kp + t];
-    }
-  } else {
-    const int a = tid & (fl - 1);
-    if (a < nl)
-      for (int t = tid >> lg; t < K; t += 256 >> lg) G[base + t * ts + a] = s_buf[cur][a * kp + t];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    int slot;
+    size_t g;
+    if (elem(u, slot, g)) G[g] = s_buf[cur][slot];
   }
 }
 

@@ -148,6 +148,7 @@ def load_lib(path: str | None = None) -> C.CDLL:
         "thp_serve_stats_get": (C.c_int, [vp, C.POINTER(ServeStats)]),
         "thp_serve_stop": (C.c_int, [vp]),
         "thp_time_kernel": (C.c_int, [vp, C.c_int, C.c_int, _dp, C.POINTER(C.c_int64)]),
+        "thp_time_kernel_batch": (C.c_int, [C.POINTER(vp), C.c_int, C.c_int, C.c_int, _dp, C.POINTER(C.c_int64)]),
         "thp_make_system": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_double, C.c_uint64,
                                       _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _dp]),
         "thp_pick_grid": (C.c_int, [C.c_double]),
@@ -436,8 +437,24 @@ class Solver:
 
     def time_kernel(self, kernel: int, reps: int = 20):
         """(ms per launch, unscaled pairs within the cutoff) of a path kernel:
-        0 = real-space matvec, 1 = spreading, 2 = gather."""
+        0 = real-space matvec, 1 = spreading, 2 = gather, 3 = permanent-field
+        pass."""
         ms = C.c_double(0)
         pairs = C.c_int64(0)
         self._c(self.lib.thp_time_kernel(self.h, kernel, reps, C.byref(ms), C.byref(pairs)), "time_kernel")
         return ms.value, pairs.value
+
+
+def time_kernel_batch(solvers, kernel: int, reps: int = 20):
+    """(ms per launch, unscaled pairs within the cutoff summed over the
+    systems) of one launch of a path kernel over the systems of `solvers`
+    (distinct, one context, one precision), laid out as a serving batch."""
+    if not solvers:
+        raise ValueError("no systems")
+    lib = solvers[0].lib
+    hs = (C.c_void_p * len(solvers))(*[s.h for s in solvers])
+    ms = C.c_double(0)
+    pairs = C.c_int64(0)
+    solvers[0].ctx.check(lib.thp_time_kernel_batch(hs, len(solvers), kernel, reps, C.byref(ms), C.byref(pairs)),
+                         "time_kernel_batch")
+    return ms.value, pairs.value
