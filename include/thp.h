@@ -39,7 +39,12 @@
  *   thp_set_solver ............. POLAR-ALG keyword (v1.2/source/prmkey.f:396-398)
  *                                selecting inducepcg_pme2 or inducejac_pme2
  *                                (v1.2/source/newinduce_pme2.f:210-217,
- *                                 638-890; diis.f)
+ *                                 638-890; diis.f) or the truncated CG
+ *                                (v1.2/source/epolar3tcg.f:18-56)
+ *   thp_set_tcg ................ tcgorder/tcgprec/tcgguess/tcgpeek/tcgomega
+ *                                (v1.2/source/initprm.f:447-455) for
+ *                                epolar3tcg1/epolar3tcg2
+ *                                (v1.2/source/epolar3tcg.f:66-621)
  *   thp_set_predictor .......... polpred keyword + ulspred and the dipole
  *                                history (v1.2/source/mdinit.f:93,185-190;
  *                                 v1.2/source/newinduce_pme2.f:165-178,
@@ -72,6 +77,7 @@ typedef int thp_status;
 /* induced-dipole solver (polalg) */
 #define THP_POLALG_PCG 1          /* preconditioned conjugate gradient (default) */
 #define THP_POLALG_JACOBI_DIIS 2  /* Jacobi iterations with DIIS extrapolation */
+#define THP_POLALG_TCG 3          /* truncated conjugate gradient (thp_set_tcg) */
 
 /* induced-dipole predictor (polpred) */
 #define THP_PRED_NONE 0
@@ -162,6 +168,13 @@ thp_status thp_induce(thp_sys* sys, const thp_solve_opts* opts, double* uind, do
  * the Jacobi increment max_rhs sqrt(|munew - mu|^2 / 3n) < poleps and keeps
  * up to 25 subspace vectors on the device (2.4 kB per atom). */
 thp_status thp_set_solver(thp_sys* sys, int polalg);
+/* Options of the truncated CG (THP_POLALG_TCG): order 1 or 2 conjugate-
+ * gradient steps in closed form, the diagonal (polarity) preconditioner,
+ * the start alpha ef instead of zero, and the peek step omega alpha r
+ * (defaults 2, 1, 0, 1, 1.0).  A truncated solve does a fixed amount of work
+ * (order + 1 or order + 2 applications of the polarization matrix) and has
+ * no convergence test; its stats report niter = order. */
+thp_status thp_set_tcg(thp_sys* sys, int order, int precond, int guess, int peek, double omega);
 /* Select the induced-dipole predictor (THP_PRED_*) and clear its history.
  * While a predictor is set, every solve of the system stores its converged
  * dipoles (the last 7 are kept); once 7 are stored, the next solve starts

@@ -326,3 +326,74 @@ def induce_jacobi_diis(orc, poleps=1e-8, politer=500, mu0=None):
     f = 332.063714  # electric (MOD_units.f:81)
     ep = -0.5 * f * float(np.sum(mu[:, 0, :] * ef[:, 1, :]))
     return dict(mu=mu, ep=ep, niter=it, resnrm=rr)
+
+
+def tcg(afun, ef, alpha, order=2, prec=True, guess=False, peek=True, omega=1.0):
+    """Truncated conjugate gradient of order 1 or 2: epolar3tcg1 and
+    epolar3tcg2 (epolar3tcg.f:66-311 and 320-621).  afun applies the
+    polarization matrix A = alpha^-1 - T (tmatxb with the diagonal, the
+    reciprocal dipole field, minus the self term); alpha multiplies like
+    diagvec (newinduce_pme2.f:1017-1038).  From mu0 = alpha ef (tcgguess) or
+    zero: r0 = ef - A mu0, preconditioned by alpha (tcgprec); one (order 1)
+    or two (order 2) conjugate-gradient steps in closed form; then the
+    optional peek step omega alpha r (tcgpeek), whose scalars are those of
+    the first right-hand side for both, as in the reference.  Vectors are
+    (n, 2, 3).  Returns (mu, scalars)."""
+    ef = np.asarray(ef, dtype=np.float64)
+
+    def dot(a, b):
+        return np.einsum("nrc,nrc->r", a, b)
+
+    def col(v):
+        return np.asarray(v)[None, :, None]
+
+    def pre(v):
+        return alpha * v if not prec else v  # the alpha of the peek step
+
+    mu0 = alpha * ef if guess else np.zeros_like(ef)
+    oldr0 = ef - afun(mu0) if guess else ef.copy()
+    r0 = alpha * oldr0 if prec else oldr0.copy()
+    oldTr0 = afun(r0)
+    Tr0 = alpha * oldTr0 if prec else oldTr0.copy()
+    n0 = dot(oldr0, r0)
+    t1 = dot(r0, oldTr0)
+    t4 = n0 / t1
+    sc = dict(n0=n0, t1=t1, t4=t4)
+    if order == 1:
+        mu = col(t4) * r0
+        if guess:
+            mu = mu + mu0
+        if peek:
+            mu = mu + (omega * pre(r0) - omega * t4[0] * pre(Tr0))
+        return mu, sc
+    if order != 2:
+        raise ValueError("tcg order is 1 or 2")
+    mupeek = omega * pre(r0) - omega * t4[0] * pre(Tr0) if peek else 0.0
+    oldT2r0 = afun(Tr0)
+    T2r0 = alpha * oldT2r0 if prec else oldT2r0
+    np1 = dot(oldTr0, Tr0)
+    t2 = n0 * np1 / (t1 * t1)
+    p2 = col(t2) * Tr0 - col(t4) * T2r0
+    t3 = t1 * dot(oldTr0, p2)
+    g1 = (t1 * t1 - n0 * np1) / t3
+    mu = col(g1 * t2 + t4) * r0 - col(g1 * t4) * Tr0
+    if guess:
+        mu = mu + mu0
+    if peek:
+        mupeek = mupeek - omega * g1[0] * t2[0] * pre(Tr0) + omega * g1[0] * t4[0] * pre(T2r0)
+        mu = mu + mupeek
+    sc.update(np1=np1, t2=t2, t3=t3, g1=g1)
+    return mu, sc
+
+
+def induce_tcg(orc, order=2, prec=True, guess=False, peek=True, omega=1.0):
+    """The reference's polalg = 3 energy of one system (epolar3tcg1/2):
+    dipoles from tcg() on the full permanent field, E_pol = -1/2 f
+    ef_p . mu_d.  Returns dict(mu, ep)."""
+    s = orc.sys
+    ef = orc.efld0_total()
+    alpha = np.asarray(s.polarity, dtype=np.float64)[:, None, None]
+    mu, sc = tcg(orc.matvec, ef, alpha, order, prec, guess, peek, omega)
+    f = 332.063714  # electric (MOD_units.f:81)
+    ep = -0.5 * f * float(np.sum(mu[:, 0, :] * ef[:, 1, :]))
+    return dict(mu=mu, ep=ep, scalars=sc)

@@ -57,5 +57,5 @@ def test_jacobi_diis_subspace_restart(ctx):
     assert st.status == thp.THP_OK and abs(st.niter - ref["niter"]) <= 1
     assert abs(ep - ref["ep"]) < 1e-11 * abs(ref["ep"]) and _rms_debye(mu, ref["mu"]) < 1e-9
     with pytest.raises(thp.ThpError):
-        solver.set_solver(3)
+        solver.set_solver(4)
     solver.close()

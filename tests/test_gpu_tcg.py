@@ -1,0 +1,76 @@
+"""Truncated conjugate gradient (the reference's polalg = 3: epolar3tcg1 and
+epolar3tcg2, epolar3tcg.f:66-621) on the GPU against the oracle's
+restatement, over every combination of order, preconditioner, guess and
+peek step, in FP64 and in mixed precision."""
+import itertools
+
+import numpy as np
+import pytest
+
+import thp
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+FP64, MIXED = thp.THP_PREC_FP64, thp.THP_PREC_MIXED
+
+
+def _rms_debye(a, b):
+    return float(np.sqrt(np.mean((np.asarray(a) - np.asarray(b)) ** 2)) * thp.DEBYE)
+
+
+@pytest.mark.parametrize("order,prec,guess,peek", list(itertools.product([1, 2], [True, False], [True, False],
+                                                                         [True, False])))
+def test_tcg_matches_oracle_fp64(ctx, order, prec, guess, peek):
+    s = thp.tiny_water(27, 20.3, 24)
+    ref = O.induce_tcg(O.Oracle(s), order=order, prec=prec, guess=guess, peek=peek, omega=0.8)
+    solver = thp.Solver(ctx, s, FP64)
+    solver.set_solver(thp.THP_POLALG_TCG)
+    solver.set_tcg(order, prec, guess, peek, 0.8)
+    mu, ep, st = solver.newinduce_pme2()
+    erel = abs(ep - ref["ep"]) / abs(ref["ep"])
+    rms = _rms_debye(mu, ref["mu"])
+    print(f"tcg{order} prec={prec} guess={guess} peek={peek}: erel {erel:.1e} rmsD {rms:.1e}", flush=True)
+    assert st.status == thp.THP_OK and st.niter == order
+    assert erel < 1e-11 and rms < 1e-9
+    solver.close()
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("prec", [FP64, MIXED])
+def test_tcg_defaults_on_a_larger_box(ctx, order, prec):
+    # the reference's defaults (preconditioner, no guess, peek with omega 1)
+    s = thp.serving_system(1500, 0.0, seed=44)
+    ref = O.induce_tcg(O.Oracle(s), order=order)
+    solver = thp.Solver(ctx, s, prec)
+    solver.set_solver(thp.THP_POLALG_TCG)
+    solver.set_tcg(order)
+    mu, ep, st = solver.newinduce_pme2()
+    erel = abs(ep - ref["ep"]) / abs(ref["ep"])
+    rms = _rms_debye(mu, ref["mu"])
+    print(f"tcg{order} prec={prec}: erel {erel:.1e} rmsD {rms:.1e}", flush=True)
+    assert st.status == thp.THP_OK and st.niter == order
+    if prec == FP64:
+        assert erel < 1e-11 and rms < 1e-9
+    else:
+        assert erel < 1e-6 and rms < 1e-5
+    # a second solve of the same system gives the same result
+    mu2, ep2, _ = solver.newinduce_pme2()
+    assert ep2 == ep and np.array_equal(mu2, mu)
+    solver.close()
+
+
+def test_tcg_options_are_checked(ctx):
+    s = thp.tiny_water(27, 20.3, 24)
+    solver = thp.Solver(ctx, s, FP64)
+    with pytest.raises(thp.ThpError):
+        solver.set_tcg(3)
+    with pytest.raises(thp.ThpError):
+        solver.set_tcg(2, omega=float("nan"))
+    # PCG again after a truncated solve
+    solver.set_solver(thp.THP_POLALG_TCG)
+    solver.newinduce_pme2()
+    solver.set_solver(thp.THP_POLALG_PCG)
+    mu, ep, st = solver.newinduce_pme2(poleps=1e-8)
+    ref = O.Oracle(s).induce(poleps=1e-8)
+    assert st.status == thp.THP_OK and abs(ep - ref["ep"]) < 1e-11 * abs(ref["ep"])
+    solver.close()

@@ -340,3 +340,61 @@ def test_jacobi_diis_solution_matches_dense_solve():
     mu = np.linalg.solve(A, orc.efld0_total().reshape(-1)).reshape(s.n, 2, 3)
     jd = O.induce_jacobi_diis(orc, poleps=1e-14, politer=200)
     assert np.abs(jd["mu"] - mu).max() < 1e-12 * np.abs(mu).max()
+
+
+# ------------------------------------------------- truncated conjugate gradient
+def _dense(orc, n):
+    A = np.zeros((6 * n, 6 * n))
+    e = np.zeros(6 * n)
+    for j in range(6 * n):
+        e[:] = 0.0
+        e[j] = 1.0
+        A[:, j] = orc.matvec(e.reshape(n, 2, 3)).reshape(-1)
+    return A
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_tcg_with_guess_is_the_pcg_iterate(order):
+    # from mu0 = alpha E with the diagonal preconditioner, TCG of order k is
+    # the k-th iterate of the preconditioned conjugate gradient
+    s = thp.tiny_water(27, 20.3, 24)
+    orc = O.Oracle(s)
+    cg = orc.induce(poleps=1e-300, politer=order)
+    assert cg["niter"] == order
+    t = O.induce_tcg(orc, order=order, prec=True, guess=True, peek=False)
+    assert np.abs(t["mu"] - cg["mu"]).max() < 1e-12 * np.abs(cg["mu"]).max()
+    assert abs(t["ep"] - cg["ep"]) < 1e-12 * abs(cg["ep"])
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_tcg_is_exact_when_the_operator_has_order_distinct_eigenvalues(order):
+    # CG reaches the solution in as many steps as the operator has distinct
+    # eigenvalues: TCG1 solves lambda I, TCG2 a two-eigenvalue operator
+    rng = np.random.default_rng(21 + order)
+    n = 10
+    Q, _ = np.linalg.qr(rng.normal(size=(3 * n, 3 * n)))
+    lam = np.where(np.arange(3 * n) % 2 == 0, 1.7, 3.1) if order == 2 else np.full(3 * n, 2.3)
+    A = (Q * lam) @ Q.T
+    afun = lambda mu: np.einsum("ij,rj->ri", A, mu.transpose(1, 0, 2).reshape(2, -1)).reshape(2, n, 3).transpose(1, 0, 2)
+    ef = rng.normal(size=(n, 2, 3))
+    mu, _ = O.tcg(afun, ef, np.ones((n, 1, 1)), order=order, prec=False, guess=False, peek=False)
+    ref = np.stack([np.linalg.solve(A, ef[:, r, :].reshape(-1)).reshape(n, 3) for r in range(2)], axis=1)
+    assert np.abs(mu - ref).max() < 1e-12 * np.abs(ref).max()
+    if order == 2:
+        # one step is not enough for two eigenvalues
+        one, _ = O.tcg(afun, ef, np.ones((n, 1, 1)), order=1, prec=False, guess=False, peek=False)
+        assert np.abs(one - ref).max() > 1e-3 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("prec", [True, False])
+def test_tcg_peek_is_a_scaled_step_along_the_preconditioned_residual(order, prec):
+    # mu_peek - mu = omega alpha (E - A mu): the residual computed afresh
+    s = thp.tiny_water(27, 20.3, 24)
+    orc = O.Oracle(s)
+    ef = orc.efld0_total()
+    alpha = np.asarray(s.polarity, dtype=np.float64)[:, None, None]
+    mu, _ = O.tcg(orc.matvec, ef, alpha, order=order, prec=prec, guess=True, peek=False)
+    mup, _ = O.tcg(orc.matvec, ef, alpha, order=order, prec=prec, guess=True, peek=True, omega=0.7)
+    step = 0.7 * alpha * (ef - orc.matvec(mu))
+    assert np.abs((mup - mu) - step).max() < 1e-10 * np.abs(step).max()

@@ -1437,6 +1437,43 @@ thp_status jacobi_iteration(thp_sys* s, int it, int* nmat, int* first) {
   return launch_check(ctx, "k_jac_extrap");
 }
 
+// A v for the truncated CG: the matvec with its diagonal into hreal, the
+// reciprocal dipole field into h (the kernels add them and the self term)
+thp_status tcg_apply(thp_sys* s, int vin) {
+  thp_ctx* ctx = s->ctx;
+  const Batch& b = s->one;
+  RET(b_pair_tmatxb(ctx, b, vin, V_HREAL, 1, kGateNone));
+  RET(spread_uind(ctx, b, vin, 0, kGateNone));
+  RET(convolve(ctx, b, kGateNone));
+  return launch_gather_uind<0>(ctx, b, vin, V_H, 0, kGateNone);
+}
+
+// epolar3tcg1 / epolar3tcg2 after the permanent field (and the guess
+// alpha ef in mu) are in place
+thp_status tcg_solve(thp_sys* s) {
+  thp_ctx* ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  const int nb = s->tcg_blocks;
+  double* part_n0 = s->tcg_part;
+  double* part_t1 = s->tcg_part + (size_t)2 * nb;
+  double* part_np1 = s->tcg_part + (size_t)4 * nb;
+  double* part_sp = s->tcg_part + (size_t)6 * nb;
+  if (s->tcg_guess) RET(tcg_apply(s, V_MU));
+  hipLaunchKernelGGL(k_tcg_start, g1(nb), dim3(256), 0, st, s->d_view, s->tcg_guess, s->tcg_prec, part_n0);
+  RET(tcg_apply(s, V_ZR));
+  hipLaunchKernelGGL(k_tcg_apply1, g1(nb), dim3(256), 0, st, s->d_view, s->tcg_prec, s->tcg_vec, part_t1, part_np1);
+  hipLaunchKernelGGL(k_tcg_scalars1, g1(1), dim3(64), 0, st, part_n0, part_t1, part_np1, nb, s->tcg_sc);
+  if (s->tcg_order == 2) {
+    RET(tcg_apply(s, V_PP));
+    hipLaunchKernelGGL(k_tcg_apply2, g1(nb), dim3(256), 0, st, s->d_view, s->tcg_prec, s->tcg_vec, s->tcg_sc, part_sp);
+    hipLaunchKernelGGL(k_tcg_scalars2, g1(1), dim3(64), 0, st, part_sp, nb, s->tcg_sc);
+  }
+  hipLaunchKernelGGL(k_tcg_final, g1(nb), dim3(256), 0, st, s->d_view, s->tcg_vec, s->tcg_sc, s->tcg_order,
+                     s->tcg_prec, s->tcg_guess, s->tcg_peek, s->tcg_omega);
+  hipLaunchKernelGGL(k_tcg_done, g1(1), dim3(64), 0, st, s->d_view, s->tcg_order);
+  return launch_check(ctx, "k_tcg");
+}
+
 thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_stats* stats) {
   hipStream_t st = s->ctx->stream;
   thp_ctx* ctx = s->ctx;
@@ -1444,20 +1481,25 @@ thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_s
   HIPCHK(ctx, hipEventRecord(s->ev[0], st));
   RET(ensure_lists(s));
   RET(ensure_pme(s));
+  const bool tcg = s->polalg == THP_POLALG_TCG;
+  // the truncated CG starts from alpha ef or zero, whatever opts->guess
+  // and the predictor say (epolar3tcg.f:194-218)
+  if (tcg) warm = false;
   // with a predictor and a full history the extrapolated guess replaces
   // opts->guess, as use_pred precedes polgsf in the reference
-  if (s->pred != THP_PRED_NONE && s->pred_n == kMaxUalt) {
+  if (!tcg && s->pred != THP_PRED_NONE && s->pred_n == kMaxUalt) {
     RET(predict_guess(s));
     warm = true;
   }
   RET(b_init_state(ctx, b, o->poleps, o->politer, warm ? s->d_one : nullptr, nullptr));
   const bool jacobi = s->polalg == THP_POLALG_JACOBI_DIIS;
-  if (jacobi) RET(b_setup_field(ctx, b, kGateNone));
+  if (jacobi || tcg) RET(b_setup_field(ctx, b, kGateNone));
   else RET(b_setup(ctx, b, kGateNone));
   s->coef_valid = true;
   HIPCHK(ctx, hipEventRecord(s->ev[1], st));
   int conv = 0, nmat = 1, first = 0;
-  for (int it = 1; it <= o->politer; ++it) {
+  if (tcg) RET(tcg_solve(s));
+  for (int it = 1; !tcg && it <= o->politer; ++it) {
     if (jacobi) RET(jacobi_iteration(s, it, &nmat, &first));
     else RET(b_iteration(ctx, b));
     HIPCHK(ctx, hipMemcpyAsync(&conv, &s->view.st->conv, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1492,8 +1534,14 @@ extern "C" {
 thp_status thp_set_solver(thp_sys* s, int polalg) {
   using namespace thp;
   if (!s) return THP_ERR_ARG;
-  if (polalg != THP_POLALG_PCG && polalg != THP_POLALG_JACOBI_DIIS)
-    return fail(s->ctx, THP_ERR_ARG, "unknown polalg %d (1: PCG, 2: Jacobi/DIIS)", polalg);
+  if (polalg != THP_POLALG_PCG && polalg != THP_POLALG_JACOBI_DIIS && polalg != THP_POLALG_TCG)
+    return fail(s->ctx, THP_ERR_ARG, "unknown polalg %d (1: PCG, 2: Jacobi/DIIS, 3: truncated CG)", polalg);
+  if (polalg == THP_POLALG_TCG && !s->tcg_vec) {
+    s->tcg_blocks = (s->n + 255) / 256;
+    RET(dalloc(s, &s->tcg_vec, (size_t)2 * 6 * s->npad));
+    RET(dalloc(s, &s->tcg_part, (size_t)8 * s->tcg_blocks));
+    RET(dalloc(s, &s->tcg_sc, (size_t)kTcgScalars));
+  }
   if (polalg == THP_POLALG_JACOBI_DIIS && !s->diis) {
     s->diis_blocks = (s->n + 255) / 256;
     RET(dalloc(s, &s->diis_x, (size_t)kDiisMax * 6 * s->npad));
@@ -1502,6 +1550,19 @@ thp_status thp_set_solver(thp_sys* s, int polalg) {
     RET(dalloc(s, &s->diis, 1));
   }
   s->polalg = polalg;
+  return THP_OK;
+}
+
+thp_status thp_set_tcg(thp_sys* s, int order, int precond, int guess, int peek, double omega) {
+  using namespace thp;
+  if (!s) return THP_ERR_ARG;
+  if (order != 1 && order != 2) return fail(s->ctx, THP_ERR_ARG, "tcg order %d (1 or 2)", order);
+  if (!std::isfinite(omega)) return fail(s->ctx, THP_ERR_ARG, "tcg omega must be finite");
+  s->tcg_order = order;
+  s->tcg_prec = precond ? 1 : 0;
+  s->tcg_guess = guess ? 1 : 0;
+  s->tcg_peek = peek ? 1 : 0;
+  s->tcg_omega = omega;
   return THP_OK;
 }
 

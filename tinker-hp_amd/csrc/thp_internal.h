@@ -228,13 +228,20 @@ struct thp_sys {
   double* pred_coef = nullptr;   // [kMaxUalt], newest entry first
   double* pred_part = nullptr;   // [pred_blocks][kPredPairs]
   int pred_blocks = 0;
-  // solver (thp_set_solver): 1 PCG, 2 Jacobi/DIIS
+  // solver (thp_set_solver): 1 PCG, 2 Jacobi/DIIS, 3 truncated CG
   int polalg = 1;
   double* diis_x = nullptr;     // [kDiisMax][6][npad] Jacobi iterates
   double* diis_e = nullptr;     // [kDiisMax][6][npad] their increments
   double* diis_part = nullptr;  // [diis_blocks][kDiisParts]
   thp::DiisState* diis = nullptr;
   int diis_blocks = 0;
+  // truncated conjugate gradient (thp_set_tcg; initprm.f:447-455 defaults)
+  int tcg_order = 2, tcg_prec = 1, tcg_guess = 0, tcg_peek = 1;
+  double tcg_omega = 1.0;
+  double* tcg_vec = nullptr;   // [2][6][npad]: A r0, then A (T r0) preconditioned
+  double* tcg_part = nullptr;  // [4][tcg_blocks][2]: n0, t1, np1, A r0 . P2
+  double* tcg_sc = nullptr;    // scalars (kTcg* slots)
+  int tcg_blocks = 0;
   std::vector<double> hx, hy, hz;       // sorted-order coordinates (host copy)
   std::vector<double> bx0, by0, bz0;    // coordinates at the last block-list build
   std::vector<double> ux0, uy0, uz0;    // coordinates at the last block-union build

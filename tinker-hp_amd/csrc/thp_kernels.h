@@ -2505,6 +2505,171 @@ __global__ void __launch_bounds__(256) k_cg_update2(const SysView* views, const 
   if (threadIdx.x == 0 && sc[2] != kRunning) st->conv = int(sc[2]);
 }
 
+// ===================================================== truncated CG (polalg 3)
+// epolar3tcg1/epolar3tcg2 (epolar3tcg.f:66-311, 320-621) on one system.  The
+// polarization matrix A = alpha^-1 - T is applied by the matvec with its
+// diagonal (into hreal), the reciprocal dipole field (into h) and the self
+// term: A v = hreal + h - term v.  Vectors: res = r0 before and zr = r0
+// after the preconditioner (diagvec: times the polarity), pp = T r0 after
+// it, vec[0] = A r0 before it, vec[1] = A (T r0) after it; mu holds the
+// guess alpha ef (tcgguess) and finally the dipoles.  Dot products are
+// per-block partials summed in a fixed order; sc holds the scalars.
+enum : int { kTcgN0 = 0, kTcgT1 = 2, kTcgNp1 = 4, kTcgT4 = 6, kTcgT2 = 8, kTcgSp = 10, kTcgG1 = 12, kTcgScalars = 14 };
+
+// r0 = ef - A mu0 (or ef), its preconditioned copy, n0 = r0 . M r0
+__global__ void __launch_bounds__(256) k_tcg_start(const SysView* views, int guess, int prec, double* part) {
+  const SysView& sv = views[0];
+  const size_t np = sv.npad;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double n0[2] = {0.0, 0.0};
+  if (i < sv.n) {
+    const double a = gp(sv.pol)[i], term = sv.term_self;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const size_t o = (size_t)c * np + i;
+      const double h = guess ? gp(sv.hreal)[o] + gp(sv.h)[o] - term * gp(sv.mu)[o] : 0.0;
+      const double oldr = gp(sv.ef)[o] - h;
+      const double r = prec ? a * oldr : oldr;
+      gw(sv.res)[o] = oldr;
+      gw(sv.zr)[o] = r;
+      n0[c / 3] += oldr * r;
+    }
+  }
+  block_partial2(n0[0], n0[1], part, blockIdx.x);
+}
+
+// A r0 and its preconditioned copy T r0; t1 = r0 . A r0, np1 = A r0 . T r0
+__global__ void __launch_bounds__(256) k_tcg_apply1(const SysView* views, int prec, double* vec, double* part_t1,
+                                                    double* part_np1) {
+  const SysView& sv = views[0];
+  const size_t np = sv.npad;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double t1[2] = {0.0, 0.0}, np1[2] = {0.0, 0.0};
+  if (i < sv.n) {
+    const double a = gp(sv.pol)[i], term = sv.term_self;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const size_t o = (size_t)c * np + i;
+      const double r = gp(sv.zr)[o];
+      const double ar = gp(sv.hreal)[o] + gp(sv.h)[o] - term * r;
+      const double tr = prec ? a * ar : ar;
+      gw(vec)[o] = ar;
+      gw(sv.pp)[o] = tr;
+      t1[c / 3] += r * ar;
+      np1[c / 3] += ar * tr;
+    }
+  }
+  block_partial2(t1[0], t1[1], part_t1, blockIdx.x);
+  block_partial2(np1[0], np1[1], part_np1, blockIdx.x);
+}
+
+// n0, t1, np1 summed; t4 = n0 / t1, t2 = n0 np1 / t1^2 (per right-hand side)
+__global__ void __launch_bounds__(64) k_tcg_scalars1(const double* part_n0, const double* part_t1,
+                                                      const double* part_np1, int nblk, double* sc) {
+  double v[6];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    v[r] = sum_partials(part_n0, nblk, r);
+    v[2 + r] = sum_partials(part_t1, nblk, r);
+    v[4 + r] = sum_partials(part_np1, nblk, r);
+  }
+  if (threadIdx.x == 0) {
+    for (int r = 0; r < 2; ++r) {
+      const double n0 = v[r], t1 = v[2 + r], np1 = v[4 + r];
+      sc[kTcgN0 + r] = n0;
+      sc[kTcgT1 + r] = t1;
+      sc[kTcgNp1 + r] = np1;
+      sc[kTcgT4 + r] = n0 / t1;
+      sc[kTcgT2 + r] = n0 * np1 / (t1 * t1);
+    }
+  }
+}
+
+// A (T r0), preconditioned into vec[1]; P2 = t2 T r0 - t4 T2 r0 and
+// A r0 . P2 (epolar3tcg.f:500-526)
+__global__ void __launch_bounds__(256) k_tcg_apply2(const SysView* views, int prec, double* vec, const double* sc,
+                                                    double* part) {
+  const SysView& sv = views[0];
+  const size_t np = sv.npad;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double sp[2] = {0.0, 0.0};
+  if (i < sv.n) {
+    const double a = gp(sv.pol)[i], term = sv.term_self;
+    const double t2[2] = {sc[kTcgT2], sc[kTcgT2 + 1]}, t4[2] = {sc[kTcgT4], sc[kTcgT4 + 1]};
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const size_t o = (size_t)c * np + i;
+      const double tr = gp(sv.pp)[o];
+      const double a2 = gp(sv.hreal)[o] + gp(sv.h)[o] - term * tr;
+      const double t2r = prec ? a * a2 : a2;
+      gw(vec)[np * 6 + o] = t2r;
+      const double p2 = t2[c / 3] * tr - t4[c / 3] * t2r;
+      sp[c / 3] += gp(vec)[o] * p2;
+    }
+  }
+  block_partial2(sp[0], sp[1], part, blockIdx.x);
+}
+
+// t3 = t1 (A r0 . P2), g1 = (t1^2 - n0 np1) / t3
+__global__ void __launch_bounds__(64) k_tcg_scalars2(const double* part, int nblk, double* sc) {
+  const double s0 = sum_partials(part, nblk, 0), s1 = sum_partials(part, nblk, 1);
+  if (threadIdx.x == 0) {
+    const double s[2] = {s0, s1};
+    for (int r = 0; r < 2; ++r) {
+      const double t1 = sc[kTcgT1 + r], n0 = sc[kTcgN0 + r], np1 = sc[kTcgNp1 + r];
+      sc[kTcgSp + r] = s[r];
+      sc[kTcgG1 + r] = (t1 * t1 - n0 * np1) / (t1 * s[r]);
+    }
+  }
+}
+
+// the dipoles (epolar3tcg.f:263-300, 527-590): order 1: t4 r0; order 2:
+// (g1 t2 + t4) r0 - g1 t4 T r0; plus the guess; plus the peek step, whose
+// scalars are those of the first right-hand side for both, and which
+// multiplies by the polarity itself when the preconditioner is off
+__global__ void __launch_bounds__(256) k_tcg_final(const SysView* views, const double* vec, const double* sc, int order,
+                                                   int prec, int guess, int peek, double omega) {
+  const SysView& sv = views[0];
+  const size_t np = sv.npad;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= sv.n) return;
+  const double a = gp(sv.pol)[i];
+  const double pa = prec ? 1.0 : a;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const int r = c / 3;
+    const size_t o = (size_t)c * np + i;
+    const double r0 = gp(sv.zr)[o], tr = gp(sv.pp)[o];
+    const double t4 = sc[kTcgT4 + r];
+    double m;
+    if (order == 1) {
+      m = t4 * r0;
+    } else {
+      const double g1 = sc[kTcgG1 + r], t2 = sc[kTcgT2 + r];
+      m = (g1 * t2 + t4) * r0 - g1 * t4 * tr;
+    }
+    if (guess) m += gp(sv.mu)[o];
+    if (peek) {
+      double pk = omega * (pa * r0) - omega * sc[kTcgT4] * (pa * tr);
+      if (order == 2)
+        pk = (pk - omega * sc[kTcgG1] * sc[kTcgT2] * (pa * tr)) + omega * sc[kTcgG1] * sc[kTcgT4] * (pa * gp(vec)[np * 6 + o]);
+      m += pk;
+    }
+    gw(sv.mu)[o] = m;
+  }
+}
+
+// a truncated solve has no convergence test: it is complete after its steps
+__global__ void __launch_bounds__(64) k_tcg_done(const SysView* views, int order) {
+  if (threadIdx.x != 0) return;
+  SolveState* st = views[0].st;
+  st->niter = order;
+  st->it_cur = order;
+  st->conv = kConverged;
+  st->resnrm = 0.0;
+}
+
+
 // E_pol = -1/2 f sum mu_d . E_p (epolar0c: real + recip + self); partials
 // into the energy buffer, summed by k_finish.
 __global__ void __launch_bounds__(256) k_energy(const SysView* views, const Work* work, int only_done) {

@@ -130,6 +130,12 @@ module thp_bind
       type(c_ptr), value :: sys
       integer(c_int), value :: polalg
     end function
+    integer(c_int) function thp_set_tcg(sys, order, precond, guess, peek, omega) bind(C, name="thp_set_tcg")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: sys
+      integer(c_int), value :: order, precond, guess, peek
+      real(c_double), value :: omega
+    end function
     integer(c_int) function thp_set_predictor(sys, method) bind(C, name="thp_set_predictor")
       import :: c_int, c_ptr
       type(c_ptr), value :: sys

@@ -181,6 +181,18 @@ contains
     call fatal_status(thp_set_solver(sys, int(polalg, c_int)), 'thp_set_solver')
   end subroutine
 
+  ! the truncated CG options of POLAR-ALG 3 (tcgorder, tcgprec, tcgguess,
+  ! tcgpeek, tcgomega; defaults 2, .true., .false., .true., 1.0 in
+  ! initprm.f:447-455)
+  subroutine polar_set_tcg(tcgorder, tcgprec, tcgguess, tcgpeek, tcgomega)
+    integer, intent(in) :: tcgorder
+    logical, intent(in) :: tcgprec, tcgguess, tcgpeek
+    real(8), intent(in) :: tcgomega
+    call fatal_status(thp_set_tcg(sys, int(tcgorder, c_int), merge(1_c_int, 0_c_int, tcgprec), &
+                                  merge(1_c_int, 0_c_int, tcgguess), merge(1_c_int, 0_c_int, tcgpeek), &
+                                  real(tcgomega, c_double)), 'thp_set_tcg')
+  end subroutine
+
   ! the POLAR-PREDICT keyword (mdinit.f:185-190): 'NONE', 'GEAR', 'ASPC'
   ! or 'LSQR'; the stored solves are cleared
   subroutine polar_set_predictor(polpred)

@@ -32,7 +32,7 @@ THP_OK = 0
 THP_PREC_FP64 = 0
 THP_PREC_MIXED = 1
 # induced-dipole solvers (polalg)
-THP_POLALG_PCG, THP_POLALG_JACOBI_DIIS = 1, 2
+THP_POLALG_PCG, THP_POLALG_JACOBI_DIIS, THP_POLALG_TCG = 1, 2, 3
 # induced-dipole predictors (polpred)
 THP_PRED_NONE, THP_PRED_GEAR, THP_PRED_ASPC, THP_PRED_LSQR = 0, 1, 2, 3
 ERRORS = {
@@ -137,6 +137,7 @@ def load_lib(path: str | None = None) -> C.CDLL:
         "thp_induce": (C.c_int, [vp, C.POINTER(SolveOpts), _dp, _dp, _dp, C.POINTER(SolveStats)]),
         "thp_set_predictor": (C.c_int, [vp, C.c_int]),
         "thp_set_solver": (C.c_int, [vp, C.c_int]),
+        "thp_set_tcg": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]),
         "thp_tmatxb_dev": (C.c_int, [vp, vp, vp, C.c_int]),
         "thp_induce_dev": (C.c_int, [vp, C.POINTER(SolveOpts), vp, _dp, C.POINTER(SolveStats)]),
         "thp_serve_start": (C.c_int, [vp, C.POINTER(ServeOpts)]),
@@ -408,8 +409,14 @@ class Solver:
         self._c(self.lib.thp_pme_convolve(self.h), "pme_convolve")
 
     def set_solver(self, polalg: int):
-        """THP_POLALG_PCG (1) or THP_POLALG_JACOBI_DIIS (2)."""
+        """THP_POLALG_PCG (1), THP_POLALG_JACOBI_DIIS (2) or THP_POLALG_TCG (3)."""
         self._c(self.lib.thp_set_solver(self.h, int(polalg)), "set_solver")
+
+    def set_tcg(self, order: int = 2, precond: bool = True, guess: bool = False, peek: bool = True,
+                omega: float = 1.0):
+        """Options of the truncated CG (the reference's defaults)."""
+        self._c(self.lib.thp_set_tcg(self.h, int(order), int(bool(precond)), int(bool(guess)), int(bool(peek)),
+                                     float(omega)), "set_tcg")
 
     def set_predictor(self, method: int):
         """THP_PRED_NONE/GEAR/ASPC/LSQR; clears the stored solves."""
