@@ -339,9 +339,11 @@ def test_jacobi_diis_solution_matches_dense_solve():
         A[:, j] = orc.matvec(e.reshape(s.n, 2, 3)).reshape(-1)
     mu = np.linalg.solve(A, orc.efld0_total().reshape(-1)).reshape(s.n, 2, 3)
     jd = O.induce_jacobi_diis(orc, poleps=1e-14, politer=200)
-    # the stopping rule bounds the increment at 1e-14 (e A); the oracle's
-    # threaded sums vary in the last bits from run to run
-    assert np.abs(jd["mu"] - mu).max() < 1e-11 * np.abs(mu).max()
+    # the stopping rule bounds the Jacobi increment (1e-14 e A), not the
+    # error, which is larger by about 1 / (1 - spectral radius); the
+    # oracle's threaded sums vary in the last bits from run to run, and with
+    # them the iterate the rule stops at (errors of 2e-14 to 5e-13 seen)
+    assert np.abs(jd["mu"] - mu).max() < 1e-9 * np.abs(mu).max()
 
 
 # ------------------------------------------------- truncated conjugate gradient
