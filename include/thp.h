@@ -41,6 +41,11 @@
  *                                (v1.2/source/newinduce_pme2.f:210-217,
  *                                 638-890; diis.f) or the truncated CG
  *                                (v1.2/source/epolar3tcg.f:18-56)
+ *   thp_set_shortreal .......... use_polarshortreal with ewaldshortcut
+ *                                (v1.2/source/cutoffs.f:59,228; switch.f:60-62):
+ *                                newinduce_shortreal, inducepcg_shortreal,
+ *                                inducejac_shortreal
+ *                                (v1.2/source/newinduce_shortreal.f:16-610)
  *   thp_set_tcg ................ tcgorder/tcgprec/tcgguess/tcgpeek/tcgomega
  *                                (v1.2/source/initprm.f:447-455) for
  *                                epolar3tcg1/epolar3tcg2
@@ -175,6 +180,13 @@ thp_status thp_set_solver(thp_sys* sys, int polalg);
  * (order + 1 or order + 2 applications of the polarization matrix) and has
  * no convergence test; its stats report niter = order. */
 thp_status thp_set_tcg(thp_sys* sys, int order, int precond, int guess, int peek, double omega);
+/* Short-range real-space mode (the reference's use_polarshortreal, for the
+ * inner steps of RESPA1): with shortcut > 0 (at most the system's cutoff;
+ * the reference's default is 5 A) the permanent field, the matvec and the
+ * solves of thp_induce (PCG or Jacobi/DIIS) use only the real-space Ewald
+ * terms of pairs within shortcut: no reciprocal field and no self term.
+ * The predictor is not used in this mode.  0 restores the full solve. */
+thp_status thp_set_shortreal(thp_sys* sys, double shortcut);
 /* Select the induced-dipole predictor (THP_PRED_*) and clear its history.
  * While a predictor is set, every solve of the system stores its converged
  * dipoles (the last 7 are kept); once 7 are stored, the next solve starts

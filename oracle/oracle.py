@@ -397,3 +397,58 @@ def induce_tcg(orc, order=2, prec=True, guess=False, peek=True, omega=1.0):
     f = 332.063714  # electric (MOD_units.f:81)
     ep = -0.5 * f * float(np.sum(mu[:, 0, :] * ef[:, 1, :]))
     return dict(mu=mu, ep=ep, scalars=sc)
+
+
+def pcg(afun, ef, diag, mu0, poleps, politer):
+    """The preconditioned conjugate gradient of inducepcg_pme2 and
+    inducepcg_shortreal (newinduce_pme2.f:285-636; newinduce_shortreal.f:
+    186-425) on an operator afun: residual norm sqrt(r.(diag r) / 3n) per
+    right-hand side, stop when the larger is below poleps.  Vectors are
+    (n, 2, 3).  Returns (mu, iterations, residual norm)."""
+    n = ef.shape[0]
+    mu = np.array(mu0, dtype=np.float64)
+    res = ef - afun(mu)
+    zr = diag * res
+    pp = zr.copy()
+    ggold = (res * zr).sum(axis=(0, 2))
+    it, rr = 0, np.inf
+    for it in range(1, politer + 1):
+        h = afun(pp)
+        gg = (pp * h).sum(axis=(0, 2))
+        if np.any(gg == 0.0):
+            break
+        a = ggold / gg
+        mu = mu + a[None, :, None] * pp
+        res = res - a[None, :, None] * h
+        zr = diag * res
+        ggnew = (res * zr).sum(axis=(0, 2))
+        rr = float(np.sqrt(ggnew / (3.0 * n)).max())
+        pp = zr + (ggnew / ggold)[None, :, None] * pp
+        ggold = ggnew
+        if rr < poleps:
+            break
+    return mu, it, rr
+
+
+TINYPOL = 1.0e-5  # tinypol (v1.2/source/MOD_polar.f:36)
+
+
+def induce_shortreal(orc, poleps=1e-8, politer=500, jacobi=False):
+    """newinduce_shortreal (newinduce_shortreal.f:16-185) of one system: the
+    direct permanent field and the real-space matvec only (no reciprocal
+    field, no self term) at the oracle's cutoff, which the caller sets to
+    the short cutoff; guess alpha ef; PCG (inducepcg_shortreal) or
+    Jacobi/DIIS (inducejac_shortreal).  Returns dict(mu, ep, niter,
+    resnrm)."""
+    s = orc.sys
+    ef = orc.efld0_direct()
+    alpha = np.asarray(s.polarity, dtype=np.float64)[:, None, None]
+    if jacobi:
+        hfun = lambda mu: orc.tmatxb_pme(mu, dodiag=False)
+        mu, it, rr, _ = jacobi_diis(hfun, ef, alpha, poleps, politer)
+    else:
+        diag = np.where(alpha == 0.0, TINYPOL, alpha)
+        mu, it, rr = pcg(lambda v: orc.tmatxb_pme(v, dodiag=True), ef, diag, alpha * ef, poleps, politer)
+    f = 332.063714  # electric (MOD_units.f:81)
+    ep = -0.5 * f * float(np.sum(mu[:, 0, :] * ef[:, 1, :]))
+    return dict(mu=mu, ep=ep, niter=it, resnrm=rr)

@@ -1,0 +1,81 @@
+"""Short-range real-space solves (the reference's use_polarshortreal with
+ewaldshortcut: newinduce_shortreal, newinduce_shortreal.f:16-610) on the GPU
+against the oracle at the short cutoff: PCG and Jacobi/DIIS, FP64 and mixed,
+the direct-field entry point, and the return to the full solve."""
+import numpy as np
+import pytest
+
+import thp
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+FP64, MIXED = thp.THP_PREC_FP64, thp.THP_PREC_MIXED
+SHORT = 5.0  # ewaldshortcut default (cutoffs.f:59)
+
+
+def _rms_debye(a, b):
+    return float(np.sqrt(np.mean((np.asarray(a) - np.asarray(b)) ** 2)) * thp.DEBYE)
+
+
+def _systems():
+    return {"water81": lambda: thp.tiny_water(27, 20.3, 24),
+            "serving1500": lambda: thp.serving_system(1500, 0.0, seed=44)}
+
+
+@pytest.mark.parametrize("name", ["water81", "serving1500"])
+@pytest.mark.parametrize("prec", [FP64, MIXED])
+@pytest.mark.parametrize("jacobi", [False, True])
+def test_short_range_solve_matches_oracle(ctx, name, prec, jacobi):
+    s = _systems()[name]()
+    ref = O.induce_shortreal(O.Oracle(s, cut=SHORT), poleps=1e-8, jacobi=jacobi)
+    solver = thp.Solver(ctx, s, prec)
+    if jacobi:
+        solver.set_solver(thp.THP_POLALG_JACOBI_DIIS)
+    solver.set_shortreal(SHORT)
+    mu, ep, st = solver.newinduce_pme2(poleps=1e-8)
+    erel = abs(ep - ref["ep"]) / abs(ref["ep"])
+    rms = _rms_debye(mu, ref["mu"])
+    print(f"{name} prec={prec} jacobi={jacobi}: niter {st.niter}/{ref['niter']} erel {erel:.1e} rmsD {rms:.1e}",
+          flush=True)
+    assert st.status == thp.THP_OK
+    if prec == FP64:
+        assert st.niter == ref["niter"] and erel < 1e-11 and rms < 1e-9
+    else:
+        assert abs(st.niter - ref["niter"]) <= 1 and erel < 1e-6 and rms < 1e-5
+    solver.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-11), (MIXED, 2e-5)])
+def test_short_range_direct_field_and_back_to_full(ctx, prec, tol):
+    s = thp.tiny_water(27, 20.3, 24)
+    short = O.Oracle(s, cut=SHORT)
+    full = O.Oracle(s)
+    solver = thp.Solver(ctx, s, prec)
+    solver.set_shortreal(SHORT)
+    ef = solver.efld0_direct()
+    ref = short.efld0_direct()
+    assert np.linalg.norm(ef - ref) < tol * np.linalg.norm(ref)
+    solver.newinduce_pme2(poleps=1e-8)
+    # the full solve again, as if the mode had never been set
+    solver.set_shortreal(0.0)
+    mu, ep, st = solver.newinduce_pme2(poleps=1e-8)
+    r = full.induce(poleps=1e-8)
+    assert st.status == thp.THP_OK and abs(ep - r["ep"]) < (1e-11 if prec == FP64 else 1e-6) * abs(r["ep"])
+    solver.close()
+
+
+def test_short_range_options_are_checked(ctx):
+    s = thp.tiny_water(27, 20.3, 24)
+    solver = thp.Solver(ctx, s, FP64)
+    with pytest.raises(thp.ThpError):
+        solver.set_shortreal(s.cut + 1.0)
+    with pytest.raises(thp.ThpError):
+        solver.set_shortreal(-1.0)
+    solver.set_solver(thp.THP_POLALG_TCG)
+    with pytest.raises(thp.ThpError):
+        solver.set_shortreal(SHORT)  # the reference's short solve takes polalg 1 or 2
+    solver.set_solver(thp.THP_POLALG_PCG)
+    solver.set_shortreal(SHORT)
+    with pytest.raises(thp.ThpError):
+        solver.set_solver(thp.THP_POLALG_TCG)
+    solver.close()

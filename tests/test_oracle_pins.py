@@ -402,3 +402,39 @@ def test_tcg_peek_is_a_scaled_step_along_the_preconditioned_residual(order, prec
     mup, _ = O.tcg(orc.matvec, ef, alpha, order=order, prec=prec, guess=True, peek=True, omega=0.7)
     step = 0.7 * alpha * (ef - orc.matvec(mu))
     assert np.abs((mup - mu) - step).max() < 1e-10 * np.abs(step).max()
+
+
+# ------------------------------------------------- short-range real-space solve
+def test_pcg_restatement_reproduces_the_oracle_pcg():
+    # the numpy PCG used for the short-range solve, on the full operator, is
+    # the C++ oracle's PCG (pinned above against a dense solve)
+    s = thp.tiny_water(27, 20.3, 24)
+    orc = O.Oracle(s)
+    ref = orc.induce(poleps=1e-10)
+    ef = orc.efld0_total()
+    alpha = np.asarray(s.polarity, dtype=np.float64)[:, None, None]
+    mu, it, rr = O.pcg(orc.matvec, ef, alpha, alpha * ef, 1e-10, 500)
+    assert it == ref["niter"]
+    assert np.abs(mu - ref["mu"]).max() < 1e-12 * np.abs(ref["mu"]).max()
+
+
+@pytest.mark.parametrize("jacobi", [False, True])
+def test_short_range_solve_matches_dense_solve(jacobi):
+    # at a 5 A cutoff without reciprocal or self terms: the solution of
+    # (alpha^-1 - T_short) mu = E_short
+    s = thp.tiny_water(27, 20.3, 24)
+    orc = O.Oracle(s, cut=5.0)
+    n6 = 6 * s.n
+    A = np.zeros((n6, n6))
+    e = np.zeros(n6)
+    for j in range(n6):
+        e[:] = 0.0
+        e[j] = 1.0
+        A[:, j] = orc.tmatxb_pme(e.reshape(s.n, 2, 3), dodiag=True).reshape(-1)
+    assert np.abs(A - A.T).max() < 1e-12 * np.abs(A).max()
+    mu = np.linalg.solve(A, orc.efld0_direct().reshape(-1)).reshape(s.n, 2, 3)
+    res = O.induce_shortreal(orc, poleps=1e-12, jacobi=jacobi)
+    assert np.abs(res["mu"] - mu).max() < 1e-9 * np.abs(mu).max()
+    # and it differs from the full solve
+    full = O.Oracle(s).induce(poleps=1e-12)
+    assert np.abs(full["mu"] - mu).max() > 1e-4 * np.abs(mu).max()

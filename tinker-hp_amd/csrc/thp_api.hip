@@ -312,14 +312,22 @@ thp_status convolve(thp_ctx* ctx, const Batch& b, int gate) {
 
 template <int MODE>
 thp_status launch_gather_uind(thp_ctx* ctx, const Batch& b, int vin, int vout, int frac_out, int gate) {
-  if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_tile<double, MODE>), g1(b.n_gat), dim3(64), 0, ctx->stream, b.d_views, b.d_gat, vin, vout, frac_out, gate);
-  else hipLaunchKernelGGL((k_gather_tile<float, MODE>), g1(b.n_gat), dim3(64), 0, ctx->stream, b.d_views, b.d_gat, vin, vout, frac_out, gate);
+  const dim3 g = g1(b.n_gat), t(64);
+  if (b.real_only) {
+    // the PCG steps of a short-range solve, without the reciprocal field
+    if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_tile<double, MODE, false>), g, t, 0, ctx->stream, b.d_views, b.d_gat, vin, vout, frac_out, gate);
+    else hipLaunchKernelGGL((k_gather_tile<float, MODE, false>), g, t, 0, ctx->stream, b.d_views, b.d_gat, vin, vout, frac_out, gate);
+  } else {
+    if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_tile<double, MODE>), g, t, 0, ctx->stream, b.d_views, b.d_gat, vin, vout, frac_out, gate);
+    else hipLaunchKernelGGL((k_gather_tile<float, MODE>), g, t, 0, ctx->stream, b.d_views, b.d_gat, vin, vout, frac_out, gate);
+  }
   return launch_check(ctx, "k_gather_tile");
 }
 
 // permanent field: ef = E_direct - grad phi_recip + self (newinduce_pme2.f:111-147)
 thp_status field(thp_ctx* ctx, const Batch& b, double* cphi_dev, int add_to_ef, int do_direct, int gate) {
   if (do_direct) RET(pair_efld0(ctx, b, gate));
+  if (b.real_only && add_to_ef) return THP_OK;  // short range: the direct field alone
   RET(spread_mpole(ctx, b, gate));
   RET(convolve(ctx, b, gate));
   if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_mpole<double>), g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, cphi_dev, add_to_ef, gate);
@@ -625,8 +633,10 @@ thp_status b_setup_field(thp_ctx* ctx, const Batch& b, int gate) {
 thp_status b_setup(thp_ctx* ctx, const Batch& b, int gate) {
   RET(b_setup_field(ctx, b, gate));
   RET(b_pair_tmatxb(ctx, b, 0 /*mu*/, 3 /*hreal*/, 1, gate));
-  RET(spread_uind(ctx, b, 0, 0, gate));
-  RET(convolve(ctx, b, gate));
+  if (!b.real_only) {
+    RET(spread_uind(ctx, b, 0, 0, gate));
+    RET(convolve(ctx, b, gate));
+  }
   return launch_gather_uind<1>(ctx, b, 0, 3, 0, gate);
 }
 
@@ -634,8 +644,10 @@ thp_status b_setup(thp_ctx* ctx, const Batch& b, int gate) {
 // each system's iteration number lives in its SolveState.
 thp_status b_iteration(thp_ctx* ctx, const Batch& b) {
   RET(b_pair_tmatxb(ctx, b, 1 /*pp*/, 3 /*hreal*/, 1, kGateIter));
-  RET(spread_uind(ctx, b, 1, 0, kGateIter));
-  RET(convolve(ctx, b, kGateIter));
+  if (!b.real_only) {
+    RET(spread_uind(ctx, b, 1, 0, kGateIter));
+    RET(convolve(ctx, b, kGateIter));
+  }
   RET(launch_gather_uind<2>(ctx, b, 1, 3, 0, kGateIter));
   hipStream_t st = ctx->stream;
   if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_cg_update1<double>), g1(b.n_red), dim3(256), 0, st, b.d_views, b.d_red);
@@ -1416,8 +1428,11 @@ thp_status jacobi_iteration(thp_sys* s, int it, int* nmat, int* first) {
   const Batch& b = s->one;
   hipStream_t st = ctx->stream;
   RET(b_pair_tmatxb(ctx, b, V_MU, V_HREAL, 0, kGateNone));
-  RET(spread_uind(ctx, b, V_MU, 0, kGateNone));
-  RET(convolve(ctx, b, kGateNone));
+  if (!b.real_only) {
+    RET(spread_uind(ctx, b, V_MU, 0, kGateNone));
+    RET(convolve(ctx, b, kGateNone));
+  }
+  // without the reciprocal terms the gather writes a zero field into h
   RET(launch_gather_uind<0>(ctx, b, V_MU, V_H, 0, kGateNone));
   const int slot = (*first + *nmat - 1) % kDiisMax;
   int shifted = 0;
@@ -1487,7 +1502,10 @@ thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_s
   if (tcg) warm = false;
   // with a predictor and a full history the extrapolated guess replaces
   // opts->guess, as use_pred precedes polgsf in the reference
-  if (!tcg && s->pred != THP_PRED_NONE && s->pred_n == kMaxUalt) {
+  // (the short-range solve keeps no predictor history here: the
+  // reference's is a separate one, udshortalt)
+  const bool short_range = s->short_cut > 0.0;
+  if (!tcg && !short_range && s->pred != THP_PRED_NONE && s->pred_n == kMaxUalt) {
     RET(predict_guess(s));
     warm = true;
   }
@@ -1507,7 +1525,7 @@ thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_s
     if (conv) break;
   }
   RET(b_finish(ctx, b, 0));
-  if (s->pred != THP_PRED_NONE) RET(push_history(s));
+  if (s->pred != THP_PRED_NONE && !short_range) RET(push_history(s));
   HIPCHK(ctx, hipEventRecord(s->ev[2], st));
   SolveState fin;
   HIPCHK(ctx, hipMemcpyAsync(&fin, s->view.st, sizeof fin, hipMemcpyDeviceToHost, st));
@@ -1536,6 +1554,8 @@ thp_status thp_set_solver(thp_sys* s, int polalg) {
   if (!s) return THP_ERR_ARG;
   if (polalg != THP_POLALG_PCG && polalg != THP_POLALG_JACOBI_DIIS && polalg != THP_POLALG_TCG)
     return fail(s->ctx, THP_ERR_ARG, "unknown polalg %d (1: PCG, 2: Jacobi/DIIS, 3: truncated CG)", polalg);
+  if (polalg == THP_POLALG_TCG && s->short_cut > 0.0)
+    return fail(s->ctx, THP_ERR_ARG, "the short-range solve takes polalg 1 or 2 (newinduce_shortreal.f:134-141)");
   if (polalg == THP_POLALG_TCG && !s->tcg_vec) {
     s->tcg_blocks = (s->n + 255) / 256;
     RET(dalloc(s, &s->tcg_vec, (size_t)2 * 6 * s->npad));
@@ -1551,6 +1571,26 @@ thp_status thp_set_solver(thp_sys* s, int polalg) {
   }
   s->polalg = polalg;
   return THP_OK;
+}
+
+thp_status thp_set_shortreal(thp_sys* s, double shortcut) {
+  using namespace thp;
+  if (!s) return THP_ERR_ARG;
+  if (!(shortcut >= 0.0) || shortcut > s->cut)
+    return fail(s->ctx, THP_ERR_ARG, "short-range cutoff %g must lie in (0, %g] (0 turns the mode off)", shortcut, s->cut);
+  if (shortcut > 0.0 && s->polalg == THP_POLALG_TCG)
+    return fail(s->ctx, THP_ERR_ARG, "the short-range solve takes polalg 1 or 2 (newinduce_shortreal.f:134-141)");
+  const bool on = shortcut > 0.0;
+  s->short_cut = shortcut;
+  s->one.real_only = on ? 1 : 0;
+  // the block-union lists (built for the full cutoff plus the skin) hold
+  // every short-range pair: only the exact cutoff test and the self term
+  // change, and the per-solve coefficients are rebuilt
+  s->view.cut2 = on ? shortcut * shortcut : s->cut * s->cut;
+  s->view.term_self = on ? 0.0 : (4.0 / 3.0) * s->aewald * s->aewald * s->aewald / std::sqrt(M_PI);
+  s->coef_valid = false;
+  RET(upload_view(s));
+  return sync(s);
 }
 
 thp_status thp_set_tcg(thp_sys* s, int order, int precond, int guess, int peek, double omega) {

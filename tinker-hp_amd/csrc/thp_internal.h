@@ -192,6 +192,7 @@ struct Batch {
   Work* d_fft = nullptr;        // FFT line groups (the most of the three axes)
   int n_fft = 0;
   int* d_ids = nullptr;         // [nsys] = 0..nsys-1
+  int real_only = 0;            // short-range solves: no reciprocal terms
 };
 
 }  // namespace thp
@@ -235,6 +236,8 @@ struct thp_sys {
   double* diis_part = nullptr;  // [diis_blocks][kDiisParts]
   thp::DiisState* diis = nullptr;
   int diis_blocks = 0;
+  // short-range real-space solves (thp_set_shortreal): the cutoff, 0 = off
+  double short_cut = 0.0;
   // truncated conjugate gradient (thp_set_tcg; initprm.f:447-455 defaults)
   int tcg_order = 2, tcg_prec = 1, tcg_guess = 0, tcg_peek = 1;
   double tcg_omega = 1.0;

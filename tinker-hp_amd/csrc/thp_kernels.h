@@ -1817,7 +1817,9 @@ __global__ void __launch_bounds__(256) k_fft_axis(const SysView* views, const Wo
 // frac_out, the fractional derivatives); MODE 1 also starts the PCG
 // (res = ef - h, zr, pp, newinduce_pme2.f:406-438) and MODE 2 also forms
 // p.h (newinduce_pme2.f:452-473), with one partial pair per brick.
-template <typename real, int MODE>
+// Without RECIP (the short-range solve of newinduce_shortreal, with no
+// reciprocal field) the kernel only carries out the PCG steps.
+template <typename real, int MODE, bool RECIP = true>
 __global__ void __launch_bounds__(64) k_gather_tile(const SysView* views, const Work* bwork, int vin, int vout,
                                                      int frac_out, int gate) {
   __shared__ real s_g[kTile3 * 2];
@@ -1837,7 +1839,7 @@ __global__ void __launch_bounds__(64) k_gather_tile(const SysView* views, const 
   const int lane = threadIdx.x;
   // tile load: a batch of loads in flight per lane before its LDS writes
   // (points outside the brick's extent are never read and are left as 0)
-  {
+  if (RECIP) {
     typedef real R2 __attribute__((ext_vector_type(2)));
     const GLOBAL R2* GRID2 = reinterpret_cast<const GLOBAL R2*>(gp(static_cast<const real*>(sv.grid)));
     R2* s_g2 = reinterpret_cast<R2*>(s_g);
@@ -1888,7 +1890,7 @@ __global__ void __launch_bounds__(64) k_gather_tile(const SysView* views, const 
     }
     double a[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll 1
-    for (int mz = 0; mz < kOrder; ++mz) {
+    for (int mz = 0; mz < (RECIP ? kOrder : 0); ++mz) {
       real v0 = tz0[0], v1 = tz1[0];
 #pragma unroll
       for (int m = 1; m < kOrder; ++m)

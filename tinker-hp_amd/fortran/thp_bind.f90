@@ -136,6 +136,11 @@ module thp_bind
       integer(c_int), value :: order, precond, guess, peek
       real(c_double), value :: omega
     end function
+    integer(c_int) function thp_set_shortreal(sys, shortcut) bind(C, name="thp_set_shortreal")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: sys
+      real(c_double), value :: shortcut
+    end function
     integer(c_int) function thp_set_predictor(sys, method) bind(C, name="thp_set_predictor")
       import :: c_int, c_ptr
       type(c_ptr), value :: sys

@@ -181,6 +181,13 @@ contains
     call fatal_status(thp_set_solver(sys, int(polalg, c_int)), 'thp_set_solver')
   end subroutine
 
+  ! use_polarshortreal with ewaldshortcut (cutoffs.f:59): the short-range
+  ! solves of RESPA1's inner steps; 0 restores the full solve
+  subroutine polar_set_shortreal(shortcut)
+    real(8), intent(in) :: shortcut
+    call fatal_status(thp_set_shortreal(sys, real(shortcut, c_double)), 'thp_set_shortreal')
+  end subroutine
+
   ! the truncated CG options of POLAR-ALG 3 (tcgorder, tcgprec, tcgguess,
   ! tcgpeek, tcgomega; defaults 2, .true., .false., .true., 1.0 in
   ! initprm.f:447-455)

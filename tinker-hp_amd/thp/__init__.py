@@ -138,6 +138,7 @@ def load_lib(path: str | None = None) -> C.CDLL:
         "thp_set_predictor": (C.c_int, [vp, C.c_int]),
         "thp_set_solver": (C.c_int, [vp, C.c_int]),
         "thp_set_tcg": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]),
+        "thp_set_shortreal": (C.c_int, [vp, C.c_double]),
         "thp_tmatxb_dev": (C.c_int, [vp, vp, vp, C.c_int]),
         "thp_induce_dev": (C.c_int, [vp, C.POINTER(SolveOpts), vp, _dp, C.POINTER(SolveStats)]),
         "thp_serve_start": (C.c_int, [vp, C.POINTER(ServeOpts)]),
@@ -411,6 +412,11 @@ class Solver:
     def set_solver(self, polalg: int):
         """THP_POLALG_PCG (1), THP_POLALG_JACOBI_DIIS (2) or THP_POLALG_TCG (3)."""
         self._c(self.lib.thp_set_solver(self.h, int(polalg)), "set_solver")
+
+    def set_shortreal(self, shortcut: float):
+        """Short-range real-space mode (use_polarshortreal) with this cutoff;
+        0 restores the full solve."""
+        self._c(self.lib.thp_set_shortreal(self.h, float(shortcut)), "set_shortreal")
 
     def set_tcg(self, order: int = 2, precond: bool = True, guess: bool = False, peek: bool = True,
                 omega: float = 1.0):
