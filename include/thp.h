@@ -41,6 +41,9 @@
  *                                (v1.2/source/newinduce_pme2.f:210-217,
  *                                 638-890; diis.f) or the truncated CG
  *                                (v1.2/source/epolar3tcg.f:18-56)
+ *   thp_system_set_frames ...... rotpole + rotmat + rotsite
+ *                                (v1.2/source/rotpole.f:24-339) and chkpole
+ *                                (v1.2/source/chkpole.f:20-98), each step
  *   thp_set_shortreal .......... use_polarshortreal with ewaldshortcut
  *                                (v1.2/source/cutoffs.f:59,228; switch.f:60-62):
  *                                newinduce_shortreal, inducepcg_shortreal,
@@ -141,6 +144,21 @@ thp_status thp_system_update_coords(thp_sys* sys, const double* x, const double*
 /* new global-frame multipoles rpole [n][13] (as rotpole leaves them each MD
  * step, v1.2/source/rotpole.f); the next field and solve use them */
 thp_status thp_system_update_multipoles(thp_sys* sys, const double* rpole);
+/* Local-frame multipoles (rotpole, v1.2/source/rotpole.f:24-339, with
+ * chkpole, v1.2/source/chkpole.f:20-98): per atom the axis type
+ * (THP_AXIS_*), its frame atoms (0-based; -1 where the type takes none)
+ * and the multipoles in that frame, [n][13] in the rpole convention (as
+ * kmpole leaves pole: dipoles in e A, quadrupoles in e A^2 divided by 3).
+ * The device rotates them to the global frame now and after every
+ * coordinate update; thp_system_update_multipoles drops the frames. */
+#define THP_AXIS_NONE 0
+#define THP_AXIS_Z_ONLY 1
+#define THP_AXIS_Z_THEN_X 2
+#define THP_AXIS_BISECTOR 3
+#define THP_AXIS_Z_BISECT 4
+#define THP_AXIS_3_FOLD 5
+thp_status thp_system_set_frames(thp_sys* sys, const int* axistype, const int* zaxis, const int* xaxis,
+                                 const int* yaxis, const double* pole);
 thp_status thp_system_destroy(thp_sys* sys);
 /* device memory held by the system (bytes) */
 thp_status thp_system_footprint(const thp_sys* sys, int64_t* bytes);

@@ -452,3 +452,62 @@ def induce_shortreal(orc, poleps=1e-8, politer=500, jacobi=False):
     f = 332.063714  # electric (MOD_units.f:81)
     ep = -0.5 * f * float(np.sum(mu[:, 0, :] * ef[:, 1, :]))
     return dict(mu=mu, ep=ep, niter=it, resnrm=rr)
+
+
+AXIS_NONE, AXIS_Z_ONLY, AXIS_Z_THEN_X, AXIS_BISECTOR, AXIS_Z_BISECT, AXIS_3_FOLD = range(6)
+
+
+def rotmat(xyz, i, axt, iz, ix, iy):
+    """Rotation matrix of site i (rotmat, rotpole.f:61-254): columns are the
+    local x, y, z axes in the global frame.  Z-Only takes the global x (or
+    y) axis where the reference draws a random vector."""
+    a = np.eye(3)
+    if axt == AXIS_NONE:
+        return a
+    unit = lambda v: v / np.linalg.norm(v)
+    d1 = unit(xyz[iz] - xyz[i])
+    if axt == AXIS_Z_ONLY:
+        z = d1
+        u = np.array([1.0, 0.0, 0.0]) if abs(z[0]) < 0.866 else np.array([0.0, 1.0, 0.0])
+    elif axt == AXIS_Z_THEN_X:
+        z = d1
+        u = xyz[ix] - xyz[i]
+    elif axt == AXIS_BISECTOR:
+        d2 = unit(xyz[ix] - xyz[i])
+        z = unit(d1 + d2)
+        u = d2
+    elif axt == AXIS_Z_BISECT:
+        z = d1
+        u = unit(unit(xyz[ix] - xyz[i]) + unit(xyz[iy] - xyz[i]))
+    else:
+        d2 = unit(xyz[ix] - xyz[i])
+        z = unit(d1 + d2 + unit(xyz[iy] - xyz[i]))
+        u = d2
+    x = unit(u - np.dot(u, z) * z)
+    a[:, 2] = z
+    a[:, 0] = x
+    a[:, 1] = np.cross(z, x)
+    return a
+
+
+def rotpole(x, y, z, axistype, zaxis, xaxis, yaxis, pole):
+    """Global-frame multipoles (n, 13) from local ones: chkpole
+    (chkpole.f:20-98: a Z-then-X site with a y-axis atom has its y
+    components inverted while the parallelepiped volume is negative), then
+    rotsite (rotpole.f:261-339): d = A d_local, Q = A Q_local A^T."""
+    xyz = np.stack([x, y, z], axis=1).astype(np.float64)
+    n = len(x)
+    out = np.zeros((n, 13))
+    for i in range(n):
+        p = np.array(pole[i], dtype=np.float64)
+        t = int(axistype[i])
+        if t == AXIS_Z_THEN_X and yaxis[i] >= 0:
+            ad, bd, cd = xyz[i] - xyz[yaxis[i]], xyz[zaxis[i]] - xyz[yaxis[i]], xyz[xaxis[i]] - xyz[yaxis[i]]
+            vol = np.dot(ad, np.cross(bd, cd))
+            if vol < 0.0:
+                p[[2, 5, 7, 9, 11]] = -p[[2, 5, 7, 9, 11]]
+        a = rotmat(xyz, i, t, zaxis[i], xaxis[i], yaxis[i])
+        out[i, 0] = p[0]
+        out[i, 1:4] = a @ p[1:4]
+        out[i, 4:13] = (a @ p[4:13].reshape(3, 3) @ a.T).reshape(-1)
+    return out

@@ -438,3 +438,102 @@ def test_short_range_solve_matches_dense_solve(jacobi):
     # and it differs from the full solve
     full = O.Oracle(s).induce(poleps=1e-12)
     assert np.abs(full["mu"] - mu).max() > 1e-4 * np.abs(mu).max()
+
+
+# ------------------------------------------------------ local frames (rotpole)
+def _frame_case(seed=5, nmol=12):
+    """waters with every axis type: O bisector or Z-Bisect or 3-Fold, H
+    Z-then-X (some with a y atom, so chiral) or Z-Only; random local
+    multipoles (axially symmetric for Z-Only)"""
+    rng = np.random.default_rng(seed)
+    xyz = []
+    for m in range(nmol):
+        o = rng.uniform(0, 12, 3)
+        r = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+        h1 = o + r @ np.array([0.9572, 0.0, 0.0])
+        ang = np.deg2rad(104.52)
+        h2 = o + r @ np.array([0.9572 * np.cos(ang), 0.9572 * np.sin(ang), 0.0])
+        xyz += [o, h1, h2]
+    xyz = np.array(xyz)
+    n = len(xyz)
+    axt = np.zeros(n, np.int32)
+    fz, fx, fy = (np.full(n, -1, np.int32) for _ in range(3))
+    pole = np.zeros((n, 13))
+    for m in range(nmol):
+        o, h1, h2 = 3 * m, 3 * m + 1, 3 * m + 2
+        other = 3 * ((m + 1) % nmol)
+        kind = m % 3
+        axt[o], fz[o], fx[o] = [O.AXIS_BISECTOR, O.AXIS_Z_BISECT, O.AXIS_3_FOLD][kind], h1, h2
+        if kind:
+            fy[o] = other if kind == 2 else h2
+            if kind == 1:
+                fx[o], fy[o] = h2, other
+        axt[h1], fz[h1], fx[h1] = O.AXIS_Z_THEN_X, o, h2
+        fy[h1] = other if m % 2 == 0 else -1
+        if m % 4 == 3:
+            axt[h2], fz[h2] = O.AXIS_Z_ONLY, o
+        else:
+            axt[h2], fz[h2], fx[h2], fy[h2] = O.AXIS_Z_THEN_X, o, h1, other
+    for i in range(n):
+        q = rng.normal(0, 0.1, (3, 3))
+        q = q + q.T
+        q -= np.trace(q) / 3 * np.eye(3)
+        d = rng.normal(0, 0.1, 3)
+        if axt[i] == O.AXIS_Z_ONLY:
+            d[:2] = 0.0
+            q = np.diag([-0.5, -0.5, 1.0]) * rng.normal(0, 0.1)
+        if not (axt[i] == O.AXIS_Z_THEN_X and fy[i] >= 0):
+            # sites without a chirality check have parameters symmetric
+            # under y -> -y, as force-field parameters do
+            d[1] = 0.0
+            q[0, 1] = q[1, 0] = q[1, 2] = q[2, 1] = 0.0
+        pole[i] = np.concatenate([[rng.normal(0, 0.3)], d, q.reshape(-1)])
+    return xyz, axt, fz, fx, fy, pole
+
+
+def _transform(rp, m):
+    out = rp.copy()
+    out[:, 1:4] = rp[:, 1:4] @ m.T
+    out[:, 4:13] = np.einsum("ij,njk,lk->nil", m, rp[:, 4:13].reshape(-1, 3, 3), m).reshape(-1, 9)
+    return out
+
+
+def test_rotpole_is_covariant_under_rigid_rotation():
+    xyz, axt, fz, fx, fy, pole = _frame_case()
+    rp = O.rotpole(*xyz.T, axt, fz, fx, fy, pole)
+    r = np.linalg.qr(np.random.default_rng(9).normal(size=(3, 3)))[0]
+    if np.linalg.det(r) < 0:
+        r[:, 0] = -r[:, 0]
+    xyz2 = xyz @ r.T + np.array([3.0, -1.0, 2.0])
+    rp2 = O.rotpole(*xyz2.T, axt, fz, fx, fy, pole)
+    np.testing.assert_allclose(rp2, _transform(rp, r), atol=1e-12)
+
+
+def test_rotpole_frames_point_at_their_atoms():
+    xyz, axt, fz, fx, fy, pole = _frame_case()
+    for i in range(len(xyz)):
+        a = O.rotmat(xyz, i, axt[i], fz[i], fx[i], fy[i])
+        assert np.abs(a.T @ a - np.eye(3)).max() < 1e-13 and abs(np.linalg.det(a) - 1.0) < 1e-13
+        if axt[i] == O.AXIS_Z_THEN_X:
+            dz, dx = xyz[fz[i]] - xyz[i], xyz[fx[i]] - xyz[i]
+            np.testing.assert_allclose(a[:, 2], dz / np.linalg.norm(dz), atol=1e-14)
+            assert abs(np.dot(a[:, 1], dx)) < 1e-13 and np.dot(a[:, 0], dx) > 0   # x atom in the xz half-plane
+        if axt[i] == O.AXIS_BISECTOR:
+            u = [(xyz[k] - xyz[i]) / np.linalg.norm(xyz[k] - xyz[i]) for k in (fz[i], fx[i])]
+            assert abs(np.dot(a[:, 2], u[0]) - np.dot(a[:, 2], u[1])) < 1e-13   # z bisects the two bonds
+
+
+def test_rotpole_mirror_image_needs_the_chirality_check():
+    # the mirror image of the geometry gives the mirror image of the global
+    # multipoles only with chkpole's inversion of the y components
+    xyz, axt, fz, fx, fy, pole = _frame_case()
+    m = np.diag([-1.0, 1.0, 1.0])
+    rp = O.rotpole(*xyz.T, axt, fz, fx, fy, pole)
+    rpm = O.rotpole(*(xyz @ m.T).T, axt, fz, fx, fy, pole)
+    np.testing.assert_allclose(rpm, _transform(rp, m), atol=1e-12)
+    chiral = (axt == O.AXIS_Z_THEN_X) & (fy >= 0)
+    assert chiral.sum() >= 4
+    no_y = fy.copy()
+    no_y[chiral] = -1   # the same frames without the check
+    rpn = O.rotpole(*(xyz @ m.T).T, axt, fz, fx, no_y, pole)
+    assert np.abs(rpn[chiral] - _transform(rp, m)[chiral]).max() > 1e-3

@@ -506,6 +506,14 @@ thp_status ensure_coef(thp_sys* s) {
 // (3*npad doubles of pinned host memory, untouched until the stream passes
 // the copies) the upload is enqueued without waiting; otherwise it is
 // synchronous.
+// local-frame multipoles to the global frame on the device (rotpole)
+thp_status rotate_frames(thp_sys* s) {
+  hipLaunchKernelGGL(k_rotpole, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream, s->d_view, s->d_frames,
+                     s->d_pole_local);
+  RET(launch_check(s->ctx, "k_rotpole"));
+  return sync(s);
+}
+
 thp_status update_coords(thp_sys* s, const double* x, const double* y, const double* z, double* pinned) {
   const int n = s->n, np = s->npad;
   // sorted copy in ordinary host memory (pinned memory is only written,
@@ -557,6 +565,8 @@ thp_status update_coords(thp_sys* s, const double* x, const double* y, const dou
   s->verlet_valid = false;
   s->pme_valid = false;
   s->coef_valid = false;
+  // the global multipoles follow the frames (rotpole each step)
+  if (s->has_frames) RET(rotate_frames(s));
   return THP_OK;
 }
 
@@ -1210,8 +1220,50 @@ thp_status thp_system_update_coords(thp_sys* s, const double* x, const double* y
   return update_coords(s, x, y, z, nullptr);
 }
 
+thp_status thp_system_set_frames(thp_sys* s, const int* axistype, const int* zaxis, const int* xaxis, const int* yaxis,
+                                 const double* pole) {
+  if (!s || !axistype || !zaxis || !xaxis || !yaxis || !pole) return THP_ERR_ARG;
+  const int n = s->n;
+  const size_t np = s->npad;
+  std::vector<int> fr((size_t)4 * np, -1);
+  std::vector<double> pl((size_t)10 * np, 0.0);
+  for (int q = 0; q < (int)np; ++q) fr[(size_t)4 * q] = kAxisNone;
+  for (int i = 0; i < n; ++i) {
+    const int t = axistype[i];
+    if (t < kAxisNone || t > kAxis3Fold) return fail(s->ctx, THP_ERR_ARG, "atom %d: axis type %d", i, t);
+    // frame atoms each type needs (rotpole.f: z for all but None, x for all
+    // but None and Z-Only, y for Z-Bisect and 3-Fold, optional for Z-then-X)
+    const bool nz = t != kAxisNone, nx = t >= kAxisZThenX, ny = t == kAxisZBisect || t == kAxis3Fold;
+    const int f[3] = {zaxis[i], xaxis[i], yaxis[i]};
+    const bool need[3] = {nz, nx, ny};
+    const bool allowed[3] = {nz, nx, ny || t == kAxisZThenX};
+    for (int k = 0; k < 3; ++k) {
+      if (f[k] >= 0 && (f[k] >= n || f[k] == i || !allowed[k]))
+        return fail(s->ctx, THP_ERR_ARG, "atom %d: frame atom %d (axis %d) invalid for axis type %d", i, f[k], k, t);
+      if (need[k] && f[k] < 0) return fail(s->ctx, THP_ERR_ARG, "atom %d: axis type %d needs frame atom %d", i, t, k);
+    }
+    const int q = s->iperm[i];
+    fr[(size_t)4 * q] = t;
+    for (int k = 0; k < 3; ++k) fr[(size_t)4 * q + 1 + k] = f[k] >= 0 && allowed[k] ? s->iperm[f[k]] : -1;
+    const double* r = pole + (size_t)i * 13;
+    const double v[10] = {r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[8], r[9], r[12]};
+    for (int c = 0; c < 10; ++c) pl[(size_t)c * np + q] = v[c];
+  }
+  if (!s->d_frames) {
+    RET(dalloc(s, &s->d_frames, (size_t)4 * np));
+    RET(dalloc(s, &s->d_pole_local, (size_t)10 * np));
+  }
+  HIPCHK(s->ctx, hipMemcpy(s->d_frames, fr.data(), sizeof(int) * fr.size(), hipMemcpyHostToDevice));
+  HIPCHK(s->ctx, hipMemcpy(s->d_pole_local, pl.data(), sizeof(double) * pl.size(), hipMemcpyHostToDevice));
+  s->has_frames = true;
+  s->pme_valid = false;
+  s->coef_valid = false;
+  return rotate_frames(s);
+}
+
 thp_status thp_system_update_multipoles(thp_sys* s, const double* rpole) {
   if (!s || !rpole) return THP_ERR_ARG;
+  s->has_frames = false;  // global multipoles given outright replace the frames
   const int n = s->n;
   const size_t np = s->npad;
   std::vector<double> rp((size_t)10 * np, 0.0);

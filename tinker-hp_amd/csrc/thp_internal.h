@@ -238,6 +238,10 @@ struct thp_sys {
   int diis_blocks = 0;
   // short-range real-space solves (thp_set_shortreal): the cutoff, 0 = off
   double short_cut = 0.0;
+  // local-frame multipoles (thp_system_set_frames), rotated on the device
+  int* d_frames = nullptr;         // [npad] x (axis type, z, x, y), sorted slots
+  double* d_pole_local = nullptr;  // [10][npad], the rpole layout
+  bool has_frames = false;
   // truncated conjugate gradient (thp_set_tcg; initprm.f:447-455 defaults)
   int tcg_order = 2, tcg_prec = 1, tcg_guess = 0, tcg_peek = 1;
   double tcg_omega = 1.0;

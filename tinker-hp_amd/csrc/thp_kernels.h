@@ -2507,6 +2507,132 @@ __global__ void __launch_bounds__(256) k_cg_update2(const SysView* views, const 
   if (threadIdx.x == 0 && sc[2] != kRunning) st->conv = int(sc[2]);
 }
 
+// ===================================================== local frames (rotpole)
+// rotmat + rotsite (rotpole.f:24-339) with chkpole (chkpole.f:20-98) for
+// every site whose frame is set: the rotation matrix from the positions of
+// the site and its frame atoms (columns: the local x, y, z axes in the
+// global frame; no minimum image, as without use_polymer), then
+// d = A d_local and Q = A Q_local A^T.  chkpole inverts the y components of
+// a Z-then-X site with a y-axis atom while the parallelepiped volume of
+// (site, z, x) about y is negative: applied here from the volume at each
+// call, it leaves the same multipoles as the reference's in-place sign
+// flips.  Z-Only takes the global x axis (or y, when z lies near x) for
+// the local x axis where the reference draws a random one: Z-Only
+// parameters are symmetric about z, so the result does not depend on it.
+// frames[q] = (axis type, z, x, y) in sorted slots, -1 for none; the local
+// multipoles are [10][npad] in the layout of rpole.
+enum : int { kAxisNone = 0, kAxisZOnly = 1, kAxisZThenX = 2, kAxisBisector = 3, kAxisZBisect = 4, kAxis3Fold = 5 };
+
+__device__ __forceinline__ void unit3(double& x, double& y, double& z) {
+  const double r = sqrt(x * x + y * y + z * z);
+  x /= r;
+  y /= r;
+  z /= r;
+}
+
+__global__ void __launch_bounds__(256) k_rotpole(const SysView* views, const int* frames, const double* pole_local) {
+  const SysView& sv = views[0];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= sv.n) return;
+  const size_t np = sv.npad;
+  const int4 f = reinterpret_cast<const int4*>(frames)[i];
+  const int axt = f.x, iz = f.y, ix = f.z, iy = f.w;
+  const double xi = sv.x[i], yi = sv.y[i], zi = sv.z[i];
+  auto vec = [&](int k, double& dx, double& dy, double& dz) {
+    dx = sv.x[k] - xi;
+    dy = sv.y[k] - yi;
+    dz = sv.z[k] - zi;
+  };
+  // a[r][c]: column 0 the local x axis, column 2 the local z axis
+  double a[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+  double zx = 0, zy = 0, zz = 1, xx = 1, xy = 0, xz = 0;
+  if (axt != kAxisNone) {
+    double d1x, d1y, d1z;
+    vec(iz, d1x, d1y, d1z);
+    unit3(d1x, d1y, d1z);
+    double ux = 0, uy = 0, uz = 0;  // the vector the x axis is taken from
+    if (axt == kAxisZOnly) {
+      zx = d1x, zy = d1y, zz = d1z;
+      if (fabs(zx) < 0.866) ux = 1.0;
+      else uy = 1.0;
+    } else if (axt == kAxisZThenX) {
+      zx = d1x, zy = d1y, zz = d1z;
+      vec(ix, ux, uy, uz);
+    } else if (axt == kAxisBisector) {
+      double d2x, d2y, d2z;
+      vec(ix, d2x, d2y, d2z);
+      unit3(d2x, d2y, d2z);
+      zx = d1x + d2x, zy = d1y + d2y, zz = d1z + d2z;
+      unit3(zx, zy, zz);
+      ux = d2x, uy = d2y, uz = d2z;
+    } else if (axt == kAxisZBisect) {
+      zx = d1x, zy = d1y, zz = d1z;
+      double d2x, d2y, d2z, d3x, d3y, d3z;
+      vec(ix, d2x, d2y, d2z);
+      unit3(d2x, d2y, d2z);
+      vec(iy, d3x, d3y, d3z);
+      unit3(d3x, d3y, d3z);
+      ux = d2x + d3x, uy = d2y + d3y, uz = d2z + d3z;
+      unit3(ux, uy, uz);
+    } else {  // 3-Fold
+      double d2x, d2y, d2z, d3x, d3y, d3z;
+      vec(ix, d2x, d2y, d2z);
+      unit3(d2x, d2y, d2z);
+      vec(iy, d3x, d3y, d3z);
+      unit3(d3x, d3y, d3z);
+      zx = d1x + d2x + d3x, zy = d1y + d2y + d3y, zz = d1z + d2z + d3z;
+      unit3(zx, zy, zz);
+      ux = d2x, uy = d2y, uz = d2z;
+    }
+    const double dot = ux * zx + uy * zy + uz * zz;
+    xx = ux - dot * zx, xy = uy - dot * zy, xz = uz - dot * zz;
+    unit3(xx, xy, xz);
+    a[0][2] = zx, a[1][2] = zy, a[2][2] = zz;
+    a[0][0] = xx, a[1][0] = xy, a[2][0] = xz;
+  }
+  // y = z x x (rotpole.f: a(1,2) = a(3,1) a(2,3) - a(2,1) a(3,3), ...)
+  a[0][1] = a[2][0] * a[1][2] - a[1][0] * a[2][2];
+  a[1][1] = a[0][0] * a[2][2] - a[2][0] * a[0][2];
+  a[2][1] = a[1][0] * a[0][2] - a[0][0] * a[1][2];
+  double p[10];
+#pragma unroll
+  for (int c = 0; c < 10; ++c) p[c] = pole_local[(size_t)c * np + i];
+  if (axt == kAxisZThenX && iy >= 0) {
+    const double xad = xi - sv.x[iy], yad = yi - sv.y[iy], zad = zi - sv.z[iy];
+    const double xbd = sv.x[iz] - sv.x[iy], ybd = sv.y[iz] - sv.y[iy], zbd = sv.z[iz] - sv.z[iy];
+    const double xcd = sv.x[ix] - sv.x[iy], ycd = sv.y[ix] - sv.y[iy], zcd = sv.z[ix] - sv.z[iy];
+    const double c1 = ybd * zcd - zbd * ycd, c2 = ycd * zad - zcd * yad, c3 = yad * zbd - zad * ybd;
+    const double vol = xad * c1 + xbd * c2 + xcd * c3;
+    if (vol < 0.0) {
+      p[2] = -p[2];  // d_y
+      p[5] = -p[5];  // Q_xy
+      p[8] = -p[8];  // Q_yz
+    }
+  }
+  // rpole layout: q, d (3), Qxx, Qxy, Qxz, Qyy, Qyz, Qzz
+  const double q2[3][3] = {{p[4], p[5], p[6]}, {p[5], p[7], p[8]}, {p[6], p[8], p[9]}};
+  double out[10];
+  out[0] = p[0];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) out[1 + r] = (p[1] * a[r][0] + p[2] * a[r][1]) + p[3] * a[r][2];
+  double g[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = r; c < 3; ++c) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) s += a[r][k] * a[c][m] * q2[k][m];
+      g[r][c] = s;
+    }
+  out[4] = g[0][0], out[5] = g[0][1], out[6] = g[0][2], out[7] = g[1][1], out[8] = g[1][2], out[9] = g[2][2];
+  double* RP = const_cast<double*>(sv.rpole);
+#pragma unroll
+  for (int c = 0; c < 10; ++c) RP[(size_t)c * np + i] = out[c];
+}
+
 // ===================================================== truncated CG (polalg 3)
 // epolar3tcg1/epolar3tcg2 (epolar3tcg.f:66-311, 320-621) on one system.  The
 // polarization matrix A = alpha^-1 - T is applied by the matvec with its

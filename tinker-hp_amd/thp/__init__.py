@@ -33,6 +33,8 @@ THP_PREC_FP64 = 0
 THP_PREC_MIXED = 1
 # induced-dipole solvers (polalg)
 THP_POLALG_PCG, THP_POLALG_JACOBI_DIIS, THP_POLALG_TCG = 1, 2, 3
+# local-frame axis types (rotpole.f)
+THP_AXIS_NONE, THP_AXIS_Z_ONLY, THP_AXIS_Z_THEN_X, THP_AXIS_BISECTOR, THP_AXIS_Z_BISECT, THP_AXIS_3_FOLD = range(6)
 # induced-dipole predictors (polpred)
 THP_PRED_NONE, THP_PRED_GEAR, THP_PRED_ASPC, THP_PRED_LSQR = 0, 1, 2, 3
 ERRORS = {
@@ -139,6 +141,7 @@ def load_lib(path: str | None = None) -> C.CDLL:
         "thp_set_solver": (C.c_int, [vp, C.c_int]),
         "thp_set_tcg": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]),
         "thp_set_shortreal": (C.c_int, [vp, C.c_double]),
+        "thp_system_set_frames": (C.c_int, [vp, _ip, _ip, _ip, _ip, _dp]),
         "thp_tmatxb_dev": (C.c_int, [vp, vp, vp, C.c_int]),
         "thp_induce_dev": (C.c_int, [vp, C.POINTER(SolveOpts), vp, _dp, C.POINTER(SolveStats)]),
         "thp_serve_start": (C.c_int, [vp, C.POINTER(ServeOpts)]),
@@ -192,6 +195,9 @@ class System:
     order: int = 5
     name: str = ""
     extra: dict = field(default_factory=dict)
+    # local-frame multipoles: dict(axistype, zaxis, xaxis, yaxis, pole), each
+    # solver rotates them on the device (rpole then serves only as storage)
+    frames: object = None
 
     def with_coords(self, x, y, z):
         s = System(**{k: getattr(self, k) for k in self.__dataclass_fields__})
@@ -329,6 +335,17 @@ class Solver:
         d = system.desc(precision)
         ctx.check(self.lib.thp_system_create(ctx.h, C.byref(d), C.byref(self.h)), "thp_system_create")
         ctx._solvers.add(self)
+        if system.frames is not None:
+            self.set_frames(**system.frames)
+
+    def set_frames(self, axistype, zaxis, xaxis, yaxis, pole):
+        """Local-frame multipoles (rotpole + chkpole on the device each step):
+        axis types THP_AXIS_*, frame atoms (0-based, -1 for none), pole (n, 13)
+        in the local frame."""
+        n = self.sys.n
+        arrs = [np.ascontiguousarray(v, dtype=np.int32).reshape(n) for v in (axistype, zaxis, xaxis, yaxis)]
+        pl = np.ascontiguousarray(pole, dtype=np.float64).reshape(n, 13)
+        self._c(self.lib.thp_system_set_frames(self.h, *(_i(v) for v in arrs), _d(pl)), "set_frames")
 
     def close(self):
         # a finalized context has already released its systems
