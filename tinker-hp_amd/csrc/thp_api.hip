@@ -297,6 +297,13 @@ template <typename real>
 void launch_convolve(thp_ctx* ctx, const Batch& b, int gate) {
   const dim3 g = g1(b.n_fft), t(256);
   hipStream_t st = ctx->stream;
+  if (b.fft_plane && ctx->plane_fft && sizeof(real) == 4) {
+    // x and y fused per plane, then the z pass: three passes over the grid
+    hipLaunchKernelGGL((k_fft_plane<real, 0>), g, t, b.fft_plane_lds, st, b.d_views, b.d_fft, gate);
+    hipLaunchKernelGGL((k_fft_axis<real, 2, 2>), g, t, 0, st, b.d_views, b.d_fft, gate);
+    hipLaunchKernelGGL((k_fft_plane<real, 1>), g, t, b.fft_plane_lds, st, b.d_views, b.d_fft, gate);
+    return;
+  }
   hipLaunchKernelGGL((k_fft_axis<real, 0, 0>), g, t, 0, st, b.d_views, b.d_fft, gate);
   hipLaunchKernelGGL((k_fft_axis<real, 1, 0>), g, t, 0, st, b.d_views, b.d_fft, gate);
   hipLaunchKernelGGL((k_fft_axis<real, 2, 2>), g, t, 0, st, b.d_views, b.d_fft, gate);
@@ -765,7 +772,9 @@ void append_work(const thp_sys* s, int index, std::vector<Work>& blk, std::vecto
   for (int q = 0; q < s->nbricks; ++q) brick.push_back({index, q});
   const int nconv = (int)((grid_points(s) + kConvChunk - 1) / kConvChunk);
   for (int q = 0; q < nconv; ++q) conv.push_back({index, q});
-  const int ng = std::max(fft_groups(s->nfft, 0), std::max(fft_groups(s->nfft, 1), fft_groups(s->nfft, 2)));
+  // line groups of the three axes, and at least one entry per z plane
+  const int ng = std::max(std::max(fft_groups(s->nfft, 0), s->nfft[2]),
+                          std::max(fft_groups(s->nfft, 1), fft_groups(s->nfft, 2)));
   for (int q = 0; q < ng; ++q) fft.push_back({index, q});
 }
 
@@ -791,6 +800,14 @@ thp_status thp_init(thp_ctx** out, int device) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
+  // the fused plane FFT needs more than the default 64 KB of dynamic LDS;
+  // THP_FFT_PLANE=0 keeps the five one-axis passes
+  ctx->plane_fft = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_plane<float, 0>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, kPlaneLds) == hipSuccess &&
+                   hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_plane<float, 1>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, kPlaneLds) == hipSuccess;
+  if (const char* e = getenv("THP_FFT_PLANE"))
+    if (atoi(e) == 0) ctx->plane_fft = false;
   // THP_SEGS forces the pair kernels' wavefronts per 64-atom block
   if (const char* e = getenv("THP_SEGS")) {
     const int v = atoi(e);
@@ -1148,7 +1165,8 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
     ALLOC(s->d_convwork, nconv);
     H2D(s->d_convwork, cw.data(), sizeof(Work) * nconv);
     s->nconv = nconv;
-    const int nfftw = std::max(fft_groups(s->nfft, 0), std::max(fft_groups(s->nfft, 1), fft_groups(s->nfft, 2)));
+    const int nfftw = std::max(std::max(fft_groups(s->nfft, 0), s->nfft[2]),
+                               std::max(fft_groups(s->nfft, 1), fft_groups(s->nfft, 2)));
     std::vector<Work> fw(nfftw);
     for (int b = 0; b < nfftw; ++b) fw[b] = {0, b};
     ALLOC(s->d_fftwork, nfftw);
@@ -1183,6 +1201,8 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
     b.d_ids = s->d_sysid;
     b.d_fft = s->d_fftwork;
     b.n_fft = s->nfftw;
+    const int* nf = s->nfft;
+    plane_fft_setup(b, &nf, 1, s->prec);
   }
   for (int e = 0; e < 4; ++e)
     if (hipEventCreate(&s->ev[e]) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "event"));
@@ -1763,6 +1783,9 @@ thp_status thp_time_kernel_batch(thp_sys* const* systems, int nsys, int kernel, 
     put(fft, &b.d_fft, &b.n_fft);
     b.d_views = d_views;
     b.d_ids = d_ids;
+    std::vector<const int*> nfs(nsys);
+    for (int q = 0; q < nsys; ++q) nfs[q] = systems[q]->nfft;
+    plane_fft_setup(b, nfs.data(), nsys, s0->prec);
     std::vector<int> ids(nsys);
     for (int q = 0; q < nsys; ++q) ids[q] = q;
     thp_status rc = THP_OK;

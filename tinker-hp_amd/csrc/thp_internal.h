@@ -193,7 +193,28 @@ struct Batch {
   int n_fft = 0;
   int* d_ids = nullptr;         // [nsys] = 0..nsys-1
   int real_only = 0;            // short-range solves: no reciprocal terms
+  int fft_plane = 0;            // x and y FFT passes fused per z plane (plane_fft_setup)
+  int fft_plane_lds = 0;        // its dynamic LDS bytes
 };
+
+// Whether a batch of these grids takes the fused x-y plane FFT kernel: FP32
+// grids whose two padded planes and twiddles fit kPlaneLds bytes of LDS.
+constexpr int kPlaneLds = 80 * 1024;
+inline int plane_fft_bytes(const int* nfft, int prec) {
+  return prec == THP_PREC_FP64 ? 0 : (2 * nfft[1] * (nfft[0] + 1) + nfft[0] + nfft[1]) * 2 * (int)sizeof(float);
+}
+inline void plane_fft_setup(Batch& b, const int* const* nffts, int m, int prec) {
+  int need = 0;
+  bool ok = prec != THP_PREC_FP64 && m > 0;
+  for (int q = 0; ok && q < m; ++q) {
+    const int bytes = plane_fft_bytes(nffts[q], prec);
+    // at most 256 lines per pass: one thread or more per line
+    ok = bytes > 0 && bytes <= kPlaneLds && nffts[q][0] <= 256 && nffts[q][1] <= 256;
+    need = bytes > need ? bytes : need;
+  }
+  b.fft_plane = ok ? 1 : 0;
+  b.fft_plane_lds = ok ? need : 0;
+}
 
 }  // namespace thp
 
@@ -205,6 +226,7 @@ struct thp_ctx {
   int n_cu = 0;
   int pair_segs = 0;           // 0: default; else forced waves per block (1..16)
   int pair_nb = 0;             // 0: default; else forced matvec entries per batch (8, 16)
+  bool plane_fft = false;       // the fused x-y plane FFT kernel may take its LDS
   std::set<thp_sys*> systems;  // live systems; thp_finalize releases what is left
 };
 

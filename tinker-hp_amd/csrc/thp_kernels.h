@@ -1643,12 +1643,13 @@ __device__ __forceinline__ void dft8(Coef2<real>* x, real s) {
   }
 }
 
-// one Stockham stage of radix R over the group's lines (line stride kp in
-// LDS): butterfly i of a line reads src[i + q K/R], applies the twiddles
-// w^(q k K/(pR)) with k = i mod p, and writes dst[(i - k) R + k + q p]
+// one Stockham stage of radix R over the group's lines (line a starts at
+// a * kp in LDS, its points ps apart): butterfly i of a line reads
+// src[i + q K/R], applies the twiddles w^(q k K/(pR)) with k = i mod p, and
+// writes dst[(i - k) R + k + q p]
 template <typename real, int R>
 __device__ __forceinline__ void fft_stage(const Coef2<real>* src, Coef2<real>* dst, const Coef2<real>* tw, int K,
-                                          int kp, int p, int nl, int tid, int tpl_shift, real s) {
+                                          int kp, int ps, int p, int nl, int tid, int tpl_shift, real s) {
   const int T = K / R;
   const int step = K / (p * R);
   const int a = tid >> tpl_shift;
@@ -1658,36 +1659,36 @@ __device__ __forceinline__ void fft_stage(const Coef2<real>* src, Coef2<real>* d
   for (int i = tid & ((1 << tpl_shift) - 1); i < T; i += 1 << tpl_shift) {
     const int k = i % p;
     Coef2<real> x[R];
-    x[0] = in[i];
+    x[0] = in[i * ps];
 #pragma unroll
     for (int q = 1; q < R; ++q) {
       Coef2<real> w = tw[q * k * step];
       w.y *= -s;  // tw holds exp(-2 pi i m / K): conjugated for the backward sign
-      x[q] = cmul<real>(in[i + q * T], coef2<real>(w.x, w.y));
+      x[q] = cmul<real>(in[(i + q * T) * ps], coef2<real>(w.x, w.y));
     }
     if constexpr (R == 8) dft8<real>(x, s);
     else dft_small<real, R>(x, s);
     const int j = (i - k) * R + k;
 #pragma unroll
-    for (int q = 0; q < R; ++q) out[j + q * p] = x[q];
+    for (int q = 0; q < R; ++q) out[(j + q * p) * ps] = x[q];
   }
 }
 
 // whole transform of the group's lines; returns the buffer holding it
 template <typename real>
 __device__ int fft_lines_lds(Coef2<real>* b0, Coef2<real>* b1, int cur, const Coef2<real>* tw, int K, int kp, int nl,
-                             int tid, int tpl_shift, real s) {
+                             int tid, int tpl_shift, real s, int ps = 1) {
   int rem = K, p = 1;
   while (rem > 1) {
     const int R = rem % 8 == 0 ? 8 : rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2 : rem % 3 == 0 ? 3 : 5;
     const Coef2<real>* src = cur ? b1 : b0;
     Coef2<real>* dst = cur ? b0 : b1;
     switch (R) {
-      case 8: fft_stage<real, 8>(src, dst, tw, K, kp, p, nl, tid, tpl_shift, s); break;
-      case 4: fft_stage<real, 4>(src, dst, tw, K, kp, p, nl, tid, tpl_shift, s); break;
-      case 2: fft_stage<real, 2>(src, dst, tw, K, kp, p, nl, tid, tpl_shift, s); break;
-      case 3: fft_stage<real, 3>(src, dst, tw, K, kp, p, nl, tid, tpl_shift, s); break;
-      default: fft_stage<real, 5>(src, dst, tw, K, kp, p, nl, tid, tpl_shift, s); break;
+      case 8: fft_stage<real, 8>(src, dst, tw, K, kp, ps, p, nl, tid, tpl_shift, s); break;
+      case 4: fft_stage<real, 4>(src, dst, tw, K, kp, ps, p, nl, tid, tpl_shift, s); break;
+      case 2: fft_stage<real, 2>(src, dst, tw, K, kp, ps, p, nl, tid, tpl_shift, s); break;
+      case 3: fft_stage<real, 3>(src, dst, tw, K, kp, ps, p, nl, tid, tpl_shift, s); break;
+      default: fft_stage<real, 5>(src, dst, tw, K, kp, ps, p, nl, tid, tpl_shift, s); break;
     }
     __syncthreads();
     cur ^= 1;
@@ -1805,6 +1806,73 @@ __global__ void __launch_bounds__(256) k_fft_axis(const SysView* views, const Wo
     int slot;
     size_t g;
     if (elem(u, slot, g)) G[g] = s_buf[cur][slot];
+  }
+}
+
+// The x and y passes of one z plane of one grid in LDS, for batches whose
+// planes fit (FP32 grids up to about 75 x 75 points per plane): MODE 0 the
+// forward x then y transforms, MODE 1 the backward y then x transforms.
+// With the fused z pass a convolution is then three passes over the grid.
+// The plane is row-major in LDS with rows padded to K0 + 1 values; the x
+// lines are its rows, the y lines its columns (points K0 + 1 apart).
+template <typename real, int MODE>
+__global__ void __launch_bounds__(256) k_fft_plane(const SysView* views, const Work* work, int gate) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char dsm[];
+  const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
+  const SysView& sv = views[w.sys];
+  if (gated_out(sv.st, gate)) return;
+  const int K0 = sv.nfft[0], K1 = sv.nfft[1], K2 = sv.nfft[2];
+  if (w.blk >= K2) return;
+  const int kp = K0 + 1, cap = K1 * kp;
+  Coef2<real>* b0 = reinterpret_cast<Coef2<real>*>(dsm);
+  Coef2<real>* b1 = b0 + cap;
+  Coef2<real>* tw = b1 + cap;  // twiddles of the x then the y axis
+  GLOBAL Coef2<real>* G = gw(static_cast<Coef2<real>*>(sv.grid)) + (size_t)w.blk * K0 * K1;
+  const GLOBAL Coef2<real>* TW = gp(static_cast<const Coef2<real>*>(sv.twid));
+  const int tid = threadIdx.x;
+  const int npts = K0 * K1;
+  // loads: a batch of unconditional loads in flight per thread (indices past
+  // the plane re-read its first value), then the LDS writes
+  constexpr int PB = 8;
+  for (int f0 = tid; f0 < npts; f0 += 256 * PB) {
+    Coef2<real> v[PB];
+#pragma unroll
+    for (int u = 0; u < PB; ++u) {
+      const int f = f0 + 256 * u;
+      v[u] = G[f < npts ? f : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < PB; ++u) {
+      const int f = f0 + 256 * u;
+      if (f < npts) {
+        const int y = f / K0;
+        b0[y * kp + (f - y * K0)] = v[u];
+      }
+    }
+  }
+  for (int m = tid; m < K0 + K1; m += 256) tw[m] = TW[m];
+  __syncthreads();
+  auto lines = [&](int cur, int axis, real s) {
+    const int K = axis == 0 ? K0 : K1, nl = axis == 0 ? K1 : K0;
+    int fl = 1;
+    while (fl < nl) fl <<= 1;
+    const int tpl_shift = 8 - (31 - __clz(fl));
+    return axis == 0 ? fft_lines_lds<real>(b0, b1, cur, tw, K, kp, nl, tid, tpl_shift, s, 1)
+                     : fft_lines_lds<real>(b0, b1, cur, tw + K0, K, 1, nl, tid, tpl_shift, s, kp);
+  };
+  int cur = 0;
+  if (MODE == 0) {
+    cur = lines(cur, 0, real(-1));
+    cur = lines(cur, 1, real(-1));
+  } else {
+    cur = lines(cur, 1, real(1));
+    cur = lines(cur, 0, real(1));
+  }
+  const Coef2<real>* res = cur ? b1 : b0;
+  for (int f = tid; f < npts; f += 256) {
+    const int y = f / K0;
+    G[f] = res[y * kp + (f - y * K0)];
   }
 }
 

@@ -335,6 +335,9 @@ thp_status build_batch(Serve* sv, BatchBuf& bb, const std::vector<thp_sys*>& mem
   put(fft, &b.d_fft, &b.n_fft);
   b.d_views = bb.views;
   b.d_ids = bb.ids;
+  std::vector<const int*> nfs(m);
+  for (int q = 0; q < m; ++q) nfs[q] = members[q]->nfft;
+  plane_fft_setup(b, nfs.data(), m, prec);
   hipStream_t st = ctx->stream;
   SHIP(ctx, hipMemcpyAsync(bb.views, bb.h_views, sizeof(SysView) * m, hipMemcpyHostToDevice, st));
   SHIP(ctx, hipMemcpyAsync(bb.tables, bb.h_tables, sizeof(Work) * nt, hipMemcpyHostToDevice, st));
