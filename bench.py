@@ -6,14 +6,17 @@ resident system.  Legs:
   plain  - FP64 pair/grid/FFT, one system per launch, a host read of the
            convergence flag after every iteration (the reference's pattern);
   mixed  - FP32 pair/grid/FFT with FP64 accumulation and CG scalars;
-  serve  - the batched serving loop (mixed), when libthp provides it.
+  serve  - the batched serving loop (mixed), when libthp provides it;
+  dhfr2  - the reference's dhfr2 example (23,558 atoms, amoebabio09, local
+           frames rotated on the device), one system, mixed; reported in
+           config, not a candidate for `value` (a different workload).
 `value` is the fastest leg's solves/s.  The roofline object is measured live
 on the dominant kernel of that leg: the real-space matvec, as one launch over
 the serving batch (the resident copies) when the serving leg is the fastest,
 timed with HIP events on the library's stream; cpu_baseline times the FP64
 CPU oracle on the host cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serve] [--no-stream]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serve] [--no-stream] [--no-dhfr2]
        python bench.py --serve-only | --kernel-probe    (profiling aids, no JSON line)
 """
 import argparse
@@ -120,6 +123,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
+    ap.add_argument("--no-dhfr2", action="store_true")
     ap.add_argument("--stream-requests", type=int, default=2000)
     ap.add_argument("--copies", type=int, default=16, help="resident 24k systems in the serving leg")
     ap.add_argument("--kernel-probe", action="store_true",
@@ -183,6 +187,18 @@ def main():
         except (ImportError, AttributeError, thp.ThpError) as e:
             log(f"bench: stream leg unavailable: {e}")
 
+    # the dhfr2 example through the Tinker reader (one system, mixed)
+    dhfr2 = None
+    gold = os.path.join(ROOT, "tests", "golden")
+    if not args.no_dhfr2 and os.path.exists(os.path.join(gold, "dhfr2.xyz.gz")):
+        import thp.tinker as tinker
+        ds = tinker.load(os.path.join(gold, "dhfr2.xyz.gz"), os.path.join(gold, "dhfr2.key"))
+        dsolver, t_d, r_d = time_leg(ctx, thp, ds, thp.THP_PREC_MIXED, args.steps, args.warmup, args.poleps)
+        dhfr2 = dict(solves_per_s=round(1.0 / t_d, 3), ms_per_solve=round(1e3 * t_d, 4), niter=r_d[2].niter,
+                     ep=r_d[1], atoms=ds.n, frames="local (rotated on the device)", aewald=round(ds.aewald, 6))
+        dsolver.close()
+        log(f"bench: dhfr2 {dhfr2}")
+
     # live roofline of the dominant kernel (mixed real-space matvec): one
     # launch over the serving batch when that leg is the fastest
     best = max(legs, key=lambda k: legs[k]["solves_per_s"])
@@ -239,6 +255,7 @@ def main():
             "p99_ms": serve["p99_ms"] if serve else None,
             "serve": serve,
             "stream": stream,
+            "dhfr2_mixed_one_system": dhfr2,
             "niter": legs[best]["niter"],
             "errors_vs_fp64_oracle": err or None,
         },
