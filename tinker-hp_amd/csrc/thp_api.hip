@@ -299,9 +299,9 @@ void launch_convolve(thp_ctx* ctx, const Batch& b, int gate) {
   hipStream_t st = ctx->stream;
   if (b.fft_plane && ctx->plane_fft && sizeof(real) == 4) {
     // x and y fused per plane, then the z pass: three passes over the grid
-    hipLaunchKernelGGL((k_fft_plane<real, 0>), g, t, b.fft_plane_lds, st, b.d_views, b.d_fft, gate);
+    hipLaunchKernelGGL((k_fft_plane<real, 0, kPlaneThreads>), g, dim3(kPlaneThreads), b.fft_plane_lds, st, b.d_views, b.d_fft, gate);
     hipLaunchKernelGGL((k_fft_axis<real, 2, 2>), g, t, 0, st, b.d_views, b.d_fft, gate);
-    hipLaunchKernelGGL((k_fft_plane<real, 1>), g, t, b.fft_plane_lds, st, b.d_views, b.d_fft, gate);
+    hipLaunchKernelGGL((k_fft_plane<real, 1, kPlaneThreads>), g, dim3(kPlaneThreads), b.fft_plane_lds, st, b.d_views, b.d_fft, gate);
     return;
   }
   hipLaunchKernelGGL((k_fft_axis<real, 0, 0>), g, t, 0, st, b.d_views, b.d_fft, gate);
@@ -802,9 +802,9 @@ thp_status thp_init(thp_ctx** out, int device) {
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
   // the fused plane FFT needs more than the default 64 KB of dynamic LDS;
   // THP_FFT_PLANE=0 keeps the five one-axis passes
-  ctx->plane_fft = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_plane<float, 0>),
+  ctx->plane_fft = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_plane<float, 0, kPlaneThreads>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, kPlaneLds) == hipSuccess &&
-                   hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_plane<float, 1>),
+                   hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_plane<float, 1, kPlaneThreads>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, kPlaneLds) == hipSuccess;
   if (const char* e = getenv("THP_FFT_PLANE"))
     if (atoi(e) == 0) ctx->plane_fft = false;

@@ -200,6 +200,7 @@ struct Batch {
 // Whether a batch of these grids takes the fused x-y plane FFT kernel: FP32
 // grids whose two padded planes and twiddles fit kPlaneLds bytes of LDS.
 constexpr int kPlaneLds = 80 * 1024;
+constexpr int kPlaneThreads = 256;  // threads per plane workgroup (at least as many as the lines)
 inline int plane_fft_bytes(const int* nfft, int prec) {
   return prec == THP_PREC_FP64 ? 0 : (2 * nfft[1] * (nfft[0] + 1) + nfft[0] + nfft[1]) * 2 * (int)sizeof(float);
 }
@@ -208,8 +209,8 @@ inline void plane_fft_setup(Batch& b, const int* const* nffts, int m, int prec) 
   bool ok = prec != THP_PREC_FP64 && m > 0;
   for (int q = 0; ok && q < m; ++q) {
     const int bytes = plane_fft_bytes(nffts[q], prec);
-    // at most 256 lines per pass: one thread or more per line
-    ok = bytes > 0 && bytes <= kPlaneLds && nffts[q][0] <= 256 && nffts[q][1] <= 256;
+    // one thread or more per line of either pass
+    ok = bytes > 0 && bytes <= kPlaneLds && nffts[q][0] <= kPlaneThreads && nffts[q][1] <= kPlaneThreads;
     need = bytes > need ? bytes : need;
   }
   b.fft_plane = ok ? 1 : 0;

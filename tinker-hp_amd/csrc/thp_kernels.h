@@ -1815,8 +1815,8 @@ __global__ void __launch_bounds__(256) k_fft_axis(const SysView* views, const Wo
 // With the fused z pass a convolution is then three passes over the grid.
 // The plane is row-major in LDS with rows padded to K0 + 1 values; the x
 // lines are its rows, the y lines its columns (points K0 + 1 apart).
-template <typename real, int MODE>
-__global__ void __launch_bounds__(256) k_fft_plane(const SysView* views, const Work* work, int gate) {
+template <typename real, int MODE, int NT>
+__global__ void __launch_bounds__(NT) k_fft_plane(const SysView* views, const Work* work, int gate) {
   extern __shared__ __attribute__((aligned(16))) unsigned char dsm[];
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
@@ -1835,29 +1835,29 @@ __global__ void __launch_bounds__(256) k_fft_plane(const SysView* views, const W
   // loads: a batch of unconditional loads in flight per thread (indices past
   // the plane re-read its first value), then the LDS writes
   constexpr int PB = 8;
-  for (int f0 = tid; f0 < npts; f0 += 256 * PB) {
+  for (int f0 = tid; f0 < npts; f0 += NT * PB) {
     Coef2<real> v[PB];
 #pragma unroll
     for (int u = 0; u < PB; ++u) {
-      const int f = f0 + 256 * u;
+      const int f = f0 + NT * u;
       v[u] = G[f < npts ? f : 0];
     }
 #pragma unroll
     for (int u = 0; u < PB; ++u) {
-      const int f = f0 + 256 * u;
+      const int f = f0 + NT * u;
       if (f < npts) {
         const int y = f / K0;
         b0[y * kp + (f - y * K0)] = v[u];
       }
     }
   }
-  for (int m = tid; m < K0 + K1; m += 256) tw[m] = TW[m];
+  for (int m = tid; m < K0 + K1; m += NT) tw[m] = TW[m];
   __syncthreads();
   auto lines = [&](int cur, int axis, real s) {
     const int K = axis == 0 ? K0 : K1, nl = axis == 0 ? K1 : K0;
     int fl = 1;
     while (fl < nl) fl <<= 1;
-    const int tpl_shift = 8 - (31 - __clz(fl));
+    const int tpl_shift = (31 - __clz(NT)) - (31 - __clz(fl));  // threads per line = NT / fl
     return axis == 0 ? fft_lines_lds<real>(b0, b1, cur, tw, K, kp, nl, tid, tpl_shift, s, 1)
                      : fft_lines_lds<real>(b0, b1, cur, tw + K0, K, 1, nl, tid, tpl_shift, s, kp);
   };
@@ -1870,7 +1870,7 @@ __global__ void __launch_bounds__(256) k_fft_plane(const SysView* views, const W
     cur = lines(cur, 0, real(1));
   }
   const Coef2<real>* res = cur ? b1 : b0;
-  for (int f = tid; f < npts; f += 256) {
+  for (int f = tid; f < npts; f += NT) {
     const int y = f / K0;
     G[f] = res[y * kp + (f - y * K0)];
   }
