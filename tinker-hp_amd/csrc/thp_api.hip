@@ -612,8 +612,7 @@ thp_status b_init_state(thp_ctx* ctx, const Batch& b, double poleps, int politer
 }
 
 template <typename real, int NB>
-void launch_tmatxb_nb(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int gate) {
-  hipStream_t st = ctx->stream;
+void launch_tmatxb_nb(const Batch& b, int vin, int vout, int dodiag, int gate, hipStream_t st) {
   const dim3 g = g1(b.n_blk);
   switch (b.segs) {
     case 1: hipLaunchKernelGGL((k_tmatxb_cl<real, 1, NB>), g, dim3(64), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, gate); break;
@@ -625,17 +624,41 @@ void launch_tmatxb_nb(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodia
 }
 
 template <typename real>
-void launch_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int gate) {
-  if (b.nb == 16) launch_tmatxb_nb<real, 16>(ctx, b, vin, vout, dodiag, gate);
-  else launch_tmatxb_nb<real, 8>(ctx, b, vin, vout, dodiag, gate);
+void launch_tmatxb(const Batch& b, int vin, int vout, int dodiag, int gate, hipStream_t st) {
+  if (b.nb == 16) launch_tmatxb_nb<real, 16>(b, vin, vout, dodiag, gate, st);
+  else launch_tmatxb_nb<real, 8>(b, vin, vout, dodiag, gate, st);
 }
 
-// real-space matvec; needs the pair coefficients of the current
-// coordinates (written by the permanent-field pass)
-thp_status b_pair_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int check_conv) {
-  if (b.prec == THP_PREC_FP64) launch_tmatxb<double>(ctx, b, vin, vout, dodiag, check_conv);
-  else launch_tmatxb<float>(ctx, b, vin, vout, dodiag, check_conv);
+// real-space matvec on stream st; needs the pair coefficients of the
+// current coordinates (written by the permanent-field pass)
+thp_status b_pair_tmatxb_on(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int gate, hipStream_t st) {
+  if (b.prec == THP_PREC_FP64) launch_tmatxb<double>(b, vin, vout, dodiag, gate, st);
+  else launch_tmatxb<float>(b, vin, vout, dodiag, gate, st);
   return launch_check(ctx, "k_tmatxb_cl");
+}
+
+thp_status b_pair_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int gate) {
+  return b_pair_tmatxb_on(ctx, b, vin, vout, dodiag, gate, ctx->stream);
+}
+
+// The real-space matvec of a vector runs on the side stream while the main
+// stream spreads and transforms the same vector (the two read it and write
+// different arrays); join_matvec makes the main stream wait for it before
+// the gather that adds the two parts.  Without a side stream the matvec
+// runs in line.
+thp_status fork_matvec(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int gate) {
+  if (!ctx->side || b.real_only) return b_pair_tmatxb(ctx, b, vin, vout, dodiag, gate);
+  HIPCHK(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
+  HIPCHK(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+  RET(b_pair_tmatxb_on(ctx, b, vin, vout, dodiag, gate, ctx->side));
+  HIPCHK(ctx, hipEventRecord(ctx->ev_join, ctx->side));
+  return THP_OK;
+}
+
+thp_status join_matvec(thp_ctx* ctx, const Batch& b) {
+  if (!ctx->side || b.real_only) return THP_OK;
+  HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+  return THP_OK;
 }
 
 // Field, guess and first residual (newinduce_pme2.f:111-196, 406-440);
@@ -649,22 +672,24 @@ thp_status b_setup_field(thp_ctx* ctx, const Batch& b, int gate) {
 
 thp_status b_setup(thp_ctx* ctx, const Batch& b, int gate) {
   RET(b_setup_field(ctx, b, gate));
-  RET(b_pair_tmatxb(ctx, b, 0 /*mu*/, 3 /*hreal*/, 1, gate));
+  RET(fork_matvec(ctx, b, 0 /*mu*/, 3 /*hreal*/, 1, gate));
   if (!b.real_only) {
     RET(spread_uind(ctx, b, 0, 0, gate));
     RET(convolve(ctx, b, gate));
   }
+  RET(join_matvec(ctx, b));
   return launch_gather_uind<1>(ctx, b, 0, 3, 0, gate);
 }
 
 // One PCG iteration of every running system (newinduce_pme2.f:442-569);
 // each system's iteration number lives in its SolveState.
 thp_status b_iteration(thp_ctx* ctx, const Batch& b) {
-  RET(b_pair_tmatxb(ctx, b, 1 /*pp*/, 3 /*hreal*/, 1, kGateIter));
+  RET(fork_matvec(ctx, b, 1 /*pp*/, 3 /*hreal*/, 1, kGateIter));
   if (!b.real_only) {
     RET(spread_uind(ctx, b, 1, 0, kGateIter));
     RET(convolve(ctx, b, kGateIter));
   }
+  RET(join_matvec(ctx, b));
   RET(launch_gather_uind<2>(ctx, b, 1, 3, 0, kGateIter));
   hipStream_t st = ctx->stream;
   if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_cg_update1<double>), g1(b.n_red), dim3(256), 0, st, b.d_views, b.d_red);
@@ -808,6 +833,17 @@ thp_status thp_init(thp_ctx** out, int device) {
                                        hipFuncAttributeMaxDynamicSharedMemorySize, kPlaneLds) == hipSuccess;
   if (const char* e = getenv("THP_FFT_PLANE"))
     if (atoi(e) == 0) ctx->plane_fft = false;
+  // a second stream (default priority) for the real-space matvec beside the
+  // reciprocal chain; THP_FORK=0 keeps every launch on one stream
+  const char* fe = getenv("THP_FORK");
+  if (!(fe && atoi(fe) == 0)) {
+    if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
+      *out = ctx;
+      return fail(ctx, THP_ERR_HIP, "cannot create the side stream of device %d", device);
+    }
+  }
   // THP_SEGS forces the pair kernels' wavefronts per 64-atom block
   if (const char* e = getenv("THP_SEGS")) {
     const int v = atoi(e);
@@ -828,6 +864,9 @@ thp_status thp_finalize(thp_ctx* ctx) {
   // systems the caller did not destroy are released here; their handles die
   // with the context
   while (!ctx->systems.empty()) thp_system_destroy(*ctx->systems.begin());
+  if (ctx->side) (void)hipStreamDestroy(ctx->side);
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return THP_OK;

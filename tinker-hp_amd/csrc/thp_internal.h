@@ -228,6 +228,8 @@ struct thp_ctx {
   int pair_segs = 0;           // 0: default; else forced waves per block (1..16)
   int pair_nb = 0;             // 0: default; else forced matvec entries per batch (8, 16)
   bool plane_fft = false;       // the fused x-y plane FFT kernel may take its LDS
+  hipStream_t side = nullptr;   // the real-space matvec beside the reciprocal chain
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::set<thp_sys*> systems;  // live systems; thp_finalize releases what is left
 };
 
