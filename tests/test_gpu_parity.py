@@ -192,6 +192,42 @@ def test_induce_water23559_full_size(ctx, prec):
     check_solve(ctx, thp.water_23559(), prec)
 
 
+@pytest.fixture(scope="module")
+def solute96000_ref():
+    s = thp.solute_96000()
+    return s, O.Oracle(s).induce(poleps=1e-8)
+
+
+# The uneven-density configuration at its full size: 96,000 atoms in a 100 A
+# box with a 20,000-atom solute blob, 108^3 grid
+@pytest.mark.parametrize("prec", [FP64, MIXED])
+def test_induce_solute96000_full_size(ctx, solute96000_ref, prec):
+    s, ref = solute96000_ref
+    check_solve(ctx, s, prec, ref)
+
+
+# The large configuration at its full size: 1,068,000 atoms, 220 A box,
+# 240^3 grid (five one-axis FFT passes: the plane kernel's LDS does not hold
+# a 240^2 plane).  An oracle solve takes minutes here, so the mixed path is
+# held to the FP64 path, which the oracle pins at every other size, with
+# the same bars as against the oracle.
+def test_water1068000_mixed_matches_fp64_full_size(ctx):
+    s = thp.water_1068000()
+    out = {}
+    for prec in (FP64, MIXED):
+        solver = thp.Solver(ctx, s, prec)
+        mu, ep, st = solver.newinduce_pme2(poleps=1e-8)
+        assert st.status == 0
+        out[prec] = (mu, ep, st.niter)
+        solver.close()
+    (mu64, ep64, it64), (mu32, ep32, it32) = out[FP64], out[MIXED]
+    erel = abs(ep32 - ep64) / abs(ep64)
+    rms = float(np.sqrt(np.mean((mu32 - mu64) ** 2)) * thp.DEBYE)
+    print(f"water-1068000 ep64={ep64:.8f} ep32={ep32:.8f} rel={erel:.2e} muRMS_D={rms:.2e} niter={it64}/{it32}",
+          flush=True)
+    assert erel < 1e-6 and rms < 1e-5 and abs(it32 - it64) <= 1
+
+
 @pytest.mark.parametrize("prec", [FP64, MIXED])
 def test_induce_solute_blob(ctx, prec):
     s = thp.make_system(6000, 40.0, n_solute=2000, blob_radius=14.0, seed=21, nfft=(40,) * 3, name="solute-6000")
