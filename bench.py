@@ -9,14 +9,16 @@ resident system.  Legs:
   serve  - the batched serving loop (mixed), when libthp provides it;
   dhfr2  - the reference's dhfr2 example (23,558 atoms, amoebabio09, local
            frames rotated on the device), one system, mixed; reported in
-           config, not a candidate for `value` (a different workload).
+           config, not a candidate for `value` (a different workload);
+  large  - the 96,000-atom uneven solute box (108^3) and the 1,068,000-atom
+           water box (240^3), one system each, mixed; reported in config.
 `value` is the fastest leg's solves/s.  The roofline object is measured live
 on the dominant kernel of that leg: the real-space matvec, as one launch over
 the serving batch (the resident copies) when the serving leg is the fastest,
 timed with HIP events on the library's stream; cpu_baseline times the FP64
 CPU oracle on the host cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serve] [--no-stream] [--no-dhfr2]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serve] [--no-stream] [--no-dhfr2] [--no-large]
        python bench.py --serve-only | --kernel-probe    (profiling aids, no JSON line)
 """
 import argparse
@@ -124,6 +126,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-dhfr2", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--stream-requests", type=int, default=2000)
     ap.add_argument("--copies", type=int, default=16, help="resident 24k systems in the serving leg")
     ap.add_argument("--kernel-probe", action="store_true",
@@ -199,6 +202,18 @@ def main():
         dsolver.close()
         log(f"bench: dhfr2 {dhfr2}")
 
+    # the uneven and the large configurations at full size (one system, mixed)
+    large = None
+    if not args.no_large:
+        large = {}
+        for name, build in (("solute-96000", thp.solute_96000), ("water-1068000", thp.water_1068000)):
+            ls = build()
+            lsolver, t_l, r_l = time_leg(ctx, thp, ls, thp.THP_PREC_MIXED, 3, 1, args.poleps)
+            large[name] = dict(solves_per_s=round(1.0 / t_l, 3), ms_per_solve=round(1e3 * t_l, 3),
+                               niter=r_l[2].niter, nfft=list(ls.nfft))
+            lsolver.close()
+            log(f"bench: {name} {large[name]}")
+
     # live roofline of the dominant kernel (mixed real-space matvec): one
     # launch over the serving batch when that leg is the fastest
     best = max(legs, key=lambda k: legs[k]["solves_per_s"])
@@ -256,6 +271,7 @@ def main():
             "serve": serve,
             "stream": stream,
             "dhfr2_mixed_one_system": dhfr2,
+            "large_mixed_one_system": large,
             "niter": legs[best]["niter"],
             "errors_vs_fp64_oracle": err or None,
         },
