@@ -6,6 +6,7 @@ fails: closed forms, finite-difference identities, a brute-force neighbor
 search, the textbook Ewald structure-factor sum, alpha-invariance of the
 full Ewald field, a dense linear solve and the energy decomposition.
 """
+import dataclasses
 import math
 
 import numpy as np
@@ -537,3 +538,103 @@ def test_rotpole_mirror_image_needs_the_chirality_check():
     no_y[chiral] = -1   # the same frames without the check
     rpn = O.rotpole(*(xyz @ m.T).T, axt, fz, fx, no_y, pole)
     assert np.abs(rpn[chiral] - _transform(rp, m)[chiral]).max() > 1e-3
+
+
+# ------------------------------------------ polarization energy gradient
+# The energy gradient (epolar1.f: epreal1c, eprecip1 and the self and
+# torque terms of epolar1c) rests on one identity.  With A = 1/alpha - T
+# symmetric, u_d = A^-1 E_d and u_p = A^-1 E_p, the derivative of
+# E_pol = -f/2 u_d . E_p along any coordinate is
+#     dE_pol = -f/2 (u_p . dE_d + u_d . dE_p + u_p . dT u_d),
+# i.e. the derivative of G = -f/2 (u_p . E_d + u_d . E_p + u_p . T u_d)
+# with both dipole sets held fixed.  The check below differentiates both
+# sides numerically on a small water box whose multipoles follow local
+# frames (so the permanent fields move with the orientations as well), the
+# left side re-solving the dipoles at every displaced geometry.
+KCOUL = 332.063714  # electric, MOD_units.f:81
+
+
+def _water_frames(s, seed=11):
+    """O bisector (H1, H2), H Z-then-X (O, other H); random local multipoles
+    symmetric under y -> -y (no site has a chirality check)"""
+    rng = np.random.default_rng(seed)
+    n = s.n
+    axt = np.zeros(n, np.int32)
+    fz, fx, fy = (np.full(n, -1, np.int32) for _ in range(3))
+    pole = np.zeros((n, 13))
+    for m in range(n // 3):
+        o, h1, h2 = 3 * m, 3 * m + 1, 3 * m + 2
+        axt[o], fz[o], fx[o] = O.AXIS_BISECTOR, h1, h2
+        axt[h1], fz[h1], fx[h1] = O.AXIS_Z_THEN_X, o, h2
+        axt[h2], fz[h2], fx[h2] = O.AXIS_Z_THEN_X, o, h1
+    for i in range(n):
+        q = rng.normal(0, 0.1, (3, 3))
+        q = q + q.T
+        q -= np.trace(q) / 3 * np.eye(3)
+        q[0, 1] = q[1, 0] = q[1, 2] = q[2, 1] = 0.0
+        d = rng.normal(0, 0.1, 3)
+        d[1] = 0.0
+        pole[i] = np.concatenate([[s.rpole[i, 0]], d, q.reshape(-1)])
+    return axt, fz, fx, fy, pole
+
+
+def _at(s, fr, xyz):
+    rp = O.rotpole(*xyz.T, *fr)
+    return O.Oracle(s.with_coords(*xyz.T).with_multipoles(rp))
+
+
+def _epol(s, fr, xyz):
+    return _at(s, fr, xyz).induce(poleps=1e-12)["ep"]
+
+
+def _g_fixed(s, fr, xyz, ud, up):
+    orc = _at(s, fr, xyz)
+    ef = orc.efld0_total()
+    alpha = np.asarray(s.polarity)[:, None]
+    tud = ud / alpha - orc.matvec(np.stack([ud, ud], axis=1))[:, 0]   # T u_d
+    return -0.5 * KCOUL * (np.sum(up * ef[:, 0]) + np.sum(ud * ef[:, 1]) + np.sum(up * tud))
+
+
+def _d4(f, h):
+    """four-point central difference"""
+    return (8.0 * (f(h) - f(-h)) - (f(2 * h) - f(-2 * h))) / (12.0 * h)
+
+
+def test_polarization_gradient_identity_with_rotating_frames():
+    s = thp.tiny_water(27, 10.0, 12, seed=4)
+    # intramolecular p-scales of 0.5 (d-scales stay 0) so that u_d, u_p and
+    # E_d, E_p differ: exchanging d and p in G then changes its derivative
+    dpu = np.array(s.scaled_dpu, dtype=np.float64).reshape(-1, 3).copy()
+    dpu[:, 1] = 0.5
+    s = dataclasses.replace(s, scaled_dpu=dpu)
+    fr = _water_frames(s)
+    xyz = np.stack([s.x, s.y, s.z], axis=1)
+    ref = _at(s, fr, xyz).induce(poleps=1e-12)
+    ud, up = ref["mu"][:, 0], ref["mu"][:, 1]
+    ef = _at(s, fr, xyz).efld0_total()
+    # the energy itself: E_pol = -f/2 u_d . E_p
+    assert abs(ref["ep"] + 0.5 * KCOUL * np.sum(ud * ef[:, 1])) < 1e-10 * abs(ref["ep"])
+    h = 2e-4
+    # the real-space sums stop at the cutoff: an atom with a partner within
+    # a few steps of it would see a jump, so the checked atoms keep clear
+    box = s.box[0]
+
+    def clear(a):
+        d = xyz - xyz[a]
+        d -= box * np.round(d / box)
+        r = np.linalg.norm(d, axis=1)
+        r[a] = 0.0
+        return np.min(np.abs(r - s.cut)) > 50 * h
+
+    atoms = [a for a in range(s.n) if clear(a)]
+    picks = [atoms[0], atoms[1], atoms[2], atoms[len(atoms) // 2], atoms[-2], atoms[-1]]
+    assert len(set(picks)) == 6 and {p % 3 for p in picks} >= {0, 1, 2}
+    for atom, axis in zip(picks, (0, 1, 2, 0, 2, 1)):
+        def move(dx):
+            x2 = xyz.copy()
+            x2[atom, axis] += dx
+            return x2
+        de = _d4(lambda dx: _epol(s, fr, move(dx)), h)
+        dg = _d4(lambda dx: _g_fixed(s, fr, move(dx), ud, up), h)
+        assert abs(de) > 1e-3                      # a real force component
+        assert abs(de - dg) < 1e-8 * max(1.0, abs(de)), (atom, axis, de, dg)
