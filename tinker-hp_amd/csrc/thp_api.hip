@@ -644,10 +644,13 @@ thp_status b_pair_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int do
 // The real-space matvec of a vector runs on the side stream while the main
 // stream spreads and transforms the same vector (the two read it and write
 // different arrays); join_matvec makes the main stream wait for it before
-// the gather that adds the two parts.  Without a side stream the matvec
-// runs in line.
+// the gather that adds the two parts.  Only one-system launches fork: a
+// batch of systems fills the device with either branch, and overlapping
+// them gains nothing.  Otherwise the matvec runs in line.
+inline bool forks(const thp_ctx* ctx, const Batch& b) { return ctx->side && !b.real_only && b.nsys == 1; }
+
 thp_status fork_matvec(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int gate) {
-  if (!ctx->side || b.real_only) return b_pair_tmatxb(ctx, b, vin, vout, dodiag, gate);
+  if (!forks(ctx, b)) return b_pair_tmatxb(ctx, b, vin, vout, dodiag, gate);
   HIPCHK(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
   HIPCHK(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
   RET(b_pair_tmatxb_on(ctx, b, vin, vout, dodiag, gate, ctx->side));
@@ -656,7 +659,7 @@ thp_status fork_matvec(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodi
 }
 
 thp_status join_matvec(thp_ctx* ctx, const Batch& b) {
-  if (!ctx->side || b.real_only) return THP_OK;
+  if (!forks(ctx, b)) return THP_OK;
   HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
   return THP_OK;
 }
