@@ -511,3 +511,146 @@ def rotpole(x, y, z, axistype, zaxis, xaxis, yaxis, pole):
         out[i, 1:4] = a @ p[1:4]
         out[i, 4:13] = (a @ p[4:13].reshape(3, 3) @ a.T).reshape(-1)
     return out
+
+
+# ------------------------------------------- polarization energy gradient
+# Real-space part of the polarization gradient of epolar1c
+# (v1.2/source/epolar1.f:1215-2543, epreal1c), written from the identity
+# dE_pol = -f/2 (u_p . dE_d + u_d . dE_p + u_p . dT u_d) at fixed dipoles.
+# Every pair interaction derives from one radial kernel psi whose radial
+# ladder dmp3, dmp5, dmp7, dmp9 (Ewald bn_n less the Thole-damped, scaled
+# rr_n: dmp_n = bn_n - (1 - s lambda_n) rr_n) obeys d dmp_n / dr =
+# -r dmp_(n+2), so the Cartesian derivative tensors of psi are
+#   T1_a = -R_a G1,  T2_ab = -d_ab G1 + R_a R_b G2,
+#   T3_abc = (d_ab R_c + d_ac R_b + d_bc R_a) G2 - R_a R_b R_c G3,
+#   T4_abcd = (d_ab d_cd + d_ac d_bd + d_ad d_bc) G2 - (six d_.. R_. R_.) G3
+#             + R_a R_b R_c R_d G4
+# (G1..G4 = dmp3..dmp9, R = field point - source).  A multipole (c, d, Q)
+# has the potential c psi - d . grad psi + Q : grad grad psi (Q as stored
+# in rpole, the reference's traceless Q/3), a dipole u the potential
+# -u . grad psi.
+
+KCOUL = 332.063714  # electric, v1.2/source/MOD_units.f:81
+
+
+def _thole_lambdas(r, pdi, pdk, thl):
+    """lambda3, 5, 7, 9 of dampthole (damping.f:74-215) and their
+    continuation to order 9 (1 - (1 + x + 18/35 x^2 + 9/35 x^3) e^-x)"""
+    damp = pdi * pdk
+    ok = (damp != 0.0) & (thl != 0.0)
+    x = np.where(ok, thl * (r / np.where(ok, damp, 1.0)) ** 3, np.inf)
+    live = x < 50.0
+    e = np.where(live, np.exp(-np.where(live, x, 0.0)), 0.0)
+    xs = np.where(live, x, 0.0)
+    return (1.0 - e, 1.0 - (1.0 + xs) * e, 1.0 - (1.0 + xs + 0.6 * xs ** 2) * e,
+            1.0 - (1.0 + xs + 18.0 / 35.0 * xs ** 2 + 9.0 / 35.0 * xs ** 3) * e)
+
+
+def _ladder(r, aewald, lam, scale):
+    """dmp3, dmp5, dmp7, dmp9 of the pairs for one set of scale factors"""
+    from scipy.special import erfc
+    r2 = r * r
+    ra = aewald * r
+    ex = np.exp(-ra * ra)
+    bn = [erfc(ra) / r]
+    afac = 1.0 / (np.sqrt(np.pi) * aewald)
+    for i in range(1, 5):
+        afac *= 2.0 * aewald * aewald
+        bn.append(((2 * i - 1) * bn[-1] + afac * ex) / r2)
+    rr = [1.0 / (r2 * r), 3.0 / (r2 * r2 * r), 15.0 / (r2 ** 3 * r), 105.0 / (r2 ** 4 * r)]
+    return [bn[n + 1] - (1.0 - scale * lam[n]) * rr[n] for n in range(4)]
+
+
+def _psi_tensors(R, G):
+    """T1..T4 of the kernel at separations R (P, 3)"""
+    G1, G2, G3, G4 = (g[:, None] for g in G)
+    I = np.eye(3)
+    T1 = -R * G1
+    RR = np.einsum("pa,pb->pab", R, R)
+    T2 = -I[None] * G1[:, :, None] + RR * G2[:, :, None]
+    dR = np.einsum("ab,pc->pabc", I, R)
+    T3 = (dR + dR.transpose(0, 1, 3, 2) + dR.transpose(0, 3, 1, 2)) * G2[:, :, None, None] \
+        - np.einsum("pab,pc->pabc", RR, R) * G3[:, :, None, None]
+    dd = np.einsum("ab,cd->abcd", I, I)
+    dd = dd + dd.transpose(0, 2, 1, 3) + dd.transpose(0, 3, 2, 1)
+    dRR = np.einsum("ab,pcd->pabcd", I, RR)
+    six = (dRR + dRR.transpose(0, 1, 3, 2, 4) + dRR.transpose(0, 1, 3, 4, 2)
+           + dRR.transpose(0, 3, 4, 1, 2) + dRR.transpose(0, 3, 1, 4, 2) + dRR.transpose(0, 3, 1, 2, 4))
+    T4 = dd[None] * G2[:, :, None, None, None] - six * G3[:, :, None, None, None] \
+        + np.einsum("pab,pcd->pabcd", RR, RR) * G4[:, :, None, None, None]
+    return T1, T2, T3, T4
+
+
+_EPS = np.zeros((3, 3, 3))
+_EPS[0, 1, 2] = _EPS[1, 2, 0] = _EPS[2, 0, 1] = 1.0
+_EPS[0, 2, 1] = _EPS[2, 1, 0] = _EPS[1, 0, 2] = -1.0
+
+
+def epolar1_real(system, rpole, ud, up, pairs=None):
+    """Real-space polarization forces and torques (kcal/mol/A and
+    kcal/mol) on every site for global multipoles `rpole` (n, 13) and the
+    dipole sets ud, up (n, 3): F = -dG/dr and T with dG/dtheta = -T for a
+    rotation of a site's multipole about its centre, where
+    G = -f/2 (u_p . E_d + u_d . E_p + u_p . T u_d) over the real-space
+    pairs within the cutoff (scaled pairs with their d, p, u factors)."""
+    s = system
+    n = s.n
+    xyz = np.stack([s.x, s.y, s.z], axis=1)
+    box = np.asarray(s.box, dtype=np.float64)
+    if pairs is None:
+        pairs = Oracle(s).nblist_pairs()
+    i, k = pairs[:, 0].astype(np.int64), pairs[:, 1].astype(np.int64)
+    d = xyz[i] - xyz[k]
+    d -= box * np.round(d / box)
+    r = np.linalg.norm(d, axis=1)
+    keep = r <= s.cut
+    i, k, d, r = i[keep], k[keep], d[keep], r[keep]
+    # both orders: field point j, source m, R = r_j - r_m
+    j = np.concatenate([i, k])
+    m = np.concatenate([k, i])
+    R = np.concatenate([d, -d])
+    r = np.concatenate([r, r])
+    sc = np.ones((len(j), 3))
+    lut = {}
+    for (a, b), v in zip(np.asarray(s.scaled_ik).reshape(-1, 2), np.asarray(s.scaled_dpu).reshape(-1, 3)):
+        lut[(int(a), int(b))] = v
+        lut[(int(b), int(a))] = v
+    for q in range(len(j)):
+        v = lut.get((int(j[q]), int(m[q])))
+        if v is not None:
+            sc[q] = v
+    pol = np.asarray(s.polarity, dtype=np.float64)
+    th = np.asarray(s.thole, dtype=np.float64)
+    pd = pol ** (1.0 / 6.0)
+    thl = np.minimum(th[j], th[m])
+    thl = np.where(thl == 0.0, np.maximum(th[j], th[m]), thl)
+    lam = _thole_lambdas(r, pd[j], pd[m], thl)
+    c = rpole[:, 0]
+    dp = rpole[:, 1:4]
+    Q = rpole[:, 4:13].reshape(n, 3, 3)
+    F = np.zeros((n, 3))
+    T = np.zeros((n, 3))
+    for which, (u_at_j, u_src, s_idx) in enumerate(((up, up, 0), (ud, ud, 1))):
+        # u_p with the d-scaled fields (s_idx 0), u_d with the p-scaled (1)
+        T1, T2, T3, T4 = _psi_tensors(R, _ladder(r, s.aewald, lam, sc[:, s_idx]))
+        # A: dipole at j in the field gradient of the multipole at m
+        ddphi = (c[m][:, None, None] * T2 - np.einsum("pc,pabc->pab", dp[m], T3)
+                 + np.einsum("pcd,pabcd->pab", Q[m], T4))
+        np.add.at(F, j, -np.einsum("pa,pab->pb", u_at_j[j], ddphi))
+        # B: multipole at j in the field of the dipole at m
+        dphi = -np.einsum("pb,pab->pa", u_src[m], T2)
+        d2phi = -np.einsum("pc,pabc->pab", u_src[m], T3)
+        d3phi = -np.einsum("pd,pabcd->pabc", u_src[m], T4)
+        E = -dphi
+        fB = c[j][:, None] * E - np.einsum("pa,pab->pb", dp[j], d2phi) - np.einsum("pac,pacb->pb", Q[j], d3phi)
+        np.add.at(F, j, fB)
+        tq = np.cross(dp[j], E)
+        A = np.einsum("pab,pbc->pac", Q[j], d2phi) - np.einsum("pab,pbc->pac", d2phi, Q[j])
+        tq -= np.einsum("cdb,pdb->pc", _EPS, A)
+        np.add.at(T, j, tq)
+    # C: the mutual term, u-scaled, both dipole sets at j
+    T1, T2, T3, T4 = _psi_tensors(R, _ladder(r, s.aewald, lam, sc[:, 2]))
+    for u_at_j, u_src in ((up, ud), (ud, up)):
+        d2phi = -np.einsum("pc,pabc->pab", u_src[m], T3)
+        np.add.at(F, j, -np.einsum("pa,pab->pb", u_at_j[j], d2phi))
+    return 0.5 * KCOUL * F, 0.5 * KCOUL * T

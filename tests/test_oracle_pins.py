@@ -638,3 +638,62 @@ def test_polarization_gradient_identity_with_rotating_frames():
         dg = _d4(lambda dx: _g_fixed(s, fr, move(dx), ud, up), h)
         assert abs(de) > 1e-3                      # a real force component
         assert abs(de - dg) < 1e-8 * max(1.0, abs(de)), (atom, axis, de, dg)
+
+
+def _rot(axis, ang):
+    a = np.asarray(axis, dtype=np.float64)
+    a = a / np.linalg.norm(a)
+    k = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(ang) * k + (1 - np.cos(ang)) * (k @ k)
+
+
+def test_real_space_polarization_forces_and_torques_match_differences():
+    # epolar1_real against central differences of the real-space part of G
+    # (efld0_direct's fields and tmatxb's dipole field, dipoles and global
+    # multipoles held fixed): forces along coordinates, torques along
+    # rotations of one site's multipole about its centre
+    s = thp.tiny_water(27, 10.0, 12, seed=4)
+    dpu = np.array(s.scaled_dpu, dtype=np.float64).reshape(-1, 3).copy()
+    dpu[:, 1] = 0.5
+    s = dataclasses.replace(s, scaled_dpu=dpu)
+    fr = _water_frames(s)
+    xyz = np.stack([s.x, s.y, s.z], axis=1)
+    rp = O.rotpole(*xyz.T, *fr)
+    s = s.with_multipoles(rp)
+    mu = O.Oracle(s).induce(poleps=1e-10)["mu"]
+    ud, up = mu[:, 0], mu[:, 1]
+    F, T = O.epolar1_real(s, rp, ud, up)
+
+    def g_real(xyz2, rp2):
+        orc = O.Oracle(s.with_coords(*xyz2.T).with_multipoles(rp2))
+        ef = orc.efld0_direct()
+        tud = -orc.tmatxb_pme(np.stack([ud, ud], axis=1), dodiag=False)[:, 0]
+        return -0.5 * KCOUL * (np.sum(up * ef[:, 0]) + np.sum(ud * ef[:, 1]) + np.sum(up * tud))
+
+    h = 2e-4
+    box = s.box[0]
+
+    def clear(a):
+        d = xyz - xyz[a]
+        d -= box * np.round(d / box)
+        r = np.linalg.norm(d, axis=1)
+        r[a] = 0.0
+        return np.min(np.abs(r - s.cut)) > 50 * h
+
+    atoms = [a for a in range(s.n) if clear(a)]
+    for atom, axis in zip([atoms[0], atoms[1], atoms[2], atoms[-1]], (0, 1, 2, 1)):
+        def g_at(dx):
+            x2 = xyz.copy()
+            x2[atom, axis] += dx
+            return g_real(x2, rp)
+        fd = -_d4(g_at, h)
+        assert abs(F[atom, axis] - fd) < 1e-7 * max(1.0, abs(fd)), (atom, axis, F[atom, axis], fd)
+    for atom, e in ((0, (1.0, 0.3, -0.2)), (4, (0.0, 1.0, 0.5)), (8, (-0.4, 0.1, 1.0))):
+        def g_rot(th):
+            m = _rot(e, th)
+            rp2 = rp.copy()
+            rp2[atom] = _transform(rp[atom:atom + 1], m)[0]
+            return g_real(xyz, rp2)
+        dg = _d4(g_rot, 1e-4)
+        ee = np.asarray(e) / np.linalg.norm(e)
+        assert abs(dg + T[atom] @ ee) < 1e-7 * max(1.0, abs(dg)), (atom, dg, -T[atom] @ ee)
