@@ -654,3 +654,128 @@ def epolar1_real(system, rpole, ud, up, pairs=None):
         d2phi = -np.einsum("pc,pabc->pab", u_src[m], T3)
         np.add.at(F, j, -np.einsum("pa,pab->pb", u_at_j[j], d2phi))
     return 0.5 * KCOUL * F, 0.5 * KCOUL * T
+
+
+# Reciprocal part (eprecip1, v1.2/source/epolar1.f:397-1203).  The PME
+# bilinear forms are differentiated through the B-spline stencils of both
+# their field points and their sources, which gives the same assembly as the
+# real-space pairs with PME potential derivatives in place of the pair
+# tensors (no scale factors in reciprocal space, so E_d = E_p there).  The
+# grid conventions are those of the C++ oracle (bspline_fill_site,
+# grid_uind_site, make_qfac, unnormalised transforms both ways); the
+# orthorhombic box turns fractional derivatives into Cartesian ones by the
+# factors a_k = nfft_k / L_k.
+
+def _pme_stencils(s):
+    """per axis: B-spline values and 3 derivatives (n, order, 4) and the
+    grid nodes (n, order) of every site (bspline_fill_site, node_index)"""
+    order = s.order
+    xyz = np.stack([s.x, s.y, s.z], axis=1)
+    th, nodes = [], []
+    for d in range(3):
+        L, K = float(s.box[d]), int(s.nfft[d])
+        w = xyz[:, d] / L
+        anint = np.sign(w) * np.floor(np.abs(w) + 0.5)
+        fr = K * (w - anint + 0.5)
+        ifr = np.trunc(fr - 1.0e-8).astype(np.int64)
+        ww = fr - ifr
+        th.append(np.stack([bsplgen(v, order) for v in ww]))
+        g1 = (ifr - order)[:, None] + ((order + 1) // 2 + 1) - (order - 1) // 2 + np.arange(order)[None, :]
+        g1 = np.where(g1 < 1, g1 + K, g1)
+        nodes.append(g1 - 1)
+    return th, nodes
+
+
+def _qfac(s):
+    """make_qfac: exp(-pi^2 h^2 / alpha^2) / (pi V h^2 bsmod1 bsmod2 bsmod3)"""
+    n1, n2, n3 = (int(v) for v in s.nfft)
+    L = [float(v) for v in s.box]
+    ms = []
+    for K, Lk in zip((n1, n2, n3), L):
+        k = np.arange(K)
+        m = np.where(k + 1 > (K + 1) // 2, k - K, k)
+        ms.append(m / Lk)
+    h3, h2, h1 = np.meshgrid(ms[2], ms[1], ms[0], indexing="ij")
+    hsq = h1 ** 2 + h2 ** 2 + h3 ** 2
+    b1, b2, b3 = (bsmod(K, s.order) for K in (n1, n2, n3))
+    bb = b3[:, None, None] * b2[None, :, None] * b1[None, None, :]
+    term = -(np.pi / s.aewald) ** 2 * hsq
+    with np.errstate(divide="ignore", invalid="ignore"):
+        q = np.where(term > -50.0, np.exp(term) / (np.pi * L[0] * L[1] * L[2] * hsq * bb), 0.0)
+    q[0, 0, 0] = 0.0
+    return q
+
+
+def _pme_dipole_potential(s, u, th, nodes, q):
+    """Cartesian derivatives 1..3 of the PME potential of the dipoles u at
+    every site: D1 (n, 3), D2 (n, 3, 3), D3 (n, 3, 3, 3)"""
+    n1, n2, n3 = (int(v) for v in s.nfft)
+    a = np.array([n1 / s.box[0], n2 / s.box[1], n3 / s.box[2]])
+    fu = u * a[None, :]
+    grid = np.zeros((n3, n2, n1))
+    t, uu, v = th[0], th[1], th[2]
+    for i in range(s.n):
+        w = (fu[i, 0] * np.einsum("a,b,c->cba", t[i, :, 1], uu[i, :, 0], v[i, :, 0])
+             + fu[i, 1] * np.einsum("a,b,c->cba", t[i, :, 0], uu[i, :, 1], v[i, :, 0])
+             + fu[i, 2] * np.einsum("a,b,c->cba", t[i, :, 0], uu[i, :, 0], v[i, :, 1]))
+        grid[np.ix_(nodes[2][i], nodes[1][i], nodes[0][i])] += w
+    pot = np.real(np.fft.ifftn(q * np.fft.fftn(grid))) * grid.size
+    D1 = np.zeros((s.n, 3))
+    D2 = np.zeros((s.n, 3, 3))
+    D3 = np.zeros((s.n, 3, 3, 3))
+    for i in range(s.n):
+        P = pot[np.ix_(nodes[2][i], nodes[1][i], nodes[0][i])]  # [z, y, x]
+
+        def der(ax):
+            o = [ax.count(0), ax.count(1), ax.count(2)]
+            return np.einsum("cba,a,b,c->", P, t[i, :, o[0]], uu[i, :, o[1]], v[i, :, o[2]]) * np.prod(a ** o)
+
+        for p in range(3):
+            D1[i, p] = der((p,))
+            for r in range(3):
+                D2[i, p, r] = der((p, r))
+                for w3 in range(3):
+                    D3[i, p, r, w3] = der((p, r, w3))
+    return D1, D2, D3
+
+
+def epolar1_recip_self(system, rpole, ud, up):
+    """Reciprocal and self parts of the polarization forces and torques,
+    with the conventions of epolar1_real.  The multipole potential's second
+    derivatives come from efld0_recip (cphi); the dipole sets' derivatives
+    up to third order from the grid above.  The Ewald self field
+    (4 alpha^3 / 3 sqrt(pi)) d of the permanent dipoles (efld0_total) gives
+    each site the torque f/2 C d x (u_d + u_p); the dipoles' own self field
+    depends on no coordinate."""
+    s = system
+    n = s.n
+    c = rpole[:, 0]
+    dp = rpole[:, 1:4]
+    Q = rpole[:, 4:13].reshape(n, 3, 3)
+    cphi = Oracle(s.with_multipoles(rpole)).efld0_recip()
+    PhiM = np.zeros((n, 3, 3))
+    for (p, r), col in zip(((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)), range(4, 10)):
+        PhiM[:, p, r] = PhiM[:, r, p] = cphi[:, col]
+    th, nodes = _pme_stencils(s)
+    q = _qfac(s)
+    Dd = _pme_dipole_potential(s, ud, th, nodes, q)
+    Dp = _pme_dipole_potential(s, up, th, nodes, q)
+    F = np.zeros((n, 3))
+    T = np.zeros((n, 3))
+    # A: both dipole sets at their sites in the multipoles' field gradient
+    F -= np.einsum("pa,pab->pb", ud + up, PhiM)
+    # B: the multipoles in the field of both dipole sets, and their torques
+    D1 = Dd[0] + Dp[0]
+    D2 = Dd[1] + Dp[1]
+    D3 = Dd[2] + Dp[2]
+    E = -D1
+    F += c[:, None] * E - np.einsum("pa,pab->pb", dp, D2) - np.einsum("pac,pacb->pb", Q, D3)
+    T += np.cross(dp, E)
+    A = np.einsum("pab,pbc->pac", Q, D2) - np.einsum("pab,pbc->pac", D2, Q)
+    T -= np.einsum("cdb,pdb->pc", _EPS, A)
+    # C: the mutual term
+    F -= np.einsum("pa,pab->pb", up, Dd[1]) + np.einsum("pa,pab->pb", ud, Dp[1])
+    # self torque
+    cself = 4.0 * s.aewald ** 3 / (3.0 * np.sqrt(np.pi))
+    T += cself * np.cross(dp, ud + up)
+    return 0.5 * KCOUL * F, 0.5 * KCOUL * T

@@ -697,3 +697,45 @@ def test_real_space_polarization_forces_and_torques_match_differences():
         dg = _d4(g_rot, 1e-4)
         ee = np.asarray(e) / np.linalg.norm(e)
         assert abs(dg + T[atom] @ ee) < 1e-7 * max(1.0, abs(dg)), (atom, dg, -T[atom] @ ee)
+
+
+def test_reciprocal_and_self_polarization_forces_and_torques_match_differences():
+    # epolar1_recip_self against central differences of the reciprocal and
+    # self parts of G (efld0_recip's field, tmatxbrecip's dipole field and
+    # the self fields, dipoles and global multipoles held fixed); its grid
+    # reproduces tmatxbrecip's first derivatives
+    s = thp.tiny_water(27, 10.0, 12, seed=4)
+    fr = _water_frames(s)
+    xyz = np.stack([s.x, s.y, s.z], axis=1)
+    rp = O.rotpole(*xyz.T, *fr)
+    s = s.with_multipoles(rp)
+    mu = O.Oracle(s).induce(poleps=1e-10)["mu"]
+    ud, up = mu[:, 0], mu[:, 1]
+    th, nodes = O._pme_stencils(s)
+    D1 = O._pme_dipole_potential(s, ud, th, nodes, O._qfac(s))[0]
+    np.testing.assert_allclose(D1, O.Oracle(s).tmatxbrecip(mu)[:, 0], rtol=0, atol=1e-12 * np.abs(D1).max())
+    F, T = O.epolar1_recip_self(s, rp, ud, up)
+    cself = 4.0 * s.aewald ** 3 / (3.0 * np.sqrt(np.pi))
+
+    def g_rs(xyz2, rp2):
+        orc = O.Oracle(s.with_coords(*xyz2.T).with_multipoles(rp2))
+        e = -orc.efld0_recip()[:, 1:4] + cself * rp2[:, 1:4]
+        tud = -orc.tmatxbrecip(np.stack([ud, ud], axis=1))[:, 0]
+        return -0.5 * KCOUL * (np.sum(up * e) + np.sum(ud * e) + np.sum(up * tud))
+
+    h = 1e-4
+    for atom, axis in ((0, 0), (1, 1), (2, 2), (40, 0), (77, 2)):
+        def g_at(dx):
+            x2 = xyz.copy()
+            x2[atom, axis] += dx
+            return g_rs(x2, rp)
+        fd = -_d4(g_at, h)
+        assert abs(F[atom, axis] - fd) < 1e-7 * max(1.0, abs(fd)), (atom, axis, F[atom, axis], fd)
+    for atom, e in ((0, (1.0, 0.3, -0.2)), (4, (0.0, 1.0, 0.5)), (8, (-0.4, 0.1, 1.0))):
+        def g_rot(t):
+            rp2 = rp.copy()
+            rp2[atom] = _transform(rp[atom:atom + 1], _rot(e, t))[0]
+            return g_rs(xyz, rp2)
+        dg = _d4(g_rot, 1e-4)
+        ee = np.asarray(e) / np.linalg.norm(e)
+        assert abs(dg + T[atom] @ ee) < 1e-7 * max(1.0, abs(dg)), (atom, dg, -T[atom] @ ee)
