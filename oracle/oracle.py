@@ -779,3 +779,46 @@ def epolar1_recip_self(system, rpole, ud, up):
     cself = 4.0 * s.aewald ** 3 / (3.0 * np.sqrt(np.pi))
     T += cself * np.cross(dp, ud + up)
     return 0.5 * KCOUL * F, 0.5 * KCOUL * T
+
+
+def torque_forces(xyz, axistype, zaxis, xaxis, yaxis, torque, h=1e-6):
+    """Forces on the frame atoms from the torques on the sites' multipoles
+    (the product that torque.f forms in closed form per axis type), by brute
+    force: a site's frame turns by theta(r), and an atom a of the frame
+    takes F_a = sum_sites T . d theta / d r_a, with d theta the axial vector
+    of dR R^T from central differences of rotmat."""
+    xyz = np.asarray(xyz, dtype=np.float64)
+    F = np.zeros_like(xyz)
+    for i in range(len(xyz)):
+        t = int(axistype[i])
+        if t == AXIS_NONE:
+            continue
+        R0 = rotmat(xyz, i, t, zaxis[i], xaxis[i], yaxis[i])
+        atoms = {i, int(zaxis[i])} | {int(v) for v in (xaxis[i], yaxis[i]) if v >= 0}
+        for a in atoms:
+            for c in range(3):
+                xp, xm = xyz.copy(), xyz.copy()
+                xp[a, c] += h
+                xm[a, c] -= h
+                dR = (rotmat(xp, i, t, zaxis[i], xaxis[i], yaxis[i]) - rotmat(xm, i, t, zaxis[i], xaxis[i], yaxis[i])) / (2 * h)
+                W = dR @ R0.T
+                dth = np.array([W[2, 1], W[0, 2], W[1, 0]])
+                F[a, c] += torque[i] @ dth
+    return F
+
+
+def epolar1(system, frames, poleps=1e-10):
+    """E_pol and its gradient (-forces, kcal/mol/A) for local multipoles on
+    frames (axistype, zaxis, xaxis, yaxis, pole): rotpole, the solve, the
+    real, reciprocal and self forces and torques, and the torques carried to
+    the frame atoms (epolar1c, epolar1.f:57-376)."""
+    s = system
+    xyz = np.stack([s.x, s.y, s.z], axis=1)
+    rp = rotpole(*xyz.T, *frames)
+    s2 = s.with_multipoles(rp)
+    res = Oracle(s2).induce(poleps=poleps)
+    ud, up = res["mu"][:, 0], res["mu"][:, 1]
+    F1, T1 = epolar1_real(s2, rp, ud, up)
+    F2, T2 = epolar1_recip_self(s2, rp, ud, up)
+    F = F1 + F2 + torque_forces(xyz, frames[0], frames[1], frames[2], frames[3], T1 + T2)
+    return res["ep"], -F, res

@@ -739,3 +739,35 @@ def test_reciprocal_and_self_polarization_forces_and_torques_match_differences()
         dg = _d4(g_rot, 1e-4)
         ee = np.asarray(e) / np.linalg.norm(e)
         assert abs(dg + T[atom] @ ee) < 1e-7 * max(1.0, abs(dg)), (atom, dg, -T[atom] @ ee)
+
+
+def test_polarization_gradient_matches_differences_of_the_solved_energy():
+    # the whole oracle gradient (real, reciprocal and self terms, torques
+    # carried to the frame atoms) against central differences of E_pol with
+    # the multipoles rotated and the dipoles re-solved at every geometry
+    s = thp.tiny_water(27, 10.0, 12, seed=4)
+    dpu = np.array(s.scaled_dpu, dtype=np.float64).reshape(-1, 3).copy()
+    dpu[:, 1] = 0.5
+    s = dataclasses.replace(s, scaled_dpu=dpu)
+    fr = _water_frames(s)
+    xyz = np.stack([s.x, s.y, s.z], axis=1)
+    ep, grad, _ = O.epolar1(s, fr, poleps=1e-12)
+    h = 2e-4
+    box = s.box[0]
+
+    def clear(a):
+        d = xyz - xyz[a]
+        d -= box * np.round(d / box)
+        r = np.linalg.norm(d, axis=1)
+        r[a] = 0.0
+        return np.min(np.abs(r - s.cut)) > 50 * h
+
+    atoms = [a for a in range(s.n) if clear(a)]
+    picks = [atoms[0], atoms[1], atoms[2], atoms[len(atoms) // 2], atoms[-1]]
+    for atom, axis in zip(picks, (0, 1, 2, 1, 0)):
+        def e_at(dx):
+            x2 = xyz.copy()
+            x2[atom, axis] += dx
+            return _epol(s, fr, x2)
+        fd = _d4(e_at, h)
+        assert abs(grad[atom, axis] - fd) < 1e-6 * max(1.0, abs(fd)), (atom, axis, grad[atom, axis], fd)
