@@ -811,7 +811,7 @@ def epolar1(system, frames, poleps=1e-10):
     """E_pol and its gradient (-forces, kcal/mol/A) for local multipoles on
     frames (axistype, zaxis, xaxis, yaxis, pole): rotpole, the solve, the
     real, reciprocal and self forces and torques, and the torques carried to
-    the frame atoms (epolar1c, epolar1.f:57-376)."""
+    the frame atoms in closed form (epolar1c, epolar1.f:57-376; torque.f)."""
     s = system
     xyz = np.stack([s.x, s.y, s.z], axis=1)
     rp = rotpole(*xyz.T, *frames)
@@ -820,5 +820,75 @@ def epolar1(system, frames, poleps=1e-10):
     ud, up = res["mu"][:, 0], res["mu"][:, 1]
     F1, T1 = epolar1_real(s2, rp, ud, up)
     F2, T2 = epolar1_recip_self(s2, rp, ud, up)
-    F = F1 + F2 + torque_forces(xyz, frames[0], frames[1], frames[2], frames[3], T1 + T2)
+    F = F1 + F2 + torque_forces_closed(xyz, frames[0], frames[1], frames[2], frames[3], T1 + T2)
     return res["ep"], -F, res
+
+
+def torque_forces_closed(xyz, axistype, zaxis, xaxis, yaxis, torque):
+    """torque_forces in closed form (the product torque.f forms per axis
+    type).  A frame (x, y, z) built from z = unit(v) and the xz-plane vector
+    u turns by omega = z x dz + (y . dx) z, with y . dx = (y . du -
+    (u . z)(y . dz)) / |u - (u . z) z|, so T . omega = g_z . dz + g_u . du
+    with g_z = T x z - (T . z)(u . z) y / |u_perp| and g_u = (T . z) y /
+    |u_perp|; dz = (1 - z z^T) dv / |v|, and v and u are sums of bond
+    vectors or of unit bond vectors (a unit vector e = unit(r_a - r_i)
+    passes a gradient g on as (1 - e e^T) g / |r_a - r_i| to r_a and its
+    negative to r_i).  Scaling u changes nothing, so Z-Bisect uses the
+    unnormalised sum of its two unit vectors."""
+    xyz = np.asarray(xyz, dtype=np.float64)
+    F = np.zeros_like(xyz)
+    unit = lambda v: v / np.linalg.norm(v)
+
+    def through_unit(a, i, g):
+        # gradient g on unit(r_a - r_i) -> forces on a and i
+        d = xyz[a] - xyz[i]
+        r = np.linalg.norm(d)
+        e = d / r
+        h = (g - e * np.dot(e, g)) / r
+        F[a] += h
+        F[i] -= h
+
+    for i in range(len(xyz)):
+        t = int(axistype[i])
+        if t == AXIS_NONE:
+            continue
+        iz, ix, iy = int(zaxis[i]), int(xaxis[i]), int(yaxis[i])
+        R = rotmat(xyz, i, t, iz, ix, iy)
+        x, y, z = R[:, 0], R[:, 1], R[:, 2]
+        Tq = np.asarray(torque[i], dtype=np.float64)
+        # v (z = unit(v)) as unit-vector or bond-vector terms, and u
+        if t == AXIS_Z_ONLY:
+            v_bonds, v_units = [iz], []
+            u = np.array([1.0, 0.0, 0.0]) if abs(z[0]) < 0.866 else np.array([0.0, 1.0, 0.0])
+            u_bonds, u_units = [], []
+        elif t == AXIS_Z_THEN_X:
+            v_bonds, v_units = [iz], []
+            u, u_bonds, u_units = xyz[ix] - xyz[i], [ix], []
+        elif t == AXIS_BISECTOR:
+            v_bonds, v_units = [], [iz, ix]
+            u, u_bonds, u_units = unit(xyz[ix] - xyz[i]), [], [ix]
+        elif t == AXIS_Z_BISECT:
+            v_bonds, v_units = [iz], []
+            u = unit(xyz[ix] - xyz[i]) + unit(xyz[iy] - xyz[i])
+            u_bonds, u_units = [], [ix, iy]
+        else:  # 3-Fold
+            v_bonds, v_units = [], [iz, ix, iy]
+            u, u_bonds, u_units = unit(xyz[ix] - xyz[i]), [], [ix]
+        v = sum((xyz[a] - xyz[i] for a in v_bonds), np.zeros(3)) + \
+            sum((unit(xyz[a] - xyz[i]) for a in v_units), np.zeros(3))
+        uperp = np.linalg.norm(u - np.dot(u, z) * z)
+        tz = np.dot(Tq, z)
+        g_z = np.cross(Tq, z) - tz * np.dot(u, z) / uperp * y
+        g_u = tz / uperp * y
+        g_v = (g_z - z * np.dot(z, g_z)) / np.linalg.norm(v)
+        for a in v_bonds:
+            F[a] += g_v
+            F[i] -= g_v
+        for a in v_units:
+            through_unit(a, i, g_v)
+        for a in u_bonds:
+            F[a] += g_u
+            F[i] -= g_u
+        for a in u_units:
+            through_unit(a, i, g_u)
+    return F

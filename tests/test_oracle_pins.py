@@ -771,3 +771,21 @@ def test_polarization_gradient_matches_differences_of_the_solved_energy():
             return _epol(s, fr, x2)
         fd = _d4(e_at, h)
         assert abs(grad[atom, axis] - fd) < 1e-6 * max(1.0, abs(fd)), (atom, axis, grad[atom, axis], fd)
+
+
+def test_closed_form_torque_forces_match_the_frame_jacobian():
+    # every axis type (with and without chirality atoms), random torques:
+    # the closed form against central differences of rotmat; the forces of
+    # a torque sum to zero and their moment about the origin is the torque
+    xyz, axt, fz, fx, fy, pole = _frame_case()
+    tq = np.random.default_rng(3).normal(size=(len(xyz), 3))
+    a = O.torque_forces_closed(xyz, axt, fz, fx, fy, tq)
+    b = O.torque_forces(xyz, axt, fz, fx, fy, tq)
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-7 * np.abs(b).max())
+    for i in range(len(xyz)):
+        one = np.zeros_like(tq)
+        one[i] = tq[i]
+        f = O.torque_forces_closed(xyz, axt, fz, fx, fy, one)
+        assert np.abs(f.sum(0)).max() < 1e-10 * max(1.0, np.abs(f).max())
+        if axt[i] != O.AXIS_Z_ONLY:   # Z-Only leaves the spin about z to the global axis
+            np.testing.assert_allclose(np.cross(xyz, f).sum(0), tq[i], rtol=0, atol=1e-9 * np.abs(tq[i]).max())
