@@ -53,6 +53,10 @@
  *                                (v1.2/source/initprm.f:447-455) for
  *                                epolar3tcg1/epolar3tcg2
  *                                (v1.2/source/epolar3tcg.f:66-621)
+ *   thp_epolar1 ................ the gradient of epolar1c after the solve:
+ *                                epreal1c, eprecip1, the self torque and
+ *                                torque (v1.2/source/epolar1.f:57-2543;
+ *                                 v1.2/source/torque.f)
  *   thp_set_predictor .......... polpred keyword + ulspred and the dipole
  *                                history (v1.2/source/mdinit.f:93,185-190;
  *                                 v1.2/source/newinduce_pme2.f:165-178,
@@ -173,6 +177,13 @@ thp_status thp_nblist_get(thp_sys* sys, int64_t* npairs, int* ik, int64_t cap);
 
 thp_status thp_efld0_direct(thp_sys* sys, double* ef /* [n][2][3] */);
 thp_status thp_efld0_recip(thp_sys* sys, double* cphi /* [n][10] */);
+/* Polarization energy gradient (kcal/mol/A, the reference's dep) for the
+ * dipoles of the system's last solve, [n][3] in caller order.  Systems with
+ * local frames get the multipole torques carried to the frame atoms; with
+ * global multipoles dep holds the forces at fixed orientations only and
+ * torque (optional, [n][3], kcal/mol) returns the torques on the sites.
+ * Not for short-range solves. */
+thp_status thp_epolar1(thp_sys* sys, double* dep, double* torque);
 thp_status thp_tmatxb(thp_sys* sys, const double* mu, double* efi, int dodiag);
 thp_status thp_tmatxb_recip(thp_sys* sys, const double* mu, double* dipfield);
 /* spread fractional induced dipoles fuind/fuinp [n][3] onto the device grid;

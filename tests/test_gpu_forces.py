@@ -1,0 +1,84 @@
+"""Polarization energy gradient on the device (thp_epolar1: epreal1c,
+eprecip1, the self torque and torque.f, v1.2/source/epolar1.f:57-2543)
+against the oracle's gradient, which central differences of the solved
+energy pin (tests/test_oracle_pins.py)."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import thp
+from oracle import oracle as O
+from test_oracle_pins import _water_frames
+
+pytestmark = pytest.mark.gpu
+FP64, MIXED = thp.THP_PREC_FP64, thp.THP_PREC_MIXED
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = thp.Context(0)
+    yield c
+    c.close()
+
+
+def _box(nmol=27, box=10.0, nfft=12, seed=4):
+    s = thp.tiny_water(nmol, box, nfft, seed=seed)
+    # intramolecular p-scales of 0.5 (d-scales 0): u_d and u_p differ
+    dpu = np.array(s.scaled_dpu, dtype=np.float64).reshape(-1, 3).copy()
+    dpu[:, 1] = 0.5
+    return dataclasses.replace(s, scaled_dpu=dpu)
+
+
+def _device(ctx, s, fr, prec, poleps):
+    if fr is not None:
+        sf = s.with_multipoles(np.zeros((s.n, 13)))
+        sf.frames = dict(axistype=fr[0], zaxis=fr[1], xaxis=fr[2], yaxis=fr[3], pole=fr[4])
+    else:
+        sf = s
+    solver = thp.Solver(ctx, sf, prec)
+    _, _, st = solver.newinduce_pme2(poleps=poleps)
+    assert st.status == 0
+    return solver
+
+
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-9), (MIXED, 1e-5)])
+def test_gradient_with_local_frames_matches_oracle(ctx, prec, tol):
+    s = _box()
+    fr = _water_frames(s)
+    _, grad, _ = O.epolar1(s, fr, poleps=1e-10)
+    solver = _device(ctx, s, fr, prec, 1e-10)
+    dep = solver.epolar1()
+    err = np.abs(dep - grad).max() / np.abs(grad).max()
+    print(f"prec={prec} max|dep|={np.abs(grad).max():.4f} rel={err:.2e}", flush=True)
+    assert err < tol
+    solver.close()
+
+
+def test_forces_and_torques_with_global_multipoles_match_oracle(ctx):
+    s = _box(seed=7)
+    fr = _water_frames(s, seed=2)
+    xyz = np.stack([s.x, s.y, s.z], axis=1)
+    rp = O.rotpole(*xyz.T, *fr)
+    s = s.with_multipoles(rp)
+    mu = O.Oracle(s).induce(poleps=1e-10)["mu"]
+    f1, t1 = O.epolar1_real(s, rp, mu[:, 0], mu[:, 1])
+    f2, t2 = O.epolar1_recip_self(s, rp, mu[:, 0], mu[:, 1])
+    solver = _device(ctx, s, None, FP64, 1e-10)
+    dep, tq = solver.epolar1(with_torque=True)
+    f, t = f1 + f2, t1 + t2
+    assert np.abs(-dep - f).max() < 1e-9 * np.abs(f).max()
+    assert np.abs(tq - t).max() < 1e-9 * np.abs(t).max()
+    solver.close()
+
+
+def test_gradient_on_the_1944_atom_box_matches_oracle(ctx):
+    s = thp.water_1944()
+    fr = _water_frames(s, seed=5)
+    _, grad, _ = O.epolar1(s, fr, poleps=1e-10)
+    solver = _device(ctx, s, fr, FP64, 1e-10)
+    dep = solver.epolar1()
+    err = np.abs(dep - grad).max() / np.abs(grad).max()
+    print(f"water-1944 rel={err:.2e}", flush=True)
+    assert err < 1e-9
+    solver.close()

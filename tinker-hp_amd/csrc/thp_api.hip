@@ -13,6 +13,7 @@
 
 #include "thp_internal.h"
 #include "thp_kernels.h"
+#include "thp_forces.h"
 
 using namespace thp;
 
@@ -1404,6 +1405,50 @@ thp_status thp_efld0_direct(thp_sys* s, double* ef) {
   RET(pair_efld0(s->ctx, s->one, kGateNone));
   s->coef_valid = true;
   return gather_out(s, ef, s->view.ef, 6);
+}
+
+thp_status thp_epolar1(thp_sys* s, double* dep, double* torque) {
+  if (!s || !dep) return THP_ERR_ARG;
+  thp_ctx* ctx = s->ctx;
+  if (s->one.real_only) return fail(ctx, THP_ERR_ARG, "thp_epolar1: short-range solves have no gradient here");
+  RET(ensure_lists(s));
+  RET(ensure_pme(s));
+  const size_t np = s->npad;
+  double* F = nullptr;
+  double* T = nullptr;
+  RET(dalloc(s, &F, 3 * np));
+  thp_status st = dalloc(s, &T, 3 * np);
+  auto done = [&](thp_status r) {
+    dfree(s, F);
+    dfree(s, T);
+    return r;
+  };
+  if (st != THP_OK) return done(st);
+  hipStream_t hs = ctx->stream;
+  const Batch& b = s->one;
+  // the multipole grid's potential derivatives into cphi, then the two
+  // induced grids of the solved dipoles
+  if ((st = field(ctx, b, s->cphi, 0, 0, kGateNone)) != THP_OK) return done(st);
+  if ((st = spread_uind(ctx, b, V_MU, 0, kGateNone)) != THP_OK) return done(st);
+  if ((st = convolve(ctx, b, kGateNone)) != THP_OK) return done(st);
+  hipLaunchKernelGGL(k_ep1_real, g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, F, T);
+  if ((st = launch_check(ctx, "k_ep1_real")) != THP_OK) return done(st);
+  if (s->prec == THP_PREC_FP64)
+    hipLaunchKernelGGL((k_ep1_recip<double>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T);
+  else
+    hipLaunchKernelGGL((k_ep1_recip<float>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T);
+  if ((st = launch_check(ctx, "k_ep1_recip")) != THP_OK) return done(st);
+  const dim3 ga((s->n + 255) / 256);
+  if (s->has_frames) {
+    hipLaunchKernelGGL(k_ep1_torque, ga, dim3(256), 0, hs, s->d_view, s->d_frames, T, F);
+    if ((st = launch_check(ctx, "k_ep1_torque")) != THP_OK) return done(st);
+  }
+  hipLaunchKernelGGL(k_ep1_finish, ga, dim3(256), 0, hs, s->n, np, F, T);
+  if ((st = launch_check(ctx, "k_ep1_finish")) != THP_OK) return done(st);
+  if ((st = gather_out(s, dep, F, 3)) != THP_OK) return done(st);
+  for (size_t q = 0; q < (size_t)s->n * 3; ++q) dep[q] = -dep[q];  // gradient = -force
+  if (torque && (st = gather_out(s, torque, T, 3)) != THP_OK) return done(st);
+  return done(THP_OK);
 }
 
 thp_status thp_efld0_recip(thp_sys* s, double* cphi) {

@@ -1185,6 +1185,26 @@ __device__ __forceinline__ int bspline_axis(double xc, double rc, int nfft, real
 #pragma unroll
     for (int m = 1; m <= kOrder; ++m) th[2][m - 1] = a[m];
   }
+  if constexpr (LEVELS >= 4) {
+    // third derivative: the order-2 values differenced three times
+    real a[kOrder + 1];
+#pragma unroll
+    for (int m = 1; m <= kOrder - 3; ++m) a[m] = t[kOrder - 3][m];
+    a[kOrder - 2] = a[kOrder - 3];
+#pragma unroll
+    for (int m = kOrder - 3; m >= 2; --m) a[m] = a[m - 1] - a[m];
+    a[1] = -a[1];
+    a[kOrder - 1] = a[kOrder - 2];
+#pragma unroll
+    for (int m = kOrder - 2; m >= 2; --m) a[m] = a[m - 1] - a[m];
+    a[1] = -a[1];
+    a[kOrder] = a[kOrder - 1];
+#pragma unroll
+    for (int m = kOrder - 1; m >= 2; --m) a[m] = a[m - 1] - a[m];
+    a[1] = -a[1];
+#pragma unroll
+    for (int m = 1; m <= kOrder; ++m) th[3][m - 1] = a[m];
+  }
   int g0 = ifr - 4;
   if (g0 < 0) g0 += nfft;
   return g0;

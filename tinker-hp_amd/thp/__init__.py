@@ -131,6 +131,7 @@ def load_lib(path: str | None = None) -> C.CDLL:
         "thp_nblist_get": (C.c_int, [vp, C.POINTER(C.c_int64), _ip, C.c_int64]),
         "thp_efld0_direct": (C.c_int, [vp, _dp]),
         "thp_efld0_recip": (C.c_int, [vp, _dp]),
+        "thp_epolar1": (C.c_int, [vp, _dp, _dp]),
         "thp_tmatxb": (C.c_int, [vp, _dp, _dp, C.c_int]),
         "thp_tmatxb_recip": (C.c_int, [vp, _dp, _dp]),
         "thp_grid_uind": (C.c_int, [vp, _dp, _dp, _dp]),
@@ -395,6 +396,15 @@ class Solver:
         cphi = np.zeros((self.sys.n, 10))
         self._c(self.lib.thp_efld0_recip(self.h, _d(cphi)), "efld0_recip")
         return cphi
+
+    def epolar1(self, with_torque: bool = False):
+        """Polarization energy gradient dep (n, 3), kcal/mol/A, for the
+        dipoles of the last solve (epolar1c); with_torque also returns the
+        site torques (n, 3), which dep leaves out for global multipoles."""
+        dep = np.zeros((self.sys.n, 3))
+        tq = np.zeros((self.sys.n, 3)) if with_torque else None
+        self._c(self.lib.thp_epolar1(self.h, _d(dep), _d(tq) if tq is not None else None), "epolar1")
+        return (dep, tq) if with_torque else dep
 
     def tmatxb_pme(self, mu, dodiag=True) -> np.ndarray:
         mu = np.ascontiguousarray(mu, dtype=np.float64).reshape(self.sys.n, 2, 3)
