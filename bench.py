@@ -199,6 +199,14 @@ def main():
         dsolver, t_d, r_d = time_leg(ctx, thp, ds, thp.THP_PREC_MIXED, args.steps, args.warmup, args.poleps)
         dhfr2 = dict(solves_per_s=round(1.0 / t_d, 3), ms_per_solve=round(1e3 * t_d, 4), niter=r_d[2].niter,
                      ep=r_d[1], atoms=ds.n, frames="local (rotated on the device)", aewald=round(ds.aewald, 6))
+        # the polarization gradient of the last solve (forces with the
+        # torques carried to the frame atoms)
+        dsolver.epolar1()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            dep = dsolver.epolar1()
+        dhfr2["gradient_ms"] = round(1e3 * (time.perf_counter() - t0) / 5, 3)
+        dhfr2["gradient_max_abs"] = round(float(abs(dep).max()), 6)
         dsolver.close()
         log(f"bench: dhfr2 {dhfr2}")
 

@@ -82,3 +82,22 @@ def test_gradient_on_the_1944_atom_box_matches_oracle(ctx):
     print(f"water-1944 rel={err:.2e}", flush=True)
     assert err < 1e-9
     solver.close()
+
+
+def test_fortran_host_gradient_matches_python_binding(ctx):
+    # the Fortran host (polar_gradient over ISO_C_BINDING) and the Python
+    # binding reach the same entry point on the same 1,944-atom box
+    import os
+    import re
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "tinker-hp_amd", "fortran", "polar_driver")
+    out = subprocess.run([exe, "1944", "18.643", "24", "1", "0"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    vals = dict(re.findall(r"(\w+)=\s*(\S+)", out.stdout))
+    solver = thp.Solver(ctx, thp.water_1944(), FP64)
+    solver.newinduce_pme2(poleps=1e-8)
+    dep = solver.epolar1()
+    rms = float(np.sqrt(np.mean(dep ** 2)))
+    assert abs(float(vals["GRADRMS"]) - rms) < 1e-12 * rms
+    solver.close()

@@ -1411,6 +1411,9 @@ thp_status thp_epolar1(thp_sys* s, double* dep, double* torque) {
   if (!s || !dep) return THP_ERR_ARG;
   thp_ctx* ctx = s->ctx;
   if (s->one.real_only) return fail(ctx, THP_ERR_ARG, "thp_epolar1: short-range solves have no gradient here");
+  // truncated-CG dipoles are not stationary: their gradient (epolar1tcg)
+  // has terms of its own
+  if (s->polalg == THP_POLALG_TCG) return fail(ctx, THP_ERR_ARG, "thp_epolar1: no gradient for truncated-CG solves");
   RET(ensure_lists(s));
   RET(ensure_pme(s));
   const size_t np = s->npad;

@@ -14,6 +14,7 @@ program polar_driver
   real(c_double) :: box, radius
   real(c_double), allocatable, target :: x(:), y(:), z(:), rpole(:,:), pol(:), thl(:), dpu(:,:)
   integer(c_int), allocatable, target :: ik(:,:)
+  real(8), allocatable :: dep(:,:)
   integer(c_int) :: nscaled
   character(len=64) :: arg
   real(c_double), parameter :: debye = 4.80321d0
@@ -47,5 +48,10 @@ program polar_driver
   murms = sqrt(sum(uind**2) / (3.0d0 * natom)) * debye
   write (*, '(a,es24.16,a,i0,a,es12.4,a,es14.6,a,i0)') ' EP=', ep, ' NITER=', niter_last, &
     ' RESNRM=', resnrm_last, ' MURMS_D=', murms, ' NPAIRS=', nblist_npairs()
+  ! the polarization gradient of that solve (global multipoles: forces at
+  ! fixed orientations)
+  allocate (dep(3, natom))
+  call polar_gradient(dep)
+  write (*, '(a,es24.16)') ' GRADRMS=', sqrt(sum(dep**2) / (3.0d0 * natom))
   call polar_finalize
 end program polar_driver

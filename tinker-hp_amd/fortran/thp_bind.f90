@@ -83,6 +83,12 @@ module thp_bind
       type(c_ptr), value :: sys
       real(c_double), intent(out) :: cphi(*)
     end function
+    integer(c_int) function thp_epolar1(sys, dep, torque) bind(C, name="thp_epolar1")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: sys
+      real(c_double), intent(out) :: dep(*)
+      type(c_ptr), value :: torque
+    end function
     integer(c_int) function thp_tmatxb(sys, mu, efi, dodiag) bind(C, name="thp_tmatxb")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: sys

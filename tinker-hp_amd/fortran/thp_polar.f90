@@ -183,6 +183,14 @@ contains
 
   ! use_polarshortreal with ewaldshortcut (cutoffs.f:59): the short-range
   ! solves of RESPA1's inner steps; 0 restores the full solve
+  ! the polarization gradient of the last solve, dep(3,n) in kcal/mol/A
+  ! (epolar1c's dep, epolar1.f:57-376): forces with the multipole torques
+  ! carried to the frame atoms when the system has local frames
+  subroutine polar_gradient(dep)
+    real(8), intent(out) :: dep(:,:)
+    call fatal_status(thp_epolar1(sys, dep, c_null_ptr), 'thp_epolar1')
+  end subroutine
+
   subroutine polar_set_shortreal(shortcut)
     real(8), intent(in) :: shortcut
     call fatal_status(thp_set_shortreal(sys, real(shortcut, c_double)), 'thp_set_shortreal')
