@@ -1434,7 +1434,7 @@ thp_status thp_epolar1(thp_sys* s, double* dep, double* torque) {
   if ((st = field(ctx, b, s->cphi, 0, 0, kGateNone)) != THP_OK) return done(st);
   if ((st = spread_uind(ctx, b, V_MU, 0, kGateNone)) != THP_OK) return done(st);
   if ((st = convolve(ctx, b, kGateNone)) != THP_OK) return done(st);
-  hipLaunchKernelGGL(k_ep1_real, g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, F, T);
+  hipLaunchKernelGGL(k_ep1_real<4>, g1(b.n_blk), dim3(256), 0, hs, b.d_views, b.d_blk, F, T);
   if ((st = launch_check(ctx, "k_ep1_real")) != THP_OK) return done(st);
   if (s->prec == THP_PREC_FP64)
     hipLaunchKernelGGL((k_ep1_recip<double>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T);

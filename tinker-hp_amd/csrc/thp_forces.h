@@ -151,24 +151,30 @@ __device__ __forceinline__ void ep1_pair(const Site& si, const Site& sk, V3 R, c
   F = F - w_ddphi_dip(si.up, sk.ud, R, Gu) - w_ddphi_dip(si.ud, sk.up, R, Gu);
 }
 
-// Real-space part, one thread per atom: the unscaled partners of the
-// block-union lists within the cutoff, then the scaled pairs with their d,
-// p, u factors.  FP64 throughout (one launch per gradient).
-__global__ void __launch_bounds__(64) k_ep1_real(const SysView* views, const Work* work, double* F, double* T) {
+// Real-space part: a workgroup of S wavefronts per 64-atom block, each
+// wavefront walking one contiguous segment of every lane's block-union
+// partners (within the cutoff) and every S-th of its scaled pairs (with
+// their d, p, u factors); the S partial forces and torques are summed in
+// LDS in a fixed order.  FP64 throughout (one launch per gradient).
+template <int S>
+__global__ void __launch_bounds__(64 * S) k_ep1_real(const SysView* views, const Work* work, double* F, double* T) {
+  __shared__ double red[S][6][64];
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
-  const int i = w.blk * kBlock + threadIdx.x;
-  if (i >= sv.n) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = w.blk * kBlock + lane;
+  const bool ok = i < sv.n;
   const size_t np = sv.npad;
+  V3 f = v3(0, 0, 0), t = v3(0, 0, 0);
+  if (ok) {
   const Site si = load_site(sv, i);
   const double xi = sv.x[i], yi = sv.y[i], zi = sv.z[i];
   const double pdi = sv.pdamp[i], thi = sv.thole[i];
-  V3 f = v3(0, 0, 0), t = v3(0, 0, 0);
   const int cnt = sv.lcount[i];
   const int* UL = sv.ulist + (size_t)w.blk * sv.umax;
   const double one[1] = {1.0};
-  for (int s = 0; s < cnt; ++s) {
+  for (int s = (cnt * wv) / S; s < (cnt * (wv + 1)) / S; ++s) {
     const int k = UL[sv.llist[(size_t)s * np + i]];
     const double xr = image_exact(xi - sv.x[k], sv.box[0]);
     const double yr = image_exact(yi - sv.y[k], sv.box[1]);
@@ -179,7 +185,7 @@ __global__ void __launch_bounds__(64) k_ep1_real(const SysView* views, const Wor
     ep1_ladders(r2, sv.aewald, pdi, sv.pdamp[k], thi, sv.thole[k], one, 1, G);
     ep1_pair(si, load_site(sv, k), v3(xr, yr, zr), G[0], G[0], G[0], f, t);
   }
-  for (int e = sv.excl_off[i]; e < sv.excl_off[i + 1]; ++e) {
+  for (int e = sv.excl_off[i] + wv; e < sv.excl_off[i + 1]; e += S) {
     const int k = sv.excl_k[e];
     const double xr = image_exact(xi - sv.x[k], sv.box[0]);
     const double yr = image_exact(yi - sv.y[k], sv.box[1]);
@@ -191,12 +197,28 @@ __global__ void __launch_bounds__(64) k_ep1_real(const SysView* views, const Wor
     ep1_ladders(r2, sv.aewald, pdi, sv.pdamp[k], thi, sv.thole[k], sc, 3, G);
     ep1_pair(si, load_site(sv, k), v3(xr, yr, zr), G[0], G[1], G[2], f, t);
   }
-  F[i] = f.x;
-  F[np + i] = f.y;
-  F[2 * np + i] = f.z;
-  T[i] = t.x;
-  T[np + i] = t.y;
-  T[2 * np + i] = t.z;
+  }
+  red[wv][0][lane] = f.x;
+  red[wv][1][lane] = f.y;
+  red[wv][2][lane] = f.z;
+  red[wv][3][lane] = t.x;
+  red[wv][4][lane] = t.y;
+  red[wv][5][lane] = t.z;
+  __syncthreads();
+  if (wv == 0 && ok) {
+    double v[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      v[c] = red[0][c][lane];
+      for (int q = 1; q < S; ++q) v[c] += red[q][c][lane];
+    }
+    F[i] = v[0];
+    F[np + i] = v[1];
+    F[2 * np + i] = v[2];
+    T[i] = v[3];
+    T[np + i] = v[4];
+    T[2 * np + i] = v[5];
+  }
 }
 
 // Reciprocal and self parts, one thread per atom, added to F and T.  The
