@@ -103,6 +103,21 @@ void dampthole(double pdi, double pdk, double pgamma, double r, int rorder, doub
   }
 }
 
+// Radial factor of a scaled pair's damped interaction of rank n:
+// dmp_n = bn_n - (1 - s lambda_n) rr_n, with bn_n the Ewald real-space
+// factor, lambda_n the Thole factor, rr_n the bare factor and s the pair's
+// d-, p- or u-scale (v1.2/source/newinduce_pme.f:1197-1230 for d and p,
+// :1564-1574 for u; GPU/source/kscalfactor.f:925-949 stores 1 - s).
+// g_scaled_form selects a planted mistake for the tests that show the pins
+// of this form catch one (1: bn scaled by s as well; 2: (1 - s) lambda in
+// place of 1 - s lambda); it is 0, the reference's form, everywhere else.
+int g_scaled_form = 0;
+inline double scaled_dmp(double bn, double lam, double rr, double s) {
+  if (g_scaled_form == 1) return s * bn - (1.0 - s * lam) * rr;
+  if (g_scaled_form == 2) return bn - (1.0 - (1.0 - s) * lam) * rr;
+  return bn - (1.0 - s * lam) * rr;
+}
+
 inline double thlval(const orc_sys* s, int i, int k) {
   // kpolar.f:280-293: min, unless zero then max
   double thl = std::min(s->thole[i], s->thole[k]);
@@ -768,9 +783,9 @@ void efld0_direct(orc_sys* s, double* ef) {
         double fid[3], fkd[3], fip[3], fkp[3];
         for (int pass = 0; pass < 2; ++pass) {
           const double scalek = pass == 0 ? dsc : psc;
-          const double dmp3 = dmpe[2] - (1.0 - scalek * dmpik[2]) * rr3;
-          const double dmp5 = dmpe[4] - (1.0 - scalek * dmpik[4]) * rr5;
-          const double dmp7 = dmpe[6] - (1.0 - scalek * dmpik[6]) * rr7;
+          const double dmp3 = scaled_dmp(dmpe[2], dmpik[2], rr3, scalek);
+          const double dmp5 = scaled_dmp(dmpe[4], dmpik[4], rr5, scalek);
+          const double dmp7 = scaled_dmp(dmpe[6], dmpik[6], rr7, scalek);
           double* fi = pass == 0 ? fid : fip;
           double* fk = pass == 0 ? fkd : fkp;
           const double ai = dmp3 * ck - dmp5 * dkr + dmp7 * qkr;
@@ -834,8 +849,8 @@ void tmatxb(orc_sys* s, const double* mu, double* efi, int dodiag) {
         dampthole(s->pdamp[i], s->pdamp[k], thlval(s, i, k), r, 5, dmpik);
         double dsc, psc, usc;
         pair_scales(s, i, k, dsc, psc, usc);
-        const double dmp3 = dmpe[2] - (1.0 - usc * dmpik[2]) * rr3;
-        const double dmp5 = dmpe[4] - (1.0 - usc * dmpik[4]) * rr5;
+        const double dmp3 = scaled_dmp(dmpe[2], dmpik[2], rr3, usc);
+        const double dmp5 = scaled_dmp(dmpe[4], dmpik[4], rr5, usc);
         const double d1 = -dmp3 + dmp5 * xr * xr, d2 = dmp5 * xr * yr, d3 = dmp5 * xr * zr;
         const double d4 = -dmp3 + dmp5 * yr * yr, d5 = dmp5 * yr * zr, d6 = -dmp3 + dmp5 * zr * zr;
         for (int rhs = 0; rhs < 2; ++rhs) {
@@ -1095,10 +1110,9 @@ void orc_epolar_decomp(orc_sys* s, const double* uind, double* ereal, double* er
       const double rr3 = f / (r * r2);
       const double rr5 = 3.0 * rr3 / r2;
       const double rr7 = 5.0 * rr5 / r2;
-      double sr3 = psc * dmpik[2] * rr3, sr5 = psc * dmpik[4] * rr5, sr7 = psc * dmpik[6] * rr7;
-      sr3 = dmpe[2] - rr3 + sr3;
-      sr5 = dmpe[4] - rr5 + sr5;
-      sr7 = dmpe[6] - rr7 + sr7;
+      const double sr3 = scaled_dmp(dmpe[2], dmpik[2], rr3, psc);
+      const double sr5 = scaled_dmp(dmpe[4], dmpik[4], rr5, psc);
+      const double sr7 = scaled_dmp(dmpe[6], dmpik[6], rr7, psc);
       const double term1 = ck * uir - ci * ukr + diu + dku;
       const double term2 = 2.0 * (qiu - qku) - uir * dkr - dir * ukr;
       const double term3 = uir * qkr - ukr * qir;
@@ -1142,5 +1156,7 @@ void orc_damptholed(double pdi, double pdk, double pgamma, double r, int rorder,
 }
 
 double orc_erfc(double x) { return std::erfc(x); }
+// test hook: plant a mistake in the scaled-pair form (0 restores it)
+void orc_plant_scaled_form(int form) { g_scaled_form = form; }
 
 }  // extern "C"

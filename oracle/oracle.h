@@ -83,6 +83,8 @@ void orc_fft3d(int n1, int n2, int n3, double* data /* interleaved, x fastest */
 void orc_dampewald(double aewald, double r, double* dmpe /* [7] index 0..6 = orders 1..7 */);
 void orc_damptholed(double pdi, double pdk, double pgamma, double r, int rorder, double* dmpik);
 double orc_erfc(double x);
+/* test hook: 0 = the reference's scaled-pair form; 1, 2 = planted mistakes */
+void orc_plant_scaled_form(int form);
 
 #ifdef __cplusplus
 }

@@ -48,6 +48,7 @@ def load():
         "orc_dampewald": (None, [C.c_double, C.c_double, _dp]),
         "orc_damptholed": (None, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp]),
         "orc_erfc": (C.c_double, [C.c_double]),
+        "orc_plant_scaled_form": (None, [C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

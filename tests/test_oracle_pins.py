@@ -207,6 +207,167 @@ def test_total_dipole_matvec_is_alpha_invariant():
         assert np.abs(o - outs[0]).max() < 1e-5 * scale
 
 
+# ------------------------------------------ scaled (excluded) pairs
+# A scaled pair enters the direct sum as dmp_n = bn_n - (1 - s lambda_n) rr_n
+# (newinduce_pme.f:1197-1230 for the d and p scales, :1564-1574 for u;
+# kscalfactor.f:925-949), while the reciprocal sum and the self term carry
+# every pair in full.  Two consequences that no line of the oracle states:
+# (1) the total is independent of the Ewald coefficient for any s, and
+# (2) the nearest image of a scaled pair adds s times its Thole-damped bare
+# interaction to the field of all other images.  A one-water box, its field
+# compared with a brute-force lattice sum over the images, checks (2); a
+# water box with d, p, u scales below 1 checks (1) at three coefficients.
+
+def _bare_field(R, q, d, Q, lam=(1.0, 1.0, 1.0)):
+    """field at R = x_i - x_k of point multipoles at x_k (textbook; rpole's
+    convention phi = q/r + d.R/r^3 + 3 R.Q.R/r^5), each rank's radial factor
+    times the Thole factor lam = (lambda3, lambda5, lambda7)"""
+    r2 = (R * R).sum(-1)
+    r = np.sqrt(r2)
+    rr3 = lam[0] / (r * r2)
+    rr5 = lam[1] * 3.0 / (r * r2 * r2)
+    rr7 = lam[2] * 15.0 / (r * r2 * r2 * r2)
+    dR = (d * R).sum(-1)
+    QR = np.einsum("...ab,...b->...a", Q, R)
+    RQR = (QR * R).sum(-1)
+    return (q * rr3 + dR * rr5 + RQR * rr7)[..., None] * R - rr3[..., None] * d - 2.0 * rr5[..., None] * QR
+
+
+def _image_field(pos, rpole, L, nshell=10):
+    """field at each site of every periodic image (cells n != 0) of all sites
+    in a cubic box, summed over a sphere of cells, plus the tin-foil surface
+    term 4 pi M / 3V that the Ewald sum implies (M = the cell's dipole)"""
+    q, d, Q = E.rpole_to_parts(rpole)
+    g = np.arange(-nshell, nshell + 1)
+    cells = np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1).T
+    cells = cells[((cells * cells).sum(1) <= nshell * nshell) & np.any(cells != 0, axis=1)]
+    out = np.zeros((len(pos), 3))
+    for i in range(len(pos)):
+        for k in range(len(pos)):
+            R = pos[i] - pos[k] - cells * L
+            out[i] += _bare_field(R, q[k], d[k], Q[k]).sum(0)
+    M = (q[:, None] * pos).sum(0) + d.sum(0)
+    return out + 4.0 * math.pi / (3.0 * L ** 3) * M
+
+
+def _one_water(scales):
+    """one water in a 24 A box (cutoff 9.9 A, 64^3 grid); its three pairs
+    carry the given (d, p, u) scales"""
+    s = thp.make_system(3, 24.0, seed=3, nfft=(64, 64, 64), cut=9.9, buffer=0.05)
+    s.scaled_dpu = np.tile(np.asarray(scales, dtype=np.float64), (len(s.scaled_ik), 1))
+    return s
+
+
+def _home_field(s, rpole, order):
+    """Thole-damped bare field of the other sites of the molecule (nearest
+    image) at each site; lambda from the pinned damptholed"""
+    pos = np.stack([s.x, s.y, s.z], 1)
+    q, d, Q = E.rpole_to_parts(rpole)
+    pd = s.polarity ** (1.0 / 6.0)
+    out = np.zeros((s.n, 3))
+    for i in range(s.n):
+        for k in range(s.n):
+            if k == i:
+                continue
+            R = pos[i] - pos[k]
+            lam = O.dampthole(pd[i], pd[k], min(s.thole[i], s.thole[k]), np.linalg.norm(R), order)
+            out[i] += _bare_field(R[None], q[k], d[k][None], Q[k][None], (lam[2], lam[4], lam[6] if order > 5 else 1.0))[0]
+    return out
+
+
+def _isolated_field_error(dsc, psc):
+    """max |oracle total permanent field - (images + s * damped home field)|
+    over both right-hand sides, and the home field's scale"""
+    s = _one_water((dsc, psc, 1.0))
+    pos = np.stack([s.x, s.y, s.z], 1)
+    img = _image_field(pos, s.rpole, s.box[0])
+    home = _home_field(s, s.rpole, 7)
+    ef = O.Oracle(s).efld0_total()
+    err = max(np.abs(ef[:, 0] - (img + dsc * home)).max(), np.abs(ef[:, 1] - (img + psc * home)).max())
+    return err, np.abs(home).max()
+
+
+def _isolated_matvec_error(usc):
+    """the same for the dipole field of the matvec: mu/alpha - h equals the
+    images' field plus u times the damped home field"""
+    s = _one_water((0.0, 0.0, usc))
+    pos = np.stack([s.x, s.y, s.z], 1)
+    mu = np.random.default_rng(5).normal(size=(s.n, 2, 3)) * 0.2
+    h = O.Oracle(s).matvec(mu)
+    err = 0.0
+    for rhs in range(2):
+        rp = np.zeros((s.n, 13))
+        rp[:, 1:4] = mu[:, rhs]
+        want = _image_field(pos, rp, s.box[0]) + usc * _home_field(s, rp, 5)
+        err = max(err, np.abs(mu[:, rhs] / s.polarity[:, None] - h[:, rhs] - want).max())
+    return err
+
+
+def _scaled_alpha_spread():
+    """spread of the total permanent field and of the full matvec over three
+    Ewald coefficients, for 27 waters (cutoff 9.9 A, 64^3 grid) whose
+    intramolecular pairs carry d = 0, p = 0.5, u = 0.7; relative to scale"""
+    s = thp.tiny_water(27, 20.3, 64, cut=9.9, buffer=0.05)
+    s.scaled_dpu = np.tile([0.0, 0.5, 0.7], (len(s.scaled_ik), 1))
+    mu = np.random.default_rng(4).normal(size=(s.n, 2, 3)) * 0.3
+    f = [O.Oracle(s, aewald=a).efld0_total() for a in (0.40, 0.45, 0.50)]
+    m = [O.Oracle(s, aewald=a).matvec(mu) for a in (0.40, 0.45, 0.50)]
+    fs = max(np.abs(x - f[0]).max() for x in f[1:]) / np.abs(f[0]).max()
+    ms = max(np.abs(x - m[0]).max() for x in m[1:]) / np.abs(m[0]).max()
+    return fs, ms
+
+
+# bars: PME interpolation error of the home pairs' smooth part at 0.375 A
+# grid spacing (observed 2.4e-6 field, 6e-7 matvec, 5e-5 and 2e-6 spreads)
+ISO_TOL = 1e-5
+ALPHA_FIELD_TOL, ALPHA_MATVEC_TOL = 1.5e-4, 5e-6
+
+
+def test_isolated_molecule_field_is_the_lattice_sum_of_its_images():
+    # d = p = 0: the molecule's own pairs drop out of the total field
+    err, scale = _isolated_field_error(0.0, 0.0)
+    assert err < ISO_TOL and scale > 0.1, (err, scale)
+
+
+def test_isolated_molecule_scaled_field_adds_the_scaled_thole_field():
+    # d = 0.3 and p = 0.6 on the same pairs: each right-hand side adds its
+    # own scale times the damped bare field
+    err, _ = _isolated_field_error(0.3, 0.6)
+    assert err < ISO_TOL, err
+
+
+@pytest.mark.parametrize("usc", [0.0, 0.7])
+def test_isolated_molecule_matvec_adds_the_u_scaled_dipole_field(usc):
+    err = _isolated_matvec_error(usc)
+    assert err < ISO_TOL, err
+
+
+def test_total_field_and_matvec_are_alpha_invariant_with_scaled_pairs():
+    fs, ms = _scaled_alpha_spread()
+    assert fs < ALPHA_FIELD_TOL and ms < ALPHA_MATVEC_TOL, (fs, ms)
+
+
+@pytest.mark.parametrize("form", [1, 2])
+def test_scaled_pair_pins_catch_planted_mistakes(form):
+    # form 1 scales the Ewald part by s as well; form 2 uses (1 - s) lambda
+    # in place of 1 - s lambda.  Each must break one of the checks above by
+    # a wide margin (the mistake is planted in the oracle, on the CPU).
+    lib = O.load()
+    lib.orc_plant_scaled_form(form)
+    try:
+        miss_iso = max(_isolated_field_error(0.0, 0.0)[0], _isolated_field_error(0.3, 0.6)[0],
+                       _isolated_matvec_error(0.0), _isolated_matvec_error(0.7)) / ISO_TOL
+        fs, ms = _scaled_alpha_spread()
+        miss_alpha = max(fs / ALPHA_FIELD_TOL, ms / ALPHA_MATVEC_TOL)
+    finally:
+        lib.orc_plant_scaled_form(0)
+    assert max(miss_iso, miss_alpha) > 30, (miss_iso, miss_alpha)
+    if form == 1:
+        assert miss_alpha > 30, miss_alpha   # not alpha-invariant
+    else:
+        assert miss_iso > 30, miss_iso       # alpha-invariant, wrong home pairs
+
+
 # ----------------------------------------------------- solver and energy
 def test_pcg_solution_matches_dense_solve():
     s = thp.tiny_water(27, 20.3, 24)
