@@ -175,6 +175,16 @@ thp_status thp_nblist(thp_sys* sys, int force_rebuild);
  * query the count.  cap = rows available in ik. */
 thp_status thp_nblist_get(thp_sys* sys, int64_t* npairs, int* ik, int64_t cap);
 
+/* The pair set the solver uses for the current coordinates, as (i,k) rows
+ * with i<k in caller indices, sorted: the in-cutoff entries of the
+ * compacted lists that the permanent-field pass writes and the matvec
+ * walks, plus the scaled pairs within the cutoff.  It replaces the list the
+ * reference's field and matvec loops walk with their r^2 <= off2 test
+ * (v1.2/source/newinduce_pme.f:1160, 1548-1608 over nblist.f:17-247; the
+ * GPU's nelstc prefix after reorder_nblist, GPU/source/nblistgpu.f90:101).
+ * THP_ERR_LIST if an entry is held by only one of its two atoms.  ik=NULL
+ * queries the count; cap = rows available in ik. */
+thp_status thp_pairs_in_cut(thp_sys* sys, int64_t* npairs, int* ik, int64_t cap);
 thp_status thp_efld0_direct(thp_sys* sys, double* ef /* [n][2][3] */);
 thp_status thp_efld0_recip(thp_sys* sys, double* cphi /* [n][10] */);
 /* Polarization energy gradient (kcal/mol/A, the reference's dep) for the
@@ -182,7 +192,10 @@ thp_status thp_efld0_recip(thp_sys* sys, double* cphi /* [n][10] */);
  * local frames get the multipole torques carried to the frame atoms; with
  * global multipoles dep holds the forces at fixed orientations only and
  * torque (optional, [n][3], kcal/mol) returns the torques on the sites.
- * Not for short-range solves. */
+ * Not for short-range or truncated-CG solves.  THP_ERR_STATE unless the
+ * dipoles are a solve of the current coordinates and multipoles (after
+ * thp_system_update_coords, _update_multipoles or _set_frames, solve
+ * again first). */
 thp_status thp_epolar1(thp_sys* sys, double* dep, double* torque);
 thp_status thp_tmatxb(thp_sys* sys, const double* mu, double* efi, int dodiag);
 thp_status thp_tmatxb_recip(thp_sys* sys, const double* mu, double* dipfield);
@@ -228,12 +241,48 @@ thp_status thp_tmatxb_dev(thp_sys* sys, const double* mu, double* efi, int dodia
 thp_status thp_induce_dev(thp_sys* sys, const thp_solve_opts* opts, double* mu_dev /* [n][2][3] */,
                           double* ep, thp_solve_stats* stats);
 
+/* Device-pointer variants ordered on the caller's stream, the analogue of
+ * the reference's native layer, which takes device arrays (from
+ * `!$acc host_data use_device`) and a cudaStream_t by value
+ * (GPU/source/MOD_interfaces.f:577-627: cu_tmatxb_pme, cu_efld0_direct;
+ * GPU/source/newinduce_pmegpu.f:1091,1352 for the device-resident
+ * efld0_recipgpu and tmatxbrecipgpu).  Arrays are device addresses in the
+ * layouts of the host entry points; `stream` is a hipStream_t.  With a
+ * stream, the call's work runs after what the caller queued on it, the
+ * stream waits for the call's results, and the host is not blocked (only a
+ * list or B-spline rebuild after atoms moved waits for the device).  With
+ * stream = NULL the call returns when the results are in place.
+ * Replace (same order): efld0_direct (v1.2/source/newinduce_pme.f:948-1389;
+ * cu_efld0_direct, GPU/source/MOD_interfaces.f:610-636), efld0_recip
+ * (newinduce_pme.f:1671-1917; efld0_recipgpu, GPU/source/newinduce_pmegpu.f:1091),
+ * tmatxb_pme (newinduce_pme.f:1392-1668; cu_tmatxb_pme,
+ * GPU/source/MOD_interfaces.f:584-606), tmatxbrecip (newinduce_pme.f:1919-2128;
+ * tmatxbrecipgpu, GPU/source/newinduce_pmegpu.f:1352), grid_uind_site
+ * (v1.2/source/pmestuff.f:280-400; grid_uind_site_p,
+ * GPU/source/MOD_interfaces.f:224-227) and fphi_uind_site
+ * (pmestuff.f:746-1037; fphi_uind_site2_p, MOD_interfaces.f:416-419). */
+thp_status thp_efld0_direct_dev(thp_sys* sys, double* ef /* [n][2][3] */, void* stream);
+thp_status thp_efld0_recip_dev(thp_sys* sys, double* cphi /* [n][10] */, void* stream);
+thp_status thp_tmatxb_dev_async(thp_sys* sys, const double* mu, double* efi, int dodiag, void* stream);
+thp_status thp_tmatxb_recip_dev(thp_sys* sys, const double* mu, double* dipfield, void* stream);
+/* qgrid (optional): the spread grid as [n3][n2][n1][2] doubles */
+thp_status thp_grid_uind_dev(thp_sys* sys, const double* fuind /* [n][3] */, const double* fuinp /* [n][3] */,
+                             double* qgrid, void* stream);
+/* qgrid NULL: the device grid (of the last spreading) */
+thp_status thp_fphi_uind_dev(thp_sys* sys, const double* qgrid, double* fdip_phi1 /* [n][3] */,
+                             double* fdip_phi2 /* [n][3] */, void* stream);
+
 /* ------------------------------------------------------------ measurement */
 /* Measurement hook, no reference counterpart: average device time (ms) of
  * one launch of a path kernel over `reps` back-to-back launches, timed with
  * HIP events on the library's stream.  kernel: 0 = real-space matvec
  * (tmatxb), 1 = induced-dipole spreading, 2 = induced-dipole gather,
- * 3 = direct permanent-field pass (efld0 with the matvec's lists).  Also
+ * 3 = direct permanent-field pass (efld0 with the matvec's lists),
+ * 4 = rebuild of the solver's lists (block candidates, block-union lists,
+ * host round trips for their counts, and the permanent-field pass that
+ * compacts them), 5 = rebuild of the Verlet pair list (thp_nblist),
+ * 6 = the grid convolution (forward 3-D FFT, scalar sum, backward FFT).
+ * Kernels 4 and 5 take one system.  Also
  * returns the number of unscaled pairs within the cutoff (i<k), the unit
  * of the matvec's algorithmic FLOP count. */
 thp_status thp_time_kernel(thp_sys* sys, int kernel, int reps, double* ms_per_launch, int64_t* pairs_in_cut);

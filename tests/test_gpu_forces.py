@@ -55,6 +55,54 @@ def test_gradient_with_local_frames_matches_oracle(ctx, prec, tol):
     solver.close()
 
 
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-9), (MIXED, 1e-5)])
+def test_gradient_with_every_axis_type_and_chiral_sites_matches_oracle(ctx, prec, tol):
+    # Z-Only, Z-Bisect, 3-Fold and chiral Z-then-X sites besides the water
+    # frames: every branch of the device's torque distribution against the
+    # oracle's closed form, which the frame Jacobian pins
+    from test_gpu_frames import _frames
+    s = _box(seed=6)
+    fd = _frames(s, seed=9)
+    fr = (fd["axistype"], fd["zaxis"], fd["xaxis"], fd["yaxis"], fd["pole"])
+    _, grad, _ = O.epolar1(s, fr, poleps=1e-10)
+    solver = _device(ctx, s, fr, prec, 1e-10)
+    dep = solver.epolar1()
+    err = np.abs(dep - grad).max() / np.abs(grad).max()
+    print(f"every axis type prec={prec} max|dep|={np.abs(grad).max():.4f} rel={err:.2e}", flush=True)
+    assert err < tol
+    solver.close()
+
+
+def test_gradient_needs_a_solve_of_the_current_state(ctx):
+    # the gradient belongs to the dipoles of the current coordinates and
+    # multipoles: after a coordinate or multipole update, before any solve,
+    # or after a truncated-CG solve it is refused
+    s = _box(seed=8)
+    solver = thp.Solver(ctx, s, FP64)
+    with pytest.raises(thp.ThpError) as e:
+        solver.epolar1()
+    assert e.value.code == -8
+    solver.newinduce_pme2(poleps=1e-8)
+    solver.epolar1()
+    solver.update_coords(s.x + 0.01, s.y, s.z)
+    with pytest.raises(thp.ThpError) as e:
+        solver.epolar1()
+    assert e.value.code == -8
+    solver.newinduce_pme2(poleps=1e-8)
+    solver.epolar1()
+    solver.update_multipoles(s.rpole * 1.01)
+    with pytest.raises(thp.ThpError) as e:
+        solver.epolar1()
+    assert e.value.code == -8
+    solver.set_solver(thp.THP_POLALG_TCG)
+    solver.newinduce_pme2(poleps=1e-8)
+    solver.set_solver(thp.THP_POLALG_PCG)
+    with pytest.raises(thp.ThpError) as e:
+        solver.epolar1()
+    assert e.value.code == -1
+    solver.close()
+
+
 def test_forces_and_torques_with_global_multipoles_match_oracle(ctx):
     s = _box(seed=7)
     fr = _water_frames(s, seed=2)

@@ -59,6 +59,48 @@ def test_nblist_pair_set_bit_exact(ctx, case):
     assert got.shape == ref.shape and np.array_equal(got, ref)
 
 
+def within_cut(s, pairs, cut):
+    """the rows of `pairs` with r^2 <= cut^2 in the reference's operation
+    order (xr = xi - xk, orthogonal image with floor, (x*x + y*y) + z*z;
+    image.f:113-140, newinduce_pme.f:1160)"""
+    i, k = pairs[:, 0], pairs[:, 1]
+    d = []
+    for c, L in zip((s.x, s.y, s.z), s.box):
+        v = c[i] - c[k]
+        big = np.abs(v) > 0.5 * L
+        v2 = np.where(big, v - np.copysign(L, v) * np.floor(np.abs(v) / L), v)
+        big2 = big & (np.abs(v2) > 0.5 * L)
+        d.append(np.where(big2, v2 - np.copysign(L, v2), v2))
+    r2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]
+    return pairs[r2 <= cut * cut]
+
+
+@pytest.mark.parametrize("prec", [FP64, MIXED])
+@pytest.mark.parametrize("case", ["water1944", "solute", "serving", "shifted"])
+def test_solver_pair_set_bit_exact(ctx, case, prec):
+    # the pairs the field pass and the matvec actually use (compacted lists
+    # plus the scaled pairs within the cutoff) against the oracle's list
+    # entries within the cutoff
+    s = {"water1944": thp.water_1944,
+         "solute": lambda: thp.make_system(3000, 34.0, n_solute=1200, blob_radius=11.0, seed=9, nfft=(36,) * 3),
+         "serving": lambda: thp.serving_system(800, 0.2, seed=5),
+         "shifted": lambda: _shifted(thp.water_1944())}[case]()
+    ref = within_cut(s, O.Oracle(s).nblist_pairs(), s.cut)
+    solver = thp.Solver(ctx, s, prec)
+    got = solver.pairs_in_cut()
+    print(f"{case} prec={prec} pairs={len(got)} ref={len(ref)}", flush=True)
+    assert got.shape == ref.shape and np.array_equal(got, ref)
+    # and after the atoms moved within the lists' skin (lists kept, the
+    # compacted entries rewritten for the new coordinates)
+    rng = np.random.default_rng(1)
+    s2 = s.with_coords(*(c + rng.normal(0, 0.05, s.n) for c in (s.x, s.y, s.z)))
+    solver.update_coords(s2.x, s2.y, s2.z)
+    got2 = solver.pairs_in_cut()
+    ref2 = within_cut(s2, O.Oracle(s2).nblist_pairs(), s.cut)
+    assert got2.shape == ref2.shape and np.array_equal(got2, ref2)
+    solver.close()
+
+
 # ---------------------------------------------------------- path entry points
 @pytest.mark.parametrize("prec,tol", [(FP64, 1e-11), (MIXED, 2e-5)])
 def test_field_and_matvec_entry_points_golden(ctx, prec, tol):

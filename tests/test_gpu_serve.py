@@ -198,3 +198,54 @@ def test_coordinates_past_the_skin_rebuild_lists(ctx):
         one.close()
         assert abs(r.ep - e1) <= E_TOL * abs(e1), (r.ep, e1)
         assert _rms_debye(mu, m1) <= MU_TOL_D
+
+
+def test_poisson_stream_of_mixed_sizes_matches_one_system_solves(ctx):
+    # a short stream as the serving benchmark runs it: 20 systems of 500 to
+    # 30,000 atoms (log-uniform, 0-30% solute), each requested 3 times at
+    # jittered coordinates (warm starts), Poisson arrivals, captured graphs;
+    # every result against a one-system solve at the same coordinates, the
+    # last dipoles of each system too, and a sample against the oracle
+    import time
+    from oracle import oracle as O
+    rng = np.random.default_rng(21)
+    sizes = [500, 30000] + [int(round(np.exp(rng.uniform(np.log(500), np.log(30000))))) for _ in range(18)]
+    systems = [thp.serving_system(n, float(rng.uniform(0.0, 0.3)), seed=900 + k) for k, n in enumerate(sizes)]
+    order = np.repeat(np.arange(len(systems)), 3)
+    rng.shuffle(order)
+    coords = [tuple(v + rng.normal(0.0, 0.01, systems[k].n) for v in (systems[k].x, systems[k].y, systems[k].z))
+              for k in order]
+    gaps = rng.exponential(1.0 / 400.0, len(order))
+    with Server(ctx, max_systems=8, warm_start=1, use_graphs=1) as srv:
+        sids = [srv.add(s) for s in systems]
+        tickets = []
+        t_next = time.perf_counter()
+        for q, k in enumerate(order):
+            t_next += gaps[q]
+            delay = t_next - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(sids[k], *coords[q]))
+        results = [srv.poll(t) for t in tickets]
+        last_mu = {int(k): srv.dipoles(sids[k]) for k in set(order.tolist())}
+        stats = srv.stats()
+    print(f"stream: {len(results)} solves, mean active {stats.mean_active:.2f}, graphs {stats.graph_launches}",
+          flush=True)
+    assert all(r.status == thp.THP_OK for r in results)
+    assert stats.mean_active > 1.0
+    last_q = {}
+    for q, k in enumerate(order):
+        last_q[int(k)] = q
+    for q, (k, r) in enumerate(zip(order, results)):
+        one = thp.Solver(ctx, systems[k].with_coords(*coords[q]), MIXED)
+        m1, e1, _ = one.newinduce_pme2()
+        one.close()
+        assert abs(r.ep - e1) <= E_TOL * abs(e1), (sizes[k], r.ep, e1)
+        if last_q[int(k)] == q:
+            assert _rms_debye(last_mu[int(k)], m1) <= MU_TOL_D
+    O.set_threads(8)
+    for k in (0, 1, 7):  # 500 and 30,000 atoms and one between
+        q = last_q[k]
+        ref = O.Oracle(systems[k].with_coords(*coords[q])).induce(poleps=1e-8)
+        assert abs(results[q].ep - ref["ep"]) <= E_TOL * abs(ref["ep"])
+        assert _rms_debye(last_mu[k], ref["mu"]) <= MU_TOL_D

@@ -29,6 +29,30 @@ thp_status thp::fail(thp_ctx* ctx, thp_status code, const char* fmt, ...) {
 
 size_t thp::grid_points(const thp_sys* s) { return (size_t)s->nfft[0] * s->nfft[1] * s->nfft[2]; }
 
+// Stream-ordered entry points: the library's stream first waits for the
+// work the caller queued on `caller`, and at the end `caller` waits for
+// the library's work, so nothing blocks the host.  A null caller makes the
+// call synchronous.
+thp_status thp::order_in(thp_ctx* ctx, hipStream_t caller) {
+  if (!caller || caller == ctx->stream) return THP_OK;
+  hipError_t e = hipEventRecord(ctx->ev_in, caller);
+  if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, ctx->ev_in, 0);
+  if (e != hipSuccess) return fail(ctx, THP_ERR_HIP, "ordering behind the caller's stream: %s", hipGetErrorString(e));
+  return THP_OK;
+}
+
+thp_status thp::order_out(thp_ctx* ctx, hipStream_t caller) {
+  hipError_t e = hipSuccess;
+  if (!caller) e = hipStreamSynchronize(ctx->stream);
+  else if (caller != ctx->stream) {
+    e = hipEventRecord(ctx->ev_out, ctx->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(caller, ctx->ev_out, 0);
+  }
+  if (e != hipSuccess) return fail(ctx, THP_ERR_HIP, "ordering the caller's stream behind the call: %s", hipGetErrorString(e));
+  return THP_OK;
+}
+
+
 namespace {
 
 #define HIPCHK(ctx, call)                                                                 \
@@ -53,16 +77,20 @@ thp_status dalloc(thp_sys* s, T** p, size_t count) {
   if (e != hipSuccess) {
     return fail(s->ctx, THP_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
   }
-  s->allocs.push_back(q);
+  s->allocs[q] = bytes;
   s->bytes += (int64_t)bytes;
   *p = static_cast<T*>(q);
   return THP_OK;
 }
 
+// releases an allocation of the system and takes it off the footprint
 void dfree(thp_sys* s, void* p) {
   if (!p) return;
-  auto it = std::find(s->allocs.begin(), s->allocs.end(), p);
-  if (it != s->allocs.end()) s->allocs.erase(it);
+  auto it = s->allocs.find(p);
+  if (it != s->allocs.end()) {
+    s->bytes -= (int64_t)it->second;
+    s->allocs.erase(it);
+  }
   (void)hipFree(p);
 }
 
@@ -343,26 +371,20 @@ thp_status field(thp_ctx* ctx, const Batch& b, double* cphi_dev, int add_to_ef, 
   return launch_check(ctx, "k_gather_mpole");
 }
 
+// host array [n][ncomp] in caller order into a sorted device array
+// [ncomp][npad] (ncomp <= 10), through the system's staging buffer
 thp_status scatter_in(thp_sys* s, double* dst_dev, const double* host, int ncomp) {
-  double* tmp = nullptr;
-  RET(dalloc(s, &tmp, (size_t)s->n * ncomp));
-  HIPCHK(s->ctx, hipMemcpyAsync(tmp, host, sizeof(double) * s->n * ncomp, hipMemcpyHostToDevice, s->ctx->stream));
-  hipLaunchKernelGGL(k_scatter_in, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream, dst_dev, tmp, s->iperm_dev, s->n, s->npad, ncomp);
-  thp_status r = launch_check(s->ctx, "k_scatter_in");
-  RET(sync(s));
-  dfree(s, tmp);
-  return r;
+  HIPCHK(s->ctx, hipMemcpyAsync(s->stage, host, sizeof(double) * s->n * ncomp, hipMemcpyHostToDevice, s->ctx->stream));
+  hipLaunchKernelGGL(k_scatter_in, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream, dst_dev, s->stage, s->iperm_dev, s->n, s->npad, ncomp);
+  RET(launch_check(s->ctx, "k_scatter_in"));
+  return sync(s);
 }
 
 thp_status gather_out(thp_sys* s, double* host, const double* src_dev, int ncomp) {
-  double* tmp = nullptr;
-  RET(dalloc(s, &tmp, (size_t)s->n * ncomp));
-  hipLaunchKernelGGL(k_gather_out, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream, tmp, src_dev, s->iperm_dev, s->n, s->npad, ncomp);
+  hipLaunchKernelGGL(k_gather_out, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream, s->stage, src_dev, s->iperm_dev, s->n, s->npad, ncomp);
   RET(launch_check(s->ctx, "k_gather_out"));
-  HIPCHK(s->ctx, hipMemcpyAsync(host, tmp, sizeof(double) * s->n * ncomp, hipMemcpyDeviceToHost, s->ctx->stream));
-  RET(sync(s));
-  dfree(s, tmp);
-  return THP_OK;
+  HIPCHK(s->ctx, hipMemcpyAsync(host, s->stage, sizeof(double) * s->n * ncomp, hipMemcpyDeviceToHost, s->ctx->stream));
+  return sync(s);
 }
 
 // bounding boxes and candidate block pairs (cut + buffer); valid until an
@@ -573,6 +595,7 @@ thp_status update_coords(thp_sys* s, const double* x, const double* y, const dou
   s->verlet_valid = false;
   s->pme_valid = false;
   s->coef_valid = false;
+  s->mu_valid = false;
   // the global multipoles follow the frames (rotpole each step)
   if (s->has_frames) RET(rotate_frames(s));
   return THP_OK;
@@ -647,8 +670,14 @@ thp_status b_pair_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int do
 // different arrays); join_matvec makes the main stream wait for it before
 // the gather that adds the two parts.  Only one-system launches fork: a
 // batch of systems fills the device with either branch, and overlapping
-// them gains nothing.  Otherwise the matvec runs in line.
-inline bool forks(const thp_ctx* ctx, const Batch& b) { return ctx->side && !b.real_only && b.nsys == 1; }
+// them gains nothing.  Only a system's own launch sequence (its batch of
+// one, launched eagerly) forks: the serving loop's batches, captured as
+// graphs, never record the fork and join events, so no event is recorded
+// both inside a capture and in eager launches.  Otherwise the matvec runs
+// in line.
+inline bool forks(const thp_ctx* ctx, const Batch& b) {
+  return ctx->side && b.fork_ok && !b.real_only && b.nsys == 1;
+}
 
 thp_status fork_matvec(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int gate) {
   if (!forks(ctx, b)) return b_pair_tmatxb(ctx, b, vin, vout, dodiag, gate);
@@ -741,6 +770,7 @@ thp_status sync_host_coords(thp_sys* s) {
   s->hx.swap(x);
   s->hy.swap(y);
   s->hz.swap(z);
+  s->mu_valid = false;
   const double lim_b = 0.5 * (s->buffer - s->skin), lim_u = 0.5 * s->skin;
   if (s->list_valid && maxd2 > lim_b * lim_b) s->list_valid = false;
   if (!s->list_valid || maxu2 > lim_u * lim_u) s->tight_valid = false;
@@ -848,6 +878,11 @@ thp_status thp_init(thp_ctx** out, int device) {
       return fail(ctx, THP_ERR_HIP, "cannot create the side stream of device %d", device);
     }
   }
+  if (hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming) != hipSuccess) {
+    *out = ctx;
+    return fail(ctx, THP_ERR_HIP, "cannot create the ordering events of device %d", device);
+  }
   // THP_SEGS forces the pair kernels' wavefronts per 64-atom block
   if (const char* e = getenv("THP_SEGS")) {
     const int v = atoi(e);
@@ -871,6 +906,8 @@ thp_status thp_finalize(thp_ctx* ctx) {
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
   if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+  if (ctx->ev_in) (void)hipEventDestroy(ctx->ev_in);
+  if (ctx->ev_out) (void)hipEventDestroy(ctx->ev_out);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return THP_OK;
@@ -1085,6 +1122,7 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
   ALLOC(v.h, (size_t)6 * np); ALLOC(v.hreal, (size_t)6 * np); ALLOC(v.pp, (size_t)6 * np);
   ALLOC(v.zr, (size_t)6 * np); ALLOC(v.partial, partial_doubles(v.gat_blocks, v.red_blocks));
   ALLOC(s->cphi, (size_t)10 * np);
+  ALLOC(s->stage, (size_t)10 * n);
   ALLOC(v.st, 1); ALLOC(v.err, 1);
   if (hipMemsetAsync(v.err, 0, sizeof(int), st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "memset"));
   {
@@ -1230,6 +1268,7 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
     b.nsys = 1;
     b.segs = ctx->pair_segs > 0 ? ctx->pair_segs : kDefaultSegs;
     b.nb = ctx->pair_nb > 0 ? ctx->pair_nb : 8;
+    b.fork_ok = 1;
     b.d_views = s->d_view;
     b.d_blk = s->d_workx;
     b.n_blk = s->nworkx;
@@ -1265,7 +1304,7 @@ thp_status thp_system_destroy(thp_sys* s) {
     if (s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
     s->ctx->systems.erase(s);
   }
-  for (void* p : s->allocs) (void)hipFree(p);
+  for (auto& a : s->allocs) (void)hipFree(a.first);
   for (auto& e : s->ev)
     if (e) (void)hipEventDestroy(e);
   delete s;
@@ -1321,12 +1360,14 @@ thp_status thp_system_set_frames(thp_sys* s, const int* axistype, const int* zax
   s->has_frames = true;
   s->pme_valid = false;
   s->coef_valid = false;
+  s->mu_valid = false;
   return rotate_frames(s);
 }
 
 thp_status thp_system_update_multipoles(thp_sys* s, const double* rpole) {
   if (!s || !rpole) return THP_ERR_ARG;
   s->has_frames = false;  // global multipoles given outright replace the frames
+  s->mu_valid = false;
   const int n = s->n;
   const size_t np = s->npad;
   std::vector<double> rp((size_t)10 * np, 0.0);
@@ -1399,6 +1440,63 @@ thp_status thp_nblist_get(thp_sys* s, int64_t* npairs, int* ik, int64_t cap) {
   return THP_OK;
 }
 
+thp_status thp_pairs_in_cut(thp_sys* s, int64_t* npairs, int* ik, int64_t cap) {
+  if (!s || !npairs) return THP_ERR_ARG;
+  // the compacted lists of the current coordinates (the permanent-field
+  // pass writes them) and the scaled pairs' coefficients (zero beyond the
+  // cutoff)
+  RET(ensure_coef(s));
+  RET(sync(s));
+  const int np = s->npad;
+  const SysView& v = s->view;
+  const size_t packs = (size_t)v.cmax / kPack;
+  std::vector<int> tcount(np), ulist((size_t)s->nblk * v.umax), eoff(s->n + 1);
+  std::vector<unsigned short> tlist(packs * np * kPack);
+  HIPCHK(s->ctx, hipMemcpy(tcount.data(), v.tcount, sizeof(int) * np, hipMemcpyDeviceToHost));
+  HIPCHK(s->ctx, hipMemcpy(ulist.data(), v.ulist, sizeof(int) * ulist.size(), hipMemcpyDeviceToHost));
+  HIPCHK(s->ctx, hipMemcpy(tlist.data(), v.tlist, sizeof(unsigned short) * tlist.size(), hipMemcpyDeviceToHost));
+  HIPCHK(s->ctx, hipMemcpy(eoff.data(), v.excl_off, sizeof(int) * (s->n + 1), hipMemcpyDeviceToHost));
+  const int nex = eoff[s->n];
+  std::vector<int> ek(std::max(nex, 1));
+  std::vector<double> sco(2 * (size_t)std::max(nex, 1));
+  if (nex > 0) {
+    HIPCHK(s->ctx, hipMemcpy(ek.data(), v.excl_k, sizeof(int) * nex, hipMemcpyDeviceToHost));
+    HIPCHK(s->ctx, hipMemcpy(sco.data(), v.scoef, sizeof(double) * 2 * nex, hipMemcpyDeviceToHost));
+  }
+  // every entry as seen from each of its two atoms; a pair counts only if
+  // both atoms hold it
+  std::vector<std::pair<int, int>> half[2];
+  for (int q = 0; q < s->n; ++q) {
+    const int b = q / kBlock;
+    for (int e = 0; e < tcount[q]; ++e) {
+      const int u = tlist[((size_t)(e / kPack) * np + q) * kPack + e % kPack];
+      const int k = ulist[(size_t)b * v.umax + u];
+      if (k < 0 || k >= s->n || k == q) return fail(s->ctx, THP_ERR_LIST, "compacted entry %d of atom %d: %d", e, q, k);
+      const int a = s->perm[q], c = s->perm[k];
+      half[a < c ? 0 : 1].push_back(a < c ? std::make_pair(a, c) : std::make_pair(c, a));
+    }
+    for (int m = eoff[q]; m < eoff[q + 1]; ++m) {
+      if (sco[2 * (size_t)m] == 0.0 && sco[2 * (size_t)m + 1] == 0.0) continue;  // beyond the cutoff
+      const int a = s->perm[q], c = s->perm[ek[m]];
+      half[a < c ? 0 : 1].push_back(a < c ? std::make_pair(a, c) : std::make_pair(c, a));
+    }
+  }
+  for (auto& h : half) std::sort(h.begin(), h.end());
+  if (half[0] != half[1])
+    return fail(s->ctx, THP_ERR_LIST, "the compacted lists are not symmetric (%zu vs %zu entries)", half[0].size(),
+                half[1].size());
+  const auto& pairs = half[0];
+  *npairs = (int64_t)pairs.size();
+  if (!ik) return THP_OK;
+  if (cap < (int64_t)pairs.size())
+    return fail(s->ctx, THP_ERR_ARG, "pair buffer too small (%lld < %zu)", (long long)cap, pairs.size());
+  for (size_t m = 0; m < pairs.size(); ++m) {
+    ik[2 * m] = pairs[m].first;
+    ik[2 * m + 1] = pairs[m].second;
+  }
+  return THP_OK;
+}
+
 thp_status thp_efld0_direct(thp_sys* s, double* ef) {
   if (!s || !ef) return THP_ERR_ARG;
   RET(ensure_lists(s));
@@ -1411,47 +1509,48 @@ thp_status thp_epolar1(thp_sys* s, double* dep, double* torque) {
   if (!s || !dep) return THP_ERR_ARG;
   thp_ctx* ctx = s->ctx;
   if (s->one.real_only) return fail(ctx, THP_ERR_ARG, "thp_epolar1: short-range solves have no gradient here");
+  // the dipoles must be a solve of the current coordinates and multipoles
+  if (!s->mu_valid)
+    return fail(ctx, THP_ERR_STATE, "thp_epolar1: no solve of the current coordinates and multipoles");
+  if (s->mu_short) return fail(ctx, THP_ERR_ARG, "thp_epolar1: the last solve was a short-range solve");
   // truncated-CG dipoles are not stationary: their gradient (epolar1tcg)
   // has terms of its own
-  if (s->polalg == THP_POLALG_TCG) return fail(ctx, THP_ERR_ARG, "thp_epolar1: no gradient for truncated-CG solves");
+  if (s->mu_polalg == THP_POLALG_TCG)
+    return fail(ctx, THP_ERR_ARG, "thp_epolar1: no gradient for truncated-CG solves");
   RET(ensure_lists(s));
   RET(ensure_pme(s));
   const size_t np = s->npad;
-  double* F = nullptr;
-  double* T = nullptr;
-  RET(dalloc(s, &F, 3 * np));
-  thp_status st = dalloc(s, &T, 3 * np);
-  auto done = [&](thp_status r) {
-    dfree(s, F);
-    dfree(s, T);
-    return r;
-  };
-  if (st != THP_OK) return done(st);
+  if (!s->grad_f) {
+    RET(dalloc(s, &s->grad_f, 3 * np));
+    RET(dalloc(s, &s->grad_t, 3 * np));
+  }
+  double* F = s->grad_f;
+  double* T = s->grad_t;
   hipStream_t hs = ctx->stream;
   const Batch& b = s->one;
   // the multipole grid's potential derivatives into cphi, then the two
   // induced grids of the solved dipoles
-  if ((st = field(ctx, b, s->cphi, 0, 0, kGateNone)) != THP_OK) return done(st);
-  if ((st = spread_uind(ctx, b, V_MU, 0, kGateNone)) != THP_OK) return done(st);
-  if ((st = convolve(ctx, b, kGateNone)) != THP_OK) return done(st);
+  RET(field(ctx, b, s->cphi, 0, 0, kGateNone));
+  RET(spread_uind(ctx, b, V_MU, 0, kGateNone));
+  RET(convolve(ctx, b, kGateNone));
   hipLaunchKernelGGL(k_ep1_real<4>, g1(b.n_blk), dim3(256), 0, hs, b.d_views, b.d_blk, F, T);
-  if ((st = launch_check(ctx, "k_ep1_real")) != THP_OK) return done(st);
+  RET(launch_check(ctx, "k_ep1_real"));
   if (s->prec == THP_PREC_FP64)
     hipLaunchKernelGGL((k_ep1_recip<double>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T);
   else
     hipLaunchKernelGGL((k_ep1_recip<float>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T);
-  if ((st = launch_check(ctx, "k_ep1_recip")) != THP_OK) return done(st);
+  RET(launch_check(ctx, "k_ep1_recip"));
   const dim3 ga((s->n + 255) / 256);
   if (s->has_frames) {
     hipLaunchKernelGGL(k_ep1_torque, ga, dim3(256), 0, hs, s->d_view, s->d_frames, T, F);
-    if ((st = launch_check(ctx, "k_ep1_torque")) != THP_OK) return done(st);
+    RET(launch_check(ctx, "k_ep1_torque"));
   }
   hipLaunchKernelGGL(k_ep1_finish, ga, dim3(256), 0, hs, s->n, np, F, T);
-  if ((st = launch_check(ctx, "k_ep1_finish")) != THP_OK) return done(st);
-  if ((st = gather_out(s, dep, F, 3)) != THP_OK) return done(st);
+  RET(launch_check(ctx, "k_ep1_finish"));
+  RET(gather_out(s, dep, F, 3));
   for (size_t q = 0; q < (size_t)s->n * 3; ++q) dep[q] = -dep[q];  // gradient = -force
-  if (torque && (st = gather_out(s, torque, T, 3)) != THP_OK) return done(st);
-  return done(THP_OK);
+  if (torque) RET(gather_out(s, torque, T, 3));
+  return THP_OK;
 }
 
 thp_status thp_efld0_recip(thp_sys* s, double* cphi) {
@@ -1546,6 +1645,133 @@ thp_status thp_fphi_uind(thp_sys* s, const double* qgrid, double* phi1, double* 
     }
   return THP_OK;
 }
+
+// ------------------------------- device pointers, ordered on the caller's stream
+// Each takes device arrays in the layouts of the host entry points and a
+// stream (hipStream_t; null: the call returns once the device is done).
+// With a stream the call enqueues its work behind what the caller queued
+// there and makes the stream wait for it, without blocking the host (a list
+// or B-spline rebuild due to moved atoms still waits for the device).
+
+#define THP_NEED_DEV_ARGS(cond) \
+  do {                          \
+    if (!s || !(cond)) return THP_ERR_ARG; \
+  } while (0)
+
+}  // extern "C"
+
+namespace {
+inline dim3 atoms_grid(const thp_sys* s) { return dim3((s->n + 255) / 256); }
+
+thp_status grid_out_dev(thp_sys* s, double* qgrid) {
+  const size_t cnt = 2 * grid_points(s);
+  hipStream_t st = s->ctx->stream;
+  if (s->prec == THP_PREC_FP64) {
+    HIPCHK(s->ctx, hipMemcpyAsync(qgrid, s->view.grid, sizeof(double) * cnt, hipMemcpyDeviceToDevice, st));
+  } else {
+    hipLaunchKernelGGL((k_grid_widen<float>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, qgrid,
+                       static_cast<const float*>(s->view.grid), cnt);
+  }
+  return launch_check(s->ctx, "grid copy");
+}
+
+thp_status grid_in_dev(thp_sys* s, const double* qgrid) {
+  const size_t cnt = 2 * grid_points(s);
+  hipStream_t st = s->ctx->stream;
+  if (s->prec == THP_PREC_FP64) {
+    HIPCHK(s->ctx, hipMemcpyAsync(s->view.grid, qgrid, sizeof(double) * cnt, hipMemcpyDeviceToDevice, st));
+  } else {
+    hipLaunchKernelGGL((k_grid_narrow<float>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st,
+                       static_cast<float*>(s->view.grid), qgrid, cnt);
+  }
+  return launch_check(s->ctx, "grid copy");
+}
+}  // namespace
+
+extern "C" {
+
+thp_status thp_efld0_direct_dev(thp_sys* s, double* ef, void* stream) {
+  THP_NEED_DEV_ARGS(ef);
+  hipStream_t cs = static_cast<hipStream_t>(stream);
+  RET(order_in(s->ctx, cs));
+  RET(ensure_lists(s));
+  RET(pair_efld0(s->ctx, s->one, kGateNone));
+  s->coef_valid = true;
+  hipLaunchKernelGGL(k_gather_out, atoms_grid(s), dim3(256), 0, s->ctx->stream, ef, s->view.ef, s->iperm_dev, s->n, s->npad, 6);
+  RET(launch_check(s->ctx, "k_gather_out"));
+  return order_out(s->ctx, cs);
+}
+
+thp_status thp_efld0_recip_dev(thp_sys* s, double* cphi, void* stream) {
+  THP_NEED_DEV_ARGS(cphi);
+  hipStream_t cs = static_cast<hipStream_t>(stream);
+  RET(order_in(s->ctx, cs));
+  RET(ensure_pme(s));
+  RET(field(s->ctx, s->one, s->cphi, 0, 0, kGateNone));
+  hipLaunchKernelGGL(k_gather_out, atoms_grid(s), dim3(256), 0, s->ctx->stream, cphi, s->cphi, s->iperm_dev, s->n, s->npad, 10);
+  RET(launch_check(s->ctx, "k_gather_out"));
+  return order_out(s->ctx, cs);
+}
+
+thp_status thp_tmatxb_dev_async(thp_sys* s, const double* mu, double* efi, int dodiag, void* stream) {
+  THP_NEED_DEV_ARGS(mu && efi);
+  hipStream_t cs = static_cast<hipStream_t>(stream);
+  RET(order_in(s->ctx, cs));
+  RET(ensure_coef(s));
+  hipLaunchKernelGGL(k_scatter_in, atoms_grid(s), dim3(256), 0, s->ctx->stream, s->view.pp, mu, s->iperm_dev, s->n, s->npad, 6);
+  RET(launch_check(s->ctx, "k_scatter_in"));
+  RET(b_pair_tmatxb(s->ctx, s->one, V_PP, V_HREAL, dodiag, 0));
+  hipLaunchKernelGGL(k_gather_out, atoms_grid(s), dim3(256), 0, s->ctx->stream, efi, s->view.hreal, s->iperm_dev, s->n, s->npad, 6);
+  RET(launch_check(s->ctx, "k_gather_out"));
+  return order_out(s->ctx, cs);
+}
+
+thp_status thp_tmatxb_recip_dev(thp_sys* s, const double* mu, double* dipfield, void* stream) {
+  THP_NEED_DEV_ARGS(mu && dipfield);
+  hipStream_t cs = static_cast<hipStream_t>(stream);
+  RET(order_in(s->ctx, cs));
+  RET(ensure_pme(s));
+  hipLaunchKernelGGL(k_scatter_in, atoms_grid(s), dim3(256), 0, s->ctx->stream, s->view.pp, mu, s->iperm_dev, s->n, s->npad, 6);
+  RET(launch_check(s->ctx, "k_scatter_in"));
+  RET(spread_uind(s->ctx, s->one, V_PP, 0, 0));
+  RET(convolve(s->ctx, s->one, 0));
+  RET(launch_gather_uind<0>(s->ctx, s->one, V_PP, V_HREAL, 0, kGateNone));
+  hipLaunchKernelGGL(k_gather_out, atoms_grid(s), dim3(256), 0, s->ctx->stream, dipfield, s->view.hreal, s->iperm_dev, s->n, s->npad, 6);
+  RET(launch_check(s->ctx, "k_gather_out"));
+  return order_out(s->ctx, cs);
+}
+
+thp_status thp_grid_uind_dev(thp_sys* s, const double* fuind, const double* fuinp, double* qgrid, void* stream) {
+  THP_NEED_DEV_ARGS(fuind && fuinp);
+  hipStream_t cs = static_cast<hipStream_t>(stream);
+  RET(order_in(s->ctx, cs));
+  RET(ensure_pme(s));
+  hipStream_t st = s->ctx->stream;
+  hipLaunchKernelGGL(k_scatter_in, atoms_grid(s), dim3(256), 0, st, s->view.pp, fuind, s->iperm_dev, s->n, s->npad, 3);
+  hipLaunchKernelGGL(k_scatter_in, atoms_grid(s), dim3(256), 0, st, s->view.pp + (size_t)3 * s->npad, fuinp,
+                     s->iperm_dev, s->n, s->npad, 3);
+  RET(launch_check(s->ctx, "k_scatter_in"));
+  RET(spread_uind(s->ctx, s->one, V_PP, 1, 0));
+  if (qgrid) RET(grid_out_dev(s, qgrid));
+  return order_out(s->ctx, cs);
+}
+
+thp_status thp_fphi_uind_dev(thp_sys* s, const double* qgrid, double* phi1, double* phi2, void* stream) {
+  THP_NEED_DEV_ARGS(phi1 && phi2);
+  hipStream_t cs = static_cast<hipStream_t>(stream);
+  RET(order_in(s->ctx, cs));
+  RET(ensure_pme(s));
+  if (qgrid) RET(grid_in_dev(s, qgrid));
+  RET(launch_gather_uind<0>(s->ctx, s->one, V_PP, V_HREAL, 1, kGateNone));
+  hipStream_t st = s->ctx->stream;
+  hipLaunchKernelGGL(k_gather_out, atoms_grid(s), dim3(256), 0, st, phi1, s->view.hreal, s->iperm_dev, s->n, s->npad, 3);
+  hipLaunchKernelGGL(k_gather_out, atoms_grid(s), dim3(256), 0, st, phi2, s->view.hreal + (size_t)3 * s->npad,
+                     s->iperm_dev, s->n, s->npad, 3);
+  RET(launch_check(s->ctx, "k_gather_out"));
+  return order_out(s->ctx, cs);
+}
+
+#undef THP_NEED_DEV_ARGS
 
 }  // extern "C"
 
@@ -1692,6 +1918,9 @@ thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_s
   SolveState fin;
   HIPCHK(ctx, hipMemcpyAsync(&fin, s->view.st, sizeof fin, hipMemcpyDeviceToHost, st));
   RET(sync(s));
+  s->mu_valid = true;
+  s->mu_polalg = s->polalg;
+  s->mu_short = short_range;
   if (stats) {
     float t1 = 0, t2 = 0;
     (void)hipEventElapsedTime(&t1, s->ev[0], s->ev[1]);
@@ -1823,10 +2052,11 @@ thp_status thp_time_kernel(thp_sys* s, int kernel, int reps, double* ms, int64_t
 }
 
 thp_status thp_time_kernel_batch(thp_sys* const* systems, int nsys, int kernel, int reps, double* ms, int64_t* pairs) {
-  if (!systems || nsys <= 0 || !ms || reps <= 0 || kernel < 0 || kernel > 3) return THP_ERR_ARG;
+  if (!systems || nsys <= 0 || !ms || reps <= 0 || kernel < 0 || kernel > 6) return THP_ERR_ARG;
   thp_sys* s0 = systems[0];
   if (!s0) return THP_ERR_ARG;
   thp_ctx* ctx = s0->ctx;
+  if ((kernel == 4 || kernel == 5) && nsys != 1) return fail(ctx, THP_ERR_ARG, "list rebuilds are timed one system at a time");
   for (int q = 0; q < nsys; ++q) {
     if (!systems[q] || systems[q]->ctx != ctx || systems[q]->prec != s0->prec)
       return fail(ctx, THP_ERR_ARG, "batch members must share one context and one precision");
@@ -1892,6 +2122,20 @@ thp_status thp_time_kernel_batch(thp_sys* const* systems, int nsys, int kernel, 
     if (kernel == 0) return b_pair_tmatxb(ctx, b, 1, 3, 1, 0);
     if (kernel == 1) return spread_uind(ctx, b, 1, 0, 0);
     if (kernel == 3) return pair_efld0(ctx, b, kGateNone);  // rewrites the same lists
+    if (kernel == 4) {
+      // the solver's lists from scratch: bounding boxes, candidate block
+      // pairs and block-union lists (host round trips for the counts
+      // included), then the compacted lists of the permanent-field pass
+      RET(build_block_lists(s0));
+      RET(build_union(s0));
+      s0->tight_valid = true;
+      return pair_efld0(ctx, b, kGateNone);
+    }
+    if (kernel == 5) {
+      s0->verlet_valid = false;
+      return thp_nblist(s0, 1);  // the reference's Verlet list (mlistcell)
+    }
+    if (kernel == 6) return convolve(ctx, b, kGateNone);  // forward FFT, qfac, backward FFT
     return launch_gather_uind<0>(ctx, b, 1, 3, 0, kGateNone);
   };
   thp_status rc = launch();  // warm

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <set>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/thp.h"
@@ -195,6 +196,9 @@ struct Batch {
   int real_only = 0;            // short-range solves: no reciprocal terms
   int fft_plane = 0;            // x and y FFT passes fused per z plane (plane_fft_setup)
   int fft_plane_lds = 0;        // its dynamic LDS bytes
+  // the real-space matvec may run on the context's side stream beside the
+  // reciprocal chain (one-system launch sequences only, never captured)
+  int fork_ok = 0;
 };
 
 // Whether a batch of these grids takes the fused x-y plane FFT kernel: FP32
@@ -230,6 +234,8 @@ struct thp_ctx {
   bool plane_fft = false;       // the fused x-y plane FFT kernel may take its LDS
   hipStream_t side = nullptr;   // the real-space matvec beside the reciprocal chain
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // ordering of the stream-ordered entry points against the caller's stream
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
   std::set<thp_sys*> systems;  // live systems; thp_finalize releases what is left
 };
 
@@ -286,9 +292,20 @@ struct thp_sys {
   int nbricks = 0;
   thp::Work* d_brickwork = nullptr;  // [nbricks]
   int maxnb = 0;
-  // device allocations
-  std::vector<void*> allocs;
+  // device allocations and their sizes; bytes is their sum
+  std::unordered_map<void*, size_t> allocs;
   int64_t bytes = 0;
+  // staging for host arrays copied in and out in caller order ([n][10]
+  // doubles: the widest array of the entry points)
+  double* stage = nullptr;
+  // dipoles in mu are the result of a solve of the current coordinates and
+  // multipoles (the gradient needs them), and the solver that made them
+  bool mu_valid = false;
+  int mu_polalg = 1;
+  bool mu_short = false;
+  // forces and torques of the gradient ([3][npad] each), made on first use
+  double* grad_f = nullptr;
+  double* grad_t = nullptr;
   thp::SysView view;           // host copy of the view
   thp::SysView* d_view = nullptr;
   int view_version = 0;        // bumped whenever the device view is rewritten
@@ -348,6 +365,11 @@ struct StateOut {
 thp_status b_collect(thp_ctx* ctx, const Batch& b, StateOut* d_out);
 // the system's dipoles [n][2][3] in caller order into host memory (synchronous)
 thp_status download_mu(thp_sys* s, double* host);
+// stream-ordered calls: the library's stream waits for the caller's work
+// (order_in), the caller's stream for the library's (order_out); a null
+// caller makes the call synchronous
+thp_status order_in(thp_ctx* ctx, hipStream_t caller);
+thp_status order_out(thp_ctx* ctx, hipStream_t caller);
 // host copy of the device's sorted coordinates and list validity (synchronous)
 thp_status sync_host_coords(thp_sys* s);
 }  // namespace thp

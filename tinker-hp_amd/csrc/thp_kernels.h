@@ -3020,5 +3020,16 @@ __global__ void k_gather_out(double* dst, const double* src, const int* iperm, i
   const int s = iperm[a];
   for (int c = 0; c < ncomp; ++c) dst[(size_t)a * ncomp + c] = src[(size_t)c * npad + s];
 }
+// a grid of the pair type to FP64 and back (the FP64 grids of the entry points)
+template <typename real>
+__global__ void k_grid_widen(double* dst, const real* src, size_t count) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < count) dst[q] = (double)src[q];
+}
+template <typename real>
+__global__ void k_grid_narrow(real* dst, const double* src, size_t count) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < count) dst[q] = (real)src[q];
+}
 
 }  // namespace thp

@@ -17,6 +17,7 @@
 // reciprocal grids of equal size sit side by side in a pool, and one set of
 // FFT launches transforms the grids of every size in the batch.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -54,9 +55,12 @@ struct ServeSystem {
   int idle_passes = 0;       // passes without a solve in flight while a member
 };
 
+struct ServeSystem;
+
 struct Request {
   uint64_t ticket = 0;
   int sid = 0;
+  ServeSystem* sys = nullptr;  // set on submission (systems never move: unique_ptr)
   double* xyz = nullptr;  // pinned [3][n] caller-order coordinates, or null (unchanged)
   size_t xyz_cap = 0;     // doubles held by xyz (a pool buffer)
   double* rp = nullptr;   // pinned [n][13] caller-order multipoles, or null (unchanged)
@@ -130,7 +134,7 @@ struct Serve {
   std::thread worker;
   // worker-owned
   std::vector<InFlight> active;
-  int64_t resident = 0;
+  std::atomic<int64_t> resident{0};  // device bytes of resident systems (written by the worker)
   int64_t budget = 0;
   uint64_t use_clock = 0;
   std::map<std::tuple<int, int, int>, GridPool> pools;
@@ -152,6 +156,9 @@ struct Serve {
   std::map<std::pair<int, int>, int> tuned;  // (log2 atoms, systems) -> pair waves per block
   std::multimap<size_t, double*> pinned_pool;  // free request coordinate buffers, by capacity (under mu)
   std::vector<InFlight> restarts;              // set-ups to repeat after list rebuilds (worker)
+  // systems with device data, in admission order (worker); the worker never
+  // walks `systems`, which other threads extend under `mu`
+  std::vector<ServeSystem*> residents;
 };
 
 // request coordinate buffers: pinned, recycled through the pool
@@ -190,13 +197,24 @@ void give_pinned(Serve* sv, double* p, size_t cap) {
     if (e_ != hipSuccess) return fail((ctx), THP_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
   } while (0)
 
-// device footprint of a system before it exists (per-atom arrays, lists,
-// B-spline tables, scalar-sum table); replaced by the real figure once created
+// device footprint of a system before it exists: per-atom arrays and
+// B-spline tables (~1.8 kB per atom), block-union and compacted lists at
+// liquid density (~3 kB per atom with FP32 coefficients, ~4.5 kB with FP64),
+// the scalar-sum table; replaced by the real figure once the system and its
+// lists exist
 int64_t estimate_bytes(const thp_system_desc& d) {
   const int64_t np = (d.n + kBlock - 1) / kBlock * kBlock;
   const int64_t pts = (int64_t)d.nfft[0] * d.nfft[1] * d.nfft[2];
   const int64_t real = d.precision == THP_PREC_FP64 ? 8 : 4;
-  return np * 1792 + pts * real + (256 << 10);
+  const int64_t lists = d.precision == THP_PREC_FP64 ? 4608 : 3072;
+  return np * (1792 + lists) + pts * real + (256 << 10);
+}
+
+// the resident figure follows the system's device memory (lists grow)
+void refresh_bytes(Serve* sv, ServeSystem* ss) {
+  if (!ss->s) return;
+  sv->resident += ss->s->bytes - ss->bytes;
+  ss->bytes = ss->s->bytes;
 }
 
 thp_status grow_device(thp_ctx* ctx, void** p, size_t* cap, size_t need, size_t elem) {
@@ -259,6 +277,9 @@ thp_status build_batch(Serve* sv, BatchBuf& bb, const std::vector<thp_sys*>& mem
   for (int q = 0; q < m; ++q) groups[{members[q]->nfft[0], members[q]->nfft[1], members[q]->nfft[2]}].push_back(q);
   const int prec = members[0]->prec;
   const size_t real = prec == THP_PREC_FP64 ? 8 : 4;
+  // the layout changes: its graphs go first, before any buffer they
+  // reference is freed or rewritten
+  drop_graph(bb);
   for (auto& g : groups) {
     GridPool& gp = sv->pools[g.first];
     const int need = (int)g.second.size();
@@ -343,7 +364,6 @@ thp_status build_batch(Serve* sv, BatchBuf& bb, const std::vector<thp_sys*>& mem
   SHIP(ctx, hipMemcpyAsync(bb.tables, bb.h_tables, sizeof(Work) * nt, hipMemcpyHostToDevice, st));
   SHIP(ctx, hipMemcpyAsync(bb.ids, bb.h_ids, sizeof(int) * m, hipMemcpyHostToDevice, st));
   SHIP(ctx, hipMemcpyAsync(bb.warm, bb.h_warm, sizeof(int) * m, hipMemcpyHostToDevice, st));
-  drop_graph(bb);
   return THP_OK;
 }
 
@@ -361,8 +381,14 @@ thp_status release_system(Serve* sv, ServeSystem* ss) {
   sv->resident -= ss->bytes;
   thp_system_destroy(ss->s);
   ss->s = nullptr;
+  sv->residents.erase(std::find(sv->residents.begin(), sv->residents.end(), ss));
+  ss->bytes = estimate_bytes(ss->desc);
   ss->has_mu = false;
-  sv->stats.evictions += 1;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stats.evictions += 1;
+    sv->stats.resident_systems = (int)sv->residents.size();
+  }
   return THP_OK;
 }
 
@@ -376,8 +402,7 @@ thp_status admit(Serve* sv, ServeSystem* ss, const std::vector<ServeSystem*>& pi
                                      ss->desc.n, (long long)need, (long long)sv->budget);
   while (sv->resident + need > sv->budget) {
     ServeSystem* victim = nullptr;
-    for (auto& u : sv->systems) {
-      ServeSystem* c = u.get();
+    for (ServeSystem* c : sv->residents) {
       if (!c->s || c->busy || c == ss) continue;
       if (std::find(pinned.begin(), pinned.end(), c) != pinned.end()) continue;
       if (!victim || c->last_use < victim->last_use) victim = c;
@@ -394,6 +419,11 @@ thp_status admit(Serve* sv, ServeSystem* ss, const std::vector<ServeSystem*>& pi
   ss->bytes = s->bytes;
   sv->resident += ss->bytes;
   ss->has_mu = false;
+  sv->residents.push_back(ss);
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stats.resident_systems = (int)sv->residents.size();
+  }
   return THP_OK;
 }
 
@@ -547,7 +577,11 @@ thp_status run_pass(Serve* sv, int variant, int iters) {
 thp_status handle_copies(Serve* sv, std::deque<DipoleCopy*>& pending) {
   for (auto it = pending.begin(); it != pending.end();) {
     DipoleCopy* c = *it;
-    ServeSystem* ss = c->sid >= 0 && c->sid < (int)sv->systems.size() ? sv->systems[c->sid].get() : nullptr;
+    ServeSystem* ss = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      if (c->sid >= 0 && c->sid < (int)sv->systems.size()) ss = sv->systems[c->sid].get();
+    }
     if (ss && ss->busy) {
       ++it;
       continue;
@@ -645,7 +679,7 @@ thp_status pass(Serve* sv, std::vector<Request>& starting) {
   std::vector<InFlight> admitted;
   std::vector<std::pair<double*, size_t>> uploads;  // pinned buffers in flight this pass
   for (auto& q : starting) {
-    ServeSystem* ss = sv->systems[q.sid].get();
+    ServeSystem* ss = q.sys;
     bool ok = true;
     const thp_status ra = admit(sv, ss, pinned, &ok);
     if (ra == THP_ERR_HIP) return ra;
@@ -681,6 +715,7 @@ thp_status pass(Serve* sv, std::vector<Request>& starting) {
     }
     const auto tc1 = Clock::now();
     rc = ensure_lists(ss->s);  // new systems only; others keep theirs until the skin check fails
+    refresh_bytes(sv, ss);
     {
       std::lock_guard<std::mutex> lk(sv->mu);
       sv->stats.coords_ms += ms_between(tc0, tc1);
@@ -784,6 +819,7 @@ thp_status pass(Serve* sv, std::vector<Request>& starting) {
       // coordinates moved past the skin of the lists the set-up used
       thp_status rc = sync_host_coords(f.sys->s);
       if (rc == THP_OK) rc = ensure_lists(f.sys->s);
+      refresh_bytes(sv, f.sys);
       if (rc == THP_ERR_HIP) return rc;
       if (rc != THP_OK) {
         f.sys->busy = false;
@@ -831,7 +867,7 @@ void worker_main(Serve* sv) {
       int free_slots = sv->o.max_systems - (int)(sv->active.size() + sv->restarts.size());
       std::vector<int> taken;
       for (auto it = sv->queue.begin(); it != sv->queue.end() && free_slots > 0;) {
-        ServeSystem* ss = sv->systems[it->sid].get();
+        ServeSystem* ss = it->sys;
         if (ss->busy || std::find(taken.begin(), taken.end(), it->sid) != taken.end()) {
           taken.push_back(it->sid);  // keeps later requests of this system behind it
           ++it;
@@ -972,6 +1008,7 @@ thp_status thp_serve_submit_multipoles(thp_ctx* ctx, int sid, const double* x, c
     if (sv->stop) return sv->fatal != THP_OK ? sv->fatal : THP_ERR_STATE;
     if (sid < 0 || sid >= (int)sv->systems.size()) return THP_ERR_ARG;
     n = (size_t)sv->systems[sid]->desc.n;
+    q.sys = sv->systems[sid].get();
   }
   // caller-order arrays into pinned buffers on the caller's thread; the
   // worker uploads them as they are
@@ -1054,10 +1091,7 @@ thp_status thp_serve_stats_get(thp_ctx* ctx, thp_serve_stats* out) {
   std::lock_guard<std::mutex> lk(sv->mu);
   *out = sv->stats;
   out->mean_active = sv->stats.chunks ? sv->active_sum / (double)sv->stats.chunks : 0.0;
-  out->resident_bytes = sv->resident;
-  int nres = 0;
-  for (auto& s : sv->systems) nres += s->s ? 1 : 0;
-  out->resident_systems = nres;
+  out->resident_bytes = sv->resident.load();
   return THP_OK;
 }
 

@@ -152,6 +152,41 @@ module thp_bind
       type(c_ptr), value :: sys
       integer(c_int), value :: method
     end function
+    integer(c_int) function thp_system_set_frames(sys, axistype, zaxis, xaxis, yaxis, pole) &
+        bind(C, name="thp_system_set_frames")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: sys
+      integer(c_int), intent(in) :: axistype(*), zaxis(*), xaxis(*), yaxis(*)
+      real(c_double), intent(in) :: pole(*)
+    end function
+    ! device-pointer entry points ordered on the caller's stream (arrays are
+    ! device addresses, e.g. from !$acc host_data use_device; stream is a
+    ! hipStream_t, c_null_ptr for a synchronous call)
+    integer(c_int) function thp_efld0_direct_dev(sys, ef, stream) bind(C, name="thp_efld0_direct_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sys, ef, stream
+    end function
+    integer(c_int) function thp_efld0_recip_dev(sys, cphi, stream) bind(C, name="thp_efld0_recip_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sys, cphi, stream
+    end function
+    integer(c_int) function thp_tmatxb_dev_async(sys, mu, efi, dodiag, stream) bind(C, name="thp_tmatxb_dev_async")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sys, mu, efi, stream
+      integer(c_int), value :: dodiag
+    end function
+    integer(c_int) function thp_tmatxb_recip_dev(sys, mu, dipfield, stream) bind(C, name="thp_tmatxb_recip_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sys, mu, dipfield, stream
+    end function
+    integer(c_int) function thp_grid_uind_dev(sys, fuind, fuinp, qgrid, stream) bind(C, name="thp_grid_uind_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sys, fuind, fuinp, qgrid, stream
+    end function
+    integer(c_int) function thp_fphi_uind_dev(sys, qgrid, phi1, phi2, stream) bind(C, name="thp_fphi_uind_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sys, qgrid, phi1, phi2, stream
+    end function
     integer(c_int) function thp_make_system(n, box, nsol, radius, seed, x, y, z, rpole, polarity, &
                                             thole, nscaled, ik, dpu) bind(C, name="thp_make_system")
       import :: c_int, c_double, c_int64_t

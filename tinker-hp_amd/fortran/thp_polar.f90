@@ -181,16 +181,44 @@ contains
     call fatal_status(thp_set_solver(sys, int(polalg, c_int)), 'thp_set_solver')
   end subroutine
 
-  ! use_polarshortreal with ewaldshortcut (cutoffs.f:59): the short-range
-  ! solves of RESPA1's inner steps; 0 restores the full solve
   ! the polarization gradient of the last solve, dep(3,n) in kcal/mol/A
-  ! (epolar1c's dep, epolar1.f:57-376): forces with the multipole torques
-  ! carried to the frame atoms when the system has local frames
-  subroutine polar_gradient(dep)
-    real(8), intent(out) :: dep(:,:)
-    call fatal_status(thp_epolar1(sys, dep, c_null_ptr), 'thp_epolar1')
+  ! (epolar1c's dep, epolar1.f:57-376).  With local frames (polar_set_frames)
+  ! the multipole torques are carried to the frame atoms on the device, as
+  ! torque.f does; with global multipoles dep holds the forces at fixed
+  ! orientations and torque(3,n), if present, the torques on the sites
+  ! (kcal/mol), for the host to distribute.
+  subroutine polar_gradient(dep, torque)
+    real(c_double), intent(out), target, contiguous :: dep(:,:)
+    real(c_double), intent(out), target, contiguous, optional :: torque(:,:)
+    if (size(dep, 1) /= 3 .or. size(dep, 2) < n) error stop 'polar_gradient: dep must be (3, n)'
+    if (present(torque)) then
+      if (size(torque, 1) /= 3 .or. size(torque, 2) < n) error stop 'polar_gradient: torque must be (3, n)'
+      call fatal_status(thp_epolar1(sys, dep, c_loc(torque)), 'thp_epolar1')
+    else
+      call fatal_status(thp_epolar1(sys, dep, c_null_ptr), 'thp_epolar1')
+    end if
   end subroutine
 
+  ! local-frame multipoles (kmpole's ipole/pole and the axis atoms) that the
+  ! device rotates to the global frame now and after every coordinate update
+  ! (rotpole.f:24-339 with chkpole.f:20-98, each step as epolar1.f:113-116
+  ! does): axistype 0 None, 1 Z-Only, 2 Z-then-X, 3 Bisector, 4 Z-Bisect,
+  ! 5 3-Fold; frame atoms 1-based (0 where the type takes none), pole(13,n)
+  ! in the local frame
+  subroutine polar_set_frames(axistype, zaxis, xaxis, yaxis, pole)
+    integer, intent(in) :: axistype(n), zaxis(n), xaxis(n), yaxis(n)
+    real(c_double), intent(in) :: pole(13, n)
+    integer(c_int), allocatable :: at(:), zi(:), xi(:), yi(:)
+    allocate (at(n), zi(n), xi(n), yi(n))
+    at = int(axistype, c_int)
+    zi = int(zaxis - 1, c_int)
+    xi = int(xaxis - 1, c_int)
+    yi = int(yaxis - 1, c_int)
+    call fatal_status(thp_system_set_frames(sys, at, zi, xi, yi, pole), 'thp_system_set_frames')
+  end subroutine
+
+  ! use_polarshortreal with ewaldshortcut (cutoffs.f:59): the short-range
+  ! solves of RESPA1's inner steps; 0 restores the full solve
   subroutine polar_set_shortreal(shortcut)
     real(8), intent(in) :: shortcut
     call fatal_status(thp_set_shortreal(sys, real(shortcut, c_double)), 'thp_set_shortreal')

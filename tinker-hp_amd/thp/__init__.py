@@ -129,6 +129,7 @@ def load_lib(path: str | None = None) -> C.CDLL:
         "thp_system_footprint": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "thp_nblist": (C.c_int, [vp, C.c_int]),
         "thp_nblist_get": (C.c_int, [vp, C.POINTER(C.c_int64), _ip, C.c_int64]),
+        "thp_pairs_in_cut": (C.c_int, [vp, C.POINTER(C.c_int64), _ip, C.c_int64]),
         "thp_efld0_direct": (C.c_int, [vp, _dp]),
         "thp_efld0_recip": (C.c_int, [vp, _dp]),
         "thp_epolar1": (C.c_int, [vp, _dp, _dp]),
@@ -145,6 +146,12 @@ def load_lib(path: str | None = None) -> C.CDLL:
         "thp_system_set_frames": (C.c_int, [vp, _ip, _ip, _ip, _ip, _dp]),
         "thp_tmatxb_dev": (C.c_int, [vp, vp, vp, C.c_int]),
         "thp_induce_dev": (C.c_int, [vp, C.POINTER(SolveOpts), vp, _dp, C.POINTER(SolveStats)]),
+        "thp_efld0_direct_dev": (C.c_int, [vp, vp, vp]),
+        "thp_efld0_recip_dev": (C.c_int, [vp, vp, vp]),
+        "thp_tmatxb_dev_async": (C.c_int, [vp, vp, vp, C.c_int, vp]),
+        "thp_tmatxb_recip_dev": (C.c_int, [vp, vp, vp, vp]),
+        "thp_grid_uind_dev": (C.c_int, [vp, vp, vp, vp, vp]),
+        "thp_fphi_uind_dev": (C.c_int, [vp, vp, vp, vp, vp]),
         "thp_serve_start": (C.c_int, [vp, C.POINTER(ServeOpts)]),
         "thp_serve_add_system": (C.c_int, [vp, C.POINTER(SystemDesc), _ip]),
         "thp_serve_submit": (C.c_int, [vp, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_uint64)]),
@@ -385,6 +392,14 @@ class Solver:
         self._c(self.lib.thp_nblist_get(self.h, C.byref(cnt), None, 0), "nblist_get")
         ik = np.zeros((cnt.value, 2), dtype=np.int32)
         self._c(self.lib.thp_nblist_get(self.h, C.byref(cnt), _i(ik), cnt.value), "nblist_get")
+        return ik
+
+    def pairs_in_cut(self) -> np.ndarray:
+        """(i, k) rows, i < k, of the pairs the solver uses (within the cutoff)"""
+        cnt = C.c_int64(0)
+        self._c(self.lib.thp_pairs_in_cut(self.h, C.byref(cnt), None, 0), "pairs_in_cut")
+        ik = np.zeros((cnt.value, 2), dtype=np.int32)
+        self._c(self.lib.thp_pairs_in_cut(self.h, C.byref(cnt), _i(ik), cnt.value), "pairs_in_cut")
         return ik
 
     def efld0_direct(self) -> np.ndarray:
