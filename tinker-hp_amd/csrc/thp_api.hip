@@ -536,6 +536,49 @@ thp_status ensure_coef(thp_sys* s) {
 // (3*npad doubles of pinned host memory, untouched until the stream passes
 // the copies) the upload is enqueued without waiting; otherwise it is
 // synchronous.
+// the dipoles must be a PCG or Jacobi/DIIS solve (full Ewald) of the
+// current coordinates and multipoles
+thp_status epolar1_check(thp_sys* s) {
+  thp_ctx* ctx = s->ctx;
+  if (s->one.real_only) return fail(ctx, THP_ERR_ARG, "thp_epolar1: short-range solves have no gradient here");
+  if (!s->mu_valid)
+    return fail(ctx, THP_ERR_STATE, "thp_epolar1: no solve of the current coordinates and multipoles");
+  if (s->mu_short) return fail(ctx, THP_ERR_ARG, "thp_epolar1: the last solve was a short-range solve");
+  // truncated-CG dipoles are not stationary: their gradient (epolar1tcg)
+  // has terms of its own
+  if (s->mu_polalg == THP_POLALG_TCG)
+    return fail(ctx, THP_ERR_ARG, "thp_epolar1: no gradient for truncated-CG solves");
+  return THP_OK;
+}
+
+// The gradient's launch sequence on the library's stream: the multipole
+// grid's potential derivatives into cphi, the two induced grids of the
+// solved dipoles, real-space and reciprocal forces and torques into F and T
+// ([3][npad], sorted order, written for every atom), the frame forces of
+// the torques, then F scaled to the gradient and T to kcal/mol.
+thp_status epolar1_launch(thp_sys* s, double* F, double* T) {
+  thp_ctx* ctx = s->ctx;
+  hipStream_t hs = ctx->stream;
+  const Batch& b = s->one;
+  RET(field(ctx, b, s->cphi, 0, 0, kGateNone));
+  RET(spread_uind(ctx, b, V_MU, 0, kGateNone));
+  RET(convolve(ctx, b, kGateNone));
+  hipLaunchKernelGGL(k_ep1_real<4>, g1(b.n_blk), dim3(256), 0, hs, b.d_views, b.d_blk, F, T);
+  RET(launch_check(ctx, "k_ep1_real"));
+  if (s->prec == THP_PREC_FP64)
+    hipLaunchKernelGGL((k_ep1_recip<double>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T);
+  else
+    hipLaunchKernelGGL((k_ep1_recip<float>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T);
+  RET(launch_check(ctx, "k_ep1_recip"));
+  const dim3 ga((s->n + 255) / 256);
+  if (s->has_frames) {
+    hipLaunchKernelGGL(k_ep1_torque, ga, dim3(256), 0, hs, s->d_view, s->d_frames, s->d_tinv_off, s->d_tinv, T, F);
+    RET(launch_check(ctx, "k_ep1_torque"));
+  }
+  hipLaunchKernelGGL(k_ep1_finish, ga, dim3(256), 0, hs, s->n, s->npad, F, T, -1.0);
+  return launch_check(ctx, "k_ep1_finish");
+}
+
 // local-frame multipoles to the global frame on the device (rotpole)
 thp_status rotate_frames(thp_sys* s) {
   hipLaunchKernelGGL(k_rotpole, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream, s->d_view, s->d_frames,
@@ -1351,11 +1394,35 @@ thp_status thp_system_set_frames(thp_sys* s, const int* axistype, const int* zax
     const double v[10] = {r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[8], r[9], r[12]};
     for (int c = 0; c < 10; ++c) pl[(size_t)c * np + q] = v[c];
   }
+  // inverse map for the gradient: for each atom, the sites (ascending)
+  // whose frame forces reach it, the atom itself first if it has a frame
+  std::vector<int> toff((size_t)np + 1, 0), tinv;
+  {
+    std::vector<std::vector<int>> by_atom(np);
+    for (int q = 0; q < (int)np; ++q) {
+      if (fr[(size_t)4 * q] == kAxisNone) continue;
+      int tg[4] = {q, fr[(size_t)4 * q + 1], fr[(size_t)4 * q + 2], fr[(size_t)4 * q + 3]};
+      for (int k = 0; k < 4; ++k) {
+        bool dup = tg[k] < 0;
+        for (int j = 0; j < k && !dup; ++j) dup = tg[j] == tg[k];
+        if (!dup) by_atom[tg[k]].push_back(q);
+      }
+    }
+    for (int a = 0; a < (int)np; ++a) {
+      toff[a + 1] = toff[a] + (int)by_atom[a].size();
+      tinv.insert(tinv.end(), by_atom[a].begin(), by_atom[a].end());
+    }
+  }
   if (!s->d_frames) {
     RET(dalloc(s, &s->d_frames, (size_t)4 * np));
     RET(dalloc(s, &s->d_pole_local, (size_t)10 * np));
+    RET(dalloc(s, &s->d_tinv_off, (size_t)np + 1));
+    RET(dalloc(s, &s->d_tinv, (size_t)4 * np));
   }
   HIPCHK(s->ctx, hipMemcpy(s->d_frames, fr.data(), sizeof(int) * fr.size(), hipMemcpyHostToDevice));
+  HIPCHK(s->ctx, hipMemcpy(s->d_tinv_off, toff.data(), sizeof(int) * toff.size(), hipMemcpyHostToDevice));
+  if (!tinv.empty())
+    HIPCHK(s->ctx, hipMemcpy(s->d_tinv, tinv.data(), sizeof(int) * tinv.size(), hipMemcpyHostToDevice));
   HIPCHK(s->ctx, hipMemcpy(s->d_pole_local, pl.data(), sizeof(double) * pl.size(), hipMemcpyHostToDevice));
   s->has_frames = true;
   s->pme_valid = false;
@@ -1507,16 +1574,7 @@ thp_status thp_efld0_direct(thp_sys* s, double* ef) {
 
 thp_status thp_epolar1(thp_sys* s, double* dep, double* torque) {
   if (!s || !dep) return THP_ERR_ARG;
-  thp_ctx* ctx = s->ctx;
-  if (s->one.real_only) return fail(ctx, THP_ERR_ARG, "thp_epolar1: short-range solves have no gradient here");
-  // the dipoles must be a solve of the current coordinates and multipoles
-  if (!s->mu_valid)
-    return fail(ctx, THP_ERR_STATE, "thp_epolar1: no solve of the current coordinates and multipoles");
-  if (s->mu_short) return fail(ctx, THP_ERR_ARG, "thp_epolar1: the last solve was a short-range solve");
-  // truncated-CG dipoles are not stationary: their gradient (epolar1tcg)
-  // has terms of its own
-  if (s->mu_polalg == THP_POLALG_TCG)
-    return fail(ctx, THP_ERR_ARG, "thp_epolar1: no gradient for truncated-CG solves");
+  RET(epolar1_check(s));
   RET(ensure_lists(s));
   RET(ensure_pme(s));
   const size_t np = s->npad;
@@ -1524,32 +1582,9 @@ thp_status thp_epolar1(thp_sys* s, double* dep, double* torque) {
     RET(dalloc(s, &s->grad_f, 3 * np));
     RET(dalloc(s, &s->grad_t, 3 * np));
   }
-  double* F = s->grad_f;
-  double* T = s->grad_t;
-  hipStream_t hs = ctx->stream;
-  const Batch& b = s->one;
-  // the multipole grid's potential derivatives into cphi, then the two
-  // induced grids of the solved dipoles
-  RET(field(ctx, b, s->cphi, 0, 0, kGateNone));
-  RET(spread_uind(ctx, b, V_MU, 0, kGateNone));
-  RET(convolve(ctx, b, kGateNone));
-  hipLaunchKernelGGL(k_ep1_real<4>, g1(b.n_blk), dim3(256), 0, hs, b.d_views, b.d_blk, F, T);
-  RET(launch_check(ctx, "k_ep1_real"));
-  if (s->prec == THP_PREC_FP64)
-    hipLaunchKernelGGL((k_ep1_recip<double>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T);
-  else
-    hipLaunchKernelGGL((k_ep1_recip<float>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T);
-  RET(launch_check(ctx, "k_ep1_recip"));
-  const dim3 ga((s->n + 255) / 256);
-  if (s->has_frames) {
-    hipLaunchKernelGGL(k_ep1_torque, ga, dim3(256), 0, hs, s->d_view, s->d_frames, T, F);
-    RET(launch_check(ctx, "k_ep1_torque"));
-  }
-  hipLaunchKernelGGL(k_ep1_finish, ga, dim3(256), 0, hs, s->n, np, F, T);
-  RET(launch_check(ctx, "k_ep1_finish"));
-  RET(gather_out(s, dep, F, 3));
-  for (size_t q = 0; q < (size_t)s->n * 3; ++q) dep[q] = -dep[q];  // gradient = -force
-  if (torque) RET(gather_out(s, torque, T, 3));
+  RET(epolar1_launch(s, s->grad_f, s->grad_t));
+  RET(gather_out(s, dep, s->grad_f, 3));
+  if (torque) RET(gather_out(s, torque, s->grad_t, 3));
   return THP_OK;
 }
 

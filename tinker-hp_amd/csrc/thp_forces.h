@@ -367,24 +367,18 @@ __global__ void __launch_bounds__(64) k_ep1_recip(const SysView* views, const Wo
 // g_u . du with g_z = T x z - (T . z)(u . z) y / |u_perp|, g_u =
 // (T . z) y / |u_perp|, dz = (1 - z z^T) dv / |v|; v and u are sums of
 // bond vectors or unit bond vectors (a unit vector e = unit(r_a - r_i)
-// passes g on as (1 - e e^T) g / |r_a - r_i|).  Frame atoms of several
-// sites are reached by several threads: FP64 atomic adds.
-__device__ __forceinline__ void add_force(double* F, size_t np, int a, V3 g) {
-  atomicAdd(F + a, g.x);
-  atomicAdd(F + np + a, g.y);
-  atomicAdd(F + 2 * np + a, g.z);
-}
+// passes g on as (1 - e e^T) g / |r_a - r_i|).  The site itself takes minus
+// the sum of what it passes on.  frame_forces lists site s's contributions
+// as (atom, force) in a fixed order, at most 5.
+constexpr int kFrameContrib = 5;
 
-__global__ void __launch_bounds__(256) k_ep1_torque(const SysView* views, const int* frames, const double* T,
-                                                    double* F) {
-  const SysView& sv = views[0];
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= sv.n) return;
+__device__ __forceinline__ int frame_forces(const SysView& sv, const int* frames, const double* T, int s,
+                                            int* atom, V3* g_out) {
   const size_t np = sv.npad;
-  const int4 fr = reinterpret_cast<const int4*>(frames)[i];
+  const int4 fr = reinterpret_cast<const int4*>(frames)[s];
   const int axt = fr.x, iz = fr.y, ix = fr.z, iy = fr.w;
-  if (axt == kAxisNone) return;
-  const V3 ri = v3(sv.x[i], sv.y[i], sv.z[i]);
+  if (axt == kAxisNone) return 0;
+  const V3 ri = v3(sv.x[s], sv.y[s], sv.z[s]);
   auto pos = [&](int k) { return v3(sv.x[k], sv.y[k], sv.z[k]); };
   auto unit = [](V3 a) { return (1.0 / sqrt(dot(a, a))) * a; };
   // frame (as k_rotpole builds it) and the v, u of its axis type
@@ -415,23 +409,25 @@ __global__ void __launch_bounds__(256) k_ep1_torque(const SysView* views, const 
   const double nup = sqrt(dot(up, up));
   const V3 x = (1.0 / nup) * up;
   const V3 y = cross(z, x);
-  const V3 Tq = v3(T[i], T[np + i], T[2 * np + i]);
+  const V3 Tq = v3(T[s], T[np + s], T[2 * np + s]);
   const double tz = dot(Tq, z);
   const V3 gz = cross(Tq, z) - (tz * dot(u, z) / nup) * y;
   const V3 gu = (tz / nup) * y;
   const V3 gv = (1.0 / sqrt(dot(v, v))) * (gz - dot(z, gz) * z);
-  // g on unit(r_a - r_i) -> forces on a and i
+  int m = 0;
   V3 fi = v3(0, 0, 0);
   auto via_unit = [&](int a, V3 g) {
     const V3 d = pos(a) - ri;
     const double r = sqrt(dot(d, d));
     const V3 e = (1.0 / r) * d;
     const V3 h = (1.0 / r) * (g - dot(e, g) * e);
-    add_force(F, np, a, h);
+    atom[m] = a;
+    g_out[m++] = h;
     fi = fi - h;
   };
   auto via_bond = [&](int a, V3 g) {
-    add_force(F, np, a, g);
+    atom[m] = a;
+    g_out[m++] = g;
     fi = fi - g;
   };
   if (axt == kAxisZOnly) {
@@ -453,15 +449,41 @@ __global__ void __launch_bounds__(256) k_ep1_torque(const SysView* views, const 
     via_unit(iy, gv);
     via_unit(ix, gu);
   }
-  add_force(F, np, i, fi);
+  atom[m] = s;
+  g_out[m++] = fi;
+  return m;
+}
+
+// One thread per atom a gathers the frame forces of every site whose frame
+// holds a (tinv[tinv_off[a] .. tinv_off[a + 1]), sites in ascending order,
+// a itself included when it has a frame) and adds them to F[a]: a fixed
+// order of sums, no atomics, the same bits on every run.
+__global__ void __launch_bounds__(256) k_ep1_torque(const SysView* views, const int* frames, const int* tinv_off,
+                                                    const int* tinv, const double* T, double* F) {
+  const SysView& sv = views[0];
+  const int a = blockIdx.x * 256 + threadIdx.x;
+  if (a >= sv.n) return;
+  const size_t np = sv.npad;
+  V3 acc = v3(0, 0, 0);
+  for (int e = tinv_off[a]; e < tinv_off[a + 1]; ++e) {
+    int atom[kFrameContrib];
+    V3 g[kFrameContrib];
+    const int m = frame_forces(sv, frames, T, tinv[e], atom, g);
+    for (int q = 0; q < m; ++q)
+      if (atom[q] == a) acc = acc + g[q];
+  }
+  F[a] += acc.x;
+  F[np + a] += acc.y;
+  F[2 * np + a] += acc.z;
 }
 
 // f/2 on the forces and torques
-__global__ void __launch_bounds__(256) k_ep1_finish(int n, size_t np, double* F, double* T) {
+// (sf = +1: forces; sf = -1: the gradient)
+__global__ void __launch_bounds__(256) k_ep1_finish(int n, size_t np, double* F, double* T, double sf) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   for (int c = 0; c < 3; ++c) {
-    F[(size_t)c * np + i] *= 0.5 * kCoulombFp;
+    F[(size_t)c * np + i] *= sf * 0.5 * kCoulombFp;
     T[(size_t)c * np + i] *= 0.5 * kCoulombFp;
   }
 }

@@ -272,6 +272,8 @@ struct thp_sys {
   // local-frame multipoles (thp_system_set_frames), rotated on the device
   int* d_frames = nullptr;         // [npad] x (axis type, z, x, y), sorted slots
   double* d_pole_local = nullptr;  // [10][npad], the rpole layout
+  int* d_tinv_off = nullptr;       // [npad + 1] CSR offsets of d_tinv
+  int* d_tinv = nullptr;           // per atom, the sites whose frame forces reach it
   bool has_frames = false;
   // truncated conjugate gradient (thp_set_tcg; initprm.f:447-455 defaults)
   int tcg_order = 2, tcg_prec = 1, tcg_guess = 0, tcg_peek = 1;
