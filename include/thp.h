@@ -272,6 +272,22 @@ thp_status thp_grid_uind_dev(thp_sys* sys, const double* fuind /* [n][3] */, con
 thp_status thp_fphi_uind_dev(thp_sys* sys, const double* qgrid, double* fdip_phi1 /* [n][3] */,
                              double* fdip_phi2 /* [n][3] */, void* stream);
 
+/* No counterpart in the reference: thp_epolar1 for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, hosts that forbid hipMalloc).  dep and torque (optional) are device
+ * addresses ([n][3], caller order, the values thp_epolar1 returns), ordered
+ * on `stream` like the entry points above.  The forces and torques before
+ * their gather live in a file-scope device array of 6 * THP_NOALLOC_MAX_ATOMS
+ * doubles (6 MiB) that each device holds until the process ends, so the call
+ * allocates nothing: THP_ERR_ARG for systems of more than
+ * THP_NOALLOC_MAX_ATOMS atoms (rounded up to 64), THP_ERR_STATE unless the
+ * last solve of the current coordinates left its lists and B-spline tables
+ * in place (as thp_induce does).  Every call on a device shares that
+ * device's array: calls that overlap on one device (two streams, two
+ * threads, two contexts) race. */
+#define THP_NOALLOC_MAX_ATOMS 131072
+thp_status thp_epolar1_noalloc(thp_sys* sys, double* dep, double* torque, void* stream);
+
 /* ------------------------------------------------------------ measurement */
 /* Measurement hook, no reference counterpart: average device time (ms) of
  * one launch of a path kernel over `reps` back-to-back launches, timed with

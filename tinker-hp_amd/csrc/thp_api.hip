@@ -17,6 +17,14 @@
 
 using namespace thp;
 
+// Scratch of thp_epolar1_noalloc: the forces and torques before they are
+// scaled and gathered ([6][npad] doubles, npad <= THP_NOALLOC_MAX_ATOMS),
+// as a file-scope device array, so that the entry point allocates nothing.
+// Each device holds its own copy until the process ends; host code uses
+// only the per-device address thp_init resolves (ctx->scratch), never the
+// array's name.
+static __device__ __align__(256) double scratch[(size_t)6 * THP_NOALLOC_MAX_ATOMS];
+
 thp_status thp::fail(thp_ctx* ctx, thp_status code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -921,6 +929,17 @@ thp_status thp_init(thp_ctx** out, int device) {
       return fail(ctx, THP_ERR_HIP, "cannot create the side stream of device %d", device);
     }
   }
+  {
+    // the no-allocation entry point's scratch on this device (the device is
+    // current here)
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      *out = ctx;
+      return fail(ctx, THP_ERR_HIP, "cannot resolve the scratch array on device %d: %s", device, hipGetErrorString(e));
+    }
+    ctx->scratch = static_cast<double*>(ptr);
+  }
   if (hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming) != hipSuccess) {
     *out = ctx;
@@ -1804,6 +1823,30 @@ thp_status thp_fphi_uind_dev(thp_sys* s, const double* qgrid, double* phi1, doub
                      s->iperm_dev, s->n, s->npad, 3);
   RET(launch_check(s->ctx, "k_gather_out"));
   return order_out(s->ctx, cs);
+}
+
+thp_status thp_epolar1_noalloc(thp_sys* s, double* dep, double* torque, void* stream) {
+  THP_NEED_DEV_ARGS(dep);
+  thp_ctx* ctx = s->ctx;
+  if (s->npad > THP_NOALLOC_MAX_ATOMS)
+    return fail(ctx, THP_ERR_ARG, "thp_epolar1_noalloc: %d atoms (%d padded) exceed the scratch of %d atoms", s->n,
+                s->npad, THP_NOALLOC_MAX_ATOMS);
+  if (!ctx->scratch) return fail(ctx, THP_ERR_STATE, "thp_epolar1_noalloc: no scratch on device %d", ctx->device);
+  RET(epolar1_check(s));
+  // a solve of the current coordinates left the lists and B-spline tables
+  // in place; rebuilding them here would allocate
+  if (!s->list_valid || !s->tight_valid || !s->pme_valid)
+    return fail(ctx, THP_ERR_STATE, "thp_epolar1_noalloc: lists or B-spline tables are not current");
+  hipStream_t cs = static_cast<hipStream_t>(stream);
+  RET(order_in(ctx, cs));
+  double* F = ctx->scratch;
+  double* T = F + (size_t)3 * s->npad;
+  RET(epolar1_launch(s, F, T));
+  hipLaunchKernelGGL(k_gather_out, atoms_grid(s), dim3(256), 0, ctx->stream, dep, F, s->iperm_dev, s->n, s->npad, 3);
+  if (torque)
+    hipLaunchKernelGGL(k_gather_out, atoms_grid(s), dim3(256), 0, ctx->stream, torque, T, s->iperm_dev, s->n, s->npad, 3);
+  RET(launch_check(ctx, "k_gather_out"));
+  return order_out(ctx, cs);
 }
 
 #undef THP_NEED_DEV_ARGS

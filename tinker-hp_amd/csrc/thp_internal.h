@@ -236,6 +236,9 @@ struct thp_ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // ordering of the stream-ordered entry points against the caller's stream
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  // this device's address of the file-scope scratch array of the entry point
+  // that allocates nothing (thp_epolar1_noalloc), resolved by thp_init
+  double* scratch = nullptr;
   std::set<thp_sys*> systems;  // live systems; thp_finalize releases what is left
 };
 

@@ -187,6 +187,12 @@ module thp_bind
       import :: c_int, c_ptr
       type(c_ptr), value :: sys, qgrid, phi1, phi2, stream
     end function
+    ! the gradient without device allocation (scratch in a file-scope
+    ! device array of libthp; at most THP_NOALLOC_MAX_ATOMS = 131072 atoms)
+    integer(c_int) function thp_epolar1_noalloc(sys, dep, torque, stream) bind(C, name="thp_epolar1_noalloc")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sys, dep, torque, stream
+    end function
     integer(c_int) function thp_make_system(n, box, nsol, radius, seed, x, y, z, rpole, polarity, &
                                             thole, nscaled, ik, dpu) bind(C, name="thp_make_system")
       import :: c_int, c_double, c_int64_t
