@@ -19,7 +19,7 @@ timed with HIP events on the library's stream; cpu_baseline times the FP64
 CPU oracle on the host cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serve] [--no-stream] [--no-dhfr2] [--no-large]
-       python bench.py --serve-only | --kernel-probe    (profiling aids, no JSON line)
+       python bench.py --serve-only | --kernel-probe | --stream-only RATE   (profiling aids, no JSON line)
 """
 import argparse
 import json
@@ -40,16 +40,27 @@ def log(msg):
 
 
 def time_leg(ctx, thp, system, precision, steps, warmup, poleps):
+    """`steps` back-to-back solves after `warmup` untimed ones; returns the
+    solver, the mean time per solve, the last result and the per-solve
+    wall times (each solve returns with its results on the host)."""
     solver = thp.Solver(ctx, system, precision)
     solver.nblist()
     for _ in range(warmup):
         solver.newinduce_pme2(poleps=poleps)
     t0 = time.perf_counter()
     last = None
+    each = []
     for _ in range(steps):
+        t1 = time.perf_counter()
         last = solver.newinduce_pme2(poleps=poleps)
+        each.append(time.perf_counter() - t1)
     dt = time.perf_counter() - t0
-    return solver, dt / steps, last
+    return solver, dt / steps, last, each
+
+
+def pct_ms(v, q):
+    import numpy as np
+    return round(1e3 * float(np.percentile(np.asarray(v), q)), 4)
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "matvec_traffic_water23559.md")
@@ -98,22 +109,63 @@ def measured_traffic(nsys):
     return None
 
 
-def cpu_baseline(system, poleps, seconds=10.0):
-    """FP64 CPU oracle on the host cores: repeated full solves (field, guess,
-    PCG to poleps, energy) of the same system for about `seconds`."""
+def host_cores():
+    """Host threads this process may use: the CPU set it may run on, capped
+    by OMP_NUM_THREADS where that is set (a GPU box grants one GPU's share of
+    its cores that way)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(avail, int(omp))) if omp.isdigit() and int(omp) > 0 else avail, avail
+
+
+def oracle_phases(orc, mu):
+    """Wall time (ms) of each phase of a solve in the FP64 oracle: the
+    permanent field's direct and reciprocal parts and one real-space and one
+    reciprocal matvec (the lists are built beforehand and timed apart)."""
+    out = {}
+    for name, fn in (("efld0_direct", orc.efld0_direct), ("efld0_recip", orc.efld0_recip),
+                     ("matvec_real", lambda: orc.tmatxb_pme(mu)), ("matvec_recip", lambda: orc.tmatxbrecip(mu))):
+        t0 = time.perf_counter()
+        fn()
+        out[name] = round(1e3 * (time.perf_counter() - t0), 2)
+    return out
+
+
+def cpu_baseline(system, poleps, seconds=8.0):
+    """FP64 CPU oracle on the host: on every core this process may use,
+    repeated full solves (field, guess, PCG to poleps, energy) of the same
+    system for about `seconds`; then one solve on one core.  Both with the
+    per-phase times of a solve and the list build timed apart."""
     from oracle import oracle as O
-    cores = max(1, min(16, os.cpu_count() or 1))
-    O.set_threads(cores)
-    orc = O.Oracle(system)
-    orc.nblist_pairs()
-    t0 = time.perf_counter()
-    count = 0
+    cores, avail = host_cores()
+    runs = {}
     res = None
-    while count == 0 or time.perf_counter() - t0 < seconds:
-        res = orc.induce(poleps=poleps)
-        count += 1
-    dt = time.perf_counter() - t0
-    return res, dt / count, cores, count
+    for threads in (cores, 1):
+        O.set_threads(threads)
+        orc = O.Oracle(system)
+        t0 = time.perf_counter()
+        orc.nblist_pairs()
+        t_list = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        count = 0
+        while count == 0 or (threads > 1 and time.perf_counter() - t0 < seconds):
+            r = orc.induce(poleps=poleps)
+            count += 1
+        dt = (time.perf_counter() - t0) / count
+        res = res or r
+        phases = oracle_phases(orc, r["mu"])
+        phases["nblist"] = round(1e3 * t_list, 2)
+        phases["solve"] = round(1e3 * dt, 2)
+        phases["cg_vectors_and_rest"] = round(
+            phases["solve"] - phases["efld0_direct"] - phases["efld0_recip"]
+            - r["niter"] * (phases["matvec_real"] + phases["matvec_recip"]), 2)
+        runs[threads] = dict(solves_per_s=round(1.0 / dt, 4), solves=count, niter=r["niter"], phases_ms=phases)
+        del orc
+    O.set_threads(cores)
+    return res, runs, cores, avail
 
 
 def main():
@@ -133,6 +185,8 @@ def main():
                     help="only launch the path kernels and a copy of known size (for rocprofv3 --pmc runs)")
     ap.add_argument("--serve-only", action="store_true",
                     help="only run the serving leg and print its figures (for kernel profiles of the batched loop)")
+    ap.add_argument("--stream-only", type=float, default=0.0, metavar="RATE",
+                    help="only run the Poisson stream leg at RATE requests/s and print its figures")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -154,38 +208,74 @@ def main():
         ctx.close()
         return
 
+    if args.stream_only > 0:
+        import thp.serve as tserve
+        ctx = thp.Context(0)
+        stream = tserve.poisson_stream(ctx, n_requests=args.stream_requests, rate_per_s=args.stream_only,
+                                       poleps=args.poleps)
+        log(f"bench: stream {stream}")
+        ctx.close()
+        return
+
     system = thp.water_23559()
     log(f"bench: workload {system.name} n={system.n} box={system.box[0]} nfft={system.nfft}")
     ctx = thp.Context(0)
 
     legs = {}
-    plain, t_plain, r_plain = time_leg(ctx, thp, system, thp.THP_PREC_FP64, args.steps, args.warmup, args.poleps)
-    legs["plain"] = dict(solves_per_s=1.0 / t_plain, ms_per_solve=1e3 * t_plain, niter=r_plain[2].niter, ep=r_plain[1])
+    plain, t_plain, r_plain, e_plain = time_leg(ctx, thp, system, thp.THP_PREC_FP64, args.steps, args.warmup,
+                                                args.poleps)
+    legs["plain"] = dict(solves_per_s=1.0 / t_plain, ms_per_solve=1e3 * t_plain, niter=r_plain[2].niter,
+                         ep=r_plain[1], p50_ms=pct_ms(e_plain, 50), p99_ms=pct_ms(e_plain, 99))
     log(f"bench: plain fp64 {legs['plain']}")
-    mixed, t_mixed, r_mixed = time_leg(ctx, thp, system, thp.THP_PREC_MIXED, args.steps, args.warmup, args.poleps)
-    legs["mixed"] = dict(solves_per_s=1.0 / t_mixed, ms_per_solve=1e3 * t_mixed, niter=r_mixed[2].niter, ep=r_mixed[1])
+    mixed, t_mixed, r_mixed, e_mixed = time_leg(ctx, thp, system, thp.THP_PREC_MIXED, args.steps, args.warmup,
+                                                args.poleps)
+    legs["mixed"] = dict(solves_per_s=1.0 / t_mixed, ms_per_solve=1e3 * t_mixed, niter=r_mixed[2].niter,
+                         ep=r_mixed[1], p50_ms=pct_ms(e_mixed, 50), p99_ms=pct_ms(e_mixed, 99))
     log(f"bench: mixed {legs['mixed']}")
 
+    # list rebuilds, timed apart from the solves: the solver's lists (block
+    # candidates, block-union lists with their host count reads, and the
+    # field pass that compacts them) and the reference-shaped Verlet list
+    rebuild = {}
+    for k, name in ((4, "solver_lists_ms"), (5, "verlet_nblist_ms")):
+        ms_k, _ = mixed.time_kernel(k, reps=5)
+        rebuild[name] = round(ms_k, 4)
+    log(f"bench: list rebuilds {rebuild}")
+
     serve = None
+    serve_last = None
     if not args.no_serve:
         try:
             import thp.serve as tserve
             serve = tserve.bench_serving(ctx, system, steps=max(args.steps, 8), warmup=args.warmup,
                                          poleps=args.poleps, copies=args.copies, iters_per_graph=4,
-                                         use_graphs=1)
+                                         use_graphs=1, keep_last=not args.no_cpu)
+            serve_last = serve.pop("last", None)
             legs["serve"] = serve
             log(f"bench: serve {serve}")
         except (ImportError, AttributeError, thp.ThpError) as e:
             log(f"bench: serving leg unavailable: {e}")
 
-    # Poisson stream of mixed-size systems at 1.5x the plain one-system rate
+    # Poisson stream of mixed-size systems at 1.5x the plain one-system
+    # rate, then shorter streams over the same systems at higher rates (the
+    # loop's capacity: the highest offered rate it keeps up with)
     stream = None
     if not args.no_serve and not args.no_stream:
         try:
             import thp.serve as tserve
             rate = 1.5 * legs["plain"]["solves_per_s"]
             stream = tserve.poisson_stream(ctx, n_requests=args.stream_requests, rate_per_s=rate,
-                                           poleps=args.poleps)
+                                           poleps=args.poleps, more_rates=[rate * f for f in (2, 3, 4, 6)],
+                                           more_requests=max(200, args.stream_requests // 3))
+            kept = [r for r in stream["rates"] if r["failed"] == 0 and r["solves_per_s"] >= 0.97 * r["offered_rate"]]
+            top = max(kept, key=lambda r: r["offered_rate"]) if kept else None
+            stream["capacity"] = dict(offered_rate=round(top["offered_rate"], 1),
+                                      solves_per_s=round(top["solves_per_s"], 1), p50_ms=top["p50_ms"],
+                                      p99_ms=top["p99_ms"], mean_active=round(top["mean_active"], 2)) if top else None
+            stream["rates"] = [dict(offered_rate=round(r["offered_rate"], 1), solves_per_s=round(r["solves_per_s"], 1),
+                                    p50_ms=r["p50_ms"], p99_ms=r["p99_ms"], failed=r["failed"],
+                                    mean_active=round(r["mean_active"], 2), requests=r["requests"])
+                               for r in stream["rates"]]
             log(f"bench: stream {stream}")
         except (ImportError, AttributeError, thp.ThpError) as e:
             log(f"bench: stream leg unavailable: {e}")
@@ -196,7 +286,7 @@ def main():
     if not args.no_dhfr2 and os.path.exists(os.path.join(gold, "dhfr2.xyz.gz")):
         import thp.tinker as tinker
         ds = tinker.load(os.path.join(gold, "dhfr2.xyz.gz"), os.path.join(gold, "dhfr2.key"))
-        dsolver, t_d, r_d = time_leg(ctx, thp, ds, thp.THP_PREC_MIXED, args.steps, args.warmup, args.poleps)
+        dsolver, t_d, r_d, _ = time_leg(ctx, thp, ds, thp.THP_PREC_MIXED, args.steps, args.warmup, args.poleps)
         dhfr2 = dict(solves_per_s=round(1.0 / t_d, 3), ms_per_solve=round(1e3 * t_d, 4), niter=r_d[2].niter,
                      ep=r_d[1], atoms=ds.n, frames="local (rotated on the device)", aewald=round(ds.aewald, 6))
         # the polarization gradient of the last solve (forces with the
@@ -216,41 +306,60 @@ def main():
         large = {}
         for name, build in (("solute-96000", thp.solute_96000), ("water-1068000", thp.water_1068000)):
             ls = build()
-            lsolver, t_l, r_l = time_leg(ctx, thp, ls, thp.THP_PREC_MIXED, 3, 1, args.poleps)
+            lsolver, t_l, r_l, _ = time_leg(ctx, thp, ls, thp.THP_PREC_MIXED, 3, 1, args.poleps)
             large[name] = dict(solves_per_s=round(1.0 / t_l, 3), ms_per_solve=round(1e3 * t_l, 3),
                                niter=r_l[2].niter, nfft=list(ls.nfft))
             lsolver.close()
             log(f"bench: {name} {large[name]}")
 
     # live roofline of the dominant kernel (mixed real-space matvec): one
-    # launch over the serving batch when that leg is the fastest
+    # launch over the serving batch when that leg is the fastest.  Its
+    # algorithmic bound is FP32 vector arithmetic (no dense contraction for
+    # MFMA); the measured HBM rate is reported beside it.
     best = max(legs, key=lambda k: legs[k]["solves_per_s"])
     batch = [mixed]
     if best == "serve":
         batch += [thp.Solver(ctx, system, thp.THP_PREC_MIXED) for _ in range(args.copies - 1)]
     ms, pairs = thp.time_kernel_batch(batch, 0, reps=30)
-    for s in batch[1:]:
-        s.close()
+    for s_ in batch[1:]:
+        s_.close()
     flops = FLOP_PER_PAIR_TMATXB * pairs
     achieved = flops / (ms * 1e-3) / 1e12
-    roofline = dict(bound="mfma", achieved=round(achieved, 4), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=round(achieved / FP32_PEAK_TFLOPS, 5), traffic=measured_traffic(len(batch)),
+    traffic = measured_traffic(len(batch))
+    roofline = dict(bound="valu", achieved=round(achieved, 4), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=round(achieved / FP32_PEAK_TFLOPS, 5), traffic=traffic,
                     kernel="k_tmatxb_cl<float,S> (real-space matvec)", systems_per_launch=len(batch),
                     ms_per_launch=round(ms, 5), pairs_in_cut=int(pairs), flop_per_pair=FLOP_PER_PAIR_TMATXB,
-                    compute="FP32 VALU (no MFMA: no dense contraction)")
+                    compute="FP32 VALU (no MFMA: no dense contraction)",
+                    hbm_tb_per_s_measured=round(traffic / (ms * 1e-3) / 1e12, 3) if traffic else None)
     log(f"bench: roofline {roofline}")
 
     cpu = None
     err = {}
     if not args.no_cpu:
-        res, dt, cores, count = cpu_baseline(system, args.poleps)
-        cpu = dict(value=round(1.0 / dt, 4), unit="solves/s", cores=cores, kind="port",
-                   sample=f"{count} FP64 solves of {system.name} (oracle, OpenMP {cores} threads, "
-                          f"list build excluded), {dt:.3f} s each")
+        res, runs, cores, avail = cpu_baseline(system, args.poleps)
+        allc, one = runs[cores], runs[1]
+        cpu = dict(value=allc["solves_per_s"], unit="solves/s", cores=cores, kind="port",
+                   sample=f"{allc['solves']} FP64 solves of {system.name} (the oracle, OpenMP {cores} threads: "
+                          f"the CPU share this process may use, of {avail} it can see; list build timed apart), "
+                          f"then 1 solve on 1 core",
+                   phases_ms=allc["phases_ms"],
+                   one_core=dict(solves_per_s=one["solves_per_s"], phases_ms=one["phases_ms"]))
+        from oracle import oracle as O
         for name, r in (("plain", r_plain), ("mixed", r_mixed)):
             err[name] = dict(
                 ep_rel=abs(r[1] - res["ep"]) / abs(res["ep"]),
                 mu_rms_debye=float(np.sqrt(np.mean((r[0] - res["mu"]) ** 2)) * thp.DEBYE))
+        if serve_last:
+            # every serving copy's last solve (warm-started, jittered
+            # coordinates) against the oracle at those coordinates
+            O.set_threads(cores)
+            ep_rel, mu_rms = [], []
+            for c in serve_last:
+                ref = O.Oracle(system.with_coords(*c["xyz"])).induce(poleps=args.poleps)
+                ep_rel.append(abs(c["ep"] - ref["ep"]) / abs(ref["ep"]))
+                mu_rms.append(float(np.sqrt(np.mean((c["mu"] - ref["mu"]) ** 2)) * thp.DEBYE))
+            err["serve"] = dict(copies=len(serve_last), ep_rel_max=max(ep_rel), mu_rms_debye_max=max(mu_rms))
         log(f"bench: cpu {cpu} errors {err}")
 
     value = legs[best]["solves_per_s"]
@@ -276,6 +385,16 @@ def main():
             "speedup_vs_plain": round(value / legs["plain"]["solves_per_s"], 3),
             "p50_ms": serve["p50_ms"] if serve else None,
             "p99_ms": serve["p99_ms"] if serve else None,
+            "plain_p50_ms": legs["plain"]["p50_ms"],
+            "plain_p99_ms": legs["plain"]["p99_ms"],
+            "mixed_p50_ms": legs["mixed"]["p50_ms"],
+            "mixed_p99_ms": legs["mixed"]["p99_ms"],
+            "stream_p50_ms": stream["p50_ms"] if stream else None,
+            "stream_p99_ms": stream["p99_ms"] if stream else None,
+            "list_rebuild": rebuild,
+            "precision": "mixed: FP32 pair and grid arithmetic; per-atom fields, CG scalars and E_pol in FP64, "
+                         "except that the real-space matvec sums each pack of 8 list entries in FP32 before "
+                         "its FP64 add",
             "serve": serve,
             "stream": stream,
             "dhfr2_mixed_one_system": dhfr2,

@@ -90,7 +90,7 @@ def _pct(v, q):
 
 def bench_serving(ctx, system: System, steps: int = 20, warmup: int = 2, poleps: float = 1e-8, copies: int = 8,
                   jitter: float = 0.01, warm_start: int = 1, iters_per_graph: int = 4, use_graphs: int = 1,
-                  variants: int = 4, depth: int = 2, seed: int = 11) -> dict:
+                  variants: int = 4, depth: int = 2, seed: int = 11, keep_last: bool = False) -> dict:
     """Closed loop over `copies` resident systems with the geometry of
     `system` (each registered as its own system): every system keeps `depth`
     requests outstanding (they run one after the other, each warm-started
@@ -100,7 +100,8 @@ def bench_serving(ctx, system: System, steps: int = 20, warmup: int = 2, poleps:
     the timed region.  `warmup` rounds per system are not timed.  Returns
     solves/s over the timed requests and latency percentiles (submit ->
     result, so with depth 2 a request's latency includes the solve ahead of
-    it)."""
+    it).  keep_last adds "last": per copy, the coordinates, E_pol and
+    dipoles of its last solve (for checks against the oracle)."""
     rng = np.random.default_rng(seed)
     srv = Server(ctx, max_systems=copies, iters_per_graph=iters_per_graph, poleps=poleps,
                  warm_start=warm_start, use_graphs=use_graphs)
@@ -109,11 +110,13 @@ def bench_serving(ctx, system: System, steps: int = 20, warmup: int = 2, poleps:
         coords = [[tuple(np.ascontiguousarray(v + rng.normal(0.0, jitter, system.n))
                          for v in (system.x, system.y, system.z)) for _ in range(variants)] for _ in sids]
         nxt = [0] * copies
+        last_ticket, last_ep = [None] * copies, [None] * copies
 
         def submit(k):
             xyz = coords[k][nxt[k] % variants]
             nxt[k] += 1
-            return srv.submit(sids[k], *xyz)
+            last_ticket[k] = srv.submit(sids[k], *xyz)
+            return last_ticket[k]
 
         # untimed rounds: device data creation, lists, plans, tuning
         for _ in range(max(1, warmup)):
@@ -141,16 +144,22 @@ def bench_serving(ctx, system: System, steps: int = 20, warmup: int = 2, poleps:
             lat.append(r.latency_ms)
             nit.append(r.niter)
             eps.append(r.ep)
+            if r.ticket == last_ticket[k]:
+                last_ep[k] = r.ep
             if issued < total:
                 owner[submit(k)] = k
                 issued += 1
         dt = time.perf_counter() - t0
         st = srv.stats()
         d = _stats_delta(st0, st)
-        return dict(solves_per_s=total / dt, ms_per_solve=1e3 * dt / total, solves=total, copies=copies,
-                    niter=float(np.mean(nit)), ep=float(eps[0]), p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99),
-                    pair_segs=st.pair_segs, warm_start=warm_start, iters_per_graph=iters_per_graph,
-                    graphs=int(st.graph_launches > 0), depth=depth, **d)
+        out = dict(solves_per_s=total / dt, ms_per_solve=1e3 * dt / total, solves=total, copies=copies,
+                   niter=float(np.mean(nit)), ep=float(eps[0]), p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99),
+                   pair_segs=st.pair_segs, warm_start=warm_start, iters_per_graph=iters_per_graph,
+                   graphs=int(st.graph_launches > 0), depth=depth, **d)
+        if keep_last:
+            out["last"] = [dict(xyz=coords[k][(nxt[k] - 1) % variants], ep=last_ep[k], mu=srv.dipoles(sids[k]))
+                           for k in range(copies)]
+        return out
     finally:
         srv.stop()
 
@@ -167,14 +176,17 @@ def _stats_delta(a, b):
 
 
 def poisson_stream(ctx, n_requests=2000, rate_per_s=100.0, resubmits=10, n_min=500, n_max=30000, max_systems=16,
-                   poleps=1e-8, seed=5, jitter=0.01, warm_start=1, iters_per_graph=4, use_graphs=1):
+                   poleps=1e-8, seed=5, jitter=0.01, warm_start=1, iters_per_graph=4, use_graphs=1,
+                   more_rates=(), more_requests=None):
     """Poisson arrivals of solves of independent mixed-size systems: sizes
     log-uniform in [n_min, n_max], 0-30% solute; each system is requested
     `resubmits` times at coordinates perturbed by N(0, jitter).  Systems are
     registered and solved once before the stream (device data resident, as
     for a server that has seen them), and every request's coordinates are
     generated before the clock starts.  Returns throughput and latency
-    percentiles (submit -> result)."""
+    percentiles (submit -> result) at rate_per_s; with more_rates, the same
+    systems then take one stream of `more_requests` requests at each of those
+    rates, reported under "rates" (the capacity sweep)."""
     rng = np.random.default_rng(seed)
     n_sys = max(1, n_requests // resubmits)
     srv = Server(ctx, max_systems=max_systems, iters_per_graph=iters_per_graph, poleps=poleps,
@@ -188,38 +200,44 @@ def poisson_stream(ctx, n_requests=2000, rate_per_s=100.0, resubmits=10, n_min=5
         for r in [srv.poll(t) for t in [srv.submit(sid) for sid in sids]]:
             if r.status != THP_OK:
                 raise ThpError(r.status, "stream warm-up solve")
-        order = np.repeat(np.arange(n_sys), resubmits)
-        rng.shuffle(order)
-        order = order[:n_requests]
-        coords = [tuple(np.ascontiguousarray(v + rng.normal(0.0, jitter, systems[k].n))
-                        for v in (systems[k].x, systems[k].y, systems[k].z)) for k in order]
-        gaps = rng.exponential(1.0 / rate_per_s, len(order))
-        tickets = []
-        st0 = srv.stats()
-        t0 = time.perf_counter()
-        t_next = t0
-        for q, k in enumerate(order):
-            t_next += gaps[q]
-            delay = t_next - time.perf_counter()
-            if delay > 0:
-                time.sleep(delay)
-            tickets.append(srv.submit(sids[k], *coords[q]))
-        t_sub = time.perf_counter() - t0
-        lat, bad, atoms, nit = [], 0, 0, []
-        for t in tickets:
-            r = srv.poll(t)
-            if r.status != THP_OK:
-                bad += 1
-            lat.append(r.latency_ms)
-            nit.append(r.niter)
-        dt = time.perf_counter() - t0
-        st = srv.stats()
-        for k in order:
-            atoms += systems[k].n
-        return dict(requests=len(order), systems=n_sys, failed=bad, seconds=round(dt, 3),
-                    solves_per_s=len(order) / dt, offered_rate=rate_per_s, submit_seconds=round(t_sub, 3),
-                    p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99), niter=float(np.mean(nit)),
-                    evictions=st.evictions, mean_atoms=atoms / len(order),
-                    max_systems=max_systems, **_stats_delta(st0, st))
+
+        def replay(rate, count):
+            order = np.repeat(np.arange(n_sys), resubmits)
+            rng.shuffle(order)
+            order = order[:count]
+            coords = [tuple(np.ascontiguousarray(v + rng.normal(0.0, jitter, systems[k].n))
+                            for v in (systems[k].x, systems[k].y, systems[k].z)) for k in order]
+            gaps = rng.exponential(1.0 / rate, len(order))
+            tickets = []
+            st0 = srv.stats()
+            t0 = time.perf_counter()
+            t_next = t0
+            for q, k in enumerate(order):
+                t_next += gaps[q]
+                delay = t_next - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                tickets.append(srv.submit(sids[k], *coords[q]))
+            t_sub = time.perf_counter() - t0
+            lat, bad, nit = [], 0, []
+            for t in tickets:
+                r = srv.poll(t)
+                if r.status != THP_OK:
+                    bad += 1
+                lat.append(r.latency_ms)
+                nit.append(r.niter)
+            dt = time.perf_counter() - t0
+            st = srv.stats()
+            atoms = sum(systems[k].n for k in order)
+            return dict(requests=len(order), systems=n_sys, failed=bad, seconds=round(dt, 3),
+                        solves_per_s=len(order) / dt, offered_rate=rate, submit_seconds=round(t_sub, 3),
+                        p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99), niter=float(np.mean(nit)),
+                        evictions=st.evictions, mean_atoms=atoms / len(order),
+                        max_systems=max_systems, **_stats_delta(st0, st))
+
+        out = replay(rate_per_s, n_requests)
+        if more_rates:
+            out["rates"] = [replay(r, more_requests or n_requests) for r in more_rates]
+        return out
     finally:
         srv.stop()
