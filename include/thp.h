@@ -370,6 +370,16 @@ thp_status thp_serve_start(thp_ctx* ctx, const thp_serve_opts* opts);
 thp_status thp_serve_add_system(thp_ctx* ctx, const thp_system_desc* desc, int* system_id);
 /* request a solve of system_id at new coordinates (NULL: unchanged);
  * requests of one system run in submission order */
+/* Local-frame multipoles for a registered serving system, before its first
+ * request (THP_ERR_STATE after), in the form thp_system_set_frames takes:
+ * axis types THP_AXIS_*, 0-based frame atoms (-1 for none) and pole [n][13]
+ * in the local frame.  Every request's set-up then rotates them at
+ * the request's coordinates on the device (rotpole and chkpole,
+ * v1.2/source/rotpole.f:24-339, chkpole.f:20-98, as epolar1.f:113-116 does
+ * before each step's field); thp_serve_submit_multipoles is refused for
+ * such a system. */
+thp_status thp_serve_set_frames(thp_ctx* ctx, int system_id, const int* axistype, const int* zaxis,
+                                const int* xaxis, const int* yaxis, const double* pole);
 thp_status thp_serve_submit(thp_ctx* ctx, int system_id, const double* x, const double* y,
                             const double* z, uint64_t* ticket);
 /* the same with new global-frame multipoles rpole [n][13] (NULL: unchanged),

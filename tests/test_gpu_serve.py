@@ -249,3 +249,51 @@ def test_poisson_stream_of_mixed_sizes_matches_one_system_solves(ctx):
         ref = O.Oracle(systems[k].with_coords(*coords[q])).induce(poleps=1e-8)
         assert abs(results[q].ep - ref["ep"]) <= E_TOL * abs(ref["ep"])
         assert _rms_debye(last_mu[k], ref["mu"]) <= MU_TOL_D
+
+
+def test_local_frames_enter_the_loop_unrotated(ctx):
+    # dhfr2 (every axis type of a protein, chiral sites, polarization
+    # groups) and a framed water box are registered with local-frame
+    # multipoles; each request's set-up rotates them at the request's
+    # coordinates on the device.  Every result against a one-system solve
+    # with the same frames at the same coordinates.
+    import os
+    import thp.tinker as tinker
+    from test_oracle_pins import _water_frames
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    ds = tinker.load(os.path.join(gold, "dhfr2.xyz.gz"), os.path.join(gold, "dhfr2.key"))
+    w = thp.water_1944(seed=5)
+    fr = _water_frames(w)
+    w = w.with_multipoles(np.zeros((w.n, 13)))
+    w.frames = dict(axistype=fr[0], zaxis=fr[1], xaxis=fr[2], yaxis=fr[3], pole=fr[4])
+    systems = [ds, w]
+    rng = np.random.default_rng(17)
+    coords = [[tuple(v + rng.normal(0.0, 0.03, s.n) for v in (s.x, s.y, s.z)) for _ in range(3)] for s in systems]
+    with Server(ctx, max_systems=4, warm_start=1, use_graphs=1) as srv:
+        sids = [srv.add(s) for s in systems]
+        with pytest.raises(thp.ThpError):
+            srv.submit(sids[1], *coords[1][0], rpole=np.zeros((w.n, 13)))
+        results, mus = [[], []], [[], []]
+        for r in range(3):
+            tickets = [srv.submit(sid, *coords[k][r]) for k, sid in enumerate(sids)]
+            for k, t in enumerate(tickets):
+                results[k].append(srv.poll(t))
+                mus[k].append(srv.dipoles(sids[k]))
+        with pytest.raises(thp.ThpError) as e:
+            srv.ctx.check(srv.lib.thp_serve_set_frames(ctx.h, sids[1], *(thp._i(np.zeros(w.n, np.int32)) for _ in range(4)),
+                                                       thp._d(np.zeros((w.n, 13)))), "thp_serve_set_frames")
+        assert e.value.code == -8
+    for k, s in enumerate(systems):
+        for r in range(3):
+            res = results[k][r]
+            assert res.status == thp.THP_OK
+            one = thp.Solver(ctx, s.with_coords(*coords[k][r]), MIXED)
+            m1, e1, _ = one.newinduce_pme2()
+            one.close()
+            print(f"frames in the loop: {s.n} atoms request {r} ep={res.ep:.9f} one-system {e1:.9f}", flush=True)
+            assert abs(res.ep - e1) <= E_TOL * abs(e1), (s.n, r, res.ep, e1)
+            assert _rms_debye(mus[k][r], m1) <= MU_TOL_D
+    # the rotation follows the coordinates: the three requests' energies differ
+    for k in range(2):
+        eps = [r.ep for r in results[k]]
+        assert max(eps) - min(eps) > 1e-4 * abs(eps[0])

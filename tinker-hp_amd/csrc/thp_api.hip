@@ -671,7 +671,10 @@ thp_status b_pme_prep(thp_ctx* ctx, const Batch& b, int gate) {
 // caller-order coordinates of the requested systems into their sorted slots
 thp_status b_load_coords(thp_ctx* ctx, const Batch& b, int gate) {
   hipLaunchKernelGGL(k_load_coords, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, gate);
-  return launch_check(ctx, "k_load_coords");
+  RET(launch_check(ctx, "k_load_coords"));
+  // local frames at the new coordinates (systems without frames exit)
+  hipLaunchKernelGGL(k_rotpole_batch, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, gate);
+  return launch_check(ctx, "k_rotpole_batch");
 }
 
 thp_status b_end_setup(thp_ctx* ctx, const Batch& b) {
@@ -1447,13 +1450,22 @@ thp_status thp_system_set_frames(thp_sys* s, const int* axistype, const int* zax
   s->pme_valid = false;
   s->coef_valid = false;
   s->mu_valid = false;
+  s->view.frames = s->d_frames;
+  s->view.pole_local = s->d_pole_local;
+  RET(upload_view(s));
   return rotate_frames(s);
 }
 
 thp_status thp_system_update_multipoles(thp_sys* s, const double* rpole) {
   if (!s || !rpole) return THP_ERR_ARG;
-  s->has_frames = false;  // global multipoles given outright replace the frames
   s->mu_valid = false;
+  if (s->has_frames) {
+    // global multipoles given outright replace the frames
+    s->has_frames = false;
+    s->view.frames = nullptr;
+    s->view.pole_local = nullptr;
+    RET(upload_view(s));
+  }
   const int n = s->n;
   const size_t np = s->npad;
   std::vector<double> rp((size_t)10 * np, 0.0);

@@ -83,6 +83,10 @@ struct SysView {
   // block-union build [3][npad] with the squared displacement they allow
   const double* raw;
   const double* raw_rpole;  // [n][13] caller-order multipoles (serving systems), else null
+  // local frames (thp_system_set_frames): [npad] x (axis type, z, x, y) in
+  // sorted slots and the local multipoles [10][npad]; null without frames
+  const int* frames;
+  const double* pole_local;
   const int* perm;
   const double* refu;
   double lim_u2;

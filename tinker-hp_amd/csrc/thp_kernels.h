@@ -2618,10 +2618,7 @@ __device__ __forceinline__ void unit3(double& x, double& y, double& z) {
   z /= r;
 }
 
-__global__ void __launch_bounds__(256) k_rotpole(const SysView* views, const int* frames, const double* pole_local) {
-  const SysView& sv = views[0];
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= sv.n) return;
+__device__ __forceinline__ void rotate_site(const SysView& sv, const int* frames, const double* pole_local, int i) {
   const size_t np = sv.npad;
   const int4 f = reinterpret_cast<const int4*>(frames)[i];
   const int axt = f.x, iz = f.y, ix = f.z, iy = f.w;
@@ -2719,6 +2716,26 @@ __global__ void __launch_bounds__(256) k_rotpole(const SysView* views, const int
   double* RP = const_cast<double*>(sv.rpole);
 #pragma unroll
   for (int c = 0; c < 10; ++c) RP[(size_t)c * np + i] = out[c];
+}
+
+// one system, one thread per site
+__global__ void __launch_bounds__(256) k_rotpole(const SysView* views, const int* frames, const double* pole_local) {
+  const SysView& sv = views[0];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= sv.n) return;
+  rotate_site(sv, frames, pole_local, i);
+}
+
+// a batch (the serving loop's set-up): the sites of every block of the
+// systems that have frames, after k_load_coords has placed the coordinates
+__global__ void __launch_bounds__(64) k_rotpole_batch(const SysView* views, const Work* work, int gate) {
+  const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
+  const SysView& sv = views[w.sys];
+  if (!sv.frames || gated_out(sv.st, gate)) return;
+  const int i = w.blk * kBlock + threadIdx.x;
+  if (i >= sv.n) return;
+  rotate_site(sv, sv.frames, sv.pole_local, i);
 }
 
 // ===================================================== truncated CG (polalg 3)
@@ -2970,8 +2987,10 @@ __global__ void __launch_bounds__(64) k_load_coords(const SysView* views, const 
   const_cast<double*>(sv.z)[i] = z;
   const double dx = x - sv.refu[i], dy = y - sv.refu[np + i], dz = z - sv.refu[2 * np + i];
   if (dx * dx + dy * dy + dz * dz > sv.lim_u2) sv.st->stale = 1;
-  if (sv.raw_rpole) {
-    // the 10 used components of rpole(13) (q, d, Qxx Qxy Qxz Qyy Qyz Qzz)
+  if (sv.raw_rpole && !sv.frames) {
+    // the 10 used components of rpole(13) (q, d, Qxx Qxy Qxz Qyy Qyz Qzz);
+    // systems with local frames rotate theirs after this pass
+    // (k_rotpole_batch)
     const GLOBAL double* r = gp(sv.raw_rpole) + (size_t)a * 13;
     GLOBAL double* RP = gw(const_cast<double*>(sv.rpole));
     RP[0 * np + i] = r[0];

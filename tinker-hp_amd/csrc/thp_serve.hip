@@ -46,6 +46,11 @@ struct ServeSystem {
   thp_system_desc desc{};
   std::vector<double> x, y, z, rpole, pol, thole, dpu;
   std::vector<int> ik;
+  // local frames (thp_serve_set_frames): axis types, frame atoms and local
+  // multipoles [n][13], caller order; empty for global multipoles
+  std::vector<int> fr_axt, fr_z, fr_x, fr_y;
+  std::vector<double> pole_local;
+  bool has_frames = false;
   thp_sys* s = nullptr;      // device data when resident
   int64_t bytes = 0;         // device footprint once created (estimate before)
   bool busy = false;         // a solve of it is in flight
@@ -415,6 +420,16 @@ thp_status admit(Serve* sv, ServeSystem* ss, const std::vector<ServeSystem*>& pi
   }
   thp_sys* s = nullptr;
   SRET(create_system(sv->ctx, &ss->desc, &s, false));
+  if (ss->has_frames) {
+    // the frames ride with the device data: the set-up rotates the local
+    // multipoles at every request's coordinates (k_rotpole_batch)
+    const thp_status rf = thp_system_set_frames(s, ss->fr_axt.data(), ss->fr_z.data(), ss->fr_x.data(),
+                                                ss->fr_y.data(), ss->pole_local.data());
+    if (rf != THP_OK) {
+      thp_system_destroy(s);
+      return rf;
+    }
+  }
   ss->s = s;
   ss->bytes = s->bytes;
   sv->resident += ss->bytes;
@@ -994,6 +1009,34 @@ thp_status thp_serve_add_system(thp_ctx* ctx, const thp_system_desc* d, int* sys
   return THP_OK;
 }
 
+thp_status thp_serve_set_frames(thp_ctx* ctx, int sid, const int* axistype, const int* zaxis, const int* xaxis,
+                                const int* yaxis, const double* pole) {
+  Serve* sv = serve_of(ctx);
+  if (!sv) return ctx ? fail(ctx, THP_ERR_STATE, "serving loop not started") : THP_ERR_ARG;
+  if (!axistype || !zaxis || !xaxis || !yaxis || !pole) return THP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (sid < 0 || sid >= (int)sv->systems.size()) return THP_ERR_ARG;
+  ServeSystem* ss = sv->systems[sid].get();
+  // before the system's first request: its device data does not exist yet
+  if (ss->s || ss->busy || ss->has_frames)
+    return fail(ctx, THP_ERR_STATE, "thp_serve_set_frames: system %d has been requested already", sid);
+  const int n = ss->desc.n;
+  for (int i = 0; i < n; ++i) {
+    if (axistype[i] < 0 || axistype[i] > 5) return fail(ctx, THP_ERR_ARG, "atom %d: axis type %d", i, axistype[i]);
+    for (const int* f : {zaxis, xaxis, yaxis})
+      if (f[i] >= n || f[i] == i) return fail(ctx, THP_ERR_ARG, "atom %d: frame atom %d", i, f[i]);
+  }
+  ss->fr_axt.assign(axistype, axistype + n);
+  ss->fr_z.assign(zaxis, zaxis + n);
+  ss->fr_x.assign(xaxis, xaxis + n);
+  ss->fr_y.assign(yaxis, yaxis + n);
+  ss->pole_local.assign(pole, pole + (size_t)13 * n);
+  ss->has_frames = true;
+  // frames and inverse map on the device
+  ss->bytes += (int64_t)(4 * sizeof(int) + 10 * sizeof(double) + 5 * sizeof(int)) * ((n + kBlock - 1) / kBlock * kBlock);
+  return THP_OK;
+}
+
 thp_status thp_serve_submit_multipoles(thp_ctx* ctx, int sid, const double* x, const double* y, const double* z,
                                        const double* rpole, uint64_t* ticket) {
   Serve* sv = serve_of(ctx);
@@ -1009,6 +1052,8 @@ thp_status thp_serve_submit_multipoles(thp_ctx* ctx, int sid, const double* x, c
     if (sid < 0 || sid >= (int)sv->systems.size()) return THP_ERR_ARG;
     n = (size_t)sv->systems[sid]->desc.n;
     q.sys = sv->systems[sid].get();
+    // a system with local frames takes its multipoles from its frames
+    if (rpole && q.sys->has_frames) return THP_ERR_ARG;
   }
   // caller-order arrays into pinned buffers on the caller's thread; the
   // worker uploads them as they are

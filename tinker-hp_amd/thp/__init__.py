@@ -153,6 +153,7 @@ def load_lib(path: str | None = None) -> C.CDLL:
         "thp_grid_uind_dev": (C.c_int, [vp, vp, vp, vp, vp]),
         "thp_fphi_uind_dev": (C.c_int, [vp, vp, vp, vp, vp]),
         "thp_epolar1_noalloc": (C.c_int, [vp, vp, vp, vp]),
+        "thp_serve_set_frames": (C.c_int, [vp, C.c_int, _ip, _ip, _ip, _ip, _dp]),
         "thp_serve_start": (C.c_int, [vp, C.POINTER(ServeOpts)]),
         "thp_serve_add_system": (C.c_int, [vp, C.POINTER(SystemDesc), _ip]),
         "thp_serve_submit": (C.c_int, [vp, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_uint64)]),

@@ -14,7 +14,7 @@ import time
 
 import numpy as np
 
-from . import (THP_OK, THP_PREC_MIXED, Result, ServeOpts, ServeStats, System, ThpError, _d,
+from . import (THP_OK, THP_PREC_MIXED, Result, ServeOpts, ServeStats, System, ThpError, _d, _i,
                serving_system)
 
 
@@ -34,6 +34,14 @@ class Server:
         sid = C.c_int(-1)
         self.ctx.check(self.lib.thp_serve_add_system(self.ctx.h, C.byref(d), C.byref(sid)), "thp_serve_add_system")
         self.systems.append(system)
+        if system.frames is not None:
+            # local-frame multipoles, rotated at every request's coordinates
+            f = system.frames
+            n = system.n
+            arrs = [np.ascontiguousarray(f[k], dtype=np.int32).reshape(n) for k in ("axistype", "zaxis", "xaxis", "yaxis")]
+            pole = np.ascontiguousarray(f["pole"], dtype=np.float64).reshape(n, 13)
+            self.ctx.check(self.lib.thp_serve_set_frames(self.ctx.h, sid.value, *(_i(v) for v in arrs), _d(pole)),
+                           "thp_serve_set_frames")
         return sid.value
 
     def submit(self, sid: int, x=None, y=None, z=None, rpole=None) -> int:
