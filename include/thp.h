@@ -370,6 +370,17 @@ thp_status thp_serve_start(thp_ctx* ctx, const thp_serve_opts* opts);
 thp_status thp_serve_add_system(thp_ctx* ctx, const thp_system_desc* desc, int* system_id);
 /* request a solve of system_id at new coordinates (NULL: unchanged);
  * requests of one system run in submission order */
+/* The grid shapes the serving loop's autotuner picked on this context for
+ * a grid (nfft[3]) and precision: the largest spreading brick edge (4..8)
+ * and the most FFT lines per workgroup (8, 16 or 32).  The loop measures
+ * them with HIP events on a scratch system the first time a request brings
+ * that grid (thp_serve_opts.autotune); every system created on the context
+ * afterwards takes them, alone or in the loop.  THP_ERR_STATE if that grid
+ * has not been tuned.  No reference counterpart: the reference picks its
+ * kernel variants from a fixed table (sub_config,
+ * GPU/source/MOD_interfaces.f:72-98, mechanic.f:907-975). */
+thp_status thp_grid_shape(thp_ctx* ctx, const int* nfft, int precision, int* brick_max, int* fft_lines_max);
+
 /* Local-frame multipoles for a registered serving system, before its first
  * request (THP_ERR_STATE after), in the form thp_system_set_frames takes:
  * axis types THP_AXIS_*, 0-based frame atoms (-1 for none) and pole [n][13]

@@ -312,7 +312,8 @@ def test_serving_resident_bytes_follow_the_lists():
     own = 2 * K * 4 + bricks * 12 ** 3 * 2 * 4
     want = one.footprint() - own + 13 * s.n * 8
     one.close()
-    with Server(ctx, max_systems=2) as srv:
+    # no grid autotuning: the serving system keeps the bricks of `one`
+    with Server(ctx, max_systems=2, autotune=0) as srv:
         sid = srv.add(s)
         r = srv.poll(srv.submit(sid))
         assert r.status == 0

@@ -231,6 +231,11 @@ def test_poisson_stream_of_mixed_sizes_matches_one_system_solves(ctx):
         stats = srv.stats()
     print(f"stream: {len(results)} solves, mean active {stats.mean_active:.2f}, graphs {stats.graph_launches}",
           flush=True)
+    # the autotuner picked grid shapes for every grid of the stream; the
+    # one-system solves below are created with them
+    shapes = {tuple(s.nfft): ctx.grid_shape(s.nfft) for s in systems}
+    print(f"grid shapes: {shapes}", flush=True)
+    assert all(v is not None and 4 <= v[0] <= 8 and v[1] in (8, 16, 32) for v in shapes.values())
     assert all(r.status == thp.THP_OK for r in results)
     assert stats.mean_active > 1.0
     last_q = {}

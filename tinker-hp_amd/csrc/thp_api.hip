@@ -252,10 +252,13 @@ std::vector<int> spatial_order(int n, const double* x, const double* y, const do
   return perm;
 }
 
-// spreading brick edge: the largest B in 8..4 with K % B == 0 or >= 4, so
-// every brick (the last one included) is at least 4 points wide
-int pick_brick(int K) {
-  for (int B = 8; B >= 4; --B)
+// spreading brick edge: the largest B in bmax..4 (bmax <= 8) with K % B == 0
+// or >= 4, so every brick (the last one included) is at least 4 points
+// wide; when no B up to bmax qualifies, the largest one up to 8
+int pick_brick(int K, int bmax) {
+  for (int B = bmax; B >= 4; --B)
+    if (K % B == 0 || K % B >= 4) return B;
+  for (int B = 8; B > bmax; --B)
     if (K % B == 0 || K % B >= 4) return B;
   return 4;
 }
@@ -587,6 +590,89 @@ thp_status epolar1_launch(thp_sys* s, double* F, double* T) {
   return launch_check(ctx, "k_ep1_finish");
 }
 
+// Grid-shape autotuner of the serving loop, once per grid and precision on
+// a context: the spreading brick edge (bricks set the spreading tiles and
+// the gather's workgroups) and the FFT lines per workgroup, measured with
+// HIP events on a scratch system of the request's size.  Bricks change the
+// order of a grid point's sums, so the pick holds for every later system of
+// that grid on the context, solved alone or in a batch; line groups change
+// no arithmetic.
+thp_status tune_grid_shape(thp_ctx* ctx, const thp_system_desc* d) {
+  const std::array<int, 4> key{d->nfft[0], d->nfft[1], d->nfft[2], d->precision};
+  if (ctx->grid_shape.count(key)) return THP_OK;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(ctx, hipEventCreate(&e0));
+  HIPCHK(ctx, hipEventCreate(&e1));
+  auto measure = [&](int bmax, int fcap, bool fft, float* ms) -> thp_status {
+    ctx->force_brick = bmax;
+    ctx->force_fft = fcap;
+    thp_sys* s = nullptr;
+    const thp_status rc = create_system(ctx, d, &s, true);
+    ctx->force_brick = ctx->force_fft = 0;
+    if (rc != THP_OK) return rc;
+    auto body = [&]() -> thp_status {
+      RET(ensure_pme(s));
+      HIPCHK(ctx, hipMemsetAsync(s->view.mu, 0, sizeof(double) * 6 * s->npad, ctx->stream));
+      for (int r = 0; r < 4; ++r) {
+        if (r == 1) HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
+        if (fft) {
+          RET(convolve(ctx, s->one, kGateNone));
+        } else {
+          RET(spread_uind(ctx, s->one, V_MU, 0, kGateNone));
+          RET(launch_gather_uind<0>(ctx, s->one, V_MU, V_HREAL, 0, kGateNone));
+        }
+      }
+      HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
+      HIPCHK(ctx, hipEventSynchronize(e1));
+      HIPCHK(ctx, hipEventElapsedTime(ms, e0, e1));
+      return THP_OK;
+    };
+    const thp_status rb = body();
+    thp_system_destroy(s);
+    return rb;
+  };
+  auto run = [&]() -> thp_status {
+    // brick edges: the distinct brick triples of bmax = 8, 6, 4
+    int best_b = 8;
+    float best = 1e30f;
+    std::vector<std::array<int, 3>> seen;
+    for (int bmax : {8, 6, 4}) {
+      const std::array<int, 3> tri{pick_brick(d->nfft[0], bmax), pick_brick(d->nfft[1], bmax),
+                                   pick_brick(d->nfft[2], bmax)};
+      if (std::find(seen.begin(), seen.end(), tri) != seen.end()) continue;
+      seen.push_back(tri);
+      float ms = 0;
+      RET(measure(bmax, 32, false, &ms));
+      if (ms < best) {
+        best = ms;
+        best_b = bmax;
+      }
+    }
+    // FFT lines per workgroup: the distinct groupings of cap 32, 16, 8
+    int best_f = 32;
+    best = 1e30f;
+    std::vector<std::array<int, 3>> fseen;
+    for (int fcap : {32, 16, 8}) {
+      const std::array<int, 3> g{fft_group_lines(d->nfft[0], fcap), fft_group_lines(d->nfft[1], fcap),
+                                 fft_group_lines(d->nfft[2], fcap)};
+      if (std::find(fseen.begin(), fseen.end(), g) != fseen.end()) continue;
+      fseen.push_back(g);
+      float ms = 0;
+      RET(measure(best_b, fcap, true, &ms));
+      if (ms < best) {
+        best = ms;
+        best_f = fcap;
+      }
+    }
+    ctx->grid_shape[key] = {best_b, best_f};
+    return THP_OK;
+  };
+  const thp_status rc = run();
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
 // local-frame multipoles to the global frame on the device (rotpole)
 thp_status rotate_frames(thp_sys* s) {
   hipLaunchKernelGGL(k_rotpole, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream, s->d_view, s->d_frames,
@@ -886,8 +972,9 @@ void append_work(const thp_sys* s, int index, std::vector<Work>& blk, std::vecto
   const int nconv = (int)((grid_points(s) + kConvChunk - 1) / kConvChunk);
   for (int q = 0; q < nconv; ++q) conv.push_back({index, q});
   // line groups of the three axes, and at least one entry per z plane
-  const int ng = std::max(std::max(fft_groups(s->nfft, 0), s->nfft[2]),
-                          std::max(fft_groups(s->nfft, 1), fft_groups(s->nfft, 2)));
+  const int fc = s->view.fft_cap;
+  const int ng = std::max(std::max(fft_groups(s->nfft, 0, fc), s->nfft[2]),
+                          std::max(fft_groups(s->nfft, 1, fc), fft_groups(s->nfft, 2, fc)));
   for (int q = 0; q < ng; ++q) fft.push_back({index, q});
 }
 
@@ -979,6 +1066,15 @@ thp_status thp_finalize(thp_ctx* ctx) {
 }
 
 const char* thp_last_error(thp_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+thp_status thp_grid_shape(thp_ctx* ctx, const int* nfft, int precision, int* brick_max, int* fft_lines_max) {
+  if (!ctx || !nfft || !brick_max || !fft_lines_max) return THP_ERR_ARG;
+  const auto it = ctx->grid_shape.find({nfft[0], nfft[1], nfft[2], precision});
+  if (it == ctx->grid_shape.end()) return THP_ERR_STATE;
+  *brick_max = it->second.first;
+  *fft_lines_max = it->second.second;
+  return THP_OK;
+}
 
 thp_status thp_system_create(thp_ctx* ctx, const thp_system_desc* d, thp_sys** out) {
   return create_system(ctx, d, out, true);
@@ -1100,8 +1196,21 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
   v.npad = np;
   v.nblk = s->nblk;
   v.red_blocks = std::max(1, std::min(256, (n + 255) / 256));
+  // grid shapes: the autotuner's while it measures, else the ones it picked
+  // for this grid on this context, else the largest bricks and line groups
+  int bmax = 8, fcap = 32;
+  {
+    const auto it = ctx->grid_shape.find({s->nfft[0], s->nfft[1], s->nfft[2], s->prec});
+    if (it != ctx->grid_shape.end()) {
+      bmax = it->second.first;
+      fcap = it->second.second;
+    }
+    if (ctx->force_brick) bmax = ctx->force_brick;
+    if (ctx->force_fft) fcap = ctx->force_fft;
+  }
+  v.fft_cap = fcap;
   for (int k = 0; k < 3; ++k) {
-    v.brick[k] = pick_brick(s->nfft[k]);
+    v.brick[k] = pick_brick(s->nfft[k], bmax);
     v.nbrick[k] = (s->nfft[k] + v.brick[k] - 1) / v.brick[k];
   }
   s->nbricks = v.nbrick[0] * v.nbrick[1] * v.nbrick[2];
@@ -1311,8 +1420,9 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
     ALLOC(s->d_convwork, nconv);
     H2D(s->d_convwork, cw.data(), sizeof(Work) * nconv);
     s->nconv = nconv;
-    const int nfftw = std::max(std::max(fft_groups(s->nfft, 0), s->nfft[2]),
-                               std::max(fft_groups(s->nfft, 1), fft_groups(s->nfft, 2)));
+    const int fc = v.fft_cap;
+    const int nfftw = std::max(std::max(fft_groups(s->nfft, 0, fc), s->nfft[2]),
+                               std::max(fft_groups(s->nfft, 1, fc), fft_groups(s->nfft, 2, fc)));
     std::vector<Work> fw(nfftw);
     for (int b = 0; b < nfftw; ++b) fw[b] = {0, b};
     ALLOC(s->d_fftwork, nfftw);

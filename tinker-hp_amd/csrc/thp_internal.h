@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <array>
+#include <map>
 #include <set>
 #include <string>
 #include <unordered_map>
@@ -71,6 +73,7 @@ struct SysView {
   double ulim2;       // (cut + skin)^2 of the block-union lists
   double afrac[3];  // nfft_k / box_k (orthorhombic fractional scaling)
   int nfft[3];
+  int fft_cap;        // FFT lines per workgroup, at most (fft_group_lines)
   int ncode;
   double ctf[10][10];  // Cartesian -> fractional multipoles (cart_to_frac)
   double ftc[10][10];  // fractional -> Cartesian potential (frac_to_cart)
@@ -156,16 +159,18 @@ constexpr int kFftBuf = 2048;
 constexpr int kFftMaxLen = 512;
 
 // lines per FFT workgroup along an axis of length K: the largest power of
-// two up to 32 whose padded lines (K + 1 values each) fit one LDS buffer
-__host__ __device__ inline int fft_group_lines(int K) {
-  int fl = 32;
+// two up to `cap` (a power of two, at most 32; the grid autotuner picks it
+// per grid) whose padded lines (K + 1 values each) fit one LDS buffer.  The
+// grouping does not change any line's arithmetic.
+__host__ __device__ inline int fft_group_lines(int K, int cap = 32) {
+  int fl = cap;
   while (fl > 1 && fl * (K + 1) > kFftBuf) fl >>= 1;
   return fl;
 }
 
 // workgroups of one FFT pass along `axis` of a K0 x K1 x K2 grid
-__host__ __device__ inline int fft_groups(const int* nfft, int axis) {
-  const int fl = fft_group_lines(nfft[axis]);
+__host__ __device__ inline int fft_groups(const int* nfft, int axis, int cap = 32) {
+  const int fl = fft_group_lines(nfft[axis], cap);
   const int inner = axis == 0 ? nfft[1] : nfft[0];
   const int outer = axis == 2 ? nfft[1] : nfft[2];
   return outer * ((inner + fl - 1) / fl);
@@ -240,6 +245,11 @@ struct thp_ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // ordering of the stream-ordered entry points against the caller's stream
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  // grid shapes picked by the serving loop's autotuner per grid and
+  // precision: (K0, K1, K2, prec) -> (largest brick edge, FFT lines per
+  // workgroup at most); systems created on this context take them
+  std::map<std::array<int, 4>, std::pair<int, int>> grid_shape;
+  int force_brick = 0, force_fft = 0;  // set while the autotuner measures
   // this device's address of the file-scope scratch array of the entry point
   // that allocates nothing (thp_epolar1_noalloc), resolved by thp_init
   double* scratch = nullptr;
@@ -343,6 +353,9 @@ struct thp_sys {
 namespace thp {
 thp_status fail(thp_ctx* ctx, thp_status code, const char* fmt, ...);
 thp_status create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** out, bool own_grid);
+// the serving loop's grid-shape autotuner (brick edge, FFT lines per
+// workgroup) for the grid of `d`, once per grid and precision on ctx
+thp_status tune_grid_shape(thp_ctx* ctx, const thp_system_desc* d);
 thp_status ensure_lists(thp_sys* s);
 thp_status update_coords(thp_sys* s, const double* x, const double* y, const double* z, double* pinned);
 size_t grid_points(const thp_sys* s);

@@ -1731,7 +1731,7 @@ __global__ void __launch_bounds__(256) k_fft_axis(const SysView* views, const Wo
   const int K0 = sv.nfft[0], K1 = sv.nfft[1], K2 = sv.nfft[2];
   const int K = sv.nfft[AXIS];
   const int kp = K + 1;  // padded line stride in LDS
-  const int fl = fft_group_lines(K);
+  const int fl = fft_group_lines(K, sv.fft_cap);
   const int lg = 31 - __clz(fl);
   const int inner_n = AXIS == 0 ? K1 : K0;
   const int outer_n = AXIS == 2 ? K1 : K2;

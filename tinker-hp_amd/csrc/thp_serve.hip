@@ -418,6 +418,7 @@ thp_status admit(Serve* sv, ServeSystem* ss, const std::vector<ServeSystem*>& pi
     }
     SRET(release_system(sv, victim));
   }
+  if (sv->o.autotune) SRET(tune_grid_shape(sv->ctx, &ss->desc));
   thp_sys* s = nullptr;
   SRET(create_system(sv->ctx, &ss->desc, &s, false));
   if (ss->has_frames) {

@@ -154,6 +154,7 @@ def load_lib(path: str | None = None) -> C.CDLL:
         "thp_fphi_uind_dev": (C.c_int, [vp, vp, vp, vp, vp]),
         "thp_epolar1_noalloc": (C.c_int, [vp, vp, vp, vp]),
         "thp_serve_set_frames": (C.c_int, [vp, C.c_int, _ip, _ip, _ip, _ip, _dp]),
+        "thp_grid_shape": (C.c_int, [vp, _ip, C.c_int, _ip, _ip]),
         "thp_serve_start": (C.c_int, [vp, C.POINTER(ServeOpts)]),
         "thp_serve_add_system": (C.c_int, [vp, C.POINTER(SystemDesc), _ip]),
         "thp_serve_submit": (C.c_int, [vp, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_uint64)]),
@@ -320,6 +321,14 @@ class Context:
     def check(self, rc, what=""):
         if rc != THP_OK:
             raise ThpError(rc, f"{what}: {self.lib.thp_last_error(self.h).decode()}")
+
+    def grid_shape(self, nfft, precision=THP_PREC_MIXED):
+        """(largest brick edge, most FFT lines per workgroup) the serving
+        loop's autotuner picked for this grid on this context, or None."""
+        nf = np.ascontiguousarray(nfft, dtype=np.int32).reshape(3)
+        b, f = C.c_int(0), C.c_int(0)
+        rc = self.lib.thp_grid_shape(self.h, _i(nf), precision, C.byref(b), C.byref(f))
+        return (b.value, f.value) if rc == THP_OK else None
 
     def close(self):
         if self.h:
