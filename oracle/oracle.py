@@ -594,6 +594,14 @@ def epolar1_real(system, rpole, ud, up, pairs=None):
     rotation of a site's multipole about its centre, where
     G = -f/2 (u_p . E_d + u_d . E_p + u_p . T u_d) over the real-space
     pairs within the cutoff (scaled pairs with their d, p, u factors)."""
+    F, T, _ = _epolar1_real_pairs(system, rpole, ud, up, pairs)
+    return F, T
+
+
+def _epolar1_real_pairs(system, rpole, ud, up, pairs=None):
+    """epolar1_real, also returning the ordered pairs' separations and
+    forces (field point j, source m, R = r_j - r_m, force on j from the
+    pair, kcal/mol/A) for the virial"""
     s = system
     n = s.n
     xyz = np.stack([s.x, s.y, s.z], axis=1)
@@ -629,7 +637,7 @@ def epolar1_real(system, rpole, ud, up, pairs=None):
     c = rpole[:, 0]
     dp = rpole[:, 1:4]
     Q = rpole[:, 4:13].reshape(n, 3, 3)
-    F = np.zeros((n, 3))
+    fp = np.zeros((len(j), 3))
     T = np.zeros((n, 3))
     for which, (u_at_j, u_src, s_idx) in enumerate(((up, up, 0), (ud, ud, 1))):
         # u_p with the d-scaled fields (s_idx 0), u_d with the p-scaled (1)
@@ -637,14 +645,14 @@ def epolar1_real(system, rpole, ud, up, pairs=None):
         # A: dipole at j in the field gradient of the multipole at m
         ddphi = (c[m][:, None, None] * T2 - np.einsum("pc,pabc->pab", dp[m], T3)
                  + np.einsum("pcd,pabcd->pab", Q[m], T4))
-        np.add.at(F, j, -np.einsum("pa,pab->pb", u_at_j[j], ddphi))
+        fp -= np.einsum("pa,pab->pb", u_at_j[j], ddphi)
         # B: multipole at j in the field of the dipole at m
         dphi = -np.einsum("pb,pab->pa", u_src[m], T2)
         d2phi = -np.einsum("pc,pabc->pab", u_src[m], T3)
         d3phi = -np.einsum("pd,pabcd->pabc", u_src[m], T4)
         E = -dphi
         fB = c[j][:, None] * E - np.einsum("pa,pab->pb", dp[j], d2phi) - np.einsum("pac,pacb->pb", Q[j], d3phi)
-        np.add.at(F, j, fB)
+        fp += fB
         tq = np.cross(dp[j], E)
         A = np.einsum("pab,pbc->pac", Q[j], d2phi) - np.einsum("pab,pbc->pac", d2phi, Q[j])
         tq -= np.einsum("cdb,pdb->pc", _EPS, A)
@@ -653,8 +661,10 @@ def epolar1_real(system, rpole, ud, up, pairs=None):
     T1, T2, T3, T4 = _psi_tensors(R, _ladder(r, s.aewald, lam, sc[:, 2]))
     for u_at_j, u_src in ((up, ud), (ud, up)):
         d2phi = -np.einsum("pc,pabc->pab", u_src[m], T3)
-        np.add.at(F, j, -np.einsum("pa,pab->pb", u_at_j[j], d2phi))
-    return 0.5 * KCOUL * F, 0.5 * KCOUL * T
+        fp -= np.einsum("pa,pab->pb", u_at_j[j], d2phi)
+    F = np.zeros((n, 3))
+    np.add.at(F, j, fp)
+    return 0.5 * KCOUL * F, 0.5 * KCOUL * T, (j, m, R, 0.5 * KCOUL * fp)
 
 
 # Reciprocal part (eprecip1, v1.2/source/epolar1.f:397-1203).  The PME
@@ -893,3 +903,116 @@ def torque_forces_closed(xyz, axistype, zaxis, xaxis, yaxis, torque):
         for a in u_units:
             through_unit(a, i, g_u)
     return F
+
+
+# Polarization virial (epolar1c's vir: v1.2/source/epolar1.f:2332-2340 for
+# the real-space pairs, :1152-1160 for the reciprocal sum, :2478-2526 for
+# the torques carried to the frame atoms; :326-334 accumulates the self
+# field's share).  The virial is the derivative of G (the form whose
+# coordinate derivative is the gradient, with both dipole sets held fixed)
+# along the homogeneous strain r -> (1 + eps) r of every position and of
+# the box: D_ab = dG/d eps_ab, reported symmetrised as vir = (D + D^T)/2.
+#  * real space: D_ab = sum over pairs of dU/dR_a R_b (R the minimum-image
+#    separation, which scales with the box);
+#  * reciprocal space, at fixed fractional positions: G_recip =
+#    f/2 (E(U, M) + E(U_p, U_d)) with E(X, Y) = sum_m qfac Re(F_X* F_Y) the
+#    PME interaction of two site sets (U = U_d + U_p).  qfac carries 1/V
+#    and m^2 = |h^-T k|^2, so dqfac/d eps_ab = qfac (-delta_ab + 2 m_a m_b
+#    (1/m^2 + pi^2/alpha^2)); and Cartesian moments held fixed turn into
+#    fractional ones as if d -> (1 - eps) d and Q -> (1 - eps) Q (1 - eps)^T,
+#    which adds -sum_i (d_b dphi/dr_a + 2 (Q grad grad phi)_ba) for each set
+#    in the other's potential;
+#  * the self terms depend on the moments only;
+#  * with local frames the multipoles turn with their frame atoms: the
+#    forces the torques put on the frame atoms add sum (r_a - r_site) x
+#    gradient, per site.
+
+def _pme_site_grid(s, th, nodes, q, d, Q):
+    """B-spline grid [z, y, x] of site multipoles (charges q (n,), Cartesian
+    dipoles d (n, 3), quadrupoles Q (n, 3, 3) in the rpole convention), such
+    that sum(grid * pot) = sum_i q phi + d . grad phi + Q : grad grad phi for
+    any interpolated potential grid pot (orthorhombic box)"""
+    n1, n2, n3 = (int(v) for v in s.nfft)
+    a = np.array([n1 / s.box[0], n2 / s.box[1], n3 / s.box[2]])
+    grid = np.zeros((n3, n2, n1))
+    t, u, v = th[0], th[1], th[2]
+    for i in range(s.n):
+        def w(o):
+            return np.einsum("a,b,c->cba", t[i, :, o[0]], u[i, :, o[1]], v[i, :, o[2]]) * np.prod(a ** np.array(o))
+        g = q[i] * w((0, 0, 0))
+        for c in range(3):
+            o = [0, 0, 0]
+            o[c] = 1
+            g = g + d[i, c] * w(o)
+            for e in range(3):
+                o2 = list(o)
+                o2[e] += 1
+                g = g + Q[i, c, e] * w(o2)
+        grid[np.ix_(nodes[2][i], nodes[1][i], nodes[0][i])] += g
+    return grid
+
+
+def _virial_kweights(s, qf):
+    """dqfac/d eps_ab = qfac (-delta_ab + 2 m_a m_b (1/m^2 + pi^2/alpha^2)),
+    shape (3, 3, K3, K2, K1)"""
+    n1, n2, n3 = (int(v) for v in s.nfft)
+    ms = []
+    for K, Lk in zip((n1, n2, n3), s.box):
+        k = np.arange(K)
+        ms.append(np.where(k + 1 > (K + 1) // 2, k - K, k) / float(Lk))
+    h3, h2, h1 = np.meshgrid(ms[2], ms[1], ms[0], indexing="ij")
+    hv = (h1, h2, h3)
+    hsq = h1 ** 2 + h2 ** 2 + h3 ** 2
+    with np.errstate(divide="ignore"):
+        fac = np.where(hsq > 0, 2.0 * (1.0 / np.where(hsq > 0, hsq, 1.0) + (np.pi / s.aewald) ** 2), 0.0)
+    W = np.zeros((3, 3) + qf.shape)
+    for a in range(3):
+        for b in range(3):
+            W[a, b] = qf * ((-1.0 if a == b else 0.0) + fac * hv[a] * hv[b])
+    return W
+
+
+def epolar1_virial(system, rpole, ud, up, frames=None, parts=False):
+    """D_ab = dG/d eps_ab (kcal/mol) for global multipoles rpole (n, 13) and
+    the dipole sets ud, up (n, 3); with frames (axistype, zaxis, xaxis,
+    yaxis, pole) the multipoles turn with their frame atoms.  The reference's
+    vir is (D + D^T) / 2.  parts: also the real, reciprocal and torque
+    shares."""
+    s = system
+    n = s.n
+    xyz = np.stack([s.x, s.y, s.z], axis=1)
+    F1, T1, (j, m, R, fp) = _epolar1_real_pairs(s, rpole, ud, up)
+    D_real = -0.5 * np.einsum("pa,pb->ab", fp, R)
+    th, nodes = _pme_stencils(s)
+    qf = _qfac(s)
+    c = rpole[:, 0]
+    dp = rpole[:, 1:4]
+    Q = rpole[:, 4:13].reshape(n, 3, 3)
+    zero, zero33 = np.zeros(n), np.zeros((n, 3, 3))
+    FM = np.fft.fftn(_pme_site_grid(s, th, nodes, c, dp, Q))
+    FD = np.fft.fftn(_pme_site_grid(s, th, nodes, zero, ud, zero33))
+    FP = np.fft.fftn(_pme_site_grid(s, th, nodes, zero, up, zero33))
+    cross = np.real(np.conj(FD + FP) * FM) + np.real(np.conj(FP) * FD)
+    Dk = np.einsum("abzyx,zyx->ab", _virial_kweights(s, qf), cross)
+    cphi = Oracle(s.with_multipoles(rpole)).efld0_recip()
+    Dd = _pme_dipole_potential(s, ud, th, nodes, qf)
+    Dp = _pme_dipole_potential(s, up, th, nodes, qf)
+    Dm = -(np.einsum("ib,ia->ab", ud + up, cphi[:, 1:4]) + np.einsum("ib,ia->ab", dp, Dd[0] + Dp[0])
+           + 2.0 * np.einsum("ibe,iae->ab", Q, Dd[1] + Dp[1])
+           + np.einsum("ib,ia->ab", up, Dd[0]) + np.einsum("ib,ia->ab", ud, Dp[0]))
+    D_recip = 0.5 * KCOUL * (Dk + Dm)
+    D_torque = np.zeros((3, 3))
+    if frames is not None:
+        _, T2 = epolar1_recip_self(s, rpole, ud, up)
+        T = T1 + T2
+        for i in range(n):
+            if int(frames[0][i]) == AXIS_NONE:
+                continue
+            Ti = np.zeros((n, 3))
+            Ti[i] = T[i]
+            Fi = torque_forces_closed(xyz, frames[0], frames[1], frames[2], frames[3], Ti)
+            D_torque -= np.einsum("ka,kb->ab", Fi, xyz - xyz[i])
+    D = D_real + D_recip + D_torque
+    if parts:
+        return D, dict(real=D_real, recip=D_recip, torque=D_torque, ksum=0.5 * KCOUL * Dk)
+    return D

@@ -950,3 +950,120 @@ def test_closed_form_torque_forces_match_the_frame_jacobian():
         assert np.abs(f.sum(0)).max() < 1e-10 * max(1.0, np.abs(f).max())
         if axt[i] != O.AXIS_Z_ONLY:   # Z-Only leaves the spin about z to the global axis
             np.testing.assert_allclose(np.cross(xyz, f).sum(0), tq[i], rtol=0, atol=1e-9 * np.abs(tq[i]).max())
+
+
+# ------------------------------------------------ polarization virial
+# epolar1c's vir (epolar1.f:326-334, 1152-1160, 2332-2340, 2478-2526) is the
+# derivative of the polarization energy along a homogeneous strain of the
+# positions and the box.  Three checks that the oracle's formula
+# (real-space pairs, the reciprocal sum's dependence on the box and on the
+# fractional moments, the torques' share) does not state:
+#  * the diagonal against central differences of the re-solved E_pol in a
+#    strained box (frames turning with their atoms, or multipoles fixed);
+#  * with frames, a rigid rotation changes nothing, so D is symmetric,
+#    although its real, reciprocal and torque shares are not;
+#  * the whole tensor, off-diagonal terms included, does not depend on the
+#    Ewald coefficient, while its real and reciprocal shares change by
+#    amounts of the order of the total between the coefficients tried.
+
+def _virial_case(aewald, nmol=27, box=10.0, nfft=16):
+    s = thp.tiny_water(nmol, box, nfft, seed=4)
+    dpu = np.array(s.scaled_dpu, dtype=np.float64).reshape(-1, 3).copy()
+    dpu[:, 1] = 0.5
+    dpu[:, 2] = 0.7
+    s = dataclasses.replace(s, scaled_dpu=dpu, aewald=aewald)
+    fr = _water_frames(s)
+    xyz = np.stack([s.x, s.y, s.z], axis=1)
+    return s, fr, xyz
+
+
+def _virial_of(s, fr, xyz, frames=True):
+    rp = O.rotpole(*xyz.T, *fr)
+    s2 = s.with_multipoles(rp)
+    mu = O.Oracle(s2).induce(poleps=1e-12)["mu"]
+    return O.epolar1_virial(s2, rp, mu[:, 0], mu[:, 1], fr if frames else None, parts=True)
+
+
+def _strain_diagonal_fd(s, fr, xyz, frames):
+    rp0 = O.rotpole(*xyz.T, *fr)
+
+    def epol(a, h):
+        f = np.ones(3)
+        f[a] += h
+        x2 = xyz * f
+        s2 = dataclasses.replace(s.with_coords(*x2.T), box=np.asarray(s.box) * f)
+        rp = O.rotpole(*x2.T, *fr) if frames else rp0
+        return O.Oracle(s2.with_multipoles(rp)).induce(poleps=1e-12)["ep"]
+
+    return np.array([_d4(lambda h, a=a: epol(a, h), 1e-4) for a in range(3)])
+
+
+@pytest.mark.parametrize("frames", [True, False])
+def test_polarization_virial_diagonal_matches_strain_differences(frames):
+    # an Ewald coefficient large enough (1.2 / A at a 4.4 A cutoff) that no
+    # pair crossing the cutoff under the strain moves the energy
+    s, fr, xyz = _virial_case(1.2)
+    D, parts = _virial_of(s, fr, xyz, frames)
+    fd = _strain_diagonal_fd(s, fr, xyz, frames)
+    print(f"virial diagonal frames={frames}: {np.diag(D)} differences {fd}")
+    assert np.abs(np.diag(D) - fd).max() < 1e-7 * np.abs(fd).max()
+    asym = np.abs(D - D.T).max() / np.abs(D).max()
+    if frames:
+        assert asym < 1e-8
+        for k in ("real", "recip", "torque"):
+            assert np.abs(parts[k] - parts[k].T).max() > 1e-2 * np.abs(D).max()
+    else:
+        assert asym > 1e-2     # fixed global multipoles feel the rotation
+
+
+@pytest.mark.parametrize("frames", [True, False])
+def test_polarization_virial_is_alpha_invariant(frames):
+    # a 14 A box, 6.4 A cutoff and 64^3 grid: the tensor agrees between
+    # alpha = 0.7 and 0.85 / A to the PME interpolation error (~3e-4 of its
+    # scale), while its real-space share moves by about half of it
+    Ds, reals = [], []
+    for a in (0.7, 0.85):
+        s, fr, xyz = _virial_case(a, box=14.0, nfft=64)
+        D, parts = _virial_of(s, fr, xyz, frames)
+        Ds.append(D)
+        reals.append(parts["real"])
+    scale = np.abs(Ds[0]).max()
+    assert np.abs(Ds[1] - Ds[0]).max() < 1e-3 * scale
+    assert np.abs(reals[1] - reals[0]).max() > 0.2 * scale
+
+
+def test_planted_virial_mistakes_fail_the_pins(monkeypatch):
+    # the strain differences catch a dropped 1/m^2 term of the reciprocal
+    # weights and a reversed real-space separation
+    s, fr, xyz = _virial_case(1.2)
+    fd = _strain_diagonal_fd(s, fr, xyz, True)
+    good = O._virial_kweights
+
+    def no_inverse_m2(s_, qf):
+        W = good(s_, qf)
+        # keep -delta_ab and the pi^2/alpha^2 part only
+        n1, n2, n3 = (int(v) for v in s_.nfft)
+        ms = [np.where(np.arange(K) + 1 > (K + 1) // 2, np.arange(K) - K, np.arange(K)) / float(L)
+              for K, L in zip((n1, n2, n3), s_.box)]
+        h3, h2, h1 = np.meshgrid(ms[2], ms[1], ms[0], indexing="ij")
+        hv = (h1, h2, h3)
+        hsq = h1 ** 2 + h2 ** 2 + h3 ** 2
+        for a in range(3):
+            for b in range(3):
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    W[a, b] -= np.where(hsq > 0, qf * 2.0 * hv[a] * hv[b] / np.where(hsq > 0, hsq, 1.0), 0.0)
+        return W
+
+    monkeypatch.setattr(O, "_virial_kweights", no_inverse_m2)
+    D1, _ = _virial_of(s, fr, xyz, True)
+    monkeypatch.setattr(O, "_virial_kweights", good)
+    good_pairs = O._epolar1_real_pairs
+
+    def reversed_r(*a, **k):
+        F, T, (j, m, R, fp) = good_pairs(*a, **k)
+        return F, T, (j, m, -R, fp)
+
+    monkeypatch.setattr(O, "_epolar1_real_pairs", reversed_r)
+    D2, _ = _virial_of(s, fr, xyz, True)
+    for D in (D1, D2):
+        assert np.abs(np.diag(D) - fd).max() > 1e-3 * np.abs(fd).max()
