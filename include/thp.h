@@ -197,6 +197,16 @@ thp_status thp_efld0_recip(thp_sys* sys, double* cphi /* [n][10] */);
  * thp_system_update_coords, _update_multipoles or _set_frames, solve
  * again first). */
 thp_status thp_epolar1(thp_sys* sys, double* dep, double* torque);
+/* thp_epolar1 with the polarization virial vir[3][3] (kcal/mol; the
+ * reference's vir from epolar1c: v1.2/source/epolar1.f:326-334 self,
+ * 1152-1160 reciprocal, 2332-2340 real-space pairs, 2478-2526 torques):
+ * the derivative of the polarization energy along a homogeneous strain of
+ * the positions and the box, symmetrised, with multipoles that follow
+ * their local frames (or stay fixed in the global frame without frames).
+ * Real-space pair shares, the reciprocal sum's box and moment terms and the
+ * frame forces' shares accumulate in FP64 and are summed in a fixed order.
+ * vir may be NULL (then the call is thp_epolar1). */
+thp_status thp_epolar1_vir(thp_sys* sys, double* dep, double* torque, double* vir);
 thp_status thp_tmatxb(thp_sys* sys, const double* mu, double* efi, int dodiag);
 thp_status thp_tmatxb_recip(thp_sys* sys, const double* mu, double* dipfield);
 /* spread fractional induced dipoles fuind/fuinp [n][3] onto the device grid;

@@ -172,3 +172,53 @@ def test_dhfr2_gradient_mixed_matches_fp64(ctx):
     rms = float(np.sqrt(np.mean((g32 - g64) ** 2)) / np.sqrt(np.mean(g64 ** 2)))
     print(f"dhfr2 gradient max|dep|={np.abs(g64).max():.4f} rel_max={err:.2e} rel_rms={rms:.2e}", flush=True)
     assert err < 1e-4 and rms < 1e-5
+
+
+def _oracle_virial(s, fr, poleps=1e-10):
+    xyz = np.stack([s.x, s.y, s.z], axis=1)
+    if fr is not None:
+        rp = O.rotpole(*xyz.T, *fr)
+        s = s.with_multipoles(rp)
+    else:
+        rp = np.asarray(s.rpole)
+    mu = O.Oracle(s).induce(poleps=poleps)["mu"]
+    D = O.epolar1_virial(s, rp, mu[:, 0], mu[:, 1], fr)
+    return 0.5 * (D + D.T)
+
+
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-9), (MIXED, 2e-5)])
+@pytest.mark.parametrize("frames", [True, False])
+def test_virial_matches_oracle(ctx, prec, tol, frames):
+    # epolar1c's vir: real-space pairs, the reciprocal sum's box and moment
+    # terms and the frame forces, against the oracle's strain derivative
+    # (pinned by differences of the re-solved energy in a strained box)
+    s = _box(nmol=27, box=10.0, nfft=16, seed=4)
+    fr = _water_frames(s) if frames else None
+    ref = _oracle_virial(s, fr)
+    solver = _device(ctx, s, fr, prec, 1e-10)
+    dep, vir = solver.epolar1_vir()
+    dep0 = solver.epolar1()
+    assert np.array_equal(dep, dep0)          # the gradient is unchanged
+    err = np.abs(vir - ref).max() / np.abs(ref).max()
+    print(f"virial frames={frames} prec={prec} rel={err:.2e}\n{vir}", flush=True)
+    assert np.allclose(vir, vir.T, rtol=0, atol=1e-12 * np.abs(vir).max())
+    assert err < tol
+    solver.close()
+
+
+def test_virial_on_dhfr2_mixed_against_fp64(ctx):
+    # every frame type of a protein, chiral sites, polarization groups: the
+    # mixed virial within 1e-5 of the FP64 one
+    import os
+    import thp.tinker as tinker
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    ds = tinker.load(os.path.join(gold, "dhfr2.xyz.gz"), os.path.join(gold, "dhfr2.key"))
+    virs = []
+    for prec in (FP64, MIXED):
+        sv = thp.Solver(ctx, ds, prec)
+        sv.newinduce_pme2(poleps=1e-8)
+        virs.append(sv.epolar1_vir()[1])
+        sv.close()
+    err = np.abs(virs[1] - virs[0]).max() / np.abs(virs[0]).max()
+    print(f"dhfr2 virial (FP64)\n{virs[0]}\nmixed rel {err:.2e}", flush=True)
+    assert np.abs(virs[0]).max() > 100.0 and err < 1e-5

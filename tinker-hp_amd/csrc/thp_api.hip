@@ -333,27 +333,33 @@ thp_status spread_mpole(thp_ctx* ctx, const Batch& b, int gate) {
 
 // the grids' convolution with qfac: x and y forward passes, the fused z
 // pass (forward, qfac, backward), y and x backward passes
+// (spec: one system's batch only; its spectrum is kept there, see MODE 3)
 template <typename real>
-void launch_convolve(thp_ctx* ctx, const Batch& b, int gate) {
+void launch_convolve(thp_ctx* ctx, const Batch& b, int gate, void* spec) {
   const dim3 g = g1(b.n_fft), t(256);
   hipStream_t st = ctx->stream;
+  void* none = nullptr;
+  auto zpass = [&]() {
+    if (spec) hipLaunchKernelGGL((k_fft_axis<real, 2, 3>), g, t, 0, st, b.d_views, b.d_fft, gate, spec);
+    else hipLaunchKernelGGL((k_fft_axis<real, 2, 2>), g, t, 0, st, b.d_views, b.d_fft, gate, none);
+  };
   if (b.fft_plane && ctx->plane_fft && sizeof(real) == 4) {
     // x and y fused per plane, then the z pass: three passes over the grid
     hipLaunchKernelGGL((k_fft_plane<real, 0, kPlaneThreads>), g, dim3(kPlaneThreads), b.fft_plane_lds, st, b.d_views, b.d_fft, gate);
-    hipLaunchKernelGGL((k_fft_axis<real, 2, 2>), g, t, 0, st, b.d_views, b.d_fft, gate);
+    zpass();
     hipLaunchKernelGGL((k_fft_plane<real, 1, kPlaneThreads>), g, dim3(kPlaneThreads), b.fft_plane_lds, st, b.d_views, b.d_fft, gate);
     return;
   }
-  hipLaunchKernelGGL((k_fft_axis<real, 0, 0>), g, t, 0, st, b.d_views, b.d_fft, gate);
-  hipLaunchKernelGGL((k_fft_axis<real, 1, 0>), g, t, 0, st, b.d_views, b.d_fft, gate);
-  hipLaunchKernelGGL((k_fft_axis<real, 2, 2>), g, t, 0, st, b.d_views, b.d_fft, gate);
-  hipLaunchKernelGGL((k_fft_axis<real, 1, 1>), g, t, 0, st, b.d_views, b.d_fft, gate);
-  hipLaunchKernelGGL((k_fft_axis<real, 0, 1>), g, t, 0, st, b.d_views, b.d_fft, gate);
+  hipLaunchKernelGGL((k_fft_axis<real, 0, 0>), g, t, 0, st, b.d_views, b.d_fft, gate, none);
+  hipLaunchKernelGGL((k_fft_axis<real, 1, 0>), g, t, 0, st, b.d_views, b.d_fft, gate, none);
+  zpass();
+  hipLaunchKernelGGL((k_fft_axis<real, 1, 1>), g, t, 0, st, b.d_views, b.d_fft, gate, none);
+  hipLaunchKernelGGL((k_fft_axis<real, 0, 1>), g, t, 0, st, b.d_views, b.d_fft, gate, none);
 }
 
-thp_status convolve(thp_ctx* ctx, const Batch& b, int gate) {
-  if (b.prec == THP_PREC_FP64) launch_convolve<double>(ctx, b, gate);
-  else launch_convolve<float>(ctx, b, gate);
+thp_status convolve(thp_ctx* ctx, const Batch& b, int gate, void* spec = nullptr) {
+  if (b.prec == THP_PREC_FP64) launch_convolve<double>(ctx, b, gate, spec);
+  else launch_convolve<float>(ctx, b, gate, spec);
   return launch_check(ctx, "k_fft_axis");
 }
 
@@ -372,11 +378,12 @@ thp_status launch_gather_uind(thp_ctx* ctx, const Batch& b, int vin, int vout, i
 }
 
 // permanent field: ef = E_direct - grad phi_recip + self (newinduce_pme2.f:111-147)
-thp_status field(thp_ctx* ctx, const Batch& b, double* cphi_dev, int add_to_ef, int do_direct, int gate) {
+thp_status field(thp_ctx* ctx, const Batch& b, double* cphi_dev, int add_to_ef, int do_direct, int gate,
+                 void* spec = nullptr) {
   if (do_direct) RET(pair_efld0(ctx, b, gate));
   if (b.real_only && add_to_ef) return THP_OK;  // short range: the direct field alone
   RET(spread_mpole(ctx, b, gate));
-  RET(convolve(ctx, b, gate));
+  RET(convolve(ctx, b, gate, spec));
   if (b.prec == THP_PREC_FP64) hipLaunchKernelGGL((k_gather_mpole<double>), g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, cphi_dev, add_to_ef, gate);
   else hipLaunchKernelGGL((k_gather_mpole<float>), g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, cphi_dev, add_to_ef, gate);
   return launch_check(ctx, "k_gather_mpole");
@@ -567,27 +574,61 @@ thp_status epolar1_check(thp_sys* s) {
 // solved dipoles, real-space and reciprocal forces and torques into F and T
 // ([3][npad], sorted order, written for every atom), the frame forces of
 // the torques, then F scaled to the gradient and T to kcal/mol.
-thp_status epolar1_launch(thp_sys* s, double* F, double* T) {
+// With vpart (virial_slots(s) x 6 doubles) and spec (two grid spectra of
+// the system's real type) the virial shares are written too: the real-space
+// pairs' (one slot per block), the reciprocal moments' (one per block), the
+// frame forces' (one per 256 atoms, with frames) and the reciprocal k-sum's
+// (one per kVirChunk grid points), in that order.
+thp_status epolar1_launch(thp_sys* s, double* F, double* T, double* vpart = nullptr, void* spec = nullptr) {
   thp_ctx* ctx = s->ctx;
   hipStream_t hs = ctx->stream;
   const Batch& b = s->one;
-  RET(field(ctx, b, s->cphi, 0, 0, kGateNone));
+  const bool vir = vpart != nullptr;
+  const size_t gbytes = grid_points(s) * 2 * (s->prec == THP_PREC_FP64 ? sizeof(double) : sizeof(float));
+  void* spec_m = vir ? spec : nullptr;
+  void* spec_u = vir ? static_cast<char*>(spec) + gbytes : nullptr;
+  RET(field(ctx, b, s->cphi, 0, 0, kGateNone, spec_m));
   RET(spread_uind(ctx, b, V_MU, 0, kGateNone));
-  RET(convolve(ctx, b, kGateNone));
-  hipLaunchKernelGGL(k_ep1_real<4>, g1(b.n_blk), dim3(256), 0, hs, b.d_views, b.d_blk, F, T);
+  RET(convolve(ctx, b, kGateNone, spec_u));
+  const size_t nb = (size_t)s->nblk, nt = (size_t)(s->n + 255) / 256;
+  double* v_real = vpart;
+  double* v_recip = vir ? vpart + 6 * nb : nullptr;
+  double* v_torque = vir ? vpart + 12 * nb : nullptr;
+  double* v_ksum = vir ? vpart + 12 * nb + 6 * nt : nullptr;
+  if (vir) hipLaunchKernelGGL((k_ep1_real<4, true>), g1(b.n_blk), dim3(256), 0, hs, b.d_views, b.d_blk, F, T, v_real);
+  else hipLaunchKernelGGL((k_ep1_real<4, false>), g1(b.n_blk), dim3(256), 0, hs, b.d_views, b.d_blk, F, T, v_real);
   RET(launch_check(ctx, "k_ep1_real"));
-  if (s->prec == THP_PREC_FP64)
-    hipLaunchKernelGGL((k_ep1_recip<double>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T);
-  else
-    hipLaunchKernelGGL((k_ep1_recip<float>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T);
+  if (s->prec == THP_PREC_FP64) {
+    if (vir) hipLaunchKernelGGL((k_ep1_recip<double, true>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T, v_recip);
+    else hipLaunchKernelGGL((k_ep1_recip<double, false>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T, v_recip);
+  } else {
+    if (vir) hipLaunchKernelGGL((k_ep1_recip<float, true>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T, v_recip);
+    else hipLaunchKernelGGL((k_ep1_recip<float, false>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T, v_recip);
+  }
   RET(launch_check(ctx, "k_ep1_recip"));
   const dim3 ga((s->n + 255) / 256);
   if (s->has_frames) {
-    hipLaunchKernelGGL(k_ep1_torque, ga, dim3(256), 0, hs, s->d_view, s->d_frames, s->d_tinv_off, s->d_tinv, T, F);
+    if (vir)
+      hipLaunchKernelGGL((k_ep1_torque<true>), ga, dim3(256), 0, hs, s->d_view, s->d_frames, s->d_tinv_off, s->d_tinv, T, F, v_torque);
+    else
+      hipLaunchKernelGGL((k_ep1_torque<false>), ga, dim3(256), 0, hs, s->d_view, s->d_frames, s->d_tinv_off, s->d_tinv, T, F, v_torque);
     RET(launch_check(ctx, "k_ep1_torque"));
+  }
+  if (vir) {
+    const dim3 gk((unsigned)((grid_points(s) + kVirChunk - 1) / kVirChunk));
+    if (s->prec == THP_PREC_FP64)
+      hipLaunchKernelGGL((k_vir_ksum<double>), gk, dim3(256), 0, hs, s->d_view, spec_m, spec_u, v_ksum);
+    else
+      hipLaunchKernelGGL((k_vir_ksum<float>), gk, dim3(256), 0, hs, s->d_view, spec_m, spec_u, v_ksum);
+    RET(launch_check(ctx, "k_vir_ksum"));
   }
   hipLaunchKernelGGL(k_ep1_finish, ga, dim3(256), 0, hs, s->n, s->npad, F, T, -1.0);
   return launch_check(ctx, "k_ep1_finish");
+}
+
+// slots of the virial shares written by epolar1_launch
+size_t virial_slots(const thp_sys* s) {
+  return 2 * (size_t)s->nblk + (size_t)(s->n + 255) / 256 + (grid_points(s) + kVirChunk - 1) / kVirChunk;
 }
 
 // Grid-shape autotuner of the serving loop, once per grid and precision on
@@ -1713,7 +1754,9 @@ thp_status thp_efld0_direct(thp_sys* s, double* ef) {
   return gather_out(s, ef, s->view.ef, 6);
 }
 
-thp_status thp_epolar1(thp_sys* s, double* dep, double* torque) {
+thp_status thp_epolar1(thp_sys* s, double* dep, double* torque) { return thp_epolar1_vir(s, dep, torque, nullptr); }
+
+thp_status thp_epolar1_vir(thp_sys* s, double* dep, double* torque, double* vir) {
   if (!s || !dep) return THP_ERR_ARG;
   RET(epolar1_check(s));
   RET(ensure_lists(s));
@@ -1723,9 +1766,31 @@ thp_status thp_epolar1(thp_sys* s, double* dep, double* torque) {
     RET(dalloc(s, &s->grad_f, 3 * np));
     RET(dalloc(s, &s->grad_t, 3 * np));
   }
-  RET(epolar1_launch(s, s->grad_f, s->grad_t));
+  const size_t slots = virial_slots(s);
+  if (vir && !s->grad_vir) {
+    RET(dalloc(s, &s->grad_vir, 6 * slots));
+    char* sp = nullptr;
+    RET(dalloc(s, &sp, 2 * grid_points(s) * 2 * (s->prec == THP_PREC_FP64 ? sizeof(double) : sizeof(float))));
+    s->vir_spec = sp;
+  }
+  RET(epolar1_launch(s, s->grad_f, s->grad_t, vir ? s->grad_vir : nullptr, vir ? s->vir_spec : nullptr));
   RET(gather_out(s, dep, s->grad_f, 3));
   if (torque) RET(gather_out(s, torque, s->grad_t, 3));
+  if (vir) {
+    // the shares in their fixed order, FP64, times f/2
+    std::vector<double> part(6 * slots);
+    HIPCHK(s->ctx, hipMemcpy(part.data(), s->grad_vir, sizeof(double) * part.size(), hipMemcpyDeviceToHost));
+    const size_t nb = (size_t)s->nblk, nt = (size_t)(s->n + 255) / 256;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (size_t q = 0; q < slots; ++q) {
+      if (!s->has_frames && q >= 2 * nb && q < 2 * nb + nt) continue;  // no frame forces
+      for (int c = 0; c < 6; ++c) v[c] += part[6 * q + c];
+    }
+    const double f2 = 0.5 * kCoulombFp;
+    const int idx[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) vir[3 * a + b] = f2 * v[idx[a][b]];
+  }
   return THP_OK;
 }
 

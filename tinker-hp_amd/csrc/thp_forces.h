@@ -151,14 +151,30 @@ __device__ __forceinline__ void ep1_pair(const Site& si, const Site& sk, V3 R, c
   F = F - w_ddphi_dip(si.up, sk.ud, R, Gu) - w_ddphi_dip(si.ud, sk.up, R, Gu);
 }
 
+// Virial shares (the strain derivative of G, symmetrised, unscaled like F:
+// xx, yy, zz, xy, xz, yz).  A pair adds -1/2 f (x) R per direction it is
+// walked in (R = r_i - r_k, f its force on i); each kernel sums its shares
+// over a workgroup in a fixed order into vpart[slot][6].
+__device__ __forceinline__ void vir_pair(V3 f, double rx, double ry, double rz, double* vw) {
+  vw[0] -= 0.5 * f.x * rx;
+  vw[1] -= 0.5 * f.y * ry;
+  vw[2] -= 0.5 * f.z * rz;
+  vw[3] -= 0.25 * (f.x * ry + f.y * rx);
+  vw[4] -= 0.25 * (f.x * rz + f.z * rx);
+  vw[5] -= 0.25 * (f.y * rz + f.z * ry);
+}
+
 // Real-space part: a workgroup of S wavefronts per 64-atom block, each
 // wavefront walking one contiguous segment of every lane's block-union
 // partners (within the cutoff) and every S-th of its scaled pairs (with
 // their d, p, u factors); the S partial forces and torques are summed in
 // LDS in a fixed order.  FP64 throughout (one launch per gradient).
-template <int S>
-__global__ void __launch_bounds__(64 * S) k_ep1_real(const SysView* views, const Work* work, double* F, double* T) {
+template <int S, bool VIR>
+__global__ void __launch_bounds__(64 * S) k_ep1_real(const SysView* views, const Work* work, double* F, double* T,
+                                                     double* vpart) {
   __shared__ double red[S][6][64];
+  __shared__ double vred[VIR ? S : 1][6][64];
+  double vw[6] = {0, 0, 0, 0, 0, 0};
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
@@ -183,7 +199,9 @@ __global__ void __launch_bounds__(64 * S) k_ep1_real(const SysView* views, const
     if (!(r2 <= sv.cut2)) continue;
     double G[3][4];
     ep1_ladders(r2, sv.aewald, pdi, sv.pdamp[k], thi, sv.thole[k], one, 1, G);
+    const V3 f0 = f;
     ep1_pair(si, load_site(sv, k), v3(xr, yr, zr), G[0], G[0], G[0], f, t);
+    if (VIR) vir_pair(f - f0, xr, yr, zr, vw);
   }
   for (int e = sv.excl_off[i] + wv; e < sv.excl_off[i + 1]; e += S) {
     const int k = sv.excl_k[e];
@@ -195,7 +213,9 @@ __global__ void __launch_bounds__(64 * S) k_ep1_real(const SysView* views, const
     const double* sc = sv.scale_tab + sv.excl_code[e] * 3;
     double G[3][4];
     ep1_ladders(r2, sv.aewald, pdi, sv.pdamp[k], thi, sv.thole[k], sc, 3, G);
+    const V3 f0 = f;
     ep1_pair(si, load_site(sv, k), v3(xr, yr, zr), G[0], G[1], G[2], f, t);
+    if (VIR) vir_pair(f - f0, xr, yr, zr, vw);
   }
   }
   red[wv][0][lane] = f.x;
@@ -204,7 +224,19 @@ __global__ void __launch_bounds__(64 * S) k_ep1_real(const SysView* views, const
   red[wv][3][lane] = t.x;
   red[wv][4][lane] = t.y;
   red[wv][5][lane] = t.z;
+  if (VIR)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) vred[wv][c][lane] = vw[c];
   __syncthreads();
+  if (VIR && wv == 0) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double v = vred[0][c][lane];
+      for (int q = 1; q < S; ++q) v += vred[q][c][lane];
+      v = wave_sum(v);
+      if (lane == 0) vpart[(size_t)w.blk * 6 + c] = v;
+    }
+  }
   if (wv == 0 && ok) {
     double v[6];
 #pragma unroll
@@ -224,14 +256,17 @@ __global__ void __launch_bounds__(64 * S) k_ep1_real(const SysView* views, const
 // Reciprocal and self parts, one thread per atom, added to F and T.  The
 // grid holds the convolved induced grids (u_d real, u_p imaginary); cphi
 // the Cartesian potential derivatives of the multipole grid ([10][npad]).
-template <typename real>
+template <typename real, bool VIR>
 __global__ void __launch_bounds__(64) k_ep1_recip(const SysView* views, const Work* work, const double* cphi,
-                                                   double* F, double* T) {
+                                                   double* F, double* T, double* vpart) {
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
-  const int i = w.blk * kBlock + threadIdx.x;
-  if (i >= sv.n) return;
+  const int i0 = w.blk * kBlock + threadIdx.x;
+  const bool ok = i0 < sv.n;
+  if (!VIR && !ok) return;
+  // lanes past the last atom evaluate atom 0 and add nothing
+  const int i = ok ? i0 : 0;
   const size_t np = sv.npad;
   const int n1 = sv.nfft[0], n2 = sv.nfft[1], n3 = sv.nfft[2];
   real th[3][4][kOrder];
@@ -305,10 +340,12 @@ __global__ void __launch_bounds__(64) k_ep1_recip(const SysView* views, const Wo
     o1 = (a == 1) + (b == 1) + (c == 1);
     o2 = (a == 2) + (b == 2) + (c == 2);
   };
-  double D1[3], D2s[3][3], D2d[3][3], D2p[3][3], D3[3][3][3];
+  double D1[3], D1d[3], D1p[3], D2s[3][3], D2d[3][3], D2p[3][3], D3[3][3][3];
   for (int a = 0; a < 3; ++a) {
     int o0 = (a == 0), o1 = (a == 1), o2 = (a == 2);
-    D1[a] = cart(pd, o0, o1, o2) + cart(pp, o0, o1, o2);
+    D1d[a] = cart(pd, o0, o1, o2);
+    D1p[a] = cart(pp, o0, o1, o2);
+    D1[a] = D1d[a] + D1p[a];
     for (int b = 0; b < 3; ++b) {
       o0 = (a == 0) + (b == 0);
       o1 = (a == 1) + (b == 1);
@@ -356,9 +393,30 @@ __global__ void __launch_bounds__(64) k_ep1_recip(const SysView* views, const Wo
   t[0] = dE.x - (A[1][2] - A[2][1]) + self.x;
   t[1] = dE.y - (A[2][0] - A[0][2]) + self.y;
   t[2] = dE.z - (A[0][1] - A[1][0]) + self.z;
-  for (int c = 0; c < 3; ++c) {
-    F[(size_t)c * np + i] += f[c];
-    T[(size_t)c * np + i] += t[c];
+  if (ok)
+    for (int c = 0; c < 3; ++c) {
+      F[(size_t)c * np + i] += f[c];
+      T[(size_t)c * np + i] += t[c];
+    }
+  if (VIR) {
+    // the moments held fixed in Cartesian form turn into fractional ones as
+    // the box strains: -(u_b dphi_M/dr_a + d_b dphi_U/dr_a + 2 (Q ddphi_U)_ba
+    // + u_p,b dphi_Ud/dr_a + u_d,b dphi_Up/dr_a), U = U_d + U_p
+    const double M1[3] = {cphi[1 * np + i], cphi[2 * np + i], cphi[3 * np + i]};
+    double v[3][3];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        double q2 = 0.0;
+        for (int e = 0; e < 3; ++e) q2 += qm[b][e] * D2s[a][e];
+        v[a][b] = -((ud[b] + up[b]) * M1[a] + dm[b] * D1[a] + 2.0 * q2 + up[b] * D1d[a] + ud[b] * D1p[a]);
+      }
+    const int pr[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double x = ok ? 0.5 * (v[pr[c][0]][pr[c][1]] + v[pr[c][1]][pr[c][0]]) : 0.0;
+      x = wave_sum(x);
+      if (threadIdx.x == 0) vpart[(size_t)w.blk * 6 + c] = x;
+    }
   }
 }
 
@@ -458,23 +516,115 @@ __device__ __forceinline__ int frame_forces(const SysView& sv, const int* frames
 // holds a (tinv[tinv_off[a] .. tinv_off[a + 1]), sites in ascending order,
 // a itself included when it has a frame) and adds them to F[a]: a fixed
 // order of sums, no atomics, the same bits on every run.
+// With VIR each atom also adds -g (x) (r_a - r_site) for what it receives
+// (the multipoles turn with their frames as the box strains); the site's
+// own share sits at zero separation.
+template <bool VIR>
 __global__ void __launch_bounds__(256) k_ep1_torque(const SysView* views, const int* frames, const int* tinv_off,
-                                                    const int* tinv, const double* T, double* F) {
+                                                    const int* tinv, const double* T, double* F, double* vpart) {
   const SysView& sv = views[0];
   const int a = blockIdx.x * 256 + threadIdx.x;
-  if (a >= sv.n) return;
+  if (!VIR && a >= sv.n) return;
   const size_t np = sv.npad;
   V3 acc = v3(0, 0, 0);
-  for (int e = tinv_off[a]; e < tinv_off[a + 1]; ++e) {
-    int atom[kFrameContrib];
-    V3 g[kFrameContrib];
-    const int m = frame_forces(sv, frames, T, tinv[e], atom, g);
-    for (int q = 0; q < m; ++q)
-      if (atom[q] == a) acc = acc + g[q];
+  double vw[6] = {0, 0, 0, 0, 0, 0};
+  if (a < sv.n) {
+    for (int e = tinv_off[a]; e < tinv_off[a + 1]; ++e) {
+      int atom[kFrameContrib];
+      V3 g[kFrameContrib];
+      const int s = tinv[e];
+      const int m = frame_forces(sv, frames, T, s, atom, g);
+      for (int q = 0; q < m; ++q)
+        if (atom[q] == a) {
+          acc = acc + g[q];
+          if (VIR) {
+            const double rx = sv.x[a] - sv.x[s], ry = sv.y[a] - sv.y[s], rz = sv.z[a] - sv.z[s];
+            vw[0] -= g[q].x * rx;
+            vw[1] -= g[q].y * ry;
+            vw[2] -= g[q].z * rz;
+            vw[3] -= 0.5 * (g[q].x * ry + g[q].y * rx);
+            vw[4] -= 0.5 * (g[q].x * rz + g[q].z * rx);
+            vw[5] -= 0.5 * (g[q].y * rz + g[q].z * ry);
+          }
+        }
+    }
+    F[a] += acc.x;
+    F[np + a] += acc.y;
+    F[2 * np + a] += acc.z;
   }
-  F[a] += acc.x;
-  F[np + a] += acc.y;
-  F[2 * np + a] += acc.z;
+  if (VIR) {
+    __shared__ double wred[4][6];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const double x = wave_sum(vw[c]);
+      if (lane == 0) wred[wv][c] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+      const int c = threadIdx.x;
+      vpart[(size_t)blockIdx.x * 6 + c] = ((wred[0][c] + wred[1][c]) + wred[2][c]) + wred[3][c];
+    }
+  }
+}
+
+// Reciprocal k-sum of the virial: sum_m dqfac/d eps_ab(m) (Re(F_U* F_M) +
+// Re(F_Up* F_Ud)) with dqfac/d eps_ab = qfac (-delta_ab + 2 m_a m_b (1/m^2 +
+// pi^2/alpha^2)), m the reciprocal vector of the grid index.  spec_m holds
+// the multipole grid's spectrum, spec_u that of the induced grid (U_d in the
+// real part, U_p in the imaginary part of the grid, so F_Ud = (G(m) +
+// conj G(-m)) / 2 and F_Up = (G(m) - conj G(-m)) / 2i).  One system; the
+// workgroups take chunks of kVirChunk points, sums in a fixed order.
+constexpr int kVirChunk = 2048;
+template <typename real>
+__global__ void __launch_bounds__(256) k_vir_ksum(const SysView* views, const void* spec_m, const void* spec_u,
+                                                  double* vpart) {
+  const SysView& sv = views[0];
+  const int K0 = sv.nfft[0], K1 = sv.nfft[1], K2 = sv.nfft[2];
+  const size_t npts = (size_t)K0 * K1 * K2;
+  const real* SM = static_cast<const real*>(spec_m);
+  const real* SU = static_cast<const real*>(spec_u);
+  const real* Q = static_cast<const real*>(sv.qfac);
+  const double pa2 = (M_PI / sv.aewald) * (M_PI / sv.aewald);
+  double vw[6] = {0, 0, 0, 0, 0, 0};
+  const size_t p0 = (size_t)blockIdx.x * kVirChunk, p1 = min(npts, p0 + kVirChunk);
+  for (size_t p = p0 + threadIdx.x; p < p1; p += 256) {
+    const int x = int(p % K0), y = int((p / K0) % K1), z = int(p / ((size_t)K0 * K1));
+    const double q = double(Q[p]);
+    if (q == 0.0) continue;
+    const int nx = x ? K0 - x : 0, ny = y ? K1 - y : 0, nz = z ? K2 - z : 0;
+    const size_t pn = ((size_t)nz * K1 + ny) * K0 + nx;
+    const double a = double(SU[2 * p]), b = double(SU[2 * p + 1]);
+    const double c = double(SU[2 * pn]), d = double(SU[2 * pn + 1]);
+    const double udr = 0.5 * (a + c), udi = 0.5 * (b - d);
+    const double upr = 0.5 * (b + d), upi = -0.5 * (a - c);
+    const double mr = double(SM[2 * p]), mi = double(SM[2 * p + 1]);
+    const double cross = (udr + upr) * mr + (udi + upi) * mi + (upr * udr + upi * udi);
+    const double m[3] = {double(x + 1 > (K0 + 1) / 2 ? x - K0 : x) / sv.box[0],
+                         double(y + 1 > (K1 + 1) / 2 ? y - K1 : y) / sv.box[1],
+                         double(z + 1 > (K2 + 1) / 2 ? z - K2 : z) / sv.box[2]};
+    const double m2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+    const double fac = 2.0 * (1.0 / m2 + pa2);
+    const double qc = q * cross;
+    vw[0] += qc * (fac * m[0] * m[0] - 1.0);
+    vw[1] += qc * (fac * m[1] * m[1] - 1.0);
+    vw[2] += qc * (fac * m[2] * m[2] - 1.0);
+    vw[3] += qc * fac * m[0] * m[1];
+    vw[4] += qc * fac * m[0] * m[2];
+    vw[5] += qc * fac * m[1] * m[2];
+  }
+  __shared__ double wred[4][6];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const double s = wave_sum(vw[c]);
+    if (lane == 0) wred[wv][c] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int c = threadIdx.x;
+    vpart[(size_t)blockIdx.x * 6 + c] = ((wred[0][c] + wred[1][c]) + wred[2][c]) + wred[3][c];
+  }
 }
 
 // f/2 on the forces and torques

@@ -325,6 +325,10 @@ struct thp_sys {
   // forces and torques of the gradient ([3][npad] each), made on first use
   double* grad_f = nullptr;
   double* grad_t = nullptr;
+  // virial shares of the gradient and the two grid spectra they need,
+  // made on the first call that asks for the virial
+  double* grad_vir = nullptr;
+  char* vir_spec = nullptr;
   thp::SysView view;           // host copy of the view
   thp::SysView* d_view = nullptr;
   int view_version = 0;        // bumped whenever the device view is rewritten

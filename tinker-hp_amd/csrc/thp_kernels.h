@@ -1719,9 +1719,13 @@ __device__ int fft_lines_lds(Coef2<real>* b0, Coef2<real>* b1, int cur, const Co
 }
 
 // MODE 0 forward, 1 backward, 2 forward * qfac * backward (z pass only)
+// MODE 0 forward, 1 backward, 2 the z pass of the convolution (forward,
+// times qfac, backward), 3 the same keeping the spectrum (the forward
+// transform of the whole grid, before qfac) in `spec`, grid layout
 template <typename real, int AXIS, int MODE>
-__global__ void __launch_bounds__(256) k_fft_axis(const SysView* views, const Work* work, int gate) {
-  static_assert(MODE != 2 || AXIS == 2, "the convolution is fused into the z pass");
+__global__ void __launch_bounds__(256) k_fft_axis(const SysView* views, const Work* work, int gate, void* spec) {
+  constexpr bool CONV = MODE == 2 || MODE == 3;
+  static_assert(!CONV || AXIS == 2, "the convolution is fused into the z pass");
   __shared__ Coef2<real> s_buf[2][kFftBuf];
   __shared__ Coef2<real> s_tw[kFftMaxLen];
   const Work w = work[blockIdx.x];
@@ -1779,14 +1783,14 @@ __global__ void __launch_bounds__(256) k_fft_axis(const SysView* views, const Wo
   // before the first LDS write; the loads are unconditional (elements past
   // the group re-read its first one) so that none waits for another
   Coef2<real> v[PER];
-  real qv[MODE == 2 ? PER : 1];
+  real qv[CONV ? PER : 1];
 #pragma unroll
   for (int u = 0; u < PER; ++u) {
     int slot;
     size_t g;
     v[u] = G[elem(u, slot, g) ? g : base];
   }
-  if constexpr (MODE == 2) {
+  if constexpr (CONV) {
     const GLOBAL real* Q = gp(static_cast<const real*>(sv.qfac));
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
@@ -1811,7 +1815,16 @@ __global__ void __launch_bounds__(256) k_fft_axis(const SysView* views, const Wo
   __syncthreads();
   const int tpl_shift = 8 - lg;  // threads per line = 256 / fl
   int cur = fft_lines_lds<real>(s_buf[0], s_buf[1], 0, s_tw, K, kp, nl, tid, tpl_shift, MODE == 1 ? real(1) : real(-1));
-  if (MODE == 2) {
+  if (CONV) {
+    if constexpr (MODE == 3) {
+      GLOBAL Coef2<real>* SP = gw(static_cast<Coef2<real>*>(spec));
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        int slot;
+        size_t g;
+        if (elem(u, slot, g)) SP[g] = s_buf[cur][slot];
+      }
+    }
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       int slot;

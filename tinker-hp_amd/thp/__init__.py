@@ -133,6 +133,7 @@ def load_lib(path: str | None = None) -> C.CDLL:
         "thp_efld0_direct": (C.c_int, [vp, _dp]),
         "thp_efld0_recip": (C.c_int, [vp, _dp]),
         "thp_epolar1": (C.c_int, [vp, _dp, _dp]),
+        "thp_epolar1_vir": (C.c_int, [vp, _dp, _dp, _dp]),
         "thp_tmatxb": (C.c_int, [vp, _dp, _dp, C.c_int]),
         "thp_tmatxb_recip": (C.c_int, [vp, _dp, _dp]),
         "thp_grid_uind": (C.c_int, [vp, _dp, _dp, _dp]),
@@ -422,6 +423,14 @@ class Solver:
         cphi = np.zeros((self.sys.n, 10))
         self._c(self.lib.thp_efld0_recip(self.h, _d(cphi)), "efld0_recip")
         return cphi
+
+    def epolar1_vir(self):
+        """Gradient dep (n, 3) and the polarization virial (3, 3), kcal/mol
+        (epolar1c's dep and vir)."""
+        dep = np.zeros((self.sys.n, 3))
+        vir = np.zeros((3, 3))
+        self._c(self.lib.thp_epolar1_vir(self.h, _d(dep), None, _d(vir)), "epolar1_vir")
+        return dep, vir
 
     def epolar1(self, with_torque: bool = False):
         """Polarization energy gradient dep (n, 3), kcal/mol/A, for the
