@@ -204,6 +204,7 @@ def main():
         ctx = thp.Context(0)
         serve = tserve.bench_serving(ctx, thp.water_23559(), steps=max(args.steps, 8), warmup=args.warmup,
                                      poleps=args.poleps, copies=args.copies, iters_per_graph=4, use_graphs=1)
+        serve["grid_shape"] = ctx.grid_shape((64, 64, 64))
         log(f"bench: serve {serve}")
         ctx.close()
         return
@@ -251,6 +252,7 @@ def main():
                                          poleps=args.poleps, copies=args.copies, iters_per_graph=4,
                                          use_graphs=1, keep_last=not args.no_cpu)
             serve_last = serve.pop("last", None)
+            serve["grid_shape"] = ctx.grid_shape(system.nfft)  # (brick edge, FFT lines) the autotuner picked
             legs["serve"] = serve
             log(f"bench: serve {serve}")
         except (ImportError, AttributeError, thp.ThpError) as e:

@@ -632,86 +632,78 @@ size_t virial_slots(const thp_sys* s) {
 }
 
 // Grid-shape autotuner of the serving loop, once per grid and precision on
-// a context: the spreading brick edge (bricks set the spreading tiles and
-// the gather's workgroups) and the FFT lines per workgroup, measured with
-// HIP events on a scratch system of the request's size.  Bricks change the
-// order of a grid point's sums, so the pick holds for every later system of
-// that grid on the context, solved alone or in a batch; line groups change
-// no arithmetic.
+// a context: the spreading brick edge (bricks set the spreading tiles, the
+// merge and the gather's workgroups) and the FFT lines per workgroup,
+// measured with HIP events on scratch copies of the request's system
+// launched together as the loop launches its batches (enough copies for
+// about 300,000 atoms, at most 12: one system alone favours small bricks
+// that lose once a batch fills the device).  Bricks change the order of a
+// grid point's sums, so the pick holds for every later system of that grid
+// on the context, solved alone or in a batch; line groups change no
+// arithmetic.
 thp_status tune_grid_shape(thp_ctx* ctx, const thp_system_desc* d) {
   const std::array<int, 4> key{d->nfft[0], d->nfft[1], d->nfft[2], d->precision};
   if (ctx->grid_shape.count(key)) return THP_OK;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  HIPCHK(ctx, hipEventCreate(&e0));
-  HIPCHK(ctx, hipEventCreate(&e1));
-  auto measure = [&](int bmax, int fcap, bool fft, float* ms) -> thp_status {
+  const int copies = std::max(1, std::min(12, (int)std::lround(300000.0 / d->n)));
+  // average ms of the batched launches of `kernels` (thp_time_kernel_batch
+  // numbering) over scratch copies made with the given shapes
+  auto measure = [&](int bmax, int fcap, std::initializer_list<int> kernels, double* ms) -> thp_status {
+    std::vector<thp_sys*> sys;
+    thp_status rc = THP_OK;
     ctx->force_brick = bmax;
     ctx->force_fft = fcap;
-    thp_sys* s = nullptr;
-    const thp_status rc = create_system(ctx, d, &s, true);
+    for (int q = 0; q < copies && rc == THP_OK; ++q) {
+      thp_sys* s = nullptr;
+      rc = create_system(ctx, d, &s, true);
+      if (rc == THP_OK) sys.push_back(s);
+    }
     ctx->force_brick = ctx->force_fft = 0;
-    if (rc != THP_OK) return rc;
-    auto body = [&]() -> thp_status {
-      RET(ensure_pme(s));
-      HIPCHK(ctx, hipMemsetAsync(s->view.mu, 0, sizeof(double) * 6 * s->npad, ctx->stream));
-      for (int r = 0; r < 4; ++r) {
-        if (r == 1) HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
-        if (fft) {
-          RET(convolve(ctx, s->one, kGateNone));
-        } else {
-          RET(spread_uind(ctx, s->one, V_MU, 0, kGateNone));
-          RET(launch_gather_uind<0>(ctx, s->one, V_MU, V_HREAL, 0, kGateNone));
-        }
-      }
-      HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
-      HIPCHK(ctx, hipEventSynchronize(e1));
-      HIPCHK(ctx, hipEventElapsedTime(ms, e0, e1));
-      return THP_OK;
-    };
-    const thp_status rb = body();
-    thp_system_destroy(s);
-    return rb;
-  };
-  auto run = [&]() -> thp_status {
-    // brick edges: the distinct brick triples of bmax = 8, 6, 4
-    int best_b = 8;
-    float best = 1e30f;
-    std::vector<std::array<int, 3>> seen;
-    for (int bmax : {8, 6, 4}) {
-      const std::array<int, 3> tri{pick_brick(d->nfft[0], bmax), pick_brick(d->nfft[1], bmax),
-                                   pick_brick(d->nfft[2], bmax)};
-      if (std::find(seen.begin(), seen.end(), tri) != seen.end()) continue;
-      seen.push_back(tri);
-      float ms = 0;
-      RET(measure(bmax, 32, false, &ms));
-      if (ms < best) {
-        best = ms;
-        best_b = bmax;
-      }
+    *ms = 0.0;
+    for (int k : kernels) {
+      if (rc != THP_OK) break;
+      double t = 0.0;
+      rc = thp_time_kernel_batch(sys.data(), (int)sys.size(), k, 4, &t, nullptr);
+      *ms += t;
     }
-    // FFT lines per workgroup: the distinct groupings of cap 32, 16, 8
-    int best_f = 32;
-    best = 1e30f;
-    std::vector<std::array<int, 3>> fseen;
-    for (int fcap : {32, 16, 8}) {
-      const std::array<int, 3> g{fft_group_lines(d->nfft[0], fcap), fft_group_lines(d->nfft[1], fcap),
-                                 fft_group_lines(d->nfft[2], fcap)};
-      if (std::find(fseen.begin(), fseen.end(), g) != fseen.end()) continue;
-      fseen.push_back(g);
-      float ms = 0;
-      RET(measure(best_b, fcap, true, &ms));
-      if (ms < best) {
-        best = ms;
-        best_f = fcap;
-      }
-    }
-    ctx->grid_shape[key] = {best_b, best_f};
-    return THP_OK;
+    for (thp_sys* s : sys) thp_system_destroy(s);
+    return rc;
   };
-  const thp_status rc = run();
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  return rc;
+  // brick edges: the distinct brick triples of bmax = 8, 6, 4, timed over a
+  // spreading and a gather
+  int best_b = 8;
+  double best = 1e30;
+  std::vector<std::array<int, 3>> seen;
+  for (int bmax : {8, 6, 4}) {
+    const std::array<int, 3> tri{pick_brick(d->nfft[0], bmax), pick_brick(d->nfft[1], bmax),
+                                 pick_brick(d->nfft[2], bmax)};
+    if (std::find(seen.begin(), seen.end(), tri) != seen.end()) continue;
+    seen.push_back(tri);
+    double ms = 0;
+    RET(measure(bmax, 32, {1, 2}, &ms));
+    if (ms < best) {
+      best = ms;
+      best_b = bmax;
+    }
+  }
+  // FFT lines per workgroup: the distinct groupings of cap 32, 16, 8, timed
+  // over a grid convolution
+  int best_f = 32;
+  best = 1e30;
+  std::vector<std::array<int, 3>> fseen;
+  for (int fcap : {32, 16, 8}) {
+    const std::array<int, 3> g{fft_group_lines(d->nfft[0], fcap), fft_group_lines(d->nfft[1], fcap),
+                               fft_group_lines(d->nfft[2], fcap)};
+    if (std::find(fseen.begin(), fseen.end(), g) != fseen.end()) continue;
+    fseen.push_back(g);
+    double ms = 0;
+    RET(measure(best_b, fcap, {6}, &ms));
+    if (ms < best) {
+      best = ms;
+      best_f = fcap;
+    }
+  }
+  ctx->grid_shape[key] = {best_b, best_f};
+  return THP_OK;
 }
 
 // local-frame multipoles to the global frame on the device (rotpole)
