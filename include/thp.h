@@ -192,7 +192,10 @@ thp_status thp_efld0_recip(thp_sys* sys, double* cphi /* [n][10] */);
  * local frames get the multipole torques carried to the frame atoms; with
  * global multipoles dep holds the forces at fixed orientations only and
  * torque (optional, [n][3], kcal/mol) returns the torques on the sites.
- * Not for short-range or truncated-CG solves.  THP_ERR_STATE unless the
+ * After a short-range solve (thp_set_shortreal) it is the short-range
+ * gradient: the real-space terms within the short cutoff, no reciprocal or
+ * self terms (epolar1c with use_polarshortreal, v1.2/source/epolar1.f:
+ * 1347-1356).  Not for truncated-CG solves.  THP_ERR_STATE unless the
  * dipoles are a solve of the current coordinates and multipoles (after
  * thp_system_update_coords, _update_multipoles or _set_frames, solve
  * again first). */
@@ -232,6 +235,16 @@ thp_status thp_set_solver(thp_sys* sys, int polalg);
  * (order + 1 or order + 2 applications of the polarization matrix) and has
  * no convergence test; its stats report niter = order. */
 thp_status thp_set_tcg(thp_sys* sys, int order, int precond, int guess, int peek, double omega);
+/* The peek step's omega fitted to the converged energy (tcgomegafit,
+ * v1.2/source/kpolar.f:1059-1326, keywords in MOD_polpot.f:62-96): a PCG
+ * solve at poleps 1e-10 gives E_CG; the truncated CG energy, linear in
+ * omega, is bisected over [-5, 5] toward E_CG (20 steps, within max(1e-4
+ * |E_CG|, 1e-6) kcal/mol), as the reference does every omegafitfreq steps.
+ * On convergence the system's omega is set (*fitted = 1), else it is kept
+ * (*fitted = 0); *omega returns the omega in use.  Needs the truncated CG
+ * with the peek step selected (THP_ERR_STATE otherwise); solve again after
+ * it. */
+thp_status thp_tcg_omega_fit(thp_sys* sys, double* omega, int* fitted);
 /* Short-range real-space mode (the reference's use_polarshortreal, for the
  * inner steps of RESPA1): with shortcut > 0 (at most the system's cutoff;
  * the reference's default is 5 A) the permanent field, the matvec and the

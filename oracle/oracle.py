@@ -400,6 +400,39 @@ def induce_tcg(orc, order=2, prec=True, guess=False, peek=True, omega=1.0):
     return dict(mu=mu, ep=ep, scalars=sc)
 
 
+def tcg_omega_fit(orc, order=2, prec=True, guess=False, omega=1.0):
+    """omegafit (kpolar.f:1059-1326): E_CG from the PCG solve at poleps
+    1e-10; the truncated CG energy with the peek step, linear in omega,
+    from its values at omega = 0 and 1 (the reference's cnp and cr terms);
+    bisection over [-5, 5] toward E_CG, 20 steps, within max(1e-4 |E_CG|,
+    1e-6) kcal/mol.  Returns (omega, fitted, dict of the energies)."""
+    epcg = orc.induce(poleps=1e-10)["ep"]
+    e0 = induce_tcg(orc, order, prec, guess, True, 0.0)["ep"]
+    e1 = induce_tcg(orc, order, prec, guess, True, 1.0)["ep"]
+    er = e1 - e0
+    cvg = abs(epcg * 1e-4)
+    if cvg <= 1e-6:
+        cvg = 1e-6
+    o1, o2 = -5.0, 5.0
+    ep1, ep2 = e0 + o1 * er, e0 + o2 * er
+    w, done = omega, False
+    if abs(er) > cvg:
+        for _ in range(20):
+            if abs(epcg - ep1) <= cvg:
+                w, done = o1, True
+                break
+            if abs(epcg - ep2) <= cvg:
+                w, done = o2, True
+                break
+            if abs(epcg - ep1) >= abs(epcg - ep2):
+                o1 = o1 + (o2 - o1) * 0.5
+                ep1 = e0 + o1 * er
+            else:
+                o2 = o2 - (o2 - o1) * 0.5
+                ep2 = e0 + o2 * er
+    return w, done, dict(epcg=epcg, e0=e0, e1=e1, cvg=cvg)
+
+
 def pcg(afun, ef, diag, mu0, poleps, politer):
     """The preconditioned conjugate gradient of inducepcg_pme2 and
     inducepcg_shortreal (newinduce_pme2.f:285-636; newinduce_shortreal.f:
@@ -972,17 +1005,50 @@ def _virial_kweights(s, qf):
     return W
 
 
-def epolar1_virial(system, rpole, ud, up, frames=None, parts=False):
+def epolar1_short(system, frames, shortcut, poleps=1e-10):
+    """The short-range polarization energy and gradient (epolar1c with
+    use_polarshortreal: newinduce_shortreal's dipoles, and epreal1c's terms
+    within ewaldshortcut, mode SHORTEWALD, epolar1.f:1347-1356 and
+    switch.f:60-62; no reciprocal or self terms), frames carried to the
+    atoms as in epolar1.  Returns (ep, gradient, dipoles)."""
+    import dataclasses
+    s = dataclasses.replace(system, cut=shortcut)
+    xyz = np.stack([s.x, s.y, s.z], axis=1)
+    rp = rotpole(*xyz.T, *frames)
+    s2 = s.with_multipoles(rp)
+    res = induce_shortreal(Oracle(s2), poleps=poleps)
+    ud, up = res["mu"][:, 0], res["mu"][:, 1]
+    F1, T1 = epolar1_real(s2, rp, ud, up)
+    F = F1 + torque_forces_closed(xyz, frames[0], frames[1], frames[2], frames[3], T1)
+    return res["ep"], -F, res
+
+
+def epolar1_virial(system, rpole, ud, up, frames=None, parts=False, real_only=False):
     """D_ab = dG/d eps_ab (kcal/mol) for global multipoles rpole (n, 13) and
     the dipole sets ud, up (n, 3); with frames (axistype, zaxis, xaxis,
     yaxis, pole) the multipoles turn with their frame atoms.  The reference's
     vir is (D + D^T) / 2.  parts: also the real, reciprocal and torque
-    shares."""
+    shares.  real_only: the short-range virial (the system's cut is the
+    short cutoff; no reciprocal or self terms)."""
     s = system
     n = s.n
     xyz = np.stack([s.x, s.y, s.z], axis=1)
     F1, T1, (j, m, R, fp) = _epolar1_real_pairs(s, rpole, ud, up)
     D_real = -0.5 * np.einsum("pa,pb->ab", fp, R)
+    if real_only:
+        D_torque = np.zeros((3, 3))
+        if frames is not None:
+            for i in range(n):
+                if int(frames[0][i]) == AXIS_NONE:
+                    continue
+                Ti = np.zeros((n, 3))
+                Ti[i] = T1[i]
+                Fi = torque_forces_closed(xyz, frames[0], frames[1], frames[2], frames[3], Ti)
+                D_torque -= np.einsum("ka,kb->ab", Fi, xyz - xyz[i])
+        D = D_real + D_torque
+        if parts:
+            return D, dict(real=D_real, recip=np.zeros((3, 3)), torque=D_torque, ksum=np.zeros((3, 3)))
+        return D
     th, nodes = _pme_stencils(s)
     qf = _qfac(s)
     c = rpole[:, 0]

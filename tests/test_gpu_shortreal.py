@@ -79,3 +79,36 @@ def test_short_range_options_are_checked(ctx):
     with pytest.raises(thp.ThpError):
         solver.set_solver(thp.THP_POLALG_TCG)
     solver.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-9), (MIXED, 2e-5)])
+def test_short_range_gradient_and_virial_match_oracle(ctx, prec, tol):
+    # epolar1c with use_polarshortreal (epolar1.f:1347-1356): the real-space
+    # terms within the short cutoff and the frames' torques, after a
+    # short-range solve; the oracle's form is pinned by differences of the
+    # short-range energy
+    import dataclasses
+    from test_oracle_pins import _virial_case
+    short = 3.8
+    s, fr, xyz = _virial_case(0.4)
+    ep, grad, res = O.epolar1_short(s, fr, short, poleps=1e-10)
+    rp = O.rotpole(*xyz.T, *fr)
+    s2 = dataclasses.replace(s, cut=short).with_multipoles(rp)
+    D = O.epolar1_virial(s2, rp, res["mu"][:, 0], res["mu"][:, 1], fr, real_only=True)
+    vref = 0.5 * (D + D.T)
+    sf = s.with_multipoles(np.zeros((s.n, 13)))
+    sf.frames = dict(axistype=fr[0], zaxis=fr[1], xaxis=fr[2], yaxis=fr[3], pole=fr[4])
+    solver = thp.Solver(ctx, sf, prec)
+    solver.set_shortreal(short)
+    _, e_dev, st = solver.newinduce_pme2(poleps=1e-10)
+    assert st.status == 0 and abs(e_dev - ep) < 1e-6 * abs(ep)
+    dep, vir = solver.epolar1_vir()
+    gerr = np.abs(dep - grad).max() / np.abs(grad).max()
+    verr = np.abs(vir - vref).max() / np.abs(vref).max()
+    print(f"short-range gradient prec={prec}: {gerr:.2e}, virial {verr:.2e}", flush=True)
+    assert gerr < tol and verr < tol
+    # the full range again: a short solve's dipoles have no full gradient
+    solver.set_shortreal(0.0)
+    with pytest.raises(thp.ThpError):
+        solver.epolar1()
+    solver.close()

@@ -74,3 +74,26 @@ def test_tcg_options_are_checked(ctx):
     ref = O.Oracle(s).induce(poleps=1e-8)
     assert st.status == thp.THP_OK and abs(ep - ref["ep"]) < 1e-11 * abs(ref["ep"])
     solver.close()
+
+
+@pytest.mark.parametrize("prec", [thp.THP_PREC_FP64, thp.THP_PREC_MIXED])
+def test_tcg_omega_fit_matches_oracle(ctx, prec):
+    # tcgomegafit (kpolar.f:1059-1326) on the device: the omega the oracle's
+    # bisection reaches, then a solve with it within the bisection's
+    # tolerance of the PCG energy
+    from test_oracle_pins import _virial_case
+    s, _, _ = _virial_case(0.4, nfft=16)   # p-scales 0.5: the peek step moves the energy
+    w_ref, done_ref, e = O.tcg_omega_fit(O.Oracle(s), order=2, prec=True)
+    solver = thp.Solver(ctx, s, prec)
+    solver.set_solver(thp.THP_POLALG_TCG)
+    solver.set_tcg(order=2, precond=True, guess=False, peek=True, omega=1.0)
+    w, done = solver.tcg_omega_fit()
+    print(f"omega fit prec={prec}: device {w} ({done}) oracle {w_ref} ({done_ref})", flush=True)
+    assert done == done_ref and done
+    assert w == w_ref
+    _, ep, _ = solver.newinduce_pme2()
+    assert abs(ep - e["epcg"]) <= e["cvg"] * (1 + 1e-6)
+    solver.set_solver(thp.THP_POLALG_PCG)
+    with pytest.raises(thp.ThpError):
+        solver.tcg_omega_fit()      # needs the truncated CG with the peek step
+    solver.close()

@@ -1067,3 +1067,61 @@ def test_planted_virial_mistakes_fail_the_pins(monkeypatch):
     D2, _ = _virial_of(s, fr, xyz, True)
     for D in (D1, D2):
         assert np.abs(np.diag(D) - fd).max() > 1e-3 * np.abs(fd).max()
+
+
+def test_short_range_gradient_matches_differences_of_the_short_energy():
+    # epolar1c with use_polarshortreal: the real-space terms within the
+    # short cutoff, with the frames' torques, against central differences
+    # of the re-solved short-range energy (newinduce_shortreal)
+    s, fr, xyz = _virial_case(0.4)
+    short = 3.8
+    ep, grad, _ = O.epolar1_short(s, fr, short, poleps=1e-12)
+
+    def e_short(x2):
+        s2 = dataclasses.replace(s.with_coords(*x2.T), cut=short)
+        rp = O.rotpole(*x2.T, *fr)
+        return O.induce_shortreal(O.Oracle(s2.with_multipoles(rp)), poleps=1e-12)["ep"]
+
+    h = 2e-4
+    box = s.box[0]
+
+    def clear(a):
+        d = xyz - xyz[a]
+        d -= box * np.round(d / box)
+        r = np.linalg.norm(d, axis=1)
+        r[a] = 0.0
+        return np.min(np.abs(r - short)) > 50 * h
+
+    atoms = [a for a in range(s.n) if clear(a)]
+    for atom, axis in zip([atoms[0], atoms[1], atoms[2], atoms[-1]], (0, 1, 2, 1)):
+        def e_at(dx):
+            x2 = xyz.copy()
+            x2[atom, axis] += dx
+            return e_short(x2)
+        fd = _d4(e_at, h)
+        assert abs(fd) > 1e-2                      # a real force component
+        assert abs(grad[atom, axis] - fd) < 1e-6 * max(1.0, abs(fd)), (atom, axis, grad[atom, axis], fd)
+
+
+def test_tcg_omega_fit_bisects_a_linear_energy_toward_the_converged_one():
+    # omegafit (kpolar.f:1059-1326): the truncated CG energy is linear in
+    # the peek step's omega (checked at a third omega), and the bisection
+    # lands within its tolerance of the PCG energy, next to the root of that
+    # line, which no line of tcg_omega_fit computes
+    # intramolecular p-scales of 0.5: with E_d = E_p and no guess the peek
+    # direction is orthogonal to the field (the CG residual property) and
+    # the energy does not depend on omega at all
+    s, _, _ = _virial_case(0.4, nfft=16)
+    orc = O.Oracle(s)
+    for order, prec in ((2, True), (1, True), (2, False)):
+        w, done, e = O.tcg_omega_fit(orc, order=order, prec=prec)
+        e37 = O.induce_tcg(orc, order, prec, False, True, 0.37)["ep"]
+        assert abs(e37 - (e["e0"] + 0.37 * (e["e1"] - e["e0"]))) < 1e-11 * abs(e37)
+        assert abs(e["e1"] - e["e0"]) > 1e-3 * abs(e["e0"])
+        root = (e["epcg"] - e["e0"]) / (e["e1"] - e["e0"])
+        print(f"order {order} prec {prec}: omega {w} fitted {done}, root {root:.6f}")
+        if -5.0 < root < 5.0:
+            assert done
+            ew = O.induce_tcg(orc, order, prec, False, True, w)["ep"]
+            assert abs(ew - e["epcg"]) <= e["cvg"] * (1 + 1e-9)
+            assert abs(w - root) <= e["cvg"] / abs(e["e1"] - e["e0"]) * (1 + 1e-9)

@@ -558,10 +558,14 @@ thp_status ensure_coef(thp_sys* s) {
 // current coordinates and multipoles
 thp_status epolar1_check(thp_sys* s) {
   thp_ctx* ctx = s->ctx;
-  if (s->one.real_only) return fail(ctx, THP_ERR_ARG, "thp_epolar1: short-range solves have no gradient here");
   if (!s->mu_valid)
     return fail(ctx, THP_ERR_STATE, "thp_epolar1: no solve of the current coordinates and multipoles");
-  if (s->mu_short) return fail(ctx, THP_ERR_ARG, "thp_epolar1: the last solve was a short-range solve");
+  // a short-range solve has the short-range gradient (the real-space terms
+  // within the short cutoff: epolar1c with use_polarshortreal,
+  // epolar1.f:1347-1356, mode SHORTEWALD of switch.f:60-62), a full solve
+  // the full one
+  if (s->mu_short != (s->one.real_only != 0))
+    return fail(ctx, THP_ERR_ARG, "thp_epolar1: the last solve's range is not the system's current range");
   // truncated-CG dipoles are not stationary: their gradient (epolar1tcg)
   // has terms of its own
   if (s->mu_polalg == THP_POLALG_TCG)
@@ -584,12 +588,16 @@ thp_status epolar1_launch(thp_sys* s, double* F, double* T, double* vpart = null
   hipStream_t hs = ctx->stream;
   const Batch& b = s->one;
   const bool vir = vpart != nullptr;
+  // short range: the real-space terms within the short cutoff (cut2) only
+  const bool recip = !b.real_only;
   const size_t gbytes = grid_points(s) * 2 * (s->prec == THP_PREC_FP64 ? sizeof(double) : sizeof(float));
   void* spec_m = vir ? spec : nullptr;
   void* spec_u = vir ? static_cast<char*>(spec) + gbytes : nullptr;
-  RET(field(ctx, b, s->cphi, 0, 0, kGateNone, spec_m));
-  RET(spread_uind(ctx, b, V_MU, 0, kGateNone));
-  RET(convolve(ctx, b, kGateNone, spec_u));
+  if (recip) {
+    RET(field(ctx, b, s->cphi, 0, 0, kGateNone, spec_m));
+    RET(spread_uind(ctx, b, V_MU, 0, kGateNone));
+    RET(convolve(ctx, b, kGateNone, spec_u));
+  }
   const size_t nb = (size_t)s->nblk, nt = (size_t)(s->n + 255) / 256;
   double* v_real = vpart;
   double* v_recip = vir ? vpart + 6 * nb : nullptr;
@@ -598,7 +606,8 @@ thp_status epolar1_launch(thp_sys* s, double* F, double* T, double* vpart = null
   if (vir) hipLaunchKernelGGL((k_ep1_real<4, true>), g1(b.n_blk), dim3(256), 0, hs, b.d_views, b.d_blk, F, T, v_real);
   else hipLaunchKernelGGL((k_ep1_real<4, false>), g1(b.n_blk), dim3(256), 0, hs, b.d_views, b.d_blk, F, T, v_real);
   RET(launch_check(ctx, "k_ep1_real"));
-  if (s->prec == THP_PREC_FP64) {
+  if (!recip) {
+  } else if (s->prec == THP_PREC_FP64) {
     if (vir) hipLaunchKernelGGL((k_ep1_recip<double, true>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T, v_recip);
     else hipLaunchKernelGGL((k_ep1_recip<double, false>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T, v_recip);
   } else {
@@ -614,7 +623,7 @@ thp_status epolar1_launch(thp_sys* s, double* F, double* T, double* vpart = null
       hipLaunchKernelGGL((k_ep1_torque<false>), ga, dim3(256), 0, hs, s->d_view, s->d_frames, s->d_tinv_off, s->d_tinv, T, F, v_torque);
     RET(launch_check(ctx, "k_ep1_torque"));
   }
-  if (vir) {
+  if (vir && recip) {
     const dim3 gk((unsigned)((grid_points(s) + kVirChunk - 1) / kVirChunk));
     if (s->prec == THP_PREC_FP64)
       hipLaunchKernelGGL((k_vir_ksum<double>), gk, dim3(256), 0, hs, s->d_view, spec_m, spec_u, v_ksum);
@@ -1774,8 +1783,10 @@ thp_status thp_epolar1_vir(thp_sys* s, double* dep, double* torque, double* vir)
     HIPCHK(s->ctx, hipMemcpy(part.data(), s->grad_vir, sizeof(double) * part.size(), hipMemcpyDeviceToHost));
     const size_t nb = (size_t)s->nblk, nt = (size_t)(s->n + 255) / 256;
     double v[6] = {0, 0, 0, 0, 0, 0};
+    const bool recip = !s->one.real_only;
     for (size_t q = 0; q < slots; ++q) {
-      if (!s->has_frames && q >= 2 * nb && q < 2 * nb + nt) continue;  // no frame forces
+      if (!s->has_frames && q >= 2 * nb && q < 2 * nb + nt) continue;                  // no frame forces
+      if (!recip && ((q >= nb && q < 2 * nb) || q >= 2 * nb + nt)) continue;           // short range
       for (int c = 0; c < 6; ++c) v[c] += part[6 * q + c];
     }
     const double f2 = 0.5 * kCoulombFp;
@@ -2014,7 +2025,7 @@ thp_status thp_epolar1_noalloc(thp_sys* s, double* dep, double* torque, void* st
   RET(epolar1_check(s));
   // a solve of the current coordinates left the lists and B-spline tables
   // in place; rebuilding them here would allocate
-  if (!s->list_valid || !s->tight_valid || !s->pme_valid)
+  if (!s->list_valid || !s->tight_valid || (!s->pme_valid && !s->one.real_only))
     return fail(ctx, THP_ERR_STATE, "thp_epolar1_noalloc: lists or B-spline tables are not current");
   hipStream_t cs = static_cast<hipStream_t>(stream);
   RET(order_in(ctx, cs));
@@ -2301,6 +2312,74 @@ thp_status thp_induce(thp_sys* s, const thp_solve_opts* o, double* uind, double*
     }
   if (ep) *ep = sp->ep;
   return sp->status == THP_ERR_NAN ? fail(s->ctx, THP_ERR_NAN, "non-finite residual") : THP_OK;
+}
+
+thp_status thp_tcg_omega_fit(thp_sys* s, double* omega, int* fitted) {
+  if (!s || !omega) return THP_ERR_ARG;
+  if (s->polalg != THP_POLALG_TCG || !s->tcg_peek)
+    return fail(s->ctx, THP_ERR_STATE, "thp_tcg_omega_fit: the truncated CG with the peek step is not selected");
+  if (s->short_cut > 0.0) return fail(s->ctx, THP_ERR_STATE, "thp_tcg_omega_fit: not in short-range mode");
+  // the reference energy: the PCG solve at poleps 1e-10 (kpolar.f:1107-1110)
+  thp_solve_opts o{1e-10, 500, 0};
+  thp_solve_stats st{};
+  s->polalg = THP_POLALG_PCG;
+  const thp_status rc = run_solve(s, &o, false, &st);
+  s->polalg = THP_POLALG_TCG;
+  RET(rc);
+  if (st.status != THP_OK) return fail(s->ctx, st.status, "thp_tcg_omega_fit: the reference PCG solve failed");
+  const double epcg = st.ep;
+  // E_TCG(omega) = -f/2 <mu_d(omega), E_p> is linear in omega: the peek
+  // step adds omega times a fixed vector.  Its two coefficients, -f/2 <munp,
+  // E_p> and -f/2 <peek direction, E_p> (cnp and cr of omegafit), from the
+  // solves at omega = 0 and 1.
+  const double keep = s->tcg_omega;
+  double e[2];
+  for (int k = 0; k < 2; ++k) {
+    s->tcg_omega = double(k);
+    thp_solve_stats t{};
+    const thp_status r2 = run_solve(s, &o, false, &t);
+    if (r2 != THP_OK) {
+      s->tcg_omega = keep;
+      return r2;
+    }
+    e[k] = t.ep;
+  }
+  s->tcg_omega = keep;
+  // omegafit (kpolar.f:1059-1326): bisection over [-5, 5] toward the PCG
+  // energy, 20 steps, within max(1e-4 |E_CG|, 1e-6) kcal/mol; no fit when the
+  // peek step's energy is itself that small
+  const double enp = e[0], er = e[1] - e[0];
+  double cvg = std::fabs(epcg * 1e-4);
+  if (cvg <= 1e-6) cvg = 1e-6;
+  double o1 = -5.0, o2 = 5.0;
+  double ep1 = enp + o1 * er, ep2 = enp + o2 * er;
+  bool cvged = false;
+  double w = keep;
+  if (std::fabs(er) > cvg) {
+    for (int it = 0; it < 20; ++it) {
+      if (std::fabs(epcg - ep1) <= cvg) {
+        w = o1;
+        cvged = true;
+        break;
+      } else if (std::fabs(epcg - ep2) <= cvg) {
+        w = o2;
+        cvged = true;
+        break;
+      }
+      if (std::fabs(epcg - ep1) >= std::fabs(epcg - ep2)) {
+        o1 = o1 + (o2 - o1) * 0.5;
+        ep1 = enp + o1 * er;
+      } else {
+        o2 = o2 - (o2 - o1) * 0.5;
+        ep2 = enp + o2 * er;
+      }
+    }
+  }
+  if (cvged) s->tcg_omega = w;
+  *omega = s->tcg_omega;
+  if (fitted) *fitted = cvged ? 1 : 0;
+  s->mu_valid = false;  // the last solve was a fitting solve
+  return THP_OK;
 }
 
 thp_status thp_time_kernel(thp_sys* s, int kernel, int reps, double* ms, int64_t* pairs) {

@@ -156,6 +156,7 @@ def load_lib(path: str | None = None) -> C.CDLL:
         "thp_epolar1_noalloc": (C.c_int, [vp, vp, vp, vp]),
         "thp_serve_set_frames": (C.c_int, [vp, C.c_int, _ip, _ip, _ip, _ip, _dp]),
         "thp_grid_shape": (C.c_int, [vp, _ip, C.c_int, _ip, _ip]),
+        "thp_tcg_omega_fit": (C.c_int, [vp, _dp, _ip]),
         "thp_serve_start": (C.c_int, [vp, C.POINTER(ServeOpts)]),
         "thp_serve_add_system": (C.c_int, [vp, C.POINTER(SystemDesc), _ip]),
         "thp_serve_submit": (C.c_int, [vp, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_uint64)]),
@@ -485,6 +486,13 @@ class Solver:
         """Options of the truncated CG (the reference's defaults)."""
         self._c(self.lib.thp_set_tcg(self.h, int(order), int(bool(precond)), int(bool(guess)), int(bool(peek)),
                                      float(omega)), "set_tcg")
+
+    def tcg_omega_fit(self):
+        """Fit the peek step's omega to the PCG energy at poleps 1e-10
+        (tcgomegafit); returns (omega in use, fitted)."""
+        w, f = C.c_double(0), C.c_int(0)
+        self._c(self.lib.thp_tcg_omega_fit(self.h, C.byref(w), C.byref(f)), "tcg_omega_fit")
+        return w.value, bool(f.value)
 
     def set_predictor(self, method: int):
         """THP_PRED_NONE/GEAR/ASPC/LSQR; clears the stored solves."""
