@@ -195,7 +195,10 @@ thp_status thp_efld0_recip(thp_sys* sys, double* cphi /* [n][10] */);
  * After a short-range solve (thp_set_shortreal) it is the short-range
  * gradient: the real-space terms within the short cutoff, no reciprocal or
  * self terms (epolar1c with use_polarshortreal, v1.2/source/epolar1.f:
- * 1347-1356).  Not for truncated-CG solves.  THP_ERR_STATE unless the
+ * 1347-1356).  After a truncated-CG solve it is that energy's own
+ * gradient (epolar1tcg, v1.2/source/epolar1tcg.f): the truncated dipoles
+ * are not stationary, and the adjoint of the truncated CG's steps adds the
+ * terms the stationary form leaves out.  THP_ERR_STATE unless the
  * dipoles are a solve of the current coordinates and multipoles (after
  * thp_system_update_coords, _update_multipoles or _set_frames, solve
  * again first). */
@@ -208,7 +211,8 @@ thp_status thp_epolar1(thp_sys* sys, double* dep, double* torque);
  * their local frames (or stay fixed in the global frame without frames).
  * Real-space pair shares, the reciprocal sum's box and moment terms and the
  * frame forces' shares accumulate in FP64 and are summed in a fixed order.
- * vir may be NULL (then the call is thp_epolar1). */
+ * vir may be NULL (then the call is thp_epolar1).  No virial for
+ * truncated-CG solves (THP_ERR_ARG). */
 thp_status thp_epolar1_vir(thp_sys* sys, double* dep, double* torque, double* vir);
 thp_status thp_tmatxb(thp_sys* sys, const double* mu, double* efi, int dodiag);
 thp_status thp_tmatxb_recip(thp_sys* sys, const double* mu, double* dipfield);
@@ -248,9 +252,11 @@ thp_status thp_tcg_omega_fit(thp_sys* sys, double* omega, int* fitted);
 /* Short-range real-space mode (the reference's use_polarshortreal, for the
  * inner steps of RESPA1): with shortcut > 0 (at most the system's cutoff;
  * the reference's default is 5 A) the permanent field, the matvec and the
- * solves of thp_induce (PCG or Jacobi/DIIS) use only the real-space Ewald
- * terms of pairs within shortcut: no reciprocal field and no self term.
- * The predictor is not used in this mode.  0 restores the full solve. */
+ * solves of thp_induce (PCG, Jacobi/DIIS or the truncated CG: the
+ * reference's polalgshort) use only the real-space Ewald terms of pairs
+ * within shortcut: no reciprocal field and no self term.  The predictor
+ * and the omega fit are not used in this mode.  0 restores the full
+ * solve. */
 thp_status thp_set_shortreal(thp_sys* sys, double shortcut);
 /* Select the induced-dipole predictor (THP_PRED_*) and clear its history.
  * While a predictor is set, every solve of the system stores its converged
@@ -303,7 +309,8 @@ thp_status thp_fphi_uind_dev(thp_sys* sys, const double* qgrid, double* fdip_phi
  * their gather live in a file-scope device array of 6 * THP_NOALLOC_MAX_ATOMS
  * doubles (6 MiB) that each device holds until the process ends, so the call
  * allocates nothing: THP_ERR_ARG for systems of more than
- * THP_NOALLOC_MAX_ATOMS atoms (rounded up to 64), THP_ERR_STATE unless the
+ * THP_NOALLOC_MAX_ATOMS atoms (rounded up to 64) and for truncated-CG
+ * solves (their gradient's adjoint allocates), THP_ERR_STATE unless the
  * last solve of the current coordinates left its lists and B-spline tables
  * in place (as thp_induce does).  Every call on a device shares that
  * device's array: calls that overlap on one device (two streams, two

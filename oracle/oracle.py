@@ -387,14 +387,23 @@ def tcg(afun, ef, alpha, order=2, prec=True, guess=False, peek=True, omega=1.0):
     return mu, sc
 
 
-def induce_tcg(orc, order=2, prec=True, guess=False, peek=True, omega=1.0):
+def induce_tcg(orc, order=2, prec=True, guess=False, peek=True, omega=1.0, short=False):
     """The reference's polalg = 3 energy of one system (epolar3tcg1/2):
     dipoles from tcg() on the full permanent field, E_pol = -1/2 f
-    ef_p . mu_d.  Returns dict(mu, ep)."""
+    ef_p . mu_d.  short: the short-range form of the RESPA1 inner steps
+    (epolar1tcg1shortreal.f:100-312 and its order-2 siblings): the direct
+    field and the real-space matvec with its diagonal at the oracle's
+    cutoff, which the caller sets to the short cutoff, no reciprocal or self
+    terms.  Returns dict(mu, ep)."""
     s = orc.sys
-    ef = orc.efld0_total()
+    if short:
+        ef = orc.efld0_direct()
+        afun = lambda v: orc.tmatxb_pme(v, dodiag=True)
+    else:
+        ef = orc.efld0_total()
+        afun = orc.matvec
     alpha = np.asarray(s.polarity, dtype=np.float64)[:, None, None]
-    mu, sc = tcg(orc.matvec, ef, alpha, order, prec, guess, peek, omega)
+    mu, sc = tcg(afun, ef, alpha, order, prec, guess, peek, omega)
     f = 332.063714  # electric (MOD_units.f:81)
     ep = -0.5 * f * float(np.sum(mu[:, 0, :] * ef[:, 1, :]))
     return dict(mu=mu, ep=ep, scalars=sc)
@@ -1082,3 +1091,176 @@ def epolar1_virial(system, rpole, ud, up, frames=None, parts=False, real_only=Fa
     if parts:
         return D, dict(real=D_real, recip=D_recip, torque=D_torque, ksum=0.5 * KCOUL * Dk)
     return D
+
+
+# Gradient of the truncated-CG energy (epolar1tcg: epolar1tcg.f:16-119 and
+# epolar1tcg2.f, for tcgorder 1 and 2 with the tcgprec, tcgguess and
+# tcgpeek options).  E_TCG = -f/2 <mu_d, E_p> with mu_d an explicit function
+# of the permanent fields and of the polarization matrix A = 1/alpha - T
+# (truncated CG is not stationary in the dipoles).  Reverse-mode
+# differentiation of that function gives
+#   dE = <efb, dE_d> + <Epb, dE_p> + sum_k <lam_k, dA v_k>
+# with adjoint vectors efb, Epb and one (lam, v) pair per application of A
+# in the forward pass; dA = -dT, and each term is one of the derivatives
+# the PCG gradient already takes (a dipole set in a permanent field, two
+# dipole sets in T).
+
+def _tcg_adjoint(A1, efd, Ep, alpha, order, prec, guess, peek, omega, f):
+    """(mu_d, efb, Epb, pairs) of the right-hand side d; A1 applies A to an
+    (n, 3) vector, alpha (n, 1)"""
+    P = (lambda v: alpha * v) if prec else (lambda v: v)
+    Q = (lambda v: v) if prec else (lambda v: alpha * v)   # the peek step's alpha
+    dot = lambda a, b: float(np.sum(a * b))
+    # forward
+    if guess:
+        m0 = alpha * efd
+        q0 = A1(m0)
+        r0 = efd - q0
+    else:
+        r0 = efd.copy()
+    z0 = P(r0)
+    w1 = A1(z0)
+    y1 = P(w1)
+    n0, t1 = dot(r0, z0), dot(z0, w1)
+    t4 = n0 / t1
+    if order == 2:
+        w2 = A1(y1)
+        y2 = P(w2)
+        np1 = dot(w1, y1)
+        t2 = n0 * np1 / t1 ** 2
+        p2 = t2 * y1 - t4 * y2
+        s3 = dot(w1, p2)
+        t3 = t1 * s3
+        num = t1 ** 2 - n0 * np1
+        g1 = num / t3
+        cz, cy = g1 * t2 + t4, -g1 * t4
+    else:
+        cz, cy = t4, 0.0
+    mu = cz * z0 + cy * y1
+    if guess:
+        mu = mu + m0
+    if peek:
+        pk = omega * Q(z0) - omega * t4 * Q(y1)
+        if order == 2:
+            pk = pk - omega * g1 * t2 * Q(y1) + omega * g1 * t4 * Q(y2)
+        mu = mu + pk
+    # reverse
+    mub = -0.5 * f * Ep
+    Epb = -0.5 * f * mu
+    zeros = np.zeros_like(efd)
+    z0b, y1b, y2b, w1b, w2b, r0b = (zeros.copy() for _ in range(6))
+    t4b = g1b = t2b = t1b = n0b = np1b = 0.0
+    z0b += cz * mub
+    y1b += cy * mub
+    czb, cyb = dot(mub, z0), dot(mub, y1)
+    m0b = mub.copy() if guess else None
+    if peek:
+        qm = Q(mub)
+        z0b += omega * qm
+        y1b += -omega * t4 * qm
+        t4b += -omega * dot(qm, y1)
+        if order == 2:
+            y1b += -omega * g1 * t2 * qm
+            y2b += omega * g1 * t4 * qm
+            g1b += -omega * t2 * dot(qm, y1) + omega * t4 * dot(qm, y2)
+            t2b += -omega * g1 * dot(qm, y1)
+            t4b += omega * g1 * dot(qm, y2)
+    pairs = []
+    if order == 2:
+        g1b += czb * t2 - cyb * t4
+        t2b += czb * g1
+        t4b += czb - cyb * g1
+        # g1 = num / t3, num = t1^2 - n0 np1, t3 = t1 s3
+        t1b += g1b * 2.0 * t1 / t3
+        n0b += -g1b * np1 / t3
+        np1b += -g1b * n0 / t3
+        t3b = -g1b * num / t3 ** 2
+        t1b += t3b * s3
+        s3b = t3b * t1
+        w1b += s3b * p2
+        p2b = s3b * w1
+        t2b += dot(p2b, y1)
+        y1b += t2 * p2b
+        t4b += -dot(p2b, y2)
+        y2b += -t4 * p2b
+        # t2 = n0 np1 / t1^2
+        n0b += t2b * np1 / t1 ** 2
+        np1b += t2b * n0 / t1 ** 2
+        t1b += -2.0 * t2b * n0 * np1 / t1 ** 3
+        # np1 = <w1, y1>
+        w1b += np1b * y1
+        y1b += np1b * w1
+        # y2 = P w2, w2 = A y1
+        w2b += P(y2b)
+        y1b += A1(w2b)
+        pairs.append((w2b, y1))
+    else:
+        t4b += czb
+    # t4 = n0 / t1, t1 = <z0, w1>, n0 = <r0, z0>
+    n0b += t4b / t1
+    t1b += -t4b * n0 / t1 ** 2
+    z0b += t1b * w1
+    w1b += t1b * z0
+    r0b += n0b * z0
+    z0b += n0b * r0
+    # y1 = P w1, w1 = A z0, z0 = P r0
+    w1b += P(y1b)
+    z0b += A1(w1b)
+    pairs.append((w1b, z0))
+    r0b += P(z0b)
+    if guess:
+        efb = r0b.copy()
+        q0b = -r0b
+        m0b = m0b + A1(q0b)
+        pairs.append((q0b, m0))
+        efb = efb + alpha * m0b
+    else:
+        efb = r0b
+    return mu, efb, Epb, pairs
+
+
+def epolar1_tcg(system, frames, order=2, prec=True, guess=False, peek=True, omega=1.0, shortcut=None):
+    """E_TCG and its gradient (kcal/mol/A) for local multipoles on frames:
+    the adjoint of the truncated CG (_tcg_adjoint), then the gradient
+    pieces of epolar1_real and epolar1_recip_self combined with it, the
+    torques carried to the frame atoms.  shortcut: the short-range form
+    (epolar1tcg1shortreal, epolar1tcg2*shortreal): real-space terms within
+    shortcut only.  Returns (ep, gradient)."""
+    import dataclasses
+    s = system if shortcut is None else dataclasses.replace(system, cut=shortcut)
+    xyz = np.stack([s.x, s.y, s.z], axis=1)
+    rp = rotpole(*xyz.T, *frames)
+    s2 = s.with_multipoles(rp)
+    orc = Oracle(s2)
+    alpha = np.asarray(s.polarity, dtype=np.float64)[:, None]
+    if shortcut is None:
+        ef = orc.efld0_total()
+        A1 = lambda v: orc.matvec(np.stack([v, v], axis=1))[:, 0]
+    else:
+        ef = orc.efld0_direct()
+        A1 = lambda v: orc.tmatxb_pme(np.stack([v, v], axis=1), dodiag=True)[:, 0]
+    f = KCOUL
+    mu, efb, Epb, pairs = _tcg_adjoint(A1, ef[:, 0], ef[:, 1], alpha, order, prec, guess, peek, omega, f)
+    ep = -0.5 * f * float(np.sum(mu * ef[:, 1]))
+
+    def FG(ud, up):
+        F1, T1 = epolar1_real(s2, rp, ud, up)
+        if shortcut is not None:
+            return F1, T1
+        F2, T2 = epolar1_recip_self(s2, rp, ud, up)
+        return F1 + F2, T1 + T2
+
+    z = np.zeros_like(efb)
+    F, T = FG(z, efb)
+    F2, T2 = FG(Epb, z)
+    F, T = F + F2, T + T2
+    for lam, v in pairs:
+        Fa, Ta = FG(v, lam)
+        Fb, Tb = FG(z, lam)
+        Fc, Tc = FG(v, z)
+        F = F - (Fa - Fb - Fc)
+        T = T - (Ta - Tb - Tc)
+    # dE/dr = (2/f) F-combination; forces and torques of E are -(2/f) times
+    FE, TE = -(2.0 / f) * F, -(2.0 / f) * T
+    Ftot = FE + torque_forces_closed(xyz, frames[0], frames[1], frames[2], frames[3], TE)
+    return ep, -Ftot

@@ -71,13 +71,11 @@ def test_short_range_options_are_checked(ctx):
         solver.set_shortreal(s.cut + 1.0)
     with pytest.raises(thp.ThpError):
         solver.set_shortreal(-1.0)
+    # the truncated CG's omega fit is for the full solve
     solver.set_solver(thp.THP_POLALG_TCG)
-    with pytest.raises(thp.ThpError):
-        solver.set_shortreal(SHORT)  # the reference's short solve takes polalg 1 or 2
-    solver.set_solver(thp.THP_POLALG_PCG)
     solver.set_shortreal(SHORT)
     with pytest.raises(thp.ThpError):
-        solver.set_solver(thp.THP_POLALG_TCG)
+        solver.tcg_omega_fit()
     solver.close()
 
 
@@ -112,3 +110,32 @@ def test_short_range_gradient_and_virial_match_oracle(ctx, prec, tol):
     with pytest.raises(thp.ThpError):
         solver.epolar1()
     solver.close()
+
+
+@pytest.mark.parametrize("order,prec_tcg,guess,peek", [(1, True, False, False), (2, True, True, True),
+                                                       (2, False, False, True), (1, False, True, True)])
+def test_short_range_tcg_energy_and_gradient_match_oracle(ctx, order, prec_tcg, guess, peek):
+    # polalgshort = 3 (epolar1.f:29-31): epolar1tcg1shortreal and
+    # epolar1tcg2*shortreal, the truncated CG on the short-range operators
+    # with its own gradient (the adjoint of the truncated steps), FP64 and
+    # mixed; the oracle's gradient is pinned by differences of its energy
+    from test_oracle_pins import _virial_case
+    s, fr, _ = _virial_case(0.4)
+    short = 3.8
+    ep, grad = O.epolar1_tcg(s, fr, order, prec_tcg, guess, peek, 0.8, shortcut=short)
+    sf = s.with_multipoles(np.zeros((s.n, 13)))
+    sf.frames = dict(axistype=fr[0], zaxis=fr[1], xaxis=fr[2], yaxis=fr[3], pole=fr[4])
+    for prec, etol, gtol in ((FP64, 1e-11, 1e-9), (MIXED, 1e-6, 2e-5)):
+        solver = thp.Solver(ctx, sf, prec)
+        solver.set_solver(thp.THP_POLALG_TCG)
+        solver.set_tcg(order, prec_tcg, guess, peek, 0.8)
+        solver.set_shortreal(short)
+        _, e_dev, st = solver.newinduce_pme2()
+        dep = solver.epolar1()
+        erel = abs(e_dev - ep) / abs(ep)
+        gerr = np.abs(dep - grad).max() / np.abs(grad).max()
+        print(f"short tcg{order} prec_tcg={prec_tcg} guess={guess} peek={peek} prec={prec}: "
+              f"erel {erel:.1e} gradient {gerr:.1e}", flush=True)
+        assert st.status == thp.THP_OK and st.niter == order
+        assert erel < etol and gerr < gtol
+        solver.close()

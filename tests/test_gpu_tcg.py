@@ -97,3 +97,56 @@ def test_tcg_omega_fit_matches_oracle(ctx, prec):
     with pytest.raises(thp.ThpError):
         solver.tcg_omega_fit()      # needs the truncated CG with the peek step
     solver.close()
+
+
+@pytest.mark.parametrize("order,prec,guess,peek", list(itertools.product([1, 2], [True, False], [True, False],
+                                                                         [True, False])))
+def test_tcg_gradient_matches_oracle_fp64(ctx, order, prec, guess, peek):
+    # epolar1tcg: the adjoint of the truncated CG with the gradient kernels'
+    # permanent-field and dipole terms, against the oracle's (pinned by
+    # differences of the truncated energy); local frames carry the torques
+    from test_oracle_pins import _virial_case
+    s, fr, _ = _virial_case(0.4)
+    ep, grad = O.epolar1_tcg(s, fr, order, prec, guess, peek, 0.8)
+    sf = s.with_multipoles(np.zeros((s.n, 13)))
+    sf.frames = dict(axistype=fr[0], zaxis=fr[1], xaxis=fr[2], yaxis=fr[3], pole=fr[4])
+    solver = thp.Solver(ctx, sf, FP64)
+    solver.set_solver(thp.THP_POLALG_TCG)
+    solver.set_tcg(order, prec, guess, peek, 0.8)
+    _, e_dev, st = solver.newinduce_pme2()
+    assert st.status == thp.THP_OK and abs(e_dev - ep) < 1e-11 * abs(ep)
+    dep = solver.epolar1()
+    gerr = np.abs(dep - grad).max() / np.abs(grad).max()
+    print(f"tcg{order} prec={prec} guess={guess} peek={peek}: gradient {gerr:.1e}", flush=True)
+    assert gerr < 1e-9
+    # the solve's dipoles are kept: the gradient again is the same
+    assert np.array_equal(solver.epolar1(), dep)
+    solver.close()
+
+
+def test_tcg_gradient_mixed_and_refusals(ctx):
+    from test_oracle_pins import _virial_case
+    s, fr, _ = _virial_case(0.4)
+    ep, grad = O.epolar1_tcg(s, fr)
+    sf = s.with_multipoles(np.zeros((s.n, 13)))
+    sf.frames = dict(axistype=fr[0], zaxis=fr[1], xaxis=fr[2], yaxis=fr[3], pole=fr[4])
+    solver = thp.Solver(ctx, sf, MIXED)
+    solver.set_solver(thp.THP_POLALG_TCG)
+    solver.set_tcg(2)
+    solver.newinduce_pme2()
+    dep = solver.epolar1()
+    gerr = np.abs(dep - grad).max() / np.abs(grad).max()
+    print(f"tcg2 mixed: gradient {gerr:.1e}", flush=True)
+    assert gerr < 2e-5
+    # no virial for the truncated CG, and the no-allocation entry refuses it
+    with pytest.raises(thp.ThpError):
+        solver.epolar1_vir()
+    from test_gpu_boundary import _Hip
+    hip = _Hip()
+    dep_d = hip.alloc(24 * s.n)
+    try:
+        assert solver.lib.thp_epolar1_noalloc(solver.h, dep_d, None, None) == -1  # THP_ERR_ARG
+        assert b"truncated-CG" in solver.lib.thp_last_error(ctx.h)
+    finally:
+        hip.free_all()
+    solver.close()

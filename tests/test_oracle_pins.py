@@ -1125,3 +1125,90 @@ def test_tcg_omega_fit_bisects_a_linear_energy_toward_the_converged_one():
             ew = O.induce_tcg(orc, order, prec, False, True, w)["ep"]
             assert abs(ew - e["epcg"]) <= e["cvg"] * (1 + 1e-9)
             assert abs(w - root) <= e["cvg"] / abs(e["e1"] - e["e0"]) * (1 + 1e-9)
+
+
+@pytest.mark.parametrize("order,prec,guess,peek", [(2, True, False, True), (1, True, True, False),
+                                                   (2, False, True, True), (1, False, False, True)])
+def test_tcg_gradient_matches_differences_of_the_truncated_energy(order, prec, guess, peek):
+    # epolar1tcg (epolar1tcg.f): the truncated CG's dipoles are not
+    # stationary, so its gradient carries the adjoint terms; against central
+    # differences of the truncated energy re-solved at displaced coordinates
+    # (frames rotated with them), for atoms clear of the real-space cutoff
+    s, fr, xyz = _virial_case(0.4)
+    ep, grad = O.epolar1_tcg(s, fr, order, prec, guess, peek, 0.8)
+
+    def e_tcg(x2):
+        rp = O.rotpole(*x2.T, *fr)
+        orc = O.Oracle(s.with_coords(*x2.T).with_multipoles(rp))
+        return O.induce_tcg(orc, order, prec, guess, peek, 0.8)["ep"]
+
+    assert abs(e_tcg(xyz) - ep) < 1e-12 * abs(ep)
+    h = 2e-4
+    box = s.box[0]
+
+    def clear(a):
+        d = xyz - xyz[a]
+        d -= box * np.round(d / box)
+        r = np.linalg.norm(d, axis=1)
+        r[a] = 0.0
+        return np.min(np.abs(r - s.cut)) > 50 * h
+
+    atoms = [a for a in range(s.n) if clear(a)]
+    # the stationary-dipole gradient (the same dipoles, no adjoint terms)
+    # misses the truncation's share: the check below must be able to tell
+    rp = O.rotpole(*xyz.T, *fr)
+    s2 = s.with_multipoles(rp)
+    mu = O.induce_tcg(O.Oracle(s2), order, prec, guess, peek, 0.8)["mu"]
+    F1, T1 = O.epolar1_real(s2, rp, mu[:, 0], mu[:, 1])
+    F2, T2 = O.epolar1_recip_self(s2, rp, mu[:, 0], mu[:, 1])
+    naive = -(F1 + F2 + O.torque_forces_closed(xyz, *fr[:4], T1 + T2))
+    for atom, axis in zip([atoms[0], atoms[1], atoms[len(atoms) // 2], atoms[-1]], (0, 1, 2, 1)):
+        def e_at(dx):
+            x2 = xyz.copy()
+            x2[atom, axis] += dx
+            return e_tcg(x2)
+        fd = _d4(e_at, h)
+        assert abs(fd) > 1e-2
+        assert abs(grad[atom, axis] - fd) < 1e-6 * max(1.0, abs(fd)), (atom, axis, grad[atom, axis], fd)
+        assert abs(naive[atom, axis] - fd) > 1e-4 * max(1.0, abs(fd))
+
+
+@pytest.mark.parametrize("order,prec,guess,peek", [(1, True, False, False), (2, True, True, True),
+                                                   (2, False, False, True)])
+def test_short_range_tcg_gradient_matches_differences_of_the_short_energy(order, prec, guess, peek):
+    # epolar1tcg1shortreal / epolar1tcg2*shortreal: the truncated CG on the
+    # short-range operators (direct field and real-space matvec within the
+    # short cutoff, no reciprocal or self terms), against central
+    # differences of the re-solved short-range truncated energy
+    s, fr, xyz = _virial_case(0.4)
+    short = 3.8
+    ep, grad = O.epolar1_tcg(s, fr, order, prec, guess, peek, 0.8, shortcut=short)
+
+    def e_short(x2):
+        s2 = dataclasses.replace(s.with_coords(*x2.T), cut=short)
+        rp = O.rotpole(*x2.T, *fr)
+        return O.induce_tcg(O.Oracle(s2.with_multipoles(rp)), order, prec, guess, peek, 0.8, short=True)["ep"]
+
+    assert abs(e_short(xyz) - ep) < 1e-12 * abs(ep)
+    # the short energy is not the full one
+    full = O.induce_tcg(O.Oracle(s.with_multipoles(O.rotpole(*xyz.T, *fr))), order, prec, guess, peek, 0.8)["ep"]
+    assert abs(full - ep) > 1e-3 * abs(ep)
+    h = 2e-4
+    box = s.box[0]
+
+    def clear(a):
+        d = xyz - xyz[a]
+        d -= box * np.round(d / box)
+        r = np.linalg.norm(d, axis=1)
+        r[a] = 0.0
+        return np.min(np.abs(r - short)) > 50 * h
+
+    atoms = [a for a in range(s.n) if clear(a)]
+    for atom, axis in zip([atoms[0], atoms[1], atoms[2], atoms[-1]], (0, 1, 2, 1)):
+        def e_at(dx):
+            x2 = xyz.copy()
+            x2[atom, axis] += dx
+            return e_short(x2)
+        fd = _d4(e_at, h)
+        assert abs(fd) > 1e-2
+        assert abs(grad[atom, axis] - fd) < 1e-6 * max(1.0, abs(fd)), (atom, axis, grad[atom, axis], fd)

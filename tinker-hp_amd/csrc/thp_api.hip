@@ -566,10 +566,6 @@ thp_status epolar1_check(thp_sys* s) {
   // the full one
   if (s->mu_short != (s->one.real_only != 0))
     return fail(ctx, THP_ERR_ARG, "thp_epolar1: the last solve's range is not the system's current range");
-  // truncated-CG dipoles are not stationary: their gradient (epolar1tcg)
-  // has terms of its own
-  if (s->mu_polalg == THP_POLALG_TCG)
-    return fail(ctx, THP_ERR_ARG, "thp_epolar1: no gradient for truncated-CG solves");
   return THP_OK;
 }
 
@@ -583,7 +579,20 @@ thp_status epolar1_check(thp_sys* s) {
 // pairs' (one slot per block), the reciprocal moments' (one per block), the
 // frame forces' (one per 256 atoms, with frames) and the reciprocal k-sum's
 // (one per kVirChunk grid points), in that order.
-thp_status epolar1_launch(thp_sys* s, double* F, double* T, double* vpart = nullptr, void* spec = nullptr) {
+template <typename real>
+thp_status launch_ep1_recip(thp_ctx* ctx, const Batch& b, const double* cphi, double* F, double* T, double* vpart,
+                            int terms) {
+  const dim3 g = g1(b.n_blk), t(64);
+  hipStream_t hs = ctx->stream;
+  if (vpart) hipLaunchKernelGGL((k_ep1_recip<real, true, kTermsAll>), g, t, 0, hs, b.d_views, b.d_blk, cphi, F, T, vpart);
+  else if (terms == kTermsAll) hipLaunchKernelGGL((k_ep1_recip<real, false, kTermsAll>), g, t, 0, hs, b.d_views, b.d_blk, cphi, F, T, vpart);
+  else if (terms == kTermsField) hipLaunchKernelGGL((k_ep1_recip<real, false, kTermsField>), g, t, 0, hs, b.d_views, b.d_blk, cphi, F, T, vpart);
+  else hipLaunchKernelGGL((k_ep1_recip<real, false, kTermsDipole>), g, t, 0, hs, b.d_views, b.d_blk, cphi, F, T, vpart);
+  return launch_check(ctx, "k_ep1_recip");
+}
+
+thp_status epolar1_launch(thp_sys* s, double* F, double* T, double* vpart = nullptr, void* spec = nullptr,
+                          int terms = kTermsAll, bool finish = true) {
   thp_ctx* ctx = s->ctx;
   hipStream_t hs = ctx->stream;
   const Batch& b = s->one;
@@ -603,19 +612,20 @@ thp_status epolar1_launch(thp_sys* s, double* F, double* T, double* vpart = null
   double* v_recip = vir ? vpart + 6 * nb : nullptr;
   double* v_torque = vir ? vpart + 12 * nb : nullptr;
   double* v_ksum = vir ? vpart + 12 * nb + 6 * nt : nullptr;
-  if (vir) hipLaunchKernelGGL((k_ep1_real<4, true>), g1(b.n_blk), dim3(256), 0, hs, b.d_views, b.d_blk, F, T, v_real);
-  else hipLaunchKernelGGL((k_ep1_real<4, false>), g1(b.n_blk), dim3(256), 0, hs, b.d_views, b.d_blk, F, T, v_real);
+  // the virial takes every term; the truncated CG's gradient one mask at a time
+  if (vir && terms != kTermsAll) return fail(ctx, THP_ERR_ARG, "epolar1_launch: the virial needs every term");
+  const dim3 gb = g1(b.n_blk);
+  if (vir) hipLaunchKernelGGL((k_ep1_real<4, true, kTermsAll>), gb, dim3(256), 0, hs, b.d_views, b.d_blk, F, T, v_real);
+  else if (terms == kTermsAll) hipLaunchKernelGGL((k_ep1_real<4, false, kTermsAll>), gb, dim3(256), 0, hs, b.d_views, b.d_blk, F, T, v_real);
+  else if (terms == kTermsField) hipLaunchKernelGGL((k_ep1_real<4, false, kTermsField>), gb, dim3(256), 0, hs, b.d_views, b.d_blk, F, T, v_real);
+  else hipLaunchKernelGGL((k_ep1_real<4, false, kTermsDipole>), gb, dim3(256), 0, hs, b.d_views, b.d_blk, F, T, v_real);
   RET(launch_check(ctx, "k_ep1_real"));
-  if (!recip) {
-  } else if (s->prec == THP_PREC_FP64) {
-    if (vir) hipLaunchKernelGGL((k_ep1_recip<double, true>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T, v_recip);
-    else hipLaunchKernelGGL((k_ep1_recip<double, false>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T, v_recip);
-  } else {
-    if (vir) hipLaunchKernelGGL((k_ep1_recip<float, true>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T, v_recip);
-    else hipLaunchKernelGGL((k_ep1_recip<float, false>), g1(b.n_blk), dim3(64), 0, hs, b.d_views, b.d_blk, s->cphi, F, T, v_recip);
+  if (recip) {
+    if (s->prec == THP_PREC_FP64) RET(launch_ep1_recip<double>(ctx, b, s->cphi, F, T, v_recip, terms));
+    else RET(launch_ep1_recip<float>(ctx, b, s->cphi, F, T, v_recip, terms));
   }
-  RET(launch_check(ctx, "k_ep1_recip"));
   const dim3 ga((s->n + 255) / 256);
+  if (!finish) return THP_OK;  // the raw forces and torques of the chosen terms
   if (s->has_frames) {
     if (vir)
       hipLaunchKernelGGL((k_ep1_torque<true>), ga, dim3(256), 0, hs, s->d_view, s->d_frames, s->d_tinv_off, s->d_tinv, T, F, v_torque);
@@ -633,6 +643,226 @@ thp_status epolar1_launch(thp_sys* s, double* F, double* T, double* vpart = null
   }
   hipLaunchKernelGGL(k_ep1_finish, ga, dim3(256), 0, hs, s->n, s->npad, F, T, -1.0);
   return launch_check(ctx, "k_ep1_finish");
+}
+
+// Gradient of the truncated-CG energy (epolar1tcg, epolar1tcg.f:16-119;
+// epolar1tcg2.f): E_TCG = -f/2 <mu_d, E_p> with mu_d an explicit function of
+// the permanent fields and of A = 1/alpha - T.  The adjoint of that function
+// (reverse-mode differentiation of the truncated CG's steps, in FP64 on the
+// host, A applied on the device as the solve applies it) gives
+//   dE = <efb, dE_d> + <Epb, dE_p> - sum_k <lam_k, dT v_k>,
+// and the device's gradient kernels evaluate the first two terms as the
+// permanent-field terms with the dipole sets (u_d, u_p) = (Epb, efb) and
+// each pair as the dipole term with (u_d, u_p) = (v_k, lam_k).  Returns the
+// forces -dE/dr at fixed orientations and the torques ([3][npad] each,
+// sorted order, kcal/mol/A and kcal/mol).
+thp_status tcg_apply(thp_sys* s, int vin);
+
+thp_status epolar1_tcg_terms(thp_sys* s, std::vector<double>& Fo, std::vector<double>& To) {
+  thp_ctx* ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  const size_t np = s->npad;
+  const int n = s->n;
+  typedef std::vector<double> Vec;  // [3][np], rhs d
+  auto down = [&](const double* dev, size_t comp0, Vec& out) -> thp_status {
+    out.assign(3 * np, 0.0);
+    HIPCHK(ctx, hipMemcpyAsync(out.data(), dev + comp0 * np, sizeof(double) * 3 * np, hipMemcpyDeviceToHost, st));
+    return sync(s);
+  };
+  // the solve's fields (total: direct, reciprocal and self) and dipoles
+  Vec efd, Ep, alpha(3 * np, 0.0);
+  RET(down(s->view.ef, 0, efd));
+  RET(down(s->view.ef, 3, Ep));
+  {
+    std::vector<double> pol(np);
+    HIPCHK(ctx, hipMemcpy(pol.data(), s->view.pol, sizeof(double) * np, hipMemcpyDeviceToHost));
+    for (int c = 0; c < 3; ++c)
+      for (int i = 0; i < n; ++i) alpha[c * np + i] = pol[i];
+  }
+  std::vector<double> mu_keep(6 * np);
+  HIPCHK(ctx, hipMemcpy(mu_keep.data(), s->view.mu, sizeof(double) * 6 * np, hipMemcpyDeviceToHost));
+  // A v on the device (both right-hand sides carry v; A is the same)
+  const double term = s->view.term_self;
+  auto A1 = [&](const Vec& v, Vec& out) -> thp_status {
+    std::vector<double> w(6 * np);
+    std::copy(v.begin(), v.end(), w.begin());
+    std::copy(v.begin(), v.end(), w.begin() + 3 * np);
+    HIPCHK(ctx, hipMemcpy(s->view.pp, w.data(), sizeof(double) * 6 * np, hipMemcpyHostToDevice));
+    RET(tcg_apply(s, V_PP));
+    Vec hr, h;
+    RET(down(s->view.hreal, 0, hr));
+    RET(down(s->view.h, 0, h));
+    out.assign(3 * np, 0.0);
+    for (size_t q = 0; q < 3 * np; ++q) out[q] = hr[q] + h[q] - term * v[q];
+    return THP_OK;
+  };
+  auto dot = [&](const Vec& a, const Vec& b) {
+    double t = 0.0;
+    for (int c = 0; c < 3; ++c)
+      for (int i = 0; i < n; ++i) t += a[c * np + i] * b[c * np + i];
+    return t;
+  };
+  auto axpy = [&](Vec& y, double a, const Vec& x) {
+    for (size_t q = 0; q < y.size(); ++q) y[q] += a * x[q];
+  };
+  auto mulal = [&](const Vec& v) {
+    Vec o(v.size());
+    for (size_t q = 0; q < v.size(); ++q) o[q] = alpha[q] * v[q];
+    return o;
+  };
+  const bool prec = s->mu_tcg[1], guess = s->mu_tcg[2], peek = s->mu_tcg[3];
+  const int order = s->mu_tcg[0];
+  const double om = s->mu_tcg_omega, f = kCoulombFp;
+  auto P = [&](const Vec& v) { return prec ? mulal(v) : v; };
+  auto Qm = [&](const Vec& v) { return prec ? v : mulal(v); };
+  // forward (right-hand side d)
+  Vec m0, q0, r0;
+  if (guess) {
+    m0 = mulal(efd);
+    RET(A1(m0, q0));
+    r0 = efd;
+    axpy(r0, -1.0, q0);
+  } else {
+    r0 = efd;
+  }
+  const Vec z0 = P(r0);
+  Vec w1, w2, y2;
+  RET(A1(z0, w1));
+  const Vec y1 = P(w1);
+  const double n0 = dot(r0, z0), t1 = dot(z0, w1), t4 = n0 / t1;
+  double t2 = 0, np1 = 0, s3 = 0, t3 = 0, num = 0, g1 = 0, cz = t4, cy = 0;
+  Vec p2;
+  if (order == 2) {
+    RET(A1(y1, w2));
+    y2 = P(w2);
+    np1 = dot(w1, y1);
+    t2 = n0 * np1 / (t1 * t1);
+    p2 = y1;
+    for (auto& x : p2) x *= t2;
+    axpy(p2, -t4, y2);
+    s3 = dot(w1, p2);
+    t3 = t1 * s3;
+    num = t1 * t1 - n0 * np1;
+    g1 = num / t3;
+    cz = g1 * t2 + t4;
+    cy = -g1 * t4;
+  }
+  Vec mu(3 * np, 0.0);
+  axpy(mu, cz, z0);
+  if (order == 2) axpy(mu, cy, y1);
+  if (guess) axpy(mu, 1.0, m0);
+  if (peek) {
+    axpy(mu, om, Qm(z0));
+    axpy(mu, -om * t4, Qm(y1));
+    if (order == 2) {
+      axpy(mu, -om * g1 * t2, Qm(y1));
+      axpy(mu, om * g1 * t4, Qm(y2));
+    }
+  }
+  // reverse
+  Vec mub(3 * np, 0.0), Epb(3 * np, 0.0);
+  axpy(mub, -0.5 * f, Ep);
+  axpy(Epb, -0.5 * f, mu);
+  Vec z0b(3 * np, 0.0), y1b(3 * np, 0.0), y2b(3 * np, 0.0), w1b(3 * np, 0.0), w2b(3 * np, 0.0), r0b(3 * np, 0.0);
+  double t4b = 0, g1b = 0, t2b = 0, t1b = 0, n0b = 0, np1b = 0;
+  axpy(z0b, cz, mub);
+  if (order == 2) axpy(y1b, cy, mub);
+  const double czb = dot(mub, z0), cyb = order == 2 ? dot(mub, y1) : 0.0;
+  if (peek) {
+    const Vec qm = Qm(mub);
+    axpy(z0b, om, qm);
+    axpy(y1b, -om * t4, qm);
+    t4b += -om * dot(qm, y1);
+    if (order == 2) {
+      axpy(y1b, -om * g1 * t2, qm);
+      axpy(y2b, om * g1 * t4, qm);
+      g1b += -om * t2 * dot(qm, y1) + om * t4 * dot(qm, y2);
+      t2b += -om * g1 * dot(qm, y1);
+      t4b += om * g1 * dot(qm, y2);
+    }
+  }
+  std::vector<std::pair<Vec, Vec>> pairs;  // (lam, v)
+  if (order == 2) {
+    g1b += czb * t2 - cyb * t4;
+    t2b += czb * g1;
+    t4b += czb - cyb * g1;
+    t1b += g1b * 2.0 * t1 / t3;
+    n0b += -g1b * np1 / t3;
+    np1b += -g1b * n0 / t3;
+    const double t3b = -g1b * num / (t3 * t3);
+    t1b += t3b * s3;
+    const double s3b = t3b * t1;
+    axpy(w1b, s3b, p2);
+    Vec p2b = w1;
+    for (auto& x : p2b) x *= s3b;
+    t2b += dot(p2b, y1);
+    axpy(y1b, t2, p2b);
+    t4b += -dot(p2b, y2);
+    axpy(y2b, -t4, p2b);
+    n0b += t2b * np1 / (t1 * t1);
+    np1b += t2b * n0 / (t1 * t1);
+    t1b += -2.0 * t2b * n0 * np1 / (t1 * t1 * t1);
+    axpy(w1b, np1b, y1);
+    axpy(y1b, np1b, w1);
+    axpy(w2b, 1.0, P(y2b));
+    Vec aw2;
+    RET(A1(w2b, aw2));
+    axpy(y1b, 1.0, aw2);
+    pairs.emplace_back(w2b, y1);
+  } else {
+    t4b += czb;
+  }
+  n0b += t4b / t1;
+  t1b += -t4b * n0 / (t1 * t1);
+  axpy(z0b, t1b, w1);
+  axpy(w1b, t1b, z0);
+  axpy(r0b, n0b, z0);
+  axpy(z0b, n0b, r0);
+  axpy(w1b, 1.0, P(y1b));
+  Vec aw1;
+  RET(A1(w1b, aw1));
+  axpy(z0b, 1.0, aw1);
+  pairs.emplace_back(w1b, z0);
+  axpy(r0b, 1.0, P(z0b));
+  Vec efb = r0b;
+  if (guess) {
+    Vec q0b = r0b;
+    for (auto& x : q0b) x = -x;
+    Vec aq;
+    RET(A1(q0b, aq));
+    Vec m0b = mub;
+    axpy(m0b, 1.0, aq);
+    pairs.emplace_back(q0b, m0);
+    axpy(efb, 1.0, mulal(m0b));
+  }
+  // the gradient terms: dE/dr = F_raw(fields; Epb, efb) - sum F_raw(dipole; v, lam)
+  if (!s->grad_f) {
+    RET(dalloc(s, &s->grad_f, 3 * np));
+    RET(dalloc(s, &s->grad_t, 3 * np));
+  }
+  Fo.assign(3 * np, 0.0);
+  To.assign(3 * np, 0.0);
+  auto run = [&](const Vec& ud, const Vec& up, int terms, double sign) -> thp_status {
+    std::vector<double> w(6 * np);
+    std::copy(ud.begin(), ud.end(), w.begin());
+    std::copy(up.begin(), up.end(), w.begin() + 3 * np);
+    HIPCHK(ctx, hipMemcpy(s->view.mu, w.data(), sizeof(double) * 6 * np, hipMemcpyHostToDevice));
+    RET(epolar1_launch(s, s->grad_f, s->grad_t, nullptr, nullptr, terms, false));
+    Vec Fr, Tr;
+    RET(down(s->grad_f, 0, Fr));
+    RET(down(s->grad_t, 0, Tr));
+    // forces -dE/dr and torques -dE/dtheta: minus the raw combination
+    for (size_t q = 0; q < 3 * np; ++q) {
+      Fo[q] -= sign * Fr[q];
+      To[q] -= sign * Tr[q];
+    }
+    return THP_OK;
+  };
+  thp_status rc = run(Epb, efb, kTermsField, 1.0);
+  for (size_t k = 0; rc == THP_OK && k < pairs.size(); ++k) rc = run(pairs[k].second, pairs[k].first, kTermsDipole, -1.0);
+  // the solve's dipoles back in place
+  HIPCHK(ctx, hipMemcpy(s->view.mu, mu_keep.data(), sizeof(double) * 6 * np, hipMemcpyHostToDevice));
+  return rc;
 }
 
 // slots of the virial shares written by epolar1_launch
@@ -1762,6 +1992,25 @@ thp_status thp_epolar1_vir(thp_sys* s, double* dep, double* torque, double* vir)
   RET(epolar1_check(s));
   RET(ensure_lists(s));
   RET(ensure_pme(s));
+  if (s->mu_polalg == THP_POLALG_TCG) {
+    // the truncated CG's own gradient; no virial for it here
+    if (vir) return fail(s->ctx, THP_ERR_ARG, "thp_epolar1_vir: no virial for truncated-CG solves");
+    std::vector<double> Fo, To;
+    RET(epolar1_tcg_terms(s, Fo, To));
+    const size_t np = s->npad;
+    HIPCHK(s->ctx, hipMemcpy(s->grad_f, Fo.data(), sizeof(double) * 3 * np, hipMemcpyHostToDevice));
+    HIPCHK(s->ctx, hipMemcpy(s->grad_t, To.data(), sizeof(double) * 3 * np, hipMemcpyHostToDevice));
+    const dim3 ga((s->n + 255) / 256);
+    if (s->has_frames) {
+      hipLaunchKernelGGL((k_ep1_torque<false>), ga, dim3(256), 0, s->ctx->stream, s->d_view, s->d_frames, s->d_tinv_off,
+                         s->d_tinv, s->grad_t, s->grad_f, (double*)nullptr);
+      RET(launch_check(s->ctx, "k_ep1_torque"));
+    }
+    RET(gather_out(s, dep, s->grad_f, 3));
+    for (size_t q = 0; q < (size_t)3 * s->n; ++q) dep[q] = -dep[q];  // gradient = -force
+    if (torque) RET(gather_out(s, torque, s->grad_t, 3));
+    return THP_OK;
+  }
   const size_t np = s->npad;
   if (!s->grad_f) {
     RET(dalloc(s, &s->grad_f, 3 * np));
@@ -2023,6 +2272,10 @@ thp_status thp_epolar1_noalloc(thp_sys* s, double* dep, double* torque, void* st
                 s->npad, THP_NOALLOC_MAX_ATOMS);
   if (!ctx->scratch) return fail(ctx, THP_ERR_STATE, "thp_epolar1_noalloc: no scratch on device %d", ctx->device);
   RET(epolar1_check(s));
+  // truncated-CG dipoles are not stationary: their gradient takes the host
+  // adjoint of thp_epolar1, which allocates
+  if (s->mu_polalg == THP_POLALG_TCG)
+    return fail(ctx, THP_ERR_ARG, "thp_epolar1_noalloc: not for truncated-CG solves");
   // a solve of the current coordinates left the lists and B-spline tables
   // in place; rebuilding them here would allocate
   if (!s->list_valid || !s->tight_valid || (!s->pme_valid && !s->one.real_only))
@@ -2114,8 +2367,12 @@ thp_status tcg_apply(thp_sys* s, int vin) {
   thp_ctx* ctx = s->ctx;
   const Batch& b = s->one;
   RET(b_pair_tmatxb(ctx, b, vin, V_HREAL, 1, kGateNone));
-  RET(spread_uind(ctx, b, vin, 0, kGateNone));
-  RET(convolve(ctx, b, kGateNone));
+  if (!b.real_only) {
+    RET(spread_uind(ctx, b, vin, 0, kGateNone));
+    RET(convolve(ctx, b, kGateNone));
+  }
+  // short range: the gather writes a zero field into h (A = 1/alpha - T
+  // within the short cutoff, no self term)
   return launch_gather_uind<0>(ctx, b, vin, V_H, 0, kGateNone);
 }
 
@@ -2189,6 +2446,11 @@ thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_s
   s->mu_valid = true;
   s->mu_polalg = s->polalg;
   s->mu_short = short_range;
+  s->mu_tcg[0] = s->tcg_order;
+  s->mu_tcg[1] = s->tcg_prec;
+  s->mu_tcg[2] = s->tcg_guess;
+  s->mu_tcg[3] = s->tcg_peek;
+  s->mu_tcg_omega = s->tcg_omega;
   if (stats) {
     float t1 = 0, t2 = 0;
     (void)hipEventElapsedTime(&t1, s->ev[0], s->ev[1]);
@@ -2213,8 +2475,6 @@ thp_status thp_set_solver(thp_sys* s, int polalg) {
   if (!s) return THP_ERR_ARG;
   if (polalg != THP_POLALG_PCG && polalg != THP_POLALG_JACOBI_DIIS && polalg != THP_POLALG_TCG)
     return fail(s->ctx, THP_ERR_ARG, "unknown polalg %d (1: PCG, 2: Jacobi/DIIS, 3: truncated CG)", polalg);
-  if (polalg == THP_POLALG_TCG && s->short_cut > 0.0)
-    return fail(s->ctx, THP_ERR_ARG, "the short-range solve takes polalg 1 or 2 (newinduce_shortreal.f:134-141)");
   if (polalg == THP_POLALG_TCG && !s->tcg_vec) {
     s->tcg_blocks = (s->n + 255) / 256;
     RET(dalloc(s, &s->tcg_vec, (size_t)2 * 6 * s->npad));
@@ -2237,8 +2497,6 @@ thp_status thp_set_shortreal(thp_sys* s, double shortcut) {
   if (!s) return THP_ERR_ARG;
   if (!(shortcut >= 0.0) || shortcut > s->cut)
     return fail(s->ctx, THP_ERR_ARG, "short-range cutoff %g must lie in (0, %g] (0 turns the mode off)", shortcut, s->cut);
-  if (shortcut > 0.0 && s->polalg == THP_POLALG_TCG)
-    return fail(s->ctx, THP_ERR_ARG, "the short-range solve takes polalg 1 or 2 (newinduce_shortreal.f:134-141)");
   const bool on = shortcut > 0.0;
   s->short_cut = shortcut;
   s->one.real_only = on ? 1 : 0;

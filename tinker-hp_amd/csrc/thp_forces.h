@@ -142,13 +142,20 @@ __device__ __forceinline__ void ep1_ladders(double r2, double aew, double pdi, d
     for (int q = 0; q < 4; ++q) G[l][q] = bn[q + 1] - (1.0 - s[l] * lam[q]) * rr[q];
 }
 
-// terms A, B, C of one pair; Gd, Gp, Gu the ladders of the d, p and u scales
+// terms A, B, C of one pair; Gd, Gp, Gu the ladders of the d, p and u
+// scales; TERMS: bit 0 the permanent-field terms A and B, bit 1 the dipole
+// term C (the truncated CG's gradient takes them with different dipole
+// sets; a compile-time mask, so the full gradient's code is unchanged)
+constexpr int kTermsField = 1, kTermsDipole = 2, kTermsAll = 3;
+template <int TERMS>
 __device__ __forceinline__ void ep1_pair(const Site& si, const Site& sk, V3 R, const double* Gd, const double* Gp,
                                          const double* Gu, V3& F, V3& T) {
-  F = F - w_ddphi_mpole(si.up, sk.c, sk.d, sk.q, R, Gd) - w_ddphi_mpole(si.ud, sk.c, sk.d, sk.q, R, Gp);
-  mpole_in_dipole(si, sk.up, R, Gd, F, T);
-  mpole_in_dipole(si, sk.ud, R, Gp, F, T);
-  F = F - w_ddphi_dip(si.up, sk.ud, R, Gu) - w_ddphi_dip(si.ud, sk.up, R, Gu);
+  if (TERMS & kTermsField) {
+    F = F - w_ddphi_mpole(si.up, sk.c, sk.d, sk.q, R, Gd) - w_ddphi_mpole(si.ud, sk.c, sk.d, sk.q, R, Gp);
+    mpole_in_dipole(si, sk.up, R, Gd, F, T);
+    mpole_in_dipole(si, sk.ud, R, Gp, F, T);
+  }
+  if (TERMS & kTermsDipole) F = F - w_ddphi_dip(si.up, sk.ud, R, Gu) - w_ddphi_dip(si.ud, sk.up, R, Gu);
 }
 
 // Virial shares (the strain derivative of G, symmetrised, unscaled like F:
@@ -169,7 +176,7 @@ __device__ __forceinline__ void vir_pair(V3 f, double rx, double ry, double rz, 
 // partners (within the cutoff) and every S-th of its scaled pairs (with
 // their d, p, u factors); the S partial forces and torques are summed in
 // LDS in a fixed order.  FP64 throughout (one launch per gradient).
-template <int S, bool VIR>
+template <int S, bool VIR, int TERMS>
 __global__ void __launch_bounds__(64 * S) k_ep1_real(const SysView* views, const Work* work, double* F, double* T,
                                                      double* vpart) {
   __shared__ double red[S][6][64];
@@ -200,7 +207,7 @@ __global__ void __launch_bounds__(64 * S) k_ep1_real(const SysView* views, const
     double G[3][4];
     ep1_ladders(r2, sv.aewald, pdi, sv.pdamp[k], thi, sv.thole[k], one, 1, G);
     const V3 f0 = f;
-    ep1_pair(si, load_site(sv, k), v3(xr, yr, zr), G[0], G[0], G[0], f, t);
+    ep1_pair<TERMS>(si, load_site(sv, k), v3(xr, yr, zr), G[0], G[0], G[0], f, t);
     if (VIR) vir_pair(f - f0, xr, yr, zr, vw);
   }
   for (int e = sv.excl_off[i] + wv; e < sv.excl_off[i + 1]; e += S) {
@@ -214,7 +221,7 @@ __global__ void __launch_bounds__(64 * S) k_ep1_real(const SysView* views, const
     double G[3][4];
     ep1_ladders(r2, sv.aewald, pdi, sv.pdamp[k], thi, sv.thole[k], sc, 3, G);
     const V3 f0 = f;
-    ep1_pair(si, load_site(sv, k), v3(xr, yr, zr), G[0], G[1], G[2], f, t);
+    ep1_pair<TERMS>(si, load_site(sv, k), v3(xr, yr, zr), G[0], G[1], G[2], f, t);
     if (VIR) vir_pair(f - f0, xr, yr, zr, vw);
   }
   }
@@ -256,7 +263,7 @@ __global__ void __launch_bounds__(64 * S) k_ep1_real(const SysView* views, const
 // Reciprocal and self parts, one thread per atom, added to F and T.  The
 // grid holds the convolved induced grids (u_d real, u_p imaginary); cphi
 // the Cartesian potential derivatives of the multipole grid ([10][npad]).
-template <typename real, bool VIR>
+template <typename real, bool VIR, int TERMS>
 __global__ void __launch_bounds__(64) k_ep1_recip(const SysView* views, const Work* work, const double* cphi,
                                                    double* F, double* T, double* vpart) {
   const Work w = work[blockIdx.x];
@@ -368,14 +375,17 @@ __global__ void __launch_bounds__(64) k_ep1_recip(const SysView* views, const Wo
                c8 = cphi[8 * np + i], c9 = cphi[9 * np + i];
   const double PM[3][3] = {{c4, c7, c8}, {c7, c5, c9}, {c8, c9, c6}};
   double f[3], t[3];
+  constexpr bool tf = (TERMS & kTermsField) != 0, tu = (TERMS & kTermsDipole) != 0;
   for (int b = 0; b < 3; ++b) {
     const double E = -D1[b];
-    double v = si.c * E;
+    double v = tf ? si.c * E : 0.0;
     for (int a = 0; a < 3; ++a) {
-      v -= (ud[a] + up[a]) * PM[a][b];        // A
-      v -= dm[a] * D2s[a][b];                  // B, dipole
-      v -= up[a] * D2d[a][b] + ud[a] * D2p[a][b];  // C
-      for (int c = 0; c < 3; ++c) v -= qm[a][c] * D3[a][c][b];  // B, quadrupole
+      if (tf) {
+        v -= (ud[a] + up[a]) * PM[a][b];        // A
+        v -= dm[a] * D2s[a][b];                  // B, dipole
+        for (int c = 0; c < 3; ++c) v -= qm[a][c] * D3[a][c][b];  // B, quadrupole
+      }
+      if (tu) v -= up[a] * D2d[a][b] + ud[a] * D2p[a][b];  // C
     }
     f[b] = v;
   }
@@ -390,9 +400,9 @@ __global__ void __launch_bounds__(64) k_ep1_recip(const SysView* views, const Wo
       A[a][b] = s;
     }
   const V3 self = sv.term_self * cross(si.d, si.ud + si.up);
-  t[0] = dE.x - (A[1][2] - A[2][1]) + self.x;
-  t[1] = dE.y - (A[2][0] - A[0][2]) + self.y;
-  t[2] = dE.z - (A[0][1] - A[1][0]) + self.z;
+  t[0] = tf ? dE.x - (A[1][2] - A[2][1]) + self.x : 0.0;
+  t[1] = tf ? dE.y - (A[2][0] - A[0][2]) + self.y : 0.0;
+  t[2] = tf ? dE.z - (A[0][1] - A[1][0]) + self.z : 0.0;
   if (ok)
     for (int c = 0; c < 3; ++c) {
       F[(size_t)c * np + i] += f[c];

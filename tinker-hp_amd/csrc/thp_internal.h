@@ -322,6 +322,10 @@ struct thp_sys {
   bool mu_valid = false;
   int mu_polalg = 1;
   bool mu_short = false;
+  // the truncated-CG options of the last solve (order, prec, guess, peek)
+  // and its omega: the gradient differentiates that solve
+  int mu_tcg[4] = {2, 1, 0, 1};
+  double mu_tcg_omega = 1.0;
   // forces and torques of the gradient ([3][npad] each), made on first use
   double* grad_f = nullptr;
   double* grad_t = nullptr;
