@@ -334,6 +334,27 @@ thp_status spread_mpole(thp_ctx* ctx, const Batch& b, int gate) {
 // the grids' convolution with qfac: x and y forward passes, the fused z
 // pass (forward, qfac, backward), y and x backward passes
 // (spec: one system's batch only; its spectrum is kept there, see MODE 3)
+template <int P>
+void launch_plane64_p(thp_ctx* ctx, const Batch& b, int gate, int mode) {
+  if (mode == 0) hipLaunchKernelGGL((k_fft_plane64<P, 0>), g1(b.n_fft), dim3(64 * P), 0, ctx->stream, b.d_views, b.d_fft, gate);
+  else hipLaunchKernelGGL((k_fft_plane64<P, 1>), g1(b.n_fft), dim3(64 * P), 0, ctx->stream, b.d_views, b.d_fft, gate);
+}
+void launch_plane64(thp_ctx* ctx, const Batch& b, int gate, int mode) {
+  if (ctx->fft_pp == 4) launch_plane64_p<4>(ctx, b, gate, mode);
+  else if (ctx->fft_pp == 2) launch_plane64_p<2>(ctx, b, gate, mode);
+  else launch_plane64_p<1>(ctx, b, gate, mode);
+}
+template <int P>
+void launch_z64_p(thp_ctx* ctx, const Batch& b, int gate, void* spec) {
+  if (spec) hipLaunchKernelGGL((k_fft_z64<P, 3>), g1(b.n_fft), dim3(64), 0, ctx->stream, b.d_views, b.d_fft, gate, spec);
+  else hipLaunchKernelGGL((k_fft_z64<P, 2>), g1(b.n_fft), dim3(64), 0, ctx->stream, b.d_views, b.d_fft, gate, spec);
+}
+void launch_z64(thp_ctx* ctx, const Batch& b, int gate, void* spec) {
+  if (ctx->fft_zp == 4) launch_z64_p<4>(ctx, b, gate, spec);
+  else if (ctx->fft_zp == 2) launch_z64_p<2>(ctx, b, gate, spec);
+  else launch_z64_p<1>(ctx, b, gate, spec);
+}
+
 template <typename real>
 void launch_convolve(thp_ctx* ctx, const Batch& b, int gate, void* spec) {
   const dim3 g = g1(b.n_fft), t(256);
@@ -343,6 +364,16 @@ void launch_convolve(thp_ctx* ctx, const Batch& b, int gate, void* spec) {
     if (spec) hipLaunchKernelGGL((k_fft_axis<real, 2, 3>), g, t, 0, st, b.d_views, b.d_fft, gate, spec);
     else hipLaunchKernelGGL((k_fft_axis<real, 2, 2>), g, t, 0, st, b.d_views, b.d_fft, gate, none);
   };
+  if constexpr (sizeof(real) == 4) {
+    if (b.fft_r64 && ctx->reg_fft) {
+      // 64^3 grids: the transforms in registers, three passes (P lanes per
+      // line in the plane and z passes: ctx->fft_pp, ctx->fft_zp)
+      launch_plane64(ctx, b, gate, 0);
+      launch_z64(ctx, b, gate, spec);
+      launch_plane64(ctx, b, gate, 1);
+      return;
+    }
+  }
   if (b.fft_plane && ctx->plane_fft && sizeof(real) == 4) {
     // x and y fused per plane, then the z pass: three passes over the grid
     hipLaunchKernelGGL((k_fft_plane<real, 0, kPlaneThreads>), g, dim3(kPlaneThreads), b.fft_plane_lds, st, b.d_views, b.d_fft, gate);
@@ -1088,7 +1119,11 @@ thp_status b_pair_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int do
 // both inside a capture and in eager launches.  Otherwise the matvec runs
 // in line.
 inline bool forks(const thp_ctx* ctx, const Batch& b) {
-  return ctx->side && b.fork_ok && !b.real_only && b.nsys == 1;
+  if (!ctx->side || b.real_only) return false;
+  if (b.fork_ok && b.nsys == 1) return true;
+  if (!ctx->fork_batch) return false;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(ctx->stream, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone;
 }
 
 thp_status fork_matvec(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int gate) {
@@ -1243,10 +1278,9 @@ void append_work(const thp_sys* s, int index, std::vector<Work>& blk, std::vecto
   for (int q = 0; q < s->nbricks; ++q) brick.push_back({index, q});
   const int nconv = (int)((grid_points(s) + kConvChunk - 1) / kConvChunk);
   for (int q = 0; q < nconv; ++q) conv.push_back({index, q});
-  // line groups of the three axes, and at least one entry per z plane
+  // line groups of the three axes, one entry per z plane, the register z pass
   const int fc = s->view.fft_cap;
-  const int ng = std::max(std::max(fft_groups(s->nfft, 0, fc), s->nfft[2]),
-                          std::max(fft_groups(s->nfft, 1, fc), fft_groups(s->nfft, 2, fc)));
+  const int ng = fft_work_count(s->nfft, fc);
   for (int q = 0; q < ng; ++q) fft.push_back({index, q});
 }
 
@@ -1280,6 +1314,12 @@ thp_status thp_init(thp_ctx** out, int device) {
                                        hipFuncAttributeMaxDynamicSharedMemorySize, kPlaneLds) == hipSuccess;
   if (const char* e = getenv("THP_FFT_PLANE"))
     if (atoi(e) == 0) ctx->plane_fft = false;
+  if (const char* e = getenv("THP_FORK_BATCH")) ctx->fork_batch = atoi(e) != 0;
+  if (const char* e = getenv("THP_FFT_REG"))
+    if (atoi(e) == 0) ctx->reg_fft = false;
+  // lanes per line of the register FFT's plane and z passes (1, 2 or 4)
+  if (const char* e = getenv("THP_FFT_PP")) ctx->fft_pp = atoi(e);
+  if (const char* e = getenv("THP_FFT_ZP")) ctx->fft_zp = atoi(e);
   // a second stream (default priority) for the real-space matvec beside the
   // reciprocal chain; THP_FORK=0 keeps every launch on one stream
   const char* fe = getenv("THP_FORK");
@@ -1693,8 +1733,7 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
     H2D(s->d_convwork, cw.data(), sizeof(Work) * nconv);
     s->nconv = nconv;
     const int fc = v.fft_cap;
-    const int nfftw = std::max(std::max(fft_groups(s->nfft, 0, fc), s->nfft[2]),
-                               std::max(fft_groups(s->nfft, 1, fc), fft_groups(s->nfft, 2, fc)));
+    const int nfftw = fft_work_count(s->nfft, fc);
     std::vector<Work> fw(nfftw);
     for (int b = 0; b < nfftw; ++b) fw[b] = {0, b};
     ALLOC(s->d_fftwork, nfftw);

@@ -4,11 +4,13 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <algorithm>
 #include <array>
 #include <map>
 #include <set>
 #include <string>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "../../include/thp.h"
@@ -175,6 +177,14 @@ __host__ __device__ inline int fft_groups(const int* nfft, int axis, int cap = 3
   const int outer = axis == 2 ? nfft[1] : nfft[2];
   return outer * ((inner + fl - 1) / fl);
 }
+// work entries one system takes in the FFT launches: the line groups of
+// the three axes, one entry per z plane, and for 64^3 grids the register
+// z pass's waves (runs of 16 x per grid row)
+inline int fft_work_count(const int* nfft, int cap) {
+  int ng = std::max(std::max(fft_groups(nfft, 0, cap), nfft[2]), std::max(fft_groups(nfft, 1, cap), fft_groups(nfft, 2, cap)));
+  if (nfft[0] == 64 && nfft[1] == 64 && nfft[2] == 64) ng = std::max(ng, nfft[1] * ((nfft[0] + 15) / 16));
+  return ng;
+}
 constexpr int kTile = 12;                      // spreading tile edge: brick (<= 8) + order - 1
 constexpr int kTile3 = kTile * kTile * kTile;  // points per tile
 constexpr int kDefaultSegs = 4;   // matvec wavefronts per block unless tuned
@@ -205,6 +215,7 @@ struct Batch {
   int real_only = 0;            // short-range solves: no reciprocal terms
   int fft_plane = 0;            // x and y FFT passes fused per z plane (plane_fft_setup)
   int fft_plane_lds = 0;        // its dynamic LDS bytes
+  int fft_r64 = 0;              // every grid 64^3 FP32: the register-FFT kernels
   // the real-space matvec may run on the context's side stream beside the
   // reciprocal chain (one-system launch sequences only, never captured)
   int fork_ok = 0;
@@ -228,6 +239,9 @@ inline void plane_fft_setup(Batch& b, const int* const* nffts, int m, int prec) 
   }
   b.fft_plane = ok ? 1 : 0;
   b.fft_plane_lds = ok ? need : 0;
+  bool r64 = prec != THP_PREC_FP64 && m > 0;
+  for (int q = 0; r64 && q < m; ++q) r64 = nffts[q][0] == 64 && nffts[q][1] == 64 && nffts[q][2] == 64;
+  b.fft_r64 = r64 ? 1 : 0;
 }
 
 }  // namespace thp
@@ -241,7 +255,10 @@ struct thp_ctx {
   int pair_segs = 0;           // 0: default; else forced waves per block (1..16)
   int pair_nb = 0;             // 0: default; else forced matvec entries per batch (8, 16)
   bool plane_fft = false;       // the fused x-y plane FFT kernel may take its LDS
+  bool reg_fft = true;          // 64^3 FP32 grids take the register-FFT kernels (THP_FFT_REG=0: not)
+  int fft_pp = 1, fft_zp = 2;   // lanes per 64-point line in its plane and z passes
   hipStream_t side = nullptr;   // the real-space matvec beside the reciprocal chain
+  bool fork_batch = false;      // eager batches of several systems fork too (THP_FORK_BATCH=1)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // ordering of the stream-ordered entry points against the caller's stream
   hipEvent_t ev_in = nullptr, ev_out = nullptr;

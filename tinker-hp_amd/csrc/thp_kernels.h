@@ -1345,6 +1345,18 @@ __global__ void __launch_bounds__(256) k_pme_binsort(const SysView* views, const
   for (int q = threadIdx.x; q < cnt; q += blockDim.x) sv.bin_list[s0 + q] = key[q];
 }
 
+// an int carried in a slot of the grid's real type (bit copy)
+template <typename real>
+__device__ __forceinline__ real int_as_real(int v) {
+  if constexpr (sizeof(real) == 4) return __int_as_float(v);
+  else return __longlong_as_double((long long)v);
+}
+template <typename real>
+__device__ __forceinline__ int real_as_int(real v) {
+  if constexpr (sizeof(real) == 4) return __float_as_int(v);
+  else return (int)__double_as_longlong(v);
+}
+
 // ---------------------------------------------------------------------
 // Tile spreading.  One workgroup per brick takes the atoms whose first
 // spline node lies in the brick (its "home" bin, sorted by atom index); the
@@ -1367,12 +1379,17 @@ __global__ void __launch_bounds__(64 * W) k_spread_tile(const SysView* views, co
                                                         int gate) {
   constexpr int NL = KIND == 0 ? 2 : 3;   // spline levels used
   constexpr int NF = KIND == 0 ? 6 : 10;  // moments per atom
-  constexpr int NT = 3 * NL * kOrder;
+  // staged per atom: the spline values of (axis, node) with their levels
+  // side by side (TW wide), and the moments with the tile offset after
+  // them (FW wide), so that a lane reads its weights in a few vector reads
+  constexpr int TW = KIND == 0 ? 2 : 4;
+  constexpr int FW = KIND == 0 ? 8 : 12;
+  typedef real TV __attribute__((ext_vector_type(TW)));
+  typedef real R4 __attribute__((ext_vector_type(4)));
   // a wavefront's tile, plus a slot the idle lanes of the second pass write
   __shared__ __attribute__((aligned(16))) real s_tile[W][kTile3 * 2 + 4];
-  __shared__ real s_th[64][NT];
-  __shared__ real s_f[64][NF];
-  __shared__ int s_off[64];
+  __shared__ __attribute__((aligned(32))) real s_th[64][3][kOrder][TW];
+  __shared__ __attribute__((aligned(32))) real s_f[64][FW];
   const Work w = bwork[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
@@ -1411,14 +1428,14 @@ __global__ void __launch_bounds__(64 * W) k_spread_tile(const SysView* views, co
     for (int q = threadIdx.x; q < cnt; q += 64 * W) {
       const int i = BLIST[c0 + q];
       const int lx = G0[4 * i] - sx, ly = G0[4 * i + 1] - sy, lz = G0[4 * i + 2] - sz;
-      s_off[q] = (lz * kTile + ly) * kTile + lx;
+      s_f[q][NF] = int_as_real<real>((lz * kTile + ly) * kTile + lx);
       const GLOBAL real* th = THETA + (size_t)i * kTheta;
 #pragma unroll
       for (int d = 0; d < 3; ++d)
 #pragma unroll
         for (int l = 0; l < NL; ++l)
 #pragma unroll
-          for (int m = 0; m < kOrder; ++m) s_th[q][(d * NL + l) * kOrder + m] = th[th_idx(d, l, m)];
+          for (int m = 0; m < kOrder; ++m) s_th[q][d][m][l] = th[th_idx(d, l, m)];
       if (KIND == 0) {
         s_f[q][0] = real(af0 * IN[0 * np + i]);
         s_f[q][1] = real(af1 * IN[1 * np + i]);
@@ -1463,25 +1480,32 @@ __global__ void __launch_bounds__(64 * W) k_spread_tile(const SysView* views, co
     // each other in program order.
     const bool live1 = lane < kOrder * kOrder * kOrder - 64;
     auto weights = [&](int q, real (&g)[2][2], int& t0, int& t1) {
-      const real* th = s_th[q];
-      const real* f = s_f[q];
-      const int base = s_off[q];
+      const R4* fv = reinterpret_cast<const R4*>(s_f[q]);
+      real f[FW];
+#pragma unroll
+      for (int c = 0; c < FW / 4; ++c) {
+        const R4 t = fv[c];
+        f[4 * c] = t.x;
+        f[4 * c + 1] = t.y;
+        f[4 * c + 2] = t.z;
+        f[4 * c + 3] = t.w;
+      }
+      const int base = real_as_int<real>(f[NF]);
 #pragma unroll
       for (int ps = 0; ps < 2; ++ps) {
-        const int mx = pmx[ps], my_ = pmy[ps], mz = pmz[ps];
-        if (KIND == 0) {
-          const real ta = th[(0 * NL + 0) * kOrder + mx], tb1 = th[(0 * NL + 1) * kOrder + mx];
-          const real u0 = th[(1 * NL + 0) * kOrder + my_], u1 = th[(1 * NL + 1) * kOrder + my_];
-          const real v0 = th[(2 * NL + 0) * kOrder + mz], v1 = th[(2 * NL + 1) * kOrder + mz];
+        const TV tx = *reinterpret_cast<const TV*>(s_th[q][0][pmx[ps]]);
+        const TV ty = *reinterpret_cast<const TV*>(s_th[q][1][pmy[ps]]);
+        const TV tz = *reinterpret_cast<const TV*>(s_th[q][2][pmz[ps]]);
+        if constexpr (KIND == 0) {
+          const real ta = tx.x, tb1 = tx.y;
+          const real u0 = ty.x, u1 = ty.y;
+          const real v0 = tz.x, v1 = tz.y;
           g[ps][0] = ta * (f[1] * u1 * v0 + f[2] * u0 * v1) + tb1 * (f[0] * u0 * v0);
           g[ps][1] = ta * (f[4] * u1 * v0 + f[5] * u0 * v1) + tb1 * (f[3] * u0 * v0);
         } else {
-          const real t0v = th[(0 * NL + 0) * kOrder + mx], t1v = th[(0 * NL + 1) * kOrder + mx],
-                     t2v = th[(0 * NL + 2) * kOrder + mx];
-          const real u0 = th[(1 * NL + 0) * kOrder + my_], u1 = th[(1 * NL + 1) * kOrder + my_],
-                     u2 = th[(1 * NL + 2) * kOrder + my_];
-          const real v0 = th[(2 * NL + 0) * kOrder + mz], v1 = th[(2 * NL + 1) * kOrder + mz],
-                     v2 = th[(2 * NL + 2) * kOrder + mz];
+          const real t0v = tx.x, t1v = tx.y, t2v = tx.z;
+          const real u0 = ty.x, u1 = ty.y, u2 = ty.z;
+          const real v0 = tz.x, v1 = tz.y, v2 = tz.z;
           const real term0 =
               f[0] * u0 * v0 + f[2] * u1 * v0 + f[3] * u0 * v1 + f[5] * u2 * v0 + f[6] * u0 * v2 + f[9] * u1 * v1;
           const real term1 = f[1] * u0 * v0 + f[7] * u1 * v0 + f[8] * u0 * v1;
@@ -1907,6 +1931,238 @@ __global__ void __launch_bounds__(NT) k_fft_plane(const SysView* views, const Wo
     const int y = f / K0;
     G[f] = res[y * kp + (f - y * K0)];
   }
+}
+
+// ----------------------------------------------------- register FFT, 64 points
+// For grids of 64 x 64 x 64 points (FP32) the transforms run in registers.
+// P lanes (1, 2 or 4, neighbours in the wave) hold one 64-point line, lane L
+// the points P n + L (n = 0 .. 64/P - 1).  With n = P n1 + L and k = k1 +
+// (64/P) k2 a line's DFT is a (64/P)-point DFT in each lane (a four-step
+// R1 x R2 split with compile-time indices and twiddles), the twiddles
+// exp(s 2 pi i L k1 / 64), and a P-point DFT across the lanes (radix-2
+// stages over DPP quad permutes; for P = 4 lane 3 is turned by s i between
+// the two); lane L then holds the frequencies k1 + (64/P) rev(L), rev
+// swapping the two bits of L when P = 4.  The same steps in reverse order
+// take the lanes from that arrangement back to the points P n + L.  A wave
+// holds 64/P lines that lie side by side in memory (consecutive x), so its
+// loads and stores are runs of 512/P bytes.
+
+// calls f(std::integral_constant<int, i>) for i = 0 .. N-1
+template <typename F, int... I>
+__device__ __forceinline__ void sfor_seq(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void sfor(F&& f) {
+  sfor_seq(f, std::make_integer_sequence<int, N>{});
+}
+
+// cos or sin of a in [0, 2 pi) at compile time (Taylor series after
+// reduction to [-pi, pi], to double precision)
+constexpr double ct_trig(double a, bool sine) {
+  constexpr double pi = 3.14159265358979323846;
+  if (a > pi) a -= 2.0 * pi;
+  double term = sine ? a : 1.0, sum = term;
+  for (int n = 1; n < 30; ++n) {
+    const double d = sine ? (2.0 * n) * (2.0 * n + 1.0) : (2.0 * n - 1.0) * (2.0 * n);
+    term *= -a * a / d;
+    sum += term;
+  }
+  return sum;
+}
+// exp(S 2 pi i M / K)
+template <int K, int M, int S>
+__device__ __forceinline__ Coef2<float> ct_twiddle() {
+  constexpr double a = 2.0 * 3.14159265358979323846 * double(M % K) / double(K);
+  constexpr float c = float(ct_trig(a, false)), sn = float(S) * float(ct_trig(a, true));
+  return coef2<float>(c, sn);
+}
+
+template <int R, int S>
+__device__ __forceinline__ void dft_reg(Coef2<float>* a) {
+  if constexpr (R == 8) dft8<float>(a, float(S));
+  else dft_small<float, R>(a, float(S));
+}
+
+// in-place DFT of x[R1 R2] with exp(S 2 pi i n k / (R1 R2)), natural order:
+// n = R2 n1 + n2, k = k1 + R1 k2
+template <int R1, int R2, int S>
+__device__ __forceinline__ void fft_reg(Coef2<float>* x) {
+  constexpr int N = R1 * R2;
+  Coef2<float> t[N];
+  sfor<R2>([&](auto n2) {
+    Coef2<float> a[R1];
+    sfor<R1>([&](auto n1) { a[n1] = x[R2 * n1 + n2]; });
+    dft_reg<R1, S>(a);
+    sfor<R1>([&](auto k1) {
+      constexpr int m = decltype(n2)::value * decltype(k1)::value;
+      if constexpr (m == 0) t[R1 * n2 + k1] = a[k1];
+      else t[R1 * n2 + k1] = cmul<float>(a[k1], ct_twiddle<N, m, S>());
+    });
+  });
+  sfor<R1>([&](auto k1) {
+    Coef2<float> a[R2];
+    sfor<R2>([&](auto n2) { a[n2] = t[R1 * n2 + k1]; });
+    dft_reg<R2, S>(a);
+    sfor<R2>([&](auto k2) { x[k1 + R1 * k2] = a[k2]; });
+  });
+}
+
+// value of lane L ^ 1 (kQuadXor1) or L ^ 2 (kQuadXor2) of each quad
+constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E;  // DPP quad_perm [1,0,3,2], [2,3,0,1]
+template <int CTRL>
+__device__ __forceinline__ float quad_perm(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+// radix-2 across lanes L and L ^ X: the lane with (L & X) == 0 keeps own +
+// partner, the other partner - own
+template <int M, int CTRL, int X>
+__device__ __forceinline__ void lane_stage(Coef2<float>* v, int L) {
+  const float sg = (L & X) ? -1.0f : 1.0f;
+  sfor<M>([&](auto k) {
+    const float px = quad_perm<CTRL>(v[k].x), py = quad_perm<CTRL>(v[k].y);
+    v[k].x = fmaf(sg, v[k].x, px);
+    v[k].y = fmaf(sg, v[k].y, py);
+  });
+}
+// lane 3 of the quad times S i
+template <int M, int S>
+__device__ __forceinline__ void lane_rot3(Coef2<float>* v, int L) {
+  const bool r = L == 3;
+  sfor<M>([&](auto k) {
+    const float nx = -float(S) * v[k].y, ny = float(S) * v[k].x;
+    v[k].x = r ? nx : v[k].x;
+    v[k].y = r ? ny : v[k].y;
+  });
+}
+// v[k] *= exp(S 2 pi i L k / 64), L < P; the lane's twiddles are picked by
+// its index bits (selects, no branches)
+template <int P, int S>
+__device__ __forceinline__ void lane_twiddle(Coef2<float>* v, int L) {
+  constexpr int M = 64 / P;
+  const bool b0 = (L & 1) != 0, b1 = (L & 2) != 0;
+  sfor<M>([&](auto k) {
+    constexpr int kk = decltype(k)::value;
+    if constexpr (kk != 0) {
+      const Coef2<float> w1 = ct_twiddle<64, kk, S>(), w2 = ct_twiddle<64, 2 * kk, S>(), w3 = ct_twiddle<64, 3 * kk, S>();
+      float c, sn;
+      if constexpr (P == 2) {
+        c = b0 ? w1.x : 1.0f;
+        sn = b0 ? w1.y : 0.0f;
+      } else {
+        c = b1 ? (b0 ? w3.x : w2.x) : (b0 ? w1.x : 1.0f);
+        sn = b1 ? (b0 ? w3.y : w2.y) : (b0 ? w1.y : 0.0f);
+      }
+      v[k] = cmul<float>(v[k], coef2<float>(c, sn));
+    }
+  });
+}
+template <int P>
+__device__ __forceinline__ int lane_rev(int L) {
+  return P == 4 ? 2 * (L & 1) + (L >> 1) : L;
+}
+// the in-lane split of a 64/P-point DFT
+template <int P, int S>
+__device__ __forceinline__ void lane_dft(Coef2<float>* v) {
+  if constexpr (P == 1) fft_reg<8, 8, S>(v);
+  else if constexpr (P == 2) fft_reg<8, 4, S>(v);
+  else fft_reg<4, 4, S>(v);
+}
+// DFT (sign S) of a line held by P lanes: points P n + L in, frequencies
+// k + (64/P) rev(L) out
+template <int P, int S>
+__device__ __forceinline__ void line_fft64(Coef2<float>* v, int L) {
+  constexpr int M = 64 / P;
+  lane_dft<P, S>(v);
+  if constexpr (P > 1) lane_twiddle<P, S>(v, L);
+  if constexpr (P == 2) lane_stage<M, kQuadXor1, 1>(v, L);
+  if constexpr (P == 4) {
+    lane_stage<M, kQuadXor2, 2>(v, L);
+    lane_rot3<M, S>(v, L);
+    lane_stage<M, kQuadXor1, 1>(v, L);
+  }
+}
+// the same DFT in the other arrangement: values at k + (64/P) rev(L) in,
+// points P n + L out
+template <int P, int S>
+__device__ __forceinline__ void line_fft64_rev(Coef2<float>* v, int L) {
+  constexpr int M = 64 / P;
+  if constexpr (P == 2) lane_stage<M, kQuadXor1, 1>(v, L);
+  if constexpr (P == 4) {
+    lane_stage<M, kQuadXor1, 1>(v, L);
+    lane_rot3<M, S>(v, L);
+    lane_stage<M, kQuadXor2, 2>(v, L);
+  }
+  if constexpr (P > 1) lane_twiddle<P, S>(v, L);
+  lane_dft<P, S>(v);
+}
+
+// The z pass of the convolution for 64^3 FP32 grids: a wave holds the z
+// lines of 64/P consecutive x of one y row; forward transform, (MODE 3:
+// the spectrum kept in `spec`), times qfac, backward transform into the
+// points the lines were loaded from.
+template <int P, int MODE>
+__global__ void __launch_bounds__(64) k_fft_z64(const SysView* views, const Work* work, int gate, void* spec) {
+  constexpr int M = 64 / P;
+  const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
+  const SysView& sv = views[w.sys];
+  if (gated_out(sv.st, gate)) return;
+  const int K0 = sv.nfft[0], K1 = sv.nfft[1];
+  const int nch = (K0 + M - 1) / M;
+  if (w.blk >= K1 * nch) return;
+  const int y = w.blk / nch;
+  const int L = threadIdx.x & (P - 1);
+  const int x = (w.blk - y * nch) * M + threadIdx.x / P;
+  const bool ok = x < K0;  // the P lanes of a line share x
+  const size_t plane = (size_t)K0 * K1;
+  const size_t base = (size_t)y * K0 + (ok ? x : 0);
+  GLOBAL Coef2<float>* G = gw(static_cast<Coef2<float>*>(sv.grid)) + base;
+  const GLOBAL float* Q = gp(static_cast<const float*>(sv.qfac)) + base;
+  Coef2<float> v[M];
+  sfor<M>([&](auto n) { v[n] = G[(P * n + L) * plane]; });
+  line_fft64<P, -1>(v, L);
+  const int kq = M * lane_rev<P>(L);
+  if constexpr (MODE == 3) {
+    GLOBAL Coef2<float>* SP = gw(static_cast<Coef2<float>*>(spec)) + base;
+    if (ok) sfor<M>([&](auto k) { SP[(k + kq) * plane] = v[k]; });
+  }
+  sfor<M>([&](auto k) { v[k] *= Q[(k + kq) * plane]; });
+  line_fft64_rev<P, 1>(v, L);
+  if (ok) sfor<M>([&](auto n) { G[(P * n + L) * plane] = v[n]; });
+}
+
+// The x and y passes of one z plane of a 64^3 FP32 grid in a workgroup of
+// P waves: line c (P lanes) loads the points P n + L of column x = c (the y
+// line) and transforms it; the plane goes through LDS, where line c takes
+// the points of row y = c (the x line) and transforms that; back through
+// LDS for the store of the columns.  MODE 0 forward, 1 backward (the sign
+// of both transforms).
+template <int P, int MODE>
+__global__ void __launch_bounds__(64 * P) k_fft_plane64(const SysView* views, const Work* work, int gate) {
+  constexpr int M = 64 / P;
+  __shared__ Coef2<float> s_t[64 * 65];
+  const Work w = work[blockIdx.x];
+  if (w.sys < 0) return;
+  const SysView& sv = views[w.sys];
+  if (gated_out(sv.st, gate)) return;
+  if (w.blk >= sv.nfft[2]) return;
+  constexpr int S = MODE == 0 ? -1 : 1;
+  const int L = threadIdx.x & (P - 1), c = threadIdx.x / P;
+  const int kq = M * lane_rev<P>(L);
+  GLOBAL Coef2<float>* G = gw(static_cast<Coef2<float>*>(sv.grid)) + (size_t)w.blk * 4096 + c;
+  Coef2<float> v[M];
+  sfor<M>([&](auto n) { v[n] = G[64 * (P * n + L)]; });
+  line_fft64<P, S>(v, L);
+  sfor<M>([&](auto k) { s_t[65 * (k + kq) + c] = v[k]; });
+  __syncthreads();
+  sfor<M>([&](auto n) { v[n] = s_t[65 * c + P * n + L]; });
+  line_fft64<P, S>(v, L);
+  __syncthreads();
+  sfor<M>([&](auto k) { s_t[65 * c + k + kq] = v[k]; });
+  __syncthreads();
+  sfor<M>([&](auto n) { v[n] = s_t[65 * (P * n + L) + c]; });
+  sfor<M>([&](auto n) { G[64 * (P * n + L)] = v[n]; });
 }
 
 // Tile gather: fphi_uind_site first derivatives (pmestuff.f:746-1037,
