@@ -1111,33 +1111,39 @@ thp_status b_pair_tmatxb(thp_ctx* ctx, const Batch& b, int vin, int vout, int do
 // The real-space matvec of a vector runs on the side stream while the main
 // stream spreads and transforms the same vector (the two read it and write
 // different arrays); join_matvec makes the main stream wait for it before
-// the gather that adds the two parts.  Only one-system launches fork: a
-// batch of systems fills the device with either branch, and overlapping
-// them gains nothing.  Only a system's own launch sequence (its batch of
-// one, launched eagerly) forks: the serving loop's batches, captured as
-// graphs, never record the fork and join events, so no event is recorded
-// both inside a capture and in eager launches.  Otherwise the matvec runs
-// in line.
+// the gather that adds the two parts.  A system's own launch sequence (its
+// batch of one, always eager) and the serving loop's batches fork.  Eager
+// launches use the context's fork and join events; a batch being captured
+// as a graph uses the events its capture was given (Batch::cap_fork,
+// cap_join), which serve that capture only and are destroyed with its
+// graphs, so no event is recorded both inside a capture and in eager
+// launches.  Without events for a capture the matvec runs in line.
+inline bool capturing(const thp_ctx* ctx) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(ctx->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone;
+}
 inline bool forks(const thp_ctx* ctx, const Batch& b) {
   if (!ctx->side || b.real_only) return false;
   if (b.fork_ok && b.nsys == 1) return true;
   if (!ctx->fork_batch) return false;
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  return hipStreamIsCapturing(ctx->stream, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone;
+  return capturing(ctx) ? b.cap_fork != nullptr : true;
 }
 
 thp_status fork_matvec(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int gate) {
   if (!forks(ctx, b)) return b_pair_tmatxb(ctx, b, vin, vout, dodiag, gate);
-  HIPCHK(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
-  HIPCHK(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+  const bool cap = b.cap_fork && capturing(ctx);
+  hipEvent_t ef = cap ? b.cap_fork : ctx->ev_fork, ej = cap ? b.cap_join : ctx->ev_join;
+  HIPCHK(ctx, hipEventRecord(ef, ctx->stream));
+  HIPCHK(ctx, hipStreamWaitEvent(ctx->side, ef, 0));
   RET(b_pair_tmatxb_on(ctx, b, vin, vout, dodiag, gate, ctx->side));
-  HIPCHK(ctx, hipEventRecord(ctx->ev_join, ctx->side));
+  HIPCHK(ctx, hipEventRecord(ej, ctx->side));
   return THP_OK;
 }
 
 thp_status join_matvec(thp_ctx* ctx, const Batch& b) {
   if (!forks(ctx, b)) return THP_OK;
-  HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+  const bool cap = b.cap_fork && capturing(ctx);
+  HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, cap ? b.cap_join : ctx->ev_join, 0));
   return THP_OK;
 }
 

@@ -219,6 +219,9 @@ struct Batch {
   // the real-space matvec may run on the context's side stream beside the
   // reciprocal chain (one-system launch sequences only, never captured)
   int fork_ok = 0;
+  // fork and join events of the graph being captured for this batch (owned
+  // by the serving loop's batch buffers, used in that capture only)
+  hipEvent_t cap_fork = nullptr, cap_join = nullptr;
 };
 
 // Whether a batch of these grids takes the fused x-y plane FFT kernel: FP32
@@ -258,7 +261,7 @@ struct thp_ctx {
   bool reg_fft = true;          // 64^3 FP32 grids take the register-FFT kernels (THP_FFT_REG=0: not)
   int fft_pp = 1, fft_zp = 2;   // lanes per 64-point line in its plane and z passes
   hipStream_t side = nullptr;   // the real-space matvec beside the reciprocal chain
-  bool fork_batch = false;      // eager batches of several systems fork too (THP_FORK_BATCH=1)
+  bool fork_batch = true;       // batches of several systems fork too (THP_FORK_BATCH=0: not)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // ordering of the stream-ordered entry points against the caller's stream
   hipEvent_t ev_in = nullptr, ev_out = nullptr;

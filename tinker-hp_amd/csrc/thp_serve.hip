@@ -118,6 +118,9 @@ struct BatchBuf {
   // each as a run of segment graphs launched back to back
   std::vector<hipGraphExec_t> exec[2];
   std::vector<hipGraph_t> graph[2];
+  // fork and join events of each captured segment (the matvec on the side
+  // stream): used only inside that capture, destroyed after its graphs
+  std::vector<hipEvent_t> events;
   int graph_iters = 0;
 };
 
@@ -249,6 +252,8 @@ void drop_graph(BatchBuf& bb) {
     bb.exec[v].clear();
     bb.graph[v].clear();
   }
+  for (hipEvent_t e : bb.events) (void)hipEventDestroy(e);
+  bb.events.clear();
   bb.graph_iters = 0;
 }
 
@@ -556,10 +561,21 @@ thp_status run_pass(Serve* sv, int variant, int iters) {
     if (bb.exec[variant].empty()) {
       bool ok = true;
       for (int s = 0; ok && s < pass_segments(variant, iters); ++s) {
+        // the segment's own fork and join events when batches fork
+        if (ctx->fork_batch && ctx->side) {
+          hipEvent_t ev[2] = {nullptr, nullptr};
+          for (hipEvent_t& x : ev)
+            if (hipEventCreateWithFlags(&x, hipEventDisableTiming) == hipSuccess) bb.events.push_back(x);
+          if (ev[0] && ev[1]) {
+            bb.b.cap_fork = ev[0];
+            bb.b.cap_join = ev[1];
+          }
+        }
         hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
         thp_status r = e == hipSuccess ? enqueue_segment(sv, variant, iters, s) : THP_ERR_HIP;
         hipGraph_t g = nullptr;
         if (e == hipSuccess && hipStreamEndCapture(ctx->stream, &g) != hipSuccess) r = THP_ERR_HIP;
+        bb.b.cap_fork = bb.b.cap_join = nullptr;
         hipGraphExec_t x = nullptr;
         if (r == THP_OK && g && hipGraphInstantiate(&x, g, nullptr, nullptr, 0) == hipSuccess) {
           bb.graph[variant].push_back(g);
