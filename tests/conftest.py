@@ -20,3 +20,21 @@ def ctx():
     c = thp.Context(0)
     yield c
     c.close()
+
+
+@pytest.fixture(scope="session")
+def ctx_stored():
+    """A context whose FP32 systems store their matvec coefficients per solve
+    (THP_MATVEC_RC=0 at thp_init) instead of recomputing them per entry."""
+    import thp
+    old = os.environ.get("THP_MATVEC_RC")
+    os.environ["THP_MATVEC_RC"] = "0"
+    try:
+        c = thp.Context(0)
+    finally:
+        if old is None:
+            del os.environ["THP_MATVEC_RC"]
+        else:
+            os.environ["THP_MATVEC_RC"] = old
+    yield c
+    c.close()

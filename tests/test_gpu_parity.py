@@ -6,6 +6,7 @@ relative of the oracle (only summation order differs); mixed path within
 per-kernel fields within 2e-5 relative.
 """
 import ctypes as C
+import dataclasses
 import os
 import re
 import subprocess
@@ -140,6 +141,33 @@ def test_field_and_matvec_water1944(ctx, prec, tol):
     # loses more at the overlapping contacts, so its permanent field is
     # checked at 2e-5 only in FP64
     assert rel(solver.efld0_direct(), orc.efld0_direct()) < (tol if prec == FP64 else 2e-5 * 10)
+
+
+# The mixed path's real-space matvec either recomputes the pair coefficients
+# per entry (FP32 systems with one Thole value, the default) or stores them
+# per solve (THP_MATVEC_RC=0, or systems whose Thole values differ): both
+# kinds against the oracle, on water and on the uneven solute box.
+def _thole_varied(s):
+    th = s.thole.copy()
+    th[::7] = 0.41
+    return dataclasses.replace(s, thole=th, name=s.name + "-thole")
+
+
+@pytest.mark.parametrize("case", ["water1944", "solute", "thole"])
+@pytest.mark.parametrize("kind", ["recomputed", "stored"])
+def test_matvec_coefficient_kinds_match_oracle(ctx, ctx_stored, case, kind):
+    s = {"water1944": thp.water_1944,
+         "solute": lambda: thp.make_system(3000, 34.0, n_solute=1200, blob_radius=11.0, seed=9, nfft=(36,) * 3),
+         "thole": lambda: _thole_varied(thp.water_1944())}[case]()
+    orc = O.Oracle(s)
+    solver = thp.Solver(ctx if kind == "recomputed" else ctx_stored, s, MIXED)
+    mu = np.random.default_rng(4).normal(size=(s.n, 2, 3)) * 0.1
+    for dodiag in (True, False):
+        e = rel(solver.tmatxb_pme(mu, dodiag=dodiag), orc.tmatxb_pme(mu, dodiag=dodiag))
+        print(f"{case} {kind} dodiag={dodiag} rel={e:.2e}", flush=True)
+        assert e < 2e-5
+    solver.close()
+    check_solve(ctx if kind == "recomputed" else ctx_stored, s, MIXED, orc.induce(poleps=1e-8))
 
 
 def numpy_grid_uind(s, fuind, fuinp):

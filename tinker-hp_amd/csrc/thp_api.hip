@@ -252,6 +252,36 @@ std::vector<int> spatial_order(int n, const double* x, const double* y, const do
   return perm;
 }
 
+// Cubic Hermite table of the Ewald functions of the recomputed matvec
+// coefficients, in x = aewald r (dampewald, damping.f:17-56, at aewald = 1):
+//   G1(x) = r^3 bn1 = erfc(x) + (2/sqrt(pi)) x exp(-x^2)
+//   G2(x) = r^5 bn2 = 3 G1(x) + (4/sqrt(pi)) x^3 exp(-x^2)
+// with G1' = -(4/sqrt(pi)) x^2 exp(-x^2) and G2' = -(8/sqrt(pi)) x^4 exp(-x^2).
+// Interval j (x in [j h, (j+1) h), h = 1/kGtabScale) holds the coefficients
+// of p(f) = c0 + c1 f + c2 f^2 + c3 f^3, f = x/h - j, for G1 and then G2,
+// computed in FP64 and stored in FP32 (largest error 1.5e-7 on G2).
+std::vector<float> ewald_gtab() {
+  const double h = 1.0 / kGtabScale, s = 2.0 / std::sqrt(M_PI);
+  auto g1 = [&](double x) { return std::erfc(x) + s * x * std::exp(-x * x); };
+  auto g2 = [&](double x) { return 3.0 * g1(x) + 2.0 * s * x * x * x * std::exp(-x * x); };
+  auto d1 = [&](double x) { return -2.0 * s * x * x * std::exp(-x * x); };
+  auto d2 = [&](double x) { return -4.0 * s * x * x * x * x * std::exp(-x * x); };
+  std::vector<float> t((size_t)kGtabN * 8);
+  for (int j = 0; j < kGtabN; ++j) {
+    const double x0 = j * h, x1 = x0 + h;
+    for (int f = 0; f < 2; ++f) {
+      const double y0 = f ? g2(x0) : g1(x0), y1 = f ? g2(x1) : g1(x1);
+      const double e0 = (f ? d2(x0) : d1(x0)) * h, e1 = (f ? d2(x1) : d1(x1)) * h;
+      float* c = &t[(size_t)j * 8 + 4 * f];
+      c[0] = float(y0);
+      c[1] = float(e0);
+      c[2] = float(3.0 * (y1 - y0) - 2.0 * e0 - e1);
+      c[3] = float(2.0 * (y0 - y1) + e0 + e1);
+    }
+  }
+  return t;
+}
+
 // spreading brick edge: the largest B in bmax..4 (bmax <= 8) with K % B == 0
 // or >= 4, so every brick (the last one included) is at least 4 points
 // wide; when no B up to bmax qualifies, the largest one up to 8
@@ -512,7 +542,9 @@ thp_status build_union(thp_sys* s) {
         s->view.tlist = nullptr;
         RET(dalloc(s, &s->view.tlist, (size_t)cmax * s->npad));
         const size_t words = (size_t)cmax * s->npad * 2;
-        if (s->prec == THP_PREC_FP64) {
+        if (s->view.rc_sqa > 0.0) {
+          // indices only: the matvec recomputes the coefficients
+        } else if (s->prec == THP_PREC_FP64) {
           double* t = nullptr;
           RET(dalloc(s, &t, words));
           s->view.tcoef = t;
@@ -1078,29 +1110,35 @@ thp_status b_init_state(thp_ctx* ctx, const Batch& b, double poleps, int politer
   return launch_check(ctx, "k_init_state");
 }
 
-template <typename real, int NB>
+template <typename real, int NB, bool RC>
 void launch_tmatxb_nb(const Batch& b, int vin, int vout, int dodiag, int gate, hipStream_t st) {
   const dim3 g = g1(b.n_blk);
   switch (b.segs) {
-    case 1: hipLaunchKernelGGL((k_tmatxb_cl<real, 1, NB>), g, dim3(64), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, gate); break;
-    case 2: hipLaunchKernelGGL((k_tmatxb_cl<real, 2, NB>), g, dim3(128), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, gate); break;
-    case 8: hipLaunchKernelGGL((k_tmatxb_cl<real, 8, NB>), g, dim3(512), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, gate); break;
-    case 16: hipLaunchKernelGGL((k_tmatxb_cl<real, 16, NB>), g, dim3(1024), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, gate); break;
-    default: hipLaunchKernelGGL((k_tmatxb_cl<real, 4, NB>), g, dim3(256), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, gate); break;
+    case 1: hipLaunchKernelGGL((k_tmatxb_cl<real, 1, NB, RC>), g, dim3(64), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, gate); break;
+    case 2: hipLaunchKernelGGL((k_tmatxb_cl<real, 2, NB, RC>), g, dim3(128), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, gate); break;
+    case 8: hipLaunchKernelGGL((k_tmatxb_cl<real, 8, NB, RC>), g, dim3(512), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, gate); break;
+    case 16: hipLaunchKernelGGL((k_tmatxb_cl<real, 16, NB, RC>), g, dim3(1024), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, gate); break;
+    default: hipLaunchKernelGGL((k_tmatxb_cl<real, 4, NB, RC>), g, dim3(256), 0, st, b.d_views, b.d_blk, vin, vout, dodiag, gate); break;
   }
 }
 
-template <typename real>
+template <typename real, bool RC>
 void launch_tmatxb(const Batch& b, int vin, int vout, int dodiag, int gate, hipStream_t st) {
-  if (b.nb == 16) launch_tmatxb_nb<real, 16>(b, vin, vout, dodiag, gate, st);
-  else launch_tmatxb_nb<real, 8>(b, vin, vout, dodiag, gate, st);
+  if (b.nb == 16) launch_tmatxb_nb<real, 16, RC>(b, vin, vout, dodiag, gate, st);
+  else launch_tmatxb_nb<real, 8, RC>(b, vin, vout, dodiag, gate, st);
 }
 
-// real-space matvec on stream st; needs the pair coefficients of the
-// current coordinates (written by the permanent-field pass)
+// real-space matvec on stream st; needs the compacted lists of the current
+// coordinates (written by the permanent-field pass).  Systems that
+// recompute their coefficients and systems that store them take separate
+// kernels, each skipping the other kind; a batch launches those it holds.
 thp_status b_pair_tmatxb_on(thp_ctx* ctx, const Batch& b, int vin, int vout, int dodiag, int gate, hipStream_t st) {
-  if (b.prec == THP_PREC_FP64) launch_tmatxb<double>(b, vin, vout, dodiag, gate, st);
-  else launch_tmatxb<float>(b, vin, vout, dodiag, gate, st);
+  if (b.prec == THP_PREC_FP64) {
+    launch_tmatxb<double, false>(b, vin, vout, dodiag, gate, st);
+  } else {
+    if (b.rc & 1) launch_tmatxb<float, true>(b, vin, vout, dodiag, gate, st);
+    if ((b.rc & 2) || !b.rc) launch_tmatxb<float, false>(b, vin, vout, dodiag, gate, st);
+  }
   return launch_check(ctx, "k_tmatxb_cl");
 }
 
@@ -1353,6 +1391,9 @@ thp_status thp_init(thp_ctx** out, int device) {
     *out = ctx;
     return fail(ctx, THP_ERR_HIP, "cannot create the ordering events of device %d", device);
   }
+  // THP_MATVEC_RC=0: every system stores its matvec coefficients per solve
+  if (const char* e = getenv("THP_MATVEC_RC"))
+    if (atoi(e) == 0) ctx->rc_matvec = false;
   // THP_SEGS forces the pair kernels' wavefronts per 64-atom block
   if (const char* e = getenv("THP_SEGS")) {
     const int v = atoi(e);
@@ -1573,6 +1614,23 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
     H2D(dec, ec.data(), sizeof(int) * ec.size());
   }
   v.x = dx; v.y = dy; v.z = dz; v.rpole = drp; v.pol = dpol; v.pdamp = dpd; v.thole = dth;
+  {
+    // matvec coefficients recomputed per entry instead of stored per solve:
+    // FP32 systems with one Thole value a > 0 (AMOEBA uses 0.39 throughout)
+    // (and aewald * cut inside the table of the Ewald functions)
+    bool one_a = s->prec != THP_PREC_FP64 && ctx->rc_matvec && n > 0 && d->thole[0] > 0.0 &&
+                 d->aewald * d->cut < 0.98 * kGtabN / kGtabScale;
+    for (int q = 1; one_a && q < n; ++q) one_a = d->thole[q] == d->thole[0];
+    if (one_a) {
+      const std::vector<float> gt = ewald_gtab();
+      float* dg;
+      ALLOC(dg, gt.size());
+      H2D(dg, gt.data(), sizeof(float) * gt.size());
+      if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(ctx, THP_ERR_HIP, "sync"));
+      v.gtab = dg;
+      v.rc_sqa = std::sqrt(d->thole[0] * M_LOG2E);
+    }
+  }
   v.scale_tab = dtab; v.excl_off = deoff; v.excl_k = dek; v.excl_code = dec;
   {
     double* sco;
@@ -1758,6 +1816,7 @@ thp_status thp::create_system(thp_ctx* ctx, const thp_system_desc* d, thp_sys** 
     Batch& b = s->one;
     b.prec = s->prec;
     b.nsys = 1;
+    b.rc = v.rc_sqa > 0.0 ? 1 : 2;
     b.segs = ctx->pair_segs > 0 ? ctx->pair_segs : kDefaultSegs;
     b.nb = ctx->pair_nb > 0 ? ctx->pair_nb : 8;
     b.fork_ok = 1;
@@ -2734,6 +2793,8 @@ thp_status thp_time_kernel_batch(thp_sys* const* systems, int nsys, int kernel, 
       tab.insert(tab.end(), v.begin(), v.end());
     };
     b.nsys = nsys;
+    b.rc = 0;
+    for (int q = 0; q < nsys; ++q) b.rc |= systems[q]->view.rc_sqa > 0.0 ? 1 : 2;
     put(blk, &b.d_blk, &b.n_blk);
     put(red, &b.d_red, &b.n_red);
     put(gat, &b.d_gat, &b.n_gat);

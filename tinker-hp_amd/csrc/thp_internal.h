@@ -23,6 +23,11 @@ constexpr int kPredPairs = kMaxUalt * (kMaxUalt + 1) / 2;
 constexpr int kDiisMax = 25;              // DIIS subspace (ndismx, newinduce_pme2.f:664)
 constexpr int kDiisParts = 2 + kDiisMax;  // per-block partials of a Jacobi step
 constexpr int kPack = 8;    // compacted list entries per lane pack (16 B of indices)
+// Ewald functions of the recomputed matvec coefficients, tabulated in
+// x = aewald * r over [0, kGtabN / kGtabScale): per interval the cubic
+// coefficients of G1 = r^3 bn1 and then of G2 = r^5 bn2
+constexpr int kGtabN = 256;
+constexpr float kGtabScale = 32.0f;  // intervals per unit of x
 constexpr int kOrder = 5;   // PME B-spline order handled by the kernels
 constexpr double kCoulomb = 332.063714;  // v1.2/source/MOD_units.f:81
 constexpr double kTinyPol = 1.0e-5;      // v1.2/source/MOD_polar.f:36
@@ -99,6 +104,11 @@ struct SysView {
   const double* pol;
   const double* pdamp;
   const double* thole;
+  // matvec coefficients recomputed per entry (FP32 systems with one Thole
+  // value a > 0): sqrt(a log2 e), else 0 (coefficients stored per solve);
+  // gtab is the cubic table of the Ewald functions of the recomputation
+  double rc_sqa;
+  const float* gtab;  // [kGtabN][8]
   const double* scale_tab;  // [ncode][3] d,p,u
   const int* excl_off;      // [n+1]
   const int* excl_k;
@@ -122,8 +132,10 @@ struct SysView {
   // entries: entry e of
   // atom i sits at ((e / kPack) * npad + i) * kPack + e % kPack, and the
   // last pack of an atom is padded with index 0 and zero coefficients
+  // (with recomputed coefficients: no tcoef, padding index = the block's
+  // union count)
   unsigned short* tlist;   // [cmax / kPack][npad][kPack]
-  void* tcoef;             // [cmax / kPack][npad][kPack] coefficient pairs of tlist
+  void* tcoef;             // [cmax / kPack][npad][kPack] coefficient pairs of tlist, or null
   int* tcount;             // [npad]
   // PME
   int brick[3];       // spreading brick edge per axis (4..8 points)
@@ -198,6 +210,7 @@ struct Batch {
   int nsys = 0;
   int segs = 4;                 // wavefronts per 64-atom block in the pair kernels
   int nb = 8;                   // list entries per prefetched batch in the matvec (8 or 16)
+  int rc = 0;                   // matvec kinds held: bit 0 recomputed coefficients, bit 1 stored (0: unknown, both)
   SysView* d_views = nullptr;   // [nsys]
   Work* d_blk = nullptr;        // 64-atom blocks
   int n_blk = 0;
@@ -262,6 +275,7 @@ struct thp_ctx {
   int fft_pp = 1, fft_zp = 2;   // lanes per 64-point line in its plane and z passes
   hipStream_t side = nullptr;   // the real-space matvec beside the reciprocal chain
   bool fork_batch = true;       // batches of several systems fork too (THP_FORK_BATCH=0: not)
+  bool rc_matvec = true;        // FP32 systems with one Thole value recompute matvec coefficients (THP_MATVEC_RC=0: not)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // ordering of the stream-ordered entry points against the caller's stream
   hipEvent_t ev_in = nullptr, ev_out = nullptr;

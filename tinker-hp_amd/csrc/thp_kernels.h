@@ -714,6 +714,11 @@ __global__ void __launch_bounds__(64 * S) k_efld0(const SysView* views, const Wo
   real* s_th = s_pd + C;
   GLOBAL unsigned short* TL = gw(sv.tlist);
   GLOBAL Coef2<real>* TC = gw(static_cast<Coef2<real>*>(sv.tcoef));
+  // systems whose matvec recomputes the coefficients keep indices only,
+  // their last pack padded with the union's count: the matvec stages a
+  // sentinel there (an atom far away with zero dipoles)
+  const bool keep_coef = sv.rc_sqa == 0.0;
+  const unsigned short pad_ix = keep_coef ? 0 : (unsigned short)ucount;
   int s = s_begin;
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (int c0 = 0; c0 < ucount; c0 += C) {
@@ -774,7 +779,7 @@ __global__ void __launch_bounds__(64 * S) k_efld0(const SysView* views, const Wo
         const real dmp7 = bn[2] - (real(1) - l[2]) * rr7;
         const size_t at = ((size_t)(t / kPack) * np + i) * kPack + t % kPack;
         TL[at] = (unsigned short)loc[b];
-        TC[at] = coef2<real>(dmp3, dmp5);
+        if (keep_coef) TC[at] = coef2<real>(dmp3, dmp5);
         ++t;
         const real ai = dmp3 * ck - dmp5 * dkr + dmp7 * qkr;
         const real d5 = real(2) * dmp5;
@@ -789,8 +794,8 @@ __global__ void __launch_bounds__(64 * S) k_efld0(const SysView* views, const Wo
   if (wv == S - 1 && ok) {
     for (; t % kPack; ++t) {
       const size_t at = ((size_t)(t / kPack) * np + i) * kPack + t % kPack;
-      TL[at] = 0;
-      TC[at] = coef2<real>(real(0), real(0));
+      TL[at] = pad_ix;
+      if (keep_coef) TC[at] = coef2<real>(real(0), real(0));
     }
   }
   // unscaled pairs give identical d and p fields
@@ -841,17 +846,39 @@ struct ClStage {
   static constexpr int BYTES = 9 * (int)sizeof(real);
   static constexpr int CHUNK = (sizeof(real) == 4 ? 48 * 1024 : 64 * 1024) / BYTES;  // atoms per LDS chunk
 };
+// Recomputed coefficients: per union atom two quads and (up_z, Thole
+// weight), 40 bytes, then the Ewald table
+struct RcStage {
+  static constexpr int TAB_BYTES = kGtabN * 8 * (int)sizeof(float);
+  static constexpr int BYTES = 48 * 1024;
+  static constexpr int CHUNK = (BYTES - TAB_BYTES) / 40;
+};
+struct __attribute__((aligned(8))) RcTail {
+  float upz, w;
+};
+// Thole weight of an atom, w = sqrt(a log2 e) / pdamp^3, so that
+// w_i w_k r^3 is a (r / (pdamp_i pdamp_k))^3 in base 2 (damptholed,
+// damping.f:142-215).  An atom without polarizability is undamped: its
+// weight 1e15 puts w_i w_k r^3 past the cut of 72 (and, with r below 60,
+// below the FP32 range) for any partner
+__device__ __forceinline__ float rc_weight(double pdamp, double sqa) {
+  const float p = float(pdamp);
+  return p > 0.0f ? float(sqa) / (p * p * p) : 1.0e15f;
+}
+constexpr float kTholeCut2 = 72.1347520444f;  // 50 log2 e: damptholed's damp < 50 in base 2
+constexpr float k3Ln2 = 2.0794415416798f;     // 3 ln 2
 
-// one lane pack of the compacted list
-template <typename real>
+// one lane pack of the compacted list (indices only when the coefficients
+// are recomputed)
+template <typename real, bool RC>
 struct LanePack {
   unsigned int ix[kPack / 2];  // two 16-bit union positions per word, low half first
-  Coef2<real> cf[kPack];
+  Coef2<real> cf[RC ? 1 : kPack];
 };
 
-template <typename real>
+template <typename real, bool RC>
 __device__ __forceinline__ void load_pack(const GLOBAL unsigned short* TL, const GLOBAL Coef2<real>* TC, size_t np,
-                                          int i, int b, LanePack<real>& p) {
+                                          int i, int b, LanePack<real, RC>& p) {
   typedef unsigned int U4 __attribute__((ext_vector_type(4)));
   typedef real R4 __attribute__((ext_vector_type(4)));
   static_assert(kPack == 8, "one 16-byte index load per pack");
@@ -861,12 +888,14 @@ __device__ __forceinline__ void load_pack(const GLOBAL unsigned short* TL, const
   p.ix[1] = v.y;
   p.ix[2] = v.z;
   p.ix[3] = v.w;
-  const GLOBAL R4* c = reinterpret_cast<const GLOBAL R4*>(TC + at * kPack);
+  if (!RC) {
+    const GLOBAL R4* c = reinterpret_cast<const GLOBAL R4*>(TC + at * kPack);
 #pragma unroll
-  for (int k = 0; k < kPack / 2; ++k) {
-    const R4 u = c[k];
-    p.cf[2 * k] = coef2<real>(u.x, u.y);
-    p.cf[2 * k + 1] = coef2<real>(u.z, u.w);
+    for (int k = 0; k < kPack / 2; ++k) {
+      const R4 u = c[k];
+      p.cf[2 * k] = coef2<real>(u.x, u.y);
+      p.cf[2 * k + 1] = coef2<real>(u.z, u.w);
+    }
   }
 }
 
@@ -876,34 +905,76 @@ struct ClGeom {
   const Quad<real>* s_a;
   const Quad<real>* s_b;
   const real* s_c;
+  const RcTail* s_t;        // recomputed coefficients: (up_z, w) per union atom
+  const Quad<float>* s_g;   // and the Ewald table
   real pix, piy, piz;  // the lane's atom relative to the block centre
   real lx, ly, lz, ilx, ily, ilz;
+  float wi, aw;  // the lane's Thole weight; aewald * kGtabScale
 };
 
-template <typename real, bool MASK, bool WRAP>
-__device__ __forceinline__ void apply_pack(const LanePack<real>& p, int c0, int cn, const ClGeom<real>& g,
+// Entries of one pack.  With RC the coefficients of the unscaled pair
+// (tmatxb_pme, newinduce_pme.f:1564-1574) are formed from r:
+//   dmp3 = bn1 - (1 - lambda3) / r^3 = (G1 - e) / r^3
+//   dmp5 = bn2 - 3 (1 - lambda5) / r^5 = (G2 - 3 (1 + t) e) / r^5
+// with G1 = r^3 bn1, G2 = r^5 bn2 (dampewald, damping.f:17-56) from the
+// cubic table in x = aewald r, and e = exp(-t), t = a u^3 (damptholed,
+// damping.f:142-215; e = 0 from t = 50 on, as there).  Entries outside
+// [c0, c0 + cn) (another chunk, or padding) add nothing.
+template <typename real, bool MASK, bool WRAP, bool RC>
+__device__ __forceinline__ void apply_pack(const LanePack<real, RC>& p, int c0, int cn, const ClGeom<real>& g,
                                            double* acc) {
   real f0 = real(0), f1 = real(0), f2 = real(0), f3 = real(0), f4 = real(0), f5 = real(0);
 #pragma unroll
   for (int e = 0; e < kPack; ++e) {
     const unsigned int word = p.ix[e >> 1];
     int q = int((e & 1) ? (word >> 16) : (word & 0xffffu));
-    real d3 = p.cf[e].x, d5 = p.cf[e].y;
+    bool in = true;
     if (MASK) {
       q -= c0;
-      const bool in = (unsigned int)q < (unsigned int)cn;
+      in = (unsigned int)q < (unsigned int)cn;
       q = in ? q : 0;
-      d3 = in ? d3 : real(0);
-      d5 = in ? d5 : real(0);
     }
     const Quad<real> A = g.s_a[q];
     const Quad<real> B = g.s_b[q];
-    const real upz = g.s_c[q];
     real dx = A.a - g.pix, dy = A.b - g.piy, dz = A.c - g.piz;
     if (WRAP) {
       dx -= g.lx * rint(dx * g.ilx);
       dy -= g.ly * rint(dy * g.ily);
       dz -= g.lz * rint(dz * g.ilz);
+    }
+    real upz, d3, d5;
+    if constexpr (RC) {
+      const RcTail T = g.s_t[q];
+      upz = T.upz;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      // r^2 of a pair is far from the denormal range: the bare rsq
+      const float rr1 = __builtin_amdgcn_rsqf(r2);
+      const float r = r2 * rr1;
+      // x < kGtabN within the cutoff (the host checks aewald * cut); the
+      // clamp keeps masked and sentinel entries inside the table
+      const float x = r * g.aw;
+      const int j = min(int(x), kGtabN - 1);
+      const float fr = x - float(j);
+      // (G2 from the table rather than from G1 and one more exp2: measured
+      // 3% faster over a 16-system launch)
+      const Quad<float> P = g.s_g[2 * j];
+      const Quad<float> Q = g.s_g[2 * j + 1];
+      const float G1 = ((P.d * fr + P.c) * fr + P.b) * fr + P.a;
+      const float G2 = ((Q.d * fr + Q.c) * fr + Q.b) * fr + Q.a;
+      const float tq = g.wi * T.w * (r2 * r);
+      const float ex = tq < kTholeCut2 ? __builtin_amdgcn_exp2f(-tq) : 0.0f;
+      const float rr2 = rr1 * rr1;
+      const float rr3 = rr2 * rr1;
+      d3 = rr3 * (G1 - ex);
+      d5 = (rr3 * rr2) * (G2 - ex * (3.0f + k3Ln2 * tq));
+    } else {
+      upz = g.s_c[q];
+      d3 = p.cf[e].x;
+      d5 = p.cf[e].y;
+    }
+    if (MASK) {
+      d3 = in ? d3 : real(0);
+      d5 = in ? d5 : real(0);
     }
     const real urd = d5 * (dx * A.d + dy * B.a + dz * B.b);
     const real urp = d5 * (dx * B.c + dy * B.d + dz * upz);
@@ -923,20 +994,23 @@ __device__ __forceinline__ void apply_pack(const LanePack<real>& p, int c0, int 
 }
 
 // whole union staged: packs [b0, b1) of the lane, NP per step, the next
-// NP in flight (loads past the segment re-read its last pack)
-template <typename real, int NP, bool WRAP>
+// NP in flight (loads past the segment re-read its last pack).  With RC the
+// padding entries reach the sentinel staged past the union (exact zeros),
+// except in blocks that wrap, where the sentinel's image could land near
+// the atom: there every entry is tested against the union's cn atoms.
+template <typename real, int NP, bool WRAP, bool RC>
 __device__ __forceinline__ void walk_packs(const GLOBAL unsigned short* TL, const GLOBAL Coef2<real>* TC, size_t np,
-                                           int i, int b0, int b1, const ClGeom<real>& g, double* acc) {
+                                           int i, int b0, int b1, int cn, const ClGeom<real>& g, double* acc) {
   if (b0 >= b1) return;
-  LanePack<real> cur[NP], nxt[NP];
+  LanePack<real, RC> cur[NP], nxt[NP];
 #pragma unroll
-  for (int k = 0; k < NP; ++k) load_pack<real>(TL, TC, np, i, min(b0 + k, b1 - 1), cur[k]);
+  for (int k = 0; k < NP; ++k) load_pack<real, RC>(TL, TC, np, i, min(b0 + k, b1 - 1), cur[k]);
   for (int b = b0; b < b1; b += NP) {
 #pragma unroll
-    for (int k = 0; k < NP; ++k) load_pack<real>(TL, TC, np, i, min(b + NP + k, b1 - 1), nxt[k]);
+    for (int k = 0; k < NP; ++k) load_pack<real, RC>(TL, TC, np, i, min(b + NP + k, b1 - 1), nxt[k]);
 #pragma unroll
     for (int k = 0; k < NP; ++k)
-      if (k == 0 || b + k < b1) apply_pack<real, false, WRAP>(cur[k], 0, 0, g, acc);
+      if (k == 0 || b + k < b1) apply_pack<real, RC && WRAP, WRAP, RC>(cur[k], 0, cn, g, acc);
 #pragma unroll
     for (int k = 0; k < NP; ++k) cur[k] = nxt[k];
   }
@@ -946,39 +1020,50 @@ __device__ __forceinline__ void walk_packs(const GLOBAL unsigned short* TL, cons
 // sorted by union position, so the walk stops at the first pack that starts
 // beyond the chunk; the last pack walked may straddle into the next chunk
 // and is walked again there (masked).
-template <typename real, bool WRAP>
+template <typename real, bool WRAP, bool RC>
 __device__ __forceinline__ void walk_chunk(const GLOBAL unsigned short* TL, const GLOBAL Coef2<real>* TC, size_t np,
                                            int i, int b1, int c0, int cn, int& resume, const ClGeom<real>& g,
                                            double* acc) {
   int last = -1;
   for (int b = resume; b < b1; ++b) {
-    LanePack<real> p;
-    load_pack<real>(TL, TC, np, i, b, p);
+    LanePack<real, RC> p;
+    load_pack<real, RC>(TL, TC, np, i, b, p);
     if (int(p.ix[0] & 0xffffu) >= c0 + cn) break;
-    apply_pack<real, true, WRAP>(p, c0, cn, g, acc);
+    apply_pack<real, true, WRAP, RC>(p, c0, cn, g, acc);
     last = b;
   }
   if (last >= 0) resume = last;
 }
 
-template <typename real, int S, int NB>
-__global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, const Work* work, int vin, int vout,
+// RC: the coefficients are recomputed per entry (systems with rc_sqa > 0);
+// each instance skips the systems of the other kind.
+// Waves per SIMD asked of RC: the LDS stage allows three workgroups per CU,
+// so four waves per workgroup can reach three waves per SIMD (without the
+// bound the register count allows two) and eight can reach four (two
+// workgroups).
+constexpr int rc_waves_per_simd(int S) { return S == 8 ? 4 : S >= 4 ? 3 : 1; }
+template <typename real, int S, int NB, bool RC>
+__global__ void __launch_bounds__(64 * S, RC ? rc_waves_per_simd(S) : 1) k_tmatxb_cl(const SysView* views, const Work* work, int vin, int vout,
                                                        int dodiag, int gate) {
-  constexpr int C = ClStage<real>::CHUNK;
+  static_assert(!RC || sizeof(real) == 4, "recomputed coefficients are FP32");
+  constexpr int C = RC ? RcStage::CHUNK : ClStage<real>::CHUNK;
   static_assert(NB % kPack == 0, "whole lane packs per step");
   constexpr int NP = NB / kPack;
   // the staging arrays and the final cross-wavefront sum share one buffer
-  constexpr int STAGE_BYTES = C * (2 * (int)sizeof(Quad<real>) + (int)sizeof(real));
+  constexpr int STAGE_BYTES = RC ? RcStage::BYTES : C * (2 * (int)sizeof(Quad<real>) + (int)sizeof(real));
   constexpr int RED_BYTES = S * 6 * 64 * (int)sizeof(double);
   __shared__ __attribute__((aligned(16))) unsigned char smem[STAGE_BYTES > RED_BYTES ? STAGE_BYTES : RED_BYTES];
   Quad<real>* s_a = reinterpret_cast<Quad<real>*>(smem);
   Quad<real>* s_b = s_a + C;
   real* s_c = reinterpret_cast<real*>(s_b + C);
+  RcTail* s_t = reinterpret_cast<RcTail*>(s_b + C);
+  Quad<float>* s_g = reinterpret_cast<Quad<float>*>(smem + 40 * C);
   double* red = reinterpret_cast<double*>(smem);
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
   if (gated_out(sv.st, gate)) return;
+  if (RC != (sv.rc_sqa > 0.0)) return;
   const int n = sv.n;
   const size_t np = sv.npad;
   const double bx = sv.box[0], by = sv.box[1], bz = sv.box[2];
@@ -986,6 +1071,8 @@ __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, cons
   const GLOBAL double* X = gp(sv.x);
   const GLOBAL double* Y = gp(sv.y);
   const GLOBAL double* Z = gp(sv.z);
+  const GLOBAL double* PD = gp(sv.pdamp);
+  const double sqa = sv.rc_sqa;
   const double* in_g = vsel(sv, vin);
   const GLOBAL double* IN = gp(in_g);
   const GLOBAL unsigned short* TL = gp(sv.tlist);
@@ -997,6 +1084,15 @@ __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, cons
   const GLOBAL int* UL = gp(sv.ulist) + (size_t)w.blk * sv.umax;
   const GLOBAL double* BB = gp(sv.bbox) + (size_t)w.blk * 6;
   const double cx = BB[0], cy = BB[1], cz = BB[2];
+  if (RC) {
+    // the Ewald table, visible after the first staging barrier
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    const GLOBAL F4* GT = reinterpret_cast<const GLOBAL F4*>(gp(sv.gtab));
+    for (int t = threadIdx.x; t < 2 * kGtabN; t += 64 * S) {
+      const F4 v = GT[t];
+      s_g[t] = Quad<float>{v.x, v.y, v.z, v.w};
+    }
+  }
   // union atoms lie within cut + skin of a block atom; the block's atoms
   // moved at most skin/2 since its box was taken.  Away from the wrap, the
   // image nearest the block centre is the pair's image, and rounding
@@ -1007,12 +1103,16 @@ __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, cons
   g.s_a = s_a;
   g.s_b = s_b;
   g.s_c = s_c;
+  g.s_t = s_t;
+  g.s_g = s_g;
   g.lx = real(bx);
   g.ly = real(by);
   g.lz = real(bz);
   g.ilx = real(ibx);
   g.ily = real(iby);
   g.ilz = real(ibz);
+  g.wi = RC && ok ? rc_weight(PD[i], sqa) : 0.0f;
+  g.aw = float(sv.aewald) * kGtabScale;
   const double xi = ok ? X[i] : 0.0, yi = ok ? Y[i] : 0.0, zi = ok ? Z[i] : 0.0;
   {
     double d = xi - cx, e = yi - cy, f = zi - cz;
@@ -1046,7 +1146,7 @@ __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, cons
       int kk[SB];
 #pragma unroll
       for (int u = 0; u < SB; ++u) kk[u] = UL[c0 + min(q0 + u * 64 * S, cn - 1)];
-      double px[SB], py[SB], pz[SB], mu[SB][6];
+      double px[SB], py[SB], pz[SB], mu[SB][6], pdk[SB];
 #pragma unroll
       for (int u = 0; u < SB; ++u) {
         px[u] = X[kk[u]];
@@ -1054,6 +1154,7 @@ __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, cons
         pz[u] = Z[kk[u]];
 #pragma unroll
         for (int c = 0; c < 6; ++c) mu[u][c] = IN[(size_t)c * np + kk[u]];
+        pdk[u] = RC ? PD[kk[u]] : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < SB; ++u) {
@@ -1065,21 +1166,31 @@ __global__ void __launch_bounds__(64 * S) k_tmatxb_cl(const SysView* views, cons
           f -= bz * rint(f * ibz);
           s_a[q] = Quad<real>{real(d), real(e), real(f), real(mu[u][0])};
           s_b[q] = Quad<real>{real(mu[u][1]), real(mu[u][2]), real(mu[u][3]), real(mu[u][4])};
-          s_c[q] = real(mu[u][5]);
+          if (RC)
+            s_t[q] = RcTail{float(mu[u][5]), rc_weight(pdk[u], sqa)};
+          else
+            s_c[q] = real(mu[u][5]);
         }
       }
     }
+    // the padding entries' sentinel: zero dipoles and weight, 1.7e3 A away
+    // (every term it adds is an exact zero)
+    if (RC && threadIdx.x == 0 && ucount < C) {
+      s_a[cn] = Quad<real>{real(1.0e3), real(1.0e3), real(1.0e3), real(0)};
+      s_b[cn] = Quad<real>{real(0), real(0), real(0), real(0)};
+      s_t[cn] = RcTail{0.0f, 0.0f};
+    }
     __syncthreads();
-    if (ucount <= C) {
+    if (RC ? ucount < C : ucount <= C) {
       if (wrap)
-        walk_packs<real, NP, true>(TL, TC, np, i, b0, b1, g, acc);
+        walk_packs<real, NP, true, RC>(TL, TC, np, i, b0, b1, cn, g, acc);
       else
-        walk_packs<real, NP, false>(TL, TC, np, i, b0, b1, g, acc);
+        walk_packs<real, NP, false, RC>(TL, TC, np, i, b0, b1, cn, g, acc);
     } else {
       if (wrap)
-        walk_chunk<real, true>(TL, TC, np, i, b1, c0, cn, resume, g, acc);
+        walk_chunk<real, true, RC>(TL, TC, np, i, b1, c0, cn, resume, g, acc);
       else
-        walk_chunk<real, false>(TL, TC, np, i, b1, c0, cn, resume, g, acc);
+        walk_chunk<real, false, RC>(TL, TC, np, i, b1, c0, cn, resume, g, acc);
     }
   }
   // scaled pairs with their per-solve FP64 coefficients (tmatxb u-scale),

@@ -338,6 +338,7 @@ thp_status build_batch(Serve* sv, BatchBuf& bb, const std::vector<thp_sys*>& mem
   b = Batch();
   b.prec = prec;
   b.nsys = m;
+  for (int q = 0; q < m; ++q) b.rc |= members[q]->view.rc_sqa > 0.0 ? 1 : 2;
   for (auto& g : groups) {
     GridPool& gp = sv->pools[g.first];
     for (size_t slot = 0; slot < g.second.size(); ++slot) {
