@@ -851,7 +851,7 @@ struct ClStage {
 struct RcStage {
   static constexpr int TAB_BYTES = kGtabN * 8 * (int)sizeof(float);
   static constexpr int BYTES = 48 * 1024;
-  static constexpr int CHUNK = (BYTES - TAB_BYTES) / 40;
+  static constexpr int CHUNK = ((BYTES - TAB_BYTES) / 40) & ~1;  // even: the table after it stays 16-byte aligned
 };
 struct __attribute__((aligned(8))) RcTail {
   float upz, w;
