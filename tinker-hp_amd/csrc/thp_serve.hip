@@ -58,6 +58,7 @@ struct ServeSystem {
   uint64_t last_use = 0;
   int member = -1;           // position in the iteration batch, -1 if not a member
   int idle_passes = 0;       // passes without a solve in flight while a member
+  uint64_t gen = 0;          // counts the creations of its device data
 };
 
 struct ServeSystem;
@@ -71,6 +72,10 @@ struct Request {
   double* rp = nullptr;   // pinned [n][13] caller-order multipoles, or null (unchanged)
   size_t rp_cap = 0;
   Clock::time_point t_submit;
+  // its coordinates (and multipoles) already copied into the system's device
+  // buffers on the upload stream, for device data of generation pre_gen
+  bool pre = false;
+  uint64_t pre_gen = 0;
 };
 
 struct InFlight {
@@ -79,6 +84,7 @@ struct InFlight {
   Clock::time_point t_start;
   int warm = 0;
   bool restart = false;  // set-up repeated after a list rebuild (stale coordinates)
+  bool setup_seen = false;  // its set-up's pass has been read back, not stale
 };
 
 struct DipoleCopy {  // thp_serve_get_dipoles handed to the worker
@@ -167,6 +173,15 @@ struct Serve {
   // systems with device data, in admission order (worker); the worker never
   // walks `systems`, which other threads extend under `mu`
   std::vector<ServeSystem*> residents;
+  // Early uploads: while a pass runs, the coordinates of the next queued
+  // request of a system are copied into its device buffers on a second
+  // stream (default priority) when nothing queued can read them before
+  // that request's set-up: the system is idle, or its running solve's set-up
+  // has been read back and was not stale (a stale set-up is repeated from
+  // the same buffers).  The next pass waits on ev_up before its launches.
+  hipStream_t up = nullptr;
+  hipEvent_t ev_up = nullptr;
+  bool up_pending = false;
 };
 
 // request coordinate buffers: pinned, recycled through the pool
@@ -388,6 +403,8 @@ void drop_member(Serve* sv, ServeSystem* ss) {
 
 thp_status release_system(Serve* sv, ServeSystem* ss) {
   if (!ss->s) return THP_OK;
+  // an early upload may still be writing the system's buffers
+  SHIP(sv->ctx, hipStreamSynchronize(sv->up));
   drop_member(sv, ss);
   sv->resident -= ss->bytes;
   thp_system_destroy(ss->s);
@@ -438,6 +455,7 @@ thp_status admit(Serve* sv, ServeSystem* ss, const std::vector<ServeSystem*>& pi
     }
   }
   ss->s = s;
+  ++ss->gen;
   ss->bytes = s->bytes;
   sv->resident += ss->bytes;
   ss->has_mu = false;
@@ -696,6 +714,39 @@ void update_members(Serve* sv) {
   }
 }
 
+// Copy the coordinates of queued requests into their systems' device
+// buffers on the upload stream while the launched pass runs (see Serve::up).
+// Only the first queued request of a system qualifies.
+thp_status early_uploads(Serve* sv) {
+  std::vector<const ServeSystem*> ready;  // running solves past a clean set-up
+  for (const InFlight& f : sv->active)
+    if (f.setup_seen) ready.push_back(f.sys);
+  bool any = false;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  std::vector<const ServeSystem*> seen;
+  for (Request& q : sv->queue) {
+    ServeSystem* ss = q.sys;
+    if (std::find(seen.begin(), seen.end(), ss) != seen.end()) continue;
+    seen.push_back(ss);
+    if (q.pre || !ss->s || (!q.xyz && !q.rp)) continue;
+    if (ss->busy && std::find(ready.begin(), ready.end(), ss) == ready.end()) continue;
+    const size_t n = (size_t)ss->s->n;
+    if (q.xyz)
+      SHIP(sv->ctx, hipMemcpyAsync((void*)ss->s->view.raw, q.xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, sv->up));
+    if (q.rp)
+      SHIP(sv->ctx, hipMemcpyAsync((void*)ss->s->view.raw_rpole, q.rp, sizeof(double) * 13 * n, hipMemcpyHostToDevice,
+                                   sv->up));
+    q.pre = true;
+    q.pre_gen = ss->gen;
+    any = true;
+  }
+  if (any) {
+    SHIP(sv->ctx, hipEventRecord(sv->ev_up, sv->up));
+    sv->up_pending = true;
+  }
+  return THP_OK;
+}
+
 // One pass of the loop: admission (residency, coordinate uploads, lists of
 // new systems), one launch sequence over the member batch, one wait, then
 // the finished solves retire and solves whose coordinates outran their
@@ -730,6 +781,12 @@ thp_status pass(Serve* sv, std::vector<Request>& starting) {
     ss->last_use = ++sv->use_clock;
     const auto tc0 = Clock::now();
     thp_status rc = THP_OK;
+    const bool pre = q.pre && q.pre_gen == ss->gen;  // copied during an earlier pass
+    if (pre) {
+      if (q.xyz) uploads.emplace_back(q.xyz, q.xyz_cap);
+      if (q.rp) uploads.emplace_back(q.rp, q.rp_cap);
+      q.xyz = q.rp = nullptr;
+    }
     if (q.xyz) {
       const size_t n = (size_t)ss->s->n;
       if (hipMemcpyAsync((void*)ss->s->view.raw, q.xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream) !=
@@ -823,7 +880,11 @@ thp_status pass(Serve* sv, std::vector<Request>& starting) {
     SHIP(ctx, hipMemcpyAsync(bb.req, bb.h_req, sizeof(int) * m, hipMemcpyHostToDevice, ctx->stream));
     SHIP(ctx, hipMemcpyAsync(bb.warm, bb.h_warm, sizeof(int) * m, hipMemcpyHostToDevice, ctx->stream));
   }
-  // ---- one launch sequence for the pass
+  // ---- one launch sequence for the pass, after the early uploads
+  if (sv->up_pending) {
+    SHIP(ctx, hipStreamWaitEvent(ctx->stream, sv->ev_up, 0));
+    sv->up_pending = false;
+  }
   const auto t_chunk = Clock::now();
   const int iters = std::max(1, sv->o.iters_per_graph);
   SRET(run_pass(sv, variant, iters));
@@ -831,6 +892,7 @@ thp_status pass(Serve* sv, std::vector<Request>& starting) {
   sv->active_sum += (double)sv->active.size();
   sv->stats.pair_segs = bb.b.segs;
   SHIP(ctx, hipMemcpyAsync(sv->h_out, sv->d_out, sizeof(StateOut) * m, hipMemcpyDeviceToHost, ctx->stream));
+  SRET(early_uploads(sv));
   const auto t_wait = Clock::now();
   SHIP(ctx, hipStreamSynchronize(ctx->stream));
   const auto t_done = Clock::now();
@@ -863,6 +925,7 @@ thp_status pass(Serve* sv, std::vector<Request>& starting) {
       sv->restarts.push_back(std::move(f));
       continue;
     }
+    f.setup_seen = true;
     if (so.err == 0 && so.st.conv == kRunning) {
       still.push_back(std::move(f));
       continue;
@@ -984,6 +1047,12 @@ thp_status thp_serve_start(thp_ctx* ctx, const thp_serve_opts* opts) {
       return fail(ctx, THP_ERR_HIP, "hipMemGetInfo");
     }
     sv->budget = (int64_t)(0.8 * (double)fr);
+  }
+  if (hipStreamCreateWithFlags(&sv->up, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&sv->ev_up, hipEventDisableTiming) != hipSuccess) {
+    if (sv->up) (void)hipStreamDestroy(sv->up);
+    delete sv;
+    return fail(ctx, THP_ERR_HIP, "cannot create the serving loop's upload stream");
   }
   ctx->serve = sv;
   sv->worker = std::thread(worker_main, sv);
@@ -1169,6 +1238,7 @@ thp_status thp_serve_stop(thp_ctx* ctx) {
   sv->cv_work.notify_all();
   if (sv->worker.joinable()) sv->worker.join();
   (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamSynchronize(sv->up);
   free_batchbuf(sv->setup);
   free_batchbuf(sv->iter);
   for (auto& kv : sv->pools) free_pool(kv.second);
@@ -1178,6 +1248,8 @@ thp_status thp_serve_stop(thp_ctx* ctx) {
     if (s->s) thp_system_destroy(s->s);
   }
   for (auto& kv : sv->pinned_pool) (void)hipHostFree(kv.second);
+  (void)hipEventDestroy(sv->ev_up);
+  (void)hipStreamDestroy(sv->up);
   const thp_status fatal = sv->fatal;
   if (fatal != THP_OK) ctx->err = sv->fatal_msg;
   ctx->serve = nullptr;
