@@ -19,6 +19,7 @@ timed with HIP events on the library's stream; cpu_baseline times the FP64
 CPU oracle on the host cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serve] [--no-stream] [--no-dhfr2] [--no-large]
+                       [--graphs 0|1]   (serving passes launched eagerly, the default, or replayed as graphs)
        python bench.py --serve-only | --kernel-probe | --stream-only RATE   (profiling aids, no JSON line)
 """
 import argparse
@@ -181,6 +182,8 @@ def main():
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--stream-requests", type=int, default=2000)
     ap.add_argument("--copies", type=int, default=16, help="resident 24k systems in the serving leg")
+    ap.add_argument("--graphs", type=int, default=0, choices=(0, 1),
+                    help="serving passes replayed as captured HIP graphs (1) or launched eagerly (0, measured faster)")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="only launch the path kernels and a copy of known size (for rocprofv3 --pmc runs)")
     ap.add_argument("--serve-only", action="store_true",
@@ -203,7 +206,7 @@ def main():
         import thp.serve as tserve
         ctx = thp.Context(0)
         serve = tserve.bench_serving(ctx, thp.water_23559(), steps=max(args.steps, 8), warmup=args.warmup,
-                                     poleps=args.poleps, copies=args.copies, iters_per_graph=4, use_graphs=1)
+                                     poleps=args.poleps, copies=args.copies, iters_per_graph=4, use_graphs=args.graphs)
         serve["grid_shape"] = ctx.grid_shape((64, 64, 64))
         log(f"bench: serve {serve}")
         ctx.close()
@@ -250,7 +253,7 @@ def main():
             import thp.serve as tserve
             serve = tserve.bench_serving(ctx, system, steps=max(args.steps, 8), warmup=args.warmup,
                                          poleps=args.poleps, copies=args.copies, iters_per_graph=4,
-                                         use_graphs=1, keep_last=not args.no_cpu)
+                                         use_graphs=args.graphs, keep_last=not args.no_cpu)
             serve_last = serve.pop("last", None)
             serve["grid_shape"] = ctx.grid_shape(system.nfft)  # (brick edge, FFT lines) the autotuner picked
             legs["serve"] = serve
