@@ -153,12 +153,24 @@ def _thole_varied(s):
     return dataclasses.replace(s, thole=th, name=s.name + "-thole")
 
 
-@pytest.mark.parametrize("case", ["water1944", "solute", "thole"])
+def _some_unpolarizable(s):
+    """every 7th water without polarizability: pdamp 0, so its atoms' pairs
+    are undamped (damptholed's damp = 0 branch) and their diagonal takes
+    tinypol (whole molecules, so that no polarizable atom shares a molecule
+    with an unpolarizable one)"""
+    pol = s.polarity.copy()
+    mol = np.arange(s.n) // 3
+    pol[mol % 7 == 3] = 0.0
+    return dataclasses.replace(s, polarity=pol, name=s.name + "-pol0")
+
+
+@pytest.mark.parametrize("case", ["water1944", "solute", "thole", "pol0"])
 @pytest.mark.parametrize("kind", ["recomputed", "stored"])
 def test_matvec_coefficient_kinds_match_oracle(ctx, ctx_stored, case, kind):
     s = {"water1944": thp.water_1944,
          "solute": lambda: thp.make_system(3000, 34.0, n_solute=1200, blob_radius=11.0, seed=9, nfft=(36,) * 3),
-         "thole": lambda: _thole_varied(thp.water_1944())}[case]()
+         "thole": lambda: _thole_varied(thp.water_1944()),
+         "pol0": lambda: _some_unpolarizable(thp.serving_system(2400, 0.0, seed=17))}[case]()
     orc = O.Oracle(s)
     solver = thp.Solver(ctx if kind == "recomputed" else ctx_stored, s, MIXED)
     mu = np.random.default_rng(4).normal(size=(s.n, 2, 3)) * 0.1
