@@ -1136,8 +1136,9 @@ thp_status b_pair_tmatxb_on(thp_ctx* ctx, const Batch& b, int vin, int vout, int
   if (b.prec == THP_PREC_FP64) {
     launch_tmatxb<double, false>(b, vin, vout, dodiag, gate, st);
   } else {
-    if (b.rc & 1) launch_tmatxb<float, true>(b, vin, vout, dodiag, gate, st);
-    if ((b.rc & 2) || !b.rc) launch_tmatxb<float, false>(b, vin, vout, dodiag, gate, st);
+    const int kinds = b.rc ? b.rc : 3;  // not recorded: both kernels, each skips the other kind
+    if (kinds & 1) launch_tmatxb<float, true>(b, vin, vout, dodiag, gate, st);
+    if (kinds & 2) launch_tmatxb<float, false>(b, vin, vout, dodiag, gate, st);
   }
   return launch_check(ctx, "k_tmatxb_cl");
 }
