@@ -1056,8 +1056,8 @@ __global__ void __launch_bounds__(64 * S, RC ? rc_waves_per_simd(S) : 1) k_tmatx
   Quad<real>* s_a = reinterpret_cast<Quad<real>*>(smem);
   Quad<real>* s_b = s_a + C;
   real* s_c = reinterpret_cast<real*>(s_b + C);
-  RcTail* s_t = reinterpret_cast<RcTail*>(s_b + C);
-  Quad<float>* s_g = reinterpret_cast<Quad<float>*>(smem + 40 * C);
+  RcTail* s_t = RC ? reinterpret_cast<RcTail*>(s_b + C) : nullptr;
+  Quad<float>* s_g = RC ? reinterpret_cast<Quad<float>*>(smem + 40 * C) : nullptr;
   double* red = reinterpret_cast<double*>(smem);
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
