@@ -1485,6 +1485,9 @@ __device__ __forceinline__ int real_as_int(real v) {
 // W wavefronts (private tiles) per workgroup: more tiles shorten one
 // brick's atom loop, fewer tiles cost less LDS and less zeroing and
 // summing per brick, which pays when a batch brings many bricks
+// home atoms staged in LDS at a time (16 to 32, for five workgroups per CU,
+// measured no faster)
+constexpr int kSpreadChunk = 64;
 template <typename real, int KIND, int W>
 __global__ void __launch_bounds__(64 * W) k_spread_tile(const SysView* views, const Work* bwork, int vin, int frac_in,
                                                         int gate) {
@@ -1499,8 +1502,8 @@ __global__ void __launch_bounds__(64 * W) k_spread_tile(const SysView* views, co
   typedef real R4 __attribute__((ext_vector_type(4)));
   // a wavefront's tile, plus a slot the idle lanes of the second pass write
   __shared__ __attribute__((aligned(16))) real s_tile[W][kTile3 * 2 + 4];
-  __shared__ __attribute__((aligned(32))) real s_th[64][3][kOrder][TW];
-  __shared__ __attribute__((aligned(32))) real s_f[64][FW];
+  __shared__ __attribute__((aligned(32))) real s_th[kSpreadChunk][3][kOrder][TW];
+  __shared__ __attribute__((aligned(32))) real s_f[kSpreadChunk][FW];
   const Work w = bwork[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
@@ -1533,8 +1536,8 @@ __global__ void __launch_bounds__(64 * W) k_spread_tile(const SysView* views, co
     pmz[ps] = l / (kOrder * kOrder);
     poff[ps] = (pmz[ps] * kTile + pmy[ps]) * kTile + pmx[ps];
   }
-  for (int c0 = a0; c0 < a1; c0 += 64) {
-    const int cnt = min(64, a1 - c0);
+  for (int c0 = a0; c0 < a1; c0 += kSpreadChunk) {
+    const int cnt = min(kSpreadChunk, a1 - c0);
     __syncthreads();
     for (int q = threadIdx.x; q < cnt; q += 64 * W) {
       const int i = BLIST[c0 + q];
