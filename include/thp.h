@@ -36,6 +36,10 @@
  *   thp_induce ................. newinduce_pme2 + inducepcg_pme2
  *                                (v1.2/source/newinduce_pme2.f:16-636;
  *                                 GPU/source/newinduce_pme2gpu.f:17-1137)
+ *   thp_induce_mu .............. the same solve with uind and uinp in one
+ *                                host array, as the reference keeps them in
+ *                                its mu(3,nrhs,npolebloc) work array
+ *                                (v1.2/source/newinduce_pme2.f:71,230-235)
  *   thp_set_solver ............. POLAR-ALG keyword (v1.2/source/prmkey.f:396-398)
  *                                selecting inducepcg_pme2 or inducejac_pme2
  *                                (v1.2/source/newinduce_pme2.f:210-217,
@@ -228,6 +232,11 @@ thp_status thp_pme_convolve(thp_sys* sys);
 
 thp_status thp_induce(thp_sys* sys, const thp_solve_opts* opts, double* uind, double* uinp,
                       double* ep, thp_solve_stats* stats);
+/* thp_induce with the dipoles as one host array mu[n][2][3] (uind, uinp per
+ * atom; with opts->guess 1 it holds the starting dipoles): the layout the
+ * solve keeps, so the result comes back in one copy without reordering */
+thp_status thp_induce_mu(thp_sys* sys, const thp_solve_opts* opts, double* mu /* [n][2][3] */, double* ep,
+                         thp_solve_stats* stats);
 /* Select the solver of thp_induce (THP_POLALG_*).  Jacobi/DIIS stops when
  * the Jacobi increment max_rhs sqrt(|munew - mu|^2 / 3n) < poleps and keeps
  * up to 25 subspace vectors on the device (2.4 kB per atom). */

@@ -328,6 +328,34 @@ def test_results_are_deterministic(ctx, prec):
     assert np.array_equal(a[0], c[0]) and a[1] == c[1]
 
 
+@pytest.mark.parametrize("prec", [FP64, MIXED])
+def test_split_and_interleaved_dipoles_agree(ctx, prec):
+    # thp_induce (uind, uinp apart, as the reference passes them) and
+    # thp_induce_mu (one [n][2][3] array) return the same solve
+    solver = thp.Solver(ctx, thp.water_1944(), prec)
+    mu, ep, st = solver.newinduce_pme2()
+    uind, uinp, ep2, st2 = solver.newinduce_pme2_split()
+    assert np.array_equal(mu[:, 0], uind) and np.array_equal(mu[:, 1], uinp)
+    assert ep == ep2 and st.niter == st2.niter
+
+
+def test_iterations_queued_past_convergence_change_nothing(ctx):
+    # a mixed solve queues the last solve's iteration count before its first
+    # read of the solve state: after a tight solve (many iterations) a loose
+    # one stops where a fresh solver stops, with the same dipoles
+    s = thp.water_1944()
+    ref = thp.Solver(ctx, s, MIXED).newinduce_pme2(poleps=1e-4)
+    solver = thp.Solver(ctx, s, MIXED)
+    tight = solver.newinduce_pme2(poleps=1e-10)
+    assert tight[2].niter > ref[2].niter + 3
+    loose = solver.newinduce_pme2(poleps=1e-4)
+    assert loose[2].niter == ref[2].niter
+    assert np.array_equal(loose[0], ref[0]) and loose[1] == ref[1]
+    # politer caps the queued iterations, and the cap is reported
+    capped = solver.newinduce_pme2(poleps=1e-10, politer=3)
+    assert capped[2].niter == 3 and capped[2].status == -6
+
+
 def test_warm_start_converges_faster(ctx):
     s = thp.water_1944()
     solver = thp.Solver(ctx, s, MIXED)

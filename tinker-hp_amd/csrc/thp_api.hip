@@ -459,11 +459,27 @@ thp_status scatter_in(thp_sys* s, double* dst_dev, const double* host, int ncomp
   return sync(s);
 }
 
-thp_status gather_out(thp_sys* s, double* host, const double* src_dev, int ncomp) {
+// per-atom device array (sorted, [ncomp][npad]) into the pinned host stage
+// in caller order ([n][ncomp]); returns once the copy has landed
+thp_status gather_to_hstage(thp_sys* s, const double* src_dev, int ncomp) {
+  const size_t len = (size_t)s->n * ncomp;
+  if (len > s->hstage_len) {
+    if (s->hstage) (void)hipHostFree(s->hstage);
+    s->hstage = nullptr;
+    s->hstage_len = 0;
+    HIPCHK(s->ctx, hipHostMalloc((void**)&s->hstage, sizeof(double) * len, hipHostMallocDefault));
+    s->hstage_len = len;
+  }
   hipLaunchKernelGGL(k_gather_out, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream, s->stage, src_dev, s->iperm_dev, s->n, s->npad, ncomp);
   RET(launch_check(s->ctx, "k_gather_out"));
-  HIPCHK(s->ctx, hipMemcpyAsync(host, s->stage, sizeof(double) * s->n * ncomp, hipMemcpyDeviceToHost, s->ctx->stream));
+  HIPCHK(s->ctx, hipMemcpyAsync(s->hstage, s->stage, sizeof(double) * len, hipMemcpyDeviceToHost, s->ctx->stream));
   return sync(s);
+}
+
+thp_status gather_out(thp_sys* s, double* host, const double* src_dev, int ncomp) {
+  RET(gather_to_hstage(s, src_dev, ncomp));
+  std::memcpy(host, s->hstage, sizeof(double) * s->n * ncomp);
+  return THP_OK;
 }
 
 // bounding boxes and candidate block pairs (cut + buffer); valid until an
@@ -1392,6 +1408,13 @@ thp_status thp_init(thp_ctx** out, int device) {
     *out = ctx;
     return fail(ctx, THP_ERR_HIP, "cannot create the ordering events of device %d", device);
   }
+  // THP_SOLVE_CHUNK=k: PCG iterations of a one-system solve between host
+  // reads of the convergence flag, FP64 and mixed alike (default 1 for FP64,
+  // the reference's read after every iteration, and 4 for mixed)
+  if (const char* e = getenv("THP_SOLVE_CHUNK")) {
+    const int v = atoi(e);
+    if (v >= 1 && v <= 16) ctx->solve_chunk[0] = ctx->solve_chunk[1] = v;
+  }
   // THP_MATVEC_RC=0: every system stores its matvec coefficients per solve
   if (const char* e = getenv("THP_MATVEC_RC"))
     if (atoi(e) == 0) ctx->rc_matvec = false;
@@ -1857,6 +1880,7 @@ thp_status thp_system_destroy(thp_sys* s) {
     s->ctx->systems.erase(s);
   }
   for (auto& a : s->allocs) (void)hipFree(a.first);
+  if (s->hstage) (void)hipHostFree(s->hstage);
   for (auto& e : s->ev)
     if (e) (void)hipEventDestroy(e);
   delete s;
@@ -2533,21 +2557,52 @@ thp_status run_solve(thp_sys* s, const thp_solve_opts* o, bool warm, thp_solve_s
   else RET(b_setup(ctx, b, kGateNone));
   s->coef_valid = true;
   HIPCHK(ctx, hipEventRecord(s->ev[1], st));
-  int conv = 0, nmat = 1, first = 0;
+  SolveState fin{};
   if (tcg) RET(tcg_solve(s));
-  for (int it = 1; !tcg && it <= o->politer; ++it) {
-    if (jacobi) RET(jacobi_iteration(s, it, &nmat, &first));
-    else RET(b_iteration(ctx, b));
-    HIPCHK(ctx, hipMemcpyAsync(&conv, &s->view.st->conv, sizeof(int), hipMemcpyDeviceToHost, st));
-    RET(sync(s));
-    if (conv) break;
+  if (jacobi) {
+    // Jacobi/DIIS: a host step after every iteration
+    int conv = 0, nmat = 1, first = 0;
+    for (int it = 1; it <= o->politer; ++it) {
+      RET(jacobi_iteration(s, it, &nmat, &first));
+      HIPCHK(ctx, hipMemcpyAsync(&conv, &s->view.st->conv, sizeof(int), hipMemcpyDeviceToHost, st));
+      RET(sync(s));
+      if (conv) break;
+    }
+  } else if (!tcg) {
+    // PCG: k iterations between host reads of the solve state.  The
+    // kernels of an iteration skip a solve that has stopped (converged,
+    // politer reached, non-finite), so iterations queued past its end
+    // change nothing.  FP64 reads after every iteration, as the reference
+    // tests convergence after each (THP_SOLVE_CHUNK changes that).  Mixed
+    // solves queue the last solve's iteration count before the first read
+    // and finish inside every read (k_energy and k_finish act only on a
+    // stopped solve), so that the read that sees the stop also brings the
+    // energy.
+    const bool fp64 = s->prec == THP_PREC_FP64;
+    const int chunk = ctx->solve_chunk[fp64 ? 0 : 1];
+    int k = fp64 ? chunk : std::max(chunk, std::min(s->last_niter, kFirstChunkMax));
+    for (int done = 0; done < o->politer; k = chunk) {
+      k = std::min(k, o->politer - done);
+      for (int q = 0; q < k; ++q) RET(b_iteration(ctx, b));
+      done += k;
+      if (!fp64) {
+        RET(b_finish(ctx, b, 1));
+        HIPCHK(ctx, hipEventRecord(s->ev[2], st));
+      }
+      HIPCHK(ctx, hipMemcpyAsync(&fin, s->view.st, sizeof fin, hipMemcpyDeviceToHost, st));
+      RET(sync(s));
+      if (fin.conv != kRunning) break;
+    }
   }
-  RET(b_finish(ctx, b, 0));
+  const bool finished = fin.finalized != 0;  // the loop's last read came after the finish
+  if (!finished) RET(b_finish(ctx, b, 0));
   if (s->pred != THP_PRED_NONE && !short_range) RET(push_history(s));
-  HIPCHK(ctx, hipEventRecord(s->ev[2], st));
-  SolveState fin;
-  HIPCHK(ctx, hipMemcpyAsync(&fin, s->view.st, sizeof fin, hipMemcpyDeviceToHost, st));
+  if (!finished) {
+    HIPCHK(ctx, hipEventRecord(s->ev[2], st));
+    HIPCHK(ctx, hipMemcpyAsync(&fin, s->view.st, sizeof fin, hipMemcpyDeviceToHost, st));
+  }
   RET(sync(s));
+  if (!tcg && !jacobi) s->last_niter = fin.niter;
   s->mu_valid = true;
   s->mu_polalg = s->polalg;
   s->mu_short = short_range;
@@ -2666,8 +2721,8 @@ thp_status thp_induce(thp_sys* s, const thp_solve_opts* o, double* uind, double*
   thp_solve_stats local;
   thp_solve_stats* sp = stats ? stats : &local;
   RET(run_solve(s, o, warm, sp));
-  std::vector<double> m((size_t)s->n * 6);
-  RET(gather_out(s, m.data(), s->view.mu, 6));
+  RET(gather_to_hstage(s, s->view.mu, 6));
+  const double* m = s->hstage;
   for (int a = 0; a < s->n; ++a)
     for (int c = 0; c < 3; ++c) {
       uind[(size_t)a * 3 + c] = m[(size_t)a * 6 + c];
@@ -2861,6 +2916,19 @@ thp_status thp_time_kernel_batch(thp_sys* const* systems, int nsys, int kernel, 
     }
   }
   return THP_OK;
+}
+
+thp_status thp_induce_mu(thp_sys* s, const thp_solve_opts* o, double* mu, double* ep, thp_solve_stats* stats) {
+  if (!s || !o || !mu) return THP_ERR_ARG;
+  if (!(o->poleps > 0) || o->politer <= 0) return fail(s->ctx, THP_ERR_ARG, "bad poleps/politer");
+  const bool warm = o->guess == 1;
+  if (warm) RET(scatter_in(s, s->view.mu, mu, 6));
+  thp_solve_stats local;
+  thp_solve_stats* sp = stats ? stats : &local;
+  RET(run_solve(s, o, warm, sp));
+  RET(gather_out(s, mu, s->view.mu, 6));
+  if (ep) *ep = sp->ep;
+  return sp->status == THP_ERR_NAN ? fail(s->ctx, THP_ERR_NAN, "non-finite residual") : THP_OK;
 }
 
 thp_status thp_induce_dev(thp_sys* s, const thp_solve_opts* o, double* mu_dev, double* ep, thp_solve_stats* stats) {

@@ -28,6 +28,7 @@ constexpr int kPack = 8;    // compacted list entries per lane pack (16 B of ind
 // coefficients of G1 = r^3 bn1 and then of G2 = r^5 bn2
 constexpr int kGtabN = 256;
 constexpr float kGtabScale = 32.0f;  // intervals per unit of x
+constexpr int kFirstChunkMax = 48;  // PCG iterations queued before a mixed solve's first state read, at most
 constexpr int kOrder = 5;   // PME B-spline order handled by the kernels
 constexpr double kCoulomb = 332.063714;  // v1.2/source/MOD_units.f:81
 constexpr double kTinyPol = 1.0e-5;      // v1.2/source/MOD_polar.f:36
@@ -276,6 +277,7 @@ struct thp_ctx {
   hipStream_t side = nullptr;   // the real-space matvec beside the reciprocal chain
   bool fork_batch = true;       // batches of several systems fork too (THP_FORK_BATCH=0: not)
   bool rc_matvec = true;        // FP32 systems with one Thole value recompute matvec coefficients (THP_MATVEC_RC=0: not)
+  int solve_chunk[2] = {1, 4};  // one-system PCG iterations per state read after the first: FP64, mixed (THP_SOLVE_CHUNK)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // ordering of the stream-ordered entry points against the caller's stream
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -351,6 +353,13 @@ struct thp_sys {
   // staging for host arrays copied in and out in caller order ([n][10]
   // doubles: the widest array of the entry points)
   double* stage = nullptr;
+  // pinned host mirror of stage: results come back through it in one copy
+  // at full rate (grown on first need, freed with the system)
+  double* hstage = nullptr;
+  size_t hstage_len = 0;  // doubles
+  // iterations of the last PCG solve; the next mixed solve queues as many
+  // (at most kFirstChunkMax) before its first read of the solve state
+  int last_niter = 0;
   // dipoles in mu are the result of a solve of the current coordinates and
   // multipoles (the gradient needs them), and the solver that made them
   bool mu_valid = false;

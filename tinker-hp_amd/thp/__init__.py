@@ -140,6 +140,7 @@ def load_lib(path: str | None = None) -> C.CDLL:
         "thp_fphi_uind": (C.c_int, [vp, _dp, _dp, _dp]),
         "thp_pme_convolve": (C.c_int, [vp]),
         "thp_induce": (C.c_int, [vp, C.POINTER(SolveOpts), _dp, _dp, _dp, C.POINTER(SolveStats)]),
+        "thp_induce_mu": (C.c_int, [vp, C.POINTER(SolveOpts), _dp, _dp, C.POINTER(SolveStats)]),
         "thp_set_predictor": (C.c_int, [vp, C.c_int]),
         "thp_set_solver": (C.c_int, [vp, C.c_int]),
         "thp_set_tcg": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]),
@@ -501,20 +502,30 @@ class Solver:
     def newinduce_pme2(self, poleps=1e-8, politer=500, mu0=None):
         """Induced dipoles uind/uinp (n,3) each, E_pol (kcal/mol) and stats."""
         n = self.sys.n
-        uind, uinp = np.zeros((n, 3)), np.zeros((n, 3))
         guess = 0
         if mu0 is not None:
-            mu0 = np.asarray(mu0).reshape(n, 2, 3)
-            uind[:] = mu0[:, 0]
-            uinp[:] = mu0[:, 1]
+            # a copy: the solve writes its dipoles over the starting ones
+            mu = np.array(mu0, dtype=np.float64, order="C").reshape(n, 2, 3)
             guess = 1
+        else:
+            mu = np.empty((n, 2, 3))
         o = SolveOpts(poleps=poleps, politer=politer, guess=guess)
+        st = SolveStats()
+        ep = C.c_double(0)
+        self._c(self.lib.thp_induce_mu(self.h, C.byref(o), _d(mu), C.byref(ep), C.byref(st)), "induce")
+        return mu, ep.value, st
+
+    def newinduce_pme2_split(self, poleps=1e-8, politer=500):
+        """The same solve through thp_induce (the reference's separate uind
+        and uinp arrays): (uind, uinp, E_pol, stats)."""
+        n = self.sys.n
+        uind, uinp = np.empty((n, 3)), np.empty((n, 3))
+        o = SolveOpts(poleps=poleps, politer=politer, guess=0)
         st = SolveStats()
         ep = C.c_double(0)
         self._c(self.lib.thp_induce(self.h, C.byref(o), _d(uind), _d(uinp), C.byref(ep), C.byref(st)),
                 "induce")
-        mu = np.stack([uind, uinp], axis=1)
-        return mu, ep.value, st
+        return uind, uinp, ep.value, st
 
     induce = newinduce_pme2
 
