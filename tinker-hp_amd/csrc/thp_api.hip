@@ -327,10 +327,17 @@ thp_status launch_check(thp_ctx* ctx, const char* what) {
 // tuned shape.
 thp_status pair_efld0(thp_ctx* ctx, const Batch& b, int gate) {
   constexpr int S = kEfld0Segs;
-  if (b.prec == THP_PREC_FP64)
-    hipLaunchKernelGGL((k_efld0<double, S>), g1(b.n_blk), dim3(64 * S), 0, ctx->stream, b.d_views, b.d_blk, gate);
-  else
-    hipLaunchKernelGGL((k_efld0<float, S>), g1(b.n_blk), dim3(64 * S), 0, ctx->stream, b.d_views, b.d_blk, gate);
+  if (b.prec == THP_PREC_FP64) {
+    hipLaunchKernelGGL((k_efld0<double, S, false>), g1(b.n_blk), dim3(64 * S), 0, ctx->stream, b.d_views, b.d_blk, gate);
+  } else {
+    // systems with the Ewald table and systems without take separate
+    // kernels, each skipping the other kind (as the matvec)
+    const int kinds = b.rc ? b.rc : 3;
+    if (kinds & 1)
+      hipLaunchKernelGGL((k_efld0<float, S, true>), g1(b.n_blk), dim3(64 * S), 0, ctx->stream, b.d_views, b.d_blk, gate);
+    if (kinds & 2)
+      hipLaunchKernelGGL((k_efld0<float, S, false>), g1(b.n_blk), dim3(64 * S), 0, ctx->stream, b.d_views, b.d_blk, gate);
+  }
   hipLaunchKernelGGL(k_efld0_scaled, g1(b.n_blk), dim3(64), 0, ctx->stream, b.d_views, b.d_blk, gate);
   return launch_check(ctx, "k_efld0");
 }

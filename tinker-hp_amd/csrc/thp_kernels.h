@@ -582,6 +582,12 @@ __global__ void __launch_bounds__(64 * kUnionWaves) k_union(const SysView* views
   }
 }
 
+// four values of the pair type, one 16- or 32-byte LDS access
+template <typename real>
+struct __attribute__((aligned(4 * sizeof(real)))) Quad {
+  real a, b, c, d;
+};
+
 // (dmp3, dmp5) of one list entry, as a two-wide vector (one 8- or 16-byte
 // access)
 template <typename real>
@@ -623,8 +629,23 @@ struct Efld0Stage {
   static constexpr int CHUNK_POS = SMEM / 24;                           // xyz only (counting pass)
 };
 
-template <typename real, int S>
+//
+// TAB (FP32 systems that carry the Ewald table of the recomputed matvec
+// coefficients): G1 = r^3 bn1 and G2 = r^5 bn2 come from the table's cubic
+// pieces, G3 = r^7 bn3 = 5 G2 + (8/sqrt(pi)) x^5 exp(-x^2) from the
+// recursion of dampewald (damping.f:17-56, x = aewald r), and the Thole
+// term from exp2 of t = a (r / (pdamp_i pdamp_k))^3 with the reciprocal
+// pdamp of each atom (damptholed, damping.f:142-215):
+//   dmp3 = (G1 - e) / r^3
+//   dmp5 = (G2 - 3 (1 + t) e) / r^5
+//   dmp7 = (G3 - 15 (1 + t + 0.6 t^2) e) / r^7,  e = exp(-t) for t < 50.
+// Without TAB the pair type's erfc and exp form bn1..bn3 as the reference
+// does.  Each kind of kernel skips the systems of the other kind.
+constexpr float kG3Coef = 4.51351666838205f;  // 8 / sqrt(pi)
+constexpr float kLog2e = 1.44269504088896f;
+template <typename real, int S, bool TAB>
 __global__ void __launch_bounds__(64 * S) k_efld0(const SysView* views, const Work* work, int gate) {
+  static_assert(!TAB || sizeof(real) == 4, "the table path is FP32");
   using ST = Efld0Stage<real>;
   constexpr int C = ST::CHUNK;
   constexpr int CP = ST::CHUNK_POS;
@@ -632,10 +653,21 @@ __global__ void __launch_bounds__(64 * S) k_efld0(const SysView* views, const Wo
   static_assert(S * 6 * 64 * (int)sizeof(double) <= ST::SMEM, "the cross-wavefront sum fits the staging buffer");
   __shared__ __attribute__((aligned(16))) unsigned char smem[ST::SMEM];
   __shared__ int s_kept[S][64];
+  __shared__ Quad<float> s_gt[TAB ? 2 * kGtabN : 1];
   const Work w = work[blockIdx.x];
   if (w.sys < 0) return;
   const SysView& sv = views[w.sys];
   if (gated_out(sv.st, gate)) return;
+  if (TAB != (sv.rc_sqa > 0.0)) return;
+  if (TAB) {
+    // visible after the barrier that ends the counting pass
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    const GLOBAL F4* GT = reinterpret_cast<const GLOBAL F4*>(gp(sv.gtab));
+    for (int t = threadIdx.x; t < 2 * kGtabN; t += 64 * S) {
+      const F4 v = GT[t];
+      s_gt[t] = Quad<float>{v.x, v.y, v.z, v.w};
+    }
+  }
   const int n = sv.n;
   const size_t np = sv.npad;
   const double bx = sv.box[0], by = sv.box[1], bz = sv.box[2], cut2 = sv.cut2;
@@ -653,7 +685,12 @@ __global__ void __launch_bounds__(64 * S) k_efld0(const SysView* views, const Wo
   const int ucount = gp(sv.ucount)[w.blk];
   const GLOBAL int* UL = gp(sv.ulist) + (size_t)w.blk * sv.umax;
   const double xi = ok ? X[i] : 0.0, yi = ok ? Y[i] : 0.0, zi = ok ? Z[i] : 0.0;
-  const real pdi = ok ? real(PD[i]) : real(0), thi = ok ? real(TH[i]) : real(0);
+  // TAB stages 1 / pdamp (1e15 for an atom without polarizability: its
+  // pairs then reach t = inf and stay undamped, as damp = 0 leaves them in
+  // damptholed)
+  const real pdi = ok ? (TAB ? (PD[i] > 0.0 ? real(1.0 / PD[i]) : real(1e15)) : real(PD[i])) : real(0);
+  const real thi = ok ? real(TH[i]) : real(0);
+  const float aws = float(sv.aewald) * kGtabScale;
   const int cnt = ok ? gp(sv.lcount)[i] : 0;
   const int s_begin = (cnt * wv) / S;
   const int s_end = (cnt * (wv + 1)) / S;
@@ -729,7 +766,7 @@ __global__ void __launch_bounds__(64 * S) k_efld0(const SysView* views, const Wo
       s_pos[0 * C + q] = X[kk];
       s_pos[1 * C + q] = Y[kk];
       s_pos[2 * C + q] = Z[kk];
-      s_pd[q] = real(PD[kk]);
+      s_pd[q] = TAB ? (PD[kk] > 0.0 ? real(1.0 / PD[kk]) : real(1e15)) : real(PD[kk]);
       s_th[q] = real(TH[kk]);
 #pragma unroll
       for (int m = 0; m < 10; ++m) s_m[m * C + q] = real(RP[(size_t)m * np + kk]);
@@ -755,12 +792,50 @@ __global__ void __launch_bounds__(64 * S) k_efld0(const SysView* views, const Wo
         if (!(r2_exact(dx, dy, dz) <= cut2)) continue;  // list entry within the skin only
         const real xr = real(dx), yr = real(dy), zr = real(dz);
         const real r2 = xr * xr + yr * yr + zr * zr;
-        const real rr1 = rsqrt_t(r2);
-        const real r = r2 * rr1;
-        const real rr2 = rr1 * rr1;
-        const real rr3 = rr2 * rr1;
-        const real rr5 = real(3) * rr2 * rr3;
-        const real rr7 = real(5) * rr2 * rr5;
+        real dmp3, dmp5, dmp7;
+        if constexpr (TAB) {
+          // r^2 of a pair within the cutoff is far from the denormal range
+          const float rr1 = __builtin_amdgcn_rsqf(r2);
+          const float r = r2 * rr1;
+          const float rr2 = rr1 * rr1;
+          const float rr3 = rr2 * rr1;
+          const float x = r * aws;
+          const int j = min(int(x), kGtabN - 1);
+          const float fr = x - float(j);
+          const Quad<float> P = s_gt[2 * j];
+          const Quad<float> Q = s_gt[2 * j + 1];
+          const float G1 = ((P.d * fr + P.c) * fr + P.b) * fr + P.a;
+          const float G2 = ((Q.d * fr + Q.c) * fr + Q.b) * fr + Q.a;
+          const float xa = x * (1.0f / kGtabScale);
+          const float xa2 = xa * xa;
+          const float G3 = 5.0f * G2 + kG3Coef * (xa2 * xa2 * xa) * __builtin_amdgcn_exp2f(-kLog2e * xa2);
+          float pg = fminf(thi, s_th[q]);
+          if (pg == 0.0f) pg = fmaxf(thi, s_th[q]);
+          const float u = r * pdi * s_pd[q];
+          const float t = pg * (u * u * u);  // inf (or NaN for pg = 0) when an atom is unpolarizable
+          const bool damped = pg > 0.0f && t < 50.0f;
+          // the polynomial factors take t clamped, so that e = 0 keeps them finite
+          const float tc = fminf(t, 50.0f);
+          const float e = damped ? __builtin_amdgcn_exp2f(-kLog2e * tc) : 0.0f;
+          dmp3 = rr3 * (G1 - e);
+          dmp5 = (rr3 * rr2) * (G2 - 3.0f * (1.0f + tc) * e);
+          dmp7 = (rr3 * rr2 * rr2) * (G3 - 15.0f * ((1.0f + tc) + 0.6f * (tc * tc)) * e);
+        } else {
+          const real rr1 = rsqrt_t(r2);
+          const real r = r2 * rr1;
+          const real rr2 = rr1 * rr1;
+          const real rr3 = rr2 * rr1;
+          const real rr5 = real(3) * rr2 * rr3;
+          const real rr7 = real(5) * rr2 * rr5;
+          real bn[3];
+          ewald_bn<real, 3>(r, r2, aew, rr1, bn);
+          real l[3];
+          thole_l<real, 1>(r, pdi, s_pd[q], thi, s_th[q], l);
+          // unscaled pairs: dmp_n = bn_n - (1 - lambda_n) rr_n, the same for d and p
+          dmp3 = bn[0] - (real(1) - l[0]) * rr3;
+          dmp5 = bn[1] - (real(1) - l[1]) * rr5;
+          dmp7 = bn[2] - (real(1) - l[2]) * rr7;
+        }
         const real ck = s_m[0 * C + q], dkx = s_m[1 * C + q], dky = s_m[2 * C + q], dkz = s_m[3 * C + q];
         const real qkxx = s_m[4 * C + q], qkxy = s_m[5 * C + q], qkxz = s_m[6 * C + q], qkyy = s_m[7 * C + q],
                    qkyz = s_m[8 * C + q], qkzz = s_m[9 * C + q];
@@ -769,14 +844,6 @@ __global__ void __launch_bounds__(64 * S) k_efld0(const SysView* views, const Wo
         const real qky = qkxy * xr + qkyy * yr + qkyz * zr;
         const real qkz = qkxz * xr + qkyz * yr + qkzz * zr;
         const real qkr = qkx * xr + qky * yr + qkz * zr;
-        real bn[3];
-        ewald_bn<real, 3>(r, r2, aew, rr1, bn);
-        real l[3];
-        thole_l<real, 1>(r, pdi, s_pd[q], thi, s_th[q], l);
-        // unscaled pairs: dmp_n = bn_n - (1 - lambda_n) rr_n, the same for d and p
-        const real dmp3 = bn[0] - (real(1) - l[0]) * rr3;
-        const real dmp5 = bn[1] - (real(1) - l[1]) * rr5;
-        const real dmp7 = bn[2] - (real(1) - l[2]) * rr7;
         const size_t at = ((size_t)(t / kPack) * np + i) * kPack + t % kPack;
         TL[at] = (unsigned short)loc[b];
         if (keep_coef) TC[at] = coef2<real>(dmp3, dmp5);
@@ -835,10 +902,6 @@ __global__ void __launch_bounds__(64 * S) k_efld0(const SysView* views, const Wo
 // reach across half the box take the minimum image of each displacement.
 // Unions larger than one LDS chunk are walked chunk by chunk, with the
 // entries outside the staged chunk masked.
-template <typename real>
-struct __attribute__((aligned(4 * sizeof(real)))) Quad {
-  real a, b, c, d;
-};
 
 template <typename real>
 struct ClStage {
