@@ -53,7 +53,6 @@ struct SolveState {
   double ggold[2];
   double gg[2];
   double ggnew[2];
-  double ene[2];
   double resnrm;
   double poleps;
   double ep;

@@ -139,19 +139,16 @@ __device__ __forceinline__ double block_sum_partials(const double* part, int cou
 }
 
 // partial-sum buffers: gather kernels write gat_blocks pairs (gg, init),
-// vector kernels write red_blocks pairs (rz ping-pong, ene, ep)
+// vector kernels write red_blocks pairs (rz ping-pong, ep)
 __device__ __forceinline__ double* part_gg(const SysView& sv) { return sv.partial; }
 __device__ __forceinline__ double* part_init(const SysView& sv) { return sv.partial + (size_t)2 * sv.gat_blocks; }
 __device__ __forceinline__ double* part_rz(const SysView& sv, int parity) {
   return sv.partial + (size_t)4 * sv.gat_blocks + (size_t)2 * sv.red_blocks * parity;
 }
-__device__ __forceinline__ double* part_ene(const SysView& sv) {
+__device__ __forceinline__ double* part_ep(const SysView& sv) {
   return sv.partial + (size_t)4 * sv.gat_blocks + (size_t)4 * sv.red_blocks;
 }
-__device__ __forceinline__ double* part_ep(const SysView& sv) {
-  return sv.partial + (size_t)4 * sv.gat_blocks + (size_t)6 * sv.red_blocks;
-}
-inline size_t partial_doubles(int gat_blocks, int red_blocks) { return (size_t)4 * gat_blocks + (size_t)8 * red_blocks; }
+inline size_t partial_doubles(int gat_blocks, int red_blocks) { return (size_t)4 * gat_blocks + (size_t)6 * red_blocks; }
 
 // sum of two values over a 256-thread block, written as partial `slot`
 __device__ __forceinline__ void block_partial2(double a, double b, double* part, int slot) {
@@ -2491,7 +2488,6 @@ __global__ void __launch_bounds__(64) k_gather_tile(const SysView* views, const 
           const double rr = ev[c] - h;
           const double z = diag * rr;
           gw(sv.res)[o] = rr;
-          gw(sv.zr)[o] = z;
           gw(sv.pp)[o] = z;
           qs[c / 3] += rr * z;
         } else {
@@ -2916,8 +2912,9 @@ __global__ void __launch_bounds__(64) k_guess(const SysView* views, const Work* 
 // iteration number lives in the solve state, so one launch can carry
 // systems at different iterations and captured launches can be replayed):
 // alpha = ggold/gg with ggold = sum of rz partials of iteration it-1;
-// mu += alpha pp; res -= alpha h; zr = diag*res; partial ggnew (into rz
-// buffer it&1) and ene.
+// mu += alpha pp; res -= alpha h; partial ggnew = sum res (diag res)
+// (into rz buffer it&1).  The reference's ene (newinduce_pme2.f:527-538)
+// feeds only its polprt printout (:546-548) and is not formed here.
 template <typename real>
 __global__ void __launch_bounds__(256) k_cg_update1(const SysView* views, const Work* work) {
   const Work w = work[blockIdx.x];
@@ -2958,12 +2955,12 @@ __global__ void __launch_bounds__(256) k_cg_update1(const SysView* views, const 
   const int n = sv.n;
   GLOBAL double* MU = gw(sv.mu);
   GLOBAL double* RES = gw(sv.res);
-  GLOBAL double* ZR = gw(sv.zr);
   const GLOBAL double* PP = gp(sv.pp);
   const GLOBAL double* H = gp(sv.h);
-  const GLOBAL double* EF = gp(sv.ef);
   const GLOBAL double* POL = gp(sv.pol);
-  double gn[2] = {0, 0}, en[2] = {0, 0};
+  // the preconditioned residual z = diag r is not stored: k_cg_update2
+  // forms it again from the stored r (the same product, the same value)
+  double gn[2] = {0, 0};
   for (int i = w.blk * blockDim.x + threadIdx.x; i < n; i += G * blockDim.x) {
     const double pl = POL[i];
     const double diag = pl == 0.0 ? kTinyPol : pl;
@@ -2976,17 +2973,14 @@ __global__ void __launch_bounds__(256) k_cg_update1(const SysView* views, const 
       const double z = diag * r;
       MU[o] = m;
       RES[o] = r;
-      ZR[o] = z;
       gn[c / 3] += r * z;
-      en[c / 3] += -0.5 * (m * (r + EF[o]));
     }
   }
   block_partial2(gn[0], gn[1], part_rz(sv, it & 1), w.blk);
-  block_partial2(en[0], en[1], part_ene(sv), w.blk);
 }
 
 // pcg_newDirection + convergence (newinduce_pme2.f:543-568) for the
-// iteration started by k_cg_update1 (it_cur): pp = zr + ggnew/ggold pp; residual norm sqrt(ggnew/(3 npolar)),
+// iteration started by k_cg_update1 (it_cur): pp = zr + ggnew/ggold pp with zr = diag res; residual norm sqrt(ggnew/(3 npolar)),
 // stop when the max over both rhs is below poleps.
 __global__ void __launch_bounds__(256) k_cg_update2(const SysView* views, const Work* work) {
   const Work w = work[blockIdx.x];
@@ -3027,12 +3021,15 @@ __global__ void __launch_bounds__(256) k_cg_update2(const SysView* views, const 
   const size_t np = sv.npad;
   const int n = sv.n;
   GLOBAL double* PP = gw(sv.pp);
-  const GLOBAL double* ZR = gp(sv.zr);
+  const GLOBAL double* RES = gp(sv.res);
+  const GLOBAL double* POL = gp(sv.pol);
   for (int i = w.blk * blockDim.x + threadIdx.x; i < n; i += G * blockDim.x) {
+    const double pl = POL[i];
+    const double diag = pl == 0.0 ? kTinyPol : pl;
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       const size_t o = (size_t)c * np + i;
-      PP[o] = ZR[o] + beta[c / 3] * PP[o];
+      PP[o] = diag * RES[o] + beta[c / 3] * PP[o];
     }
   }
   // every block computed the same decision; the flag is read by the next
@@ -3370,11 +3367,8 @@ __global__ void __launch_bounds__(64) k_finish(const SysView* views, const int* 
   const SysView& sv = views[sys_ids[blockIdx.x]];
   if (only_done && (sv.st->conv == kRunning || sv.st->finalized || sv.st->in_setup)) return;
   const double e = sum_partials(part_ep(sv), sv.red_blocks, 0);
-  const double e0 = sum_partials(part_ene(sv), sv.red_blocks, 0), e1 = sum_partials(part_ene(sv), sv.red_blocks, 1);
   if (threadIdx.x == 0) {
     sv.st->ep = -0.5 * kCoulomb * e;
-    sv.st->ene[0] = e0;
-    sv.st->ene[1] = e1;
     sv.st->finalized = 1;
   }
 }
@@ -3390,7 +3384,6 @@ __global__ void k_init_state(const SysView* views, const int* sys_ids, int nsys,
   SolveState* st = views[sys_ids[q]].st;
   SolveState z;
   z.ggold[0] = z.ggold[1] = z.gg[0] = z.gg[1] = z.ggnew[0] = z.ggnew[1] = 0.0;
-  z.ene[0] = z.ene[1] = 0.0;
   z.resnrm = 0.0;
   z.poleps = poleps;
   z.ep = 0.0;
