@@ -798,7 +798,7 @@ __global__ void __launch_bounds__(64 * S) k_efld0(const SysView* views, const Wo
           const float rr3 = rr2 * rr1;
           const float x = r * aws;
           const int j = min(int(x), kGtabN - 1);
-          const float fr = x - float(j);
+          const float fr = __builtin_amdgcn_fractf(x);
           const Quad<float> P = s_gt[2 * j];
           const Quad<float> Q = s_gt[2 * j + 1];
           const float G1 = ((P.d * fr + P.c) * fr + P.b) * fr + P.a;
@@ -1014,7 +1014,7 @@ __device__ __forceinline__ void apply_pack(const LanePack<real, RC>& p, int c0, 
       // clamp keeps masked and sentinel entries inside the table
       const float x = r * g.aw;
       const int j = min(int(x), kGtabN - 1);
-      const float fr = x - float(j);
+      const float fr = __builtin_amdgcn_fractf(x);  // x - j wherever j is not clamped
       // (G2 from the table rather than from G1 and one more exp2: measured
       // 3% faster over a 16-system launch)
       const Quad<float> P = g.s_g[2 * j];
