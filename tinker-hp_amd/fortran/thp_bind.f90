@@ -126,6 +126,16 @@ module thp_bind
       real(c_double), intent(out) :: ep
       type(thp_solve_stats), intent(out) :: stats
     end function
+    ! the same solve with the dipoles in the layout of the reference's
+    ! mu(3,nrhs,npolebloc) work array (newinduce_pme2.f:71)
+    integer(c_int) function thp_induce_mu(sys, opts, mu, ep, stats) bind(C, name="thp_induce_mu")
+      import :: c_int, c_ptr, c_double, thp_solve_opts, thp_solve_stats
+      type(c_ptr), value :: sys
+      type(thp_solve_opts), intent(in) :: opts
+      real(c_double), intent(inout) :: mu(*)
+      real(c_double), intent(out) :: ep
+      type(thp_solve_stats), intent(out) :: stats
+    end function
     integer(c_int) function thp_system_update_multipoles(sys, rpole) bind(C, name="thp_system_update_multipoles")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: sys
